@@ -1,0 +1,141 @@
+/*
+ * mhdp — MI355X-native MAC pressure projection (C ABI of libmhdp.so).
+ *
+ * Drop-in for the pressure projection of sdfgeoff/mhd-simulator:
+ *   FluidState::project(&mut self, warm_start=false) -> Result<(u32, f32), String>
+ *   (src/fluid/mod.rs:1015-1115): divergence of the candidate face velocities,
+ *   PCG with a Jacobi or aggregation-multigrid preconditioner on the
+ *   variable-coefficient 7-point pressure operator, gradient subtraction.
+ *
+ * Conventions
+ *   cells   x + nx*(y + ny*z) (README.md:41 of the reference)
+ *   faces   three SoA planes packed x|y|z; plane a has dims (n + e_a):
+ *           u: (nx+1)*ny*nz, v: nx*(ny+1)*nz, w: nx*ny*(nz+1).  The value at
+ *           index n_a of a periodic axis duplicates index 0: it is ignored on
+ *           input and written on output (canonical faces, solver.wgsl:30).
+ *   phi     the pressure potential of solver.wgsl:7 (physical pressure is
+ *           rho_liquid * (phi - phi[anchor]) / dt, solver.wgsl:269-270);
+ *           phi[anchor] = 0 in every component.
+ *
+ * Errors: every entry point returns an mhdp_status; on a non-zero status the
+ * outputs are untouched (the reference restores its state on error,
+ * src/fluid/mod.rs:1313-1329) and mhdp_last_error() holds a message using the
+ * reference's texts where one exists.
+ *
+ * Threading: one context per HIP stream; a context is not thread-safe;
+ * independent contexts may be used concurrently.  One GPU per process.
+ */
+#ifndef MHDP_H
+#define MHDP_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mhdp_ctx mhdp_ctx;
+
+typedef enum {
+    MHDP_OK = 0,
+    MHDP_NOT_CONVERGED = 1, /* "Fluid pressure solve did not converge ..." (mod.rs:1097-1104) */
+    MHDP_NO_PROGRESS = 2,   /* "Fluid pressure iteration made no progress" (mod.rs:1079) */
+    MHDP_NONFINITE = 3,     /* "Nonfinite fluid pressure residual" (mod.rs:974) */
+    MHDP_INVALID = 4,       /* invalid argument / limits (FluidConfig::validate, mod.rs:58-82) */
+    MHDP_NO_MEMORY = 5,     /* device allocation failed (GpuContext::ensure_memory, gpu.rs:156-170) */
+    MHDP_DEVICE = 6         /* HIP runtime error */
+} mhdp_status;
+
+enum { MHDP_PRECOND_AUTO = 0, MHDP_PRECOND_JACOBI = 1, MHDP_PRECOND_MG = 2 };
+enum { MHDP_FP32 = 0, MHDP_MIXED = 1 };
+enum { MHDP_GAUGE_AUTO = 0, MHDP_GAUGE_PINNED = 2 };
+
+/* Grid of the pressure cells: Params.dims / flags.x / physical.x (mod.rs:118-130). */
+typedef struct {
+    uint32_t nx, ny, nz;
+    uint32_t periodic; /* bit a set: axis a periodic (FluidConfig.periodic) */
+    float h;           /* cell size (GridSi.cell_m) */
+} mhdp_grid;
+
+/* Density model of the face mobility (solver.wgsl:40-51): beta = rho_liquid / rho_face
+ * with rho_face the mean of the two cell densities mix(rho_air, rho_liquid, alpha) when
+ * two_phase (SurfaceConfig.enabled), else beta = 1. */
+typedef struct {
+    int two_phase;
+    float rho_liquid; /* Params.physical.w */
+    float rho_air;    /* Params.surface.y  */
+} mhdp_phase;
+
+typedef struct {
+    float rel_tol;       /* FluidConfig.relative_tolerance, [1e-7, 1e-2] */
+    float abs_div_tol;   /* FluidConfig.absolute_divergence_tolerance (1/s), >= 0 */
+    uint32_t max_iter;   /* FluidConfig.max_iterations, [1, 20000] (per inner solve) */
+    int precond;         /* MHDP_PRECOND_*: AUTO = multigrid at >= 96^3 cells (multigrid.rs:4) */
+    int precision;       /* MHDP_FP32: the reference's FP32 PCG; MHDP_MIXED: FP64 refinement
+                            around FP32 PCG with a BF16-stored V-cycle and FP64 coarsest solve */
+    int gauge;           /* MHDP_GAUGE_AUTO: mean-free iff one component (mod.rs:981) */
+    double refine_tol;   /* MHDP_MIXED: FP64 relative residual target (e.g. 1e-8) */
+    uint32_t max_outer;  /* MHDP_MIXED: refinement step cap (0 = 16) */
+    float inner_tol;     /* MHDP_MIXED: inner relative reduction per step (0 = 1e-4) */
+} mhdp_opts;
+
+typedef struct {
+    uint32_t iterations;    /* PCG iterations (sum over inner solves) */
+    uint32_t outer_steps;   /* FP64 refinement steps (MHDP_MIXED) */
+    uint32_t components;    /* connected fluid components (anchors(), mod.rs:1447) */
+    uint32_t fluid_cells;
+    uint32_t levels;        /* multigrid levels incl. the fine grid, 0 for Jacobi */
+    int meanfree;           /* 1: mean-free gauge, 0: pinned anchors */
+    int multigrid;
+    float rel_residual;     /* sqrt(rr/bb) as FluidState::convergence() reports it */
+    double rel_residual_f64;/* MHDP_MIXED: FP64 ||b - L phi|| / ||b|| over all fluid rows */
+    double seconds;         /* device time of the whole projection (HIP events) */
+} mhdp_stats;
+
+/* FluidState::new (mod.rs:182): allocate a context for one grid on the current device. */
+int mhdp_create(mhdp_ctx **out, const mhdp_grid *grid, const mhdp_phase *phase);
+/* Use a caller stream (hipStream_t) instead of the context's own stream. */
+int mhdp_set_stream(mhdp_ctx *ctx, void *hip_stream);
+/* Fluid mask (1 = liquid or air cell, 0 = solid) and liquid fraction alpha (NULL for
+ * single phase), host pointers.  Labels components and encodes the gauge mask exactly as
+ * anchors() (mod.rs:1447-1493, called from mod.rs:330,866). */
+int mhdp_set_cells(mhdp_ctx *ctx, const uint8_t *fluid, const float *alpha);
+/* The encoded mask: 0 solid, 1|root<<2 member, 2|root<<2 anchor (root = lowest index). */
+int mhdp_get_mask(const mhdp_ctx *ctx, uint32_t *encoded);
+/* FluidState::project(false), host pointers: u_star in (faces), u_out / phi_out out
+ * (either may be NULL). */
+int mhdp_project(mhdp_ctx *ctx, const float *u_star, const mhdp_opts *opts, float *u_out, double *phi_out,
+                 mhdp_stats *stats);
+/* Same with device pointers already resident in HBM (no host transfer). */
+int mhdp_project_dev(mhdp_ctx *ctx, const float *u_star_dev, const mhdp_opts *opts, float *u_out_dev,
+                     double *phi_out_dev, mhdp_stats *stats);
+/* Many independent projections (uneven grids) on one stream, device pointers. */
+int mhdp_project_batch(mhdp_ctx *const *ctxs, int count, const float *const *u_star_dev, const mhdp_opts *opts,
+                       float *const *u_out_dev, double *const *phi_out_dev, mhdp_stats *stats);
+const char *mhdp_last_error(const mhdp_ctx *ctx);
+void mhdp_destroy(mhdp_ctx *ctx);
+/* Device bytes held by the context (FluidState.bytes, mod.rs:591). */
+uint64_t mhdp_device_bytes(const mhdp_ctx *ctx);
+
+/* anchors() (mod.rs:1447-1493) on the host, no device needed: mask in 0/1 per cell, out
+ * encoded as above; returns the number of components. */
+uint32_t mhdp_anchors(uint32_t *mask, const uint32_t dims[3], uint32_t periodic);
+/* hierarchy() (multigrid.rs:25-65) on the host: level dims (nx,ny,nz,count per level,
+ * fine first) into level_dims[4*max_levels]; returns the level count or -1 on the
+ * reference's error cases (zero or inconsistent dims, coordinates beyond i32, u32 offsets). */
+int mhdp_hierarchy(const uint32_t dims4[4], uint32_t *level_dims, int max_levels, uint64_t *coarse_cells);
+
+/* ---- test hooks (the reference's cfg(test) probes: multigrid_probes.wgsl, tests.rs) ---- */
+/* pressure_coefficients: lower-face mobilities (3*cells, x|y|z), cached diagonal, 1/diag. */
+int mhdp_probe_coefficients(mhdp_ctx *ctx, int meanfree, float *beta3, float *diag, float *dinv);
+/* apply(): q = A p with the cached operator (identity rows at pinned anchors). */
+int mhdp_probe_apply(mhdp_ctx *ctx, int meanfree, const float *p, float *q);
+/* One V-cycle z = M r (precision MHDP_FP32 or MHDP_MIXED storage). */
+int mhdp_probe_vcycle(mhdp_ctx *ctx, int meanfree, int precision, const float *r, float *z);
+/* Coarse graph of level l >= 1 (lower-slot edges x,y,z, reaction, diagonal: 5 floats/cell). */
+int mhdp_probe_mg_graph(mhdp_ctx *ctx, int meanfree, int level, float *graph5, uint32_t *dims4);
+/* Divergence right-hand side b = -h^2 div u* (physical, no anchor zeroing), FP64. */
+int mhdp_probe_rhs(mhdp_ctx *ctx, int meanfree, const float *u_star, double *b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,89 @@
+"""ctypes binding of libmhdp.so (include/mhdp.h).
+
+The library is built in-tree by ``python -m mhd_simulator_amd.build``; loading
+fails loudly when it is missing — there is no CPU fallback for the projection.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmhdp.so")
+
+# exported entry points, in include/mhdp.h order
+SYMBOLS = (
+    "mhdp_create", "mhdp_set_stream", "mhdp_set_cells", "mhdp_get_mask", "mhdp_project", "mhdp_project_dev",
+    "mhdp_project_batch", "mhdp_last_error", "mhdp_destroy", "mhdp_device_bytes",
+    "mhdp_anchors", "mhdp_hierarchy",
+    "mhdp_probe_coefficients", "mhdp_probe_apply", "mhdp_probe_vcycle", "mhdp_probe_mg_graph", "mhdp_probe_rhs",
+)
+
+OK, NOT_CONVERGED, NO_PROGRESS, NONFINITE, INVALID, NO_MEMORY, DEVICE = range(7)
+PRECOND = {"auto": 0, "jacobi": 1, "mg": 2}
+PRECISION = {"fp32": 0, "mixed": 1}
+GAUGE = {"auto": 0, "pinned": 2}
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32), ("periodic", C.c_uint32),
+                ("h", C.c_float)]
+
+
+class Phase(C.Structure):
+    _fields_ = [("two_phase", C.c_int), ("rho_liquid", C.c_float), ("rho_air", C.c_float)]
+
+
+class Opts(C.Structure):
+    _fields_ = [("rel_tol", C.c_float), ("abs_div_tol", C.c_float), ("max_iter", C.c_uint32),
+                ("precond", C.c_int), ("precision", C.c_int), ("gauge", C.c_int), ("refine_tol", C.c_double),
+                ("max_outer", C.c_uint32), ("inner_tol", C.c_float)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("iterations", C.c_uint32), ("outer_steps", C.c_uint32), ("components", C.c_uint32),
+                ("fluid_cells", C.c_uint32), ("levels", C.c_uint32), ("meanfree", C.c_int),
+                ("multigrid", C.c_int), ("rel_residual", C.c_float), ("rel_residual_f64", C.c_double),
+                ("seconds", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load libmhdp.so and declare every signature; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build the HIP library with `python -m mhd_simulator_amd.build`")
+    L = C.CDLL(path)
+    P, I, U32 = C.c_void_p, C.c_int, C.c_uint32
+    sig = {
+        "mhdp_create": (I, [C.POINTER(P), C.POINTER(Grid), C.POINTER(Phase)]),
+        "mhdp_set_stream": (I, [P, P]),
+        "mhdp_set_cells": (I, [P, P, P]),
+        "mhdp_get_mask": (I, [P, P]),
+        "mhdp_project": (I, [P, P, C.POINTER(Opts), P, P, C.POINTER(Stats)]),
+        "mhdp_project_dev": (I, [P, P, C.POINTER(Opts), P, P, C.POINTER(Stats)]),
+        "mhdp_project_batch": (I, [P, I, P, C.POINTER(Opts), P, P, P]),
+        "mhdp_last_error": (C.c_char_p, [P]),
+        "mhdp_destroy": (None, [P]),
+        "mhdp_device_bytes": (C.c_uint64, [P]),
+        "mhdp_anchors": (U32, [P, P, U32]),
+        "mhdp_hierarchy": (I, [P, P, I, P]),
+        "mhdp_probe_coefficients": (I, [P, I, P, P, P]),
+        "mhdp_probe_apply": (I, [P, I, P, P]),
+        "mhdp_probe_vcycle": (I, [P, I, I, P, P]),
+        "mhdp_probe_mg_graph": (I, [P, I, I, P, P]),
+        "mhdp_probe_rhs": (I, [P, I, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L

@@ -1,0 +1,806 @@
+// C ABI of libmhdp.so: the pressure projection context and the host-side orchestration
+// of FluidState::project (src/fluid/mod.rs:969-1115) over the HIP kernels.
+#include "mhdp.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mhdp_kernels.h"
+
+void launch_from_bf16(hipStream_t s, uint32_t n, const void* src, float* dst);
+
+namespace {
+
+constexpr uint32_t kMaxPartials = 4096;
+constexpr uint32_t kMgMinCells = 96u * 96u * 96u;  // multigrid.rs:4
+
+struct CoarseArrays {
+    float *bx = nullptr, *by = nullptr, *bz = nullptr, *react = nullptr, *diag = nullptr, *dinv = nullptr;
+    void *bx16 = nullptr, *by16 = nullptr, *bz16 = nullptr, *react16 = nullptr, *dinv16 = nullptr;
+    void *r = nullptr, *z = nullptr, *t = nullptr;
+};
+
+}  // namespace
+
+struct mhdp_ctx {
+    mhdp_grid grid{};
+    mhdp_phase phase{};
+    Grid3 g{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string err;
+    // host copy of the encoded mask (anchors())
+    std::vector<uint32_t> mask_h;
+    uint32_t components = 0, fluid_cells = 0, anchor0 = 0;
+    bool cells = false;
+    bool has_alpha = false;
+    // fine-level device arrays
+    uint32_t* mask = nullptr;
+    float* alpha = nullptr;
+    float *bx = nullptr, *by = nullptr, *bz = nullptr, *diag = nullptr, *dinv = nullptr;
+    float *b = nullptr, *x = nullptr, *r = nullptr, *p = nullptr, *q = nullptr;
+    void *z = nullptr, *t = nullptr;
+    double *x64 = nullptr, *b64 = nullptr;
+    void *bx16 = nullptr, *by16 = nullptr, *bz16 = nullptr, *dinv16 = nullptr;
+    // multigrid
+    std::vector<CoarseArrays> coarse;  // index l-1 for level l
+    std::vector<Grid3> levels;         // level grids, [0] = fine
+    bool mg_alloc = false, mg16_alloc = false;
+    // scalars
+    Scal* sc = nullptr;
+    Scal* sc_h = nullptr;
+    double* part = nullptr;
+    // staging for the host-pointer entry point
+    float *ustar_stage = nullptr, *uout_stage = nullptr;
+    double* phi_stage = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint64_t bytes = 0;
+};
+
+namespace {
+
+int fail(mhdp_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIP_OK(call)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) return fail(c, MHDP_DEVICE, std::string("HIP error: ") + \
+                                                              hipGetErrorString(e_) + " at " #call); \
+    } while (0)
+
+int dalloc(mhdp_ctx* c, void** p, size_t bytes) {
+    if (*p) return MHDP_OK;
+    if (hipMalloc(p, bytes ? bytes : 16) != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(c, MHDP_NO_MEMORY, "Fluid allocation exceeds the available device memory");
+    }
+    c->bytes += bytes;
+    return MHDP_OK;
+}
+void dfree(mhdp_ctx* c, void* p) {
+    if (p) (void)hipFree(p);
+    (void)c;
+}
+
+uint64_t face_count(const mhdp_grid& g) {
+    return (uint64_t)(g.nx + 1) * g.ny * g.nz + (uint64_t)g.nx * (g.ny + 1) * g.nz +
+           (uint64_t)g.nx * g.ny * (g.nz + 1);
+}
+
+// anchors(), src/fluid/mod.rs:1447-1493: BFS components over the fluid mask with periodic
+// wrap; every cell of a component is labelled with its root (lowest flat index, since the
+// outer scan runs in index order); the root is the anchor (2|root<<2), the others members
+// (1|root<<2).
+uint32_t label_components(std::vector<uint32_t>& mask, const uint32_t d[3], uint32_t periodic) {
+    const uint64_t n = (uint64_t)d[0] * d[1] * d[2];
+    std::vector<uint32_t> roots(n, 0);
+    std::vector<uint32_t> queue(n > 0 ? n : 1);
+    uint32_t components = 0;
+    for (uint64_t root = 0; root < n; root++) {
+        if (mask[root] != 1u) continue;
+        components++;
+        mask[root] = 4u;
+        roots[root] = (uint32_t)root;
+        uint64_t head = 0, tail = 0;
+        queue[tail++] = (uint32_t)root;
+        while (head < tail) {
+            const uint32_t i = queue[head++];
+            const int64_t p[3] = {(int64_t)(i % d[0]), (int64_t)((i / d[0]) % d[1]), (int64_t)(i / ((uint64_t)d[0] * d[1]))};
+            for (int axis = 0; axis < 3; axis++) {
+                for (int dir = -1; dir <= 1; dir += 2) {
+                    int64_t q[3] = {p[0], p[1], p[2]};
+                    int64_t v = p[axis] + dir;
+                    if (periodic & (1u << axis)) {
+                        const int64_t m = d[axis];
+                        v = ((v % m) + m) % m;
+                    } else if (v < 0 || v >= (int64_t)d[axis]) {
+                        continue;
+                    }
+                    q[axis] = v;
+                    const uint64_t j = (uint64_t)q[0] + (uint64_t)d[0] * ((uint64_t)q[1] + (uint64_t)d[1] * (uint64_t)q[2]);
+                    if (mask[j] == 1u) {
+                        mask[j] = 3u;
+                        roots[j] = (uint32_t)root;
+                        queue[tail++] = (uint32_t)j;
+                    }
+                }
+            }
+        }
+    }
+    for (uint64_t i = 0; i < n; i++) {
+        if (mask[i] == 3u) mask[i] = 1u | (roots[i] << 2);
+        else if (mask[i] == 4u) mask[i] = 2u | (roots[i] << 2);
+    }
+    return components;
+}
+
+// hierarchy(), multigrid.rs:25-65: ceil-half until every dimension is <= 4.
+std::vector<Grid3> hierarchy(const Grid3& fine) {
+    std::vector<Grid3> out{fine};
+    while (out.back().nx > 4 || out.back().ny > 4 || out.back().nz > 4) {
+        Grid3 c = out.back();
+        c.nx = (c.nx + 1) / 2;
+        c.ny = (c.ny + 1) / 2;
+        c.nz = (c.nz + 1) / 2;
+        c.n = (uint32_t)c.nx * (uint32_t)c.ny * (uint32_t)c.nz;
+        out.push_back(c);
+    }
+    return out;
+}
+
+int validate_opts(mhdp_ctx* c, const mhdp_opts* o) {
+    if (!o) return fail(c, MHDP_INVALID, "Missing projection options");
+    if (!(o->rel_tol >= 1e-7f && o->rel_tol <= 1e-2f) || o->max_iter < 1 || o->max_iter > 20000 ||
+        !(o->abs_div_tol >= 0.0f) || !std::isfinite(o->abs_div_tol))
+        return fail(c, MHDP_INVALID, "Invalid fluid timestep, initial condition or solver limits/tolerances");
+    if (o->precond < 0 || o->precond > 2 || (o->precision != MHDP_FP32 && o->precision != MHDP_MIXED) ||
+        (o->gauge != MHDP_GAUGE_AUTO && o->gauge != MHDP_GAUGE_PINNED))
+        return fail(c, MHDP_INVALID, "Invalid pressure solver selection");
+    if (o->precision == MHDP_MIXED &&
+        (!(o->refine_tol > 0.0) || !(o->refine_tol < 1.0) || o->inner_tol < 0.0f || o->inner_tol >= 1.0f))
+        return fail(c, MHDP_INVALID, "Invalid refinement tolerance");
+    return MHDP_OK;
+}
+
+int alloc_mg(mhdp_ctx* c, bool f16) {
+    int rc;
+    if (!c->mg_alloc) {
+        c->levels = hierarchy(c->g);
+        c->coarse.assign(c->levels.size() - 1, CoarseArrays{});
+        for (size_t l = 1; l < c->levels.size(); l++) {
+            CoarseArrays& a = c->coarse[l - 1];
+            const size_t n = c->levels[l].n;
+            const size_t fb = n * sizeof(float);
+            if ((rc = dalloc(c, (void**)&a.bx, fb)) || (rc = dalloc(c, (void**)&a.by, fb)) ||
+                (rc = dalloc(c, (void**)&a.bz, fb)) || (rc = dalloc(c, (void**)&a.react, fb)) ||
+                (rc = dalloc(c, (void**)&a.diag, fb)) || (rc = dalloc(c, (void**)&a.dinv, fb)) ||
+                (rc = dalloc(c, &a.r, fb)) || (rc = dalloc(c, &a.z, fb)) || (rc = dalloc(c, &a.t, fb)))
+                return rc;
+        }
+        c->mg_alloc = true;
+    }
+    if (f16 && !c->mg16_alloc) {
+        const size_t hb = (size_t)c->g.n * 2;
+        if ((rc = dalloc(c, &c->bx16, hb)) || (rc = dalloc(c, &c->by16, hb)) || (rc = dalloc(c, &c->bz16, hb)) ||
+            (rc = dalloc(c, &c->dinv16, hb)))
+            return rc;
+        for (size_t l = 1; l < c->levels.size(); l++) {
+            CoarseArrays& a = c->coarse[l - 1];
+            const size_t n2 = (size_t)c->levels[l].n * 2;
+            if ((rc = dalloc(c, &a.bx16, n2)) || (rc = dalloc(c, &a.by16, n2)) || (rc = dalloc(c, &a.bz16, n2)) ||
+                (rc = dalloc(c, &a.react16, n2)) || (rc = dalloc(c, &a.dinv16, n2)))
+                return rc;
+        }
+        c->mg16_alloc = true;
+    }
+    return MHDP_OK;
+}
+
+// Galerkin coarse graphs of all levels (Multigrid::build, multigrid.rs:266-273) and, for
+// the mixed solver, their BF16 copies.
+void build_mg(mhdp_ctx* c, int meanfree, bool f16) {
+    for (size_t l = 1; l < c->levels.size(); l++) {
+        CoarseArrays& a = c->coarse[l - 1];
+        const bool fine = l == 1;
+        const float* fbx = fine ? c->bx : c->coarse[l - 2].bx;
+        const float* fby = fine ? c->by : c->coarse[l - 2].by;
+        const float* fbz = fine ? c->bz : c->coarse[l - 2].bz;
+        const float* fdinv = fine ? c->dinv : c->coarse[l - 2].dinv;
+        const float* freact = fine ? nullptr : c->coarse[l - 2].react;
+        launch_mg_build(c->stream, c->levels[l], c->levels[l - 1], fine, meanfree, c->mask, fdinv, fbx, fby, fbz,
+                        freact, a.bx, a.by, a.bz, a.react, a.diag, a.dinv);
+    }
+    if (f16) {
+        launch_to_bf16(c->stream, c->g.n, c->bx, c->bx16);
+        launch_to_bf16(c->stream, c->g.n, c->by, c->by16);
+        launch_to_bf16(c->stream, c->g.n, c->bz, c->bz16);
+        launch_to_bf16(c->stream, c->g.n, c->dinv, c->dinv16);
+        for (size_t l = 1; l < c->levels.size(); l++) {
+            CoarseArrays& a = c->coarse[l - 1];
+            const uint32_t n = c->levels[l].n;
+            launch_to_bf16(c->stream, n, a.bx, a.bx16);
+            launch_to_bf16(c->stream, n, a.by, a.by16);
+            launch_to_bf16(c->stream, n, a.bz, a.bz16);
+            launch_to_bf16(c->stream, n, a.react, a.react16);
+            launch_to_bf16(c->stream, n, a.dinv, a.dinv16);
+        }
+    }
+}
+
+std::vector<LevelDev> level_view(mhdp_ctx* c) {
+    std::vector<LevelDev> lv(c->levels.size());
+    LevelDev& f = lv[0];
+    f.g = c->g;
+    f.bx = c->bx; f.by = c->by; f.bz = c->bz; f.react = nullptr; f.diag = c->diag; f.dinv = c->dinv;
+    f.bx16 = c->bx16; f.by16 = c->by16; f.bz16 = c->bz16; f.react16 = nullptr; f.dinv16 = c->dinv16;
+    f.r = c->r; f.z = c->z; f.t = c->t;
+    for (size_t l = 1; l < c->levels.size(); l++) {
+        CoarseArrays& a = c->coarse[l - 1];
+        LevelDev& d = lv[l];
+        d.g = c->levels[l];
+        d.bx = a.bx; d.by = a.by; d.bz = a.bz; d.react = a.react; d.diag = a.diag; d.dinv = a.dinv;
+        d.bx16 = a.bx16; d.by16 = a.by16; d.bz16 = a.bz16; d.react16 = a.react16; d.dinv16 = a.dinv16;
+        d.r = a.r; d.z = a.z; d.t = a.t;
+    }
+    return lv;
+}
+
+int read_scalars(mhdp_ctx* c) {
+    HIP_OK(hipMemcpyAsync(c->sc_h, c->sc, sizeof(Scal), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return MHDP_OK;
+}
+
+// convergence(), mod.rs:969-977
+int convergence(mhdp_ctx* c, float* residual, bool* conv, uint32_t* iters) {
+    int rc = read_scalars(c);
+    if (rc) return rc;
+    const Scal& s = *c->sc_h;
+    const double res = std::sqrt(s.rr / std::max(s.bb, 1e-38));
+    const double v[8] = {s.rz, s.rr, s.bb, s.sr, s.alpha, s.beta, s.conv, s.iters};
+    bool bad = !std::isfinite(res) || s.rr < 0.0 || s.bb < 0.0;
+    for (double x : v) bad |= !std::isfinite(x);
+    if (bad) return fail(c, MHDP_NONFINITE, "Nonfinite fluid pressure residual");
+    *residual = (float)res;
+    *conv = s.conv > 0.5;
+    *iters = (uint32_t)s.iters;
+    return MHDP_OK;
+}
+
+struct Solve {
+    bool meanfree = false;
+    bool mg = false;
+    int store = STORE_F32;
+    bool coarse64 = false;
+    std::vector<LevelDev> lv;
+};
+
+void refresh_stages(mhdp_ctx* c, const Solve& S) {  // mod.rs:1004-1014
+    launch_refresh(c->stream, c->g, c->x, c->b, c->bx, c->by, c->bz, c->dinv, c->r, c->p, c->part, c->sc,
+                   S.meanfree, 0, S.meanfree ? FIN_REFRESH_MEAN : FIN_INITIAL);
+    if (S.meanfree) launch_subspace(c->stream, c->g, c->r, c->p, c->dinv, c->part, c->sc, 1, FIN_INITIAL_SUB);
+}
+
+void mg_initialize(mhdp_ctx* c, Solve& S) {  // multigrid.rs:287-291
+    void* zf = nullptr;
+    launch_vcycle(c->stream, S.store, (int)S.lv.size(), S.lv.data(), c->r, c->mask, c->sc, S.coarse64, &zf);
+    launch_pdot(c->stream, c->g, S.store, c->r, zf, c->dinv, c->p, c->part, c->sc, 1, FIN_INITIAL_MG);
+}
+
+void iterate_once(mhdp_ctx* c, Solve& S) {  // mod.rs:1057-1075, multigrid.rs:302-326
+    hipStream_t s = c->stream;
+    launch_apply(s, c->g, c->p, c->bx, c->by, c->bz, c->q, c->part, c->sc);
+    const int upd = S.meanfree ? FIN_MEAN : (S.mg ? FIN_STASH_RR : FIN_BETA);
+    launch_update(s, c->g, c->x, c->r, c->p, c->q, c->dinv, c->part, c->sc, S.meanfree, upd);
+    if (S.meanfree) launch_subspace(s, c->g, c->r, c->p, c->dinv, c->part, c->sc, 0, S.mg ? FIN_STASH_RR : FIN_BETA);
+    if (S.mg) {
+        void* zf = nullptr;
+        launch_vcycle(s, S.store, (int)S.lv.size(), S.lv.data(), c->r, c->mask, c->sc, S.coarse64, &zf);
+        launch_pdot(s, c->g, S.store, c->r, zf, c->dinv, c->p, c->part, c->sc, 0, FIN_BETA_MG);
+        launch_psearch(s, c->g, S.store, zf, c->dinv, c->p, c->sc);
+    } else {
+        launch_search(s, c->g, c->r, c->dinv, c->p, c->sc);
+    }
+}
+
+void set_solve_params(mhdp_ctx* c, Scal& s, bool meanfree, double tol2, double abs2, double bound) {
+    std::memset(&s, 0, sizeof s);
+    s.meanfree = meanfree ? 1.0 : 0.0;
+    s.tol2 = tol2;
+    s.abs2 = abs2;
+    s.bound = bound;
+    s.inv_nf = meanfree ? 1.0 / (double)std::max<uint32_t>(c->fluid_cells, 1) : 0.0;
+    s.anchor = c->anchor0;
+}
+
+// The PCG loop of FluidState::project (mod.rs:1015-1104) on the FP32 vectors x (from 0),
+// right-hand side c->b.  `reference_checks`: refresh on convergence and the independent
+// physical check every 64 iterations, as the reference; the inner solves of the mixed
+// solver skip them (the FP64 outer residual is the acceptance test).
+// `want_mg`: build the multigrid preconditioner lazily, only once a nonzero residual
+// needs it (mod.rs:1031-1049).
+int pcg(mhdp_ctx* c, Solve& S, bool want_mg, uint32_t max_iter, bool reference_checks, uint32_t* iterations,
+        float* residual) {
+    launch_fill_f32(c->stream, c->g.n, c->x, 0.0f);
+    refresh_stages(c, S);
+    bool conv = false;
+    uint32_t seen = 0, it = 0;
+    int rc = convergence(c, residual, &conv, &seen);
+    if (rc) return rc;
+    if (!conv && want_mg && !S.mg) {
+        build_mg(c, S.meanfree ? 1 : 0, S.store == STORE_BF16);
+        S.mg = true;
+        S.lv = level_view(c);
+    }
+    if (!conv && S.mg) mg_initialize(c, S);
+    while (!conv && it < max_iter) {
+        uint32_t batch = S.mg ? 8u : 32u;
+        batch = std::min(batch, max_iter - it);
+        batch = std::min(batch, 64u - it % 64u);
+        for (uint32_t k = 0; k < batch; k++) iterate_once(c, S);
+        if ((rc = convergence(c, residual, &conv, &seen))) return rc;
+        if (seen <= it && !conv) return fail(c, MHDP_NO_PROGRESS, "Fluid pressure iteration made no progress");
+        it = seen;
+        if (conv && reference_checks) {
+            refresh_stages(c, S);
+            if ((rc = convergence(c, residual, &conv, &seen))) return rc;
+            if (!conv && S.mg) mg_initialize(c, S);
+        } else if (!conv && reference_checks && (it % 64u == 0u || it == max_iter)) {
+            launch_refresh(c->stream, c->g, c->x, c->b, c->bx, c->by, c->bz, c->dinv, c->r, c->p, c->part, c->sc,
+                           S.meanfree, 1, FIN_CHECK);
+            if ((rc = convergence(c, residual, &conv, &seen))) return rc;
+        }
+    }
+    *iterations = it;
+    return conv ? MHDP_OK : MHDP_NOT_CONVERGED;
+}
+
+int upload_scal(mhdp_ctx* c, const Scal& s) {
+    *c->sc_h = s;
+    HIP_OK(hipMemcpyAsync(c->sc, c->sc_h, sizeof(Scal), hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return MHDP_OK;
+}
+
+int ensure_fine(mhdp_ctx* c) {
+    const size_t n = c->g.n, fb = n * sizeof(float);
+    int rc;
+    if ((rc = dalloc(c, (void**)&c->mask, n * 4)) || (rc = dalloc(c, (void**)&c->alpha, fb)) ||
+        (rc = dalloc(c, (void**)&c->bx, fb)) || (rc = dalloc(c, (void**)&c->by, fb)) ||
+        (rc = dalloc(c, (void**)&c->bz, fb)) || (rc = dalloc(c, (void**)&c->diag, fb)) ||
+        (rc = dalloc(c, (void**)&c->dinv, fb)) || (rc = dalloc(c, (void**)&c->b, fb)) ||
+        (rc = dalloc(c, (void**)&c->x, fb)) || (rc = dalloc(c, (void**)&c->r, fb)) ||
+        (rc = dalloc(c, (void**)&c->p, fb)) || (rc = dalloc(c, (void**)&c->q, fb)) ||
+        (rc = dalloc(c, &c->z, fb)) || (rc = dalloc(c, &c->t, fb)) ||
+        (rc = dalloc(c, (void**)&c->sc, sizeof(Scal))) ||
+        (rc = dalloc(c, (void**)&c->part, kMaxPartials * 4 * sizeof(double))))
+        return rc;
+    return MHDP_OK;
+}
+
+PhaseP phase_params(const mhdp_ctx* c, bool meanfree) {
+    PhaseP ph;
+    ph.two_phase = c->phase.two_phase ? 1 : 0;
+    ph.rho_l = c->phase.rho_liquid;
+    ph.rho_air = c->phase.rho_air;
+    ph.meanfree = meanfree ? 1 : 0;
+    return ph;
+}
+
+int prepare(mhdp_ctx* c, bool meanfree) {
+    if (!c->cells) return fail(c, MHDP_INVALID, "Fluid cells have not been set");
+    launch_coefficients(c->stream, c->g, phase_params(c, meanfree), c->mask, c->has_alpha ? c->alpha : nullptr,
+                        c->bx, c->by, c->bz, c->diag, c->dinv);
+    return MHDP_OK;
+}
+
+int project_impl(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, float* uout, double* phi, mhdp_stats* st) {
+    int rc = validate_opts(c, o);
+    if (rc) return rc;
+    if (!c->cells) return fail(c, MHDP_INVALID, "Fluid cells have not been set");
+    if (!ustar) return fail(c, MHDP_INVALID, "Missing candidate face velocities");
+    const bool meanfree = o->gauge != MHDP_GAUGE_PINNED && c->components == 1;
+    const bool mixed = o->precision == MHDP_MIXED;
+    const bool preferred = o->precond == MHDP_PRECOND_MG || (o->precond == MHDP_PRECOND_AUTO && c->g.n >= kMgMinCells);
+    if (mixed) {
+        if ((rc = dalloc(c, (void**)&c->x64, (size_t)c->g.n * 8)) || (rc = dalloc(c, (void**)&c->b64, (size_t)c->g.n * 8)))
+            return rc;
+    }
+    if (preferred && (rc = alloc_mg(c, mixed))) return rc;
+
+    Solve S;
+    S.meanfree = meanfree;
+    S.store = mixed ? STORE_BF16 : STORE_F32;
+    S.coarse64 = mixed;
+    const float nf = (float)std::max<uint32_t>(c->fluid_cells, 1);
+
+    HIP_OK(hipEventRecord(c->ev0, c->stream));
+    if ((rc = prepare(c, meanfree))) return rc;
+    uint32_t iterations = 0, outer = 0;
+    float residual = 0.0f;
+    double res64 = 0.0;
+    if (!mixed) {
+        // Params.solve (mod.rs:398-404, 978-1002) in FP32 as the reference computes it
+        const float tol = o->rel_tol, ab = o->abs_div_tol, h = c->grid.h;
+        const float h2 = h * h;
+        const float tol2 = tol * tol;
+        const float abs2 = ((ab * ab) * (h2 * h2)) * (float)c->fluid_cells;
+        Scal s;
+        set_solve_params(c, s, meanfree, tol2, abs2, meanfree ? 1.0 : (double)nf);
+        if ((rc = upload_scal(c, s))) return rc;
+        launch_rhs_f32(c->stream, c->g, c->grid.h, c->mask, c->bx, c->by, c->bz, ustar, c->b);
+        rc = pcg(c, S, preferred, o->max_iter, true, &iterations, &residual);
+        if (rc == MHDP_NOT_CONVERGED) {
+            char buf[256];
+            std::snprintf(buf, sizeof buf,
+                          "Fluid pressure solve did not converge after %u iterations (true relative residual "
+                          "%.3e, requested %.3e, RMS divergence floor %.3e s\xe2\x81\xbb\xc2\xb9)",
+                          iterations, (double)residual, (double)o->rel_tol, (double)o->abs_div_tol);
+            return fail(c, rc, buf);
+        }
+        if (rc) return rc;
+        if (uout) launch_correct_f32(c->stream, c->g, c->grid.h, c->bx, c->by, c->bz, c->x, ustar, uout);
+        if (phi) launch_cast_f32_f64(c->stream, c->g.n, c->x, phi);
+    } else {
+        // FP64 iterative refinement: residual in FP64, inner FP32 PCG with the BF16 V-cycle.
+        const double tau = o->inner_tol > 0.0f ? (double)o->inner_tol : 1e-4;
+        const uint32_t max_outer = o->max_outer ? o->max_outer : 16u;
+        launch_rhs_f64(c->stream, c->g, c->grid.h, c->mask, c->bx, c->by, c->bz, ustar, c->b64);
+        launch_fill_f64(c->stream, c->g.n, c->x64, 0.0);
+        Scal s;
+        set_solve_params(c, s, meanfree, tau * tau, 0.0, 1.0);
+        if ((rc = upload_scal(c, s))) return rc;
+        bool done = false;
+        for (;;) {
+            launch_residual64(c->stream, c->g, c->x64, c->b64, c->bx, c->by, c->bz, c->dinv, c->b, c->part, c->sc);
+            if ((rc = read_scalars(c))) return rc;
+            const double rr = c->sc_h->rr, bb = c->sc_h->spare;
+            if (!std::isfinite(rr) || !std::isfinite(bb)) return fail(c, MHDP_NONFINITE, "Nonfinite fluid pressure residual");
+            res64 = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
+            if (bb == 0.0 || res64 <= o->refine_tol) {
+                done = true;
+                break;
+            }
+            if (outer >= max_outer) break;
+            // inner tolerance: no tighter than the remaining factor asks for
+            double t = std::max(tau, 0.5 * o->refine_tol / res64);
+            if (t > 0.5) t = 0.5;
+            Scal si;
+            set_solve_params(c, si, meanfree, t * t, 0.0, 1.0);
+            if ((rc = upload_scal(c, si))) return rc;
+            uint32_t it = 0;
+            rc = pcg(c, S, preferred, o->max_iter, false, &it, &residual);
+            iterations += it;
+            if (rc && rc != MHDP_NOT_CONVERGED) return rc;
+            launch_xupdate64(c->stream, c->g, c->x64, c->x, c->dinv);
+            outer++;
+        }
+        if (!done) {
+            char buf[256];
+            std::snprintf(buf, sizeof buf,
+                          "Fluid pressure solve did not converge after %u iterations (true relative residual %.3e, "
+                          "requested %.3e)",
+                          iterations, res64, o->refine_tol);
+            return fail(c, MHDP_NOT_CONVERGED, buf);
+        }
+        residual = (float)res64;
+        if (uout) launch_correct_f64(c->stream, c->g, c->grid.h, c->bx, c->by, c->bz, c->x64, ustar, uout);
+        if (phi) HIP_OK(hipMemcpyAsync(phi, c->x64, (size_t)c->g.n * 8, hipMemcpyDeviceToDevice, c->stream));
+    }
+    HIP_OK(hipEventRecord(c->ev1, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    HIP_OK(hipGetLastError());
+    float ms = 0.0f;
+    HIP_OK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (st) {
+        st->iterations = iterations;
+        st->outer_steps = outer;
+        st->components = c->components;
+        st->fluid_cells = c->fluid_cells;
+        st->levels = S.mg ? (uint32_t)c->levels.size() : 0u;
+        st->meanfree = meanfree ? 1 : 0;
+        st->multigrid = S.mg ? 1 : 0;
+        st->rel_residual = residual;
+        st->rel_residual_f64 = mixed ? res64 : -1.0;
+        st->seconds = ms * 1e-3;
+    }
+    return MHDP_OK;
+}
+
+}  // namespace
+
+// ============================================================================ C ABI
+
+extern "C" {
+
+int mhdp_create(mhdp_ctx** out, const mhdp_grid* grid, const mhdp_phase* phase) {
+    if (!out || !grid || !phase) return MHDP_INVALID;
+    *out = nullptr;
+    const uint64_t n = (uint64_t)grid->nx * grid->ny * grid->nz;
+    if (grid->nx == 0 || grid->ny == 0 || grid->nz == 0 || n > (1ull << 30) || face_count(*grid) > 0xffffffffull ||
+        !(grid->h > 0.0f) || !std::isfinite(grid->h) || grid->periodic > 7u)
+        return MHDP_INVALID;
+    if (phase->two_phase && (!(phase->rho_liquid > 0.0f) || !(phase->rho_air > 0.0f) ||
+                             !std::isfinite(phase->rho_liquid) || !std::isfinite(phase->rho_air)))
+        return MHDP_INVALID;
+    mhdp_ctx* c = new mhdp_ctx();
+    c->grid = *grid;
+    c->phase = *phase;
+    if (!phase->two_phase) c->phase.rho_liquid = phase->rho_liquid > 0.0f ? phase->rho_liquid : 1.0f;
+    c->g.nx = (int)grid->nx;
+    c->g.ny = (int)grid->ny;
+    c->g.nz = (int)grid->nz;
+    c->g.per = grid->periodic;
+    c->g.n = (uint32_t)n;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipHostMalloc((void**)&c->sc_h, sizeof(Scal), hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        mhdp_destroy(c);
+        return MHDP_DEVICE;
+    }
+    c->own_stream = true;
+    int rc = ensure_fine(c);
+    if (rc) {
+        mhdp_destroy(c);
+        return rc;
+    }
+    // the reduction tickets start (and always end) at zero
+    std::memset(c->sc_h, 0, sizeof(Scal));
+    if (hipMemcpy(c->sc, c->sc_h, sizeof(Scal), hipMemcpyHostToDevice) != hipSuccess) {
+        mhdp_destroy(c);
+        return MHDP_DEVICE;
+    }
+    *out = c;
+    return MHDP_OK;
+}
+
+int mhdp_set_stream(mhdp_ctx* c, void* stream) {
+    if (!c) return MHDP_INVALID;
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    c->stream = (hipStream_t)stream;
+    c->own_stream = false;
+    return MHDP_OK;
+}
+
+int mhdp_set_cells(mhdp_ctx* c, const uint8_t* fluid, const float* alpha) {
+    if (!c) return MHDP_INVALID;
+    if (!fluid) return fail(c, MHDP_INVALID, "Missing fluid mask");
+    if (c->phase.two_phase && !alpha) return fail(c, MHDP_INVALID, "Two-phase projection needs the liquid fraction");
+    const uint32_t n = c->g.n;
+    std::vector<uint32_t> mask(n);
+    uint32_t fluid_cells = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        mask[i] = fluid[i] ? 1u : 0u;
+        fluid_cells += mask[i];
+    }
+    if (alpha)
+        for (uint32_t i = 0; i < n; i++)
+            if (!std::isfinite(alpha[i])) return fail(c, MHDP_INVALID, "Nonfinite liquid fraction");
+    const uint32_t d[3] = {(uint32_t)c->g.nx, (uint32_t)c->g.ny, (uint32_t)c->g.nz};
+    const uint32_t comps = label_components(mask, d, c->g.per);
+    uint32_t anchor = 0;
+    for (uint32_t i = 0; i < n; i++)
+        if ((mask[i] & 3u) == 2u) {
+            anchor = i;
+            break;
+        }
+    if (hipMemcpy(c->mask, mask.data(), (size_t)n * 4, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(c, MHDP_DEVICE, "HIP error uploading the fluid mask");
+    if (alpha && hipMemcpy(c->alpha, alpha, (size_t)n * 4, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(c, MHDP_DEVICE, "HIP error uploading the liquid fraction");
+    c->has_alpha = alpha != nullptr;
+    c->mask_h.swap(mask);
+    c->components = comps;
+    c->fluid_cells = fluid_cells;
+    c->anchor0 = anchor;
+    c->cells = true;
+    return MHDP_OK;
+}
+
+int mhdp_get_mask(const mhdp_ctx* c, uint32_t* encoded) {
+    if (!c || !encoded || !c->cells) return MHDP_INVALID;
+    std::memcpy(encoded, c->mask_h.data(), c->mask_h.size() * 4);
+    return MHDP_OK;
+}
+
+int mhdp_project_dev(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, float* uout, double* phi, mhdp_stats* st) {
+    if (!c) return MHDP_INVALID;
+    c->err.clear();
+    return project_impl(c, ustar, o, uout, phi, st);
+}
+
+int mhdp_project(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, float* uout, double* phi, mhdp_stats* st) {
+    if (!c) return MHDP_INVALID;
+    c->err.clear();
+    if (!ustar) return fail(c, MHDP_INVALID, "Missing candidate face velocities");
+    const size_t fb = face_count(c->grid) * sizeof(float);
+    int rc;
+    if ((rc = dalloc(c, (void**)&c->ustar_stage, fb)) || (rc = dalloc(c, (void**)&c->uout_stage, fb)) ||
+        (rc = dalloc(c, (void**)&c->phi_stage, (size_t)c->g.n * 8)))
+        return rc;
+    HIP_OK(hipMemcpyAsync(c->ustar_stage, ustar, fb, hipMemcpyHostToDevice, c->stream));
+    rc = project_impl(c, c->ustar_stage, o, c->uout_stage, c->phi_stage, st);
+    if (rc) return rc;
+    if (uout) HIP_OK(hipMemcpyAsync(uout, c->uout_stage, fb, hipMemcpyDeviceToHost, c->stream));
+    if (phi) HIP_OK(hipMemcpyAsync(phi, c->phi_stage, (size_t)c->g.n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return MHDP_OK;
+}
+
+int mhdp_project_batch(mhdp_ctx* const* ctxs, int count, const float* const* ustar, const mhdp_opts* o,
+                       float* const* uout, double* const* phi, mhdp_stats* st) {
+    if (!ctxs || count < 0 || !ustar) return MHDP_INVALID;
+    for (int k = 0; k < count; k++) {
+        int rc = mhdp_project_dev(ctxs[k], ustar[k], o, uout ? uout[k] : nullptr, phi ? phi[k] : nullptr,
+                                  st ? st + k : nullptr);
+        if (rc) return rc;
+    }
+    return MHDP_OK;
+}
+
+uint32_t mhdp_anchors(uint32_t* mask, const uint32_t dims[3], uint32_t periodic) {
+    if (!mask || !dims) return 0;
+    const uint64_t n = (uint64_t)dims[0] * dims[1] * dims[2];
+    std::vector<uint32_t> m(mask, mask + n);
+    for (auto& v : m) v = v ? 1u : 0u;
+    const uint32_t comps = label_components(m, dims, periodic);
+    std::memcpy(mask, m.data(), n * 4);
+    return comps;
+}
+
+int mhdp_hierarchy(const uint32_t dims4[4], uint32_t* level_dims, int max_levels, uint64_t* coarse_cells) {
+    if (!dims4) return -1;
+    uint64_t count = 1;
+    for (int a = 0; a < 3; a++) {
+        if (dims4[a] == 0 || dims4[a] > 0x7fffffffu) return -1;
+        count *= dims4[a];
+        if (count > 0xffffffffull) return -1;
+    }
+    if (count != dims4[3]) return -1;
+    uint32_t cur[4] = {dims4[0], dims4[1], dims4[2], dims4[3]};
+    int levels = 0;
+    uint64_t offset = 0;
+    for (;;) {
+        if (level_dims && levels < max_levels) std::memcpy(level_dims + 4 * levels, cur, sizeof cur);
+        levels++;
+        if (cur[0] <= 4 && cur[1] <= 4 && cur[2] <= 4) break;
+        for (int a = 0; a < 3; a++) cur[a] = (cur[a] + 1) / 2;
+        cur[3] = cur[0] * cur[1] * cur[2];
+        offset += cur[3];
+        if (offset > 0xffffffffull) return -1;
+    }
+    if (coarse_cells) *coarse_cells = offset;
+    return levels;
+}
+
+const char* mhdp_last_error(const mhdp_ctx* c) { return c ? c->err.c_str() : "invalid context"; }
+
+uint64_t mhdp_device_bytes(const mhdp_ctx* c) { return c ? c->bytes : 0; }
+
+void mhdp_destroy(mhdp_ctx* c) {
+    if (!c) return;
+    (void)hipDeviceSynchronize();
+    void* ptrs[] = {c->mask, c->alpha, c->bx, c->by, c->bz, c->diag, c->dinv, c->b, c->x, c->r, c->p, c->q, c->z,
+                    c->t, c->x64, c->b64, c->bx16, c->by16, c->bz16, c->dinv16, c->sc, c->part, c->ustar_stage,
+                    c->uout_stage, c->phi_stage};
+    for (void* p : ptrs) dfree(c, p);
+    for (CoarseArrays& a : c->coarse) {
+        void* q[] = {a.bx, a.by, a.bz, a.react, a.diag, a.dinv, a.bx16, a.by16, a.bz16, a.react16, a.dinv16,
+                     a.r, a.z, a.t};
+        for (void* p : q) dfree(c, p);
+    }
+    if (c->sc_h) (void)hipHostFree(c->sc_h);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+// ---------------------------------------------------------------- test hooks
+
+int mhdp_probe_coefficients(mhdp_ctx* c, int meanfree, float* beta3, float* diag, float* dinv) {
+    if (!c) return MHDP_INVALID;
+    int rc = prepare(c, meanfree != 0);
+    if (rc) return rc;
+    const size_t fb = (size_t)c->g.n * 4;
+    if (beta3) {
+        HIP_OK(hipMemcpyAsync(beta3, c->bx, fb, hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipMemcpyAsync(beta3 + c->g.n, c->by, fb, hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipMemcpyAsync(beta3 + 2 * (size_t)c->g.n, c->bz, fb, hipMemcpyDeviceToHost, c->stream));
+    }
+    if (diag) HIP_OK(hipMemcpyAsync(diag, c->diag, fb, hipMemcpyDeviceToHost, c->stream));
+    if (dinv) HIP_OK(hipMemcpyAsync(dinv, c->dinv, fb, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return MHDP_OK;
+}
+
+int mhdp_probe_apply(mhdp_ctx* c, int meanfree, const float* p, float* q) {
+    if (!c || !p || !q) return MHDP_INVALID;
+    int rc = prepare(c, meanfree != 0);
+    if (rc) return rc;
+    const size_t fb = (size_t)c->g.n * 4;
+    HIP_OK(hipMemcpyAsync(c->p, p, fb, hipMemcpyHostToDevice, c->stream));
+    launch_apply_plain(c->stream, c->g, c->p, c->bx, c->by, c->bz, c->dinv, 1, c->q);
+    HIP_OK(hipMemcpyAsync(q, c->q, fb, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    HIP_OK(hipGetLastError());
+    return MHDP_OK;
+}
+
+int mhdp_probe_vcycle(mhdp_ctx* c, int meanfree, int precision, const float* r, float* z) {
+    if (!c || !r || !z) return MHDP_INVALID;
+    const bool f16 = precision == MHDP_MIXED;
+    int rc = prepare(c, meanfree != 0);
+    if (rc) return rc;
+    if ((rc = alloc_mg(c, f16))) return rc;
+    build_mg(c, meanfree != 0, f16);
+    Scal s;
+    set_solve_params(c, s, meanfree != 0, 0.0, 0.0, 1.0);
+    if ((rc = upload_scal(c, s))) return rc;
+    const size_t fb = (size_t)c->g.n * 4;
+    HIP_OK(hipMemcpyAsync(c->r, r, fb, hipMemcpyHostToDevice, c->stream));
+    std::vector<LevelDev> lv = level_view(c);
+    void* zf = nullptr;
+    launch_vcycle(c->stream, f16 ? STORE_BF16 : STORE_F32, (int)lv.size(), lv.data(), c->r, c->mask, c->sc, f16 ? 1 : 0,
+                  &zf);
+    if (f16) {
+        launch_from_bf16(c->stream, c->g.n, zf, c->q);
+        HIP_OK(hipMemcpyAsync(z, c->q, fb, hipMemcpyDeviceToHost, c->stream));
+    } else {
+        HIP_OK(hipMemcpyAsync(z, zf, fb, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_OK(hipStreamSynchronize(c->stream));
+    HIP_OK(hipGetLastError());
+    return MHDP_OK;
+}
+
+int mhdp_probe_mg_graph(mhdp_ctx* c, int meanfree, int level, float* graph5, uint32_t* dims4) {
+    if (!c) return MHDP_INVALID;
+    int rc = prepare(c, meanfree != 0);
+    if (rc) return rc;
+    if ((rc = alloc_mg(c, false))) return rc;
+    if (level < 1 || level >= (int)c->levels.size()) return fail(c, MHDP_INVALID, "No such multigrid level");
+    build_mg(c, meanfree != 0, false);
+    const Grid3& g = c->levels[level];
+    if (dims4) {
+        dims4[0] = (uint32_t)g.nx;
+        dims4[1] = (uint32_t)g.ny;
+        dims4[2] = (uint32_t)g.nz;
+        dims4[3] = g.n;
+    }
+    if (graph5) {
+        CoarseArrays& a = c->coarse[level - 1];
+        const size_t fb = (size_t)g.n * 4;
+        const float* src[5] = {a.bx, a.by, a.bz, a.react, a.diag};
+        for (int k = 0; k < 5; k++)
+            HIP_OK(hipMemcpyAsync(graph5 + (size_t)k * g.n, src[k], fb, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return MHDP_OK;
+}
+
+int mhdp_probe_rhs(mhdp_ctx* c, int meanfree, const float* ustar, double* b) {
+    if (!c || !ustar || !b) return MHDP_INVALID;
+    int rc = prepare(c, meanfree != 0);
+    if (rc) return rc;
+    const size_t fb = face_count(c->grid) * sizeof(float);
+    if ((rc = dalloc(c, (void**)&c->ustar_stage, fb)) || (rc = dalloc(c, (void**)&c->b64, (size_t)c->g.n * 8)))
+        return rc;
+    HIP_OK(hipMemcpyAsync(c->ustar_stage, ustar, fb, hipMemcpyHostToDevice, c->stream));
+    launch_rhs_f64(c->stream, c->g, c->grid.h, c->mask, c->bx, c->by, c->bz, c->ustar_stage, c->b64);
+    HIP_OK(hipMemcpyAsync(b, c->b64, (size_t)c->g.n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return MHDP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,251 @@
+// Device-side building blocks shared by the projection kernels (gfx950 / CDNA4).
+//
+// Layouts in HBM (one projection context):
+//   cells      x + nx*(y + ny*z), 32-bit indices (grids up to 2^31 cells)
+//   beta_a[i]  mobility of the LOWER face of cell i along axis a (between i-e_a, wrapped,
+//              and i); 0 for a closed face.  The upper face of i is beta_a[i+e_a] (wrapped)
+//              or 0 at a non-periodic boundary.
+//   dinv[i]    1/diag for an active cell (a cell carrying pressure unknowns), 0 otherwise
+//              (solid cells and, in the pinned gauge, each component's anchor cell).
+//   faces      three SoA planes packed x|y|z, plane a has dims n + e_a.
+//
+// Work distribution: every cell-parallel kernel runs a fixed grid of blocks and gives each
+// block one contiguous range of cells.  Hardware dispatches consecutive workgroups round-robin
+// over the 8 XCDs, so the logical block index is remapped to keep each XCD on one contiguous
+// z-slab of the grid (its y/z stencil neighbours then hit that XCD's own L2).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <stdint.h>
+
+#define MHDP_BLOCK 256
+#define MHDP_XCDS 8
+
+struct Grid3 {
+    int nx, ny, nz;
+    uint32_t per;  // periodic bits
+    uint32_t n;    // cells
+};
+
+// Device scalar block — the reference's `scalars` buffer (solver.wgsl:11) in FP64,
+// plus the solve parameters and the reduction tickets.
+struct Scal {
+    double rz, rr, bb, sr;            // scalars[0]: r.z, r.r, b.b, sum r
+    double alpha, beta, conv, iters;  // scalars[1]
+    double meanfree, mean;            // scalars[2].xy
+    double tol2, abs2, bound, inv_nf; // Params.solve.xyz, Params.initial.w
+    double pq, rr_stash, rz_old, spare;
+    uint32_t anchor;                  // gauge cell of the single component (mean-free)
+    uint32_t pad;
+    uint32_t ticket[8];
+};
+
+enum Fin : int {
+    FIN_NONE = 0,
+    FIN_INITIAL,       // initial_reduce (solver.wgsl:198-203)
+    FIN_REFRESH_MEAN,  // refresh partials + residual mean (mean-free refresh)
+    FIN_INITIAL_SUB,   // initial_reduce after the initial mean-free subspace
+    FIN_ALPHA,         // alpha_reduce (solver.wgsl:211-215)
+    FIN_MEAN,          // pressure_residual_mean (pressure_subspace.wgsl:3-7)
+    FIN_BETA,          // beta_reduce (solver.wgsl:231-239): rz=x, rr=y of this reduction
+    FIN_STASH_RR,      // keep rr for the MG beta_reduce
+    FIN_BETA_MG,       // beta_reduce with rz from preconditioned_dot and the stashed rr
+    FIN_INITIAL_MG,    // initial_reduce after initial_direction
+    FIN_CHECK,         // check_reduce (solver.wgsl:324-328)
+    FIN_NORM64,        // outer refinement: rr of the FP64 residual
+};
+
+__device__ __forceinline__ float ldf(float v) { return v; }
+__device__ __forceinline__ float ldf(__hip_bfloat16 v) { return __bfloat162float(v); }
+__device__ __forceinline__ double ldd(double v) { return v; }
+template <class T> __device__ __forceinline__ T cvt(float v);
+template <> __device__ __forceinline__ float cvt<float>(float v) { return v; }
+template <> __device__ __forceinline__ __hip_bfloat16 cvt<__hip_bfloat16>(float v) { return __float2bfloat16(v); }
+
+// Logical block (XCD-aware) and the cell range it owns.
+__device__ __forceinline__ uint32_t logical_block() {
+    uint32_t b = blockIdx.x, G = gridDim.x;
+    uint32_t per = G / MHDP_XCDS;
+    if (per * MHDP_XCDS != G) return b;
+    return (b % MHDP_XCDS) * per + b / MHDP_XCDS;
+}
+__device__ __forceinline__ void block_range(uint32_t n, uint32_t& lo, uint32_t& hi) {
+    uint32_t G = gridDim.x;
+    uint32_t chunk = (n + G - 1) / G;
+    chunk = (chunk + MHDP_BLOCK - 1) / MHDP_BLOCK * MHDP_BLOCK;
+    uint32_t lb = logical_block();
+    uint64_t l = (uint64_t)lb * chunk;
+    uint64_t h = l + chunk;
+    lo = l < n ? (uint32_t)l : n;
+    hi = h < n ? (uint32_t)h : n;
+}
+
+__device__ __forceinline__ void decode(const Grid3& g, uint32_t i, int& x, int& y, int& z) {
+    uint32_t nxy = (uint32_t)g.nx * (uint32_t)g.ny;
+    z = (int)(i / nxy);
+    uint32_t r = i - (uint32_t)z * nxy;
+    y = (int)(r / (uint32_t)g.nx);
+    x = (int)(r - (uint32_t)y * (uint32_t)g.nx);
+}
+
+// Neighbour indices and face weights of cell i in the reference's summation order
+// x-, x+, y-, y+, z-, z+ (solver.wgsl:81).  Indices are always in range: a closed
+// boundary face gets weight 0 and the cell itself as neighbour.
+template <class CT>
+__device__ __forceinline__ void stencil6(const Grid3& g, int x, int y, int z, uint32_t i,
+                                         const CT* __restrict__ bx, const CT* __restrict__ by,
+                                         const CT* __restrict__ bz, uint32_t nb[6], float w[6]) {
+    const uint32_t sy = (uint32_t)g.nx, sz = (uint32_t)g.nx * (uint32_t)g.ny;
+    // x
+    nb[0] = x > 0 ? i - 1 : ((g.per & 1u) ? i + (uint32_t)(g.nx - 1) : i);
+    w[0] = ldf(bx[i]);
+    if (x + 1 < g.nx) { nb[1] = i + 1; w[1] = ldf(bx[i + 1]); }
+    else if (g.per & 1u) { nb[1] = i - (uint32_t)(g.nx - 1); w[1] = ldf(bx[nb[1]]); }
+    else { nb[1] = i; w[1] = 0.f; }
+    // y
+    nb[2] = y > 0 ? i - sy : ((g.per & 2u) ? i + (uint32_t)(g.ny - 1) * sy : i);
+    w[2] = ldf(by[i]);
+    if (y + 1 < g.ny) { nb[3] = i + sy; w[3] = ldf(by[i + sy]); }
+    else if (g.per & 2u) { nb[3] = i - (uint32_t)(g.ny - 1) * sy; w[3] = ldf(by[nb[3]]); }
+    else { nb[3] = i; w[3] = 0.f; }
+    // z
+    nb[4] = z > 0 ? i - sz : ((g.per & 4u) ? i + (uint32_t)(g.nz - 1) * sz : i);
+    w[4] = ldf(bz[i]);
+    if (z + 1 < g.nz) { nb[5] = i + sz; w[5] = ldf(bz[i + sz]); }
+    else if (g.per & 4u) { nb[5] = i - (uint32_t)(g.nz - 1) * sz; w[5] = ldf(bz[nb[5]]); }
+    else { nb[5] = i; w[5] = 0.f; }
+}
+
+// ---------------------------------------------------------------- reductions
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+// Fixed-order block sum of NV lanes; thread 0 receives the totals.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV]) {
+    __shared__ double red[MHDP_BLOCK / 64][NV];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double s = wave_sum(v[k]);
+        if (lane == 0) red[wid][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            double s = 0.0;
+            for (int w = 0; w < MHDP_BLOCK / 64; w++) s += red[w][k];
+            v[k] = s;
+        }
+    }
+    __syncthreads();
+}
+
+// The scalar updates of the reference's single-workgroup reduction kernels.
+__device__ inline void finalize(Scal* s, int mode, const double t[4]) {
+    switch (mode) {
+    case FIN_INITIAL:  // initial_reduce: scalars[0] = total; scalars[1] = (0, 0, conv, iters)
+        s->rz = t[0]; s->rr = t[1]; s->bb = t[2]; s->sr = t[3];
+        s->alpha = 0.0; s->beta = 0.0;
+        s->conv = (t[1] <= s->tol2 * t[2] + s->abs2 || t[2] == 0.0) ? 1.0 : 0.0;
+        break;
+    case FIN_REFRESH_MEAN:  // keep the physical residual and b.b; mean of r for the subspace step
+        s->rr = t[1]; s->bb = t[2]; s->sr = t[3];
+        s->mean = t[3] * s->inv_nf;
+        break;
+    case FIN_INITIAL_SUB:  // initial_reduce after pressure_initial_subspace (old.y, old.z kept)
+        s->rz = t[0];
+        s->alpha = 0.0; s->beta = 0.0;
+        s->conv = (s->rr <= s->tol2 * s->bb + s->abs2 || s->bb == 0.0) ? 1.0 : 0.0;
+        break;
+    case FIN_ALPHA:
+        s->pq = t[0];
+        s->alpha = s->conv > 0.5 ? 0.0 : s->rz / fmax(t[0], 1e-35);
+        break;
+    case FIN_MEAN:
+        s->mean = t[3] * s->inv_nf;
+        s->rr_stash = t[1];
+        break;
+    case FIN_BETA:
+    case FIN_BETA_MG:
+        if (s->conv < 0.5) {
+            double rr = mode == FIN_BETA ? t[1] : s->rr_stash;
+            s->iters += 1.0;
+            s->rz_old = s->rz;
+            s->beta = t[0] / fmax(s->rz, 1e-35);
+            s->rz = t[0];
+            s->rr = rr;
+            s->conv = rr <= (s->tol2 * s->bb + s->abs2) / s->bound ? 1.0 : 0.0;
+        }
+        break;
+    case FIN_STASH_RR:
+        s->rr_stash = t[1];
+        break;
+    case FIN_INITIAL_MG:  // initial_reduce after initial_direction: x lane = r.z
+        s->rz = t[0];
+        s->alpha = 0.0; s->beta = 0.0;
+        s->conv = (s->rr <= s->tol2 * s->bb + s->abs2 || s->bb == 0.0) ? 1.0 : 0.0;
+        break;
+    case FIN_CHECK:
+        s->rr = t[1];
+        s->conv = t[1] <= s->tol2 * s->bb + s->abs2 ? 1.0 : 0.0;
+        break;
+    case FIN_NORM64:  // FP64 outer residual: rr, b.b (in `spare`), sum r
+        s->rr = t[1];
+        s->spare = t[2];
+        s->sr = t[3];
+        break;
+    default:
+        break;
+    }
+}
+
+// Every block stores its partial; the last block to finish folds all partials in a
+// fixed order (deterministic) and runs the scalar update `mode` — one launch per
+// reference reduction pair (e.g. pressure_apply + alpha_reduce).
+template <int NV>
+__device__ __forceinline__ void reduce_finish(double (&v)[NV], double* __restrict__ part, Scal* s, int ticket,
+                                              int mode) {
+    block_sum<NV>(v);
+    __shared__ int last;
+    const uint32_t lb = logical_block();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) part[4 * lb + k] = k < NV ? v[k] : 0.0;
+        __threadfence();
+        unsigned t = atomicAdd(&s->ticket[ticket], 1u);
+        last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const volatile double* vp = part;
+    for (uint32_t j = threadIdx.x; j < gridDim.x; j += MHDP_BLOCK) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) acc[k] += vp[4 * j + k];
+    }
+    block_sum<4>(acc);
+    if (threadIdx.x == 0) {
+        finalize(s, mode, acc);
+        s->ticket[ticket] = 0u;
+        __threadfence();
+    }
+}
+
+// ---------------------------------------------------------------- launch helpers (host)
+
+struct Launch {
+    uint32_t blocks;
+};
+static inline uint32_t grid_blocks(uint32_t n, uint32_t cap = 4096) {
+    uint32_t need = (n + MHDP_BLOCK - 1) / MHDP_BLOCK;
+    uint32_t b = need < cap ? need : cap;
+    if (b >= 64) b = (b + MHDP_XCDS - 1) / MHDP_XCDS * MHDP_XCDS;
+    return b ? b : 1;
+}

@@ -1,0 +1,63 @@
+// Host-callable launchers of the projection kernels (one HIP stream per context).
+#pragma once
+#include "mhdp_device.h"
+
+struct PhaseP {
+    int two_phase;
+    float rho_l, rho_air;
+    int meanfree;
+};
+
+// Storage precision of the multigrid preconditioner.
+enum Store : int { STORE_F32 = 0, STORE_BF16 = 1 };
+
+// One multigrid level on the device.  Level 0 aliases the fine coefficient arrays.
+struct LevelDev {
+    Grid3 g;
+    float *bx, *by, *bz, *react, *diag, *dinv;  // FP32 graph (level 0: fine mobilities)
+    void *bx16, *by16, *bz16, *react16, *dinv16; // BF16 copies (mixed precision)
+    void *r, *z, *t;                              // level vectors (storage type of the cycle)
+};
+
+// setup.hip
+void launch_coefficients(hipStream_t s, const Grid3& g, const PhaseP& ph, const uint32_t* mask, const float* alpha,
+                         float* bx, float* by, float* bz, float* diag, float* dinv);
+void launch_rhs_f32(hipStream_t s, const Grid3& g, float h, const uint32_t* mask, const float* bx, const float* by,
+                    const float* bz, const float* ustar, float* b);
+void launch_rhs_f64(hipStream_t s, const Grid3& g, float h, const uint32_t* mask, const float* bx, const float* by,
+                    const float* bz, const float* ustar, double* b);
+void launch_correct_f32(hipStream_t s, const Grid3& g, float h, const float* bx, const float* by, const float* bz,
+                        const float* phi, const float* ustar, float* uout);
+void launch_correct_f64(hipStream_t s, const Grid3& g, float h, const float* bx, const float* by, const float* bz,
+                        const double* phi, const float* ustar, float* uout);
+void launch_mg_build(hipStream_t s, const Grid3& cg, const Grid3& fg, bool fine, int meanfree, const uint32_t* mask,
+                     const float* fdinv, const float* fbx, const float* fby, const float* fbz, const float* freact,
+                     float* cbx, float* cby, float* cbz, float* creact, float* cdiag, float* cdinv);
+void launch_to_bf16(hipStream_t s, uint32_t n, const float* src, void* dst);
+
+// pcg.hip
+void launch_refresh(hipStream_t s, const Grid3& g, const float* x, const float* b, const float* bx, const float* by,
+                    const float* bz, const float* dinv, float* r, float* p, double* part, Scal* sc, int meanfree,
+                    int check_only, int mode);
+void launch_subspace(hipStream_t s, const Grid3& g, float* r, float* p, const float* dinv, double* part, Scal* sc,
+                     int initial, int mode);
+void launch_apply(hipStream_t s, const Grid3& g, const float* p, const float* bx, const float* by, const float* bz,
+                  float* q, double* part, Scal* sc);
+void launch_update(hipStream_t s, const Grid3& g, float* x, float* r, const float* p, const float* q,
+                   const float* dinv, double* part, Scal* sc, int meanfree, int mode);
+void launch_search(hipStream_t s, const Grid3& g, const float* r, const float* dinv, float* p, Scal* sc);
+void launch_pdot(hipStream_t s, const Grid3& g, int store, const float* r, const void* z, const float* dinv, float* p,
+                 double* part, Scal* sc, int initial, int mode);
+void launch_psearch(hipStream_t s, const Grid3& g, int store, const void* z, const float* dinv, float* p, Scal* sc);
+void launch_residual64(hipStream_t s, const Grid3& g, const double* x64, const double* b64, const float* bx,
+                       const float* by, const float* bz, const float* dinv, float* r32, double* part, Scal* sc);
+void launch_xupdate64(hipStream_t s, const Grid3& g, double* x64, const float* e, const float* dinv);
+void launch_fill_f32(hipStream_t s, uint32_t n, float* dst, float v);
+void launch_fill_f64(hipStream_t s, uint32_t n, double* dst, double v);
+void launch_apply_plain(hipStream_t s, const Grid3& g, const float* p, const float* bx, const float* by,
+                        const float* bz, const float* dinv, int identity_inactive, float* q);
+void launch_cast_f32_f64(hipStream_t s, uint32_t n, const float* src, double* dst);
+
+// mg.hip — one V-cycle z = M r on the fine level (cycle_stages, multigrid.rs:67-82)
+void launch_vcycle(hipStream_t s, int store, int L, LevelDev* lv, const float* r0, const uint32_t* mask, Scal* sc,
+                   int coarse_f64, void** zfinal);

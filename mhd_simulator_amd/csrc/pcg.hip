@@ -1,0 +1,310 @@
+// Conjugate-gradient kernels of the projection: the cached 7-point operator with fused
+// dot products, the vector updates with fused norms, the mean-free residual projection and
+// the FP64 refinement residual.  Each reduction kernel ends with a deterministic
+// last-block fold that performs the scalar step of the reference's single-workgroup
+// reduction kernels (alpha_reduce, beta_reduce, ...), so no separate launch is needed.
+#include "mhdp_kernels.h"
+
+#define CONVERGED(sc) ((sc)->conv > 0.5)
+
+// pressure_refresh (solver.wgsl:185-197) and pressure_check (solver.wgsl:317-323):
+// t = b - L x on every cell (solid cells give 0, a pinned anchor its full physical row);
+// r = t on active cells, 0 elsewhere; p = r/d.  Partials (r^2/d, t^2, b^2, r).
+__global__ void __launch_bounds__(MHDP_BLOCK) k_refresh(Grid3 g, const float* __restrict__ x,
+                                                        const float* __restrict__ b, const float* __restrict__ bx,
+                                                        const float* __restrict__ by, const float* __restrict__ bz,
+                                                        const float* __restrict__ dinv, float* __restrict__ r,
+                                                        float* __restrict__ p, double* __restrict__ part, Scal* sc,
+                                                        int meanfree, int check_only, int mode) {
+    uint32_t lo, hi;
+    block_range(g.n, lo, hi);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+        int cx, cy, cz;
+        decode(g, i, cx, cy, cz);
+        uint32_t nb[6];
+        float w[6];
+        stencil6(g, cx, cy, cz, i, bx, by, bz, nb, w);
+        const float c = x[i];
+        float a = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 6; k++) a += w[k] * (c - x[nb[k]]);
+        const float bi = b[i];
+        const float t = bi - a;
+        const float di = dinv[i];
+        const float ri = di != 0.0f ? t : 0.0f;
+        if (!check_only) {
+            r[i] = ri;
+            p[i] = ri * di;
+        }
+        acc[0] += (double)ri * (double)ri * (double)di;
+        acc[1] += (double)t * (double)t;
+        acc[2] += (double)bi * (double)bi;
+        acc[3] += meanfree ? (double)ri : 0.0;
+    }
+    reduce_finish<4>(acc, part, sc, 0, mode);
+}
+
+// project_pressure_residual (pressure_subspace.wgsl:8-24): r -= mean on active cells;
+// initial form also restarts p = r/d.  Partials (r^2/d, r^2).
+__global__ void __launch_bounds__(MHDP_BLOCK) k_subspace(Grid3 g, float* __restrict__ r, float* __restrict__ p,
+                                                         const float* __restrict__ dinv, double* __restrict__ part,
+                                                         Scal* sc, int initial, int mode) {
+    if (!initial && CONVERGED(sc)) return;
+    uint32_t lo, hi;
+    block_range(g.n, lo, hi);
+    const float mean = (float)sc->mean;
+    double acc[2] = {0.0, 0.0};
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+        const float di = dinv[i];
+        if (di == 0.0f) continue;
+        const float ri = r[i] - mean;
+        r[i] = ri;
+        if (initial) p[i] = ri * di;
+        acc[0] += (double)ri * (double)ri * (double)di;
+        acc[1] += (double)ri * (double)ri;
+    }
+    reduce_finish<2>(acc, part, sc, 1, mode);
+}
+
+// pressure_apply + alpha_reduce (solver.wgsl:204-215): q = A p, alpha = r.z / p.q.
+__global__ void __launch_bounds__(MHDP_BLOCK) k_apply(Grid3 g, const float* __restrict__ p,
+                                                      const float* __restrict__ bx, const float* __restrict__ by,
+                                                      const float* __restrict__ bz, float* __restrict__ q,
+                                                      double* __restrict__ part, Scal* sc) {
+    if (CONVERGED(sc)) return;
+    uint32_t lo, hi;
+    block_range(g.n, lo, hi);
+    double acc[1] = {0.0};
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+        int cx, cy, cz;
+        decode(g, i, cx, cy, cz);
+        uint32_t nb[6];
+        float w[6];
+        stencil6(g, cx, cy, cz, i, bx, by, bz, nb, w);
+        const float c = p[i];
+        float a = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 6; k++) a += w[k] * (c - p[nb[k]]);
+        q[i] = a;
+        acc[0] += (double)c * (double)a;
+    }
+    reduce_finish<1>(acc, part, sc, 2, FIN_ALPHA);
+}
+
+// update (solver.wgsl:216-230): x += alpha p (mean-free: alpha (p - p[anchor])), r -= alpha q.
+// Partials (r^2/d, r^2, 0, r).
+__global__ void __launch_bounds__(MHDP_BLOCK) k_update(Grid3 g, float* __restrict__ x, float* __restrict__ r,
+                                                       const float* __restrict__ p, const float* __restrict__ q,
+                                                       const float* __restrict__ dinv, double* __restrict__ part,
+                                                       Scal* sc, int meanfree, int mode) {
+    if (CONVERGED(sc)) return;
+    uint32_t lo, hi;
+    block_range(g.n, lo, hi);
+    const float alpha = (float)sc->alpha;
+    const float pa = meanfree ? p[sc->anchor] : 0.0f;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+        const float di = dinv[i];
+        if (di == 0.0f) continue;
+        const float ri = r[i] - alpha * q[i];
+        x[i] = x[i] + alpha * (p[i] - pa);
+        r[i] = ri;
+        acc[0] += (double)ri * (double)ri * (double)di;
+        acc[1] += (double)ri * (double)ri;
+        acc[3] += (double)ri;
+    }
+    reduce_finish<4>(acc, part, sc, 3, mode);
+}
+
+// search (solver.wgsl:240-244): p = r/d + beta p.
+__global__ void __launch_bounds__(MHDP_BLOCK) k_search(Grid3 g, const float* __restrict__ r,
+                                                       const float* __restrict__ dinv, float* __restrict__ p,
+                                                       Scal* sc) {
+    if (CONVERGED(sc)) return;
+    uint32_t lo, hi;
+    block_range(g.n, lo, hi);
+    const float beta = (float)sc->beta;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+        const float di = dinv[i];
+        if (di == 0.0f) continue;
+        p[i] = r[i] * di + beta * p[i];
+    }
+}
+
+// dot_partial (multigrid.wgsl:90-100) + the following initial_reduce / beta_reduce.
+template <class VT>
+__global__ void __launch_bounds__(MHDP_BLOCK) k_pdot(Grid3 g, const float* __restrict__ r,
+                                                     const VT* __restrict__ z, const float* __restrict__ dinv,
+                                                     float* __restrict__ p, double* __restrict__ part, Scal* sc,
+                                                     int initial, int mode) {
+    if (CONVERGED(sc)) return;
+    uint32_t lo, hi;
+    block_range(g.n, lo, hi);
+    double acc[1] = {0.0};
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+        if (dinv[i] == 0.0f) continue;
+        const float zi = ldf(z[i]);
+        acc[0] += (double)r[i] * (double)zi;
+        if (initial) p[i] = zi;
+    }
+    reduce_finish<1>(acc, part, sc, 4, mode);
+}
+
+// preconditioned_search (multigrid.wgsl:102): p = z + beta p.
+template <class VT>
+__global__ void __launch_bounds__(MHDP_BLOCK) k_psearch(Grid3 g, const VT* __restrict__ z,
+                                                        const float* __restrict__ dinv, float* __restrict__ p,
+                                                        Scal* sc) {
+    if (CONVERGED(sc)) return;
+    uint32_t lo, hi;
+    block_range(g.n, lo, hi);
+    const float beta = (float)sc->beta;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+        if (dinv[i] == 0.0f) continue;
+        p[i] = ldf(z[i]) + beta * p[i];
+    }
+}
+
+// FP64 refinement residual: t = b - L x in FP64 (FP32 mobilities promoted) on every cell;
+// the next inner FP32 right-hand side is r32 = t on active cells.  Partials (0, t^2, b^2, r).
+__global__ void __launch_bounds__(MHDP_BLOCK) k_residual64(Grid3 g, const double* __restrict__ x,
+                                                           const double* __restrict__ b, const float* __restrict__ bx,
+                                                           const float* __restrict__ by, const float* __restrict__ bz,
+                                                           const float* __restrict__ dinv, float* __restrict__ r32,
+                                                           double* __restrict__ part, Scal* sc) {
+    uint32_t lo, hi;
+    block_range(g.n, lo, hi);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+        int cx, cy, cz;
+        decode(g, i, cx, cy, cz);
+        uint32_t nb[6];
+        float w[6];
+        stencil6(g, cx, cy, cz, i, bx, by, bz, nb, w);
+        const double c = x[i];
+        double a = 0.0;
+#pragma unroll
+        for (int k = 0; k < 6; k++) a += (double)w[k] * (c - x[nb[k]]);
+        const double bi = b[i];
+        const double t = bi - a;
+        const bool active = dinv[i] != 0.0f;
+        r32[i] = active ? (float)t : 0.0f;
+        acc[1] += t * t;
+        acc[2] += bi * bi;
+        acc[3] += active ? t : 0.0;
+    }
+    reduce_finish<4>(acc, part, sc, 5, FIN_NORM64);
+}
+
+__global__ void __launch_bounds__(MHDP_BLOCK) k_xupdate64(Grid3 g, double* __restrict__ x,
+                                                          const float* __restrict__ e,
+                                                          const float* __restrict__ dinv) {
+    uint32_t lo, hi;
+    block_range(g.n, lo, hi);
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK)
+        if (dinv[i] != 0.0f) x[i] += (double)e[i];
+}
+
+template <class T>
+__global__ void __launch_bounds__(MHDP_BLOCK) k_fill(uint32_t n, T* __restrict__ dst, T v) {
+    uint32_t lo, hi;
+    block_range(n, lo, hi);
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) dst[i] = v;
+}
+
+__global__ void __launch_bounds__(MHDP_BLOCK) k_cast_f32_f64(uint32_t n, const float* __restrict__ s,
+                                                             double* __restrict__ d) {
+    uint32_t lo, hi;
+    block_range(n, lo, hi);
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) d[i] = (double)s[i];
+}
+
+// Plain operator application (test hook): q = sum beta (p_c - p_n) on every cell; with
+// identity_inactive, inactive cells return p (the pinned-anchor identity row, solver.wgsl:79).
+__global__ void __launch_bounds__(MHDP_BLOCK) k_apply_plain(Grid3 g, const float* __restrict__ p,
+                                                            const float* __restrict__ bx,
+                                                            const float* __restrict__ by,
+                                                            const float* __restrict__ bz,
+                                                            const float* __restrict__ dinv, int identity_inactive,
+                                                            float* __restrict__ q) {
+    uint32_t lo, hi;
+    block_range(g.n, lo, hi);
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+        int cx, cy, cz;
+        decode(g, i, cx, cy, cz);
+        uint32_t nb[6];
+        float w[6];
+        stencil6(g, cx, cy, cz, i, bx, by, bz, nb, w);
+        const float c = p[i];
+        float a = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 6; k++) a += w[k] * (c - p[nb[k]]);
+        q[i] = (identity_inactive && dinv[i] == 0.0f) ? c : a;
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+
+void launch_refresh(hipStream_t s, const Grid3& g, const float* x, const float* b, const float* bx, const float* by,
+                    const float* bz, const float* dinv, float* r, float* p, double* part, Scal* sc, int meanfree,
+                    int check_only, int mode) {
+    hipLaunchKernelGGL(k_refresh, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g, x, b, bx, by, bz, dinv, r, p,
+                       part, sc, meanfree, check_only, mode);
+}
+void launch_subspace(hipStream_t s, const Grid3& g, float* r, float* p, const float* dinv, double* part, Scal* sc,
+                     int initial, int mode) {
+    hipLaunchKernelGGL(k_subspace, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g, r, p, dinv, part, sc, initial,
+                       mode);
+}
+void launch_apply(hipStream_t s, const Grid3& g, const float* p, const float* bx, const float* by, const float* bz,
+                  float* q, double* part, Scal* sc) {
+    hipLaunchKernelGGL(k_apply, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g, p, bx, by, bz, q, part, sc);
+}
+void launch_update(hipStream_t s, const Grid3& g, float* x, float* r, const float* p, const float* q,
+                   const float* dinv, double* part, Scal* sc, int meanfree, int mode) {
+    hipLaunchKernelGGL(k_update, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g, x, r, p, q, dinv, part, sc,
+                       meanfree, mode);
+}
+void launch_search(hipStream_t s, const Grid3& g, const float* r, const float* dinv, float* p, Scal* sc) {
+    hipLaunchKernelGGL(k_search, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g, r, dinv, p, sc);
+}
+void launch_pdot(hipStream_t s, const Grid3& g, int store, const float* r, const void* z, const float* dinv, float* p,
+                 double* part, Scal* sc, int initial, int mode) {
+    if (store == STORE_BF16)
+        hipLaunchKernelGGL(k_pdot<__hip_bfloat16>, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g, r,
+                           (const __hip_bfloat16*)z, dinv, p, part, sc, initial, mode);
+    else
+        hipLaunchKernelGGL(k_pdot<float>, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g, r, (const float*)z, dinv,
+                           p, part, sc, initial, mode);
+}
+void launch_psearch(hipStream_t s, const Grid3& g, int store, const void* z, const float* dinv, float* p, Scal* sc) {
+    if (store == STORE_BF16)
+        hipLaunchKernelGGL(k_psearch<__hip_bfloat16>, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g,
+                           (const __hip_bfloat16*)z, dinv, p, sc);
+    else
+        hipLaunchKernelGGL(k_psearch<float>, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g, (const float*)z, dinv,
+                           p, sc);
+}
+void launch_residual64(hipStream_t s, const Grid3& g, const double* x64, const double* b64, const float* bx,
+                       const float* by, const float* bz, const float* dinv, float* r32, double* part, Scal* sc) {
+    hipLaunchKernelGGL(k_residual64, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g, x64, b64, bx, by, bz, dinv,
+                       r32, part, sc);
+}
+void launch_xupdate64(hipStream_t s, const Grid3& g, double* x64, const float* e, const float* dinv) {
+    hipLaunchKernelGGL(k_xupdate64, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g, x64, e, dinv);
+}
+void launch_fill_f32(hipStream_t s, uint32_t n, float* dst, float v) {
+    hipLaunchKernelGGL(k_fill<float>, dim3(grid_blocks(n)), dim3(MHDP_BLOCK), 0, s, n, dst, v);
+}
+void launch_fill_f64(hipStream_t s, uint32_t n, double* dst, double v) {
+    hipLaunchKernelGGL(k_fill<double>, dim3(grid_blocks(n)), dim3(MHDP_BLOCK), 0, s, n, dst, v);
+}
+void launch_apply_plain(hipStream_t s, const Grid3& g, const float* p, const float* bx, const float* by,
+                        const float* bz, const float* dinv, int identity_inactive, float* q) {
+    hipLaunchKernelGGL(k_apply_plain, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g, p, bx, by, bz, dinv,
+                       identity_inactive, q);
+}
+void launch_cast_f32_f64(hipStream_t s, uint32_t n, const float* src, double* dst) {
+    hipLaunchKernelGGL(k_cast_f32_f64, dim3(grid_blocks(n)), dim3(MHDP_BLOCK), 0, s, n, src, dst);
+}

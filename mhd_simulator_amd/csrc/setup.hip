@@ -1,0 +1,324 @@
+// Once-per-projection kernels: face mobilities + diagonal, divergence RHS, velocity
+// correction and the Galerkin coarse graphs.  Built with -ffp-contract=off: their FP32
+// results are bit-identical to the reference formulas evaluated in IEEE order.
+#include "mhdp_kernels.h"
+
+namespace {
+
+__device__ __forceinline__ int wrap1(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+struct Pos {
+    int p[3];
+};
+
+__device__ __forceinline__ bool canon_valid(const Grid3& g, Pos& q) {
+    const int d[3] = {g.nx, g.ny, g.nz};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        if (g.per & (1u << a)) q.p[a] = wrap1(q.p[a], d[a]);
+        if (q.p[a] < 0 || q.p[a] >= d[a]) return false;
+    }
+    return true;
+}
+__device__ __forceinline__ uint32_t cidx(const Grid3& g, const Pos& q) {
+    return (uint32_t)q.p[0] + (uint32_t)g.nx * ((uint32_t)q.p[1] + (uint32_t)g.ny * (uint32_t)q.p[2]);
+}
+// fluid(), solver.wgsl:29
+__device__ __forceinline__ bool fluid_at(const Grid3& g, const uint32_t* __restrict__ mask, Pos q) {
+    if (!canon_valid(g, q)) return false;
+    return mask[cidx(g, q)] != 0u;
+}
+// occupancy()/density(), solver.wgsl:40-41
+__device__ __forceinline__ float density_at(const Grid3& g, const PhaseP& ph, const float* __restrict__ alpha, Pos q) {
+    float a = 0.0f;
+    if (canon_valid(g, q)) a = alpha ? alpha[cidx(g, q)] : 0.0f;
+    return ph.rho_air * (1.0f - a) + ph.rho_l * a;
+}
+// mobility(), solver.wgsl:43-51 (mushy factor off)
+__device__ __forceinline__ float mobility_at(const Grid3& g, const PhaseP& ph, const float* __restrict__ alpha, Pos pos,
+                                             int axis) {
+    if (!ph.two_phase) return 1.0f;
+    Pos left = pos;
+    left.p[axis] -= 1;
+    float fd = 0.5f * (density_at(g, ph, alpha, left) + density_at(g, ph, alpha, pos));
+    return ph.rho_l / fd;
+}
+
+}  // namespace
+
+// pressure_coefficients (solver.wgsl:93-103) in cell-indexed lower-face form, plus
+// diagonal() (solver.wgsl:60-64) and the active-cell reciprocal.
+__global__ void __launch_bounds__(MHDP_BLOCK) k_coefficients(Grid3 g, PhaseP ph, const uint32_t* __restrict__ mask,
+                                                             const float* __restrict__ alpha, float* __restrict__ bx,
+                                                             float* __restrict__ by, float* __restrict__ bz,
+                                                             float* __restrict__ diag, float* __restrict__ dinv) {
+    uint32_t lo, hi;
+    block_range(g.n, lo, hi);
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+        int x, y, z;
+        decode(g, i, x, y, z);
+        Pos pos{{x, y, z}};
+        const uint32_t m = mask[i];
+        const bool fc = m != 0u;
+        float b[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            Pos left = pos;
+            left.p[a] -= 1;
+            b[a] = (fc && fluid_at(g, mask, left)) ? mobility_at(g, ph, alpha, pos, a) : 0.0f;
+        }
+        bx[i] = b[0];
+        by[i] = b[1];
+        bz[i] = b[2];
+        float d = 0.0f;
+        if (fc) {
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+#pragma unroll
+                for (int s = -1; s <= 1; s += 2) {
+                    Pos q = pos;
+                    q.p[a] += s;
+                    if (fluid_at(g, mask, q)) {
+                        Pos fp = pos;
+                        if (s > 0) fp.p[a] += 1;
+                        d += mobility_at(g, ph, alpha, fp, a);
+                    }
+                }
+            }
+            if (!ph.meanfree && (m & 3u) == 2u) d = 1.0f;
+            else d = fmaxf(d, 1e-30f);
+        }
+        diag[i] = d;
+        const bool active = fc && (ph.meanfree || (m & 3u) == 1u);
+        dinv[i] = active ? 1.0f / d : 0.0f;
+    }
+}
+
+// Canonical face index in plane `a` of the upper face of cell (x,y,z) along a.
+__device__ __forceinline__ uint32_t plane_index(const Grid3& g, int a, int fx, int fy, int fz) {
+    const uint32_t Px = (uint32_t)g.nx + (a == 0), Py = (uint32_t)g.ny + (a == 1);
+    return (uint32_t)fx + Px * ((uint32_t)fy + Py * (uint32_t)fz);
+}
+
+// rhs() without the pinned-anchor zeroing (solver.wgsl:104-107, 192): b = -h^2 div(u*)
+// over open faces.  Openness is read from the mobilities (a face is open iff beta > 0).
+template <class T>
+__global__ void __launch_bounds__(MHDP_BLOCK) k_rhs(Grid3 g, float h, const uint32_t* __restrict__ mask,
+                                                    const float* __restrict__ bx, const float* __restrict__ by,
+                                                    const float* __restrict__ bz, const float* __restrict__ ustar,
+                                                    uint64_t off1, uint64_t off2, T* __restrict__ b) {
+    uint32_t lo, hi;
+    block_range(g.n, lo, hi);
+    const float* U[3] = {ustar, ustar + off1, ustar + off2};
+    const float* B[3] = {bx, by, bz};
+    const int d[3] = {g.nx, g.ny, g.nz};
+    const uint32_t st[3] = {1u, (uint32_t)g.nx, (uint32_t)g.nx * (uint32_t)g.ny};
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+        T out = (T)0;
+        if (mask[i] != 0u) {
+            int x, y, z;
+            decode(g, i, x, y, z);
+            int c[3] = {x, y, z};
+            T sum = (T)0;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                // upper face: cell c + e_a (wrapped) lower face
+                float up = 0.0f;
+                int u[3] = {c[0], c[1], c[2]};
+                u[a] += 1;
+                bool exists = true;
+                uint32_t ui = i + st[a];
+                if (u[a] >= d[a]) {
+                    if (g.per & (1u << a)) { u[a] = 0; ui = i - (uint32_t)(d[a] - 1) * st[a]; }
+                    else exists = false;
+                }
+                if (exists && B[a][ui] > 0.0f) up = U[a][plane_index(g, a, u[0], u[1], u[2])];
+                float dn = 0.0f;
+                if (B[a][i] > 0.0f) dn = U[a][plane_index(g, a, c[0], c[1], c[2])];
+                sum += (T)up - (T)dn;
+            }
+            T div = sum / (T)h;
+            out = -(T)h * (T)h * div;
+        }
+        b[i] = out;
+    }
+}
+
+// correct() (solver.wgsl:245-250) over every entry of the three face planes.
+template <class PT>
+__global__ void __launch_bounds__(MHDP_BLOCK) k_correct(Grid3 g, float h, const float* __restrict__ bx,
+                                                        const float* __restrict__ by, const float* __restrict__ bz,
+                                                        const PT* __restrict__ phi, const float* __restrict__ ustar,
+                                                        float* __restrict__ uout, uint32_t f0, uint32_t f1,
+                                                        uint32_t f2) {
+    const uint32_t total = f0 + f1 + f2;
+    uint32_t lo, hi;
+    block_range(total, lo, hi);
+    const float* B[3] = {bx, by, bz};
+    const int d[3] = {g.nx, g.ny, g.nz};
+    const uint32_t st[3] = {1u, (uint32_t)g.nx, (uint32_t)g.nx * (uint32_t)g.ny};
+    for (uint32_t f = lo + threadIdx.x; f < hi; f += MHDP_BLOCK) {
+        int a;
+        uint32_t k;
+        if (f < f0) { a = 0; k = f; }
+        else if (f < f0 + f1) { a = 1; k = f - f0; }
+        else { a = 2; k = f - f0 - f1; }
+        const uint32_t Px = (uint32_t)g.nx + (a == 0), Py = (uint32_t)g.ny + (a == 1);
+        int q[3];
+        q[0] = (int)(k % Px);
+        q[1] = (int)((k / Px) % Py);
+        q[2] = (int)(k / (Px * Py));
+        float out = 0.0f;
+        bool ok = true;
+        if (q[a] >= d[a]) {
+            if (g.per & (1u << a)) q[a] = 0;
+            else ok = false;
+        }
+        if (ok) {
+            const uint32_t ci = (uint32_t)q[0] + (uint32_t)g.nx * ((uint32_t)q[1] + (uint32_t)g.ny * (uint32_t)q[2]);
+            const float beta = B[a][ci];
+            if (beta > 0.0f) {
+                uint32_t li = q[a] > 0 ? ci - st[a] : ci + (uint32_t)(d[a] - 1) * st[a];
+                const float* U = ustar + (a == 0 ? 0 : (a == 1 ? f0 : f0 + f1));
+                const float uf = U[plane_index(g, a, q[0], q[1], q[2])];
+                if (sizeof(PT) == 4) {
+                    out = uf - (float)beta * ((float)phi[ci] - (float)phi[li]) / h;
+                } else {
+                    double o = (double)uf - (double)beta * ((double)phi[ci] - (double)phi[li]) / (double)h;
+                    out = (float)o;
+                }
+            }
+        }
+        uout[f] = out;
+    }
+}
+
+// build() (multigrid.wgsl:50-65): exact piecewise-constant Galerkin graph of one coarse
+// level.  FINE: children are the fine cells (cached mobilities, active mask, anchors
+// become reactions in the pinned gauge); otherwise children are the previous coarse level.
+template <bool FINE>
+__global__ void __launch_bounds__(MHDP_BLOCK) k_mg_build(Grid3 cg, Grid3 fg, int meanfree,
+                                                         const uint32_t* __restrict__ mask,
+                                                         const float* __restrict__ fdinv,
+                                                         const float* __restrict__ fbx, const float* __restrict__ fby,
+                                                         const float* __restrict__ fbz,
+                                                         const float* __restrict__ freact, float* __restrict__ cbx,
+                                                         float* __restrict__ cby, float* __restrict__ cbz,
+                                                         float* __restrict__ creact, float* __restrict__ cdiag,
+                                                         float* __restrict__ cdinv) {
+    uint32_t lo, hi;
+    block_range(cg.n, lo, hi);
+    const float* FB[3] = {fbx, fby, fbz};
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+        int qx, qy, qz;
+        decode(cg, i, qx, qy, qz);
+        float c4[4] = {0.f, 0.f, 0.f, 0.f};
+        float diag = 0.f;
+        for (int z = 0; z < 2; z++)
+            for (int y = 0; y < 2; y++)
+                for (int x = 0; x < 2; x++) {
+                    Pos child{{2 * qx + x, 2 * qy + y, 2 * qz + z}};
+                    if (child.p[0] >= fg.nx || child.p[1] >= fg.ny || child.p[2] >= fg.nz) continue;
+                    const uint32_t ci = cidx(fg, child);
+                    if (FINE) {
+                        if (fdinv[ci] == 0.0f) continue;
+                    } else {
+                        const float gamma = freact[ci];
+                        c4[3] += gamma;
+                        diag += gamma;
+                    }
+                    for (int a = 0; a < 3; a++)
+                        for (int s = -1; s <= 1; s += 2) {
+                            Pos n = child;
+                            n.p[a] += s;
+                            if (!canon_valid(fg, n)) continue;
+                            const uint32_t ni = cidx(fg, n);
+                            const float beta = s > 0 ? FB[a][ni] : FB[a][ci];
+                            if (beta == 0.0f) continue;
+                            if (FINE && !meanfree && (mask[ni] & 3u) == 2u) {
+                                c4[3] += beta;
+                                diag += beta;
+                                continue;
+                            }
+                            if (n.p[0] / 2 == qx && n.p[1] / 2 == qy && n.p[2] / 2 == qz) continue;
+                            diag += beta;
+                            if (s < 0) c4[a] += beta;
+                        }
+                }
+        cbx[i] = c4[0];
+        cby[i] = c4[1];
+        cbz[i] = c4[2];
+        creact[i] = c4[3];
+        cdiag[i] = diag;
+        cdinv[i] = diag > 0.0f ? 1.0f / diag : 0.0f;
+    }
+}
+
+__global__ void __launch_bounds__(MHDP_BLOCK) k_to_bf16(uint32_t n, const float* __restrict__ src,
+                                                        __hip_bfloat16* __restrict__ dst) {
+    uint32_t lo, hi;
+    block_range(n, lo, hi);
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) dst[i] = __float2bfloat16(src[i]);
+}
+
+__global__ void __launch_bounds__(MHDP_BLOCK) k_from_bf16(uint32_t n, const __hip_bfloat16* __restrict__ src,
+                                                          float* __restrict__ dst) {
+    uint32_t lo, hi;
+    block_range(n, lo, hi);
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) dst[i] = __bfloat162float(src[i]);
+}
+
+// ---------------------------------------------------------------- launchers
+
+void launch_coefficients(hipStream_t s, const Grid3& g, const PhaseP& ph, const uint32_t* mask, const float* alpha,
+                         float* bx, float* by, float* bz, float* diag, float* dinv) {
+    hipLaunchKernelGGL(k_coefficients, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g, ph, mask, alpha, bx, by, bz,
+                       diag, dinv);
+}
+void launch_rhs_f32(hipStream_t s, const Grid3& g, float h, const uint32_t* mask, const float* bx, const float* by,
+                    const float* bz, const float* ustar, float* b) {
+    uint64_t f0 = (uint64_t)(g.nx + 1) * g.ny * g.nz, f1 = (uint64_t)g.nx * (g.ny + 1) * g.nz;
+    hipLaunchKernelGGL(k_rhs<float>, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g, h, mask, bx, by, bz, ustar,
+                       f0, f0 + f1, b);
+}
+void launch_rhs_f64(hipStream_t s, const Grid3& g, float h, const uint32_t* mask, const float* bx, const float* by,
+                    const float* bz, const float* ustar, double* b) {
+    uint64_t f0 = (uint64_t)(g.nx + 1) * g.ny * g.nz, f1 = (uint64_t)g.nx * (g.ny + 1) * g.nz;
+    hipLaunchKernelGGL(k_rhs<double>, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g, h, mask, bx, by, bz, ustar,
+                       f0, f0 + f1, b);
+}
+static void face_counts(const Grid3& g, uint32_t& f0, uint32_t& f1, uint32_t& f2) {
+    f0 = (uint32_t)((g.nx + 1) * (uint64_t)g.ny * g.nz);
+    f1 = (uint32_t)(g.nx * (uint64_t)(g.ny + 1) * g.nz);
+    f2 = (uint32_t)(g.nx * (uint64_t)g.ny * (g.nz + 1));
+}
+void launch_correct_f32(hipStream_t s, const Grid3& g, float h, const float* bx, const float* by, const float* bz,
+                        const float* phi, const float* ustar, float* uout) {
+    uint32_t f0, f1, f2;
+    face_counts(g, f0, f1, f2);
+    hipLaunchKernelGGL(k_correct<float>, dim3(grid_blocks(f0 + f1 + f2)), dim3(MHDP_BLOCK), 0, s, g, h, bx, by, bz,
+                       phi, ustar, uout, f0, f1, f2);
+}
+void launch_correct_f64(hipStream_t s, const Grid3& g, float h, const float* bx, const float* by, const float* bz,
+                        const double* phi, const float* ustar, float* uout) {
+    uint32_t f0, f1, f2;
+    face_counts(g, f0, f1, f2);
+    hipLaunchKernelGGL(k_correct<double>, dim3(grid_blocks(f0 + f1 + f2)), dim3(MHDP_BLOCK), 0, s, g, h, bx, by, bz,
+                       phi, ustar, uout, f0, f1, f2);
+}
+void launch_mg_build(hipStream_t s, const Grid3& cg, const Grid3& fg, bool fine, int meanfree, const uint32_t* mask,
+                     const float* fdinv, const float* fbx, const float* fby, const float* fbz, const float* freact,
+                     float* cbx, float* cby, float* cbz, float* creact, float* cdiag, float* cdinv) {
+    if (fine)
+        hipLaunchKernelGGL(k_mg_build<true>, dim3(grid_blocks(cg.n)), dim3(MHDP_BLOCK), 0, s, cg, fg, meanfree, mask,
+                           fdinv, fbx, fby, fbz, freact, cbx, cby, cbz, creact, cdiag, cdinv);
+    else
+        hipLaunchKernelGGL(k_mg_build<false>, dim3(grid_blocks(cg.n)), dim3(MHDP_BLOCK), 0, s, cg, fg, meanfree, mask,
+                           fdinv, fbx, fby, fbz, freact, cbx, cby, cbz, creact, cdiag, cdinv);
+}
+void launch_to_bf16(hipStream_t s, uint32_t n, const float* src, void* dst) {
+    hipLaunchKernelGGL(k_to_bf16, dim3(grid_blocks(n)), dim3(MHDP_BLOCK), 0, s, n, src, (__hip_bfloat16*)dst);
+}
+void launch_from_bf16(hipStream_t s, uint32_t n, const void* src, float* dst) {
+    hipLaunchKernelGGL(k_from_bf16, dim3(grid_blocks(n)), dim3(MHDP_BLOCK), 0, s, n, (const __hip_bfloat16*)src, dst);
+}

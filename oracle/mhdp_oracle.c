@@ -1,0 +1,373 @@
+/*
+ * CPU oracle for the MAC pressure projection — TEST INFRASTRUCTURE ONLY
+ * (checker and CPU baseline; never part of the product library).
+ * Shared grid helpers; the precision-generic solver body lives in core.inc.
+ */
+#include "mhdp_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef struct {
+    uint32_t d[3];
+    uint64_t n;
+    uint32_t periodic;
+    const uint32_t *mask;
+    const float *alpha;
+    float h, rho_l, rho_air;
+    int two_phase;
+    int meanfree;
+} ctx_t;
+
+static void ctx_init(ctx_t *c, const orc_grid *g, const uint32_t *mask, const float *alpha, int meanfree) {
+    c->d[0] = g->nx; c->d[1] = g->ny; c->d[2] = g->nz;
+    c->n = (uint64_t)g->nx * g->ny * g->nz;
+    c->periodic = g->periodic;
+    c->mask = mask;
+    c->alpha = alpha;
+    c->h = g->h;
+    c->rho_l = g->rho_liquid;
+    c->rho_air = g->rho_air;
+    c->two_phase = g->two_phase;
+    c->meanfree = meanfree;
+}
+
+/* src/periodic.wgsl:5-10 — unsigned Euclidean wrap */
+static inline int32_t wrap_periodic(int32_t value, uint32_t size) {
+    if (value >= 0) return (int32_t)((uint32_t)value % size);
+    uint32_t magnitude = (uint32_t)(-(value + 1)) + 1u;
+    uint32_t remainder = magnitude % size;
+    return (int32_t)(remainder == 0u ? 0u : size - remainder);
+}
+/* solver.wgsl:23-27 wrap_cell; multigrid.wgsl:18-20 canonical */
+static inline void wrap_pos(uint32_t periodic, const uint32_t d[3], int32_t q[3]) {
+    for (int a = 0; a < 3; a++)
+        if (periodic & (1u << a)) q[a] = wrap_periodic(q[a], d[a]);
+}
+static inline int valid_pos(const uint32_t d[3], const int32_t q[3]) {
+    return q[0] >= 0 && q[1] >= 0 && q[2] >= 0 && q[0] < (int32_t)d[0] && q[1] < (int32_t)d[1] &&
+           q[2] < (int32_t)d[2];
+}
+static inline uint64_t idx3(const uint32_t d[3], const int32_t q[3]) {
+    return (uint64_t)q[0] + (uint64_t)d[0] * ((uint64_t)q[1] + (uint64_t)d[1] * (uint64_t)q[2]);
+}
+static inline void decode3(uint64_t i, const uint32_t d[3], int32_t q[3]) {
+    q[0] = (int32_t)(i % d[0]);
+    q[1] = (int32_t)((i / d[0]) % d[1]);
+    q[2] = (int32_t)(i / ((uint64_t)d[0] * d[1]));
+}
+static inline void plane_dims(const uint32_t d[3], int axis, uint32_t P[3]) {
+    P[0] = d[0]; P[1] = d[1]; P[2] = d[2];
+    P[axis] += 1;
+}
+static inline uint64_t plane_size(const uint32_t d[3], int axis) {
+    uint32_t P[3];
+    plane_dims(d, axis, P);
+    return (uint64_t)P[0] * P[1] * P[2];
+}
+static inline uint64_t plane_offset(const uint32_t d[3], int axis) {
+    uint64_t o = 0;
+    for (int a = 0; a < axis; a++) o += plane_size(d, a);
+    return o;
+}
+/* index of a canonical face position in its SoA plane */
+static inline uint64_t face_idx(const uint32_t d[3], int axis, const int32_t q[3]) {
+    uint32_t P[3];
+    plane_dims(d, axis, P);
+    return plane_offset(d, axis) + idx3(P, q);
+}
+
+uint64_t orc_face_plane_size(const orc_grid *g, int axis) {
+    uint32_t d[3] = {g->nx, g->ny, g->nz};
+    return plane_size(d, axis);
+}
+
+/* solver.wgsl:28-29 */
+static inline int fluid(const ctx_t *c, const int32_t pos[3]) {
+    int32_t q[3] = {pos[0], pos[1], pos[2]};
+    wrap_pos(c->periodic, c->d, q);
+    if (!valid_pos(c->d, q)) return 0;
+    return c->mask[idx3(c->d, q)] != 0u;
+}
+/* solver.wgsl:31-32 */
+static inline int valid_face(const ctx_t *c, const int32_t q[3], int axis) {
+    uint32_t P[3];
+    plane_dims(c->d, axis, P);
+    return valid_pos(P, q);
+}
+static inline int open_face(const ctx_t *c, const int32_t pos[3], int axis) {
+    int32_t q[3] = {pos[0], pos[1], pos[2]};
+    wrap_pos(c->periodic, c->d, q);
+    if (!valid_face(c, q, axis)) return 0;
+    int32_t left[3] = {q[0], q[1], q[2]};
+    left[axis] -= 1;
+    return fluid(c, left) && fluid(c, q);
+}
+/* solver.wgsl:33-36 face(): value at the canonical face of an open face, else 0 */
+static inline float face_value(const ctx_t *c, const float *planes, const int32_t pos[3], int axis) {
+    int32_t q[3] = {pos[0], pos[1], pos[2]};
+    wrap_pos(c->periodic, c->d, q);
+    if (!open_face(c, q, axis)) return 0.0f;
+    return planes[face_idx(c->d, axis, q)];
+}
+/* solver.wgsl:40-51 (mushy factor off: flags.w == 0) */
+static inline float occupancy(const ctx_t *c, const int32_t pos[3]) {
+    int32_t q[3] = {pos[0], pos[1], pos[2]};
+    wrap_pos(c->periodic, c->d, q);
+    if (!valid_pos(c->d, q)) return 0.0f;
+    return c->alpha ? c->alpha[idx3(c->d, q)] : 0.0f;
+}
+static inline float density(const ctx_t *c, const int32_t pos[3]) {
+    float a = occupancy(c, pos);
+    return c->rho_air * (1.0f - a) + c->rho_l * a; /* mix(rho_air, rho_l, a) */
+}
+static inline float face_density(const ctx_t *c, const int32_t pos[3], int axis) {
+    int32_t left[3] = {pos[0], pos[1], pos[2]};
+    left[axis] -= 1;
+    return 0.5f * (density(c, left) + density(c, pos));
+}
+static inline float mobility(const ctx_t *c, const int32_t pos[3], int axis) {
+    float beta = 1.0f;
+    if (c->two_phase) beta = c->rho_l / face_density(c, pos, axis);
+    return beta;
+}
+/* solver.wgsl:60-64 */
+static float diagonal(const ctx_t *c, const int32_t pos[3]) {
+    float d = 0.0f;
+    for (int axis = 0; axis < 3; axis++)
+        for (int sign = -1; sign <= 1; sign += 2) {
+            int32_t q[3] = {pos[0], pos[1], pos[2]};
+            q[axis] += sign;
+            if (fluid(c, q)) {
+                int32_t fp[3] = {pos[0], pos[1], pos[2]};
+                if (sign > 0) fp[axis] += 1;
+                d += mobility(c, fp, axis);
+            }
+        }
+    if (!c->meanfree && (c->mask[idx3(c->d, pos)] & 3u) == 2u) return 1.0f;
+    return fmaxf(d, 1e-30f);
+}
+
+/* src/fluid/mod.rs:1447-1493 */
+uint32_t orc_anchors(uint32_t *mask, const uint32_t dims[3], uint32_t periodic) {
+    uint64_t n = (uint64_t)dims[0] * dims[1] * dims[2];
+    uint32_t components = 0;
+    uint32_t *roots = (uint32_t *)calloc(n ? n : 1, sizeof(uint32_t));
+    uint64_t *queue = (uint64_t *)malloc((n ? n : 1) * sizeof(uint64_t));
+    for (uint64_t root = 0; root < n; root++) {
+        if (mask[root] != 1u) continue;
+        components++;
+        mask[root] = 4u;
+        roots[root] = (uint32_t)root;
+        uint64_t head = 0, tail = 0;
+        queue[tail++] = root;
+        while (head < tail) {
+            uint64_t i = queue[head++];
+            int32_t p[3];
+            decode3(i, dims, p);
+            for (int axis = 0; axis < 3; axis++)
+                for (int dir = -1; dir <= 1; dir += 2) {
+                    int32_t q[3] = {p[0], p[1], p[2]};
+                    int32_t v = p[axis] + dir;
+                    if (periodic & (1u << axis)) {
+                        int32_t m = (int32_t)dims[axis];
+                        v = ((v % m) + m) % m; /* rem_euclid */
+                    } else if (v < 0 || v >= (int32_t)dims[axis]) {
+                        continue;
+                    }
+                    q[axis] = v;
+                    uint64_t j = idx3(dims, q);
+                    if (mask[j] == 1u) {
+                        mask[j] = 3u;
+                        roots[j] = (uint32_t)root;
+                        queue[tail++] = j;
+                    }
+                }
+        }
+    }
+    for (uint64_t i = 0; i < n; i++) {
+        if (mask[i] == 3u) mask[i] = 1u | (roots[i] << 2);
+        else if (mask[i] == 4u) mask[i] = 2u | (roots[i] << 2);
+    }
+    free(roots);
+    free(queue);
+    return components;
+}
+
+/* src/fluid/multigrid.rs:25-65 */
+int orc_hierarchy(const uint32_t dims[4], uint32_t *level_dims, int max_levels, uint64_t *coarse_cells) {
+    uint64_t count = 1;
+    for (int a = 0; a < 3; a++) {
+        if (dims[a] > 0x7fffffffu) return -1;
+        count *= dims[a];
+        if (count > 0xffffffffull && a < 2) { /* checked_mul on u64 never overflows here */ }
+    }
+    if (dims[0] == 0 || dims[1] == 0 || dims[2] == 0 || count != dims[3]) return -1;
+    int levels = 0;
+    uint32_t cur[4] = {dims[0], dims[1], dims[2], dims[3]};
+    if (levels < max_levels) memcpy(level_dims + 4 * levels, cur, sizeof cur);
+    levels++;
+    uint64_t offset = 0;
+    while (cur[0] > 4 || cur[1] > 4 || cur[2] > 4) {
+        uint32_t nd[4];
+        for (int a = 0; a < 3; a++) nd[a] = (cur[a] + 1) / 2;
+        nd[3] = nd[0] * nd[1] * nd[2];
+        if (offset > 0xffffffffull) return -1;
+        if (levels < max_levels) memcpy(level_dims + 4 * levels, nd, sizeof nd);
+        levels++;
+        offset += nd[3];
+        if (offset > 0xffffffffull) return -1;
+        memcpy(cur, nd, sizeof nd);
+    }
+    if (coarse_cells) *coarse_cells = offset;
+    return levels;
+}
+
+/* pressure_coefficients (solver.wgsl:93-103) over SoA face planes + per-cell diagonal */
+void orc_coefficients(const orc_grid *g, const uint32_t *mask, const float *alpha, int meanfree,
+                      float *beta, float *diag) {
+    ctx_t c;
+    ctx_init(&c, g, mask, alpha, meanfree);
+    for (int axis = 0; axis < 3; axis++) {
+        uint32_t P[3];
+        plane_dims(c.d, axis, P);
+        uint64_t np = (uint64_t)P[0] * P[1] * P[2];
+        float *out = beta + plane_offset(c.d, axis);
+#pragma omp parallel for schedule(static)
+        for (int64_t f = 0; f < (int64_t)np; f++) {
+            int32_t pos[3];
+            decode3((uint64_t)f, P, pos);
+            out[f] = open_face(&c, pos, axis) ? mobility(&c, pos, axis) : 0.0f;
+        }
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)c.n; i++) {
+        float d = 0.0f;
+        if (mask[i] != 0u) {
+            int32_t pos[3];
+            decode3((uint64_t)i, c.d, pos);
+            d = diagonal(&c, pos);
+        }
+        diag[i] = d;
+    }
+}
+
+void orc_diagonal_uncached(const orc_grid *g, const uint32_t *mask, const float *alpha, int meanfree,
+                           float *diag) {
+    ctx_t c;
+    ctx_init(&c, g, mask, alpha, meanfree);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)c.n; i++) {
+        int32_t pos[3];
+        decode3((uint64_t)i, c.d, pos);
+        diag[i] = mask[i] ? diagonal(&c, pos) : 0.0f;
+    }
+}
+
+static double now_seconds(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+#define R float
+#define F(name) name##_f32
+#define RC(x) x##f
+#include "core.inc"
+#undef R
+#undef F
+#undef RC
+
+#define R double
+#define F(name) name##_f64
+#define RC(x) x
+#include "core.inc"
+#undef R
+#undef F
+#undef RC
+
+double orc_true_residual(const orc_grid *g, const uint32_t *mask, const float *alpha,
+                         const float *ustar, const double *phi) {
+    ctx_t c;
+    ctx_init(&c, g, mask, alpha, 0);
+    double num = 0.0, den = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : num, den)
+    for (int64_t i = 0; i < (int64_t)c.n; i++) {
+        if (!mask[i]) continue;
+        int32_t pos[3];
+        decode3((uint64_t)i, c.d, pos);
+        double b = rhs_cell_f64(&c, ustar, pos, 0);
+        double r = b - laplacian_cell_f64(&c, phi, pos);
+        num += r * r;
+        den += b * b;
+    }
+    return den > 0.0 ? sqrt(num / den) : 0.0;
+}
+
+void orc_correct_f64(const orc_grid *g, const uint32_t *mask, const float *alpha, const float *ustar,
+                     const double *phi, float *u_out) {
+    ctx_t c;
+    ctx_init(&c, g, mask, alpha, 0);
+    correct_f64(&c, ustar, phi, u_out);
+}
+
+int orc_project(const orc_grid *g, const uint32_t *mask, const float *alpha, const float *ustar,
+                const orc_opts *o, double *phi_out, float *u_out, orc_stats *st) {
+#ifdef _OPENMP
+    int saved = omp_get_max_threads();
+    if (o->threads > 0) omp_set_num_threads(o->threads);
+#endif
+    int rc = o->precision == 64 ? project_f64(g, mask, alpha, ustar, o, phi_out, u_out, st)
+                                : project_f32(g, mask, alpha, ustar, o, phi_out, u_out, st);
+#ifdef _OPENMP
+    if (o->threads > 0) omp_set_num_threads(saved);
+#endif
+    return rc;
+}
+
+struct orc_mg {
+    int precision;
+    void *impl;
+};
+
+orc_mg *orc_mg_new(const orc_grid *g, const uint32_t *mask, const float *alpha, int meanfree, int precision) {
+    orc_mg *m = (orc_mg *)calloc(1, sizeof(orc_mg));
+    m->precision = precision;
+    m->impl = precision == 64 ? (void *)probe_new_f64(g, mask, alpha, meanfree)
+                              : (void *)probe_new_f32(g, mask, alpha, meanfree);
+    return m;
+}
+void orc_mg_free(orc_mg *m) {
+    if (!m) return;
+    if (m->precision == 64) probe_free_f64(m->impl);
+    else probe_free_f32(m->impl);
+    free(m);
+}
+int orc_mg_levels(const orc_mg *m, uint32_t *dims4) {
+    return m->precision == 64 ? probe_levels_f64(m->impl, dims4) : probe_levels_f32(m->impl, dims4);
+}
+void orc_mg_action(orc_mg *m, int level, const double *v, double *out) {
+    if (m->precision == 64) probe_action_f64(m->impl, level, v, out);
+    else probe_action_f32(m->impl, level, v, out);
+}
+void orc_mg_vcycle(orc_mg *m, const double *r, double *z) {
+    if (m->precision == 64) probe_vcycle_f64(m->impl, r, z);
+    else probe_vcycle_f32(m->impl, r, z);
+}
+void orc_mg_coarse_solve(orc_mg *m, const double *rhs, double *z) {
+    if (m->precision == 64) probe_coarse_f64(m->impl, rhs, z, -1);
+    else probe_coarse_f32(m->impl, rhs, z, -1);
+}
+void orc_mg_coarse_sweeps(orc_mg *m, const double *rhs, double *z, int sweeps) {
+    if (m->precision == 64) probe_coarse_f64(m->impl, rhs, z, sweeps);
+    else probe_coarse_f32(m->impl, rhs, z, sweeps);
+}
+void orc_mg_graph(const orc_mg *m, int level, double *coeff4, double *diag) {
+    if (m->precision == 64) probe_graph_f64(m->impl, level, coeff4, diag);
+    else probe_graph_f32(m->impl, level, coeff4, diag);
+}

@@ -1,0 +1,131 @@
+/*
+ * CPU oracle for the MAC pressure projection — TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of the reference's pressure projection
+ * (sdfgeoff/mhd-simulator), used as the checker for the HIP library and as the
+ * timed CPU baseline.  It is never linked into the product library.
+ *
+ * It follows, function by function:
+ *   src/fluid/solver.wgsl:17-107     (MAC helpers, mobility, cached/uncached stencil, rhs)
+ *   src/fluid/solver.wgsl:164-250    (guess, refresh, reductions, apply, update, search, correct)
+ *   src/fluid/solver.wgsl:317-328    (independent physical residual check)
+ *   src/fluid/pressure_subspace.wgsl (mean-free residual projection)
+ *   src/fluid/multigrid.wgsl:1-144   (aggregation build, V(1,1) Jacobi cycle, coarse solve)
+ *   src/periodic.wgsl:5-10           (unsigned Euclidean wrap)
+ *   src/fluid/mod.rs:969-1115        (project(): staging, batches, stopping rule, checks)
+ *   src/fluid/mod.rs:1447-1493       (anchors(): BFS components and gauge encoding)
+ *   src/fluid/multigrid.rs:25-82     (hierarchy(), cycle_stages())
+ *
+ * Two arithmetic modes share one code body (core.inc):
+ *   32: FP32 with the reference's operation order, including its two-level
+ *       256-wide tree reductions, so per-step rounding follows the WGSL.
+ *   64: FP64 on the same discrete system (FP32 face mobilities promoted, RHS in
+ *       FP64 from the FP32 face velocities) — the ground truth for 1e-8 parity.
+ *
+ * Pinning (what each part is checked against) is listed in tests/test_oracle.py;
+ * exact iteration counts and residual histories of the reference run are not
+ * pinned by anything available offline ("parity unpinned" for those).
+ *
+ * Layout conventions (shared with include/mhdp.h):
+ *   cells  x + nx*(y + ny*z)
+ *   faces  three SoA planes; plane a has dims (n + e_a), index
+ *          x + Px*(y + Py*z) with P = n + e_a.  Periodic duplicates at
+ *          index n_a are ignored on input and written on output.
+ */
+#ifndef MHDP_ORACLE_H
+#define MHDP_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    uint32_t nx, ny, nz;
+    uint32_t periodic; /* bit a set: axis a periodic */
+    float h;           /* cell size (Params.physical.x) */
+    int two_phase;     /* Params.surface.x > 0.5 */
+    float rho_liquid;  /* Params.physical.w */
+    float rho_air;     /* Params.surface.y */
+} orc_grid;
+
+typedef struct {
+    float rel_tol;       /* FluidConfig.relative_tolerance */
+    float abs_div_tol;   /* FluidConfig.absolute_divergence_tolerance */
+    uint32_t max_iter;   /* FluidConfig.max_iterations */
+    int precond;         /* 0 auto (MG iff cells >= 96^3), 1 Jacobi, 2 MG */
+    int gauge;           /* 0 auto (mean-free iff one component), 2 pinned */
+    int precision;       /* 32 or 64 */
+    double tol64;        /* precision 64: relative tolerance (no 1e-7 floor); <=0 uses rel_tol */
+    int threads;         /* OpenMP threads, <=0 = default */
+} orc_opts;
+
+typedef struct {
+    uint32_t iterations;
+    uint32_t levels;        /* MG levels incl. fine; 0 for Jacobi */
+    uint32_t components;
+    uint32_t fluid_cells;
+    int meanfree;
+    int multigrid;
+    double rel_residual;    /* sqrt(rr/bb) as reported by convergence() */
+    double seconds;
+} orc_stats;
+
+/* Status codes (mirroring the reference's error strings, src/fluid/mod.rs:974,1079,1097) */
+enum { ORC_OK = 0, ORC_NOT_CONVERGED = 1, ORC_NO_PROGRESS = 2, ORC_NONFINITE = 3, ORC_INVALID = 4 };
+
+uint64_t orc_face_plane_size(const orc_grid *g, int axis);
+
+/* anchors(): mask in = 0/1 per cell, out = 0 | (1|root<<2) | (2|root<<2); returns components. */
+uint32_t orc_anchors(uint32_t *mask, const uint32_t dims[3], uint32_t periodic);
+
+/* hierarchy(): fills level dims (4 u32 each: nx,ny,nz,count) and returns the level count
+ * (fine included), or -1 on the reference's error cases.  *coarse_cells = sum of coarse counts. */
+int orc_hierarchy(const uint32_t dims[4], uint32_t *level_dims, int max_levels, uint64_t *coarse_cells);
+
+/* pressure_coefficients: beta planes (3, packed x|y|z) and the cached diagonal per cell. */
+void orc_coefficients(const orc_grid *g, const uint32_t *mask, const float *alpha, int meanfree,
+                      float *beta, float *diag);
+/* the uncached diagonal() for every fluid cell (0 elsewhere) */
+void orc_diagonal_uncached(const orc_grid *g, const uint32_t *mask, const float *alpha, int meanfree,
+                           float *diag);
+/* rhs(): b = -h^2 div(u*) over open faces, 0 at pinned anchors (zero_anchor=1) */
+void orc_rhs_f32(const orc_grid *g, const uint32_t *mask, int zero_anchor, const float *ustar, float *b);
+void orc_rhs_f64(const orc_grid *g, const uint32_t *mask, int zero_anchor, const float *ustar, double *b);
+/* apply(): cached 7-point operator (identity at pinned anchors) */
+void orc_apply_f32(const orc_grid *g, const uint32_t *mask, int meanfree, const float *beta,
+                   const float *v, float *out);
+void orc_apply_f64(const orc_grid *g, const uint32_t *mask, int meanfree, const float *beta,
+                   const double *v, double *out);
+/* laplacian(): uncached physical operator on every fluid row */
+void orc_laplacian_f32(const orc_grid *g, const uint32_t *mask, const float *alpha, const float *v, float *out);
+void orc_laplacian_f64(const orc_grid *g, const uint32_t *mask, const float *alpha, const double *v, double *out);
+/* FP64 ||b - L phi|| / ||b|| over every fluid row (physical operator, anchors included). */
+double orc_true_residual(const orc_grid *g, const uint32_t *mask, const float *alpha,
+                         const float *ustar, const double *phi);
+/* correct(): u = u* - beta (phi_R - phi_L)/h on open faces, 0 elsewhere (all plane entries). */
+void orc_correct_f64(const orc_grid *g, const uint32_t *mask, const float *alpha, const float *ustar,
+                     const double *phi, float *u_out);
+
+/* FluidState::project(false): encoded mask in, phi (potential) and projected faces out. */
+int orc_project(const orc_grid *g, const uint32_t *mask, const float *alpha, const float *ustar,
+                const orc_opts *o, double *phi_out, float *u_out, orc_stats *st);
+
+/* ---- multigrid probes (mirror the reference's test-only entry points) ---- */
+typedef struct orc_mg orc_mg;
+orc_mg *orc_mg_new(const orc_grid *g, const uint32_t *mask, const float *alpha, int meanfree, int precision);
+void orc_mg_free(orc_mg *m);
+int orc_mg_levels(const orc_mg *m, uint32_t *dims4 /* [levels*4] */);
+/* action of level l (l=0 fine, uses cached beta; l>=1 Galerkin coarse graph) */
+void orc_mg_action(orc_mg *m, int level, const double *v, double *out);
+/* one V-cycle z = M r on the fine level */
+void orc_mg_vcycle(orc_mg *m, const double *r, double *z);
+/* coarsest level: fused solve vs n separate weighted-Jacobi sweeps from zero */
+void orc_mg_coarse_solve(orc_mg *m, const double *rhs, double *z);
+void orc_mg_coarse_sweeps(orc_mg *m, const double *rhs, double *z, int sweeps);
+/* coarse graph lanes of level l>=1: c (4 per cell: lower x,y,z edge, reaction) and diag */
+void orc_mg_graph(const orc_mg *m, int level, double *coeff4, double *diag);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,226 @@
+"""ctypes wrapper of the C oracle (liboracle.so) — TEST INFRASTRUCTURE ONLY.
+
+Imported only by tests/, ``__graft_entry__.smoke()`` and bench.py's
+cpu_baseline leg, always as the checker / CPU baseline, never as the product
+path.  See mhdp_oracle.h for the reference file:line each function restates.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+class _Grid(C.Structure):
+    _fields_ = [("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32), ("periodic", C.c_uint32),
+                ("h", C.c_float), ("two_phase", C.c_int), ("rho_liquid", C.c_float), ("rho_air", C.c_float)]
+
+
+class _Opts(C.Structure):
+    _fields_ = [("rel_tol", C.c_float), ("abs_div_tol", C.c_float), ("max_iter", C.c_uint32),
+                ("precond", C.c_int), ("gauge", C.c_int), ("precision", C.c_int), ("tol64", C.c_double),
+                ("threads", C.c_int)]
+
+
+class _Stats(C.Structure):
+    _fields_ = [("iterations", C.c_uint32), ("levels", C.c_uint32), ("components", C.c_uint32),
+                ("fluid_cells", C.c_uint32), ("meanfree", C.c_int), ("multigrid", C.c_int),
+                ("rel_residual", C.c_double), ("seconds", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.orc_anchors.restype = C.c_uint32
+        L.orc_anchors.argtypes = [P, P, C.c_uint32]
+        L.orc_hierarchy.restype = C.c_int
+        L.orc_hierarchy.argtypes = [P, P, C.c_int, P]
+        L.orc_coefficients.argtypes = [P, P, P, C.c_int, P, P]
+        L.orc_diagonal_uncached.argtypes = [P, P, P, C.c_int, P]
+        for nm in ("orc_rhs_f32", "orc_rhs_f64"):
+            getattr(L, nm).argtypes = [P, P, C.c_int, P, P]
+        for nm in ("orc_apply_f32", "orc_apply_f64"):
+            getattr(L, nm).argtypes = [P, P, C.c_int, P, P, P]
+        for nm in ("orc_laplacian_f32", "orc_laplacian_f64"):
+            getattr(L, nm).argtypes = [P, P, P, P, P]
+        L.orc_true_residual.restype = C.c_double
+        L.orc_true_residual.argtypes = [P, P, P, P, P]
+        L.orc_correct_f64.argtypes = [P, P, P, P, P, P]
+        L.orc_project.restype = C.c_int
+        L.orc_project.argtypes = [P, P, P, P, P, P, P, P]
+        L.orc_mg_new.restype = P
+        L.orc_mg_new.argtypes = [P, P, P, C.c_int, C.c_int]
+        L.orc_mg_free.argtypes = [P]
+        L.orc_mg_levels.restype = C.c_int
+        L.orc_mg_levels.argtypes = [P, P]
+        L.orc_mg_action.argtypes = [P, C.c_int, P, P]
+        L.orc_mg_vcycle.argtypes = [P, P, P]
+        L.orc_mg_coarse_solve.argtypes = [P, P, P]
+        L.orc_mg_coarse_sweeps.argtypes = [P, P, P, C.c_int]
+        L.orc_mg_graph.argtypes = [P, C.c_int, P, P]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _grid(g) -> _Grid:
+    return _Grid(g.nx, g.ny, g.nz, g.periodic_bits, g.h, int(g.two_phase), g.rho_liquid, g.rho_air)
+
+
+def anchors(fluid: np.ndarray, dims, periodic=(False, False, False)):
+    """src/fluid/mod.rs:1447-1493 -> (encoded uint32 mask, components)."""
+    mask = np.ascontiguousarray((np.asarray(fluid) != 0).astype(np.uint32))
+    d = np.array(dims, dtype=np.uint32)
+    bits = sum(1 << a for a in range(3) if periodic[a])
+    comps = lib().orc_anchors(_ptr(mask), _ptr(d), bits)
+    return mask, int(comps)
+
+
+def hierarchy(dims4):
+    """src/fluid/multigrid.rs:25-65 -> (list of level dims (nx,ny,nz,count), coarse cells) or None."""
+    d = np.array(dims4, dtype=np.uint32)
+    out = np.zeros(64 * 4, dtype=np.uint32)
+    coarse = np.zeros(1, dtype=np.uint64)
+    n = lib().orc_hierarchy(_ptr(d), _ptr(out), 64, _ptr(coarse))
+    if n < 0:
+        return None
+    return [tuple(int(v) for v in out[4 * l:4 * l + 4]) for l in range(n)], int(coarse[0])
+
+
+def coefficients(g, mask, alpha, meanfree: bool):
+    beta = np.zeros(g.faces, dtype=np.float32)
+    diag = np.zeros(g.cells, dtype=np.float32)
+    gg = _grid(g)
+    lib().orc_coefficients(C.byref(gg), _ptr(mask), _ptr(alpha), int(meanfree), _ptr(beta), _ptr(diag))
+    return beta, diag
+
+
+def diagonal_uncached(g, mask, alpha, meanfree: bool):
+    diag = np.zeros(g.cells, dtype=np.float32)
+    gg = _grid(g)
+    lib().orc_diagonal_uncached(C.byref(gg), _ptr(mask), _ptr(alpha), int(meanfree), _ptr(diag))
+    return diag
+
+
+def rhs(g, mask, ustar, precision=64, zero_anchor=True):
+    dt = np.float64 if precision == 64 else np.float32
+    b = np.zeros(g.cells, dtype=dt)
+    gg = _grid(g)
+    fn = lib().orc_rhs_f64 if precision == 64 else lib().orc_rhs_f32
+    fn(C.byref(gg), _ptr(mask), int(zero_anchor), _ptr(np.ascontiguousarray(ustar, dtype=np.float32)), _ptr(b))
+    return b
+
+
+def apply(g, mask, meanfree, beta, v, precision=64):
+    dt = np.float64 if precision == 64 else np.float32
+    v = np.ascontiguousarray(v, dtype=dt)
+    out = np.zeros(g.cells, dtype=dt)
+    gg = _grid(g)
+    fn = lib().orc_apply_f64 if precision == 64 else lib().orc_apply_f32
+    fn(C.byref(gg), _ptr(mask), int(meanfree), _ptr(beta), _ptr(v), _ptr(out))
+    return out
+
+
+def laplacian(g, mask, alpha, v, precision=64):
+    dt = np.float64 if precision == 64 else np.float32
+    v = np.ascontiguousarray(v, dtype=dt)
+    out = np.zeros(g.cells, dtype=dt)
+    gg = _grid(g)
+    fn = lib().orc_laplacian_f64 if precision == 64 else lib().orc_laplacian_f32
+    fn(C.byref(gg), _ptr(mask), _ptr(alpha), _ptr(v), _ptr(out))
+    return out
+
+
+def true_residual(g, mask, alpha, ustar, phi) -> float:
+    gg = _grid(g)
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    return float(lib().orc_true_residual(C.byref(gg), _ptr(mask), _ptr(alpha),
+                                         _ptr(np.ascontiguousarray(ustar, dtype=np.float32)), _ptr(phi)))
+
+
+def correct(g, mask, alpha, ustar, phi):
+    gg = _grid(g)
+    out = np.zeros(g.faces, dtype=np.float32)
+    lib().orc_correct_f64(C.byref(gg), _ptr(mask), _ptr(alpha), _ptr(np.ascontiguousarray(ustar, dtype=np.float32)),
+                          _ptr(np.ascontiguousarray(phi, dtype=np.float64)), _ptr(out))
+    return out
+
+
+def project(g, mask, alpha, ustar, rel_tol=1e-5, abs_div_tol=0.0, max_iter=1000, precond=0, gauge=0,
+            precision=32, tol64=0.0, threads=0):
+    """FluidState::project(false) (src/fluid/mod.rs:1015-1115) -> (status, phi f64, u f32, stats dict)."""
+    gg = _grid(g)
+    o = _Opts(rel_tol, abs_div_tol, max_iter, precond, gauge, precision, tol64, threads)
+    st = _Stats()
+    phi = np.zeros(g.cells, dtype=np.float64)
+    u = np.zeros(g.faces, dtype=np.float32)
+    rc = lib().orc_project(C.byref(gg), _ptr(mask), _ptr(alpha), _ptr(np.ascontiguousarray(ustar, dtype=np.float32)),
+                           C.byref(o), _ptr(phi), _ptr(u), C.byref(st))
+    stats = {k: getattr(st, k) for k, _ in _Stats._fields_}
+    return rc, phi, u, stats
+
+
+class MG:
+    """Multigrid probes (src/fluid/multigrid_probes.wgsl and the test helpers around it)."""
+
+    def __init__(self, g, mask, alpha, meanfree: bool, precision=32):
+        self.g = g
+        gg = _grid(g)
+        self._keep = (mask, alpha)
+        self.h = lib().orc_mg_new(C.byref(gg), _ptr(mask), _ptr(alpha), int(meanfree), precision)
+        d = np.zeros(64 * 4, dtype=np.uint32)
+        n = lib().orc_mg_levels(self.h, _ptr(d))
+        self.levels = [tuple(int(v) for v in d[4 * l:4 * l + 4]) for l in range(n)]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_mg_free(self.h)
+            self.h = None
+
+    def action(self, level, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        out = np.zeros(self.levels[level][3], dtype=np.float64)
+        lib().orc_mg_action(self.h, level, _ptr(v), _ptr(out))
+        return out
+
+    def vcycle(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.zeros(self.g.cells, dtype=np.float64)
+        lib().orc_mg_vcycle(self.h, _ptr(r), _ptr(z))
+        return z
+
+    def coarse(self, rhs, sweeps=None):
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        z = np.zeros(self.levels[-1][3], dtype=np.float64)
+        if sweeps is None:
+            lib().orc_mg_coarse_solve(self.h, _ptr(rhs), _ptr(z))
+        else:
+            lib().orc_mg_coarse_sweeps(self.h, _ptr(rhs), _ptr(z), sweeps)
+        return z
+
+    def graph(self, level):
+        n = self.levels[level][3]
+        c4 = np.zeros(4 * n, dtype=np.float64)
+        diag = np.zeros(n, dtype=np.float64)
+        lib().orc_mg_graph(self.h, level, _ptr(c4), _ptr(diag))
+        return c4.reshape(n, 4), diag

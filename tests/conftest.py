@@ -1,0 +1,29 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library through its C ABI)")
+
+
+@pytest.fixture(scope="session")
+def orc():
+    from oracle import oracle
+    oracle.build()
+    return oracle
+
+
+@pytest.fixture(scope="session")
+def gpu_lib():
+    """The HIP library on a GPU box; fails loudly (no fallback) when it is not built."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("GPU test collected without a GPU")
+    from mhd_simulator_amd import _lib
+    return _lib.load()
