@@ -123,6 +123,15 @@ uint32_t mhdp_anchors(uint32_t *mask, const uint32_t dims[3], uint32_t periodic)
  * reference's error cases (zero or inconsistent dims, coordinates beyond i32, u32 offsets). */
 int mhdp_hierarchy(const uint32_t dims4[4], uint32_t *level_dims, int max_levels, uint64_t *coarse_cells);
 
+/* Live kernel timing: while enabled, HIP events bracket every launch of `kernel` on the
+ * context's stream during mhdp_project*; mhdp_kernel_time() returns the summed duration
+ * (ms) and launch count of the last projection.  Kernels: MHDP_K_APPLY (operator apply +
+ * p.q), MHDP_K_UPDATE (x/r update + norms), MHDP_K_SMOOTH (fine-level smoother),
+ * MHDP_K_RESTRICT (fine residual restriction). */
+enum { MHDP_K_NONE = -1, MHDP_K_APPLY = 0, MHDP_K_UPDATE = 1, MHDP_K_SMOOTH = 2, MHDP_K_RESTRICT = 3 };
+int mhdp_set_kernel_timing(mhdp_ctx *ctx, int kernel);
+int mhdp_kernel_time(const mhdp_ctx *ctx, double *total_ms, uint32_t *launches);
+
 /* ---- test hooks (the reference's cfg(test) probes: multigrid_probes.wgsl, tests.rs) ---- */
 /* pressure_coefficients: lower-face mobilities (3*cells, x|y|z), cached diagonal, 1/diag. */
 int mhdp_probe_coefficients(mhdp_ctx *ctx, int meanfree, float *beta3, float *diag, float *dinv);

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libmhdp.so")
 SYMBOLS = (
     "mhdp_create", "mhdp_set_stream", "mhdp_set_cells", "mhdp_get_mask", "mhdp_project", "mhdp_project_dev",
     "mhdp_project_batch", "mhdp_last_error", "mhdp_destroy", "mhdp_device_bytes",
-    "mhdp_anchors", "mhdp_hierarchy",
+    "mhdp_anchors", "mhdp_hierarchy", "mhdp_set_kernel_timing", "mhdp_kernel_time",
     "mhdp_probe_coefficients", "mhdp_probe_apply", "mhdp_probe_vcycle", "mhdp_probe_mg_graph", "mhdp_probe_rhs",
 )
 
@@ -23,6 +23,7 @@ OK, NOT_CONVERGED, NO_PROGRESS, NONFINITE, INVALID, NO_MEMORY, DEVICE = range(7)
 PRECOND = {"auto": 0, "jacobi": 1, "mg": 2}
 PRECISION = {"fp32": 0, "mixed": 1}
 GAUGE = {"auto": 0, "pinned": 2}
+KERNELS = {"apply": 0, "update": 1, "smooth": 2, "restrict": 3}
 
 
 class Grid(C.Structure):
@@ -60,6 +61,13 @@ def load(path: str = LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise RuntimeError(f"{path} is missing: build the HIP library with `python -m mhd_simulator_amd.build`")
+    # torch (when installed) bundles its own libamdhip64.so.7; the dynamic linker resolves
+    # libmhdp's NEEDED entry to whichever copy with that soname is loaded first.  Load torch's
+    # first so one HIP runtime serves both and torch device tensors can be passed in.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(path)
     P, I, U32 = C.c_void_p, C.c_int, C.c_uint32
     sig = {
@@ -74,6 +82,8 @@ def load(path: str = LIB_PATH):
         "mhdp_destroy": (None, [P]),
         "mhdp_device_bytes": (C.c_uint64, [P]),
         "mhdp_anchors": (U32, [P, P, U32]),
+        "mhdp_set_kernel_timing": (I, [P, I]),
+        "mhdp_kernel_time": (I, [P, P, P]),
         "mhdp_hierarchy": (I, [P, P, I, P]),
         "mhdp_probe_coefficients": (I, [P, I, P, P, P]),
         "mhdp_probe_apply": (I, [P, I, P, P]),

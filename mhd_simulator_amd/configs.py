@@ -196,12 +196,14 @@ def config2(n: int = 128, seed: int | None = None) -> Problem:
 def nozzle_geometry(n: int):
     """Revolved hotend profile (scripts/make_fixtures.py:38-39) about the z axis at
     (0.5, 0.5)L: chamber OD 0.6L / ID 0.3L, tapering to a 0.1L outlet; liquid fills
-    the chamber to 0.875L.  Returns (solid, liquid) boolean [z, y, x]."""
+    the chamber to 0.7875L.  The outlet (bottom, 0.1L) and the chamber top (0.85L) open into
+    the surrounding air, so liquid and air form one connected region.
+    Returns (solid, liquid) boolean [z, y, x]."""
     L = float(n)
     c = (np.arange(n) + 0.5)
     rad = np.sqrt((c[None, :] - 0.5 * L) ** 2 + (c[:, None] - 0.5 * L) ** 2)  # [y, x]
-    # profile units: radius 1 unit = 0.1 L; height z_unit in [-2, 10] -> [0.05L, 0.95L]
-    zu = (c - 0.05 * L) / (0.075 * L) - 2.0  # [z]
+    # profile units: radius 1 unit = 0.1 L; height z_unit in [-2, 10] -> [0.1L, 0.85L]
+    zu = (c - 0.1 * L) / (0.0625 * L) - 2.0  # [z]
     r_u = rad / (0.1 * L)
     zz = zu[:, None, None]
     rr = r_u[None, :, :]

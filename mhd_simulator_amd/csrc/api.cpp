@@ -60,6 +60,9 @@ struct mhdp_ctx {
     double* phi_stage = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint64_t bytes = 0;
+    KernelTimer timer;
+    double timer_ms = 0.0;
+    uint32_t timer_launches = 0;
 };
 
 namespace {
@@ -298,13 +301,17 @@ void mg_initialize(mhdp_ctx* c, Solve& S) {  // multigrid.rs:287-291
 
 void iterate_once(mhdp_ctx* c, Solve& S) {  // mod.rs:1057-1075, multigrid.rs:302-326
     hipStream_t s = c->stream;
+    c->timer.before(MHDP_K_APPLY, s);
     launch_apply(s, c->g, c->p, c->bx, c->by, c->bz, c->q, c->part, c->sc);
+    c->timer.after(MHDP_K_APPLY, s);
     const int upd = S.meanfree ? FIN_MEAN : (S.mg ? FIN_STASH_RR : FIN_BETA);
+    c->timer.before(MHDP_K_UPDATE, s);
     launch_update(s, c->g, c->x, c->r, c->p, c->q, c->dinv, c->part, c->sc, S.meanfree, upd);
+    c->timer.after(MHDP_K_UPDATE, s);
     if (S.meanfree) launch_subspace(s, c->g, c->r, c->p, c->dinv, c->part, c->sc, 0, S.mg ? FIN_STASH_RR : FIN_BETA);
     if (S.mg) {
         void* zf = nullptr;
-        launch_vcycle(s, S.store, (int)S.lv.size(), S.lv.data(), c->r, c->mask, c->sc, S.coarse64, &zf);
+        launch_vcycle(s, S.store, (int)S.lv.size(), S.lv.data(), c->r, c->mask, c->sc, S.coarse64, &zf, &c->timer);
         launch_pdot(s, c->g, S.store, c->r, zf, c->dinv, c->p, c->part, c->sc, 0, FIN_BETA_MG);
         launch_psearch(s, c->g, S.store, zf, c->dinv, c->p, c->sc);
     } else {
@@ -423,6 +430,7 @@ int project_impl(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, float* uou
     S.coarse64 = mixed;
     const float nf = (float)std::max<uint32_t>(c->fluid_cells, 1);
 
+    c->timer.used = 0;
     HIP_OK(hipEventRecord(c->ev0, c->stream));
     if ((rc = prepare(c, meanfree))) return rc;
     uint32_t iterations = 0, outer = 0;
@@ -501,6 +509,14 @@ int project_impl(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, float* uou
     HIP_OK(hipGetLastError());
     float ms = 0.0f;
     HIP_OK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->timer_ms = 0.0;
+    c->timer_launches = 0;
+    for (size_t k = 0; k + 1 < c->timer.used; k += 2) {
+        float t = 0.0f;
+        HIP_OK(hipEventElapsedTime(&t, c->timer.ev[k], c->timer.ev[k + 1]));
+        c->timer_ms += t;
+        c->timer_launches++;
+    }
     if (st) {
         st->iterations = iterations;
         st->outer_steps = outer;
@@ -683,6 +699,19 @@ int mhdp_hierarchy(const uint32_t dims4[4], uint32_t* level_dims, int max_levels
     return levels;
 }
 
+int mhdp_set_kernel_timing(mhdp_ctx* c, int kernel) {
+    if (!c || kernel < MHDP_K_NONE || kernel > MHDP_K_RESTRICT) return MHDP_INVALID;
+    c->timer.kernel = kernel;
+    return MHDP_OK;
+}
+
+int mhdp_kernel_time(const mhdp_ctx* c, double* total_ms, uint32_t* launches) {
+    if (!c) return MHDP_INVALID;
+    if (total_ms) *total_ms = c->timer_ms;
+    if (launches) *launches = c->timer_launches;
+    return MHDP_OK;
+}
+
 const char* mhdp_last_error(const mhdp_ctx* c) { return c ? c->err.c_str() : "invalid context"; }
 
 uint64_t mhdp_device_bytes(const mhdp_ctx* c) { return c ? c->bytes : 0; }
@@ -699,6 +728,7 @@ void mhdp_destroy(mhdp_ctx* c) {
                      a.r, a.z, a.t};
         for (void* p : q) dfree(c, p);
     }
+    for (hipEvent_t e : c->timer.ev) (void)hipEventDestroy(e);
     if (c->sc_h) (void)hipHostFree(c->sc_h);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);

@@ -218,7 +218,10 @@ __global__ void __launch_bounds__(MHDP_BLOCK) k_coarse(Grid3 g, const RT* __rest
 }
 
 template <class CT, class VT, class AT>
-static void vcycle_t(hipStream_t s, int L, LevelDev* lv, const float* r0, Scal* sc, void** zfinal, bool f16) {
+static void vcycle_t(hipStream_t s, int L, LevelDev* lv, const float* r0, Scal* sc, void** zfinal, bool f16,
+                     KernelTimer* timer) {
+    KernelTimer dummy;
+    KernelTimer& T = timer ? *timer : dummy;
     auto C = [&](LevelDev& l, int which) -> const CT* {
         if (f16) {
             switch (which) {
@@ -252,8 +255,10 @@ static void vcycle_t(hipStream_t s, int L, LevelDev* lv, const float* r0, Scal* 
         if (l == 0) {
             hipLaunchKernelGGL((k_dstart<CT, float, VT>), dim3(grid_blocks(a.g.n)), B, 0, s, a.g.n, r0, C(a, 4),
                                (VT*)a.z, sc);
+            T.before(3, s);
             hipLaunchKernelGGL((k_restrict_fine<CT, VT>), dim3(grid_blocks(c.g.n)), B, 0, s, c.g, a.g, r0, C(a, 4),
                                C(a, 0), C(a, 1), C(a, 2), (const VT*)a.z, (VT*)c.r, sc);
+            T.after(3, s);
         } else {
             hipLaunchKernelGGL((k_dstart<CT, VT, VT>), dim3(grid_blocks(a.g.n)), B, 0, s, a.g.n, (const VT*)a.r,
                                C(a, 4), (VT*)a.z, sc);
@@ -273,10 +278,12 @@ static void vcycle_t(hipStream_t s, int L, LevelDev* lv, const float* r0, Scal* 
         const VT* zc = (const VT*)(l + 1 == L - 1 ? c.z : c.t);
         hipLaunchKernelGGL((k_prolong<CT, VT>), dim3(grid_blocks(a.g.n)), B, 0, s, a.g, c.g, l == 0 ? 1 : 0, zc,
                            C(a, 4), (VT*)a.z, sc);
-        if (l == 0)
+        if (l == 0) {
+            T.before(2, s);
             hipLaunchKernelGGL((k_smooth<CT, float, VT, true>), dim3(grid_blocks(a.g.n)), B, 0, s, a.g, r0, C(a, 4),
                                C(a, 0), C(a, 1), C(a, 2), (const CT*)nullptr, (const VT*)a.z, (VT*)a.t, sc);
-        else
+            T.after(2, s);
+        } else
             hipLaunchKernelGGL((k_smooth<CT, VT, VT, false>), dim3(grid_blocks(a.g.n)), B, 0, s, a.g, (const VT*)a.r,
                                C(a, 4), C(a, 0), C(a, 1), C(a, 2), C(a, 3), (const VT*)a.z, (VT*)a.t, sc);
     }
@@ -284,12 +291,12 @@ static void vcycle_t(hipStream_t s, int L, LevelDev* lv, const float* r0, Scal* 
 }
 
 void launch_vcycle(hipStream_t s, int store, int L, LevelDev* lv, const float* r0, const uint32_t* /*mask*/,
-                   Scal* sc, int coarse_f64, void** zfinal) {
+                   Scal* sc, int coarse_f64, void** zfinal, KernelTimer* timer) {
     if (store == STORE_BF16) {
-        if (coarse_f64) vcycle_t<__hip_bfloat16, __hip_bfloat16, double>(s, L, lv, r0, sc, zfinal, true);
-        else vcycle_t<__hip_bfloat16, __hip_bfloat16, float>(s, L, lv, r0, sc, zfinal, true);
+        if (coarse_f64) vcycle_t<__hip_bfloat16, __hip_bfloat16, double>(s, L, lv, r0, sc, zfinal, true, timer);
+        else vcycle_t<__hip_bfloat16, __hip_bfloat16, float>(s, L, lv, r0, sc, zfinal, true, timer);
     } else {
-        if (coarse_f64) vcycle_t<float, float, double>(s, L, lv, r0, sc, zfinal, false);
-        else vcycle_t<float, float, float>(s, L, lv, r0, sc, zfinal, false);
+        if (coarse_f64) vcycle_t<float, float, double>(s, L, lv, r0, sc, zfinal, false, timer);
+        else vcycle_t<float, float, float>(s, L, lv, r0, sc, zfinal, false, timer);
     }
 }

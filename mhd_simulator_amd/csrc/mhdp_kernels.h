@@ -1,5 +1,7 @@
 // Host-callable launchers of the projection kernels (one HIP stream per context).
 #pragma once
+#include <vector>
+
 #include "mhdp_device.h"
 
 struct PhaseP {
@@ -58,6 +60,29 @@ void launch_apply_plain(hipStream_t s, const Grid3& g, const float* p, const flo
                         const float* bz, const float* dinv, int identity_inactive, float* q);
 void launch_cast_f32_f64(hipStream_t s, uint32_t n, const float* src, double* dst);
 
+// Optional event pairs around one kernel family (live per-kernel timing).
+struct KernelTimer {
+    int kernel = -1;                         // MHDP_K_* being timed, -1 off
+    std::vector<hipEvent_t> ev;              // pool, two per launch
+    size_t used = 0;
+    void before(int k, hipStream_t s) {
+        if (k != kernel) return;
+        if (used + 2 > ev.size()) {
+            for (int i = 0; i < 64; i++) {
+                hipEvent_t e;
+                if (hipEventCreate(&e) != hipSuccess) return;
+                ev.push_back(e);
+            }
+        }
+        (void)hipEventRecord(ev[used], s);
+    }
+    void after(int k, hipStream_t s) {
+        if (k != kernel || used + 2 > ev.size()) return;
+        (void)hipEventRecord(ev[used + 1], s);
+        used += 2;
+    }
+};
+
 // mg.hip — one V-cycle z = M r on the fine level (cycle_stages, multigrid.rs:67-82)
 void launch_vcycle(hipStream_t s, int store, int L, LevelDev* lv, const float* r0, const uint32_t* mask, Scal* sc,
-                   int coarse_f64, void** zfinal);
+                   int coarse_f64, void** zfinal, KernelTimer* timer = nullptr);
