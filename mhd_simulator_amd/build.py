@@ -26,6 +26,7 @@ SOURCES = {
     "setup.hip": ["-ffp-contract=off"],
     "pcg.hip": [],
     "mg.hip": [],
+    "fused.hip": [],
     "api.cpp": ["-x", "hip"],
 }
 

@@ -6,6 +6,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -44,6 +45,7 @@ struct mhdp_ctx {
     float* alpha = nullptr;
     float *bx = nullptr, *by = nullptr, *bz = nullptr, *diag = nullptr, *dinv = nullptr;
     float *b = nullptr, *x = nullptr, *r = nullptr, *p = nullptr, *q = nullptr;
+    float* p2 = nullptr;  // second search-direction buffer of the fused path
     void *z = nullptr, *t = nullptr;
     double *x64 = nullptr, *b64 = nullptr;
     void *bx16 = nullptr, *by16 = nullptr, *bz16 = nullptr, *dinv16 = nullptr;
@@ -60,6 +62,7 @@ struct mhdp_ctx {
     double* phi_stage = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint64_t bytes = 0;
+    bool fused_enabled = true;  // MHDP_UNFUSED=1 selects the one-kernel-per-stage path
     KernelTimer timer;
     double timer_ms = 0.0;
     uint32_t timer_launches = 0;
@@ -282,6 +285,7 @@ int convergence(mhdp_ctx* c, float* residual, bool* conv, uint32_t* iters) {
 struct Solve {
     bool meanfree = false;
     bool mg = false;
+    bool fused = false;  // fused fine-grid passes (multigrid with at least one coarse level)
     int store = STORE_F32;
     bool coarse64 = false;
     std::vector<LevelDev> lv;
@@ -293,7 +297,38 @@ void refresh_stages(mhdp_ctx* c, const Solve& S) {  // mod.rs:1004-1014
     if (S.meanfree) launch_subspace(c->stream, c->g, c->r, c->p, c->dinv, c->part, c->sc, 1, FIN_INITIAL_SUB);
 }
 
+// V-cycle with the fine level fused into two passes; its last pass reduces r.z (`mode`).
+void vcycle_fused(mhdp_ctx* c, Solve& S, int mode) {
+    hipStream_t s = c->stream;
+    const bool f16 = S.store == STORE_BF16;
+    const void* dinv = f16 ? c->dinv16 : (const void*)c->dinv;
+    const void* bx = f16 ? c->bx16 : (const void*)c->bx;
+    const void* by = f16 ? c->by16 : (const void*)c->by;
+    const void* bz = f16 ? c->bz16 : (const void*)c->bz;
+    LevelDev* lv = S.lv.data();
+    const int L = (int)S.lv.size();
+    c->timer.before(MHDP_K_RESTRICT, s);
+    launch_fused_restrict(s, S.store, lv[1].g, lv[0].g, c->r, dinv, bx, by, bz, lv[1].r, c->sc);
+    c->timer.after(MHDP_K_RESTRICT, s);
+    void* z1 = nullptr;
+    launch_vcycle_coarse(s, S.store, L, lv, c->sc, S.coarse64, &z1);
+    c->timer.before(MHDP_K_SMOOTH, s);
+    launch_fused_smooth(s, S.store, lv[0].g, lv[1].g, c->r, dinv, bx, by, bz, z1, lv[0].t, c->part, c->sc, mode);
+    c->timer.after(MHDP_K_SMOOTH, s);
+}
+
+// The x update the fused path still owes (before any use of x).
+void flush_x(mhdp_ctx* c, const Solve& S) {
+    if (!S.fused) return;
+    launch_fused_xflush(c->stream, c->g, c->p, c->p2, c->x, c->dinv, c->sc, S.meanfree ? 1 : 0);
+    launch_clear_pending(c->stream, c->sc);
+}
+
 void mg_initialize(mhdp_ctx* c, Solve& S) {  // multigrid.rs:287-291
+    if (S.fused) {
+        vcycle_fused(c, S, FIN_INITIAL_MG);
+        return;
+    }
     void* zf = nullptr;
     launch_vcycle(c->stream, S.store, (int)S.lv.size(), S.lv.data(), c->r, c->mask, c->sc, S.coarse64, &zf);
     launch_pdot(c->stream, c->g, S.store, c->r, zf, c->dinv, c->p, c->part, c->sc, 1, FIN_INITIAL_MG);
@@ -301,6 +336,17 @@ void mg_initialize(mhdp_ctx* c, Solve& S) {  // multigrid.rs:287-291
 
 void iterate_once(mhdp_ctx* c, Solve& S) {  // mod.rs:1057-1075, multigrid.rs:302-326
     hipStream_t s = c->stream;
+    if (S.fused) {
+        c->timer.before(MHDP_K_APPLY, s);
+        launch_fused_apply(s, c->g, S.store, S.lv[0].t, c->p, c->p2, c->x, c->bx, c->by, c->bz, c->dinv, c->q,
+                           c->part, c->sc, S.meanfree ? 1 : 0);
+        c->timer.after(MHDP_K_APPLY, s);
+        c->timer.before(MHDP_K_UPDATE, s);
+        launch_fused_update(s, c->g, c->r, c->q, c->dinv, c->part, c->sc);
+        c->timer.after(MHDP_K_UPDATE, s);
+        vcycle_fused(c, S, FIN_BETA_MG);
+        return;
+    }
     c->timer.before(MHDP_K_APPLY, s);
     launch_apply(s, c->g, c->p, c->bx, c->by, c->bz, c->q, c->part, c->sc);
     c->timer.after(MHDP_K_APPLY, s);
@@ -347,6 +393,11 @@ int pcg(mhdp_ctx* c, Solve& S, bool want_mg, uint32_t max_iter, bool reference_c
         build_mg(c, S.meanfree ? 1 : 0, S.store == STORE_BF16);
         S.mg = true;
         S.lv = level_view(c);
+        S.fused = S.lv.size() >= 2 && c->fused_enabled;
+    }
+    if (!conv && S.fused) {  // finite search directions for the first p = z + 0 p
+        launch_fill_f32(c->stream, c->g.n, c->p, 0.0f);
+        launch_fill_f32(c->stream, c->g.n, c->p2, 0.0f);
     }
     if (!conv && S.mg) mg_initialize(c, S);
     while (!conv && it < max_iter) {
@@ -358,15 +409,18 @@ int pcg(mhdp_ctx* c, Solve& S, bool want_mg, uint32_t max_iter, bool reference_c
         if (seen <= it && !conv) return fail(c, MHDP_NO_PROGRESS, "Fluid pressure iteration made no progress");
         it = seen;
         if (conv && reference_checks) {
+            flush_x(c, S);
             refresh_stages(c, S);
             if ((rc = convergence(c, residual, &conv, &seen))) return rc;
             if (!conv && S.mg) mg_initialize(c, S);
         } else if (!conv && reference_checks && (it % 64u == 0u || it == max_iter)) {
+            flush_x(c, S);
             launch_refresh(c->stream, c->g, c->x, c->b, c->bx, c->by, c->bz, c->dinv, c->r, c->p, c->part, c->sc,
                            S.meanfree, 1, FIN_CHECK);
             if ((rc = convergence(c, residual, &conv, &seen))) return rc;
         }
     }
+    flush_x(c, S);
     *iterations = it;
     return conv ? MHDP_OK : MHDP_NOT_CONVERGED;
 }
@@ -387,6 +441,7 @@ int ensure_fine(mhdp_ctx* c) {
         (rc = dalloc(c, (void**)&c->dinv, fb)) || (rc = dalloc(c, (void**)&c->b, fb)) ||
         (rc = dalloc(c, (void**)&c->x, fb)) || (rc = dalloc(c, (void**)&c->r, fb)) ||
         (rc = dalloc(c, (void**)&c->p, fb)) || (rc = dalloc(c, (void**)&c->q, fb)) ||
+        (rc = dalloc(c, (void**)&c->p2, fb)) ||
         (rc = dalloc(c, &c->z, fb)) || (rc = dalloc(c, &c->t, fb)) ||
         (rc = dalloc(c, (void**)&c->sc, sizeof(Scal))) ||
         (rc = dalloc(c, (void**)&c->part, kMaxPartials * 4 * sizeof(double))))
@@ -565,6 +620,7 @@ int mhdp_create(mhdp_ctx** out, const mhdp_grid* grid, const mhdp_phase* phase) 
         return MHDP_DEVICE;
     }
     c->own_stream = true;
+    if (const char* e = std::getenv("MHDP_UNFUSED")) c->fused_enabled = e[0] != '1';
     int rc = ensure_fine(c);
     if (rc) {
         mhdp_destroy(c);
@@ -719,7 +775,7 @@ uint64_t mhdp_device_bytes(const mhdp_ctx* c) { return c ? c->bytes : 0; }
 void mhdp_destroy(mhdp_ctx* c) {
     if (!c) return;
     (void)hipDeviceSynchronize();
-    void* ptrs[] = {c->mask, c->alpha, c->bx, c->by, c->bz, c->diag, c->dinv, c->b, c->x, c->r, c->p, c->q, c->z,
+    void* ptrs[] = {c->mask, c->alpha, c->bx, c->by, c->bz, c->diag, c->dinv, c->b, c->x, c->r, c->p, c->p2, c->q, c->z,
                     c->t, c->x64, c->b64, c->bx16, c->by16, c->bz16, c->dinv16, c->sc, c->part, c->ustar_stage,
                     c->uout_stage, c->phi_stage};
     for (void* p : ptrs) dfree(c, p);
@@ -779,10 +835,20 @@ int mhdp_probe_vcycle(mhdp_ctx* c, int meanfree, int precision, const float* r, 
     if ((rc = upload_scal(c, s))) return rc;
     const size_t fb = (size_t)c->g.n * 4;
     HIP_OK(hipMemcpyAsync(c->r, r, fb, hipMemcpyHostToDevice, c->stream));
-    std::vector<LevelDev> lv = level_view(c);
+    Solve S;
+    S.meanfree = meanfree != 0;
+    S.mg = true;
+    S.store = f16 ? STORE_BF16 : STORE_F32;
+    S.coarse64 = f16;
+    S.lv = level_view(c);
+    S.fused = S.lv.size() >= 2 && c->fused_enabled;
     void* zf = nullptr;
-    launch_vcycle(c->stream, f16 ? STORE_BF16 : STORE_F32, (int)lv.size(), lv.data(), c->r, c->mask, c->sc, f16 ? 1 : 0,
-                  &zf);
+    if (S.fused) {  // the solver's own cycle: fine level fused into two passes
+        vcycle_fused(c, S, FIN_NONE);
+        zf = S.lv[0].t;
+    } else {
+        launch_vcycle(c->stream, S.store, (int)S.lv.size(), S.lv.data(), c->r, c->mask, c->sc, S.coarse64, &zf);
+    }
     if (f16) {
         launch_from_bf16(c->stream, c->g.n, zf, c->q);
         HIP_OK(hipMemcpyAsync(z, c->q, fb, hipMemcpyDeviceToHost, c->stream));

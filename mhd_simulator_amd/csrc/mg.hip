@@ -290,6 +290,50 @@ static void vcycle_t(hipStream_t s, int L, LevelDev* lv, const float* r0, Scal* 
     *zfinal = lv[0].t;
 }
 
+// Levels 1..L-1 of the V-cycle (L >= 2) from the level-1 residual in lv[1].r.
+template <class CT, class VT, class AT>
+static void vcycle_coarse_t(hipStream_t s, int L, LevelDev* lv, Scal* sc, void** z1out, bool f16) {
+    auto C = [&](LevelDev& l, int which) -> const CT* {
+        void* const a16[5] = {l.bx16, l.by16, l.bz16, l.react16, l.dinv16};
+        float* const a32[5] = {l.bx, l.by, l.bz, l.react, l.dinv};
+        return f16 ? (const CT*)a16[which] : (const CT*)a32[which];
+    };
+    const dim3 B(MHDP_BLOCK);
+    for (int l = 1; l < L - 1; l++) {
+        LevelDev& a = lv[l];
+        LevelDev& c = lv[l + 1];
+        hipLaunchKernelGGL((k_dstart<CT, VT, VT>), dim3(grid_blocks(a.g.n)), B, 0, s, a.g.n, (const VT*)a.r, C(a, 4),
+                           (VT*)a.z, sc);
+        hipLaunchKernelGGL((k_restrict_coarse<CT, VT>), dim3(grid_blocks(c.g.n)), B, 0, s, c.g, a.g, (const VT*)a.r,
+                           C(a, 0), C(a, 1), C(a, 2), C(a, 3), (const VT*)a.z, (VT*)c.r, sc);
+    }
+    {
+        LevelDev& c = lv[L - 1];
+        hipLaunchKernelGGL((k_coarse<CT, VT, VT, AT, false>), dim3(1), B, 0, s, c.g, (const VT*)c.r, C(c, 4), C(c, 0),
+                           C(c, 1), C(c, 2), C(c, 3), (VT*)c.z, sc);
+    }
+    for (int l = L - 2; l >= 1; l--) {
+        LevelDev& a = lv[l];
+        LevelDev& c = lv[l + 1];
+        const VT* zc = (const VT*)(l + 1 == L - 1 ? c.z : c.t);
+        hipLaunchKernelGGL((k_prolong<CT, VT>), dim3(grid_blocks(a.g.n)), B, 0, s, a.g, c.g, 0, zc, C(a, 4),
+                           (VT*)a.z, sc);
+        hipLaunchKernelGGL((k_smooth<CT, VT, VT, false>), dim3(grid_blocks(a.g.n)), B, 0, s, a.g, (const VT*)a.r,
+                           C(a, 4), C(a, 0), C(a, 1), C(a, 2), C(a, 3), (const VT*)a.z, (VT*)a.t, sc);
+    }
+    *z1out = (L - 1 == 1) ? lv[1].z : lv[1].t;
+}
+
+void launch_vcycle_coarse(hipStream_t s, int store, int L, LevelDev* lv, Scal* sc, int coarse_f64, void** z1out) {
+    if (store == STORE_BF16) {
+        if (coarse_f64) vcycle_coarse_t<__hip_bfloat16, __hip_bfloat16, double>(s, L, lv, sc, z1out, true);
+        else vcycle_coarse_t<__hip_bfloat16, __hip_bfloat16, float>(s, L, lv, sc, z1out, true);
+    } else {
+        if (coarse_f64) vcycle_coarse_t<float, float, double>(s, L, lv, sc, z1out, false);
+        else vcycle_coarse_t<float, float, float>(s, L, lv, sc, z1out, false);
+    }
+}
+
 void launch_vcycle(hipStream_t s, int store, int L, LevelDev* lv, const float* r0, const uint32_t* /*mask*/,
                    Scal* sc, int coarse_f64, void** zfinal, KernelTimer* timer) {
     if (store == STORE_BF16) {

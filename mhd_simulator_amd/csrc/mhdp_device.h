@@ -36,7 +36,7 @@ struct Scal {
     double tol2, abs2, bound, inv_nf; // Params.solve.xyz, Params.initial.w
     double pq, rr_stash, rz_old, spare;
     uint32_t anchor;                  // gauge cell of the single component (mean-free)
-    uint32_t pad;
+    uint32_t pending;                 // fused path: x += alpha p of the last iteration still owed
     uint32_t ticket[8];
 };
 
@@ -53,6 +53,7 @@ enum Fin : int {
     FIN_INITIAL_MG,    // initial_reduce after initial_direction
     FIN_CHECK,         // check_reduce (solver.wgsl:324-328)
     FIN_NORM64,        // outer refinement: rr of the FP64 residual
+    FIN_UPDATE_MG,     // fused update: mean of r (mean-free) and |r - mean|^2 for beta_reduce
 };
 
 __device__ __forceinline__ float ldf(float v) { return v; }
@@ -152,6 +153,8 @@ __device__ inline void finalize(Scal* s, int mode, const double t[4]) {
     case FIN_INITIAL:  // initial_reduce: scalars[0] = total; scalars[1] = (0, 0, conv, iters)
         s->rz = t[0]; s->rr = t[1]; s->bb = t[2]; s->sr = t[3];
         s->alpha = 0.0; s->beta = 0.0;
+        s->mean = 0.0;
+        s->pending = 0u;
         s->conv = (t[1] <= s->tol2 * t[2] + s->abs2 || t[2] == 0.0) ? 1.0 : 0.0;
         break;
     case FIN_REFRESH_MEAN:  // keep the physical residual and b.b; mean of r for the subspace step
@@ -161,11 +164,18 @@ __device__ inline void finalize(Scal* s, int mode, const double t[4]) {
     case FIN_INITIAL_SUB:  // initial_reduce after pressure_initial_subspace (old.y, old.z kept)
         s->rz = t[0];
         s->alpha = 0.0; s->beta = 0.0;
+        s->mean = 0.0;  // r is projected now
+        s->pending = 0u;
         s->conv = (s->rr <= s->tol2 * s->bb + s->abs2 || s->bb == 0.0) ? 1.0 : 0.0;
         break;
     case FIN_ALPHA:
         s->pq = t[0];
         s->alpha = s->conv > 0.5 ? 0.0 : s->rz / fmax(t[0], 1e-35);
+        s->pending = 1u;
+        break;
+    case FIN_UPDATE_MG:  // sum r^2 and sum r of the updated (unprojected) residual
+        s->mean = t[3] * s->inv_nf;
+        s->rr_stash = t[1] - s->mean * t[3];
         break;
     case FIN_MEAN:
         s->mean = t[3] * s->inv_nf;
@@ -189,6 +199,7 @@ __device__ inline void finalize(Scal* s, int mode, const double t[4]) {
     case FIN_INITIAL_MG:  // initial_reduce after initial_direction: x lane = r.z
         s->rz = t[0];
         s->alpha = 0.0; s->beta = 0.0;
+        s->pending = 0u;
         s->conv = (s->rr <= s->tol2 * s->bb + s->abs2 || s->bb == 0.0) ? 1.0 : 0.0;
         break;
     case FIN_CHECK:

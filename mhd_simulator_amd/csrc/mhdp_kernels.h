@@ -86,3 +86,23 @@ struct KernelTimer {
 // mg.hip — one V-cycle z = M r on the fine level (cycle_stages, multigrid.rs:67-82)
 void launch_vcycle(hipStream_t s, int store, int L, LevelDev* lv, const float* r0, const uint32_t* mask, Scal* sc,
                    int coarse_f64, void** zfinal, KernelTimer* timer = nullptr);
+
+// mg.hip — the coarse part of the V-cycle: levels 1..L-1 given the level-1 residual in lv[1].r;
+// returns the level-1 correction to prolong (lv[1].z if level 1 is the coarsest, else lv[1].t).
+void launch_vcycle_coarse(hipStream_t s, int store, int L, LevelDev* lv, Scal* sc, int coarse_f64, void** z1out);
+
+// fused.hip — the fused fine-grid MG-PCG passes
+bool fused_vec4(const Grid3& g);
+void launch_fused_apply(hipStream_t s, const Grid3& g, int store, const void* z, float* p0, float* p1, float* x,
+                        const float* bx, const float* by, const float* bz, const float* dinv, float* q, double* part,
+                        Scal* sc, int meanfree);
+void launch_fused_xflush(hipStream_t s, const Grid3& g, const float* p0, const float* p1, float* x, const float* dinv,
+                         Scal* sc, int meanfree);
+void launch_clear_pending(hipStream_t s, Scal* sc);
+void launch_fused_update(hipStream_t s, const Grid3& g, float* r, const float* q, const float* dinv, double* part,
+                         Scal* sc);
+void launch_fused_restrict(hipStream_t s, int store, const Grid3& cg, const Grid3& fg, const float* r, const void* dinv,
+                           const void* bx, const void* by, const void* bz, void* rc, Scal* sc);
+void launch_fused_smooth(hipStream_t s, int store, const Grid3& g, const Grid3& cg, const float* r, const void* dinv,
+                         const void* bx, const void* by, const void* bz, const void* zc, void* t, double* part,
+                         Scal* sc, int mode);
