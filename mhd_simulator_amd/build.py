@@ -27,6 +27,7 @@ SOURCES = {
     "pcg.hip": [],
     "mg.hip": [],
     "fused.hip": [],
+    "brick.hip": [],
     "api.cpp": ["-x", "hip"],
 }
 

@@ -63,6 +63,7 @@ struct mhdp_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint64_t bytes = 0;
     bool fused_enabled = true;  // MHDP_UNFUSED=1 selects the one-kernel-per-stage path
+    bool brick_enabled = true;  // MHDP_NO_BRICK=1 keeps the 4-cell-group fused kernels
     KernelTimer timer;
     double timer_ms = 0.0;
     uint32_t timer_launches = 0;
@@ -307,13 +308,16 @@ void vcycle_fused(mhdp_ctx* c, Solve& S, int mode) {
     const void* bz = f16 ? c->bz16 : (const void*)c->bz;
     LevelDev* lv = S.lv.data();
     const int L = (int)S.lv.size();
+    const bool brick = c->brick_enabled && brick_ok(c->g);
     c->timer.before(MHDP_K_RESTRICT, s);
-    launch_fused_restrict(s, S.store, lv[1].g, lv[0].g, c->r, dinv, bx, by, bz, lv[1].r, c->sc);
+    if (brick) launch_brick_restrict(s, S.store, lv[1].g, lv[0].g, c->r, dinv, bx, by, bz, lv[1].r, c->sc);
+    else launch_fused_restrict(s, S.store, lv[1].g, lv[0].g, c->r, dinv, bx, by, bz, lv[1].r, c->sc);
     c->timer.after(MHDP_K_RESTRICT, s);
     void* z1 = nullptr;
     launch_vcycle_coarse(s, S.store, L, lv, c->sc, S.coarse64, &z1);
     c->timer.before(MHDP_K_SMOOTH, s);
-    launch_fused_smooth(s, S.store, lv[0].g, lv[1].g, c->r, dinv, bx, by, bz, z1, lv[0].t, c->part, c->sc, mode);
+    if (brick) launch_brick_smooth(s, S.store, lv[0].g, lv[1].g, c->r, dinv, bx, by, bz, z1, lv[0].t, c->part, c->sc, mode);
+    else launch_fused_smooth(s, S.store, lv[0].g, lv[1].g, c->r, dinv, bx, by, bz, z1, lv[0].t, c->part, c->sc, mode);
     c->timer.after(MHDP_K_SMOOTH, s);
 }
 
@@ -338,8 +342,12 @@ void iterate_once(mhdp_ctx* c, Solve& S) {  // mod.rs:1057-1075, multigrid.rs:30
     hipStream_t s = c->stream;
     if (S.fused) {
         c->timer.before(MHDP_K_APPLY, s);
-        launch_fused_apply(s, c->g, S.store, S.lv[0].t, c->p, c->p2, c->x, c->bx, c->by, c->bz, c->dinv, c->q,
-                           c->part, c->sc, S.meanfree ? 1 : 0);
+        if (c->brick_enabled && brick_ok(c->g))
+            launch_brick_apply(s, c->g, S.store, S.lv[0].t, c->p, c->p2, c->x, c->bx, c->by, c->bz, c->dinv, c->q,
+                               c->part, c->sc, S.meanfree ? 1 : 0);
+        else
+            launch_fused_apply(s, c->g, S.store, S.lv[0].t, c->p, c->p2, c->x, c->bx, c->by, c->bz, c->dinv, c->q,
+                               c->part, c->sc, S.meanfree ? 1 : 0);
         c->timer.after(MHDP_K_APPLY, s);
         c->timer.before(MHDP_K_UPDATE, s);
         launch_fused_update(s, c->g, c->r, c->q, c->dinv, c->part, c->sc);
@@ -621,6 +629,7 @@ int mhdp_create(mhdp_ctx** out, const mhdp_grid* grid, const mhdp_phase* phase) 
     }
     c->own_stream = true;
     if (const char* e = std::getenv("MHDP_UNFUSED")) c->fused_enabled = e[0] != '1';
+    if (const char* e = std::getenv("MHDP_NO_BRICK")) c->brick_enabled = e[0] != '1';
     int rc = ensure_fine(c);
     if (rc) {
         mhdp_destroy(c);

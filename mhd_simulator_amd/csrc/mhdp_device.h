@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
 #include <stdint.h>
+#include <stdlib.h>
 
 #define MHDP_BLOCK 256
 #define MHDP_XCDS 8
@@ -254,7 +255,27 @@ __device__ __forceinline__ void reduce_finish(double (&v)[NV], double* __restric
 struct Launch {
     uint32_t blocks;
 };
-static inline uint32_t grid_blocks(uint32_t n, uint32_t cap = 4096) {
+// Upper bound on the blocks of a cell-parallel launch (each block then owns a contiguous
+// range).  MHDP_GRID_CAP overrides it for launch-shape sweeps.
+static inline uint32_t grid_cap() {
+    static uint32_t cap = [] {
+        const char* e = std::getenv("MHDP_GRID_CAP");
+        long v = e ? std::strtol(e, nullptr, 10) : 0;
+        return (uint32_t)(v >= 8 && v <= 4096 ? v : 4096);
+    }();
+    return cap;
+}
+// Block cap of the brick stencil kernels (MHDP_BRICK_CAP overrides it).
+static inline uint32_t brick_grid_cap() {
+    static uint32_t cap = [] {
+        const char* e = std::getenv("MHDP_BRICK_CAP");
+        long v = e ? std::strtol(e, nullptr, 10) : 0;
+        return (uint32_t)(v >= 8 && v <= 4096 ? v : 4096);
+    }();
+    return cap;
+}
+static inline uint32_t grid_blocks(uint32_t n, uint32_t cap = 0) {
+    if (cap == 0) cap = grid_cap();
     uint32_t need = (n + MHDP_BLOCK - 1) / MHDP_BLOCK;
     uint32_t b = need < cap ? need : cap;
     if (b >= 64) b = (b + MHDP_XCDS - 1) / MHDP_XCDS * MHDP_XCDS;
