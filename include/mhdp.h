@@ -75,6 +75,10 @@ typedef struct {
     double refine_tol;   /* MHDP_MIXED: FP64 relative residual target (e.g. 1e-8) */
     uint32_t max_outer;  /* MHDP_MIXED: refinement step cap (0 = 16) */
     float inner_tol;     /* MHDP_MIXED: inner relative reduction per step (0 = 1e-4) */
+    float smooth_weight; /* V-cycle weighted-Jacobi weight; 0 = tuned default 0.9 (reference: 2/3,
+                            multigrid.wgsl:69) */
+    float coarse_scale;  /* V-cycle coarse-correction scale; 0 = tuned default 1.6 (reference: 1,
+                            multigrid.wgsl:87) */
 } mhdp_opts;
 
 typedef struct {
@@ -137,8 +141,10 @@ int mhdp_kernel_time(const mhdp_ctx *ctx, double *total_ms, uint32_t *launches);
 int mhdp_probe_coefficients(mhdp_ctx *ctx, int meanfree, float *beta3, float *diag, float *dinv);
 /* apply(): q = A p with the cached operator (identity rows at pinned anchors). */
 int mhdp_probe_apply(mhdp_ctx *ctx, int meanfree, const float *p, float *q);
-/* One V-cycle z = M r (precision MHDP_FP32 or MHDP_MIXED storage). */
-int mhdp_probe_vcycle(mhdp_ctx *ctx, int meanfree, int precision, const float *r, float *z);
+/* One V-cycle z = M r (precision MHDP_FP32 or MHDP_MIXED storage) with the given smoother
+ * weight and coarse-correction scale (2/3 and 1 reproduce the reference's cycle). */
+int mhdp_probe_vcycle(mhdp_ctx *ctx, int meanfree, int precision, float smooth_weight, float coarse_scale,
+                      const float *r, float *z);
 /* Coarse graph of level l >= 1 (lower-slot edges x,y,z, reaction, diagonal: 5 floats/cell). */
 int mhdp_probe_mg_graph(mhdp_ctx *ctx, int meanfree, int level, float *graph5, uint32_t *dims4);
 /* Divergence right-hand side b = -h^2 div u* (physical, no anchor zeroing), FP64. */

@@ -172,6 +172,8 @@ int validate_opts(mhdp_ctx* c, const mhdp_opts* o) {
     if (o->precond < 0 || o->precond > 2 || (o->precision != MHDP_FP32 && o->precision != MHDP_MIXED) ||
         (o->gauge != MHDP_GAUGE_AUTO && o->gauge != MHDP_GAUGE_PINNED))
         return fail(c, MHDP_INVALID, "Invalid pressure solver selection");
+    if (!(o->smooth_weight >= 0.0f && o->smooth_weight < 1.0f) || !(o->coarse_scale >= 0.0f && o->coarse_scale < 2.0f))
+        return fail(c, MHDP_INVALID, "Invalid V-cycle smoother weight or coarse-correction scale");
     if (o->precision == MHDP_MIXED &&
         (!(o->refine_tol > 0.0) || !(o->refine_tol < 1.0) || o->inner_tol < 0.0f || o->inner_tol >= 1.0f))
         return fail(c, MHDP_INVALID, "Invalid refinement tolerance");
@@ -373,8 +375,14 @@ void iterate_once(mhdp_ctx* c, Solve& S) {  // mod.rs:1057-1075, multigrid.rs:30
     }
 }
 
-void set_solve_params(mhdp_ctx* c, Scal& s, bool meanfree, double tol2, double abs2, double bound) {
+constexpr double kTunedOmega = 0.9;   // smoother weight picked by the V-cycle tuner
+constexpr double kTunedScale = 1.6;   // coarse-correction scale picked by the V-cycle tuner
+
+void set_solve_params(mhdp_ctx* c, Scal& s, bool meanfree, double tol2, double abs2, double bound,
+                      double omega = kTunedOmega, double pscale = kTunedScale) {
     std::memset(&s, 0, sizeof s);
+    s.omega = omega;
+    s.pscale = pscale;
     s.meanfree = meanfree ? 1.0 : 0.0;
     s.tol2 = tol2;
     s.abs2 = abs2;
@@ -487,6 +495,8 @@ int project_impl(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, float* uou
     }
     if (preferred && (rc = alloc_mg(c, mixed))) return rc;
 
+    const double omega = o->smooth_weight > 0.0f ? (double)o->smooth_weight : kTunedOmega;
+    const double pscale = o->coarse_scale > 0.0f ? (double)o->coarse_scale : kTunedScale;
     Solve S;
     S.meanfree = meanfree;
     S.store = mixed ? STORE_BF16 : STORE_F32;
@@ -506,7 +516,7 @@ int project_impl(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, float* uou
         const float tol2 = tol * tol;
         const float abs2 = ((ab * ab) * (h2 * h2)) * (float)c->fluid_cells;
         Scal s;
-        set_solve_params(c, s, meanfree, tol2, abs2, meanfree ? 1.0 : (double)nf);
+        set_solve_params(c, s, meanfree, tol2, abs2, meanfree ? 1.0 : (double)nf, omega, pscale);
         if ((rc = upload_scal(c, s))) return rc;
         launch_rhs_f32(c->stream, c->g, c->grid.h, c->mask, c->bx, c->by, c->bz, ustar, c->b);
         rc = pcg(c, S, preferred, o->max_iter, true, &iterations, &residual);
@@ -528,7 +538,7 @@ int project_impl(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, float* uou
         launch_rhs_f64(c->stream, c->g, c->grid.h, c->mask, c->bx, c->by, c->bz, ustar, c->b64);
         launch_fill_f64(c->stream, c->g.n, c->x64, 0.0);
         Scal s;
-        set_solve_params(c, s, meanfree, tau * tau, 0.0, 1.0);
+        set_solve_params(c, s, meanfree, tau * tau, 0.0, 1.0, omega, pscale);
         if ((rc = upload_scal(c, s))) return rc;
         bool done = false;
         for (;;) {
@@ -546,7 +556,7 @@ int project_impl(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, float* uou
             double t = std::max(tau, 0.5 * o->refine_tol / res64);
             if (t > 0.5) t = 0.5;
             Scal si;
-            set_solve_params(c, si, meanfree, t * t, 0.0, 1.0);
+            set_solve_params(c, si, meanfree, t * t, 0.0, 1.0, omega, pscale);
             if ((rc = upload_scal(c, si))) return rc;
             uint32_t it = 0;
             rc = pcg(c, S, preferred, o->max_iter, false, &it, &residual);
@@ -832,7 +842,8 @@ int mhdp_probe_apply(mhdp_ctx* c, int meanfree, const float* p, float* q) {
     return MHDP_OK;
 }
 
-int mhdp_probe_vcycle(mhdp_ctx* c, int meanfree, int precision, const float* r, float* z) {
+int mhdp_probe_vcycle(mhdp_ctx* c, int meanfree, int precision, float smooth_weight, float coarse_scale,
+                      const float* r, float* z) {
     if (!c || !r || !z) return MHDP_INVALID;
     const bool f16 = precision == MHDP_MIXED;
     int rc = prepare(c, meanfree != 0);
@@ -840,7 +851,7 @@ int mhdp_probe_vcycle(mhdp_ctx* c, int meanfree, int precision, const float* r, 
     if ((rc = alloc_mg(c, f16))) return rc;
     build_mg(c, meanfree != 0, f16);
     Scal s;
-    set_solve_params(c, s, meanfree != 0, 0.0, 0.0, 1.0);
+    set_solve_params(c, s, meanfree != 0, 0.0, 0.0, 1.0, smooth_weight, coarse_scale);
     if ((rc = upload_scal(c, s))) return rc;
     const size_t fb = (size_t)c->g.n * 4;
     HIP_OK(hipMemcpyAsync(c->r, r, fb, hipMemcpyHostToDevice, c->stream));

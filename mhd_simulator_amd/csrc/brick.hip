@@ -8,7 +8,6 @@
 #include "mhdp_kernels.h"
 
 #define CONVERGED(sc) ((sc)->conv > 0.5)
-#define OMEGA (2.0f / 3.0f)
 
 namespace {
 
@@ -265,6 +264,7 @@ __global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_restrict(Grid3 cg, G
                                                                const CT* __restrict__ by, const CT* __restrict__ bz,
                                                                VT* __restrict__ rc, Scal* sc) {
     if (CONVERGED(sc)) return;
+    const float omega = (float)sc->omega;  // smoother weight (reference: 2/3)
     const float m = (float)sc->mean;
     uint32_t lo, hi;
     block_range(g.n >> 4, lo, hi);
@@ -279,10 +279,10 @@ __global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_restrict(Grid3 cg, G
                 const F4 a = ld4(r, j), d = ld4(dinv, j);
                 F4 o4;
 #pragma unroll
-                for (int k = 0; k < 4; k++) o4.v[k] = OMEGA * (a.v[k] - m) * d.v[k];
+                for (int k = 0; k < 4; k++) o4.v[k] = omega * (a.v[k] - m) * d.v[k];
                 return o4;
             },
-            [&](uint32_t j, int, int, int) { return OMEGA * (r[j] - m) * ldf(dinv[j]); });
+            [&](uint32_t j, int, int, int) { return omega * (r[j] - m) * ldf(dinv[j]); });
         float s0 = 0.0f, s1 = 0.0f;
 #pragma unroll
         for (int dz = 0; dz < 2; dz++)
@@ -311,7 +311,9 @@ __global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_smooth(Grid3 g, Grid
                                                              const VT* __restrict__ zc, VT* __restrict__ t,
                                                              double* __restrict__ part, Scal* sc, int mode) {
     if (CONVERGED(sc)) return;
+    const float omega = (float)sc->omega;  // smoother weight (reference: 2/3)
     const float m = (float)sc->mean;
+    const float ps = (float)sc->pscale;  // coarse-correction scale (reference: 1)
     const uint32_t csy = (uint32_t)cg.nx, csz = (uint32_t)cg.nx * (uint32_t)cg.ny;
     uint32_t lo, hi;
     block_range(g.n >> 4, lo, hi);
@@ -331,13 +333,13 @@ __global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_smooth(Grid3 g, Grid
                 F4 o4;
 #pragma unroll
                 for (int k = 0; k < 4; k++)
-                    o4.v[k] = d.v[k] != 0.0f ? OMEGA * (a.v[k] - m) * d.v[k] + (k < 2 ? c0 : c1) : 0.0f;
+                    o4.v[k] = d.v[k] != 0.0f ? omega * (a.v[k] - m) * d.v[k] + ps * (k < 2 ? c0 : c1) : 0.0f;
                 return o4;
             },
             [&](uint32_t j, int xq, int yq, int zq) {
                 const float dj = ldf(dinv[j]);
-                const float v = OMEGA * (r[j] - m) * dj +
-                                ldf(zc[(uint32_t)(xq >> 1) + csy * (uint32_t)(yq >> 1) + csz * (uint32_t)(zq >> 1)]);
+                const float v = omega * (r[j] - m) * dj +
+                                ps * ldf(zc[(uint32_t)(xq >> 1) + csy * (uint32_t)(yq >> 1) + csz * (uint32_t)(zq >> 1)]);
                 return dj != 0.0f ? v : 0.0f;
             });
 #pragma unroll
@@ -352,7 +354,7 @@ __global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_smooth(Grid3 g, Grid
                     float out = 0.0f;
                     if (di.v[k] != 0.0f) {
                         const float rk = rv.v[k] - m;
-                        out = f.c[dz][dy][k] + OMEGA * (rk - act_at(w, f, dz, dy, k)) * di.v[k];
+                        out = f.c[dz][dy][k] + omega * (rk - act_at(w, f, dz, dy, k)) * di.v[k];
                         acc[0] += (double)rk * (double)out;
                     }
                     tv.v[k] = out;

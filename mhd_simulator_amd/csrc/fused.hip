@@ -20,7 +20,6 @@
 #include "mhdp_kernels.h"
 
 #define CONVERGED(sc) ((sc)->conv > 0.5)
-#define OMEGA (2.0f / 3.0f)
 
 namespace {
 
@@ -307,6 +306,7 @@ __global__ void __launch_bounds__(MHDP_BLOCK) k_fused_restrict(Grid3 cg, Grid3 f
                                                                const CT* __restrict__ bz, VT* __restrict__ rc,
                                                                Scal* sc) {
     if (CONVERGED(sc)) return;
+    const float omega = (float)sc->omega;  // smoother weight (reference: 2/3)
     const float m = (float)sc->mean;
     uint32_t lo, hi;
     block_range(cg.n, lo, hi);
@@ -314,7 +314,7 @@ __global__ void __launch_bounds__(MHDP_BLOCK) k_fused_restrict(Grid3 cg, Grid3 f
     for (uint32_t ci = lo + threadIdx.x; ci < hi; ci += MHDP_BLOCK) {
         int qx, qy, qz;
         decode(cg, ci, qx, qy, qz);
-        auto z1 = [&](uint32_t j) { return OMEGA * (r[j] - m) * ldf(dinv[j]); };
+        auto z1 = [&](uint32_t j) { return omega * (r[j] - m) * ldf(dinv[j]); };
         float rhs = 0.0f;
 #pragma unroll
         for (int dz = 0; dz < 2; dz++)
@@ -328,7 +328,7 @@ __global__ void __launch_bounds__(MHDP_BLOCK) k_fused_restrict(Grid3 cg, Grid3 f
                     const float di = ldf(dinv[i]);
                     if (di == 0.0f) continue;
                     const float rc_i = r[i] - m;
-                    const float c = OMEGA * rc_i * di;
+                    const float c = omega * rc_i * di;
                     uint32_t nb[6];
                     float w[6];
                     stencil6(fg, x, y, z, i, bx, by, bz, nb, w);
@@ -350,7 +350,9 @@ __global__ void __launch_bounds__(MHDP_BLOCK) k_fused_smooth(Grid3 g, Grid3 cg, 
                                                              const VT* __restrict__ zc, VT* __restrict__ t,
                                                              double* __restrict__ part, Scal* sc, int mode) {
     if (CONVERGED(sc)) return;
+    const float omega = (float)sc->omega;  // smoother weight (reference: 2/3)
     const float m = (float)sc->mean;
+    const float ps = (float)sc->pscale;  // coarse-correction scale (reference: 1)
     const uint32_t csy = (uint32_t)cg.nx, csz = (uint32_t)cg.nx * (uint32_t)cg.ny;
     uint32_t lo, hi;
     block_range(g.n / V, lo, hi);
@@ -359,7 +361,7 @@ __global__ void __launch_bounds__(MHDP_BLOCK) k_fused_smooth(Grid3 g, Grid3 cg, 
     auto z2 = [&](uint32_t j, int xq, int yq, int zq) {
         const float dj = ldf(dinv[j]);
         if (dj == 0.0f) return 0.0f;
-        return OMEGA * (r[j] - m) * dj + ldf(zc[(uint32_t)(xq >> 1) + csy * (uint32_t)(yq >> 1) + csz * (uint32_t)(zq >> 1)]);
+        return omega * (r[j] - m) * dj + ps * ldf(zc[(uint32_t)(xq >> 1) + csy * (uint32_t)(yq >> 1) + csz * (uint32_t)(zq >> 1)]);
     };
     for (uint32_t gi = lo + threadIdx.x; gi < hi; gi += MHDP_BLOCK) {
         const uint32_t i0 = gi * V;
@@ -374,7 +376,7 @@ __global__ void __launch_bounds__(MHDP_BLOCK) k_fused_smooth(Grid3 g, Grid3 cg, 
             VF<V> o;
 #pragma unroll
             for (int k = 0; k < V; k++)
-                o.v[k] = dv.v[k] != 0.0f ? OMEGA * (rv.v[k] - m) * dv.v[k] + ldf(zc[cb + (uint32_t)((x0 + k) >> 1)])
+                o.v[k] = dv.v[k] != 0.0f ? omega * (rv.v[k] - m) * dv.v[k] + ps * ldf(zc[cb + (uint32_t)((x0 + k) >> 1)])
                                          : 0.0f;
             return o;
         };
@@ -401,7 +403,7 @@ __global__ void __launch_bounds__(MHDP_BLOCK) k_fused_smooth(Grid3 g, Grid3 cg, 
             float out = 0.0f;
             if (di.v[k] != 0.0f) {
                 const float rk = rv.v[k] - m;
-                out = s.c[k] + OMEGA * (rk - star_action<V, CT>(W, s, k)) * di.v[k];
+                out = s.c[k] + omega * (rk - star_action<V, CT>(W, s, k)) * di.v[k];
                 acc[0] += (double)rk * (double)out;
             }
             tv.v[k] = out;

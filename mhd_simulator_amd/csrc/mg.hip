@@ -7,7 +7,6 @@
 #include "mhdp_kernels.h"
 
 #define CONVERGED(sc) ((sc)->conv > 0.5)
-#define OMEGA (2.0f / 3.0f)
 
 namespace {
 
@@ -62,11 +61,12 @@ template <class CT, class RT, class VT>
 __global__ void __launch_bounds__(MHDP_BLOCK) k_dstart(uint32_t n, const RT* __restrict__ r,
                                                        const CT* __restrict__ dinv, VT* __restrict__ z, Scal* sc) {
     if (CONVERGED(sc)) return;
+    const float omega = (float)sc->omega;  // smoother weight (reference: 2/3)
     uint32_t lo, hi;
     block_range(n, lo, hi);
     for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
         const float di = ldf(dinv[i]);
-        z[i] = cvt<VT>(di != 0.0f ? OMEGA * ldf(r[i]) * di : 0.0f);
+        z[i] = cvt<VT>(di != 0.0f ? omega * ldf(r[i]) * di : 0.0f);
     }
 }
 
@@ -139,7 +139,7 @@ __global__ void __launch_bounds__(MHDP_BLOCK) k_prolong(Grid3 fg, Grid3 cg, int 
         if (fine && ldf(dinv[i]) == 0.0f) continue;
         const uint32_t pi =
             (uint32_t)(x / 2) + (uint32_t)cg.nx * ((uint32_t)(y / 2) + (uint32_t)cg.ny * (uint32_t)(z / 2));
-        zf[i] = cvt<VT>(ldf(zf[i]) + ldf(zc[pi]));
+        zf[i] = cvt<VT>(ldf(zf[i]) + (float)sc->pscale * ldf(zc[pi]));
     }
 }
 
@@ -150,6 +150,7 @@ __global__ void __launch_bounds__(MHDP_BLOCK) k_smooth(Grid3 g, const RT* __rest
                                                        const CT* __restrict__ b2, const CT* __restrict__ react,
                                                        const VT* __restrict__ z, VT* __restrict__ t, Scal* sc) {
     if (CONVERGED(sc)) return;
+    const float omega = (float)sc->omega;  // smoother weight (reference: 2/3)
     uint32_t lo, hi;
     block_range(g.n, lo, hi);
     for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
@@ -161,7 +162,7 @@ __global__ void __launch_bounds__(MHDP_BLOCK) k_smooth(Grid3 g, const RT* __rest
             const float c = ldf(z[i]);
             const float a = FINE ? fine_action(g, x, y, zz, i, b0, b1, b2, z, c)
                                  : coarse_action(g, x, y, zz, i, b0, b1, b2, react, z, c);
-            out = c + OMEGA * (ldf(r[i]) - a) * di;
+            out = c + omega * (ldf(r[i]) - a) * di;
         }
         t[i] = cvt<VT>(out);
     }
@@ -208,7 +209,7 @@ __global__ void __launch_bounds__(MHDP_BLOCK) k_coarse(Grid3 g, const RT* __rest
                     const AT beta = (AT)ldf(s > 0 ? B[ax][ni] : B[ax][i]);
                     a += beta * (center - nbv);
                 }
-            next = center + (AT)(2.0 / 3.0) * (rhs - a) * di;
+            next = center + (AT)sc->omega * (rhs - a) * di;
         }
         __syncthreads();
         cs[i] = next;

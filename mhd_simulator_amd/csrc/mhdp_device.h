@@ -36,6 +36,7 @@ struct Scal {
     double meanfree, mean;            // scalars[2].xy
     double tol2, abs2, bound, inv_nf; // Params.solve.xyz, Params.initial.w
     double pq, rr_stash, rz_old, spare;
+    double omega, pscale;             // V-cycle smoother weight and coarse-correction scale
     uint32_t anchor;                  // gauge cell of the single component (mean-free)
     uint32_t pending;                 // fused path: x += alpha p of the last iteration still owed
     uint32_t ticket[8];

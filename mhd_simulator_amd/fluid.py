@@ -114,21 +114,22 @@ class PressureProjection:
     def device_bytes(self) -> int:
         return int(self.lib.mhdp_device_bytes(self.handle))
 
-    def _opts(self, precision, precond, gauge, refine_tol, max_outer, inner_tol, rel_tol=None):
+    def _opts(self, precision, precond, gauge, refine_tol, max_outer, inner_tol, smooth_weight=0.0,
+              coarse_scale=0.0):
         c = self.config
-        return _lib.Opts(c.relative_tolerance if rel_tol is None else rel_tol, c.absolute_divergence_tolerance,
+        return _lib.Opts(c.relative_tolerance, c.absolute_divergence_tolerance,
                          c.max_iterations, _lib.PRECOND[precond], _lib.PRECISION[precision], _lib.GAUGE[gauge],
-                         refine_tol, max_outer, inner_tol)
+                         refine_tol, max_outer, inner_tol, smooth_weight, coarse_scale)
 
     # -- FluidState::project(false)
     def project(self, u_star, *, precision="fp32", precond="auto", gauge="auto", refine_tol=1e-8,
-                max_outer=0, inner_tol=0.0, u_out=None, phi_out=None):
+                max_outer=0, inner_tol=0.0, smooth_weight=0.0, coarse_scale=0.0, u_out=None, phi_out=None):
         """Project u_star; returns (iterations, relative residual) like the reference.
 
         numpy input: host path, results in ``self.velocity`` / ``self.phi``.
         torch CUDA input: device path; ``u_out`` (float32, faces) and ``phi_out``
         (float64, cells) are device tensors the caller provides or None."""
-        o = self._opts(precision, precond, gauge, refine_tol, max_outer, inner_tol)
+        o = self._opts(precision, precond, gauge, refine_tol, max_outer, inner_tol, smooth_weight, coarse_scale)
         st = _lib.Stats()
         if hasattr(u_star, "data_ptr"):
             if u_star.numel() != self.grid.faces:
@@ -163,11 +164,13 @@ class PressureProjection:
         self._check(self.lib.mhdp_probe_apply(self.handle, int(meanfree), _ptr(p), _ptr(q)))
         return q
 
-    def vcycle(self, r: np.ndarray, meanfree: bool, precision="fp32"):
+    def vcycle(self, r: np.ndarray, meanfree: bool, precision="fp32", smooth_weight=2.0 / 3.0,
+               coarse_scale=1.0):
+        """One V-cycle; the default weights reproduce the reference's cycle."""
         r = np.ascontiguousarray(r, dtype=np.float32)
         z = np.zeros(self.grid.cells, dtype=np.float32)
-        self._check(self.lib.mhdp_probe_vcycle(self.handle, int(meanfree), _lib.PRECISION[precision], _ptr(r),
-                                               _ptr(z)))
+        self._check(self.lib.mhdp_probe_vcycle(self.handle, int(meanfree), _lib.PRECISION[precision],
+                                               smooth_weight, coarse_scale, _ptr(r), _ptr(z)))
         return z
 
     def mg_graph(self, level: int, meanfree: bool):

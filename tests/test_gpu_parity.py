@@ -132,6 +132,11 @@ def test_operator_and_vcycle_match_oracle(orc, name, meanfree):
         # mixed-precision V-cycle: BF16 storage, FP32 accumulation, FP64 coarsest solve
         mr16 = pr.vcycle(r, meanfree, "mixed")
         assert relerr(mr16, mr) < 3e-2 and float(r @ mr16.astype(np.float64)) > 0
+        # the tuned cycle (smoother weight 0.9, coarse correction x1.6) stays symmetric positive
+        tr_, ts_ = (pr.vcycle(v_, meanfree, smooth_weight=0.9, coarse_scale=1.6) for v_ in (r, s))
+        rr, ss = float(r @ tr_.astype(np.float64)), float(s @ ts_.astype(np.float64))
+        sym = abs(float(r @ ts_.astype(np.float64)) - float(s @ tr_.astype(np.float64))) / math.sqrt(rr * ss)
+        assert rr > 0 and ss > 0 and sym < 3e-6
 
 
 def oracle_solution(orc, p, mask):
