@@ -65,7 +65,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--kernel", default="", help="kernel timed live (default: the dominant one found)")
-    ap.add_argument("--cpu-size", type=int, default=128)
+    ap.add_argument("--cpu-size", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
@@ -162,21 +162,27 @@ def main():
                 "avg_launch_us": avg_launch_s * 1e6, "algorithmic_bytes_per_launch": alg_bytes}
 
     # whole-solve algorithmic bandwidth (bytes every kernel must move) for context
+    # plain FP32 PCG-MG on the same problem (same V-cycle, FP32 storage, the reference's
+    # stopping rule): it cannot reach 1e-8 (the API floor is 1e-7 and FP32 CG stalls near it),
+    # so it is timed at 1e-7 (capped at 400 iterations) and at 1e-6.
     fp32 = None
     if not args.no_fp32:
-        t1 = time.perf_counter()
-        try:
-            it32, res32 = pr.project(u_in, precision="fp32", precond="mg", u_out=u_out, phi_out=phi)
-            ok = True
-        except Exception as e:  # FP32 PCG stalls above 1e-7 on this contrast: report where
-            it32, res32, ok = None, None, False
-            log(f"fp32: {e}")
-        dt32 = time.perf_counter() - t1
-        fp32 = {"precision": "fp32 PCG-MG", "rel_tol": 1e-7, "converged": ok, "seconds": dt32,
-                "iterations": it32 if ok else None, "rel_residual": res32,
-                "cells_per_s": fluid / dt32 if ok else None,
-                "note": "1e-7 is the lowest tolerance the reference API accepts; FP32 PCG cannot reach 1e-8"}
-        log(f"fp32: {fp32}")
+        fp32 = {"precision": "fp32 PCG-MG (FP32 V-cycle)", "reaches_1e-8": False, "runs": []}
+        for tol in (1e-7, 1e-6):
+            pr.config.relative_tolerance = tol
+            pr.config.max_iterations = 400
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            try:
+                it32, res32 = pr.project(u_in, precision="fp32", precond="mg", u_out=u_out, phi_out=phi)
+                ok, note = True, ""
+            except Exception as e:  # stalls at its FP32 floor: report where
+                it32, res32, ok, note = None, None, False, str(e)
+            dt32 = time.perf_counter() - t1
+            fp32["runs"].append({"rel_tol": tol, "converged": ok, "seconds": dt32, "iterations": it32,
+                                 "rel_residual": res32, "cells_per_s": fluid / dt32 if ok else None,
+                                 "error": note or None})
+            log(f"fp32 @ {tol}: converged={ok} {dt32 * 1e3:.1f} ms it={it32} {note[:120]}")
 
     cpu = None
     if not args.no_cpu:

@@ -129,7 +129,8 @@ int mhdp_hierarchy(const uint32_t dims4[4], uint32_t *level_dims, int max_levels
 
 /* Live kernel timing: while enabled, HIP events bracket every launch of `kernel` on the
  * context's stream during mhdp_project*; mhdp_kernel_time() returns the summed duration
- * (ms) and launch count of the last projection.  Kernels: MHDP_K_APPLY (operator apply +
+ * (ms) and count of the launches of the last projection that did work (a launch queued
+ * after convergence inside a batch returns at once and is not counted).  Kernels: MHDP_K_APPLY (operator apply +
  * p.q), MHDP_K_UPDATE (x/r update + norms), MHDP_K_SMOOTH (fine-level smoother),
  * MHDP_K_RESTRICT (fine residual restriction). */
 enum { MHDP_K_NONE = -1, MHDP_K_APPLY = 0, MHDP_K_UPDATE = 1, MHDP_K_SMOOTH = 2, MHDP_K_RESTRICT = 3 };

@@ -262,7 +262,7 @@ static inline uint32_t grid_cap() {
     static uint32_t cap = [] {
         const char* e = std::getenv("MHDP_GRID_CAP");
         long v = e ? std::strtol(e, nullptr, 10) : 0;
-        return (uint32_t)(v >= 8 && v <= 4096 ? v : 4096);
+        return (uint32_t)(v >= 8 && v <= 4096 ? v : 1024);
     }();
     return cap;
 }
@@ -271,7 +271,7 @@ static inline uint32_t brick_grid_cap() {
     static uint32_t cap = [] {
         const char* e = std::getenv("MHDP_BRICK_CAP");
         long v = e ? std::strtol(e, nullptr, 10) : 0;
-        return (uint32_t)(v >= 8 && v <= 4096 ? v : 4096);
+        return (uint32_t)(v >= 8 && v <= 4096 ? v : 1024);
     }();
     return cap;
 }
