@@ -27,12 +27,15 @@ sys.path.insert(0, ROOT)
 
 METRIC = "pressure-solve cells/s to 1e-8 rel. residual (mixed vs FP32 PCG-MG); HBM GB/s"
 HBM_PEAK = 8.0e12
-# algorithmic bytes per cell and launch of the timed kernels (FP32 vectors, cell-indexed
-# lower-face mobilities): apply reads p and 3 mobilities, writes q; update reads
-# x, r, p, q, 1/diag and writes x, r; the BF16 fine smoother reads r (FP32), 1/diag, 3
-# mobilities, z and writes t (BF16); the BF16 fine restriction reads r, 1/diag,
-# 3 mobilities, z and writes 1/8 of a BF16 coarse residual.
-KERNEL_BYTES = {"apply": 20.0, "update": 28.0, "smooth": 16.0, "restrict": 14.25}
+# Algorithmic bytes per cell and launch of the four fine-grid passes of the mixed solver
+# (every array touched once; neighbour reads are not counted again):
+#   apply    z (bf16) 2 + p_old 4 + x 4 + 3 FP32 mobilities 12 + 1/diag 4 read,
+#            p_new 4 + q 4 + x 4 written                                        = 38
+#   update   r 4 + q 4 + 1/diag 4 read, r 4 written                              = 16
+#   smooth   r 4 + 1/diag (bf16) 2 + 3 bf16 mobilities 6 + coarse z 1/8*2 read,
+#            z (bf16) 2 written                                                  = 14.25
+#   restrict r 4 + 1/diag 2 + mobilities 6 read, coarse residual 1/8*2 written   = 12.25
+KERNEL_BYTES = {"apply": 38.0, "update": 16.0, "smooth": 14.25, "restrict": 12.25}
 
 
 def log(msg):
@@ -157,8 +160,18 @@ def main():
     avg_launch_s = kern_ms / max(kern_launches, 1) * 1e-3
     alg_bytes = KERNEL_BYTES[kernel] * p.grid.cells
     achieved = alg_bytes / avg_launch_s if avg_launch_s > 0 else 0.0
+    # HBM bytes per launch of the same kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    # summary (KiB per dispatch, scaled by the counters' reading on a plain 1 GiB copy)
+    traffic = None
+    try:
+        prof = json.load(open(os.path.join(ROOT, "profiles", "projection_512_hbm_counters.json")))
+        if prof["cells"] == p.grid.cells and kernel in prof["kernels"]:
+            kk = prof["kernels"][kernel]
+            traffic = 1024.0 * (kk["fetch_kib"] * prof["copy_scale_fetch"] + kk["write_kib"] * prof["copy_scale_write"])
+    except (OSError, KeyError, ValueError):
+        traffic = None
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK, "traffic": None, "kernel": kernel,
+                "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": kernel,
                 "avg_launch_us": avg_launch_s * 1e6, "algorithmic_bytes_per_launch": alg_bytes}
 
     # whole-solve algorithmic bandwidth (bytes every kernel must move) for context
