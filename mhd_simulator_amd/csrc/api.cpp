@@ -648,7 +648,7 @@ int mhdp_create(mhdp_ctx** out, const mhdp_grid* grid, const mhdp_phase* phase) 
     }
     c->own_stream = true;
     if (const char* e = std::getenv("MHDP_UNFUSED")) c->fused_enabled = e[0] != '1';
-    if (const char* e = std::getenv("MHDP_NO_BRICK")) c->brick_enabled = e[0] != '1';
+    c->brick_enabled = brick_enabled_env();
     int rc = ensure_fine(c);
     if (rc) {
         mhdp_destroy(c);

@@ -198,6 +198,14 @@ __device__ __forceinline__ float act_at(const Wts& w, const Field& f, int dz, in
 }  // namespace
 
 bool brick_ok(const Grid3& g) { return (g.nx & 3) == 0 && (g.ny & 1) == 0 && (g.nz & 1) == 0; }
+// MHDP_NO_BRICK=1 keeps the one-cell / four-cell kernels (comparison runs).
+bool brick_enabled_env() {
+    static const bool on = [] {
+        const char* e = std::getenv("MHDP_NO_BRICK");
+        return !(e && e[0] == '1');
+    }();
+    return on;
+}
 
 // pass 1 (k_fused_apply on bricks)
 template <class VT, int MINB>
@@ -257,15 +265,19 @@ __global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_apply(Grid3 g, const
     reduce_finish<1>(acc, part, sc, 2, FIN_ALPHA);
 }
 
-// pass 3 (k_fused_restrict on bricks): the two aggregates of a brick.
-template <class CT, class VT, int MINB>
-__global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_restrict(Grid3 cg, Grid3 g, const float* __restrict__ r,
-                                                               const CT* __restrict__ dinv, const CT* __restrict__ bx,
-                                                               const CT* __restrict__ by, const CT* __restrict__ bz,
-                                                               VT* __restrict__ rc, Scal* sc) {
+// pass 3 (k_fused_restrict on bricks): the two aggregates of a brick.  COARSE: a coarse level
+// (Galerkin graph with reaction term; every child contributes, multigrid.wgsl:75-83; no mean).
+template <class CT, class VT, class RT, bool COARSE, int MINB>
+__global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_restrict(Grid3 cg, Grid3 g, const RT* __restrict__ r,
+                                                                     const CT* __restrict__ dinv,
+                                                                     const CT* __restrict__ bx,
+                                                                     const CT* __restrict__ by,
+                                                                     const CT* __restrict__ bz,
+                                                                     const CT* __restrict__ react,
+                                                                     VT* __restrict__ rc, Scal* sc) {
     if (CONVERGED(sc)) return;
     const float omega = (float)sc->omega;  // smoother weight (reference: 2/3)
-    const float m = (float)sc->mean;
+    const float m = COARSE ? 0.0f : (float)sc->mean;
     uint32_t lo, hi;
     block_range(g.n >> 4, lo, hi);
     for (uint32_t b = lo + threadIdx.x; b < hi; b += MHDP_BLOCK) {
@@ -282,7 +294,7 @@ __global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_restrict(Grid3 cg, G
                 for (int k = 0; k < 4; k++) o4.v[k] = omega * (a.v[k] - m) * d.v[k];
                 return o4;
             },
-            [&](uint32_t j, int, int, int) { return omega * (r[j] - m) * ldf(dinv[j]); });
+            [&](uint32_t j, int, int, int) { return omega * (ldf(r[j]) - m) * ldf(dinv[j]); });
         float s0 = 0.0f, s1 = 0.0f;
 #pragma unroll
         for (int dz = 0; dz < 2; dz++)
@@ -290,10 +302,14 @@ __global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_restrict(Grid3 cg, G
             for (int dy = 0; dy < 2; dy++) {
                 const uint32_t row = o.row[dz][dy];
                 const F4 rv = ld4(r, row), di = ld4(dinv, row);
+                F4 re{{0.f, 0.f, 0.f, 0.f}};
+                if (COARSE) re = ld4(react, row);
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
-                    if (di.v[k] == 0.0f) continue;
-                    const float v = (rv.v[k] - m) - act_at(w, f, dz, dy, k);
+                    if (!COARSE && di.v[k] == 0.0f) continue;
+                    float a = act_at(w, f, dz, dy, k);
+                    if (COARSE) a = re.v[k] * f.c[dz][dy][k] + a;
+                    const float v = (rv.v[k] - m) - a;
                     if (k < 2) s0 += v;
                     else s1 += v;
                 }
@@ -303,16 +319,19 @@ __global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_restrict(Grid3 cg, G
     }
 }
 
-// pass 4 (k_fused_smooth on bricks)
-template <class CT, class VT, int MINB>
-__global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_smooth(Grid3 g, Grid3 cg, const float* __restrict__ r,
-                                                             const CT* __restrict__ dinv, const CT* __restrict__ bx,
-                                                             const CT* __restrict__ by, const CT* __restrict__ bz,
-                                                             const VT* __restrict__ zc, VT* __restrict__ t,
-                                                             double* __restrict__ part, Scal* sc, int mode) {
+// pass 4 (k_fused_smooth on bricks).  COARSE: coarse level (reaction term, no mean, no r.z).
+template <class CT, class VT, class RT, bool COARSE, int MINB>
+__global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_smooth(Grid3 g, Grid3 cg, const RT* __restrict__ r,
+                                                                   const CT* __restrict__ dinv,
+                                                                   const CT* __restrict__ bx,
+                                                                   const CT* __restrict__ by,
+                                                                   const CT* __restrict__ bz,
+                                                                   const CT* __restrict__ react,
+                                                                   const VT* __restrict__ zc, VT* __restrict__ t,
+                                                                   double* __restrict__ part, Scal* sc, int mode) {
     if (CONVERGED(sc)) return;
     const float omega = (float)sc->omega;  // smoother weight (reference: 2/3)
-    const float m = (float)sc->mean;
+    const float m = COARSE ? 0.0f : (float)sc->mean;
     const float ps = (float)sc->pscale;  // coarse-correction scale (reference: 1)
     const uint32_t csy = (uint32_t)cg.nx, csz = (uint32_t)cg.nx * (uint32_t)cg.ny;
     uint32_t lo, hi;
@@ -338,7 +357,7 @@ __global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_smooth(Grid3 g, Grid
             },
             [&](uint32_t j, int xq, int yq, int zq) {
                 const float dj = ldf(dinv[j]);
-                const float v = omega * (r[j] - m) * dj +
+                const float v = omega * (ldf(r[j]) - m) * dj +
                                 ps * ldf(zc[(uint32_t)(xq >> 1) + csy * (uint32_t)(yq >> 1) + csz * (uint32_t)(zq >> 1)]);
                 return dj != 0.0f ? v : 0.0f;
             });
@@ -348,21 +367,25 @@ __global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_smooth(Grid3 g, Grid
             for (int dy = 0; dy < 2; dy++) {
                 const uint32_t row = o.row[dz][dy];
                 const F4 rv = ld4(r, row), di = ld4(dinv, row);
+                F4 re{{0.f, 0.f, 0.f, 0.f}};
+                if (COARSE) re = ld4(react, row);
                 F4 tv;
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
                     float out = 0.0f;
                     if (di.v[k] != 0.0f) {
                         const float rk = rv.v[k] - m;
-                        out = f.c[dz][dy][k] + omega * (rk - act_at(w, f, dz, dy, k)) * di.v[k];
-                        acc[0] += (double)rk * (double)out;
+                        float a = act_at(w, f, dz, dy, k);
+                        if (COARSE) a = re.v[k] * f.c[dz][dy][k] + a;
+                        out = f.c[dz][dy][k] + omega * (rk - a) * di.v[k];
+                        if (!COARSE) acc[0] += (double)rk * (double)out;
                     }
                     tv.v[k] = out;
                 }
                 st4(t, row, tv);
             }
     }
-    reduce_finish<1>(acc, part, sc, 4, mode);
+    if constexpr (!COARSE) reduce_finish<1>(acc, part, sc, 4, mode);
 }
 
 // ---------------------------------------------------------------- launchers
@@ -400,12 +423,12 @@ void launch_brick_restrict(hipStream_t s, int store, const Grid3& cg, const Grid
     const dim3 B(MHDP_BLOCK), G(grid_blocks(g.n >> 4, brick_grid_cap()));
     if (store == STORE_BF16) {
         using T = __hip_bfloat16;
-        BRICK_DISPATCH(k_brick_restrict, T COMMA T, cg, g, r, (const T*)dinv, (const T*)bx, (const T*)by, (const T*)bz,
-                       (T*)rc, sc);
+        BRICK_DISPATCH(k_brick_restrict, T COMMA T COMMA float COMMA false, cg, g, r, (const T*)dinv, (const T*)bx,
+                       (const T*)by, (const T*)bz, (const T*)nullptr, (T*)rc, sc);
     } else {
         using T = float;
-        BRICK_DISPATCH(k_brick_restrict, T COMMA T, cg, g, r, (const T*)dinv, (const T*)bx, (const T*)by, (const T*)bz,
-                       (T*)rc, sc);
+        BRICK_DISPATCH(k_brick_restrict, T COMMA T COMMA float COMMA false, cg, g, r, (const T*)dinv, (const T*)bx,
+                       (const T*)by, (const T*)bz, (const T*)nullptr, (T*)rc, sc);
     }
 }
 void launch_brick_smooth(hipStream_t s, int store, const Grid3& g, const Grid3& cg, const float* r, const void* dinv,
@@ -414,11 +437,40 @@ void launch_brick_smooth(hipStream_t s, int store, const Grid3& g, const Grid3& 
     const dim3 B(MHDP_BLOCK), G(grid_blocks(g.n >> 4, brick_grid_cap()));
     if (store == STORE_BF16) {
         using T = __hip_bfloat16;
-        BRICK_DISPATCH(k_brick_smooth, T COMMA T, g, cg, r, (const T*)dinv, (const T*)bx, (const T*)by, (const T*)bz,
-                       (const T*)zc, (T*)t, part, sc, mode);
+        BRICK_DISPATCH(k_brick_smooth, T COMMA T COMMA float COMMA false, g, cg, r, (const T*)dinv, (const T*)bx,
+                       (const T*)by, (const T*)bz, (const T*)nullptr, (const T*)zc, (T*)t, part, sc, mode);
     } else {
         using T = float;
-        BRICK_DISPATCH(k_brick_smooth, T COMMA T, g, cg, r, (const T*)dinv, (const T*)bx, (const T*)by, (const T*)bz,
-                       (const T*)zc, (T*)t, part, sc, mode);
+        BRICK_DISPATCH(k_brick_smooth, T COMMA T COMMA float COMMA false, g, cg, r, (const T*)dinv, (const T*)bx,
+                       (const T*)by, (const T*)bz, (const T*)nullptr, (const T*)zc, (T*)t, part, sc, mode);
+    }
+}
+// Coarse level l (Galerkin graph, residual in the cycle's storage type).
+void launch_brick_restrict_coarse(hipStream_t s, int store, const Grid3& cg, const Grid3& g, const void* r,
+                                  const void* dinv, const void* bx, const void* by, const void* bz, const void* react,
+                                  void* rc, Scal* sc) {
+    const dim3 B(MHDP_BLOCK), G(grid_blocks(g.n >> 4, brick_grid_cap()));
+    if (store == STORE_BF16) {
+        using T = __hip_bfloat16;
+        BRICK_DISPATCH(k_brick_restrict, T COMMA T COMMA T COMMA true, cg, g, (const T*)r, (const T*)dinv,
+                       (const T*)bx, (const T*)by, (const T*)bz, (const T*)react, (T*)rc, sc);
+    } else {
+        using T = float;
+        BRICK_DISPATCH(k_brick_restrict, T COMMA T COMMA T COMMA true, cg, g, (const T*)r, (const T*)dinv,
+                       (const T*)bx, (const T*)by, (const T*)bz, (const T*)react, (T*)rc, sc);
+    }
+}
+void launch_brick_smooth_coarse(hipStream_t s, int store, const Grid3& g, const Grid3& cg, const void* r,
+                                const void* dinv, const void* bx, const void* by, const void* bz, const void* react,
+                                const void* zc, void* t, Scal* sc) {
+    const dim3 B(MHDP_BLOCK), G(grid_blocks(g.n >> 4, brick_grid_cap()));
+    if (store == STORE_BF16) {
+        using T = __hip_bfloat16;
+        BRICK_DISPATCH(k_brick_smooth, T COMMA T COMMA T COMMA true, g, cg, (const T*)r, (const T*)dinv, (const T*)bx,
+                       (const T*)by, (const T*)bz, (const T*)react, (const T*)zc, (T*)t, (double*)nullptr, sc, 0);
+    } else {
+        using T = float;
+        BRICK_DISPATCH(k_brick_smooth, T COMMA T COMMA T COMMA true, g, cg, (const T*)r, (const T*)dinv, (const T*)bx,
+                       (const T*)by, (const T*)bz, (const T*)react, (const T*)zc, (T*)t, (double*)nullptr, sc, 0);
     }
 }

@@ -300,9 +300,16 @@ static void vcycle_coarse_t(hipStream_t s, int L, LevelDev* lv, Scal* sc, void**
         return f16 ? (const CT*)a16[which] : (const CT*)a32[which];
     };
     const dim3 B(MHDP_BLOCK);
+    const int store = f16 ? STORE_BF16 : STORE_F32;
+    // levels whose dims allow bricks run the fused restrict / smooth passes
+    auto brick = [&](const LevelDev& l) { return brick_enabled_env() && brick_ok(l.g); };
     for (int l = 1; l < L - 1; l++) {
         LevelDev& a = lv[l];
         LevelDev& c = lv[l + 1];
+        if (brick(a)) {
+            launch_brick_restrict_coarse(s, store, c.g, a.g, a.r, C(a, 4), C(a, 0), C(a, 1), C(a, 2), C(a, 3), c.r, sc);
+            continue;
+        }
         hipLaunchKernelGGL((k_dstart<CT, VT, VT>), dim3(grid_blocks(a.g.n)), B, 0, s, a.g.n, (const VT*)a.r, C(a, 4),
                            (VT*)a.z, sc);
         hipLaunchKernelGGL((k_restrict_coarse<CT, VT>), dim3(grid_blocks(c.g.n)), B, 0, s, c.g, a.g, (const VT*)a.r,
@@ -317,6 +324,10 @@ static void vcycle_coarse_t(hipStream_t s, int L, LevelDev* lv, Scal* sc, void**
         LevelDev& a = lv[l];
         LevelDev& c = lv[l + 1];
         const VT* zc = (const VT*)(l + 1 == L - 1 ? c.z : c.t);
+        if (brick(a)) {
+            launch_brick_smooth_coarse(s, store, a.g, c.g, a.r, C(a, 4), C(a, 0), C(a, 1), C(a, 2), C(a, 3), zc, a.t, sc);
+            continue;
+        }
         hipLaunchKernelGGL((k_prolong<CT, VT>), dim3(grid_blocks(a.g.n)), B, 0, s, a.g, c.g, 0, zc, C(a, 4),
                            (VT*)a.z, sc);
         hipLaunchKernelGGL((k_smooth<CT, VT, VT, false>), dim3(grid_blocks(a.g.n)), B, 0, s, a.g, (const VT*)a.r,

@@ -109,6 +109,7 @@ void launch_fused_smooth(hipStream_t s, int store, const Grid3& g, const Grid3& 
 
 // brick.hip — 4x2x2-cell-per-thread variants (nx % 4 == 0, even ny and nz)
 bool brick_ok(const Grid3& g);
+bool brick_enabled_env();
 void launch_brick_apply(hipStream_t s, const Grid3& g, int store, const void* z, float* p0, float* p1, float* x,
                         const float* bx, const float* by, const float* bz, const float* dinv, float* q, double* part,
                         Scal* sc, int meanfree);
@@ -117,3 +118,9 @@ void launch_brick_restrict(hipStream_t s, int store, const Grid3& cg, const Grid
 void launch_brick_smooth(hipStream_t s, int store, const Grid3& g, const Grid3& cg, const float* r, const void* dinv,
                          const void* bx, const void* by, const void* bz, const void* zc, void* t, double* part,
                          Scal* sc, int mode);
+void launch_brick_restrict_coarse(hipStream_t s, int store, const Grid3& cg, const Grid3& g, const void* r,
+                                  const void* dinv, const void* bx, const void* by, const void* bz, const void* react,
+                                  void* rc, Scal* sc);
+void launch_brick_smooth_coarse(hipStream_t s, int store, const Grid3& g, const Grid3& cg, const void* r,
+                                const void* dinv, const void* bx, const void* by, const void* bz, const void* react,
+                                const void* zc, void* t, Scal* sc);
