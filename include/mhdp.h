@@ -111,7 +111,12 @@ int mhdp_project(mhdp_ctx *ctx, const float *u_star, const mhdp_opts *opts, floa
 /* Same with device pointers already resident in HBM (no host transfer). */
 int mhdp_project_dev(mhdp_ctx *ctx, const float *u_star_dev, const mhdp_opts *opts, float *u_out_dev,
                      double *phi_out_dev, mhdp_stats *stats);
-/* Many independent projections (uneven grids) on one stream, device pointers. */
+/* Many independent projections (uneven grids), device pointers.  No counterpart in the
+ * reference (one FluidState per grid, mod.rs:1015): each context keeps its own stream and
+ * a pool of host threads (MHDP_BATCH_THREADS, default 8) drives them concurrently, largest
+ * grids first.  Contexts must be distinct.  Returns the status of the first failing problem
+ * in index order (its text in that context's mhdp_last_error); stats[k] is filled for every
+ * problem that succeeded. */
 int mhdp_project_batch(mhdp_ctx *const *ctxs, int count, const float *const *u_star_dev, const mhdp_opts *opts,
                        float *const *u_out_dev, double *const *phi_out_dev, mhdp_stats *stats);
 const char *mhdp_last_error(const mhdp_ctx *ctx);

@@ -28,7 +28,7 @@ SOURCES = {
     "mg.hip": [],
     "fused.hip": [],
     "brick.hip": [],
-    "api.cpp": ["-x", "hip"],
+    "api.cpp": ["-x", "hip", "-pthread"],
 }
 
 
@@ -59,7 +59,7 @@ def build(verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=4) as ex:
         objs = list(ex.map(lambda kv: _compile(*kv), SOURCES.items()))
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")

@@ -4,11 +4,15 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "mhdp_kernels.h"
@@ -614,6 +618,13 @@ int project_impl(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, float* uou
     return MHDP_OK;
 }
 
+// Host threads of mhdp_project_batch (MHDP_BATCH_THREADS, 1..64; default 8), read per call.
+int batch_workers() {
+    const char* e = std::getenv("MHDP_BATCH_THREADS");
+    const int v = e ? std::atoi(e) : 0;
+    return (v >= 1 && v <= 64) ? v : 8;
+}
+
 }  // namespace
 
 // ============================================================================ C ABI
@@ -739,12 +750,43 @@ int mhdp_project(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, float* uou
 
 int mhdp_project_batch(mhdp_ctx* const* ctxs, int count, const float* const* ustar, const mhdp_opts* o,
                        float* const* uout, double* const* phi, mhdp_stats* st) {
-    if (!ctxs || count < 0 || !ustar) return MHDP_INVALID;
-    for (int k = 0; k < count; k++) {
-        int rc = mhdp_project_dev(ctxs[k], ustar[k], o, uout ? uout[k] : nullptr, phi ? phi[k] : nullptr,
-                                  st ? st + k : nullptr);
-        if (rc) return rc;
+    if (!ctxs || count < 0 || !ustar || !o) return MHDP_INVALID;
+    std::vector<const mhdp_ctx*> seen(ctxs, ctxs + count);
+    std::sort(seen.begin(), seen.end());
+    if (count && (!seen[0] || std::adjacent_find(seen.begin(), seen.end()) != seen.end())) return MHDP_INVALID;
+    // Each context runs on its own stream; a small pool of host threads drives them so the
+    // problems' kernels and host synchronisations overlap.  Largest grids start first.
+    std::vector<int> order((size_t)count);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ctxs[a]->g.n > ctxs[b]->g.n; });
+    std::vector<int> rcs((size_t)count, MHDP_OK);
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess) return MHDP_DEVICE;
+    std::atomic<int> next{0};
+    auto work = [&] {
+        if (hipSetDevice(device) != hipSuccess) return;
+        for (;;) {
+            const int j = next.fetch_add(1);
+            if (j >= count) break;
+            const int k = order[(size_t)j];
+            mhdp_ctx* c = ctxs[k];
+            c->err.clear();
+            rcs[(size_t)k] = project_impl(c, ustar[k], o, uout ? uout[k] : nullptr, phi ? phi[k] : nullptr,
+                                          st ? st + k : nullptr);
+        }
+    };
+    const int workers = std::min(count, batch_workers());
+    if (workers <= 1) {
+        work();
+    } else {
+        std::vector<std::thread> pool;
+        pool.reserve((size_t)workers);
+        for (int w = 0; w < workers; w++) pool.emplace_back(work);
+        for (auto& t : pool) t.join();
     }
+    if (next.load() < count) return MHDP_DEVICE;  // a worker could not select the device
+    for (int k = 0; k < count; k++)
+        if (rcs[(size_t)k]) return rcs[(size_t)k];
     return MHDP_OK;
 }
 

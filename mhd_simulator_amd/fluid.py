@@ -66,6 +66,38 @@ def hierarchy(dims4):
     return [tuple(int(v) for v in out[4 * l:4 * l + 4]) for l in range(n)], int(coarse[0])
 
 
+def project_batch(projections, u_stars, *, precision="mixed", precond="auto", gauge="auto", refine_tol=1e-8,
+                  max_outer=0, inner_tol=0.0, smooth_weight=0.0, coarse_scale=0.0, u_outs=None, phi_outs=None):
+    """Many independent projections in one call (mhdp_project_batch): device tensors only.
+    The options of the first projection's FluidConfig apply to all.  Returns a list of
+    (iterations, relative residual); raises PressureError for the first failing problem."""
+    n = len(projections)
+    if n == 0:
+        return []
+    if len(u_stars) != n or (u_outs is not None and len(u_outs) != n) or (phi_outs is not None and len(phi_outs) != n):
+        raise ValueError("one u_star (and output) per projection")
+    for pr, u in zip(projections, u_stars):
+        if not hasattr(u, "data_ptr") or u.numel() != pr.grid.faces:
+            raise ValueError("u_star must be a device tensor holding the three face planes")
+    lib = projections[0].lib
+    o = projections[0]._opts(precision, precond, gauge, refine_tol, max_outer, inner_tol, smooth_weight, coarse_scale)
+    P = C.c_void_p * n
+    ctxs = P(*[pr.handle for pr in projections])
+    us = P(*[u.data_ptr() for u in u_stars])
+    uo = P(*[t.data_ptr() for t in u_outs]) if u_outs is not None else None
+    po = P(*[t.data_ptr() for t in phi_outs]) if phi_outs is not None else None
+    st = (_lib.Stats * n)()
+    rc = lib.mhdp_project_batch(ctxs, n, us, C.byref(o), uo, po, st)
+    out = []
+    for k, pr in enumerate(projections):
+        pr.stats = st[k].as_dict()
+        out.append((int(st[k].iterations), float(st[k].rel_residual if precision == "fp32" else st[k].rel_residual_f64)))
+    if rc != _lib.OK:
+        bad = next((pr for pr in projections if lib.mhdp_last_error(pr.handle)), projections[0])
+        raise PressureError(rc, lib.mhdp_last_error(bad.handle).decode() or "batch projection failed")
+    return out
+
+
 def _ptr(a):
     if a is None:
         return None
