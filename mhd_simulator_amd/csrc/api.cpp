@@ -57,6 +57,7 @@ struct mhdp_ctx {
     std::vector<CoarseArrays> coarse;  // index l-1 for level l
     std::vector<Grid3> levels;         // level grids, [0] = fine
     bool mg_alloc = false, mg16_alloc = false;
+    bool fine16_fresh = false;  // BF16 copies of the fine coefficients match the FP32 ones
     // scalars
     Scal* sc = nullptr;
     Scal* sc_h = nullptr;
@@ -233,10 +234,13 @@ void build_mg(mhdp_ctx* c, int meanfree, bool f16) {
                         freact, a.bx, a.by, a.bz, a.react, a.diag, a.dinv);
     }
     if (f16) {
-        launch_to_bf16(c->stream, c->g.n, c->bx, c->bx16);
-        launch_to_bf16(c->stream, c->g.n, c->by, c->by16);
-        launch_to_bf16(c->stream, c->g.n, c->bz, c->bz16);
-        launch_to_bf16(c->stream, c->g.n, c->dinv, c->dinv16);
+        if (!c->fine16_fresh) {  // the coefficient pass did not write the BF16 copies
+            launch_to_bf16(c->stream, c->g.n, c->bx, c->bx16);
+            launch_to_bf16(c->stream, c->g.n, c->by, c->by16);
+            launch_to_bf16(c->stream, c->g.n, c->bz, c->bz16);
+            launch_to_bf16(c->stream, c->g.n, c->dinv, c->dinv16);
+            c->fine16_fresh = true;
+        }
         for (size_t l = 1; l < c->levels.size(); l++) {
             CoarseArrays& a = c->coarse[l - 1];
             const uint32_t n = c->levels[l].n;
@@ -478,10 +482,15 @@ PhaseP phase_params(const mhdp_ctx* c, bool meanfree) {
     return ph;
 }
 
-int prepare(mhdp_ctx* c, bool meanfree) {
+// Face mobilities, diagonal and 1/diag of this projection; `want16`: also their BF16 copies
+// for the mixed solver's V-cycle (when allocated).
+int prepare(mhdp_ctx* c, bool meanfree, bool want16 = false) {
     if (!c->cells) return fail(c, MHDP_INVALID, "Fluid cells have not been set");
-    launch_coefficients(c->stream, c->g, phase_params(c, meanfree), c->mask, c->has_alpha ? c->alpha : nullptr,
-                        c->bx, c->by, c->bz, c->diag, c->dinv);
+    const bool h = want16 && c->mg16_alloc;
+    c->fine16_fresh = launch_coefficients(c->stream, c->g, phase_params(c, meanfree), c->mask,
+                                          c->has_alpha ? c->alpha : nullptr, c->bx, c->by, c->bz, c->diag, c->dinv,
+                                          h ? c->bx16 : nullptr, h ? c->by16 : nullptr, h ? c->bz16 : nullptr,
+                                          h ? c->dinv16 : nullptr);
     return MHDP_OK;
 }
 
@@ -509,7 +518,7 @@ int project_impl(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, float* uou
 
     c->timer.used = 0;
     HIP_OK(hipEventRecord(c->ev0, c->stream));
-    if ((rc = prepare(c, meanfree))) return rc;
+    if ((rc = prepare(c, meanfree, mixed && preferred))) return rc;
     uint32_t iterations = 0, outer = 0;
     float residual = 0.0f;
     double res64 = 0.0;

@@ -22,8 +22,10 @@ struct LevelDev {
 };
 
 // setup.hip
-void launch_coefficients(hipStream_t s, const Grid3& g, const PhaseP& ph, const uint32_t* mask, const float* alpha,
-                         float* bx, float* by, float* bz, float* diag, float* dinv);
+// Returns true when the BF16 copies (all four pointers non-null) were written too.
+bool launch_coefficients(hipStream_t s, const Grid3& g, const PhaseP& ph, const uint32_t* mask, const float* alpha,
+                         float* bx, float* by, float* bz, float* diag, float* dinv, void* bx16 = nullptr,
+                         void* by16 = nullptr, void* bz16 = nullptr, void* dinv16 = nullptr);
 void launch_rhs_f32(hipStream_t s, const Grid3& g, float h, const uint32_t* mask, const float* bx, const float* by,
                     const float* bz, const float* ustar, float* b);
 void launch_rhs_f64(hipStream_t s, const Grid3& g, float h, const uint32_t* mask, const float* bx, const float* by,

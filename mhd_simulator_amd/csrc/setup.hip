@@ -254,6 +254,344 @@ __global__ void __launch_bounds__(MHDP_BLOCK) k_mg_build(Grid3 cg, Grid3 fg, int
     }
 }
 
+// ------------------------------------------------ row-vector forms (nx % 4 == 0)
+// One thread owns four consecutive cells of a row, so cell arrays move as 16-byte loads and
+// stores and every face mobility is evaluated once.  Each value is produced by the same
+// FP32 expression, in the same order, as in the one-cell kernels above (bit-identical).
+
+namespace {
+
+struct Row4 {
+    int x0, y, z;
+    uint32_t i0;
+    uint32_t ym, zm;        // start of the rows y-1 and z-1 (wrapped; the row itself if none)
+    uint32_t yp, zp;        // start of the rows y+1 and z+1 (wrapped; the row itself if none)
+    uint32_t xm, xp;        // cells x0-1 and x0+4 (wrapped; x0 / x0+3 if none)
+    bool ym_ok, zm_ok, yp_ok, zp_ok, xm_ok, xp_ok;
+};
+__device__ __forceinline__ Row4 row4(const Grid3& g, uint32_t t) {
+    Row4 r;
+    r.i0 = t << 2;
+    int x, y, z;
+    decode(g, r.i0, x, y, z);
+    r.x0 = x;
+    r.y = y;
+    r.z = z;
+    const uint32_t sy = (uint32_t)g.nx, sz = (uint32_t)g.nx * (uint32_t)g.ny;
+    r.ym_ok = y > 0 || (g.per & 2u);
+    r.ym = y > 0 ? r.i0 - sy : ((g.per & 2u) ? r.i0 + (uint32_t)(g.ny - 1) * sy : r.i0);
+    r.yp_ok = y + 1 < g.ny || (g.per & 2u);
+    r.yp = y + 1 < g.ny ? r.i0 + sy : ((g.per & 2u) ? r.i0 - (uint32_t)(g.ny - 1) * sy : r.i0);
+    r.zm_ok = z > 0 || (g.per & 4u);
+    r.zm = z > 0 ? r.i0 - sz : ((g.per & 4u) ? r.i0 + (uint32_t)(g.nz - 1) * sz : r.i0);
+    r.zp_ok = z + 1 < g.nz || (g.per & 4u);
+    r.zp = z + 1 < g.nz ? r.i0 + sz : ((g.per & 4u) ? r.i0 - (uint32_t)(g.nz - 1) * sz : r.i0);
+    r.xm_ok = x > 0 || (g.per & 1u);
+    r.xm = x > 0 ? r.i0 - 1 : ((g.per & 1u) ? r.i0 + (uint32_t)(g.nx - 1) : r.i0);
+    r.xp_ok = x + 4 < g.nx || (g.per & 1u);
+    r.xp = x + 4 < g.nx ? r.i0 + 4 : ((g.per & 1u) ? r.i0 + 4 - (uint32_t)g.nx : r.i0 + 3);
+    return r;
+}
+__device__ __forceinline__ void ld4f(const float* __restrict__ p, uint32_t i, float (&v)[4]) {
+    const float4 a = *reinterpret_cast<const float4*>(p + i);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+}
+__device__ __forceinline__ void ld4u(const uint32_t* __restrict__ p, uint32_t i, uint32_t (&v)[4]) {
+    const uint4 a = *reinterpret_cast<const uint4*>(p + i);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+}
+__device__ __forceinline__ void st4f(float* __restrict__ p, uint32_t i, const float (&v)[4]) {
+    *reinterpret_cast<float4*>(p + i) = make_float4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ void st4h(__hip_bfloat16* __restrict__ p, uint32_t i, const float (&v)[4]) {
+    uint32_t a = (uint32_t)__bfloat16_as_ushort(__float2bfloat16(v[0])) |
+                 ((uint32_t)__bfloat16_as_ushort(__float2bfloat16(v[1])) << 16);
+    uint32_t b = (uint32_t)__bfloat16_as_ushort(__float2bfloat16(v[2])) |
+                 ((uint32_t)__bfloat16_as_ushort(__float2bfloat16(v[3])) << 16);
+    *reinterpret_cast<uint2*>(p + i) = make_uint2(a, b);
+}
+// density() of a cell with liquid fraction a (solver.wgsl:40-41)
+__device__ __forceinline__ float dens(const PhaseP& ph, float a) { return ph.rho_air * (1.0f - a) + ph.rho_l * a; }
+// mobility of the face between cells with densities dl (lower) and dc (solver.wgsl:43-51)
+__device__ __forceinline__ float mob(const PhaseP& ph, float dl, float dc) {
+    if (!ph.two_phase) return 1.0f;
+    float fd = 0.5f * (dl + dc);
+    return ph.rho_l / fd;
+}
+
+}  // namespace
+
+// Pass 1 of the coefficient build: the three lower-face mobilities of each cell (optional
+// BF16 copies for the mixed solver's V-cycle).
+__global__ void __launch_bounds__(MHDP_BLOCK) k_beta4(Grid3 g, PhaseP ph, const uint32_t* __restrict__ mask,
+                                                      const float* __restrict__ alpha, float* __restrict__ bx,
+                                                      float* __restrict__ by, float* __restrict__ bz,
+                                                      __hip_bfloat16* __restrict__ bx16,
+                                                      __hip_bfloat16* __restrict__ by16,
+                                                      __hip_bfloat16* __restrict__ bz16) {
+    uint32_t lo, hi;
+    block_range(g.n >> 2, lo, hi);
+    for (uint32_t t = lo + threadIdx.x; t < hi; t += MHDP_BLOCK) {
+        const Row4 r = row4(g, t);
+        uint32_t m[4], my[4], mz[4];
+        ld4u(mask, r.i0, m);
+        ld4u(mask, r.ym, my);
+        ld4u(mask, r.zm, mz);
+        const uint32_t mx = mask[r.xm];
+        float a[4] = {0.f, 0.f, 0.f, 0.f}, ay[4] = {0.f, 0.f, 0.f, 0.f}, az[4] = {0.f, 0.f, 0.f, 0.f}, ax = 0.f;
+        if (alpha) {
+            ld4f(alpha, r.i0, a);
+            ld4f(alpha, r.ym, ay);
+            ld4f(alpha, r.zm, az);
+            ax = alpha[r.xm];
+        }
+        float ox[4], oy[4], oz[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const bool fc = m[k] != 0u;
+            const bool lx = k > 0 ? m[k - 1] != 0u : (r.xm_ok && mx != 0u);
+            const float dlx = dens(ph, k > 0 ? a[k - 1] : ax);
+            const float dc = dens(ph, a[k]);
+            ox[k] = (fc && lx) ? mob(ph, dlx, dc) : 0.0f;
+            oy[k] = (fc && r.ym_ok && my[k] != 0u) ? mob(ph, dens(ph, ay[k]), dc) : 0.0f;
+            oz[k] = (fc && r.zm_ok && mz[k] != 0u) ? mob(ph, dens(ph, az[k]), dc) : 0.0f;
+        }
+        st4f(bx, r.i0, ox);
+        st4f(by, r.i0, oy);
+        st4f(bz, r.i0, oz);
+        if (bx16) {
+            st4h(bx16, r.i0, ox);
+            st4h(by16, r.i0, oy);
+            st4h(bz16, r.i0, oz);
+        }
+    }
+}
+
+// Pass 2: diagonal() = sum of the six face mobilities in the order x-, x+, y-, y+, z-, z+
+// (a closed face contributes +0), the anchor row, and the active-cell reciprocal.
+__global__ void __launch_bounds__(MHDP_BLOCK) k_diag4(Grid3 g, PhaseP ph, const uint32_t* __restrict__ mask,
+                                                      const float* __restrict__ bx, const float* __restrict__ by,
+                                                      const float* __restrict__ bz, float* __restrict__ diag,
+                                                      float* __restrict__ dinv, __hip_bfloat16* __restrict__ dinv16) {
+    uint32_t lo, hi;
+    block_range(g.n >> 2, lo, hi);
+    for (uint32_t t = lo + threadIdx.x; t < hi; t += MHDP_BLOCK) {
+        const Row4 r = row4(g, t);
+        uint32_t m[4];
+        ld4u(mask, r.i0, m);
+        float xl[4], yl[4], zl[4], yu[4] = {0.f, 0.f, 0.f, 0.f}, zu[4] = {0.f, 0.f, 0.f, 0.f};
+        ld4f(bx, r.i0, xl);
+        ld4f(by, r.i0, yl);
+        ld4f(bz, r.i0, zl);
+        if (r.yp_ok) ld4f(by, r.yp, yu);
+        if (r.zp_ok) ld4f(bz, r.zp, zu);
+        const float xe = r.xp_ok ? bx[r.xp] : 0.0f;
+        float d[4], di[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            float v = 0.0f;
+            if (m[k] != 0u) {
+                v += xl[k];
+                v += k < 3 ? xl[k + 1] : xe;
+                v += yl[k];
+                v += yu[k];
+                v += zl[k];
+                v += zu[k];
+                if (!ph.meanfree && (m[k] & 3u) == 2u) v = 1.0f;
+                else v = fmaxf(v, 1e-30f);
+            }
+            d[k] = v;
+            const bool active = m[k] != 0u && (ph.meanfree || (m[k] & 3u) == 1u);
+            di[k] = active ? 1.0f / v : 0.0f;
+        }
+        st4f(diag, r.i0, d);
+        st4f(dinv, r.i0, di);
+        if (dinv16) st4h(dinv16, r.i0, di);
+    }
+}
+
+// rhs() (solver.wgsl:104-107) on four cells of a row: faces of the x plane are read as five
+// scalars (its rows hold nx+1 faces), the y and z planes as 16-byte rows.
+template <class T>
+__global__ void __launch_bounds__(MHDP_BLOCK) k_rhs4(Grid3 g, float h, const uint32_t* __restrict__ mask,
+                                                     const float* __restrict__ bx, const float* __restrict__ by,
+                                                     const float* __restrict__ bz, const float* __restrict__ ustar,
+                                                     uint64_t off1, uint64_t off2, T* __restrict__ b) {
+    uint32_t lo, hi;
+    block_range(g.n >> 2, lo, hi);
+    const float* __restrict__ U = ustar;
+    const float* __restrict__ V = ustar + off1;
+    const float* __restrict__ W = ustar + off2;
+    for (uint32_t t = lo + threadIdx.x; t < hi; t += MHDP_BLOCK) {
+        const Row4 r = row4(g, t);
+        uint32_t m[4];
+        ld4u(mask, r.i0, m);
+        T out[4] = {(T)0, (T)0, (T)0, (T)0};
+        if ((m[0] | m[1] | m[2] | m[3]) != 0u) {
+            float xl[4], yl[4], zl[4], yu[4] = {0.f, 0.f, 0.f, 0.f}, zu[4] = {0.f, 0.f, 0.f, 0.f};
+            ld4f(bx, r.i0, xl);
+            ld4f(by, r.i0, yl);
+            ld4f(bz, r.i0, zl);
+            if (r.yp_ok) ld4f(by, r.yp, yu);
+            if (r.zp_ok) ld4f(bz, r.zp, zu);
+            const float xe = r.xp_ok ? bx[r.xp] : 0.0f;
+            // x plane: faces x0..x0+4 of this row (the last wraps to face 0 when periodic)
+            const uint32_t ux = (uint32_t)(g.nx + 1) * ((uint32_t)r.y + (uint32_t)g.ny * (uint32_t)r.z);
+            float uf[5];
+#pragma unroll
+            for (int k = 0; k < 4; k++) uf[k] = U[ux + (uint32_t)r.x0 + (uint32_t)k];
+            const bool wrapx = r.x0 + 4 >= g.nx && (g.per & 1u);
+            uf[4] = U[ux + (wrapx ? 0u : (uint32_t)r.x0 + 4u)];
+            // y plane rows y and y+1 (y+1 wraps to 0 when periodic)
+            const uint32_t Py = (uint32_t)g.ny + 1u;
+            const uint32_t vy = (uint32_t)r.x0 + (uint32_t)g.nx * ((uint32_t)r.y + Py * (uint32_t)r.z);
+            const int yu_i = r.y + 1 < g.ny ? r.y + 1 : ((g.per & 2u) ? 0 : r.y + 1);
+            const uint32_t vyu = (uint32_t)r.x0 + (uint32_t)g.nx * ((uint32_t)yu_i + Py * (uint32_t)r.z);
+            float vl[4], vu[4];
+            ld4f(V, vy, vl);
+            ld4f(V, vyu, vu);
+            // z plane rows z and z+1
+            const uint32_t sz = (uint32_t)g.nx * (uint32_t)g.ny;
+            const int zu_i = r.z + 1 < g.nz ? r.z + 1 : ((g.per & 4u) ? 0 : r.z + 1);
+            const uint32_t wz = (uint32_t)r.x0 + (uint32_t)g.nx * (uint32_t)r.y;
+            float wl[4], wu[4];
+            ld4f(W, wz + sz * (uint32_t)r.z, wl);
+            ld4f(W, wz + sz * (uint32_t)zu_i, wu);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (m[k] == 0u) continue;
+                const float bxu = k < 3 ? xl[k + 1] : xe;
+                T sum = (T)0;
+                {
+                    const float up = bxu > 0.0f ? uf[k + 1] : 0.0f;
+                    const float dn = xl[k] > 0.0f ? uf[k] : 0.0f;
+                    sum += (T)up - (T)dn;
+                }
+                {
+                    const float up = yu[k] > 0.0f ? vu[k] : 0.0f;
+                    const float dn = yl[k] > 0.0f ? vl[k] : 0.0f;
+                    sum += (T)up - (T)dn;
+                }
+                {
+                    const float up = zu[k] > 0.0f ? wu[k] : 0.0f;
+                    const float dn = zl[k] > 0.0f ? wl[k] : 0.0f;
+                    sum += (T)up - (T)dn;
+                }
+                T div = sum / (T)h;
+                out[k] = -(T)h * (T)h * div;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) b[r.i0 + k] = out[k];
+    }
+}
+
+// correct() (solver.wgsl:245-250) on four cells of a row: each thread writes the lower faces
+// of its cells in all three planes, and the planes' last faces (x = nx, y = ny, z = nz) where
+// its cells border them (the periodic copy of face 0, or 0 when closed).
+template <class PT>
+__device__ __forceinline__ float corrected(float uf, float beta, PT pc, PT pl, float h) {
+    if (sizeof(PT) == 4) return uf - (float)beta * ((float)pc - (float)pl) / h;
+    double o = (double)uf - (double)beta * ((double)pc - (double)pl) / (double)h;
+    return (float)o;
+}
+__device__ __forceinline__ void ld4p(const float* __restrict__ p, uint32_t i, float (&v)[4]) { ld4f(p, i, v); }
+__device__ __forceinline__ void ld4p(const double* __restrict__ p, uint32_t i, double (&v)[4]) {
+    const double2 a = *reinterpret_cast<const double2*>(p + i), b = *reinterpret_cast<const double2*>(p + i + 2);
+    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+}
+template <class PT>
+__global__ void __launch_bounds__(MHDP_BLOCK) k_correct4(Grid3 g, float h, const float* __restrict__ bx,
+                                                         const float* __restrict__ by, const float* __restrict__ bz,
+                                                         const PT* __restrict__ phi, const float* __restrict__ ustar,
+                                                         float* __restrict__ uout, uint32_t f0, uint32_t f1) {
+    uint32_t lo, hi;
+    block_range(g.n >> 2, lo, hi);
+    const uint32_t sy = (uint32_t)g.nx, sz = (uint32_t)g.nx * (uint32_t)g.ny;
+    const float* __restrict__ U = ustar;
+    const float* __restrict__ V = ustar + f0;
+    const float* __restrict__ W = ustar + f0 + f1;
+    float* __restrict__ UO = uout;
+    float* __restrict__ VO = uout + f0;
+    float* __restrict__ WO = uout + f0 + f1;
+    for (uint32_t t = lo + threadIdx.x; t < hi; t += MHDP_BLOCK) {
+        const Row4 r = row4(g, t);
+        PT pc[4], py[4], pz[4];
+        ld4p(phi, r.i0, pc);
+        ld4p(phi, r.ym, py);
+        ld4p(phi, r.zm, pz);
+        const PT px = phi[r.xm];
+        float xl[4], yl[4], zl[4];
+        ld4f(bx, r.i0, xl);
+        ld4f(by, r.i0, yl);
+        ld4f(bz, r.i0, zl);
+        // x plane
+        const uint32_t ux = (uint32_t)(g.nx + 1) * ((uint32_t)r.y + (uint32_t)g.ny * (uint32_t)r.z) + (uint32_t)r.x0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            float o = 0.0f;
+            if (xl[k] > 0.0f) o = corrected<PT>(U[ux + k], xl[k], pc[k], k > 0 ? pc[k - 1] : px, h);
+            UO[ux + k] = o;
+        }
+        if (r.x0 + 4 == g.nx) {  // face x = nx of this row
+            float o = 0.0f;
+            if (g.per & 1u) {
+                const uint32_t c0 = r.i0 + 4u - (uint32_t)g.nx;  // cell x = 0 of the row
+                const float b0 = bx[c0];
+                if (b0 > 0.0f) o = corrected<PT>(U[ux + 4u - (uint32_t)g.nx], b0, phi[c0], pc[3], h);
+            }
+            UO[ux + 4] = o;
+        }
+        // y plane
+        const uint32_t Py = (uint32_t)g.ny + 1u;
+        const uint32_t vy = (uint32_t)r.x0 + sy * ((uint32_t)r.y + Py * (uint32_t)r.z);
+        {
+            float v[4], o[4];
+            ld4f(V, vy, v);
+#pragma unroll
+            for (int k = 0; k < 4; k++) o[k] = yl[k] > 0.0f ? corrected<PT>(v[k], yl[k], pc[k], py[k], h) : 0.0f;
+            st4f(VO, vy, o);
+        }
+        if (r.y + 1 == g.ny) {  // faces y = ny
+            float o[4] = {0.f, 0.f, 0.f, 0.f};
+            if (g.per & 2u) {
+                const uint32_t c0 = r.i0 - (uint32_t)(g.ny - 1) * sy;  // row y = 0
+                const uint32_t v0 = (uint32_t)r.x0 + sy * (Py * (uint32_t)r.z);
+                float b0[4], v[4];
+                ld4f(by, c0, b0);
+                ld4f(V, v0, v);
+                PT p0[4];
+                ld4p(phi, c0, p0);
+#pragma unroll
+                for (int k = 0; k < 4; k++) o[k] = b0[k] > 0.0f ? corrected<PT>(v[k], b0[k], p0[k], pc[k], h) : 0.0f;
+            }
+            st4f(VO, vy + sy, o);
+        }
+        // z plane
+        {
+            float w[4], o[4];
+            ld4f(W, r.i0, w);
+#pragma unroll
+            for (int k = 0; k < 4; k++) o[k] = zl[k] > 0.0f ? corrected<PT>(w[k], zl[k], pc[k], pz[k], h) : 0.0f;
+            st4f(WO, r.i0, o);
+        }
+        if (r.z + 1 == g.nz) {  // faces z = nz
+            float o[4] = {0.f, 0.f, 0.f, 0.f};
+            if (g.per & 4u) {
+                const uint32_t c0 = r.i0 - (uint32_t)(g.nz - 1) * sz;  // plane z = 0
+                float b0[4], w[4];
+                ld4f(bz, c0, b0);
+                ld4f(W, c0, w);
+                PT p0[4];
+                ld4p(phi, c0, p0);
+#pragma unroll
+                for (int k = 0; k < 4; k++) o[k] = b0[k] > 0.0f ? corrected<PT>(w[k], b0[k], p0[k], pc[k], h) : 0.0f;
+            }
+            st4f(WO, r.i0 + sz, o);
+        }
+    }
+}
+
 __global__ void __launch_bounds__(MHDP_BLOCK) k_to_bf16(uint32_t n, const float* __restrict__ src,
                                                         __hip_bfloat16* __restrict__ dst) {
     uint32_t lo, hi;
@@ -270,22 +608,43 @@ __global__ void __launch_bounds__(MHDP_BLOCK) k_from_bf16(uint32_t n, const __hi
 
 // ---------------------------------------------------------------- launchers
 
-void launch_coefficients(hipStream_t s, const Grid3& g, const PhaseP& ph, const uint32_t* mask, const float* alpha,
-                         float* bx, float* by, float* bz, float* diag, float* dinv) {
+static bool row_vector_ok(const Grid3& g) { return (g.nx & 3) == 0 && std::getenv("MHDP_SCALAR_SETUP") == nullptr; }
+
+bool launch_coefficients(hipStream_t s, const Grid3& g, const PhaseP& ph, const uint32_t* mask, const float* alpha,
+                         float* bx, float* by, float* bz, float* diag, float* dinv, void* bx16, void* by16,
+                         void* bz16, void* dinv16) {
+    if (row_vector_ok(g)) {
+        const dim3 G(grid_blocks(g.n >> 2)), B(MHDP_BLOCK);
+        const bool h = bx16 && by16 && bz16 && dinv16;
+        using T = __hip_bfloat16;
+        hipLaunchKernelGGL(k_beta4, G, B, 0, s, g, ph, mask, alpha, bx, by, bz, h ? (T*)bx16 : (T*)nullptr,
+                           h ? (T*)by16 : (T*)nullptr, h ? (T*)bz16 : (T*)nullptr);
+        hipLaunchKernelGGL(k_diag4, G, B, 0, s, g, ph, mask, bx, by, bz, diag, dinv, h ? (T*)dinv16 : (T*)nullptr);
+        return h;
+    }
     hipLaunchKernelGGL(k_coefficients, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g, ph, mask, alpha, bx, by, bz,
                        diag, dinv);
+    return false;
 }
 void launch_rhs_f32(hipStream_t s, const Grid3& g, float h, const uint32_t* mask, const float* bx, const float* by,
                     const float* bz, const float* ustar, float* b) {
     uint64_t f0 = (uint64_t)(g.nx + 1) * g.ny * g.nz, f1 = (uint64_t)g.nx * (g.ny + 1) * g.nz;
-    hipLaunchKernelGGL(k_rhs<float>, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g, h, mask, bx, by, bz, ustar,
-                       f0, f0 + f1, b);
+    if (row_vector_ok(g))
+        hipLaunchKernelGGL(k_rhs4<float>, dim3(grid_blocks(g.n >> 2)), dim3(MHDP_BLOCK), 0, s, g, h, mask, bx, by, bz,
+                           ustar, f0, f0 + f1, b);
+    else
+        hipLaunchKernelGGL(k_rhs<float>, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g, h, mask, bx, by, bz, ustar,
+                           f0, f0 + f1, b);
 }
 void launch_rhs_f64(hipStream_t s, const Grid3& g, float h, const uint32_t* mask, const float* bx, const float* by,
                     const float* bz, const float* ustar, double* b) {
     uint64_t f0 = (uint64_t)(g.nx + 1) * g.ny * g.nz, f1 = (uint64_t)g.nx * (g.ny + 1) * g.nz;
-    hipLaunchKernelGGL(k_rhs<double>, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g, h, mask, bx, by, bz, ustar,
-                       f0, f0 + f1, b);
+    if (row_vector_ok(g))
+        hipLaunchKernelGGL(k_rhs4<double>, dim3(grid_blocks(g.n >> 2)), dim3(MHDP_BLOCK), 0, s, g, h, mask, bx, by, bz,
+                           ustar, f0, f0 + f1, b);
+    else
+        hipLaunchKernelGGL(k_rhs<double>, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g, h, mask, bx, by, bz, ustar,
+                           f0, f0 + f1, b);
 }
 static void face_counts(const Grid3& g, uint32_t& f0, uint32_t& f1, uint32_t& f2) {
     f0 = (uint32_t)((g.nx + 1) * (uint64_t)g.ny * g.nz);
@@ -296,15 +655,23 @@ void launch_correct_f32(hipStream_t s, const Grid3& g, float h, const float* bx,
                         const float* phi, const float* ustar, float* uout) {
     uint32_t f0, f1, f2;
     face_counts(g, f0, f1, f2);
-    hipLaunchKernelGGL(k_correct<float>, dim3(grid_blocks(f0 + f1 + f2)), dim3(MHDP_BLOCK), 0, s, g, h, bx, by, bz,
-                       phi, ustar, uout, f0, f1, f2);
+    if (row_vector_ok(g))
+        hipLaunchKernelGGL(k_correct4<float>, dim3(grid_blocks(g.n >> 2)), dim3(MHDP_BLOCK), 0, s, g, h, bx, by, bz, phi,
+                           ustar, uout, f0, f1);
+    else
+        hipLaunchKernelGGL(k_correct<float>, dim3(grid_blocks(f0 + f1 + f2)), dim3(MHDP_BLOCK), 0, s, g, h, bx, by, bz,
+                           phi, ustar, uout, f0, f1, f2);
 }
 void launch_correct_f64(hipStream_t s, const Grid3& g, float h, const float* bx, const float* by, const float* bz,
                         const double* phi, const float* ustar, float* uout) {
     uint32_t f0, f1, f2;
     face_counts(g, f0, f1, f2);
-    hipLaunchKernelGGL(k_correct<double>, dim3(grid_blocks(f0 + f1 + f2)), dim3(MHDP_BLOCK), 0, s, g, h, bx, by, bz,
-                       phi, ustar, uout, f0, f1, f2);
+    if (row_vector_ok(g))
+        hipLaunchKernelGGL(k_correct4<double>, dim3(grid_blocks(g.n >> 2)), dim3(MHDP_BLOCK), 0, s, g, h, bx, by, bz, phi,
+                           ustar, uout, f0, f1);
+    else
+        hipLaunchKernelGGL(k_correct<double>, dim3(grid_blocks(f0 + f1 + f2)), dim3(MHDP_BLOCK), 0, s, g, h, bx, by, bz,
+                           phi, ustar, uout, f0, f1, f2);
 }
 void launch_mg_build(hipStream_t s, const Grid3& cg, const Grid3& fg, bool fine, int meanfree, const uint32_t* mask,
                      const float* fdinv, const float* fbx, const float* fby, const float* fbz, const float* freact,

@@ -36,6 +36,8 @@ def fixture_problem(dims, periodic, walls, reverse, seed=11):
 CASES = {
     "periodic-odd": lambda: fixture_problem((17, 13, 9), (True,) * 3, False, False),
     "walls-pinned": lambda: fixture_problem((16, 12, 8), (False,) * 3, True, True),
+    "periodic-even": lambda: fixture_problem((16, 12, 8), (True,) * 3, False, False),
+    "thin-periodic-row": lambda: fixture_problem((16, 1, 4), (True,) * 3, True, False),
     "thin-periodic": lambda: fixture_problem((13, 1, 3), (True,) * 3, True, False),
     "config1-32": lambda: C.config1(32),
     "config2-32": lambda: C.config2(32),
@@ -46,7 +48,7 @@ CASES = {
 # FP32 tolerance per case: the reference's FP32 PCG stagnates near 2e-5 .. 7e-5 on the
 # 7000:1 random-RHS fixtures (its documented FP32 floor, docs/PRESSURE_PRECONDITIONING.md:71;
 # the oracle's FP32 restatement reproduces it), so those run at 1e-4.
-FP32_TOL = {"periodic-odd": 1e-4, "thin-periodic": 1e-4}
+FP32_TOL = {"periodic-odd": 1e-4, "thin-periodic": 1e-4, "periodic-even": 1e-4, "thin-periodic-row": 1e-4}
 _cache = {}
 
 
