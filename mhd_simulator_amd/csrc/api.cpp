@@ -449,11 +449,57 @@ int pcg(mhdp_ctx* c, Solve& S, bool want_mg, uint32_t max_iter, bool reference_c
     return conv ? MHDP_OK : MHDP_NOT_CONVERGED;
 }
 
+
 int upload_scal(mhdp_ctx* c, const Scal& s) {
     *c->sc_h = s;
     HIP_OK(hipMemcpyAsync(c->sc, c->sc_h, sizeof(Scal), hipMemcpyHostToDevice, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
     return MHDP_OK;
+}
+
+// FP64 iterative refinement over the fused multigrid path.  Each outer step is one residual
+// pass that also starts the inner solve on the device (inner tolerance, deferred mean-free
+// projection, zeroed x and search directions), the inner V-cycle PCG in batches of 8, and one
+// pass folding the inner solution (with its owed last update) into x64.  The host reads the
+// scalars once per batch; the first read of a step carries the FP64 residual.
+int refine_fused(mhdp_ctx* c, Solve& S, const mhdp_opts* o, uint32_t max_outer, uint32_t* iterations,
+                 uint32_t* outer, double* res64) {
+    hipStream_t s = c->stream;
+    *iterations = 0;
+    *outer = 0;
+    for (bool first = true;; first = false) {
+        launch_outer_start(s, c->g, c->x64, c->b64, c->bx, c->by, c->bz, c->dinv, c->r, c->x, c->p, c->p2, c->part,
+                           c->sc, first);
+        mg_initialize(c, S);
+        uint32_t it = 0;
+        for (bool checked = false;;) {
+            const uint32_t batch = std::min(8u, o->max_iter - it);
+            for (uint32_t k = 0; k < batch; k++) iterate_once(c, S);
+            int rc = read_scalars(c);
+            if (rc) return rc;
+            const Scal& v = *c->sc_h;
+            const double vals[10] = {v.rr64, v.bb64, v.rz, v.rr, v.bb, v.sr, v.alpha, v.beta, v.conv, v.iters};
+            for (double x : vals)
+                if (!std::isfinite(x)) return fail(c, MHDP_NONFINITE, "Nonfinite fluid pressure residual");
+            if (!checked) {
+                *res64 = v.bb64 > 0.0 ? std::sqrt(v.rr64 / v.bb64) : 0.0;
+                if (v.outer_done) {
+                    if (first) launch_fill_f64(s, c->g.n, c->x64, 0.0);
+                    return MHDP_OK;
+                }
+                if (*outer >= max_outer) return MHDP_NOT_CONVERGED;
+                checked = true;
+            }
+            const bool conv = v.conv > 0.5;
+            const uint32_t seen = (uint32_t)v.iters;
+            if (seen <= it && !conv) return fail(c, MHDP_NO_PROGRESS, "Fluid pressure iteration made no progress");
+            it = seen;
+            if (conv || it >= o->max_iter) break;
+        }
+        *iterations += it;
+        launch_outer_update(s, c->g, c->x64, c->x, c->p, c->p2, c->dinv, c->sc, S.meanfree, first);
+        (*outer)++;
+    }
 }
 
 int ensure_fine(mhdp_ctx* c) {
@@ -549,12 +595,25 @@ int project_impl(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, float* uou
         const double tau = o->inner_tol > 0.0f ? (double)o->inner_tol : 1e-4;
         const uint32_t max_outer = o->max_outer ? o->max_outer : 16u;
         launch_rhs_f64(c->stream, c->g, c->grid.h, c->mask, c->bx, c->by, c->bz, ustar, c->b64);
-        launch_fill_f64(c->stream, c->g.n, c->x64, 0.0);
         Scal s;
         set_solve_params(c, s, meanfree, tau * tau, 0.0, 1.0, omega, pscale);
+        s.refine_tol = o->refine_tol;
+        s.tau = tau;
         if ((rc = upload_scal(c, s))) return rc;
         bool done = false;
-        for (;;) {
+        const bool fast = preferred && c->fused_enabled && c->levels.size() >= 2;
+        if (fast) {
+            build_mg(c, meanfree ? 1 : 0, true);
+            S.mg = true;
+            S.lv = level_view(c);
+            S.fused = true;
+            rc = refine_fused(c, S, o, max_outer, &iterations, &outer, &res64);
+            if (rc && rc != MHDP_NOT_CONVERGED) return rc;
+            done = rc == MHDP_OK;
+        } else {
+            launch_fill_f64(c->stream, c->g.n, c->x64, 0.0);
+        }
+        for (; !fast;) {
             launch_residual64(c->stream, c->g, c->x64, c->b64, c->bx, c->by, c->bz, c->dinv, c->b, c->part, c->sc);
             if ((rc = read_scalars(c))) return rc;
             const double rr = c->sc_h->rr, bb = c->sc_h->spare;

@@ -108,30 +108,122 @@ struct Field {
     float zm[2][4], zp[2][4];  // [dy][k]
     float xm[2][2], xp[2][2];  // [dz][dy]
 };
-// rowf(start, y, z) -> F4 of the row starting at `start` (coordinates y, z of that row);
-// ptf(index, x, y, z) -> value at one cell.
-template <class RowF, class PtF>
-__device__ __forceinline__ void gather(const Geo& o, Field& f, RowF&& rowf, PtF&& ptf) {
+// ---- loads issued up front.  Every value a brick needs is loaded unconditionally (all
+// indices are in range by construction) into raw registers, then each raw register passes an
+// empty asm statement that claims to modify it: the compiler can then neither sink a load
+// into a data-dependent branch nor wait for the loads one group at a time, so one brick costs
+// one memory round trip.
+template <class T> struct R4;
+template <> struct R4<float> { float4 v; };
+template <> struct R4<__hip_bfloat16> { uint2 v; };
+template <class T> struct R1;
+template <> struct R1<float> { float v; };
+template <> struct R1<__hip_bfloat16> { uint32_t v; };
+template <class T> struct R2;
+template <> struct R2<float> { float2 v; };
+template <> struct R2<__hip_bfloat16> { uint32_t v; };
+
+__device__ __forceinline__ R4<float> ldr4(const float* __restrict__ p, uint32_t i) {
+    return R4<float>{*reinterpret_cast<const float4*>(p + i)};
+}
+__device__ __forceinline__ R4<__hip_bfloat16> ldr4(const __hip_bfloat16* __restrict__ p, uint32_t i) {
+    return R4<__hip_bfloat16>{*reinterpret_cast<const uint2*>(p + i)};
+}
+__device__ __forceinline__ R1<float> ldr1(const float* __restrict__ p, uint32_t i) { return R1<float>{p[i]}; }
+__device__ __forceinline__ R1<__hip_bfloat16> ldr1(const __hip_bfloat16* __restrict__ p, uint32_t i) {
+    return R1<__hip_bfloat16>{(uint32_t)*reinterpret_cast<const unsigned short*>(p + i)};
+}
+__device__ __forceinline__ R2<float> ldr2(const float* __restrict__ p, uint32_t i) {
+    return R2<float>{*reinterpret_cast<const float2*>(p + i)};
+}
+__device__ __forceinline__ R2<__hip_bfloat16> ldr2(const __hip_bfloat16* __restrict__ p, uint32_t i) {
+    return R2<__hip_bfloat16>{*reinterpret_cast<const uint32_t*>(p + i)};
+}
+__device__ __forceinline__ void fence(R4<float>& r) {
+    asm volatile("" : "+v"(r.v.x), "+v"(r.v.y), "+v"(r.v.z), "+v"(r.v.w));
+}
+__device__ __forceinline__ void fence(R4<__hip_bfloat16>& r) { asm volatile("" : "+v"(r.v.x), "+v"(r.v.y)); }
+__device__ __forceinline__ void fence(R1<float>& r) { asm volatile("" : "+v"(r.v)); }
+__device__ __forceinline__ void fence(R1<__hip_bfloat16>& r) { asm volatile("" : "+v"(r.v)); }
+__device__ __forceinline__ void fence(R2<float>& r) { asm volatile("" : "+v"(r.v.x), "+v"(r.v.y)); }
+__device__ __forceinline__ void fence(R2<__hip_bfloat16>& r) { asm volatile("" : "+v"(r.v)); }
+__device__ __forceinline__ F4 un(const R4<float>& r) { return F4{{r.v.x, r.v.y, r.v.z, r.v.w}}; }
+__device__ __forceinline__ F4 un(const R4<__hip_bfloat16>& r) {
+    return F4{{bfl(r.v.x), bfh(r.v.x), bfl(r.v.y), bfh(r.v.y)}};
+}
+__device__ __forceinline__ float un(const R1<float>& r) { return r.v; }
+__device__ __forceinline__ float un(const R1<__hip_bfloat16>& r) { return bfl(r.v); }
+__device__ __forceinline__ void un(const R2<float>& r, float& a, float& b) { a = r.v.x; b = r.v.y; }
+__device__ __forceinline__ void un(const R2<__hip_bfloat16>& r, float& a, float& b) { a = bfl(r.v); b = bfh(r.v); }
+
+// One array on the brick and its face halo: 4 core rows, 8 halo rows, 8 halo cells.
+template <class T>
+struct Rows {
+    R4<T> c[2][2];            // [dz][dy]
+    R4<T> ym[2], yp[2];       // [dz]
+    R4<T> zm[2], zp[2];       // [dy]
+    R1<T> xm[2][2], xp[2][2]; // [dz][dy]
+};
+template <class T>
+__device__ __forceinline__ void load_rows(const Geo& o, const T* __restrict__ p, Rows<T>& r) {
 #pragma unroll
     for (int dz = 0; dz < 2; dz++)
 #pragma unroll
         for (int dy = 0; dy < 2; dy++) {
-            const F4 a = rowf(o.row[dz][dy], o.y0 + dy, o.z0 + dz);
-#pragma unroll
-            for (int k = 0; k < 4; k++) f.c[dz][dy][k] = a.v[k];
-            f.xm[dz][dy] = ptf(o.row[dz][dy] + (uint32_t)o.xm_off, o.xmx, o.y0 + dy, o.z0 + dz);
-            f.xp[dz][dy] = ptf(o.row[dz][dy] + (uint32_t)o.xp_off, o.xpx, o.y0 + dy, o.z0 + dz);
+            r.c[dz][dy] = ldr4(p, o.row[dz][dy]);
+            r.xm[dz][dy] = ldr1(p, o.row[dz][dy] + (uint32_t)o.xm_off);
+            r.xp[dz][dy] = ldr1(p, o.row[dz][dy] + (uint32_t)o.xp_off);
         }
 #pragma unroll
     for (int d = 0; d < 2; d++) {
-        const F4 a = rowf(o.ymr[d], o.ymy, o.z0 + d), b = rowf(o.ypr[d], o.ypy, o.z0 + d);
-        const F4 c = rowf(o.zmr[d], o.y0 + d, o.zmz), e = rowf(o.zpr[d], o.y0 + d, o.zpz);
+        r.ym[d] = ldr4(p, o.ymr[d]);
+        r.yp[d] = ldr4(p, o.ypr[d]);
+        r.zm[d] = ldr4(p, o.zmr[d]);
+        r.zp[d] = ldr4(p, o.zpr[d]);
+    }
+}
+template <class T>
+__device__ __forceinline__ void fence_rows(Rows<T>& r) {
+#pragma unroll
+    for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+        for (int dy = 0; dy < 2; dy++) {
+            fence(r.c[dz][dy]);
+            fence(r.xm[dz][dy]);
+            fence(r.xp[dz][dy]);
+        }
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
+        fence(r.ym[d]);
+        fence(r.yp[d]);
+        fence(r.zm[d]);
+        fence(r.zp[d]);
+    }
+}
+// Field from two arrays A, B: rows through fr(a, b, row-id) -> F4, cells through fp(a, b, id).
+// Row ids: 0..3 core (dz*2+dy), 4+d ym, 6+d yp, 8+d zm, 10+d zp; cell ids 0..3 xm, 4..7 xp.
+template <class TA, class TB, class FR, class FP>
+__device__ __forceinline__ void field2(const Rows<TA>& a, const Rows<TB>& b, Field& f, FR&& fr, FP&& fp) {
+#pragma unroll
+    for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+        for (int dy = 0; dy < 2; dy++) {
+            const F4 v = fr(un(a.c[dz][dy]), un(b.c[dz][dy]), dz * 2 + dy);
+#pragma unroll
+            for (int k = 0; k < 4; k++) f.c[dz][dy][k] = v.v[k];
+            f.xm[dz][dy] = fp(un(a.xm[dz][dy]), un(b.xm[dz][dy]), dz * 2 + dy);
+            f.xp[dz][dy] = fp(un(a.xp[dz][dy]), un(b.xp[dz][dy]), 4 + dz * 2 + dy);
+        }
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
+        const F4 v0 = fr(un(a.ym[d]), un(b.ym[d]), 4 + d), v1 = fr(un(a.yp[d]), un(b.yp[d]), 6 + d);
+        const F4 v2 = fr(un(a.zm[d]), un(b.zm[d]), 8 + d), v3 = fr(un(a.zp[d]), un(b.zp[d]), 10 + d);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            f.ym[d][k] = a.v[k];
-            f.yp[d][k] = b.v[k];
-            f.zm[d][k] = c.v[k];
-            f.zp[d][k] = e.v[k];
+            f.ym[d][k] = v0.v[k];
+            f.yp[d][k] = v1.v[k];
+            f.zm[d][k] = v2.v[k];
+            f.zp[d][k] = v3.v[k];
         }
     }
 }
@@ -144,31 +236,70 @@ struct Wts {
     float bzp[2][4];       // [dy] upper z faces of plane dz=1
 };
 template <class CT>
-__device__ __forceinline__ void weights(const Geo& o, Wts& w, const CT* __restrict__ bx, const CT* __restrict__ by,
-                                        const CT* __restrict__ bz) {
+struct WRaw {
+    R4<CT> bx[2][2], by[2][2], bz[2][2];
+    R1<CT> bxp[2][2];
+    R4<CT> byp[2], bzp[2];
+};
+template <class CT>
+__device__ __forceinline__ void load_weights(const Geo& o, WRaw<CT>& w, const CT* __restrict__ bx,
+                                             const CT* __restrict__ by, const CT* __restrict__ bz) {
 #pragma unroll
     for (int dz = 0; dz < 2; dz++)
 #pragma unroll
         for (int dy = 0; dy < 2; dy++) {
             const uint32_t r = o.row[dz][dy];
-            const F4 a = ld4(bx, r), b = ld4(by, r), c = ld4(bz, r);
+            w.bx[dz][dy] = ldr4(bx, r);
+            w.by[dz][dy] = ldr4(by, r);
+            w.bz[dz][dy] = ldr4(bz, r);
+            w.bxp[dz][dy] = ldr1(bx, r + (uint32_t)o.xp_off);
+        }
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
+        w.byp[d] = ldr4(by, o.ypr[d]);
+        w.bzp[d] = ldr4(bz, o.zpr[d]);
+    }
+}
+template <class CT>
+__device__ __forceinline__ void fence_weights(WRaw<CT>& w) {
+#pragma unroll
+    for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+        for (int dy = 0; dy < 2; dy++) {
+            fence(w.bx[dz][dy]);
+            fence(w.by[dz][dy]);
+            fence(w.bz[dz][dy]);
+            fence(w.bxp[dz][dy]);
+        }
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
+        fence(w.byp[d]);
+        fence(w.bzp[d]);
+    }
+}
+// Closed high-side faces (non-periodic boundary) get weight 0.
+template <class CT>
+__device__ __forceinline__ void weights(const Geo& o, const WRaw<CT>& raw, Wts& w) {
+#pragma unroll
+    for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+        for (int dy = 0; dy < 2; dy++) {
+            const F4 a = un(raw.bx[dz][dy]), b = un(raw.by[dz][dy]), c = un(raw.bz[dz][dy]);
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 w.bx[dz][dy][k] = a.v[k];
                 w.by[dz][dy][k] = b.v[k];
                 w.bz[dz][dy][k] = c.v[k];
             }
-            w.bxp[dz][dy] = o.xp_ok ? ldf(bx[r + (uint32_t)o.xp_off]) : 0.0f;
+            w.bxp[dz][dy] = o.xp_ok ? un(raw.bxp[dz][dy]) : 0.0f;
         }
 #pragma unroll
     for (int d = 0; d < 2; d++) {
-        F4 a{{0.f, 0.f, 0.f, 0.f}}, b{{0.f, 0.f, 0.f, 0.f}};
-        if (o.yp_ok) a = ld4(by, o.ypr[d]);
-        if (o.zp_ok) b = ld4(bz, o.zpr[d]);
+        const F4 a = un(raw.byp[d]), b = un(raw.bzp[d]);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            w.byp[d][k] = a.v[k];
-            w.bzp[d][k] = b.v[k];
+            w.byp[d][k] = o.yp_ok ? a.v[k] : 0.0f;
+            w.bzp[d][k] = o.zp_ok ? b.v[k] : 0.0f;
         }
     }
 }
@@ -227,22 +358,45 @@ __global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_apply(Grid3 g, const
     double acc[1] = {0.0};
     for (uint32_t b = lo + threadIdx.x; b < hi; b += MHDP_BLOCK) {
         const Geo o = geo(g, b);
+        Rows<VT> zr;
+        Rows<float> pr;
+        WRaw<float> wr;
+        R4<float> dr[2][2], xr[2][2];
+        load_rows(o, z, zr);
+        load_rows(o, pold, pr);
+        load_weights(o, wr, bx, by, bz);
+#pragma unroll
+        for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+            for (int dy = 0; dy < 2; dy++) {
+                dr[dz][dy] = ldr4(dinv, o.row[dz][dy]);
+                xr[dz][dy] = ldr4(x, o.row[dz][dy]);
+            }
+        fence_rows(zr);
+        fence_rows(pr);
+        fence_weights(wr);
+#pragma unroll
+        for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+            for (int dy = 0; dy < 2; dy++) {
+                fence(dr[dz][dy]);
+                fence(xr[dz][dy]);
+            }
         Wts w;
-        weights(o, w, bx, by, bz);
+        weights(o, wr, w);
         Field f;
-        gather(
-            o, f,
-            [&](uint32_t j, int, int) {
-                const F4 a = ld4(z, j), c = ld4(pold, j);
+        field2(
+            zr, pr, f,
+            [&](const F4& a, const F4& c, int) {
                 return F4{{a.v[0] + beta * c.v[0], a.v[1] + beta * c.v[1], a.v[2] + beta * c.v[2], a.v[3] + beta * c.v[3]}};
             },
-            [&](uint32_t j, int, int, int) { return ldf(z[j]) + beta * pold[j]; });
+            [&](float a, float c, int) { return a + beta * c; });
 #pragma unroll
         for (int dz = 0; dz < 2; dz++)
 #pragma unroll
             for (int dy = 0; dy < 2; dy++) {
                 const uint32_t r = o.row[dz][dy];
-                const F4 di = ld4(dinv, r);
+                const F4 di = un(dr[dz][dy]);
                 F4 qv, pn;
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
@@ -253,8 +407,8 @@ __global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_apply(Grid3 g, const
                 st4(pnew, r, pn);
                 st4(q, r, qv);
                 if (pending) {
-                    const F4 po = ld4(pold, r);
-                    F4 xv = ld4(x, r);
+                    const F4 po = un(pr.c[dz][dy]);
+                    F4 xv = un(xr[dz][dy]);
 #pragma unroll
                     for (int k = 0; k < 4; k++)
                         if (di.v[k] != 0.0f) xv.v[k] = xv.v[k] + alpha * (po.v[k] - pa);
@@ -282,34 +436,54 @@ __global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_restrict(Grid3 cg, G
     block_range(g.n >> 4, lo, hi);
     for (uint32_t b = lo + threadIdx.x; b < hi; b += MHDP_BLOCK) {
         const Geo o = geo(g, b);
+        Rows<RT> rr;
+        Rows<CT> dd;
+        WRaw<CT> wr;
+        R4<CT> er[2][2];
+        load_rows(o, r, rr);
+        load_rows(o, dinv, dd);
+        load_weights(o, wr, bx, by, bz);
+        if (COARSE) {
+#pragma unroll
+            for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+                for (int dy = 0; dy < 2; dy++) er[dz][dy] = ldr4(react, o.row[dz][dy]);
+        }
+        fence_rows(rr);
+        fence_rows(dd);
+        fence_weights(wr);
+        if (COARSE) {
+#pragma unroll
+            for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+                for (int dy = 0; dy < 2; dy++) fence(er[dz][dy]);
+        }
         Wts w;
-        weights(o, w, bx, by, bz);
+        weights(o, wr, w);
         Field f;
-        gather(
-            o, f,
-            [&](uint32_t j, int, int) {
-                const F4 a = ld4(r, j), d = ld4(dinv, j);
+        field2(
+            rr, dd, f,
+            [&](const F4& a, const F4& d, int) {
                 F4 o4;
 #pragma unroll
                 for (int k = 0; k < 4; k++) o4.v[k] = omega * (a.v[k] - m) * d.v[k];
                 return o4;
             },
-            [&](uint32_t j, int, int, int) { return omega * (ldf(r[j]) - m) * ldf(dinv[j]); });
+            [&](float a, float d, int) { return omega * (a - m) * d; });
         float s0 = 0.0f, s1 = 0.0f;
 #pragma unroll
         for (int dz = 0; dz < 2; dz++)
 #pragma unroll
             for (int dy = 0; dy < 2; dy++) {
-                const uint32_t row = o.row[dz][dy];
-                const F4 rv = ld4(r, row), di = ld4(dinv, row);
+                const F4 rv = un(rr.c[dz][dy]), di = un(dd.c[dz][dy]);
                 F4 re{{0.f, 0.f, 0.f, 0.f}};
-                if (COARSE) re = ld4(react, row);
+                if (COARSE) re = un(er[dz][dy]);
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
-                    if (!COARSE && di.v[k] == 0.0f) continue;
                     float a = act_at(w, f, dz, dy, k);
                     if (COARSE) a = re.v[k] * f.c[dz][dy][k] + a;
-                    const float v = (rv.v[k] - m) - a;
+                    float v = (rv.v[k] - m) - a;
+                    if (!COARSE) v = di.v[k] != 0.0f ? v : 0.0f;
                     if (k < 2) s0 += v;
                     else s1 += v;
                 }
@@ -339,48 +513,87 @@ __global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_smooth(Grid3 g, Grid
     double acc[1] = {0.0};
     for (uint32_t b = lo + threadIdx.x; b < hi; b += MHDP_BLOCK) {
         const Geo o = geo(g, b);
+        Rows<RT> rr;
+        Rows<CT> dd;
+        WRaw<CT> wr;
+        R4<CT> er[2][2];
+        load_rows(o, r, rr);
+        load_rows(o, dinv, dd);
+        load_weights(o, wr, bx, by, bz);
+        if (COARSE) {
+#pragma unroll
+            for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+                for (int dy = 0; dy < 2; dy++) er[dz][dy] = ldr4(react, o.row[dz][dy]);
+        }
+        // coarse corrections: one aggregate pair per row group (the brick's rows share parents)
+        const uint32_t cx = (uint32_t)(o.x0 >> 1), cy = (uint32_t)(o.y0 >> 1), cz = (uint32_t)(o.z0 >> 1);
+        R2<VT> zcore = ldr2(zc, cx + csy * cy + csz * cz);
+        R2<VT> zym = ldr2(zc, cx + csy * (uint32_t)(o.ymy >> 1) + csz * cz);
+        R2<VT> zyp = ldr2(zc, cx + csy * (uint32_t)(o.ypy >> 1) + csz * cz);
+        R2<VT> zzm = ldr2(zc, cx + csy * cy + csz * (uint32_t)(o.zmz >> 1));
+        R2<VT> zzp = ldr2(zc, cx + csy * cy + csz * (uint32_t)(o.zpz >> 1));
+        R1<VT> zxm = ldr1(zc, (uint32_t)(o.xmx >> 1) + csy * cy + csz * cz);
+        R1<VT> zxp = ldr1(zc, (uint32_t)(o.xpx >> 1) + csy * cy + csz * cz);
+        fence_rows(rr);
+        fence_rows(dd);
+        fence_weights(wr);
+        if (COARSE) {
+#pragma unroll
+            for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+                for (int dy = 0; dy < 2; dy++) fence(er[dz][dy]);
+        }
+        fence(zcore);
+        fence(zym);
+        fence(zyp);
+        fence(zzm);
+        fence(zzp);
+        fence(zxm);
+        fence(zxp);
+        float cc[5][2];
+        un(zcore, cc[0][0], cc[0][1]);
+        un(zym, cc[1][0], cc[1][1]);
+        un(zyp, cc[2][0], cc[2][1]);
+        un(zzm, cc[3][0], cc[3][1]);
+        un(zzp, cc[4][0], cc[4][1]);
+        const float cxm = un(zxm), cxp = un(zxp);
         Wts w;
-        weights(o, w, bx, by, bz);
-        const uint32_t cx = (uint32_t)(o.x0 >> 1);
+        weights(o, wr, w);
         Field f;
-        gather(
-            o, f,
-            [&](uint32_t j, int yq, int zq) {
-                const F4 a = ld4(r, j), d = ld4(dinv, j);
-                float c0, c1;
-                ld2(zc, cx + csy * (uint32_t)(yq >> 1) + csz * (uint32_t)(zq >> 1), c0, c1);
+        field2(
+            rr, dd, f,
+            [&](const F4& a, const F4& d, int id) {
+                const int grp = id < 4 ? 0 : (id < 6 ? 1 : (id < 8 ? 2 : (id < 10 ? 3 : 4)));
+                const float c0 = cc[grp][0], c1 = cc[grp][1];
                 F4 o4;
 #pragma unroll
                 for (int k = 0; k < 4; k++)
                     o4.v[k] = d.v[k] != 0.0f ? omega * (a.v[k] - m) * d.v[k] + ps * (k < 2 ? c0 : c1) : 0.0f;
                 return o4;
             },
-            [&](uint32_t j, int xq, int yq, int zq) {
-                const float dj = ldf(dinv[j]);
-                const float v = omega * (ldf(r[j]) - m) * dj +
-                                ps * ldf(zc[(uint32_t)(xq >> 1) + csy * (uint32_t)(yq >> 1) + csz * (uint32_t)(zq >> 1)]);
-                return dj != 0.0f ? v : 0.0f;
+            [&](float a, float d, int id) {
+                const float v = omega * (a - m) * d + ps * (id < 4 ? cxm : cxp);
+                return d != 0.0f ? v : 0.0f;
             });
 #pragma unroll
         for (int dz = 0; dz < 2; dz++)
 #pragma unroll
             for (int dy = 0; dy < 2; dy++) {
                 const uint32_t row = o.row[dz][dy];
-                const F4 rv = ld4(r, row), di = ld4(dinv, row);
+                const F4 rv = un(rr.c[dz][dy]), di = un(dd.c[dz][dy]);
                 F4 re{{0.f, 0.f, 0.f, 0.f}};
-                if (COARSE) re = ld4(react, row);
+                if (COARSE) re = un(er[dz][dy]);
                 F4 tv;
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
-                    float out = 0.0f;
-                    if (di.v[k] != 0.0f) {
-                        const float rk = rv.v[k] - m;
-                        float a = act_at(w, f, dz, dy, k);
-                        if (COARSE) a = re.v[k] * f.c[dz][dy][k] + a;
-                        out = f.c[dz][dy][k] + omega * (rk - a) * di.v[k];
-                        if (!COARSE) acc[0] += (double)rk * (double)out;
-                    }
-                    tv.v[k] = out;
+                    const float rk = rv.v[k] - m;
+                    float a = act_at(w, f, dz, dy, k);
+                    if (COARSE) a = re.v[k] * f.c[dz][dy][k] + a;
+                    const float out = f.c[dz][dy][k] + omega * (rk - a) * di.v[k];
+                    const bool on = di.v[k] != 0.0f;
+                    tv.v[k] = on ? out : 0.0f;
+                    if (!COARSE) acc[0] += on ? (double)rk * (double)out : 0.0;
                 }
                 st4(t, row, tv);
             }

@@ -37,8 +37,12 @@ struct Scal {
     double tol2, abs2, bound, inv_nf; // Params.solve.xyz, Params.initial.w
     double pq, rr_stash, rz_old, spare;
     double omega, pscale;             // V-cycle smoother weight and coarse-correction scale
+    double rr64, bb64;                // FP64 refinement: |b - L x|^2 and |b|^2
+    double refine_tol, tau;           // FP64 target and the floor of the inner tolerance
     uint32_t anchor;                  // gauge cell of the single component (mean-free)
     uint32_t pending;                 // fused path: x += alpha p of the last iteration still owed
+    uint32_t outer_done;              // FP64 refinement reached refine_tol (inner work is skipped)
+    uint32_t pad_;
     uint32_t ticket[8];
 };
 
@@ -56,6 +60,7 @@ enum Fin : int {
     FIN_CHECK,         // check_reduce (solver.wgsl:324-328)
     FIN_NORM64,        // outer refinement: rr of the FP64 residual
     FIN_UPDATE_MG,     // fused update: mean of r (mean-free) and |r - mean|^2 for beta_reduce
+    FIN_OUTER,         // FP64 refinement step: outer test, inner tolerance and the inner start
 };
 
 __device__ __forceinline__ float ldf(float v) { return v; }
@@ -213,6 +218,30 @@ __device__ inline void finalize(Scal* s, int mode, const double t[4]) {
         s->spare = t[2];
         s->sr = t[3];
         break;
+    case FIN_OUTER: {  // t = (sum r32^2, |b - L x|^2, |b|^2, sum r32) of the FP64 residual r
+        s->rr64 = t[1];
+        s->bb64 = t[2];
+        const double res = t[2] > 0.0 ? sqrt(t[1] / t[2]) : 0.0;
+        const bool done = t[2] == 0.0 || res <= s->refine_tol;
+        // inner tolerance: no tighter than the remaining factor asks for, at most 0.5
+        double tl = res > 0.0 ? fmax(s->tau, 0.5 * s->refine_tol / res) : 0.5;
+        if (tl > 0.5) tl = 0.5;
+        s->tol2 = tl * tl;
+        s->abs2 = 0.0;
+        s->bound = 1.0;
+        // the inner solve starts from x = 0 with b = r32; its mean-free projection is
+        // deferred (the next passes read r - mean)
+        const double m = s->meanfree > 0.5 ? t[3] * s->inv_nf : 0.0;
+        s->mean = m;
+        s->bb = t[0];
+        s->rr = t[0] - m * t[3];
+        s->sr = t[3];
+        s->rz = 0.0; s->alpha = 0.0; s->beta = 0.0; s->iters = 0.0;
+        s->pending = 0u;
+        s->outer_done = done ? 1u : 0u;
+        s->conv = (done || s->rr <= s->tol2 * s->bb || s->bb == 0.0) ? 1.0 : 0.0;
+        break;
+    }
     default:
         break;
     }

@@ -55,6 +55,11 @@ void launch_pdot(hipStream_t s, const Grid3& g, int store, const float* r, const
 void launch_psearch(hipStream_t s, const Grid3& g, int store, const void* z, const float* dinv, float* p, Scal* sc);
 void launch_residual64(hipStream_t s, const Grid3& g, const double* x64, const double* b64, const float* bx,
                        const float* by, const float* bz, const float* dinv, float* r32, double* part, Scal* sc);
+void launch_outer_start(hipStream_t s, const Grid3& g, const double* x64, const double* b64, const float* bx,
+                        const float* by, const float* bz, const float* dinv, float* r32, float* x32, float* p0,
+                        float* p1, double* part, Scal* sc, bool zero_x);
+void launch_outer_update(hipStream_t s, const Grid3& g, double* x64, const float* x, const float* p0, const float* p1,
+                         const float* dinv, const Scal* sc, bool meanfree, bool assign);
 void launch_xupdate64(hipStream_t s, const Grid3& g, double* x64, const float* e, const float* dinv);
 void launch_fill_f32(hipStream_t s, uint32_t n, float* dst, float v);
 void launch_fill_f64(hipStream_t s, uint32_t n, double* dst, double v);

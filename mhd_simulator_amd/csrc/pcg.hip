@@ -206,6 +206,74 @@ __global__ void __launch_bounds__(MHDP_BLOCK) k_xupdate64(Grid3 g, double* __res
         if (dinv[i] != 0.0f) x[i] += (double)e[i];
 }
 
+// One FP64 refinement step's start (fused multigrid path): t = b - L x in FP64 (x = 0 on
+// the first step: t = b), r32 = t on active cells, and the inner solve's x, p, p' zeroed.
+// Partials (sum r32^2, sum t^2, sum b^2, sum r32) -> FIN_OUTER.
+__global__ void __launch_bounds__(MHDP_BLOCK) k_outer_start(Grid3 g, const double* __restrict__ x,
+                                                            const double* __restrict__ b,
+                                                            const float* __restrict__ bx,
+                                                            const float* __restrict__ by,
+                                                            const float* __restrict__ bz,
+                                                            const float* __restrict__ dinv, float* __restrict__ r32,
+                                                            float* __restrict__ x32, float* __restrict__ p0,
+                                                            float* __restrict__ p1, double* __restrict__ part,
+                                                            Scal* sc, int zero_x) {
+    uint32_t lo, hi;
+    block_range(g.n, lo, hi);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+        const double bi = b[i];
+        double t = bi;
+        if (!zero_x) {
+            int cx, cy, cz;
+            decode(g, i, cx, cy, cz);
+            uint32_t nb[6];
+            float w[6];
+            stencil6(g, cx, cy, cz, i, bx, by, bz, nb, w);
+            const double c = x[i];
+            double a = 0.0;
+#pragma unroll
+            for (int k = 0; k < 6; k++) a += (double)w[k] * (c - x[nb[k]]);
+            t = bi - a;
+        }
+        const float rv = dinv[i] != 0.0f ? (float)t : 0.0f;
+        r32[i] = rv;
+        x32[i] = 0.0f;
+        p0[i] = 0.0f;
+        p1[i] = 0.0f;
+        acc[0] += (double)rv * (double)rv;
+        acc[1] += t * t;
+        acc[2] += bi * bi;
+        acc[3] += (double)rv;
+    }
+    reduce_finish<4>(acc, part, sc, 5, FIN_OUTER);
+}
+
+// x64 += e on active cells, e = the inner FP32 solution with the fused path's owed update
+// x += alpha (p_k - p_k[anchor]) applied first (same FP32 arithmetic as the flush);
+// `assign`: x64 = e (x64 was zero), inactive cells 0.
+__global__ void __launch_bounds__(MHDP_BLOCK) k_outer_update(Grid3 g, double* __restrict__ x64,
+                                                             const float* __restrict__ x, const float* p0,
+                                                             const float* p1, const float* __restrict__ dinv,
+                                                             const Scal* sc, int meanfree, int assign) {
+    const bool pending = sc->pending != 0u;
+    const uint32_t it = (uint32_t)sc->iters;
+    const float* __restrict__ pk = ((it - 1u) & 1u) ? p1 : p0;
+    const float alpha = (float)sc->alpha;
+    const float pa = (pending && meanfree) ? pk[sc->anchor] : 0.0f;
+    uint32_t lo, hi;
+    block_range(g.n, lo, hi);
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+        if (dinv[i] != 0.0f) {
+            float e = x[i];
+            if (pending) e = e + alpha * (pk[i] - pa);
+            x64[i] = assign ? (double)e : x64[i] + (double)e;
+        } else if (assign) {
+            x64[i] = 0.0;
+        }
+    }
+}
+
 template <class T>
 __global__ void __launch_bounds__(MHDP_BLOCK) k_fill(uint32_t n, T* __restrict__ dst, T v) {
     uint32_t lo, hi;
@@ -290,6 +358,17 @@ void launch_residual64(hipStream_t s, const Grid3& g, const double* x64, const d
                        const float* by, const float* bz, const float* dinv, float* r32, double* part, Scal* sc) {
     hipLaunchKernelGGL(k_residual64, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g, x64, b64, bx, by, bz, dinv,
                        r32, part, sc);
+}
+void launch_outer_start(hipStream_t s, const Grid3& g, const double* x64, const double* b64, const float* bx,
+                        const float* by, const float* bz, const float* dinv, float* r32, float* x32, float* p0,
+                        float* p1, double* part, Scal* sc, bool zero_x) {
+    hipLaunchKernelGGL(k_outer_start, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g, x64, b64, bx, by, bz, dinv,
+                       r32, x32, p0, p1, part, sc, zero_x ? 1 : 0);
+}
+void launch_outer_update(hipStream_t s, const Grid3& g, double* x64, const float* x, const float* p0, const float* p1,
+                         const float* dinv, const Scal* sc, bool meanfree, bool assign) {
+    hipLaunchKernelGGL(k_outer_update, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g, x64, x, p0, p1, dinv, sc,
+                       meanfree ? 1 : 0, assign ? 1 : 0);
 }
 void launch_xupdate64(hipStream_t s, const Grid3& g, double* x64, const float* e, const float* dinv) {
     hipLaunchKernelGGL(k_xupdate64, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g, x64, e, dinv);
