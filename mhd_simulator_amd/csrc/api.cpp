@@ -50,6 +50,7 @@ struct mhdp_ctx {
     float *bx = nullptr, *by = nullptr, *bz = nullptr, *diag = nullptr, *dinv = nullptr;
     float *b = nullptr, *x = nullptr, *r = nullptr, *p = nullptr, *q = nullptr;
     float* p2 = nullptr;  // second search-direction buffer of the fused path
+    float* r2 = nullptr;  // twin of r for the brick path's fused residual update
     void *z = nullptr, *t = nullptr;
     double *x64 = nullptr, *b64 = nullptr;
     void *bx16 = nullptr, *by16 = nullptr, *bz16 = nullptr, *dinv16 = nullptr;
@@ -309,7 +310,8 @@ void refresh_stages(mhdp_ctx* c, const Solve& S) {  // mod.rs:1004-1014
 }
 
 // V-cycle with the fine level fused into two passes; its last pass reduces r.z (`mode`).
-void vcycle_fused(mhdp_ctx* c, Solve& S, int mode) {
+// `update` (brick path): the first pass also applies the PCG residual update.
+void vcycle_fused(mhdp_ctx* c, Solve& S, int mode, bool update = false) {
     hipStream_t s = c->stream;
     const bool f16 = S.store == STORE_BF16;
     const void* dinv = f16 ? c->dinv16 : (const void*)c->dinv;
@@ -320,14 +322,22 @@ void vcycle_fused(mhdp_ctx* c, Solve& S, int mode) {
     const int L = (int)S.lv.size();
     const bool brick = c->brick_enabled && brick_ok(c->g);
     c->timer.before(MHDP_K_RESTRICT, s);
-    if (brick) launch_brick_restrict(s, S.store, lv[1].g, lv[0].g, c->r, dinv, bx, by, bz, lv[1].r, c->sc);
-    else launch_fused_restrict(s, S.store, lv[1].g, lv[0].g, c->r, dinv, bx, by, bz, lv[1].r, c->sc);
+    if (brick && update)
+        launch_brick_restrict_update(s, S.store, lv[1].g, lv[0].g, c->r, c->r2, c->q, dinv, bx, by, bz, lv[1].r,
+                                     c->part, c->sc);
+    else if (brick)
+        launch_brick_restrict(s, S.store, lv[1].g, lv[0].g, c->r, dinv, bx, by, bz, lv[1].r, c->sc);
+    else
+        launch_fused_restrict(s, S.store, lv[1].g, lv[0].g, c->r, dinv, bx, by, bz, lv[1].r, c->sc);
     c->timer.after(MHDP_K_RESTRICT, s);
     void* z1 = nullptr;
     launch_vcycle_coarse(s, S.store, L, lv, c->sc, S.coarse64, &z1);
     c->timer.before(MHDP_K_SMOOTH, s);
-    if (brick) launch_brick_smooth(s, S.store, lv[0].g, lv[1].g, c->r, dinv, bx, by, bz, z1, lv[0].t, c->part, c->sc, mode);
-    else launch_fused_smooth(s, S.store, lv[0].g, lv[1].g, c->r, dinv, bx, by, bz, z1, lv[0].t, c->part, c->sc, mode);
+    if (brick)
+        launch_brick_smooth(s, S.store, lv[0].g, lv[1].g, c->r, c->r2, dinv, bx, by, bz, z1, lv[0].t, c->part, c->sc,
+                            mode, update);
+    else
+        launch_fused_smooth(s, S.store, lv[0].g, lv[1].g, c->r, dinv, bx, by, bz, z1, lv[0].t, c->part, c->sc, mode);
     c->timer.after(MHDP_K_SMOOTH, s);
 }
 
@@ -352,17 +362,20 @@ void iterate_once(mhdp_ctx* c, Solve& S) {  // mod.rs:1057-1075, multigrid.rs:30
     hipStream_t s = c->stream;
     if (S.fused) {
         c->timer.before(MHDP_K_APPLY, s);
-        if (c->brick_enabled && brick_ok(c->g))
+        const bool brick = c->brick_enabled && brick_ok(c->g);
+        if (brick)
             launch_brick_apply(s, c->g, S.store, S.lv[0].t, c->p, c->p2, c->x, c->bx, c->by, c->bz, c->dinv, c->q,
                                c->part, c->sc, S.meanfree ? 1 : 0);
         else
             launch_fused_apply(s, c->g, S.store, S.lv[0].t, c->p, c->p2, c->x, c->bx, c->by, c->bz, c->dinv, c->q,
                                c->part, c->sc, S.meanfree ? 1 : 0);
         c->timer.after(MHDP_K_APPLY, s);
-        c->timer.before(MHDP_K_UPDATE, s);
-        launch_fused_update(s, c->g, c->r, c->q, c->dinv, c->part, c->sc);
-        c->timer.after(MHDP_K_UPDATE, s);
-        vcycle_fused(c, S, FIN_BETA_MG);
+        if (!brick) {
+            c->timer.before(MHDP_K_UPDATE, s);
+            launch_fused_update(s, c->g, c->r, c->q, c->dinv, c->part, c->sc);
+            c->timer.after(MHDP_K_UPDATE, s);
+        }
+        vcycle_fused(c, S, FIN_BETA_MG, brick);
         return;
     }
     c->timer.before(MHDP_K_APPLY, s);
@@ -511,7 +524,7 @@ int ensure_fine(mhdp_ctx* c) {
         (rc = dalloc(c, (void**)&c->dinv, fb)) || (rc = dalloc(c, (void**)&c->b, fb)) ||
         (rc = dalloc(c, (void**)&c->x, fb)) || (rc = dalloc(c, (void**)&c->r, fb)) ||
         (rc = dalloc(c, (void**)&c->p, fb)) || (rc = dalloc(c, (void**)&c->q, fb)) ||
-        (rc = dalloc(c, (void**)&c->p2, fb)) ||
+        (rc = dalloc(c, (void**)&c->p2, fb)) || (rc = dalloc(c, (void**)&c->r2, fb)) ||
         (rc = dalloc(c, &c->z, fb)) || (rc = dalloc(c, &c->t, fb)) ||
         (rc = dalloc(c, (void**)&c->sc, sizeof(Scal))) ||
         (rc = dalloc(c, (void**)&c->part, kMaxPartials * 4 * sizeof(double))))
@@ -912,8 +925,9 @@ uint64_t mhdp_device_bytes(const mhdp_ctx* c) { return c ? c->bytes : 0; }
 
 void mhdp_destroy(mhdp_ctx* c) {
     if (!c) return;
-    (void)hipDeviceSynchronize();
-    void* ptrs[] = {c->mask, c->alpha, c->bx, c->by, c->bz, c->diag, c->dinv, c->b, c->x, c->r, c->p, c->p2, c->q, c->z,
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    else (void)hipDeviceSynchronize();
+    void* ptrs[] = {c->mask, c->alpha, c->bx, c->by, c->bz, c->diag, c->dinv, c->b, c->x, c->r, c->r2, c->p, c->p2, c->q, c->z,
                     c->t, c->x64, c->b64, c->bx16, c->by16, c->bz16, c->dinv16, c->sc, c->part, c->ustar_stage,
                     c->uout_stage, c->phi_stage};
     for (void* p : ptrs) dfree(c, p);

@@ -355,7 +355,7 @@ __global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_apply(Grid3 g, const
     const float pa = (pending && meanfree) ? pold[sc->anchor] : 0.0f;
     uint32_t lo, hi;
     block_range(g.n >> 4, lo, hi);
-    double acc[1] = {0.0};
+    double acc[2] = {0.0, 0.0};
     for (uint32_t b = lo + threadIdx.x; b < hi; b += MHDP_BLOCK) {
         const Geo o = geo(g, b);
         Rows<VT> zr;
@@ -403,6 +403,7 @@ __global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_apply(Grid3 g, const
                     pn.v[k] = di.v[k] != 0.0f ? f.c[dz][dy][k] : 0.0f;
                     qv.v[k] = act_at(w, f, dz, dy, k);
                     acc[0] += (double)pn.v[k] * (double)qv.v[k];
+                    acc[1] += di.v[k] != 0.0f ? (double)qv.v[k] : 0.0;
                 }
                 st4(pnew, r, pn);
                 st4(q, r, qv);
@@ -416,7 +417,7 @@ __global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_apply(Grid3 g, const
                 }
             }
     }
-    reduce_finish<1>(acc, part, sc, 2, FIN_ALPHA);
+    reduce_finish<2>(acc, part, sc, 2, FIN_ALPHA);
 }
 
 // pass 3 (k_fused_restrict on bricks): the two aggregates of a brick.  COARSE: a coarse level
@@ -493,20 +494,126 @@ __global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_restrict(Grid3 cg, G
     }
 }
 
+// passes 2+3 in one: the residual update r' = (r - mean) - alpha q (k_fused_update's
+// arithmetic) evaluated on the brick and its halo, r' stored for the brick's own cells in the
+// twin buffer, and the restriction of the pre-smoothed residual of r' - mean'; mean' is the
+// mean of r' known from the apply pass.  Partials (0, sum r'^2, 0, sum r') -> FIN_RESTRICT_UPD.
+template <class CT, class VT, int MINB>
+__global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_restrict_update(Grid3 cg, Grid3 g, float* r0, float* r1,
+                                                                            const float* __restrict__ q,
+                                                                            const CT* __restrict__ dinv,
+                                                                            const CT* __restrict__ bx,
+                                                                            const CT* __restrict__ by,
+                                                                            const CT* __restrict__ bz,
+                                                                            VT* __restrict__ rc,
+                                                                            double* __restrict__ part, Scal* sc) {
+    if (CONVERGED(sc)) return;
+    const float omega = (float)sc->omega;
+    const float alpha = (float)sc->alpha;
+    const float m = (float)sc->mean;        // mean of r (deferred projection)
+    const float mn = (float)sc->mean_next;  // mean of r'
+    const float* __restrict__ rin = sc->rsel ? r1 : r0;
+    float* __restrict__ rout = sc->rsel ? r0 : r1;
+    uint32_t lo, hi;
+    block_range(g.n >> 4, lo, hi);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (uint32_t b = lo + threadIdx.x; b < hi; b += MHDP_BLOCK) {
+        const Geo o = geo(g, b);
+        Rows<float> rr, qq;
+        Rows<CT> dd;
+        WRaw<CT> wr;
+        load_rows(o, rin, rr);
+        load_rows(o, q, qq);
+        load_rows(o, dinv, dd);
+        load_weights(o, wr, bx, by, bz);
+        fence_rows(rr);
+        fence_rows(qq);
+        fence_rows(dd);
+        fence_weights(wr);
+        Wts w;
+        weights(o, wr, w);
+        // r' on the brick and its halo; the pre-smoothed z1 = omega (r' - mean') / diag
+        Field rn, f;
+        field2(
+            rr, qq, rn,
+            [&](const F4& a, const F4& c, int) {
+                F4 o4;
+#pragma unroll
+                for (int k = 0; k < 4; k++) o4.v[k] = (a.v[k] - m) - alpha * c.v[k];
+                return o4;
+            },
+            [&](float a, float c, int) { return (a - m) - alpha * c; });
+        {
+            Rows<CT>& d = dd;
+#pragma unroll
+            for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+                for (int dy = 0; dy < 2; dy++) {
+                    const F4 dv = un(d.c[dz][dy]);
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        rn.c[dz][dy][k] = dv.v[k] != 0.0f ? rn.c[dz][dy][k] : 0.0f;
+                        f.c[dz][dy][k] = omega * (rn.c[dz][dy][k] - mn) * dv.v[k];
+                    }
+                    const float dxm = un(d.xm[dz][dy]), dxp = un(d.xp[dz][dy]);
+                    f.xm[dz][dy] = omega * ((dxm != 0.0f ? rn.xm[dz][dy] : 0.0f) - mn) * dxm;
+                    f.xp[dz][dy] = omega * ((dxp != 0.0f ? rn.xp[dz][dy] : 0.0f) - mn) * dxp;
+                }
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                const F4 a = un(d.ym[e]), b2 = un(d.yp[e]), c = un(d.zm[e]), h = un(d.zp[e]);
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    f.ym[e][k] = omega * ((a.v[k] != 0.0f ? rn.ym[e][k] : 0.0f) - mn) * a.v[k];
+                    f.yp[e][k] = omega * ((b2.v[k] != 0.0f ? rn.yp[e][k] : 0.0f) - mn) * b2.v[k];
+                    f.zm[e][k] = omega * ((c.v[k] != 0.0f ? rn.zm[e][k] : 0.0f) - mn) * c.v[k];
+                    f.zp[e][k] = omega * ((h.v[k] != 0.0f ? rn.zp[e][k] : 0.0f) - mn) * h.v[k];
+                }
+            }
+        }
+        float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+        for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+            for (int dy = 0; dy < 2; dy++) {
+                const F4 di = un(dd.c[dz][dy]);
+                F4 ro;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const float rk = rn.c[dz][dy][k];
+                    ro.v[k] = rk;
+                    acc[1] += (double)rk * (double)rk;
+                    acc[3] += (double)rk;
+                    const float a = act_at(w, f, dz, dy, k);
+                    float v = (rk - mn) - a;
+                    v = di.v[k] != 0.0f ? v : 0.0f;
+                    if (k < 2) s0 += v;
+                    else s1 += v;
+                }
+                st4(rout, o.row[dz][dy], ro);
+            }
+        const uint32_t ci = (uint32_t)(o.x0 >> 1) + (uint32_t)cg.nx * ((uint32_t)(o.y0 >> 1) + (uint32_t)cg.ny * (uint32_t)(o.z0 >> 1));
+        st2(rc, ci, s0, s1);
+    }
+    reduce_finish<4>(acc, part, sc, 3, FIN_RESTRICT_UPD);
+}
+
 // pass 4 (k_fused_smooth on bricks).  COARSE: coarse level (reaction term, no mean, no r.z).
 template <class CT, class VT, class RT, bool COARSE, int MINB>
-__global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_smooth(Grid3 g, Grid3 cg, const RT* __restrict__ r,
+__global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_smooth(Grid3 g, Grid3 cg, const RT* r0, const RT* r1,
                                                                    const CT* __restrict__ dinv,
                                                                    const CT* __restrict__ bx,
                                                                    const CT* __restrict__ by,
                                                                    const CT* __restrict__ bz,
                                                                    const CT* __restrict__ react,
                                                                    const VT* __restrict__ zc, VT* __restrict__ t,
-                                                                   double* __restrict__ part, Scal* sc, int mode) {
+                                                                   double* __restrict__ part, Scal* sc, int mode,
+                                                                   int use_next) {
     if (CONVERGED(sc)) return;
     const float omega = (float)sc->omega;  // smoother weight (reference: 2/3)
-    const float m = COARSE ? 0.0f : (float)sc->mean;
+    const float m = COARSE ? 0.0f : (float)(use_next ? sc->mean_next : sc->mean);
     const float ps = (float)sc->pscale;  // coarse-correction scale (reference: 1)
+    const RT* __restrict__ r = (!COARSE && sc->rsel) ? r1 : r0;
     const uint32_t csy = (uint32_t)cg.nx, csz = (uint32_t)cg.nx * (uint32_t)cg.ny;
     uint32_t lo, hi;
     block_range(g.n >> 4, lo, hi);
@@ -644,18 +751,33 @@ void launch_brick_restrict(hipStream_t s, int store, const Grid3& cg, const Grid
                        (const T*)by, (const T*)bz, (const T*)nullptr, (T*)rc, sc);
     }
 }
-void launch_brick_smooth(hipStream_t s, int store, const Grid3& g, const Grid3& cg, const float* r, const void* dinv,
-                         const void* bx, const void* by, const void* bz, const void* zc, void* t, double* part,
-                         Scal* sc, int mode) {
+void launch_brick_smooth(hipStream_t s, int store, const Grid3& g, const Grid3& cg, const float* r, const float* r2,
+                         const void* dinv, const void* bx, const void* by, const void* bz, const void* zc, void* t,
+                         double* part, Scal* sc, int mode, bool use_next) {
+    const dim3 B(MHDP_BLOCK), G(grid_blocks(g.n >> 4, brick_grid_cap()));
+    const int un = use_next ? 1 : 0;
+    if (store == STORE_BF16) {
+        using T = __hip_bfloat16;
+        BRICK_DISPATCH(k_brick_smooth, T COMMA T COMMA float COMMA false, g, cg, r, r2, (const T*)dinv, (const T*)bx,
+                       (const T*)by, (const T*)bz, (const T*)nullptr, (const T*)zc, (T*)t, part, sc, mode, un);
+    } else {
+        using T = float;
+        BRICK_DISPATCH(k_brick_smooth, T COMMA T COMMA float COMMA false, g, cg, r, r2, (const T*)dinv, (const T*)bx,
+                       (const T*)by, (const T*)bz, (const T*)nullptr, (const T*)zc, (T*)t, part, sc, mode, un);
+    }
+}
+void launch_brick_restrict_update(hipStream_t s, int store, const Grid3& cg, const Grid3& g, float* r, float* r2,
+                                  const float* q, const void* dinv, const void* bx, const void* by, const void* bz,
+                                  void* rc, double* part, Scal* sc) {
     const dim3 B(MHDP_BLOCK), G(grid_blocks(g.n >> 4, brick_grid_cap()));
     if (store == STORE_BF16) {
         using T = __hip_bfloat16;
-        BRICK_DISPATCH(k_brick_smooth, T COMMA T COMMA float COMMA false, g, cg, r, (const T*)dinv, (const T*)bx,
-                       (const T*)by, (const T*)bz, (const T*)nullptr, (const T*)zc, (T*)t, part, sc, mode);
+        BRICK_DISPATCH(k_brick_restrict_update, T COMMA T, cg, g, r, r2, q, (const T*)dinv, (const T*)bx, (const T*)by,
+                       (const T*)bz, (T*)rc, part, sc);
     } else {
         using T = float;
-        BRICK_DISPATCH(k_brick_smooth, T COMMA T COMMA float COMMA false, g, cg, r, (const T*)dinv, (const T*)bx,
-                       (const T*)by, (const T*)bz, (const T*)nullptr, (const T*)zc, (T*)t, part, sc, mode);
+        BRICK_DISPATCH(k_brick_restrict_update, T COMMA T, cg, g, r, r2, q, (const T*)dinv, (const T*)bx, (const T*)by,
+                       (const T*)bz, (T*)rc, part, sc);
     }
 }
 // Coarse level l (Galerkin graph, residual in the cycle's storage type).
@@ -679,11 +801,13 @@ void launch_brick_smooth_coarse(hipStream_t s, int store, const Grid3& g, const 
     const dim3 B(MHDP_BLOCK), G(grid_blocks(g.n >> 4, brick_grid_cap()));
     if (store == STORE_BF16) {
         using T = __hip_bfloat16;
-        BRICK_DISPATCH(k_brick_smooth, T COMMA T COMMA T COMMA true, g, cg, (const T*)r, (const T*)dinv, (const T*)bx,
-                       (const T*)by, (const T*)bz, (const T*)react, (const T*)zc, (T*)t, (double*)nullptr, sc, 0);
+        BRICK_DISPATCH(k_brick_smooth, T COMMA T COMMA T COMMA true, g, cg, (const T*)r, (const T*)r, (const T*)dinv,
+                       (const T*)bx, (const T*)by, (const T*)bz, (const T*)react, (const T*)zc, (T*)t, (double*)nullptr,
+                       sc, 0, 0);
     } else {
         using T = float;
-        BRICK_DISPATCH(k_brick_smooth, T COMMA T COMMA T COMMA true, g, cg, (const T*)r, (const T*)dinv, (const T*)bx,
-                       (const T*)by, (const T*)bz, (const T*)react, (const T*)zc, (T*)t, (double*)nullptr, sc, 0);
+        BRICK_DISPATCH(k_brick_smooth, T COMMA T COMMA T COMMA true, g, cg, (const T*)r, (const T*)r, (const T*)dinv,
+                       (const T*)bx, (const T*)by, (const T*)bz, (const T*)react, (const T*)zc, (T*)t, (double*)nullptr,
+                       sc, 0, 0);
     }
 }

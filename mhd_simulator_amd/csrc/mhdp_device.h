@@ -42,7 +42,8 @@ struct Scal {
     uint32_t anchor;                  // gauge cell of the single component (mean-free)
     uint32_t pending;                 // fused path: x += alpha p of the last iteration still owed
     uint32_t outer_done;              // FP64 refinement reached refine_tol (inner work is skipped)
-    uint32_t pad_;
+    uint32_t rsel;                    // brick path: the residual lives in r (0) or its twin (1)
+    double mean_next;                 // brick path: mean of the residual the next V-cycle sees
     uint32_t ticket[8];
 };
 
@@ -61,6 +62,7 @@ enum Fin : int {
     FIN_NORM64,        // outer refinement: rr of the FP64 residual
     FIN_UPDATE_MG,     // fused update: mean of r (mean-free) and |r - mean|^2 for beta_reduce
     FIN_OUTER,         // FP64 refinement step: outer test, inner tolerance and the inner start
+    FIN_RESTRICT_UPD,  // restriction with the residual update: mean and |r - mean|^2 of the new r
 };
 
 __device__ __forceinline__ float ldf(float v) { return v; }
@@ -162,27 +164,40 @@ __device__ inline void finalize(Scal* s, int mode, const double t[4]) {
         s->alpha = 0.0; s->beta = 0.0;
         s->mean = 0.0;
         s->pending = 0u;
+        s->rsel = 0u;
         s->conv = (t[1] <= s->tol2 * t[2] + s->abs2 || t[2] == 0.0) ? 1.0 : 0.0;
         break;
     case FIN_REFRESH_MEAN:  // keep the physical residual and b.b; mean of r for the subspace step
         s->rr = t[1]; s->bb = t[2]; s->sr = t[3];
         s->mean = t[3] * s->inv_nf;
+        s->rsel = 0u;
         break;
     case FIN_INITIAL_SUB:  // initial_reduce after pressure_initial_subspace (old.y, old.z kept)
         s->rz = t[0];
         s->alpha = 0.0; s->beta = 0.0;
         s->mean = 0.0;  // r is projected now
+        s->sr = 0.0;
+        s->rsel = 0u;
         s->pending = 0u;
         s->conv = (s->rr <= s->tol2 * s->bb + s->abs2 || s->bb == 0.0) ? 1.0 : 0.0;
         break;
-    case FIN_ALPHA:
+    case FIN_ALPHA:  // t[1] = sum of q over the active cells (brick apply)
         s->pq = t[0];
         s->alpha = s->conv > 0.5 ? 0.0 : s->rz / fmax(t[0], 1e-35);
         s->pending = 1u;
+        // mean of the next residual r' = (r - mean) - alpha q, known before r' exists
+        s->mean_next = (s->sr * s->inv_nf - s->mean) - s->alpha * t[1] * s->inv_nf;
         break;
     case FIN_UPDATE_MG:  // sum r^2 and sum r of the updated (unprojected) residual
         s->mean = t[3] * s->inv_nf;
+        s->sr = t[3];
         s->rr_stash = t[1] - s->mean * t[3];
+        break;
+    case FIN_RESTRICT_UPD:  // as FIN_UPDATE_MG; the new residual is in the other buffer
+        s->mean = t[3] * s->inv_nf;
+        s->sr = t[3];
+        s->rr_stash = t[1] - s->mean * t[3];
+        s->rsel ^= 1u;
         break;
     case FIN_MEAN:
         s->mean = t[3] * s->inv_nf;
@@ -233,6 +248,8 @@ __device__ inline void finalize(Scal* s, int mode, const double t[4]) {
         // deferred (the next passes read r - mean)
         const double m = s->meanfree > 0.5 ? t[3] * s->inv_nf : 0.0;
         s->mean = m;
+        s->mean_next = m;
+        s->rsel = 0u;
         s->bb = t[0];
         s->rr = t[0] - m * t[3];
         s->sr = t[3];

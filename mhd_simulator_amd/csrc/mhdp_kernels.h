@@ -122,9 +122,16 @@ void launch_brick_apply(hipStream_t s, const Grid3& g, int store, const void* z,
                         Scal* sc, int meanfree);
 void launch_brick_restrict(hipStream_t s, int store, const Grid3& cg, const Grid3& g, const float* r, const void* dinv,
                            const void* bx, const void* by, const void* bz, void* rc, Scal* sc);
-void launch_brick_smooth(hipStream_t s, int store, const Grid3& g, const Grid3& cg, const float* r, const void* dinv,
-                         const void* bx, const void* by, const void* bz, const void* zc, void* t, double* part,
-                         Scal* sc, int mode);
+// r2: the residual's twin buffer (the kernel reads the one Scal.rsel names); use_next: the
+// V-cycle's mean is Scal.mean_next (after the fused residual update) instead of Scal.mean.
+void launch_brick_smooth(hipStream_t s, int store, const Grid3& g, const Grid3& cg, const float* r, const float* r2,
+                         const void* dinv, const void* bx, const void* by, const void* bz, const void* zc, void* t,
+                         double* part, Scal* sc, int mode, bool use_next);
+// Residual update fused into the restriction: r' = (r - mean) - alpha q from the buffer Scal.rsel
+// names into its twin (FIN_RESTRICT_UPD flips rsel).
+void launch_brick_restrict_update(hipStream_t s, int store, const Grid3& cg, const Grid3& g, float* r, float* r2,
+                                  const float* q, const void* dinv, const void* bx, const void* by, const void* bz,
+                                  void* rc, double* part, Scal* sc);
 void launch_brick_restrict_coarse(hipStream_t s, int store, const Grid3& cg, const Grid3& g, const void* r,
                                   const void* dinv, const void* bx, const void* by, const void* bz, const void* react,
                                   void* rc, Scal* sc);
