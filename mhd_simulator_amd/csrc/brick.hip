@@ -279,7 +279,7 @@ __device__ __forceinline__ void fence_weights(WRaw<CT>& w) {
 }
 // Closed high-side faces (non-periodic boundary) get weight 0.
 template <class CT>
-__device__ __forceinline__ void weights(const Geo& o, const WRaw<CT>& raw, Wts& w) {
+__device__ __forceinline__ void weights(const Geo& o, const WRaw<CT>& raw, Wts& w) {  // FP32 weights
 #pragma unroll
     for (int dz = 0; dz < 2; dz++)
 #pragma unroll
@@ -304,8 +304,54 @@ __device__ __forceinline__ void weights(const Geo& o, const WRaw<CT>& raw, Wts& 
     }
 }
 
+// BF16 mobilities stay packed two to a register until use (halves their register footprint).
+struct WtsH {
+    uint32_t bx[2][2][2], by[2][2][2], bz[2][2][2];  // [dz][dy][word], word w holds lanes 2w, 2w+1
+    float bxp[2][2];
+    uint32_t byp[2][2], bzp[2][2];                   // [dz or dy][word]
+};
+__device__ __forceinline__ void weights(const Geo& o, const WRaw<__hip_bfloat16>& raw, WtsH& w) {
+#pragma unroll
+    for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+        for (int dy = 0; dy < 2; dy++) {
+            w.bx[dz][dy][0] = raw.bx[dz][dy].v.x; w.bx[dz][dy][1] = raw.bx[dz][dy].v.y;
+            w.by[dz][dy][0] = raw.by[dz][dy].v.x; w.by[dz][dy][1] = raw.by[dz][dy].v.y;
+            w.bz[dz][dy][0] = raw.bz[dz][dy].v.x; w.bz[dz][dy][1] = raw.bz[dz][dy].v.y;
+            w.bxp[dz][dy] = o.xp_ok ? un(raw.bxp[dz][dy]) : 0.0f;
+        }
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
+        w.byp[d][0] = o.yp_ok ? raw.byp[d].v.x : 0u;
+        w.byp[d][1] = o.yp_ok ? raw.byp[d].v.y : 0u;
+        w.bzp[d][0] = o.zp_ok ? raw.bzp[d].v.x : 0u;
+        w.bzp[d][1] = o.zp_ok ? raw.bzp[d].v.y : 0u;
+    }
+}
+__device__ __forceinline__ float hx(const uint32_t (&w)[2], int k) { return (k & 1) ? bfh(w[k >> 1]) : bfl(w[k >> 1]); }
+// the face weights of core cell (dz, dy, k): lower x, upper x, lower y, upper y, lower z, upper z
+__device__ __forceinline__ void face_w(const Wts& w, int dz, int dy, int k, float (&o)[6]) {
+    o[0] = w.bx[dz][dy][k];
+    o[1] = k < 3 ? w.bx[dz][dy][k + 1] : w.bxp[dz][dy];
+    o[2] = w.by[dz][dy][k];
+    o[3] = dy == 0 ? w.by[dz][1][k] : w.byp[dz][k];
+    o[4] = w.bz[dz][dy][k];
+    o[5] = dz == 0 ? w.bz[1][dy][k] : w.bzp[dy][k];
+}
+__device__ __forceinline__ void face_w(const WtsH& w, int dz, int dy, int k, float (&o)[6]) {
+    o[0] = hx(w.bx[dz][dy], k);
+    o[1] = k < 3 ? hx(w.bx[dz][dy], k + 1) : w.bxp[dz][dy];
+    o[2] = hx(w.by[dz][dy], k);
+    o[3] = dy == 0 ? hx(w.by[dz][1], k) : hx(w.byp[dz], k);
+    o[4] = hx(w.bz[dz][dy], k);
+    o[5] = dz == 0 ? hx(w.bz[1][dy], k) : hx(w.bzp[dy], k);
+}
+template <class CT> struct WtsOf { using type = Wts; };
+template <> struct WtsOf<__hip_bfloat16> { using type = WtsH; };
+
 // A f at core cell (dz, dy, k) in the reference order x-, x+, y-, y+, z-, z+.
-__device__ __forceinline__ float act_at(const Wts& w, const Field& f, int dz, int dy, int k) {
+template <class W>
+__device__ __forceinline__ float act_at(const W& w, const Field& f, int dz, int dy, int k) {
     const float c = f.c[dz][dy][k];
     const float xm = k > 0 ? f.c[dz][dy][k - 1] : f.xm[dz][dy];
     const float xp = k < 3 ? f.c[dz][dy][k + 1] : f.xp[dz][dy];
@@ -313,16 +359,15 @@ __device__ __forceinline__ float act_at(const Wts& w, const Field& f, int dz, in
     const float yp = dy == 0 ? f.c[dz][1][k] : f.yp[dz][k];
     const float zm = dz == 1 ? f.c[0][dy][k] : f.zm[dy][k];
     const float zp = dz == 0 ? f.c[1][dy][k] : f.zp[dy][k];
-    const float wxp = k < 3 ? w.bx[dz][dy][k + 1] : w.bxp[dz][dy];
-    const float wyp = dy == 0 ? w.by[dz][1][k] : w.byp[dz][k];
-    const float wzp = dz == 0 ? w.bz[1][dy][k] : w.bzp[dy][k];
+    float fw[6];
+    face_w(w, dz, dy, k, fw);
     float a = 0.0f;
-    a += w.bx[dz][dy][k] * (c - xm);
-    a += wxp * (c - xp);
-    a += w.by[dz][dy][k] * (c - ym);
-    a += wyp * (c - yp);
-    a += w.bz[dz][dy][k] * (c - zm);
-    a += wzp * (c - zp);
+    a += fw[0] * (c - xm);
+    a += fw[1] * (c - xp);
+    a += fw[2] * (c - ym);
+    a += fw[3] * (c - yp);
+    a += fw[4] * (c - zm);
+    a += fw[5] * (c - zp);
     return a;
 }
 
@@ -459,7 +504,7 @@ __global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_restrict(Grid3 cg, G
 #pragma unroll
                 for (int dy = 0; dy < 2; dy++) fence(er[dz][dy]);
         }
-        Wts w;
+        typename WtsOf<CT>::type w;
         weights(o, wr, w);
         Field f;
         field2(
@@ -530,7 +575,7 @@ __global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_restrict_update(Grid
         fence_rows(qq);
         fence_rows(dd);
         fence_weights(wr);
-        Wts w;
+        typename WtsOf<CT>::type w;
         weights(o, wr, w);
         // r' on the brick and its halo; the pre-smoothed z1 = omega (r' - mean') / diag
         Field rn, f;
@@ -665,7 +710,7 @@ __global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_smooth(Grid3 g, Grid
         un(zzm, cc[3][0], cc[3][1]);
         un(zzp, cc[4][0], cc[4][1]);
         const float cxm = un(zxm), cxp = un(zxp);
-        Wts w;
+        typename WtsOf<CT>::type w;
         weights(o, wr, w);
         Field f;
         field2(

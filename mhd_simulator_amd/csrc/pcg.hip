@@ -4,6 +4,7 @@
 // last-block fold that performs the scalar step of the reference's single-workgroup
 // reduction kernels (alpha_reduce, beta_reduce, ...), so no separate launch is needed.
 #include "mhdp_kernels.h"
+#include "rows.h"
 
 #define CONVERGED(sc) ((sc)->conv > 0.5)
 
@@ -274,6 +275,104 @@ __global__ void __launch_bounds__(MHDP_BLOCK) k_outer_update(Grid3 g, double* __
     }
 }
 
+// Row-vector forms of the two refinement passes (nx % 4 == 0): four cells of a row per thread,
+// the same FP64 arithmetic per cell (stencil order x-, x+, y-, y+, z-, z+).
+__global__ void __launch_bounds__(MHDP_BLOCK) k_outer_start4(Grid3 g, const double* __restrict__ x,
+                                                             const double* __restrict__ b,
+                                                             const float* __restrict__ bx,
+                                                             const float* __restrict__ by,
+                                                             const float* __restrict__ bz,
+                                                             const float* __restrict__ dinv, float* __restrict__ r32,
+                                                             float* __restrict__ x32, float* __restrict__ p0,
+                                                             float* __restrict__ p1, double* __restrict__ part,
+                                                             Scal* sc, int zero_x) {
+    uint32_t lo, hi;
+    block_range(g.n >> 2, lo, hi);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const float zero4[4] = {0.f, 0.f, 0.f, 0.f};
+    for (uint32_t t = lo + threadIdx.x; t < hi; t += MHDP_BLOCK) {
+        const Row4 r = row4(g, t);
+        double bv[4], tv[4];
+        ld4d(b, r.i0, bv);
+        float di[4];
+        ld4f(dinv, r.i0, di);
+#pragma unroll
+        for (int k = 0; k < 4; k++) tv[k] = bv[k];
+        if (!zero_x) {
+            double xc[4], xym[4], xyp[4], xzm[4], xzp[4];
+            ld4d(x, r.i0, xc);
+            ld4d(x, r.ym, xym);
+            ld4d(x, r.yp, xyp);
+            ld4d(x, r.zm, xzm);
+            ld4d(x, r.zp, xzp);
+            const double xxm = x[r.xm], xxp = x[r.xp];
+            float wx[4], wy[4], wz[4], wyu[4], wzu[4];
+            ld4f(bx, r.i0, wx);
+            ld4f(by, r.i0, wy);
+            ld4f(bz, r.i0, wz);
+            ld4f(by, r.yp, wyu);
+            ld4f(bz, r.zp, wzu);
+            const float wxe = bx[r.xp];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const double c = xc[k];
+                const double w1 = k < 3 ? (double)wx[k + 1] : (r.xp_ok ? (double)wxe : 0.0);
+                double a = 0.0;
+                a += (double)wx[k] * (c - (k > 0 ? xc[k - 1] : xxm));
+                a += w1 * (c - (k < 3 ? xc[k + 1] : xxp));
+                a += (double)wy[k] * (c - xym[k]);
+                a += (r.yp_ok ? (double)wyu[k] : 0.0) * (c - xyp[k]);
+                a += (double)wz[k] * (c - xzm[k]);
+                a += (r.zp_ok ? (double)wzu[k] : 0.0) * (c - xzp[k]);
+                tv[k] = bv[k] - a;
+            }
+        }
+        float rv[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            rv[k] = di[k] != 0.0f ? (float)tv[k] : 0.0f;
+            acc[0] += (double)rv[k] * (double)rv[k];
+            acc[1] += tv[k] * tv[k];
+            acc[2] += bv[k] * bv[k];
+            acc[3] += (double)rv[k];
+        }
+        st4f(r32, r.i0, rv);
+        st4f(x32, r.i0, zero4);
+        st4f(p0, r.i0, zero4);
+        st4f(p1, r.i0, zero4);
+    }
+    reduce_finish<4>(acc, part, sc, 5, FIN_OUTER);
+}
+
+__global__ void __launch_bounds__(MHDP_BLOCK) k_outer_update4(Grid3 g, double* __restrict__ x64,
+                                                              const float* __restrict__ x, const float* p0,
+                                                              const float* p1, const float* __restrict__ dinv,
+                                                              const Scal* sc, int meanfree, int assign) {
+    const bool pending = sc->pending != 0u;
+    const uint32_t it = (uint32_t)sc->iters;
+    const float* __restrict__ pk = ((it - 1u) & 1u) ? p1 : p0;
+    const float alpha = (float)sc->alpha;
+    const float pa = (pending && meanfree) ? pk[sc->anchor] : 0.0f;
+    uint32_t lo, hi;
+    block_range(g.n >> 2, lo, hi);
+    for (uint32_t t = lo + threadIdx.x; t < hi; t += MHDP_BLOCK) {
+        const uint32_t i0 = t << 2;
+        float xv[4], pv[4], di[4];
+        ld4f(x, i0, xv);
+        ld4f(pk, i0, pv);
+        ld4f(dinv, i0, di);
+        double xo[4] = {0.0, 0.0, 0.0, 0.0};
+        if (!assign) ld4d(x64, i0, xo);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            float e = xv[k];
+            if (pending) e = e + alpha * (pv[k] - pa);
+            if (di[k] != 0.0f) xo[k] = assign ? (double)e : xo[k] + (double)e;
+        }
+        st4d(x64, i0, xo);
+    }
+}
+
 template <class T>
 __global__ void __launch_bounds__(MHDP_BLOCK) k_fill(uint32_t n, T* __restrict__ dst, T v) {
     uint32_t lo, hi;
@@ -362,13 +461,21 @@ void launch_residual64(hipStream_t s, const Grid3& g, const double* x64, const d
 void launch_outer_start(hipStream_t s, const Grid3& g, const double* x64, const double* b64, const float* bx,
                         const float* by, const float* bz, const float* dinv, float* r32, float* x32, float* p0,
                         float* p1, double* part, Scal* sc, bool zero_x) {
-    hipLaunchKernelGGL(k_outer_start, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g, x64, b64, bx, by, bz, dinv,
-                       r32, x32, p0, p1, part, sc, zero_x ? 1 : 0);
+    if ((g.nx & 3) == 0)
+        hipLaunchKernelGGL(k_outer_start4, dim3(grid_blocks(g.n >> 2)), dim3(MHDP_BLOCK), 0, s, g, x64, b64, bx, by, bz,
+                           dinv, r32, x32, p0, p1, part, sc, zero_x ? 1 : 0);
+    else
+        hipLaunchKernelGGL(k_outer_start, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g, x64, b64, bx, by, bz, dinv,
+                           r32, x32, p0, p1, part, sc, zero_x ? 1 : 0);
 }
 void launch_outer_update(hipStream_t s, const Grid3& g, double* x64, const float* x, const float* p0, const float* p1,
                          const float* dinv, const Scal* sc, bool meanfree, bool assign) {
-    hipLaunchKernelGGL(k_outer_update, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g, x64, x, p0, p1, dinv, sc,
-                       meanfree ? 1 : 0, assign ? 1 : 0);
+    if ((g.nx & 3) == 0)
+        hipLaunchKernelGGL(k_outer_update4, dim3(grid_blocks(g.n >> 2)), dim3(MHDP_BLOCK), 0, s, g, x64, x, p0, p1, dinv,
+                           sc, meanfree ? 1 : 0, assign ? 1 : 0);
+    else
+        hipLaunchKernelGGL(k_outer_update, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g, x64, x, p0, p1, dinv, sc,
+                           meanfree ? 1 : 0, assign ? 1 : 0);
 }
 void launch_xupdate64(hipStream_t s, const Grid3& g, double* x64, const float* e, const float* dinv) {
     hipLaunchKernelGGL(k_xupdate64, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g, x64, e, dinv);
