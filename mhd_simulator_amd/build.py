@@ -28,6 +28,7 @@ SOURCES = {
     "mg.hip": [],
     "fused.hip": [],
     "brick.hip": [],
+    "march.hip": [],
     "api.cpp": ["-x", "hip", "-pthread"],
 }
 

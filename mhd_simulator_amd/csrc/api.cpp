@@ -70,6 +70,7 @@ struct mhdp_ctx {
     uint64_t bytes = 0;
     bool fused_enabled = true;  // MHDP_UNFUSED=1 selects the one-kernel-per-stage path
     bool brick_enabled = true;  // MHDP_NO_BRICK=1 keeps the 4-cell-group fused kernels
+    bool march_enabled = true;  // z-marching fine passes where the grid allows (MHDP_NO_MARCH=1: bricks)
     KernelTimer timer;
     double timer_ms = 0.0;
     uint32_t timer_launches = 0;
@@ -333,7 +334,10 @@ void vcycle_fused(mhdp_ctx* c, Solve& S, int mode, bool update = false) {
     void* z1 = nullptr;
     launch_vcycle_coarse(s, S.store, L, lv, c->sc, S.coarse64, &z1);
     c->timer.before(MHDP_K_SMOOTH, s);
-    if (brick)
+    if (brick && c->march_enabled && march_ok(c->g))
+        launch_march_smooth(s, S.store, lv[0].g, lv[1].g, c->r, c->r2, dinv, bx, by, bz, z1, lv[0].t, c->part, c->sc,
+                            mode, update);
+    else if (brick)
         launch_brick_smooth(s, S.store, lv[0].g, lv[1].g, c->r, c->r2, dinv, bx, by, bz, z1, lv[0].t, c->part, c->sc,
                             mode, update);
     else

@@ -138,3 +138,9 @@ void launch_brick_restrict_coarse(hipStream_t s, int store, const Grid3& cg, con
 void launch_brick_smooth_coarse(hipStream_t s, int store, const Grid3& g, const Grid3& cg, const void* r,
                                 const void* dinv, const void* bx, const void* by, const void* bz, const void* react,
                                 const void* zc, void* t, Scal* sc);
+
+// march.hip — z-marching forms of the fine passes (nx % 4 == 0, even ny and nz).
+bool march_ok(const Grid3& g);
+void launch_march_smooth(hipStream_t s, int store, const Grid3& g, const Grid3& cg, const float* r, const float* r2,
+                         const void* dinv, const void* bx, const void* by, const void* bz, const void* zc, void* t,
+                         double* part, Scal* sc, int mode, bool use_next);
