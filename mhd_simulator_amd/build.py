@@ -29,6 +29,7 @@ SOURCES = {
     "fused.hip": [],
     "brick.hip": [],
     "march.hip": [],
+    "cc.hip": [],
     "api.cpp": ["-x", "hip", "-pthread"],
 }
 

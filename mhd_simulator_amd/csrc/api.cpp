@@ -18,6 +18,7 @@
 #include "mhdp_kernels.h"
 
 void launch_from_bf16(hipStream_t s, uint32_t n, const void* src, float* dst);
+void launch_components(hipStream_t s, const Grid3& g, const uint8_t* fluid, uint32_t* mask, uint32_t* counts);
 
 namespace {
 
@@ -39,8 +40,9 @@ struct mhdp_ctx {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     std::string err;
-    // host copy of the encoded mask (anchors())
-    std::vector<uint32_t> mask_h;
+    bool labels_on_host = false;  // the last mhdp_set_cells labelled on the host
+    uint8_t* fluid8 = nullptr;    // fluid flags as given (device staging for the labelling)
+    uint32_t* cc_counts = nullptr;
     uint32_t components = 0, fluid_cells = 0, anchor0 = 0;
     bool cells = false;
     bool has_alpha = false;
@@ -703,6 +705,12 @@ int project_impl(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, float* uou
     return MHDP_OK;
 }
 
+// MHDP_HOST_ANCHORS=1: label the gauge mask with the host BFS (comparison runs).
+bool host_anchors_env() {
+    const char* e = std::getenv("MHDP_HOST_ANCHORS");
+    return e && e[0] == '1';
+}
+
 // Host threads of mhdp_project_batch (MHDP_BATCH_THREADS, 1..64; default 8), read per call.
 int batch_workers() {
     const char* e = std::getenv("MHDP_BATCH_THREADS");
@@ -773,39 +781,59 @@ int mhdp_set_cells(mhdp_ctx* c, const uint8_t* fluid, const float* alpha) {
     if (!fluid) return fail(c, MHDP_INVALID, "Missing fluid mask");
     if (c->phase.two_phase && !alpha) return fail(c, MHDP_INVALID, "Two-phase projection needs the liquid fraction");
     const uint32_t n = c->g.n;
-    std::vector<uint32_t> mask(n);
-    uint32_t fluid_cells = 0;
-    for (uint32_t i = 0; i < n; i++) {
-        mask[i] = fluid[i] ? 1u : 0u;
-        fluid_cells += mask[i];
-    }
     if (alpha)
         for (uint32_t i = 0; i < n; i++)
             if (!std::isfinite(alpha[i])) return fail(c, MHDP_INVALID, "Nonfinite liquid fraction");
-    const uint32_t d[3] = {(uint32_t)c->g.nx, (uint32_t)c->g.ny, (uint32_t)c->g.nz};
-    const uint32_t comps = label_components(mask, d, c->g.per);
-    uint32_t anchor = 0;
-    for (uint32_t i = 0; i < n; i++)
-        if ((mask[i] & 3u) == 2u) {
-            anchor = i;
-            break;
+    c->cells = false;
+    int rc;
+    if ((rc = dalloc(c, (void**)&c->fluid8, n)) || (rc = dalloc(c, (void**)&c->cc_counts, 4 * sizeof(uint32_t))))
+        return rc;
+    HIP_OK(hipMemcpyAsync(c->fluid8, fluid, n, hipMemcpyHostToDevice, c->stream));
+    if (alpha) HIP_OK(hipMemcpyAsync(c->alpha, alpha, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+    // anchors() on the device; the host BFS takes over if a union-find bound was reached
+    // (never seen; the bounds exist so that no input can keep the kernels running).
+    uint32_t counts[4] = {0u, 0u, 0u, 0u};
+    bool host = host_anchors_env();
+    if (!host) {
+        launch_components(c->stream, c->g, c->fluid8, c->mask, c->cc_counts);
+        HIP_OK(hipMemcpyAsync(counts, c->cc_counts, sizeof counts, hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        HIP_OK(hipGetLastError());
+        host = counts[3] != 0u;
+    }
+    if (host) {
+        std::vector<uint32_t> mask(n);
+        uint32_t fluid_cells = 0;
+        for (uint32_t i = 0; i < n; i++) {
+            mask[i] = fluid[i] ? 1u : 0u;
+            fluid_cells += mask[i];
         }
-    if (hipMemcpy(c->mask, mask.data(), (size_t)n * 4, hipMemcpyHostToDevice) != hipSuccess)
-        return fail(c, MHDP_DEVICE, "HIP error uploading the fluid mask");
-    if (alpha && hipMemcpy(c->alpha, alpha, (size_t)n * 4, hipMemcpyHostToDevice) != hipSuccess)
-        return fail(c, MHDP_DEVICE, "HIP error uploading the liquid fraction");
+        const uint32_t d[3] = {(uint32_t)c->g.nx, (uint32_t)c->g.ny, (uint32_t)c->g.nz};
+        counts[0] = label_components(mask, d, c->g.per);
+        counts[1] = fluid_cells;
+        counts[2] = 0u;
+        for (uint32_t i = 0; i < n; i++)
+            if ((mask[i] & 3u) == 2u) {
+                counts[2] = i;
+                break;
+            }
+        HIP_OK(hipMemcpyAsync(c->mask, mask.data(), (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+    }
     c->has_alpha = alpha != nullptr;
-    c->mask_h.swap(mask);
-    c->components = comps;
-    c->fluid_cells = fluid_cells;
-    c->anchor0 = anchor;
+    c->components = counts[0];
+    c->fluid_cells = counts[1];
+    c->anchor0 = counts[0] ? counts[2] : 0u;
+    c->labels_on_host = host;
     c->cells = true;
     return MHDP_OK;
 }
 
 int mhdp_get_mask(const mhdp_ctx* c, uint32_t* encoded) {
     if (!c || !encoded || !c->cells) return MHDP_INVALID;
-    std::memcpy(encoded, c->mask_h.data(), c->mask_h.size() * 4);
+    if (hipMemcpyAsync(encoded, c->mask, (size_t)c->g.n * 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return MHDP_DEVICE;
     return MHDP_OK;
 }
 
@@ -933,7 +961,7 @@ void mhdp_destroy(mhdp_ctx* c) {
     else (void)hipDeviceSynchronize();
     void* ptrs[] = {c->mask, c->alpha, c->bx, c->by, c->bz, c->diag, c->dinv, c->b, c->x, c->r, c->r2, c->p, c->p2, c->q, c->z,
                     c->t, c->x64, c->b64, c->bx16, c->by16, c->bz16, c->dinv16, c->sc, c->part, c->ustar_stage,
-                    c->uout_stage, c->phi_stage};
+                    c->uout_stage, c->phi_stage, c->fluid8, c->cc_counts};
     for (void* p : ptrs) dfree(c, p);
     for (CoarseArrays& a : c->coarse) {
         void* q[] = {a.bx, a.by, a.bz, a.react, a.diag, a.dinv, a.bx16, a.by16, a.bz16, a.react16, a.dinv16,

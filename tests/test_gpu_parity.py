@@ -242,3 +242,18 @@ def test_deterministic_and_device_path_identical(precision):
         torch.cuda.synchronize()
         pr.project(u_in, precision=precision, precond="mg", u_out=u_out, phi_out=phi)
         assert np.array_equal(u_out.cpu().numpy(), u1) and np.array_equal(phi.cpu().numpy(), phi1)
+
+
+@pytest.mark.parametrize("periodic", [(False,) * 3, (True, False, True), (True,) * 3, (False, True, False)])
+@pytest.mark.parametrize("dims,frac,seed", [((40, 24, 16), 0.45, 3), ((24, 1, 9), 0.6, 4), ((64, 48, 32), 0.32, 5),
+                                            ((7, 5, 3), 0.3, 6)])
+def test_device_labelling_matches_oracle(orc, dims, frac, seed, periodic):
+    """anchors() labelled on the GPU (union-find) against the oracle's BFS, bit for bit, on
+    random masks with many components (near the 3-D site-percolation threshold)."""
+    g = C.Grid(*dims, periodic=periodic)
+    fluid = (C.unit_uniform(seed, g.cells) < frac).astype(np.uint8)
+    print(f"[gpu] labelling dims={dims} periodic={periodic} frac={frac}", flush=True)
+    with PressureProjection(g, fluid) as pr:
+        mask, comps = orc.anchors(fluid, dims, periodic)
+        assert pr.components == comps and comps > 1
+        assert np.array_equal(pr.mask, mask)
