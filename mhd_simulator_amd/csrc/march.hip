@@ -306,13 +306,6 @@ __global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_march_smooth(Grid3 g, Grid
 
 // ---------------------------------------------------------------- launchers
 
-bool march_ok(const Grid3& g) {
-    static const bool on = [] {
-        const char* e = std::getenv("MHDP_NO_MARCH");
-        return !(e && e[0] == '1');
-    }();
-    return on && (g.nx & 3) == 0 && (g.ny & 1) == 0 && (g.nz & 1) == 0 && g.nz >= 2;
-}
 // Planes per work item (MHDP_MARCH_Z: even, 2..512; default 32) and the block cap
 // (MHDP_MARCH_CAP; default 1024).
 static uint32_t march_z() {
@@ -322,6 +315,21 @@ static uint32_t march_z() {
         return (uint32_t)((v >= 2 && v <= 512 && (v % 2) == 0) ? v : 32);
     }();
     return z;
+}
+bool march_ok(const Grid3& g) {
+    static const bool on = [] {
+        const char* e = std::getenv("MHDP_NO_MARCH");
+        return !(e && e[0] == '1');
+    }();
+    if (!on || (g.nx & 3) != 0 || (g.ny & 1) != 0 || (g.nz & 1) != 0 || g.nz < 2) return false;
+    // a chunk is walked by one thread: small grids have too few columns to fill the device and
+    // keep the 4x2x2 bricks (one thread per 16 cells).  MHDP_MARCH_MIN (work items, read per
+    // call) moves the threshold, e.g. 0 to exercise this path on small grids.
+    const char* e = std::getenv("MHDP_MARCH_MIN");
+    const uint64_t min_items = e ? (uint64_t)std::strtoull(e, nullptr, 10) : 65536u;
+    const uint32_t zl = march_z();
+    const uint64_t items = (uint64_t)((uint32_t)g.nx >> 2) * ((uint32_t)g.ny >> 1) * (((uint32_t)g.nz + zl - 1) / zl);
+    return items >= min_items;
 }
 static uint32_t march_blocks(const Grid3& g) {
     static const uint32_t cap = [] {
