@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "mhdp_kernels.h"
+#include "launch.h"
 
 void launch_from_bf16(hipStream_t s, uint32_t n, const void* src, float* dst);
 void launch_components(hipStream_t s, const Grid3& g, const uint8_t* fluid, uint32_t* mask, uint32_t* counts);
@@ -705,10 +706,363 @@ int project_impl(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, float* uou
     return MHDP_OK;
 }
 
+// ============================================================================ batched projection
+// Lockstep driver of many mixed-precision multigrid projections.  The launches of every stage of
+// every problem are recorded (go<K> appends instead of launching), the per-problem sequences are
+// merged kernel by kernel, and each merged group is ONE launch over all its problems (k_many).
+// Block numbering, arithmetic and reduction order per problem are those of a single projection,
+// so every result is bit-identical to mhdp_project_dev's.
+
+struct MergedOp {
+    void (*issue)(hipStream_t, const void*, const uint32_t*, uint32_t, uint32_t);
+    uint32_t count, blocks;
+    size_t args_off, starts_off;
+};
+struct Plan {
+    std::vector<MergedOp> ops;
+    void* dev = nullptr;
+};
+struct BatchMemory {
+    std::vector<void*> blocks;
+    ~BatchMemory() {
+        for (void* p : blocks) (void)hipFree(p);
+    }
+    void* get(size_t bytes) {
+        void* p = nullptr;
+        if (hipMalloc(&p, bytes ? bytes : 16) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        blocks.push_back(p);
+        return p;
+    }
+};
+
+// Record f(p) for every member p, merge the sequences and upload the merged arguments.
+template <class F>
+bool record_plan(const std::vector<int>& members, BatchMemory& mem, Plan& plan, F&& f) {
+    std::vector<Recorder> recs(members.size());
+    for (size_t i = 0; i < members.size(); i++) {
+        Recorder::active() = &recs[i];
+        f(members[i]);
+        Recorder::active() = nullptr;
+    }
+    plan.ops.clear();
+    std::vector<unsigned char> blob;
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    std::vector<size_t> head(recs.size(), 0);
+    for (;;) {
+        // the kernel waiting at the most sequence heads goes next (ties: lowest problem)
+        std::vector<std::pair<const void*, size_t>> kinds;
+        for (size_t p = 0; p < recs.size(); p++) {
+            if (head[p] >= recs[p].ops.size()) continue;
+            const void* k = recs[p].ops[head[p]].kind;
+            auto it = std::find_if(kinds.begin(), kinds.end(), [&](const auto& e) { return e.first == k; });
+            if (it == kinds.end()) kinds.push_back({k, 1});
+            else it->second++;
+        }
+        if (kinds.empty()) break;
+        const void* best = kinds[0].first;
+        size_t bestn = kinds[0].second;
+        for (const auto& e : kinds)
+            if (e.second > bestn) {
+                best = e.first;
+                bestn = e.second;
+            }
+        MergedOp m{};
+        std::vector<uint32_t> starts{0u};
+        m.args_off = align(blob.size());
+        for (size_t p = 0; p < recs.size(); p++) {
+            if (head[p] >= recs[p].ops.size() || recs[p].ops[head[p]].kind != best) continue;
+            const Recorded& op = recs[p].ops[head[p]];
+            if (blob.size() < m.args_off) blob.resize(m.args_off);
+            blob.insert(blob.end(), op.args.begin(), op.args.end());
+            starts.push_back(starts.back() + op.blocks);
+            m.issue = op.issue;
+            head[p]++;
+        }
+        m.count = (uint32_t)(starts.size() - 1);
+        m.blocks = starts.back();
+        m.starts_off = align(blob.size());
+        blob.resize(m.starts_off);
+        const unsigned char* sb = reinterpret_cast<const unsigned char*>(starts.data());
+        blob.insert(blob.end(), sb, sb + starts.size() * sizeof(uint32_t));
+        if (m.blocks) plan.ops.push_back(m);
+    }
+    plan.dev = mem.get(blob.size());
+    if (!plan.dev) return false;
+    return hipMemcpy(plan.dev, blob.data(), blob.size(), hipMemcpyHostToDevice) == hipSuccess;
+}
+void issue_plan(hipStream_t s, const Plan& plan) {
+    const unsigned char* base = static_cast<const unsigned char*>(plan.dev);
+    for (const MergedOp& m : plan.ops)
+        m.issue(s, base + m.args_off, reinterpret_cast<const uint32_t*>(base + m.starts_off), m.count, m.blocks);
+}
+
+// Copies the scalar blocks of `count` problems into one array (read back with one copy).
+struct KGatherScal {
+    static constexpr int MINB = 0;
+    struct Args {
+        const Scal* const* src;
+        uint32_t* dst;
+        uint32_t count;
+    };
+    static __device__ void run(const Args& A, const VGrid& vg) {
+        constexpr uint32_t words = sizeof(Scal) / 4;
+        const uint32_t total = words * A.count;
+        for (uint32_t i = vg.b * MHDP_BLOCK + threadIdx.x; i < total; i += vg.n * MHDP_BLOCK) {
+            const uint32_t p = i / words, w = i - p * words;
+            A.dst[i] = reinterpret_cast<const uint32_t*>(A.src[p])[w];
+        }
+    }
+};
+
+// Eligible for the lockstep driver: the fused multigrid path of the mixed solver.
+bool lockstep_ok(const mhdp_ctx* c, const mhdp_opts* o) {
+    const bool preferred = o->precond == MHDP_PRECOND_MG || (o->precond == MHDP_PRECOND_AUTO && c->g.n >= kMgMinCells);
+    return c->cells && o->precision == MHDP_MIXED && preferred && c->fused_enabled && hierarchy(c->g).size() >= 2;
+}
+
+int project_lockstep(mhdp_ctx* const* ctxs, int count, const float* const* ustar, const mhdp_opts* o,
+                     float* const* uout, double* const* phi, mhdp_stats* st, std::vector<int>& rcs) {
+    const int n = count;
+    mhdp_ctx* c0 = ctxs[0];
+    const hipStream_t s = c0->stream;
+    for (int p = 0; p < n; p++)
+        if (hipStreamSynchronize(ctxs[p]->stream) != hipSuccess) return MHDP_DEVICE;
+    struct PS {
+        Solve S;
+        bool running = false, done = false, inner = false;
+        uint32_t outer = 0, iterations = 0, it = 0;
+        bool checked = false;
+        double res64 = 0.0;
+        int timer = -1;
+    };
+    std::vector<PS> ps((size_t)n);
+    const double tau = o->inner_tol > 0.0f ? (double)o->inner_tol : 1e-4;
+    const uint32_t max_outer = o->max_outer ? o->max_outer : 16u;
+    const double omega = o->smooth_weight > 0.0f ? (double)o->smooth_weight : kTunedOmega;
+    const double pscale = o->coarse_scale > 0.0f ? (double)o->coarse_scale : kTunedScale;
+    for (int p = 0; p < n; p++) {
+        mhdp_ctx* c = ctxs[p];
+        c->err.clear();
+        int rc = validate_opts(c, o);
+        if (!rc && !ustar[p]) rc = fail(c, MHDP_INVALID, "Missing candidate face velocities");
+        if (!rc && ((rc = dalloc(c, (void**)&c->x64, (size_t)c->g.n * 8)) ||
+                    (rc = dalloc(c, (void**)&c->b64, (size_t)c->g.n * 8)) || (rc = alloc_mg(c, true)))) {
+        }
+        PS& q = ps[(size_t)p];
+        q.S.meanfree = o->gauge != MHDP_GAUGE_PINNED && c->components == 1;
+        q.S.store = STORE_BF16;
+        q.S.coarse64 = true;
+        if (!rc) {
+            Scal sc;
+            set_solve_params(c, sc, q.S.meanfree, tau * tau, 0.0, 1.0, omega, pscale);
+            sc.refine_tol = o->refine_tol;
+            sc.tau = tau;
+            rc = upload_scal(c, sc);
+        }
+        rcs[(size_t)p] = rc;
+        q.running = rc == MHDP_OK;
+        q.timer = c->timer.kernel;
+        c->timer.kernel = -1;
+        c->timer.used = 0;
+    }
+    auto members = [&](auto pred) {
+        std::vector<int> m;
+        for (int p = 0; p < n; p++)
+            if (pred(ps[(size_t)p])) m.push_back(p);
+        return m;
+    };
+    BatchMemory mem;
+    hipEvent_t e0 = c0->ev0, e1 = c0->ev1;
+    if (hipEventRecord(e0, s) != hipSuccess) return MHDP_DEVICE;
+    // set-up: mobilities, diagonal, BF16 copies, FP64 RHS, Galerkin hierarchy
+    {
+        Plan plan;
+        auto run = members([](const PS& q) { return q.running; });
+        if (!record_plan(run, mem, plan, [&](int p) {
+                mhdp_ctx* c = ctxs[p];
+                PS& q = ps[(size_t)p];
+                prepare(c, q.S.meanfree, true);
+                launch_rhs_f64(s, c->g, c->grid.h, c->mask, c->bx, c->by, c->bz, ustar[p], c->b64);
+                build_mg(c, q.S.meanfree ? 1 : 0, true);
+                q.S.mg = true;
+                q.S.lv = level_view(c);
+                q.S.fused = true;
+            }))
+            return MHDP_NO_MEMORY;
+        issue_plan(s, plan);
+    }
+    std::vector<const Scal*> srcs((size_t)n);
+    for (int p = 0; p < n; p++) srcs[(size_t)p] = ctxs[p]->sc;
+    void* src_d = mem.get(sizeof(Scal*) * (size_t)n);
+    uint32_t* dst_d = static_cast<uint32_t*>(mem.get(sizeof(Scal) * (size_t)n));
+    if (!src_d || !dst_d) return MHDP_NO_MEMORY;
+    if (hipMemcpy(src_d, srcs.data(), sizeof(Scal*) * (size_t)n, hipMemcpyHostToDevice) != hipSuccess) return MHDP_DEVICE;
+    std::vector<Scal> sh((size_t)n);
+    auto read_all = [&]() -> bool {
+        go<KGatherScal>(s, grid_blocks((uint32_t)(n * sizeof(Scal) / 4)), {(const Scal* const*)src_d, dst_d, (uint32_t)n});
+        return hipMemcpyAsync(sh.data(), dst_d, sizeof(Scal) * (size_t)n, hipMemcpyDeviceToHost, s) == hipSuccess &&
+               hipStreamSynchronize(s) == hipSuccess;
+    };
+    auto fail_p = [&](int p, int code, const char* msg) {
+        rcs[(size_t)p] = fail(ctxs[p], code, msg);
+        ps[(size_t)p].running = false;
+    };
+    for (bool first = true;; first = false) {
+        auto run = members([](const PS& q) { return q.running; });
+        if (run.empty()) break;
+        Plan start;
+        if (!record_plan(run, mem, start, [&](int p) {
+                mhdp_ctx* c = ctxs[p];
+                launch_outer_start(s, c->g, c->x64, c->b64, c->bx, c->by, c->bz, c->dinv, c->r, c->x, c->p, c->p2,
+                                   c->part, c->sc, first);
+                mg_initialize(c, ps[(size_t)p].S);
+            }))
+            return MHDP_NO_MEMORY;
+        issue_plan(s, start);
+        for (int p : run) {
+            ps[(size_t)p].it = 0;
+            ps[(size_t)p].checked = false;
+            ps[(size_t)p].inner = true;
+        }
+        // inner solves: batches of 8 iterations over the problems still iterating
+        std::vector<int> iterating;
+        Plan iter8;
+        for (;;) {
+            auto now = members([](const PS& q) { return q.running && q.inner; });
+            if (now.empty()) break;
+            if (now != iterating) {
+                iterating = now;
+                uint32_t most = 0;
+                for (int p : now) most = std::max(most, ps[(size_t)p].it);
+                const uint32_t batch = std::min(8u, o->max_iter - most);
+                if (!record_plan(now, mem, iter8, [&](int p) {
+                        for (uint32_t k = 0; k < batch; k++) iterate_once(ctxs[p], ps[(size_t)p].S);
+                    }))
+                    return MHDP_NO_MEMORY;
+            }
+            issue_plan(s, iter8);
+            if (!read_all()) return MHDP_DEVICE;
+            for (int p : now) {
+                PS& q = ps[(size_t)p];
+                const Scal& v = sh[(size_t)p];
+                const double vals[10] = {v.rr64, v.bb64, v.rz, v.rr, v.bb, v.sr, v.alpha, v.beta, v.conv, v.iters};
+                bool bad = false;
+                for (double x : vals) bad |= !std::isfinite(x);
+                if (bad) {
+                    fail_p(p, MHDP_NONFINITE, "Nonfinite fluid pressure residual");
+                    continue;
+                }
+                if (!q.checked) {
+                    q.res64 = v.bb64 > 0.0 ? std::sqrt(v.rr64 / v.bb64) : 0.0;
+                    q.checked = true;
+                    if (v.outer_done) {
+                        q.done = true;
+                        q.running = false;
+                        q.inner = false;
+                        continue;
+                    }
+                    if (q.outer >= max_outer) {
+                        q.running = false;
+                        q.inner = false;
+                        continue;
+                    }
+                }
+                const bool conv = v.conv > 0.5;
+                const uint32_t seen = (uint32_t)v.iters;
+                if (seen <= q.it && !conv) {
+                    fail_p(p, MHDP_NO_PROGRESS, "Fluid pressure iteration made no progress");
+                    continue;
+                }
+                q.it = seen;
+                if (conv || q.it >= o->max_iter) q.inner = false;
+            }
+            // a problem stopped by max_iter must not iterate further: re-plan without it
+            auto left = members([](const PS& q) { return q.running && q.inner; });
+            bool changed = false;
+            for (int p : iterating)
+                if (std::find(left.begin(), left.end(), p) == left.end() && !(sh[(size_t)p].conv > 0.5)) changed = true;
+            if (changed) iterating.clear();
+        }
+        // fold the inner solutions into the FP64 potentials
+        auto fold = members([](const PS& q) { return q.running; });
+        if (fold.empty()) continue;
+        Plan upd;
+        if (!record_plan(fold, mem, upd, [&](int p) {
+                mhdp_ctx* c = ctxs[p];
+                launch_outer_update(s, c->g, c->x64, c->x, c->p, c->p2, c->dinv, c->sc, ps[(size_t)p].S.meanfree,
+                                    first);
+            }))
+            return MHDP_NO_MEMORY;
+        issue_plan(s, upd);
+        for (int p : fold) {
+            ps[(size_t)p].iterations += ps[(size_t)p].it;
+            ps[(size_t)p].outer++;
+        }
+    }
+    // outputs of the problems that reached the target
+    {
+        auto fin = members([](const PS& q) { return q.done; });
+        Plan plan;
+        if (!record_plan(fin, mem, plan, [&](int p) {
+                mhdp_ctx* c = ctxs[p];
+                if (ps[(size_t)p].outer == 0) launch_fill_f64(s, c->g.n, c->x64, 0.0);
+                if (uout && uout[p])
+                    launch_correct_f64(s, c->g, c->grid.h, c->bx, c->by, c->bz, c->x64, ustar[p], uout[p]);
+            }))
+            return MHDP_NO_MEMORY;
+        issue_plan(s, plan);
+        for (int p : fin)
+            if (phi && phi[p] &&
+                hipMemcpyAsync(phi[p], ctxs[p]->x64, (size_t)ctxs[p]->g.n * 8, hipMemcpyDeviceToDevice, s) != hipSuccess)
+                return MHDP_DEVICE;
+    }
+    if (hipEventRecord(e1, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) return MHDP_DEVICE;
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    for (int p = 0; p < n; p++) {
+        mhdp_ctx* c = ctxs[p];
+        PS& q = ps[(size_t)p];
+        c->timer.kernel = q.timer;
+        if (rcs[(size_t)p] != MHDP_OK) continue;
+        if (!q.done) {
+            char buf[256];
+            std::snprintf(buf, sizeof buf,
+                          "Fluid pressure solve did not converge after %u iterations (true relative residual %.3e, "
+                          "requested %.3e)",
+                          q.iterations, q.res64, o->refine_tol);
+            rcs[(size_t)p] = fail(c, MHDP_NOT_CONVERGED, buf);
+            continue;
+        }
+        if (st) {
+            mhdp_stats& t = st[p];
+            t.iterations = q.iterations;
+            t.outer_steps = q.outer;
+            t.components = c->components;
+            t.fluid_cells = c->fluid_cells;
+            t.levels = (uint32_t)c->levels.size();
+            t.meanfree = q.S.meanfree ? 1 : 0;
+            t.multigrid = 1;
+            t.rel_residual = (float)q.res64;
+            t.rel_residual_f64 = q.res64;
+            t.seconds = ms * 1e-3;
+        }
+    }
+    return hipGetLastError() == hipSuccess ? MHDP_OK : MHDP_DEVICE;
+}
+
 // MHDP_HOST_ANCHORS=1: label the gauge mask with the host BFS (comparison runs).
 bool host_anchors_env() {
     const char* e = std::getenv("MHDP_HOST_ANCHORS");
     return e && e[0] == '1';
+}
+
+// MHDP_BATCH_MODE=threads: mhdp_project_batch drives per-context streams from host threads.
+bool batch_threads_env() {
+    const char* e = std::getenv("MHDP_BATCH_MODE");
+    return e && std::strcmp(e, "threads") == 0;
 }
 
 // Host threads of mhdp_project_batch (MHDP_BATCH_THREADS, 1..64; default 8), read per call.
@@ -867,12 +1221,23 @@ int mhdp_project_batch(mhdp_ctx* const* ctxs, int count, const float* const* ust
     std::vector<const mhdp_ctx*> seen(ctxs, ctxs + count);
     std::sort(seen.begin(), seen.end());
     if (count && (!seen[0] || std::adjacent_find(seen.begin(), seen.end()) != seen.end())) return MHDP_INVALID;
-    // Each context runs on its own stream; a small pool of host threads drives them so the
-    // problems' kernels and host synchronisations overlap.  Largest grids start first.
+    std::vector<int> rcs((size_t)count, MHDP_OK);
+    // The fused multigrid path of the mixed solver runs in lockstep: one launch per kernel for
+    // all problems (MHDP_BATCH_MODE=threads keeps the per-context streams below).
+    bool lockstep = count > 0 && !batch_threads_env();
+    for (int k = 0; k < count && lockstep; k++) lockstep = lockstep_ok(ctxs[k], o);
+    if (lockstep) {
+        const int rc = project_lockstep(ctxs, count, ustar, o, uout, phi, st, rcs);
+        if (rc) return rc;
+        for (int k = 0; k < count; k++)
+            if (rcs[(size_t)k]) return rcs[(size_t)k];
+        return MHDP_OK;
+    }
+    // Otherwise each context runs on its own stream; a small pool of host threads drives them so
+    // the problems' kernels and host synchronisations overlap.  Largest grids start first.
     std::vector<int> order((size_t)count);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ctxs[a]->g.n > ctxs[b]->g.n; });
-    std::vector<int> rcs((size_t)count, MHDP_OK);
     int device = 0;
     if (hipGetDevice(&device) != hipSuccess) return MHDP_DEVICE;
     std::atomic<int> next{0};

@@ -6,6 +6,7 @@
 // drop from ~11 (one group of 4 cells) to ~4.  Arithmetic and summation order per cell are
 // those of fused.hip.
 #include "mhdp_kernels.h"
+#include "launch.h"
 #include "raw.h"
 
 #define CONVERGED(sc) ((sc)->conv > 0.5)
@@ -297,374 +298,473 @@ bool brick_enabled_env() {
 }
 
 // pass 1 (k_fused_apply on bricks)
-template <class VT, int MINB>
-__global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_apply(Grid3 g, const VT* __restrict__ z, float* p0, float* p1,
-                                                            float* __restrict__ x, const float* __restrict__ bx,
-                                                            const float* __restrict__ by, const float* __restrict__ bz,
-                                                            const float* __restrict__ dinv, float* __restrict__ q,
-                                                            double* __restrict__ part, Scal* sc, int meanfree) {
-    if (CONVERGED(sc)) return;
-    const uint32_t it = (uint32_t)sc->iters;
-    const float* __restrict__ pold = (it & 1u) ? p0 : p1;
-    float* __restrict__ pnew = (it & 1u) ? p1 : p0;
-    const float beta = (float)sc->beta;
-    const bool pending = sc->pending != 0;
-    const float alpha = (float)sc->alpha;
-    const float pa = (pending && meanfree) ? pold[sc->anchor] : 0.0f;
-    uint32_t lo, hi;
-    block_range(g.n >> 4, lo, hi);
-    double acc[2] = {0.0, 0.0};
-    for (uint32_t b = lo + threadIdx.x; b < hi; b += MHDP_BLOCK) {
-        const Geo o = geo(g, b);
-        Rows<VT> zr;
-        Rows<float> pr;
-        WRaw<float> wr;
-        R4<float> dr[2][2], xr[2][2];
-        load_rows(o, z, zr);
-        load_rows(o, pold, pr);
-        load_weights(o, wr, bx, by, bz);
-#pragma unroll
-        for (int dz = 0; dz < 2; dz++)
-#pragma unroll
-            for (int dy = 0; dy < 2; dy++) {
-                dr[dz][dy] = ldr4(dinv, o.row[dz][dy]);
-                xr[dz][dy] = ldr4(x, o.row[dz][dy]);
-            }
-        fence_rows(zr);
-        fence_rows(pr);
-        fence_weights(wr);
-#pragma unroll
-        for (int dz = 0; dz < 2; dz++)
-#pragma unroll
-            for (int dy = 0; dy < 2; dy++) {
-                fence(dr[dz][dy]);
-                fence(xr[dz][dy]);
-            }
-        Wts w;
-        weights(o, wr, w);
-        Field f;
-        field2(
-            zr, pr, f,
-            [&](const F4& a, const F4& c, int) {
-                return F4{{a.v[0] + beta * c.v[0], a.v[1] + beta * c.v[1], a.v[2] + beta * c.v[2], a.v[3] + beta * c.v[3]}};
-            },
-            [&](float a, float c, int) { return a + beta * c; });
-#pragma unroll
-        for (int dz = 0; dz < 2; dz++)
-#pragma unroll
-            for (int dy = 0; dy < 2; dy++) {
-                const uint32_t r = o.row[dz][dy];
-                const F4 di = un(dr[dz][dy]);
-                F4 qv, pn;
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    pn.v[k] = di.v[k] != 0.0f ? f.c[dz][dy][k] : 0.0f;
-                    qv.v[k] = act_at(w, f, dz, dy, k);
-                    acc[0] += (double)pn.v[k] * (double)qv.v[k];
-                    acc[1] += di.v[k] != 0.0f ? (double)qv.v[k] : 0.0;
+template <class VT, int MB_>
+struct K_k_brick_apply {
+    static constexpr int MINB = MB_;
+    struct Args {
+        Grid3 g;
+        const VT* z;
+        float* p0;
+        float* p1;
+        float* x;
+        const float* bx;
+        const float* by;
+        const float* bz;
+        const float* dinv;
+        float* q;
+        double* part;
+        Scal* sc;
+        int meanfree;
+    };
+    static __device__ void run(const Args& A_, const VGrid& vg) {
+        Grid3 g = A_.g;
+        const VT* __restrict__ z = A_.z;
+        float* p0 = A_.p0;
+        float* p1 = A_.p1;
+        float* __restrict__ x = A_.x;
+        const float* __restrict__ bx = A_.bx;
+        const float* __restrict__ by = A_.by;
+        const float* __restrict__ bz = A_.bz;
+        const float* __restrict__ dinv = A_.dinv;
+        float* __restrict__ q = A_.q;
+        double* __restrict__ part = A_.part;
+        Scal* sc = A_.sc;
+        int meanfree = A_.meanfree;
+        (void)vg;
+        if (CONVERGED(sc)) return;
+        const uint32_t it = (uint32_t)sc->iters;
+        const float* __restrict__ pold = (it & 1u) ? p0 : p1;
+        float* __restrict__ pnew = (it & 1u) ? p1 : p0;
+        const float beta = (float)sc->beta;
+        const bool pending = sc->pending != 0;
+        const float alpha = (float)sc->alpha;
+        const float pa = (pending && meanfree) ? pold[sc->anchor] : 0.0f;
+        uint32_t lo, hi;
+        block_range(vg, g.n >> 4, lo, hi);
+        double acc[2] = {0.0, 0.0};
+        for (uint32_t b = lo + threadIdx.x; b < hi; b += MHDP_BLOCK) {
+            const Geo o = geo(g, b);
+            Rows<VT> zr;
+            Rows<float> pr;
+            WRaw<float> wr;
+            R4<float> dr[2][2], xr[2][2];
+            load_rows(o, z, zr);
+            load_rows(o, pold, pr);
+            load_weights(o, wr, bx, by, bz);
+    #pragma unroll
+            for (int dz = 0; dz < 2; dz++)
+    #pragma unroll
+                for (int dy = 0; dy < 2; dy++) {
+                    dr[dz][dy] = ldr4(dinv, o.row[dz][dy]);
+                    xr[dz][dy] = ldr4(x, o.row[dz][dy]);
                 }
-                st4(pnew, r, pn);
-                st4(q, r, qv);
-                if (pending) {
-                    const F4 po = un(pr.c[dz][dy]);
-                    F4 xv = un(xr[dz][dy]);
-#pragma unroll
-                    for (int k = 0; k < 4; k++)
-                        if (di.v[k] != 0.0f) xv.v[k] = xv.v[k] + alpha * (po.v[k] - pa);
-                    st4(x, r, xv);
+            fence_rows(zr);
+            fence_rows(pr);
+            fence_weights(wr);
+    #pragma unroll
+            for (int dz = 0; dz < 2; dz++)
+    #pragma unroll
+                for (int dy = 0; dy < 2; dy++) {
+                    fence(dr[dz][dy]);
+                    fence(xr[dz][dy]);
                 }
-            }
+            Wts w;
+            weights(o, wr, w);
+            Field f;
+            field2(
+                zr, pr, f,
+                [&](const F4& a, const F4& c, int) {
+                    return F4{{a.v[0] + beta * c.v[0], a.v[1] + beta * c.v[1], a.v[2] + beta * c.v[2], a.v[3] + beta * c.v[3]}};
+                },
+                [&](float a, float c, int) { return a + beta * c; });
+    #pragma unroll
+            for (int dz = 0; dz < 2; dz++)
+    #pragma unroll
+                for (int dy = 0; dy < 2; dy++) {
+                    const uint32_t r = o.row[dz][dy];
+                    const F4 di = un(dr[dz][dy]);
+                    F4 qv, pn;
+    #pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        pn.v[k] = di.v[k] != 0.0f ? f.c[dz][dy][k] : 0.0f;
+                        qv.v[k] = act_at(w, f, dz, dy, k);
+                        acc[0] += (double)pn.v[k] * (double)qv.v[k];
+                        acc[1] += di.v[k] != 0.0f ? (double)qv.v[k] : 0.0;
+                    }
+                    st4(pnew, r, pn);
+                    st4(q, r, qv);
+                    if (pending) {
+                        const F4 po = un(pr.c[dz][dy]);
+                        F4 xv = un(xr[dz][dy]);
+    #pragma unroll
+                        for (int k = 0; k < 4; k++)
+                            if (di.v[k] != 0.0f) xv.v[k] = xv.v[k] + alpha * (po.v[k] - pa);
+                        st4(x, r, xv);
+                    }
+                }
+        }
+        reduce_finish<2>(vg, acc, part, sc, 2, FIN_ALPHA);
     }
-    reduce_finish<2>(acc, part, sc, 2, FIN_ALPHA);
-}
+};
 
 // pass 3 (k_fused_restrict on bricks): the two aggregates of a brick.  COARSE: a coarse level
 // (Galerkin graph with reaction term; every child contributes, multigrid.wgsl:75-83; no mean).
-template <class CT, class VT, class RT, bool COARSE, int MINB>
-__global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_restrict(Grid3 cg, Grid3 g, const RT* __restrict__ r,
-                                                                     const CT* __restrict__ dinv,
-                                                                     const CT* __restrict__ bx,
-                                                                     const CT* __restrict__ by,
-                                                                     const CT* __restrict__ bz,
-                                                                     const CT* __restrict__ react,
-                                                                     VT* __restrict__ rc, Scal* sc) {
-    if (CONVERGED(sc)) return;
-    const float omega = (float)sc->omega;  // smoother weight (reference: 2/3)
-    const float m = COARSE ? 0.0f : (float)sc->mean;
-    uint32_t lo, hi;
-    block_range(g.n >> 4, lo, hi);
-    for (uint32_t b = lo + threadIdx.x; b < hi; b += MHDP_BLOCK) {
-        const Geo o = geo(g, b);
-        Rows<RT> rr;
-        Rows<CT> dd;
-        WRaw<CT> wr;
-        R4<CT> er[2][2];
-        load_rows(o, r, rr);
-        load_rows(o, dinv, dd);
-        load_weights(o, wr, bx, by, bz);
-        if (COARSE) {
-#pragma unroll
-            for (int dz = 0; dz < 2; dz++)
-#pragma unroll
-                for (int dy = 0; dy < 2; dy++) er[dz][dy] = ldr4(react, o.row[dz][dy]);
-        }
-        fence_rows(rr);
-        fence_rows(dd);
-        fence_weights(wr);
-        if (COARSE) {
-#pragma unroll
-            for (int dz = 0; dz < 2; dz++)
-#pragma unroll
-                for (int dy = 0; dy < 2; dy++) fence(er[dz][dy]);
-        }
-        typename WtsOf<CT>::type w;
-        weights(o, wr, w);
-        Field f;
-        field2(
-            rr, dd, f,
-            [&](const F4& a, const F4& d, int) {
-                F4 o4;
-#pragma unroll
-                for (int k = 0; k < 4; k++) o4.v[k] = omega * (a.v[k] - m) * d.v[k];
-                return o4;
-            },
-            [&](float a, float d, int) { return omega * (a - m) * d; });
-        float s0 = 0.0f, s1 = 0.0f;
-#pragma unroll
-        for (int dz = 0; dz < 2; dz++)
-#pragma unroll
-            for (int dy = 0; dy < 2; dy++) {
-                const F4 rv = un(rr.c[dz][dy]), di = un(dd.c[dz][dy]);
-                F4 re{{0.f, 0.f, 0.f, 0.f}};
-                if (COARSE) re = un(er[dz][dy]);
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    float a = act_at(w, f, dz, dy, k);
-                    if (COARSE) a = re.v[k] * f.c[dz][dy][k] + a;
-                    float v = (rv.v[k] - m) - a;
-                    if (!COARSE) v = di.v[k] != 0.0f ? v : 0.0f;
-                    if (k < 2) s0 += v;
-                    else s1 += v;
-                }
+template <class CT, class VT, class RT, bool COARSE, int MB_>
+struct K_k_brick_restrict {
+    static constexpr int MINB = MB_;
+    struct Args {
+        Grid3 cg;
+        Grid3 g;
+        const RT* r;
+        const CT* dinv;
+        const CT* bx;
+        const CT* by;
+        const CT* bz;
+        const CT* react;
+        VT* rc;
+        Scal* sc;
+    };
+    static __device__ void run(const Args& A_, const VGrid& vg) {
+        Grid3 cg = A_.cg;
+        Grid3 g = A_.g;
+        const RT* __restrict__ r = A_.r;
+        const CT* __restrict__ dinv = A_.dinv;
+        const CT* __restrict__ bx = A_.bx;
+        const CT* __restrict__ by = A_.by;
+        const CT* __restrict__ bz = A_.bz;
+        const CT* __restrict__ react = A_.react;
+        VT* __restrict__ rc = A_.rc;
+        Scal* sc = A_.sc;
+        (void)vg;
+        if (CONVERGED(sc)) return;
+        const float omega = (float)sc->omega;  // smoother weight (reference: 2/3)
+        const float m = COARSE ? 0.0f : (float)sc->mean;
+        uint32_t lo, hi;
+        block_range(vg, g.n >> 4, lo, hi);
+        for (uint32_t b = lo + threadIdx.x; b < hi; b += MHDP_BLOCK) {
+            const Geo o = geo(g, b);
+            Rows<RT> rr;
+            Rows<CT> dd;
+            WRaw<CT> wr;
+            R4<CT> er[2][2];
+            load_rows(o, r, rr);
+            load_rows(o, dinv, dd);
+            load_weights(o, wr, bx, by, bz);
+            if (COARSE) {
+    #pragma unroll
+                for (int dz = 0; dz < 2; dz++)
+    #pragma unroll
+                    for (int dy = 0; dy < 2; dy++) er[dz][dy] = ldr4(react, o.row[dz][dy]);
             }
-        const uint32_t ci = (uint32_t)(o.x0 >> 1) + (uint32_t)cg.nx * ((uint32_t)(o.y0 >> 1) + (uint32_t)cg.ny * (uint32_t)(o.z0 >> 1));
-        st2(rc, ci, s0, s1);
+            fence_rows(rr);
+            fence_rows(dd);
+            fence_weights(wr);
+            if (COARSE) {
+    #pragma unroll
+                for (int dz = 0; dz < 2; dz++)
+    #pragma unroll
+                    for (int dy = 0; dy < 2; dy++) fence(er[dz][dy]);
+            }
+            typename WtsOf<CT>::type w;
+            weights(o, wr, w);
+            Field f;
+            field2(
+                rr, dd, f,
+                [&](const F4& a, const F4& d, int) {
+                    F4 o4;
+    #pragma unroll
+                    for (int k = 0; k < 4; k++) o4.v[k] = omega * (a.v[k] - m) * d.v[k];
+                    return o4;
+                },
+                [&](float a, float d, int) { return omega * (a - m) * d; });
+            float s0 = 0.0f, s1 = 0.0f;
+    #pragma unroll
+            for (int dz = 0; dz < 2; dz++)
+    #pragma unroll
+                for (int dy = 0; dy < 2; dy++) {
+                    const F4 rv = un(rr.c[dz][dy]), di = un(dd.c[dz][dy]);
+                    F4 re{{0.f, 0.f, 0.f, 0.f}};
+                    if (COARSE) re = un(er[dz][dy]);
+    #pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        float a = act_at(w, f, dz, dy, k);
+                        if (COARSE) a = re.v[k] * f.c[dz][dy][k] + a;
+                        float v = (rv.v[k] - m) - a;
+                        if (!COARSE) v = di.v[k] != 0.0f ? v : 0.0f;
+                        if (k < 2) s0 += v;
+                        else s1 += v;
+                    }
+                }
+            const uint32_t ci = (uint32_t)(o.x0 >> 1) + (uint32_t)cg.nx * ((uint32_t)(o.y0 >> 1) + (uint32_t)cg.ny * (uint32_t)(o.z0 >> 1));
+            st2(rc, ci, s0, s1);
+        }
     }
-}
+};
 
 // passes 2+3 in one: the residual update r' = (r - mean) - alpha q (k_fused_update's
 // arithmetic) evaluated on the brick and its halo, r' stored for the brick's own cells in the
 // twin buffer, and the restriction of the pre-smoothed residual of r' - mean'; mean' is the
 // mean of r' known from the apply pass.  Partials (0, sum r'^2, 0, sum r') -> FIN_RESTRICT_UPD.
-template <class CT, class VT, int MINB>
-__global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_restrict_update(Grid3 cg, Grid3 g, float* r0, float* r1,
-                                                                            const float* __restrict__ q,
-                                                                            const CT* __restrict__ dinv,
-                                                                            const CT* __restrict__ bx,
-                                                                            const CT* __restrict__ by,
-                                                                            const CT* __restrict__ bz,
-                                                                            VT* __restrict__ rc,
-                                                                            double* __restrict__ part, Scal* sc) {
-    if (CONVERGED(sc)) return;
-    const float omega = (float)sc->omega;
-    const float alpha = (float)sc->alpha;
-    const float m = (float)sc->mean;        // mean of r (deferred projection)
-    const float mn = (float)sc->mean_next;  // mean of r'
-    const float* __restrict__ rin = sc->rsel ? r1 : r0;
-    float* __restrict__ rout = sc->rsel ? r0 : r1;
-    uint32_t lo, hi;
-    block_range(g.n >> 4, lo, hi);
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (uint32_t b = lo + threadIdx.x; b < hi; b += MHDP_BLOCK) {
-        const Geo o = geo(g, b);
-        Rows<float> rr, qq;
-        Rows<CT> dd;
-        WRaw<CT> wr;
-        load_rows(o, rin, rr);
-        load_rows(o, q, qq);
-        load_rows(o, dinv, dd);
-        load_weights(o, wr, bx, by, bz);
-        fence_rows(rr);
-        fence_rows(qq);
-        fence_rows(dd);
-        fence_weights(wr);
-        typename WtsOf<CT>::type w;
-        weights(o, wr, w);
-        // r' on the brick and its halo; the pre-smoothed z1 = omega (r' - mean') / diag
-        Field rn, f;
-        field2(
-            rr, qq, rn,
-            [&](const F4& a, const F4& c, int) {
-                F4 o4;
-#pragma unroll
-                for (int k = 0; k < 4; k++) o4.v[k] = (a.v[k] - m) - alpha * c.v[k];
-                return o4;
-            },
-            [&](float a, float c, int) { return (a - m) - alpha * c; });
-        {
-            Rows<CT>& d = dd;
-#pragma unroll
-            for (int dz = 0; dz < 2; dz++)
-#pragma unroll
-                for (int dy = 0; dy < 2; dy++) {
-                    const F4 dv = un(d.c[dz][dy]);
-#pragma unroll
-                    for (int k = 0; k < 4; k++) {
-                        rn.c[dz][dy][k] = dv.v[k] != 0.0f ? rn.c[dz][dy][k] : 0.0f;
-                        f.c[dz][dy][k] = omega * (rn.c[dz][dy][k] - mn) * dv.v[k];
+template <class CT, class VT, int MB_>
+struct K_k_brick_restrict_update {
+    static constexpr int MINB = MB_;
+    struct Args {
+        Grid3 cg;
+        Grid3 g;
+        float* r0;
+        float* r1;
+        const float* q;
+        const CT* dinv;
+        const CT* bx;
+        const CT* by;
+        const CT* bz;
+        VT* rc;
+        double* part;
+        Scal* sc;
+    };
+    static __device__ void run(const Args& A_, const VGrid& vg) {
+        Grid3 cg = A_.cg;
+        Grid3 g = A_.g;
+        float* r0 = A_.r0;
+        float* r1 = A_.r1;
+        const float* __restrict__ q = A_.q;
+        const CT* __restrict__ dinv = A_.dinv;
+        const CT* __restrict__ bx = A_.bx;
+        const CT* __restrict__ by = A_.by;
+        const CT* __restrict__ bz = A_.bz;
+        VT* __restrict__ rc = A_.rc;
+        double* __restrict__ part = A_.part;
+        Scal* sc = A_.sc;
+        (void)vg;
+        if (CONVERGED(sc)) return;
+        const float omega = (float)sc->omega;
+        const float alpha = (float)sc->alpha;
+        const float m = (float)sc->mean;        // mean of r (deferred projection)
+        const float mn = (float)sc->mean_next;  // mean of r'
+        const float* __restrict__ rin = sc->rsel ? r1 : r0;
+        float* __restrict__ rout = sc->rsel ? r0 : r1;
+        uint32_t lo, hi;
+        block_range(vg, g.n >> 4, lo, hi);
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (uint32_t b = lo + threadIdx.x; b < hi; b += MHDP_BLOCK) {
+            const Geo o = geo(g, b);
+            Rows<float> rr, qq;
+            Rows<CT> dd;
+            WRaw<CT> wr;
+            load_rows(o, rin, rr);
+            load_rows(o, q, qq);
+            load_rows(o, dinv, dd);
+            load_weights(o, wr, bx, by, bz);
+            fence_rows(rr);
+            fence_rows(qq);
+            fence_rows(dd);
+            fence_weights(wr);
+            typename WtsOf<CT>::type w;
+            weights(o, wr, w);
+            // r' on the brick and its halo; the pre-smoothed z1 = omega (r' - mean') / diag
+            Field rn, f;
+            field2(
+                rr, qq, rn,
+                [&](const F4& a, const F4& c, int) {
+                    F4 o4;
+    #pragma unroll
+                    for (int k = 0; k < 4; k++) o4.v[k] = (a.v[k] - m) - alpha * c.v[k];
+                    return o4;
+                },
+                [&](float a, float c, int) { return (a - m) - alpha * c; });
+            {
+                Rows<CT>& d = dd;
+    #pragma unroll
+                for (int dz = 0; dz < 2; dz++)
+    #pragma unroll
+                    for (int dy = 0; dy < 2; dy++) {
+                        const F4 dv = un(d.c[dz][dy]);
+    #pragma unroll
+                        for (int k = 0; k < 4; k++) {
+                            rn.c[dz][dy][k] = dv.v[k] != 0.0f ? rn.c[dz][dy][k] : 0.0f;
+                            f.c[dz][dy][k] = omega * (rn.c[dz][dy][k] - mn) * dv.v[k];
+                        }
+                        const float dxm = un(d.xm[dz][dy]), dxp = un(d.xp[dz][dy]);
+                        f.xm[dz][dy] = omega * ((dxm != 0.0f ? rn.xm[dz][dy] : 0.0f) - mn) * dxm;
+                        f.xp[dz][dy] = omega * ((dxp != 0.0f ? rn.xp[dz][dy] : 0.0f) - mn) * dxp;
                     }
-                    const float dxm = un(d.xm[dz][dy]), dxp = un(d.xp[dz][dy]);
-                    f.xm[dz][dy] = omega * ((dxm != 0.0f ? rn.xm[dz][dy] : 0.0f) - mn) * dxm;
-                    f.xp[dz][dy] = omega * ((dxp != 0.0f ? rn.xp[dz][dy] : 0.0f) - mn) * dxp;
-                }
-#pragma unroll
-            for (int e = 0; e < 2; e++) {
-                const F4 a = un(d.ym[e]), b2 = un(d.yp[e]), c = un(d.zm[e]), h = un(d.zp[e]);
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    f.ym[e][k] = omega * ((a.v[k] != 0.0f ? rn.ym[e][k] : 0.0f) - mn) * a.v[k];
-                    f.yp[e][k] = omega * ((b2.v[k] != 0.0f ? rn.yp[e][k] : 0.0f) - mn) * b2.v[k];
-                    f.zm[e][k] = omega * ((c.v[k] != 0.0f ? rn.zm[e][k] : 0.0f) - mn) * c.v[k];
-                    f.zp[e][k] = omega * ((h.v[k] != 0.0f ? rn.zp[e][k] : 0.0f) - mn) * h.v[k];
+    #pragma unroll
+                for (int e = 0; e < 2; e++) {
+                    const F4 a = un(d.ym[e]), b2 = un(d.yp[e]), c = un(d.zm[e]), h = un(d.zp[e]);
+    #pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        f.ym[e][k] = omega * ((a.v[k] != 0.0f ? rn.ym[e][k] : 0.0f) - mn) * a.v[k];
+                        f.yp[e][k] = omega * ((b2.v[k] != 0.0f ? rn.yp[e][k] : 0.0f) - mn) * b2.v[k];
+                        f.zm[e][k] = omega * ((c.v[k] != 0.0f ? rn.zm[e][k] : 0.0f) - mn) * c.v[k];
+                        f.zp[e][k] = omega * ((h.v[k] != 0.0f ? rn.zp[e][k] : 0.0f) - mn) * h.v[k];
+                    }
                 }
             }
+            float s0 = 0.0f, s1 = 0.0f;
+    #pragma unroll
+            for (int dz = 0; dz < 2; dz++)
+    #pragma unroll
+                for (int dy = 0; dy < 2; dy++) {
+                    const F4 di = un(dd.c[dz][dy]);
+                    F4 ro;
+    #pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const float rk = rn.c[dz][dy][k];
+                        ro.v[k] = rk;
+                        acc[1] += (double)rk * (double)rk;
+                        acc[3] += (double)rk;
+                        const float a = act_at(w, f, dz, dy, k);
+                        float v = (rk - mn) - a;
+                        v = di.v[k] != 0.0f ? v : 0.0f;
+                        if (k < 2) s0 += v;
+                        else s1 += v;
+                    }
+                    st4(rout, o.row[dz][dy], ro);
+                }
+            const uint32_t ci = (uint32_t)(o.x0 >> 1) + (uint32_t)cg.nx * ((uint32_t)(o.y0 >> 1) + (uint32_t)cg.ny * (uint32_t)(o.z0 >> 1));
+            st2(rc, ci, s0, s1);
         }
-        float s0 = 0.0f, s1 = 0.0f;
-#pragma unroll
-        for (int dz = 0; dz < 2; dz++)
-#pragma unroll
-            for (int dy = 0; dy < 2; dy++) {
-                const F4 di = un(dd.c[dz][dy]);
-                F4 ro;
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const float rk = rn.c[dz][dy][k];
-                    ro.v[k] = rk;
-                    acc[1] += (double)rk * (double)rk;
-                    acc[3] += (double)rk;
-                    const float a = act_at(w, f, dz, dy, k);
-                    float v = (rk - mn) - a;
-                    v = di.v[k] != 0.0f ? v : 0.0f;
-                    if (k < 2) s0 += v;
-                    else s1 += v;
-                }
-                st4(rout, o.row[dz][dy], ro);
-            }
-        const uint32_t ci = (uint32_t)(o.x0 >> 1) + (uint32_t)cg.nx * ((uint32_t)(o.y0 >> 1) + (uint32_t)cg.ny * (uint32_t)(o.z0 >> 1));
-        st2(rc, ci, s0, s1);
+        reduce_finish<4>(vg, acc, part, sc, 3, FIN_RESTRICT_UPD);
     }
-    reduce_finish<4>(acc, part, sc, 3, FIN_RESTRICT_UPD);
-}
+};
 
 // pass 4 (k_fused_smooth on bricks).  COARSE: coarse level (reaction term, no mean, no r.z).
-template <class CT, class VT, class RT, bool COARSE, int MINB>
-__global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_brick_smooth(Grid3 g, Grid3 cg, const RT* r0, const RT* r1,
-                                                                   const CT* __restrict__ dinv,
-                                                                   const CT* __restrict__ bx,
-                                                                   const CT* __restrict__ by,
-                                                                   const CT* __restrict__ bz,
-                                                                   const CT* __restrict__ react,
-                                                                   const VT* __restrict__ zc, VT* __restrict__ t,
-                                                                   double* __restrict__ part, Scal* sc, int mode,
-                                                                   int use_next) {
-    if (CONVERGED(sc)) return;
-    const float omega = (float)sc->omega;  // smoother weight (reference: 2/3)
-    const float m = COARSE ? 0.0f : (float)(use_next ? sc->mean_next : sc->mean);
-    const float ps = (float)sc->pscale;  // coarse-correction scale (reference: 1)
-    const RT* __restrict__ r = (!COARSE && sc->rsel) ? r1 : r0;
-    const uint32_t csy = (uint32_t)cg.nx, csz = (uint32_t)cg.nx * (uint32_t)cg.ny;
-    uint32_t lo, hi;
-    block_range(g.n >> 4, lo, hi);
-    double acc[1] = {0.0};
-    for (uint32_t b = lo + threadIdx.x; b < hi; b += MHDP_BLOCK) {
-        const Geo o = geo(g, b);
-        Rows<RT> rr;
-        Rows<CT> dd;
-        WRaw<CT> wr;
-        R4<CT> er[2][2];
-        load_rows(o, r, rr);
-        load_rows(o, dinv, dd);
-        load_weights(o, wr, bx, by, bz);
-        if (COARSE) {
-#pragma unroll
-            for (int dz = 0; dz < 2; dz++)
-#pragma unroll
-                for (int dy = 0; dy < 2; dy++) er[dz][dy] = ldr4(react, o.row[dz][dy]);
-        }
-        // coarse corrections: one aggregate pair per row group (the brick's rows share parents)
-        const uint32_t cx = (uint32_t)(o.x0 >> 1), cy = (uint32_t)(o.y0 >> 1), cz = (uint32_t)(o.z0 >> 1);
-        R2<VT> zcore = ldr2(zc, cx + csy * cy + csz * cz);
-        R2<VT> zym = ldr2(zc, cx + csy * (uint32_t)(o.ymy >> 1) + csz * cz);
-        R2<VT> zyp = ldr2(zc, cx + csy * (uint32_t)(o.ypy >> 1) + csz * cz);
-        R2<VT> zzm = ldr2(zc, cx + csy * cy + csz * (uint32_t)(o.zmz >> 1));
-        R2<VT> zzp = ldr2(zc, cx + csy * cy + csz * (uint32_t)(o.zpz >> 1));
-        R1<VT> zxm = ldr1(zc, (uint32_t)(o.xmx >> 1) + csy * cy + csz * cz);
-        R1<VT> zxp = ldr1(zc, (uint32_t)(o.xpx >> 1) + csy * cy + csz * cz);
-        fence_rows(rr);
-        fence_rows(dd);
-        fence_weights(wr);
-        if (COARSE) {
-#pragma unroll
-            for (int dz = 0; dz < 2; dz++)
-#pragma unroll
-                for (int dy = 0; dy < 2; dy++) fence(er[dz][dy]);
-        }
-        fence(zcore);
-        fence(zym);
-        fence(zyp);
-        fence(zzm);
-        fence(zzp);
-        fence(zxm);
-        fence(zxp);
-        float cc[5][2];
-        un(zcore, cc[0][0], cc[0][1]);
-        un(zym, cc[1][0], cc[1][1]);
-        un(zyp, cc[2][0], cc[2][1]);
-        un(zzm, cc[3][0], cc[3][1]);
-        un(zzp, cc[4][0], cc[4][1]);
-        const float cxm = un(zxm), cxp = un(zxp);
-        typename WtsOf<CT>::type w;
-        weights(o, wr, w);
-        Field f;
-        field2(
-            rr, dd, f,
-            [&](const F4& a, const F4& d, int id) {
-                const int grp = id < 4 ? 0 : (id < 6 ? 1 : (id < 8 ? 2 : (id < 10 ? 3 : 4)));
-                const float c0 = cc[grp][0], c1 = cc[grp][1];
-                F4 o4;
-#pragma unroll
-                for (int k = 0; k < 4; k++)
-                    o4.v[k] = d.v[k] != 0.0f ? omega * (a.v[k] - m) * d.v[k] + ps * (k < 2 ? c0 : c1) : 0.0f;
-                return o4;
-            },
-            [&](float a, float d, int id) {
-                const float v = omega * (a - m) * d + ps * (id < 4 ? cxm : cxp);
-                return d != 0.0f ? v : 0.0f;
-            });
-#pragma unroll
-        for (int dz = 0; dz < 2; dz++)
-#pragma unroll
-            for (int dy = 0; dy < 2; dy++) {
-                const uint32_t row = o.row[dz][dy];
-                const F4 rv = un(rr.c[dz][dy]), di = un(dd.c[dz][dy]);
-                F4 re{{0.f, 0.f, 0.f, 0.f}};
-                if (COARSE) re = un(er[dz][dy]);
-                F4 tv;
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const float rk = rv.v[k] - m;
-                    float a = act_at(w, f, dz, dy, k);
-                    if (COARSE) a = re.v[k] * f.c[dz][dy][k] + a;
-                    const float out = f.c[dz][dy][k] + omega * (rk - a) * di.v[k];
-                    const bool on = di.v[k] != 0.0f;
-                    tv.v[k] = on ? out : 0.0f;
-                    if (!COARSE) acc[0] += on ? (double)rk * (double)out : 0.0;
-                }
-                st4(t, row, tv);
+template <class CT, class VT, class RT, bool COARSE, int MB_>
+struct K_k_brick_smooth {
+    static constexpr int MINB = MB_;
+    struct Args {
+        Grid3 g;
+        Grid3 cg;
+        const RT* r0;
+        const RT* r1;
+        const CT* dinv;
+        const CT* bx;
+        const CT* by;
+        const CT* bz;
+        const CT* react;
+        const VT* zc;
+        VT* t;
+        double* part;
+        Scal* sc;
+        int mode;
+        int use_next;
+    };
+    static __device__ void run(const Args& A_, const VGrid& vg) {
+        Grid3 g = A_.g;
+        Grid3 cg = A_.cg;
+        const RT* r0 = A_.r0;
+        const RT* r1 = A_.r1;
+        const CT* __restrict__ dinv = A_.dinv;
+        const CT* __restrict__ bx = A_.bx;
+        const CT* __restrict__ by = A_.by;
+        const CT* __restrict__ bz = A_.bz;
+        const CT* __restrict__ react = A_.react;
+        const VT* __restrict__ zc = A_.zc;
+        VT* __restrict__ t = A_.t;
+        double* __restrict__ part = A_.part;
+        Scal* sc = A_.sc;
+        int mode = A_.mode;
+        int use_next = A_.use_next;
+        (void)vg;
+        if (CONVERGED(sc)) return;
+        const float omega = (float)sc->omega;  // smoother weight (reference: 2/3)
+        const float m = COARSE ? 0.0f : (float)(use_next ? sc->mean_next : sc->mean);
+        const float ps = (float)sc->pscale;  // coarse-correction scale (reference: 1)
+        const RT* __restrict__ r = (!COARSE && sc->rsel) ? r1 : r0;
+        const uint32_t csy = (uint32_t)cg.nx, csz = (uint32_t)cg.nx * (uint32_t)cg.ny;
+        uint32_t lo, hi;
+        block_range(vg, g.n >> 4, lo, hi);
+        double acc[1] = {0.0};
+        for (uint32_t b = lo + threadIdx.x; b < hi; b += MHDP_BLOCK) {
+            const Geo o = geo(g, b);
+            Rows<RT> rr;
+            Rows<CT> dd;
+            WRaw<CT> wr;
+            R4<CT> er[2][2];
+            load_rows(o, r, rr);
+            load_rows(o, dinv, dd);
+            load_weights(o, wr, bx, by, bz);
+            if (COARSE) {
+    #pragma unroll
+                for (int dz = 0; dz < 2; dz++)
+    #pragma unroll
+                    for (int dy = 0; dy < 2; dy++) er[dz][dy] = ldr4(react, o.row[dz][dy]);
             }
+            // coarse corrections: one aggregate pair per row group (the brick's rows share parents)
+            const uint32_t cx = (uint32_t)(o.x0 >> 1), cy = (uint32_t)(o.y0 >> 1), cz = (uint32_t)(o.z0 >> 1);
+            R2<VT> zcore = ldr2(zc, cx + csy * cy + csz * cz);
+            R2<VT> zym = ldr2(zc, cx + csy * (uint32_t)(o.ymy >> 1) + csz * cz);
+            R2<VT> zyp = ldr2(zc, cx + csy * (uint32_t)(o.ypy >> 1) + csz * cz);
+            R2<VT> zzm = ldr2(zc, cx + csy * cy + csz * (uint32_t)(o.zmz >> 1));
+            R2<VT> zzp = ldr2(zc, cx + csy * cy + csz * (uint32_t)(o.zpz >> 1));
+            R1<VT> zxm = ldr1(zc, (uint32_t)(o.xmx >> 1) + csy * cy + csz * cz);
+            R1<VT> zxp = ldr1(zc, (uint32_t)(o.xpx >> 1) + csy * cy + csz * cz);
+            fence_rows(rr);
+            fence_rows(dd);
+            fence_weights(wr);
+            if (COARSE) {
+    #pragma unroll
+                for (int dz = 0; dz < 2; dz++)
+    #pragma unroll
+                    for (int dy = 0; dy < 2; dy++) fence(er[dz][dy]);
+            }
+            fence(zcore);
+            fence(zym);
+            fence(zyp);
+            fence(zzm);
+            fence(zzp);
+            fence(zxm);
+            fence(zxp);
+            float cc[5][2];
+            un(zcore, cc[0][0], cc[0][1]);
+            un(zym, cc[1][0], cc[1][1]);
+            un(zyp, cc[2][0], cc[2][1]);
+            un(zzm, cc[3][0], cc[3][1]);
+            un(zzp, cc[4][0], cc[4][1]);
+            const float cxm = un(zxm), cxp = un(zxp);
+            typename WtsOf<CT>::type w;
+            weights(o, wr, w);
+            Field f;
+            field2(
+                rr, dd, f,
+                [&](const F4& a, const F4& d, int id) {
+                    const int grp = id < 4 ? 0 : (id < 6 ? 1 : (id < 8 ? 2 : (id < 10 ? 3 : 4)));
+                    const float c0 = cc[grp][0], c1 = cc[grp][1];
+                    F4 o4;
+    #pragma unroll
+                    for (int k = 0; k < 4; k++)
+                        o4.v[k] = d.v[k] != 0.0f ? omega * (a.v[k] - m) * d.v[k] + ps * (k < 2 ? c0 : c1) : 0.0f;
+                    return o4;
+                },
+                [&](float a, float d, int id) {
+                    const float v = omega * (a - m) * d + ps * (id < 4 ? cxm : cxp);
+                    return d != 0.0f ? v : 0.0f;
+                });
+    #pragma unroll
+            for (int dz = 0; dz < 2; dz++)
+    #pragma unroll
+                for (int dy = 0; dy < 2; dy++) {
+                    const uint32_t row = o.row[dz][dy];
+                    const F4 rv = un(rr.c[dz][dy]), di = un(dd.c[dz][dy]);
+                    F4 re{{0.f, 0.f, 0.f, 0.f}};
+                    if (COARSE) re = un(er[dz][dy]);
+                    F4 tv;
+    #pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const float rk = rv.v[k] - m;
+                        float a = act_at(w, f, dz, dy, k);
+                        if (COARSE) a = re.v[k] * f.c[dz][dy][k] + a;
+                        const float out = f.c[dz][dy][k] + omega * (rk - a) * di.v[k];
+                        const bool on = di.v[k] != 0.0f;
+                        tv.v[k] = on ? out : 0.0f;
+                        if (!COARSE) acc[0] += on ? (double)rk * (double)out : 0.0;
+                    }
+                    st4(t, row, tv);
+                }
+        }
+        if constexpr (!COARSE) reduce_finish<1>(vg, acc, part, sc, 4, mode);
     }
-    if constexpr (!COARSE) reduce_finish<1>(acc, part, sc, 4, mode);
-}
+};
 
 // ---------------------------------------------------------------- launchers
 // Minimum resident blocks per CU the register allocation targets (MHDP_BRICK_OCC: 2, 3, 4).
@@ -679,9 +779,9 @@ static int brick_occ() {
 #define BRICK_DISPATCH(KERNEL, TARGS, ...)                                                          \
     do {                                                                                          \
         switch (brick_occ()) {                                                                    \
-        case 4: hipLaunchKernelGGL((KERNEL<TARGS, 4>), G, B, 0, s, __VA_ARGS__); break;             \
-        case 3: hipLaunchKernelGGL((KERNEL<TARGS, 3>), G, B, 0, s, __VA_ARGS__); break;             \
-        default: hipLaunchKernelGGL((KERNEL<TARGS, 2>), G, B, 0, s, __VA_ARGS__); break;            \
+        case 4: go<K_##KERNEL<TARGS, 4>>(s, G.x, {__VA_ARGS__}); break;             \
+        case 3: go<K_##KERNEL<TARGS, 3>>(s, G.x, {__VA_ARGS__}); break;             \
+        default: go<K_##KERNEL<TARGS, 2>>(s, G.x, {__VA_ARGS__}); break;            \
         }                                                                                         \
     } while (0)
 

@@ -18,6 +18,7 @@
 // Vector path: each thread owns V = 4 consecutive x cells (nx % 4 == 0), so FP32 arrays
 // move as 16-byte and BF16 arrays as 8-byte loads; V = 1 serves any other grid.
 #include "mhdp_kernels.h"
+#include "launch.h"
 
 #define CONVERGED(sc) ((sc)->conv > 0.5)
 
@@ -187,239 +188,352 @@ __device__ __forceinline__ float star_action(const W6<V, CT>& W, const Star<V>& 
 // q = A p_k (FP32 operator); partial p.q -> FIN_ALPHA.  p buffers alternate with the
 // iteration count held on the device (iters even: write p0).
 template <int V, class VT>
-__global__ void __launch_bounds__(MHDP_BLOCK) k_fused_apply(Grid3 g, const VT* __restrict__ z, float* p0, float* p1,
-                                                            float* __restrict__ x, const float* __restrict__ bx,
-                                                            const float* __restrict__ by, const float* __restrict__ bz,
-                                                            const float* __restrict__ dinv, float* __restrict__ q,
-                                                            double* __restrict__ part, Scal* sc, int meanfree) {
-    if (CONVERGED(sc)) return;
-    const uint32_t it = (uint32_t)sc->iters;
-    const float* __restrict__ pold = (it & 1u) ? p0 : p1;
-    float* __restrict__ pnew = (it & 1u) ? p1 : p0;
-    const float beta = (float)sc->beta;
-    const bool pending = sc->pending != 0;
-    const float alpha = (float)sc->alpha;
-    const float pa = (pending && meanfree) ? pold[sc->anchor] : 0.0f;
-    uint32_t lo, hi;
-    block_range(g.n / V, lo, hi);
-    double acc[1] = {0.0};
-    for (uint32_t gi = lo + threadIdx.x; gi < hi; gi += MHDP_BLOCK) {
-        const uint32_t i0 = gi * V;
-        int x0, y, zz;
-        decode(g, i0, x0, y, zz);
-        const Nb n = neighbours<V>(g, x0, y, zz, i0);
-        const W6<V, float> W = weights<V, float>(n, i0, bx, by, bz);
-        auto pv = [&](uint32_t j) {
-            const VF<V> a = ldv<V>(z, j), b = ldv<V>(pold, j);
-            VF<V> o;
-#pragma unroll
-            for (int k = 0; k < V; k++) o.v[k] = a.v[k] + beta * b.v[k];
-            return o;
-        };
-        const float pxm = ldf(z[n.xm]) + beta * pold[n.xm];
-        const float pxp = ldf(z[n.xp]) + beta * pold[n.xp];
-        const Star<V> s = gather<V>(n, i0, pv, pxm, pxp);
-        const VF<V> di = ldv<V>(dinv, i0);
-        VF<V> qv, pn;
-#pragma unroll
-        for (int k = 0; k < V; k++) {
-            const bool act = di.v[k] != 0.0f;
-            pn.v[k] = act ? s.c[k] : 0.0f;
-            qv.v[k] = star_action<V, float>(W, s, k);
-            acc[0] += (double)pn.v[k] * (double)qv.v[k];
+struct K_k_fused_apply {
+    static constexpr int MINB = 0;
+    struct Args {
+        Grid3 g;
+        const VT* z;
+        float* p0;
+        float* p1;
+        float* x;
+        const float* bx;
+        const float* by;
+        const float* bz;
+        const float* dinv;
+        float* q;
+        double* part;
+        Scal* sc;
+        int meanfree;
+    };
+    static __device__ void run(const Args& A_, const VGrid& vg) {
+        Grid3 g = A_.g;
+        const VT* __restrict__ z = A_.z;
+        float* p0 = A_.p0;
+        float* p1 = A_.p1;
+        float* __restrict__ x = A_.x;
+        const float* __restrict__ bx = A_.bx;
+        const float* __restrict__ by = A_.by;
+        const float* __restrict__ bz = A_.bz;
+        const float* __restrict__ dinv = A_.dinv;
+        float* __restrict__ q = A_.q;
+        double* __restrict__ part = A_.part;
+        Scal* sc = A_.sc;
+        int meanfree = A_.meanfree;
+        (void)vg;
+        if (CONVERGED(sc)) return;
+        const uint32_t it = (uint32_t)sc->iters;
+        const float* __restrict__ pold = (it & 1u) ? p0 : p1;
+        float* __restrict__ pnew = (it & 1u) ? p1 : p0;
+        const float beta = (float)sc->beta;
+        const bool pending = sc->pending != 0;
+        const float alpha = (float)sc->alpha;
+        const float pa = (pending && meanfree) ? pold[sc->anchor] : 0.0f;
+        uint32_t lo, hi;
+        block_range(vg, g.n / V, lo, hi);
+        double acc[1] = {0.0};
+        for (uint32_t gi = lo + threadIdx.x; gi < hi; gi += MHDP_BLOCK) {
+            const uint32_t i0 = gi * V;
+            int x0, y, zz;
+            decode(g, i0, x0, y, zz);
+            const Nb n = neighbours<V>(g, x0, y, zz, i0);
+            const W6<V, float> W = weights<V, float>(n, i0, bx, by, bz);
+            auto pv = [&](uint32_t j) {
+                const VF<V> a = ldv<V>(z, j), b = ldv<V>(pold, j);
+                VF<V> o;
+    #pragma unroll
+                for (int k = 0; k < V; k++) o.v[k] = a.v[k] + beta * b.v[k];
+                return o;
+            };
+            const float pxm = ldf(z[n.xm]) + beta * pold[n.xm];
+            const float pxp = ldf(z[n.xp]) + beta * pold[n.xp];
+            const Star<V> s = gather<V>(n, i0, pv, pxm, pxp);
+            const VF<V> di = ldv<V>(dinv, i0);
+            VF<V> qv, pn;
+    #pragma unroll
+            for (int k = 0; k < V; k++) {
+                const bool act = di.v[k] != 0.0f;
+                pn.v[k] = act ? s.c[k] : 0.0f;
+                qv.v[k] = star_action<V, float>(W, s, k);
+                acc[0] += (double)pn.v[k] * (double)qv.v[k];
+            }
+            stv<V>(pnew, i0, pn);
+            stv<V>(q, i0, qv);
+            if (pending) {
+                const VF<V> po = ldv<V>(pold, i0);
+                VF<V> xv = ldv<V>(x, i0);
+    #pragma unroll
+                for (int k = 0; k < V; k++)
+                    if (di.v[k] != 0.0f) xv.v[k] = xv.v[k] + alpha * (po.v[k] - pa);
+                stv<V>(x, i0, xv);
+            }
         }
-        stv<V>(pnew, i0, pn);
-        stv<V>(q, i0, qv);
-        if (pending) {
-            const VF<V> po = ldv<V>(pold, i0);
+        reduce_finish<1>(vg, acc, part, sc, 2, FIN_ALPHA);
+    }
+};
+
+// Apply the x update still pending from the last iteration (before a residual check,
+// a refresh or the end of the solve).
+template <int V>
+struct K_k_fused_xflush {
+    static constexpr int MINB = 0;
+    struct Args {
+        Grid3 g;
+        const float* p0;
+        const float* p1;
+        float* x;
+        const float* dinv;
+        Scal* sc;
+        int meanfree;
+    };
+    static __device__ void run(const Args& A_, const VGrid& vg) {
+        Grid3 g = A_.g;
+        const float* p0 = A_.p0;
+        const float* p1 = A_.p1;
+        float* __restrict__ x = A_.x;
+        const float* __restrict__ dinv = A_.dinv;
+        Scal* sc = A_.sc;
+        int meanfree = A_.meanfree;
+        (void)vg;
+        if (sc->pending == 0) return;
+        const uint32_t it = (uint32_t)sc->iters;
+        // the last completed iteration k = iters-1 wrote p_k to p0 when k is even
+        const float* __restrict__ pk = ((it - 1u) & 1u) ? p1 : p0;
+        const float alpha = (float)sc->alpha;
+        const float pa = meanfree ? pk[sc->anchor] : 0.0f;
+        uint32_t lo, hi;
+        block_range(vg, g.n / V, lo, hi);
+        for (uint32_t gi = lo + threadIdx.x; gi < hi; gi += MHDP_BLOCK) {
+            const uint32_t i0 = gi * V;
+            const VF<V> po = ldv<V>(pk, i0), di = ldv<V>(dinv, i0);
             VF<V> xv = ldv<V>(x, i0);
-#pragma unroll
+    #pragma unroll
             for (int k = 0; k < V; k++)
                 if (di.v[k] != 0.0f) xv.v[k] = xv.v[k] + alpha * (po.v[k] - pa);
             stv<V>(x, i0, xv);
         }
     }
-    reduce_finish<1>(acc, part, sc, 2, FIN_ALPHA);
-}
-
-// Apply the x update still pending from the last iteration (before a residual check,
-// a refresh or the end of the solve).
-template <int V>
-__global__ void __launch_bounds__(MHDP_BLOCK) k_fused_xflush(Grid3 g, const float* p0, const float* p1,
-                                                             float* __restrict__ x, const float* __restrict__ dinv,
-                                                             Scal* sc, int meanfree) {
-    if (sc->pending == 0) return;
-    const uint32_t it = (uint32_t)sc->iters;
-    // the last completed iteration k = iters-1 wrote p_k to p0 when k is even
-    const float* __restrict__ pk = ((it - 1u) & 1u) ? p1 : p0;
-    const float alpha = (float)sc->alpha;
-    const float pa = meanfree ? pk[sc->anchor] : 0.0f;
-    uint32_t lo, hi;
-    block_range(g.n / V, lo, hi);
-    for (uint32_t gi = lo + threadIdx.x; gi < hi; gi += MHDP_BLOCK) {
-        const uint32_t i0 = gi * V;
-        const VF<V> po = ldv<V>(pk, i0), di = ldv<V>(dinv, i0);
-        VF<V> xv = ldv<V>(x, i0);
-#pragma unroll
-        for (int k = 0; k < V; k++)
-            if (di.v[k] != 0.0f) xv.v[k] = xv.v[k] + alpha * (po.v[k] - pa);
-        stv<V>(x, i0, xv);
-    }
-}
+};
 
 // ---------------------------------------------------------------- pass 2
 // r = (r - m) - alpha q on active cells; partials (0, sum r^2, 0, sum r) -> FIN_UPDATE_MG.
 template <int V>
-__global__ void __launch_bounds__(MHDP_BLOCK) k_fused_update(Grid3 g, float* __restrict__ r,
-                                                             const float* __restrict__ q,
-                                                             const float* __restrict__ dinv,
-                                                             double* __restrict__ part, Scal* sc) {
-    if (CONVERGED(sc)) return;
-    const float alpha = (float)sc->alpha;
-    const float m = (float)sc->mean;
-    uint32_t lo, hi;
-    block_range(g.n / V, lo, hi);
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (uint32_t gi = lo + threadIdx.x; gi < hi; gi += MHDP_BLOCK) {
-        const uint32_t i0 = gi * V;
-        const VF<V> di = ldv<V>(dinv, i0), qv = ldv<V>(q, i0);
-        VF<V> rv = ldv<V>(r, i0);
-#pragma unroll
-        for (int k = 0; k < V; k++) {
-            const bool act = di.v[k] != 0.0f;
-            const float rn = act ? (rv.v[k] - m) - alpha * qv.v[k] : 0.0f;
-            rv.v[k] = rn;
-            acc[1] += (double)rn * (double)rn;
-            acc[3] += (double)rn;
+struct K_k_fused_update {
+    static constexpr int MINB = 0;
+    struct Args {
+        Grid3 g;
+        float* r;
+        const float* q;
+        const float* dinv;
+        double* part;
+        Scal* sc;
+    };
+    static __device__ void run(const Args& A_, const VGrid& vg) {
+        Grid3 g = A_.g;
+        float* __restrict__ r = A_.r;
+        const float* __restrict__ q = A_.q;
+        const float* __restrict__ dinv = A_.dinv;
+        double* __restrict__ part = A_.part;
+        Scal* sc = A_.sc;
+        (void)vg;
+        if (CONVERGED(sc)) return;
+        const float alpha = (float)sc->alpha;
+        const float m = (float)sc->mean;
+        uint32_t lo, hi;
+        block_range(vg, g.n / V, lo, hi);
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (uint32_t gi = lo + threadIdx.x; gi < hi; gi += MHDP_BLOCK) {
+            const uint32_t i0 = gi * V;
+            const VF<V> di = ldv<V>(dinv, i0), qv = ldv<V>(q, i0);
+            VF<V> rv = ldv<V>(r, i0);
+    #pragma unroll
+            for (int k = 0; k < V; k++) {
+                const bool act = di.v[k] != 0.0f;
+                const float rn = act ? (rv.v[k] - m) - alpha * qv.v[k] : 0.0f;
+                rv.v[k] = rn;
+                acc[1] += (double)rn * (double)rn;
+                acc[3] += (double)rn;
+            }
+            stv<V>(r, i0, rv);
         }
-        stv<V>(r, i0, rv);
+        reduce_finish<4>(vg, acc, part, sc, 3, FIN_UPDATE_MG);
     }
-    reduce_finish<4>(acc, part, sc, 3, FIN_UPDATE_MG);
-}
+};
 
 // ---------------------------------------------------------------- pass 3
 // For each coarse cell: sum over its active children of (r - m) - A z1, z1 = omega (r-m)/d.
 template <class CT, class VT>
-__global__ void __launch_bounds__(MHDP_BLOCK) k_fused_restrict(Grid3 cg, Grid3 fg, const float* __restrict__ r,
-                                                               const CT* __restrict__ dinv,
-                                                               const CT* __restrict__ bx, const CT* __restrict__ by,
-                                                               const CT* __restrict__ bz, VT* __restrict__ rc,
-                                                               Scal* sc) {
-    if (CONVERGED(sc)) return;
-    const float omega = (float)sc->omega;  // smoother weight (reference: 2/3)
-    const float m = (float)sc->mean;
-    uint32_t lo, hi;
-    block_range(cg.n, lo, hi);
-    const uint32_t sy = (uint32_t)fg.nx, sz = (uint32_t)fg.nx * (uint32_t)fg.ny;
-    for (uint32_t ci = lo + threadIdx.x; ci < hi; ci += MHDP_BLOCK) {
-        int qx, qy, qz;
-        decode(cg, ci, qx, qy, qz);
-        auto z1 = [&](uint32_t j) { return omega * (r[j] - m) * ldf(dinv[j]); };
-        float rhs = 0.0f;
-#pragma unroll
-        for (int dz = 0; dz < 2; dz++)
-#pragma unroll
-            for (int dy = 0; dy < 2; dy++)
-#pragma unroll
-                for (int dx = 0; dx < 2; dx++) {
-                    const int x = 2 * qx + dx, y = 2 * qy + dy, z = 2 * qz + dz;
-                    if (x >= fg.nx || y >= fg.ny || z >= fg.nz) continue;
-                    const uint32_t i = (uint32_t)x + sy * (uint32_t)y + sz * (uint32_t)z;
-                    const float di = ldf(dinv[i]);
-                    if (di == 0.0f) continue;
-                    const float rc_i = r[i] - m;
-                    const float c = omega * rc_i * di;
-                    uint32_t nb[6];
-                    float w[6];
-                    stencil6(fg, x, y, z, i, bx, by, bz, nb, w);
-                    float a = 0.0f;
-#pragma unroll
-                    for (int k = 0; k < 6; k++) a += w[k] * (c - z1(nb[k]));
-                    rhs += rc_i - a;
-                }
-        rc[ci] = cvt<VT>(rhs);
+struct K_k_fused_restrict {
+    static constexpr int MINB = 0;
+    struct Args {
+        Grid3 cg;
+        Grid3 fg;
+        const float* r;
+        const CT* dinv;
+        const CT* bx;
+        const CT* by;
+        const CT* bz;
+        VT* rc;
+        Scal* sc;
+    };
+    static __device__ void run(const Args& A_, const VGrid& vg) {
+        Grid3 cg = A_.cg;
+        Grid3 fg = A_.fg;
+        const float* __restrict__ r = A_.r;
+        const CT* __restrict__ dinv = A_.dinv;
+        const CT* __restrict__ bx = A_.bx;
+        const CT* __restrict__ by = A_.by;
+        const CT* __restrict__ bz = A_.bz;
+        VT* __restrict__ rc = A_.rc;
+        Scal* sc = A_.sc;
+        (void)vg;
+        if (CONVERGED(sc)) return;
+        const float omega = (float)sc->omega;  // smoother weight (reference: 2/3)
+        const float m = (float)sc->mean;
+        uint32_t lo, hi;
+        block_range(vg, cg.n, lo, hi);
+        const uint32_t sy = (uint32_t)fg.nx, sz = (uint32_t)fg.nx * (uint32_t)fg.ny;
+        for (uint32_t ci = lo + threadIdx.x; ci < hi; ci += MHDP_BLOCK) {
+            int qx, qy, qz;
+            decode(cg, ci, qx, qy, qz);
+            auto z1 = [&](uint32_t j) { return omega * (r[j] - m) * ldf(dinv[j]); };
+            float rhs = 0.0f;
+    #pragma unroll
+            for (int dz = 0; dz < 2; dz++)
+    #pragma unroll
+                for (int dy = 0; dy < 2; dy++)
+    #pragma unroll
+                    for (int dx = 0; dx < 2; dx++) {
+                        const int x = 2 * qx + dx, y = 2 * qy + dy, z = 2 * qz + dz;
+                        if (x >= fg.nx || y >= fg.ny || z >= fg.nz) continue;
+                        const uint32_t i = (uint32_t)x + sy * (uint32_t)y + sz * (uint32_t)z;
+                        const float di = ldf(dinv[i]);
+                        if (di == 0.0f) continue;
+                        const float rc_i = r[i] - m;
+                        const float c = omega * rc_i * di;
+                        uint32_t nb[6];
+                        float w[6];
+                        stencil6(fg, x, y, z, i, bx, by, bz, nb, w);
+                        float a = 0.0f;
+    #pragma unroll
+                        for (int k = 0; k < 6; k++) a += w[k] * (c - z1(nb[k]));
+                        rhs += rc_i - a;
+                    }
+            rc[ci] = cvt<VT>(rhs);
+        }
     }
-}
+};
 
 // ---------------------------------------------------------------- pass 4
 // z2 = z1 + z_coarse[parent] (active), z = z2 + omega (r - m - A z2)/d; partial (r-m).z.
 template <int V, class CT, class VT>
-__global__ void __launch_bounds__(MHDP_BLOCK) k_fused_smooth(Grid3 g, Grid3 cg, const float* __restrict__ r,
-                                                             const CT* __restrict__ dinv, const CT* __restrict__ bx,
-                                                             const CT* __restrict__ by, const CT* __restrict__ bz,
-                                                             const VT* __restrict__ zc, VT* __restrict__ t,
-                                                             double* __restrict__ part, Scal* sc, int mode) {
-    if (CONVERGED(sc)) return;
-    const float omega = (float)sc->omega;  // smoother weight (reference: 2/3)
-    const float m = (float)sc->mean;
-    const float ps = (float)sc->pscale;  // coarse-correction scale (reference: 1)
-    const uint32_t csy = (uint32_t)cg.nx, csz = (uint32_t)cg.nx * (uint32_t)cg.ny;
-    uint32_t lo, hi;
-    block_range(g.n / V, lo, hi);
-    double acc[1] = {0.0};
-    // z2 at one cell (j at coordinates xq, yq, zq): 0 where inactive
-    auto z2 = [&](uint32_t j, int xq, int yq, int zq) {
-        const float dj = ldf(dinv[j]);
-        if (dj == 0.0f) return 0.0f;
-        return omega * (r[j] - m) * dj + ps * ldf(zc[(uint32_t)(xq >> 1) + csy * (uint32_t)(yq >> 1) + csz * (uint32_t)(zq >> 1)]);
+struct K_k_fused_smooth {
+    static constexpr int MINB = 0;
+    struct Args {
+        Grid3 g;
+        Grid3 cg;
+        const float* r;
+        const CT* dinv;
+        const CT* bx;
+        const CT* by;
+        const CT* bz;
+        const VT* zc;
+        VT* t;
+        double* part;
+        Scal* sc;
+        int mode;
     };
-    for (uint32_t gi = lo + threadIdx.x; gi < hi; gi += MHDP_BLOCK) {
-        const uint32_t i0 = gi * V;
-        int x0, y, zz;
-        decode(g, i0, x0, y, zz);
-        const Nb n = neighbours<V>(g, x0, y, zz, i0);
-        const W6<V, CT> W = weights<V, CT>(n, i0, bx, by, bz);
-        // z2 along a row of V cells starting at j (row coordinates yq, zq)
-        auto zrow = [&](uint32_t j, int yq, int zq) {
-            const VF<V> rv = ldv<V>(r, j), dv = ldv<V>(dinv, j);
-            const uint32_t cb = csy * (uint32_t)(yq >> 1) + csz * (uint32_t)(zq >> 1);
-            VF<V> o;
-#pragma unroll
-            for (int k = 0; k < V; k++)
-                o.v[k] = dv.v[k] != 0.0f ? omega * (rv.v[k] - m) * dv.v[k] + ps * ldf(zc[cb + (uint32_t)((x0 + k) >> 1)])
-                                         : 0.0f;
-            return o;
+    static __device__ void run(const Args& A_, const VGrid& vg) {
+        Grid3 g = A_.g;
+        Grid3 cg = A_.cg;
+        const float* __restrict__ r = A_.r;
+        const CT* __restrict__ dinv = A_.dinv;
+        const CT* __restrict__ bx = A_.bx;
+        const CT* __restrict__ by = A_.by;
+        const CT* __restrict__ bz = A_.bz;
+        const VT* __restrict__ zc = A_.zc;
+        VT* __restrict__ t = A_.t;
+        double* __restrict__ part = A_.part;
+        Scal* sc = A_.sc;
+        int mode = A_.mode;
+        (void)vg;
+        if (CONVERGED(sc)) return;
+        const float omega = (float)sc->omega;  // smoother weight (reference: 2/3)
+        const float m = (float)sc->mean;
+        const float ps = (float)sc->pscale;  // coarse-correction scale (reference: 1)
+        const uint32_t csy = (uint32_t)cg.nx, csz = (uint32_t)cg.nx * (uint32_t)cg.ny;
+        uint32_t lo, hi;
+        block_range(vg, g.n / V, lo, hi);
+        double acc[1] = {0.0};
+        // z2 at one cell (j at coordinates xq, yq, zq): 0 where inactive
+        auto z2 = [&](uint32_t j, int xq, int yq, int zq) {
+            const float dj = ldf(dinv[j]);
+            if (dj == 0.0f) return 0.0f;
+            return omega * (r[j] - m) * dj + ps * ldf(zc[(uint32_t)(xq >> 1) + csy * (uint32_t)(yq >> 1) + csz * (uint32_t)(zq >> 1)]);
         };
-        Star<V> s;
-        {
-            const VF<V> c = zrow(i0, y, zz), ym = zrow(n.ym, n.cym, zz), yp = zrow(n.yp, n.cyp, zz),
-                        zm = zrow(n.zm, y, n.czm), zp = zrow(n.zp, y, n.czp);
-            const float xm = z2(n.xm, n.cxm, y, zz), xp = z2(n.xp, n.cxp, y, zz);
-#pragma unroll
+        for (uint32_t gi = lo + threadIdx.x; gi < hi; gi += MHDP_BLOCK) {
+            const uint32_t i0 = gi * V;
+            int x0, y, zz;
+            decode(g, i0, x0, y, zz);
+            const Nb n = neighbours<V>(g, x0, y, zz, i0);
+            const W6<V, CT> W = weights<V, CT>(n, i0, bx, by, bz);
+            // z2 along a row of V cells starting at j (row coordinates yq, zq)
+            auto zrow = [&](uint32_t j, int yq, int zq) {
+                const VF<V> rv = ldv<V>(r, j), dv = ldv<V>(dinv, j);
+                const uint32_t cb = csy * (uint32_t)(yq >> 1) + csz * (uint32_t)(zq >> 1);
+                VF<V> o;
+    #pragma unroll
+                for (int k = 0; k < V; k++)
+                    o.v[k] = dv.v[k] != 0.0f ? omega * (rv.v[k] - m) * dv.v[k] + ps * ldf(zc[cb + (uint32_t)((x0 + k) >> 1)])
+                                             : 0.0f;
+                return o;
+            };
+            Star<V> s;
+            {
+                const VF<V> c = zrow(i0, y, zz), ym = zrow(n.ym, n.cym, zz), yp = zrow(n.yp, n.cyp, zz),
+                            zm = zrow(n.zm, y, n.czm), zp = zrow(n.zp, y, n.czp);
+                const float xm = z2(n.xm, n.cxm, y, zz), xp = z2(n.xp, n.cxp, y, zz);
+    #pragma unroll
+                for (int k = 0; k < V; k++) {
+                    s.c[k] = c.v[k];
+                    s.ym[k] = ym.v[k];
+                    s.yp[k] = yp.v[k];
+                    s.zm[k] = zm.v[k];
+                    s.zp[k] = zp.v[k];
+                    s.xm[k] = k > 0 ? c.v[k - 1] : xm;
+                    s.xp[k] = k + 1 < V ? c.v[k + 1] : xp;
+                }
+            }
+            const VF<V> rv = ldv<V>(r, i0), di = ldv<V>(dinv, i0);
+            VF<V> tv;
+    #pragma unroll
             for (int k = 0; k < V; k++) {
-                s.c[k] = c.v[k];
-                s.ym[k] = ym.v[k];
-                s.yp[k] = yp.v[k];
-                s.zm[k] = zm.v[k];
-                s.zp[k] = zp.v[k];
-                s.xm[k] = k > 0 ? c.v[k - 1] : xm;
-                s.xp[k] = k + 1 < V ? c.v[k + 1] : xp;
+                float out = 0.0f;
+                if (di.v[k] != 0.0f) {
+                    const float rk = rv.v[k] - m;
+                    out = s.c[k] + omega * (rk - star_action<V, CT>(W, s, k)) * di.v[k];
+                    acc[0] += (double)rk * (double)out;
+                }
+                tv.v[k] = out;
             }
+            stv<V>(t, i0, tv);
         }
-        const VF<V> rv = ldv<V>(r, i0), di = ldv<V>(dinv, i0);
-        VF<V> tv;
-#pragma unroll
-        for (int k = 0; k < V; k++) {
-            float out = 0.0f;
-            if (di.v[k] != 0.0f) {
-                const float rk = rv.v[k] - m;
-                out = s.c[k] + omega * (rk - star_action<V, CT>(W, s, k)) * di.v[k];
-                acc[0] += (double)rk * (double)out;
-            }
-            tv.v[k] = out;
-        }
-        stv<V>(t, i0, tv);
+        reduce_finish<1>(vg, acc, part, sc, 4, mode);
     }
-    reduce_finish<1>(acc, part, sc, 4, mode);
-}
+};
 
-__global__ void k_clear_pending(Scal* sc) {
-    if (threadIdx.x == 0) sc->pending = 0u;
-}
+struct K_k_clear_pending {
+    static constexpr int MINB = 0;
+    struct Args {
+        Scal* sc;
+    };
+    static __device__ void run(const Args& A_, const VGrid&) {
+        if (threadIdx.x == 0) A_.sc->pending = 0u;
+    }
+};
 
 // ---------------------------------------------------------------- launchers
 
-void launch_clear_pending(hipStream_t s, Scal* sc) { hipLaunchKernelGGL(k_clear_pending, dim3(1), dim3(64), 0, s, sc); }
+void launch_clear_pending(hipStream_t s, Scal* sc) { go<K_k_clear_pending>(s, 1, {sc}); }
 
 bool fused_vec4(const Grid3& g) { return (g.nx & 3) == 0; }
 
@@ -429,37 +543,34 @@ void launch_fused_apply(hipStream_t s, const Grid3& g, int store, const void* z,
     const bool v4 = fused_vec4(g);
     const dim3 B(MHDP_BLOCK), G(grid_blocks(v4 ? g.n / 4 : g.n));
     if (store == STORE_BF16) {
-        if (v4) hipLaunchKernelGGL((k_fused_apply<4, __hip_bfloat16>), G, B, 0, s, g, (const __hip_bfloat16*)z, p0, p1, x, bx, by, bz, dinv, q, part, sc, meanfree);
-        else hipLaunchKernelGGL((k_fused_apply<1, __hip_bfloat16>), G, B, 0, s, g, (const __hip_bfloat16*)z, p0, p1, x, bx, by, bz, dinv, q, part, sc, meanfree);
+        if (v4) go<K_k_fused_apply<4, __hip_bfloat16>>(s, G.x, {g, (const __hip_bfloat16*)z, p0, p1, x, bx, by, bz, dinv, q, part, sc, meanfree});
+        else go<K_k_fused_apply<1, __hip_bfloat16>>(s, G.x, {g, (const __hip_bfloat16*)z, p0, p1, x, bx, by, bz, dinv, q, part, sc, meanfree});
     } else {
-        if (v4) hipLaunchKernelGGL((k_fused_apply<4, float>), G, B, 0, s, g, (const float*)z, p0, p1, x, bx, by, bz, dinv, q, part, sc, meanfree);
-        else hipLaunchKernelGGL((k_fused_apply<1, float>), G, B, 0, s, g, (const float*)z, p0, p1, x, bx, by, bz, dinv, q, part, sc, meanfree);
+        if (v4) go<K_k_fused_apply<4, float>>(s, G.x, {g, (const float*)z, p0, p1, x, bx, by, bz, dinv, q, part, sc, meanfree});
+        else go<K_k_fused_apply<1, float>>(s, G.x, {g, (const float*)z, p0, p1, x, bx, by, bz, dinv, q, part, sc, meanfree});
     }
 }
 void launch_fused_xflush(hipStream_t s, const Grid3& g, const float* p0, const float* p1, float* x, const float* dinv,
                          Scal* sc, int meanfree) {
     const bool v4 = fused_vec4(g);
     const dim3 B(MHDP_BLOCK), G(grid_blocks(v4 ? g.n / 4 : g.n));
-    if (v4) hipLaunchKernelGGL(k_fused_xflush<4>, G, B, 0, s, g, p0, p1, x, dinv, sc, meanfree);
-    else hipLaunchKernelGGL(k_fused_xflush<1>, G, B, 0, s, g, p0, p1, x, dinv, sc, meanfree);
+    if (v4) go<K_k_fused_xflush<4>>(s, G.x, {g, p0, p1, x, dinv, sc, meanfree});
+    else go<K_k_fused_xflush<1>>(s, G.x, {g, p0, p1, x, dinv, sc, meanfree});
 }
 void launch_fused_update(hipStream_t s, const Grid3& g, float* r, const float* q, const float* dinv, double* part,
                          Scal* sc) {
     const bool v4 = fused_vec4(g);
     const dim3 B(MHDP_BLOCK), G(grid_blocks(v4 ? g.n / 4 : g.n));
-    if (v4) hipLaunchKernelGGL(k_fused_update<4>, G, B, 0, s, g, r, q, dinv, part, sc);
-    else hipLaunchKernelGGL(k_fused_update<1>, G, B, 0, s, g, r, q, dinv, part, sc);
+    if (v4) go<K_k_fused_update<4>>(s, G.x, {g, r, q, dinv, part, sc});
+    else go<K_k_fused_update<1>>(s, G.x, {g, r, q, dinv, part, sc});
 }
 void launch_fused_restrict(hipStream_t s, int store, const Grid3& cg, const Grid3& fg, const float* r, const void* dinv,
                            const void* bx, const void* by, const void* bz, void* rc, Scal* sc) {
     const dim3 B(MHDP_BLOCK), G(grid_blocks(cg.n));
     if (store == STORE_BF16)
-        hipLaunchKernelGGL((k_fused_restrict<__hip_bfloat16, __hip_bfloat16>), G, B, 0, s, cg, fg, r,
-                           (const __hip_bfloat16*)dinv, (const __hip_bfloat16*)bx, (const __hip_bfloat16*)by,
-                           (const __hip_bfloat16*)bz, (__hip_bfloat16*)rc, sc);
+        go<K_k_fused_restrict<__hip_bfloat16, __hip_bfloat16>>(s, G.x, {cg, fg, r, (const __hip_bfloat16*)dinv, (const __hip_bfloat16*)bx, (const __hip_bfloat16*)by, (const __hip_bfloat16*)bz, (__hip_bfloat16*)rc, sc});
     else
-        hipLaunchKernelGGL((k_fused_restrict<float, float>), G, B, 0, s, cg, fg, r, (const float*)dinv,
-                           (const float*)bx, (const float*)by, (const float*)bz, (float*)rc, sc);
+        go<K_k_fused_restrict<float, float>>(s, G.x, {cg, fg, r, (const float*)dinv, (const float*)bx, (const float*)by, (const float*)bz, (float*)rc, sc});
 }
 void launch_fused_smooth(hipStream_t s, int store, const Grid3& g, const Grid3& cg, const float* r, const void* dinv,
                          const void* bx, const void* by, const void* bz, const void* zc, void* t, double* part,
@@ -468,11 +579,11 @@ void launch_fused_smooth(hipStream_t s, int store, const Grid3& g, const Grid3& 
     const dim3 B(MHDP_BLOCK), G(grid_blocks(v4 ? g.n / 4 : g.n));
     if (store == STORE_BF16) {
         using T = __hip_bfloat16;
-        if (v4) hipLaunchKernelGGL((k_fused_smooth<4, T, T>), G, B, 0, s, g, cg, r, (const T*)dinv, (const T*)bx, (const T*)by, (const T*)bz, (const T*)zc, (T*)t, part, sc, mode);
-        else hipLaunchKernelGGL((k_fused_smooth<1, T, T>), G, B, 0, s, g, cg, r, (const T*)dinv, (const T*)bx, (const T*)by, (const T*)bz, (const T*)zc, (T*)t, part, sc, mode);
+        if (v4) go<K_k_fused_smooth<4, T, T>>(s, G.x, {g, cg, r, (const T*)dinv, (const T*)bx, (const T*)by, (const T*)bz, (const T*)zc, (T*)t, part, sc, mode});
+        else go<K_k_fused_smooth<1, T, T>>(s, G.x, {g, cg, r, (const T*)dinv, (const T*)bx, (const T*)by, (const T*)bz, (const T*)zc, (T*)t, part, sc, mode});
     } else {
         using T = float;
-        if (v4) hipLaunchKernelGGL((k_fused_smooth<4, T, T>), G, B, 0, s, g, cg, r, (const T*)dinv, (const T*)bx, (const T*)by, (const T*)bz, (const T*)zc, (T*)t, part, sc, mode);
-        else hipLaunchKernelGGL((k_fused_smooth<1, T, T>), G, B, 0, s, g, cg, r, (const T*)dinv, (const T*)bx, (const T*)by, (const T*)bz, (const T*)zc, (T*)t, part, sc, mode);
+        if (v4) go<K_k_fused_smooth<4, T, T>>(s, G.x, {g, cg, r, (const T*)dinv, (const T*)bx, (const T*)by, (const T*)bz, (const T*)zc, (T*)t, part, sc, mode});
+        else go<K_k_fused_smooth<1, T, T>>(s, G.x, {g, cg, r, (const T*)dinv, (const T*)bx, (const T*)by, (const T*)bz, (const T*)zc, (T*)t, part, sc, mode});
     }
 }

@@ -10,6 +10,7 @@
 // cells x0-1, x0+4).  The arithmetic and summation order per cell are those of brick.hip.
 // Needs nx % 4 == 0 and even ny, nz; a chunk is an even number of planes.
 #include "mhdp_kernels.h"
+#include "launch.h"
 #include "raw.h"
 
 #define CONVERGED(sc) ((sc)->conv > 0.5)
@@ -171,138 +172,167 @@ __device__ __forceinline__ Item item_of(const Grid3& g, uint32_t it, uint32_t zc
 
 // Pass 4 (post-smoother): z2 = omega (r - m)/d + ps * zc[parent] on active cells (prolongation
 // and pre-smoother, never stored), t = z2 + omega (r - m - A z2)/d, and (r - m).t.
-template <class CT, class VT, int MINB>
-__global__ void __launch_bounds__(MHDP_BLOCK, MINB) k_march_smooth(Grid3 g, Grid3 cg, const float* r0, const float* r1,
-                                                                   const CT* __restrict__ dinv,
-                                                                   const CT* __restrict__ bx,
-                                                                   const CT* __restrict__ by,
-                                                                   const CT* __restrict__ bz,
-                                                                   const VT* __restrict__ zc, VT* __restrict__ t,
-                                                                   double* __restrict__ part, Scal* sc, int mode,
-                                                                   int use_next, uint32_t zlen) {
-    if (CONVERGED(sc)) return;
-    const float omega = (float)sc->omega;
-    const float m = (float)(use_next ? sc->mean_next : sc->mean);
-    const float ps = (float)sc->pscale;
-    const float* __restrict__ r = sc->rsel ? r1 : r0;
-    const uint32_t sy = (uint32_t)g.nx, sz = (uint32_t)g.nx * (uint32_t)g.ny;
-    const uint32_t csy = (uint32_t)cg.nx, csz = (uint32_t)cg.nx * (uint32_t)cg.ny;
-    const uint32_t items = ((uint32_t)g.nx >> 2) * ((uint32_t)g.ny >> 1) * (((uint32_t)g.nz + zlen - 1) / zlen);
-    double acc[1] = {0.0};
-    for (uint32_t it = logical_block() * MHDP_BLOCK + threadIdx.x; it < items; it += gridDim.x * MHDP_BLOCK) {
-        const Item w0 = item_of(g, it, zlen);
-        const Col o = column(g, w0.cx, w0.cy);
-        const uint32_t cx = w0.cx << 1;  // parents of cells x0, x0+1 and x0+2, x0+3: cx, cx+1
-        const uint32_t cy = (uint32_t)o.y0 >> 1, cym = (uint32_t)o.ymy >> 1, cyp = (uint32_t)o.ypy >> 1;
-        const uint32_t cxm = (uint32_t)o.xmx >> 1, cxp = (uint32_t)o.xpx >> 1;
-        struct Raw {
-            PR<float> r;
-            PR<CT> d;
-            R2<VT> zo, zym, zyp;
-            R1<VT> zxm, zxp;
-            PW<CT> w;
-        };
-        auto load = [&](int p, Raw& q) {
-            const int pp = plane_of(g, p);
-            const uint32_t base = sz * (uint32_t)pp, cz = csz * ((uint32_t)pp >> 1);
-            load_pr(o, base, sy, r, q.r);
-            load_pr(o, base, sy, dinv, q.d);
-            load_pw(o, base, sy, bx, by, bz, q.w);
-            q.zo = ldr2(zc, cz + csy * cy + cx);
-            q.zym = ldr2(zc, cz + csy * cym + cx);
-            q.zyp = ldr2(zc, cz + csy * cyp + cx);
-            q.zxm = ldr1(zc, cz + csy * cy + cxm);
-            q.zxp = ldr1(zc, cz + csy * cy + cxp);
-        };
-        auto settle = [&](Raw& q) {
-            fence_pr(q.r);
-            fence_pr(q.d);
-            fence_pw(q.w);
-            fence(q.zo);
-            fence(q.zym);
-            fence(q.zyp);
-            fence(q.zxm);
-            fence(q.zxp);
-        };
-        // z2 on the plane's own cells and in-plane halo
-        auto field = [&](const Raw& q, PF& f) {
-            float c0, c1, y0a, y0b, y1a, y1b;
-            un(q.zo, c0, c1);
-            un(q.zym, y0a, y0b);
-            un(q.zyp, y1a, y1b);
-            const float cxmv = un(q.zxm), cxpv = un(q.zxp);
-#pragma unroll
-            for (int dy = 0; dy < 2; dy++) {
-                const F4 a = un(q.r.c[dy]), d = un(q.d.c[dy]);
-#pragma unroll
-                for (int k = 0; k < 4; k++)
-                    f.c[dy][k] = d.v[k] != 0.0f ? omega * (a.v[k] - m) * d.v[k] + ps * (k < 2 ? c0 : c1) : 0.0f;
-                const float dm = un(q.d.xm[dy]), dp = un(q.d.xp[dy]);
-                const float vm = omega * (un(q.r.xm[dy]) - m) * dm + ps * cxmv;
-                const float vp = omega * (un(q.r.xp[dy]) - m) * dp + ps * cxpv;
-                f.xm[dy] = dm != 0.0f ? vm : 0.0f;
-                f.xp[dy] = dp != 0.0f ? vp : 0.0f;
-            }
-            const F4 am = un(q.r.ym), dm = un(q.d.ym), ap = un(q.r.yp), dp = un(q.d.yp);
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                f.ym[k] = dm.v[k] != 0.0f ? omega * (am.v[k] - m) * dm.v[k] + ps * (k < 2 ? y0a : y0b) : 0.0f;
-                f.yp[k] = dp.v[k] != 0.0f ? omega * (ap.v[k] - m) * dp.v[k] + ps * (k < 2 ? y1a : y1b) : 0.0f;
-            }
-        };
-        Raw cur, nxt;
-        PF fc, fn;
-        float below[2][4];
-        {
-            Raw prev;
-            load(w0.za - 1, prev);
-            load(w0.za, cur);
-            load(w0.za + 1, nxt);
-            settle(prev);
-            settle(cur);
-            PF fp;
-            field(prev, fp);
-#pragma unroll
-            for (int dy = 0; dy < 2; dy++)
-#pragma unroll
-                for (int k = 0; k < 4; k++) below[dy][k] = fp.c[dy][k];
-            field(cur, fc);
-        }
-        for (int k = w0.za; k < w0.zb; k++) {
-            Raw pre;
-            load(k + 2, pre);
-            settle(nxt);
-            field(nxt, fn);
-            const bool zp_ok = k + 1 < g.nz || (g.per & 4u);
-            const uint32_t base = sz * (uint32_t)k;
-#pragma unroll
-            for (int dy = 0; dy < 2; dy++) {
-                const F4 rv = un(cur.r.c[dy]), di = un(cur.d.c[dy]);
-                F4 tv;
-#pragma unroll
-                for (int kk = 0; kk < 4; kk++) {
-                    float fw[6];
-                    face6(o, cur.w, nxt.w, zp_ok, dy, kk, fw);
-                    const float rk = rv.v[kk] - m;
-                    const float a = act6(fc, below, fn.c, dy, kk, fw);
-                    const float out = fc.c[dy][kk] + omega * (rk - a) * di.v[kk];
-                    const bool on = di.v[kk] != 0.0f;
-                    tv.v[kk] = on ? out : 0.0f;
-                    acc[0] += on ? (double)rk * (double)out : 0.0;
+template <class CT, class VT, int MB_>
+struct K_k_march_smooth {
+    static constexpr int MINB = MB_;
+    struct Args {
+        Grid3 g;
+        Grid3 cg;
+        const float* r0;
+        const float* r1;
+        const CT* dinv;
+        const CT* bx;
+        const CT* by;
+        const CT* bz;
+        const VT* zc;
+        VT* t;
+        double* part;
+        Scal* sc;
+        int mode;
+        int use_next;
+        uint32_t zlen;
+    };
+    static __device__ void run(const Args& A_, const VGrid& vg) {
+        Grid3 g = A_.g;
+        Grid3 cg = A_.cg;
+        const float* r0 = A_.r0;
+        const float* r1 = A_.r1;
+        const CT* __restrict__ dinv = A_.dinv;
+        const CT* __restrict__ bx = A_.bx;
+        const CT* __restrict__ by = A_.by;
+        const CT* __restrict__ bz = A_.bz;
+        const VT* __restrict__ zc = A_.zc;
+        VT* __restrict__ t = A_.t;
+        double* __restrict__ part = A_.part;
+        Scal* sc = A_.sc;
+        int mode = A_.mode;
+        int use_next = A_.use_next;
+        uint32_t zlen = A_.zlen;
+        (void)vg;
+        if (CONVERGED(sc)) return;
+        const float omega = (float)sc->omega;
+        const float m = (float)(use_next ? sc->mean_next : sc->mean);
+        const float ps = (float)sc->pscale;
+        const float* __restrict__ r = sc->rsel ? r1 : r0;
+        const uint32_t sy = (uint32_t)g.nx, sz = (uint32_t)g.nx * (uint32_t)g.ny;
+        const uint32_t csy = (uint32_t)cg.nx, csz = (uint32_t)cg.nx * (uint32_t)cg.ny;
+        const uint32_t items = ((uint32_t)g.nx >> 2) * ((uint32_t)g.ny >> 1) * (((uint32_t)g.nz + zlen - 1) / zlen);
+        double acc[1] = {0.0};
+        for (uint32_t it = vg.b * MHDP_BLOCK + threadIdx.x; it < items; it += vg.n * MHDP_BLOCK) {
+            const Item w0 = item_of(g, it, zlen);
+            const Col o = column(g, w0.cx, w0.cy);
+            const uint32_t cx = w0.cx << 1;  // parents of cells x0, x0+1 and x0+2, x0+3: cx, cx+1
+            const uint32_t cy = (uint32_t)o.y0 >> 1, cym = (uint32_t)o.ymy >> 1, cyp = (uint32_t)o.ypy >> 1;
+            const uint32_t cxm = (uint32_t)o.xmx >> 1, cxp = (uint32_t)o.xpx >> 1;
+            struct Raw {
+                PR<float> r;
+                PR<CT> d;
+                R2<VT> zo, zym, zyp;
+                R1<VT> zxm, zxp;
+                PW<CT> w;
+            };
+            auto load = [&](int p, Raw& q) {
+                const int pp = plane_of(g, p);
+                const uint32_t base = sz * (uint32_t)pp, cz = csz * ((uint32_t)pp >> 1);
+                load_pr(o, base, sy, r, q.r);
+                load_pr(o, base, sy, dinv, q.d);
+                load_pw(o, base, sy, bx, by, bz, q.w);
+                q.zo = ldr2(zc, cz + csy * cy + cx);
+                q.zym = ldr2(zc, cz + csy * cym + cx);
+                q.zyp = ldr2(zc, cz + csy * cyp + cx);
+                q.zxm = ldr1(zc, cz + csy * cy + cxm);
+                q.zxp = ldr1(zc, cz + csy * cy + cxp);
+            };
+            auto settle = [&](Raw& q) {
+                fence_pr(q.r);
+                fence_pr(q.d);
+                fence_pw(q.w);
+                fence(q.zo);
+                fence(q.zym);
+                fence(q.zyp);
+                fence(q.zxm);
+                fence(q.zxp);
+            };
+            // z2 on the plane's own cells and in-plane halo
+            auto field = [&](const Raw& q, PF& f) {
+                float c0, c1, y0a, y0b, y1a, y1b;
+                un(q.zo, c0, c1);
+                un(q.zym, y0a, y0b);
+                un(q.zyp, y1a, y1b);
+                const float cxmv = un(q.zxm), cxpv = un(q.zxp);
+    #pragma unroll
+                for (int dy = 0; dy < 2; dy++) {
+                    const F4 a = un(q.r.c[dy]), d = un(q.d.c[dy]);
+    #pragma unroll
+                    for (int k = 0; k < 4; k++)
+                        f.c[dy][k] = d.v[k] != 0.0f ? omega * (a.v[k] - m) * d.v[k] + ps * (k < 2 ? c0 : c1) : 0.0f;
+                    const float dm = un(q.d.xm[dy]), dp = un(q.d.xp[dy]);
+                    const float vm = omega * (un(q.r.xm[dy]) - m) * dm + ps * cxmv;
+                    const float vp = omega * (un(q.r.xp[dy]) - m) * dp + ps * cxpv;
+                    f.xm[dy] = dm != 0.0f ? vm : 0.0f;
+                    f.xp[dy] = dp != 0.0f ? vp : 0.0f;
                 }
-                st4(t, base + o.row0 + (uint32_t)dy * sy, tv);
+                const F4 am = un(q.r.ym), dm = un(q.d.ym), ap = un(q.r.yp), dp = un(q.d.yp);
+    #pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    f.ym[k] = dm.v[k] != 0.0f ? omega * (am.v[k] - m) * dm.v[k] + ps * (k < 2 ? y0a : y0b) : 0.0f;
+                    f.yp[k] = dp.v[k] != 0.0f ? omega * (ap.v[k] - m) * dp.v[k] + ps * (k < 2 ? y1a : y1b) : 0.0f;
+                }
+            };
+            Raw cur, nxt;
+            PF fc, fn;
+            float below[2][4];
+            {
+                Raw prev;
+                load(w0.za - 1, prev);
+                load(w0.za, cur);
+                load(w0.za + 1, nxt);
+                settle(prev);
+                settle(cur);
+                PF fp;
+                field(prev, fp);
+    #pragma unroll
+                for (int dy = 0; dy < 2; dy++)
+    #pragma unroll
+                    for (int k = 0; k < 4; k++) below[dy][k] = fp.c[dy][k];
+                field(cur, fc);
             }
-#pragma unroll
-            for (int dy = 0; dy < 2; dy++)
-#pragma unroll
-                for (int kk = 0; kk < 4; kk++) below[dy][kk] = fc.c[dy][kk];
-            fc = fn;
-            cur = nxt;
-            nxt = pre;
+            for (int k = w0.za; k < w0.zb; k++) {
+                Raw pre;
+                load(k + 2, pre);
+                settle(nxt);
+                field(nxt, fn);
+                const bool zp_ok = k + 1 < g.nz || (g.per & 4u);
+                const uint32_t base = sz * (uint32_t)k;
+    #pragma unroll
+                for (int dy = 0; dy < 2; dy++) {
+                    const F4 rv = un(cur.r.c[dy]), di = un(cur.d.c[dy]);
+                    F4 tv;
+    #pragma unroll
+                    for (int kk = 0; kk < 4; kk++) {
+                        float fw[6];
+                        face6(o, cur.w, nxt.w, zp_ok, dy, kk, fw);
+                        const float rk = rv.v[kk] - m;
+                        const float a = act6(fc, below, fn.c, dy, kk, fw);
+                        const float out = fc.c[dy][kk] + omega * (rk - a) * di.v[kk];
+                        const bool on = di.v[kk] != 0.0f;
+                        tv.v[kk] = on ? out : 0.0f;
+                        acc[0] += on ? (double)rk * (double)out : 0.0;
+                    }
+                    st4(t, base + o.row0 + (uint32_t)dy * sy, tv);
+                }
+    #pragma unroll
+                for (int dy = 0; dy < 2; dy++)
+    #pragma unroll
+                    for (int kk = 0; kk < 4; kk++) below[dy][kk] = fc.c[dy][kk];
+                fc = fn;
+                cur = nxt;
+                nxt = pre;
+            }
         }
+        reduce_finish<1>(vg, acc, part, sc, 4, mode);
     }
-    reduce_finish<1>(acc, part, sc, 4, mode);
-}
+};
 
 // ---------------------------------------------------------------- launchers
 
@@ -349,11 +379,9 @@ void launch_march_smooth(hipStream_t s, int store, const Grid3& g, const Grid3& 
     const int un = use_next ? 1 : 0;
     if (store == STORE_BF16) {
         using T = __hip_bfloat16;
-        hipLaunchKernelGGL((k_march_smooth<T, T, 2>), G, B, 0, s, g, cg, r, r2, (const T*)dinv, (const T*)bx,
-                           (const T*)by, (const T*)bz, (const T*)zc, (T*)t, part, sc, mode, un, march_z());
+        go<K_k_march_smooth<T, T, 2>>(s, G.x, {g, cg, r, r2, (const T*)dinv, (const T*)bx, (const T*)by, (const T*)bz, (const T*)zc, (T*)t, part, sc, mode, un, march_z()});
     } else {
         using T = float;
-        hipLaunchKernelGGL((k_march_smooth<T, T, 2>), G, B, 0, s, g, cg, r, r2, (const T*)dinv, (const T*)bx,
-                           (const T*)by, (const T*)bz, (const T*)zc, (T*)t, part, sc, mode, un, march_z());
+        go<K_k_march_smooth<T, T, 2>>(s, G.x, {g, cg, r, r2, (const T*)dinv, (const T*)bx, (const T*)by, (const T*)bz, (const T*)zc, (T*)t, part, sc, mode, un, march_z()});
     }
 }

@@ -5,6 +5,7 @@
 // float for the FP32 solver, bfloat16 for the mixed-precision solver; arithmetic is FP32
 // (FP64 on the coarsest grid in the mixed solver).
 #include "mhdp_kernels.h"
+#include "launch.h"
 
 #define CONVERGED(sc) ((sc)->conv > 0.5)
 
@@ -58,165 +59,284 @@ __device__ __forceinline__ float fine_action(const Grid3& g, int x, int y, int z
 
 // diagonal_start (multigrid.wgsl:134-139): z = omega r / d from a zero guess.
 template <class CT, class RT, class VT>
-__global__ void __launch_bounds__(MHDP_BLOCK) k_dstart(uint32_t n, const RT* __restrict__ r,
-                                                       const CT* __restrict__ dinv, VT* __restrict__ z, Scal* sc) {
-    if (CONVERGED(sc)) return;
-    const float omega = (float)sc->omega;  // smoother weight (reference: 2/3)
-    uint32_t lo, hi;
-    block_range(n, lo, hi);
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
-        const float di = ldf(dinv[i]);
-        z[i] = cvt<VT>(di != 0.0f ? omega * ldf(r[i]) * di : 0.0f);
+struct K_k_dstart {
+    static constexpr int MINB = 0;
+    struct Args {
+        uint32_t n;
+        const RT* r;
+        const CT* dinv;
+        VT* z;
+        Scal* sc;
+    };
+    static __device__ void run(const Args& A_, const VGrid& vg) {
+        uint32_t n = A_.n;
+        const RT* __restrict__ r = A_.r;
+        const CT* __restrict__ dinv = A_.dinv;
+        VT* __restrict__ z = A_.z;
+        Scal* sc = A_.sc;
+        (void)vg;
+        if (CONVERGED(sc)) return;
+        const float omega = (float)sc->omega;  // smoother weight (reference: 2/3)
+        uint32_t lo, hi;
+        block_range(vg, n, lo, hi);
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+            const float di = ldf(dinv[i]);
+            z[i] = cvt<VT>(di != 0.0f ? omega * ldf(r[i]) * di : 0.0f);
+        }
     }
-}
+};
 
 // restrict_residual (multigrid.wgsl:75-83) from the fine level: sum over active children
 // of r - A z.
 template <class CT, class VT>
-__global__ void __launch_bounds__(MHDP_BLOCK) k_restrict_fine(Grid3 cg, Grid3 fg, const float* __restrict__ r,
-                                                              const CT* __restrict__ dinv, const CT* __restrict__ bx,
-                                                              const CT* __restrict__ by, const CT* __restrict__ bz,
-                                                              const VT* __restrict__ z, VT* __restrict__ rc,
-                                                              Scal* sc) {
-    if (CONVERGED(sc)) return;
-    uint32_t lo, hi;
-    block_range(cg.n, lo, hi);
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
-        int qx, qy, qz;
-        decode(cg, i, qx, qy, qz);
-        float rhs = 0.0f;
-        for (int dz = 0; dz < 2; dz++)
-            for (int dy = 0; dy < 2; dy++)
-                for (int dx = 0; dx < 2; dx++) {
-                    const int x = 2 * qx + dx, y = 2 * qy + dy, zz = 2 * qz + dz;
-                    if (x >= fg.nx || y >= fg.ny || zz >= fg.nz) continue;
-                    const uint32_t ci = (uint32_t)x + (uint32_t)fg.nx * ((uint32_t)y + (uint32_t)fg.ny * (uint32_t)zz);
-                    if (ldf(dinv[ci]) == 0.0f) continue;
-                    rhs += r[ci] - fine_action(fg, x, y, zz, ci, bx, by, bz, z, ldf(z[ci]));
-                }
-        rc[i] = cvt<VT>(rhs);
+struct K_k_restrict_fine {
+    static constexpr int MINB = 0;
+    struct Args {
+        Grid3 cg;
+        Grid3 fg;
+        const float* r;
+        const CT* dinv;
+        const CT* bx;
+        const CT* by;
+        const CT* bz;
+        const VT* z;
+        VT* rc;
+        Scal* sc;
+    };
+    static __device__ void run(const Args& A_, const VGrid& vg) {
+        Grid3 cg = A_.cg;
+        Grid3 fg = A_.fg;
+        const float* __restrict__ r = A_.r;
+        const CT* __restrict__ dinv = A_.dinv;
+        const CT* __restrict__ bx = A_.bx;
+        const CT* __restrict__ by = A_.by;
+        const CT* __restrict__ bz = A_.bz;
+        const VT* __restrict__ z = A_.z;
+        VT* __restrict__ rc = A_.rc;
+        Scal* sc = A_.sc;
+        (void)vg;
+        if (CONVERGED(sc)) return;
+        uint32_t lo, hi;
+        block_range(vg, cg.n, lo, hi);
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+            int qx, qy, qz;
+            decode(cg, i, qx, qy, qz);
+            float rhs = 0.0f;
+            for (int dz = 0; dz < 2; dz++)
+                for (int dy = 0; dy < 2; dy++)
+                    for (int dx = 0; dx < 2; dx++) {
+                        const int x = 2 * qx + dx, y = 2 * qy + dy, zz = 2 * qz + dz;
+                        if (x >= fg.nx || y >= fg.ny || zz >= fg.nz) continue;
+                        const uint32_t ci = (uint32_t)x + (uint32_t)fg.nx * ((uint32_t)y + (uint32_t)fg.ny * (uint32_t)zz);
+                        if (ldf(dinv[ci]) == 0.0f) continue;
+                        rhs += r[ci] - fine_action(fg, x, y, zz, ci, bx, by, bz, z, ldf(z[ci]));
+                    }
+            rc[i] = cvt<VT>(rhs);
+        }
     }
-}
+};
 
 // restrict_residual between coarse levels: every child contributes.
 template <class CT, class VT>
-__global__ void __launch_bounds__(MHDP_BLOCK) k_restrict_coarse(Grid3 cg, Grid3 fg, const VT* __restrict__ rf,
-                                                                const CT* __restrict__ cb0,
-                                                                const CT* __restrict__ cb1,
-                                                                const CT* __restrict__ cb2,
-                                                                const CT* __restrict__ react,
-                                                                const VT* __restrict__ zf, VT* __restrict__ rc,
-                                                                Scal* sc) {
-    if (CONVERGED(sc)) return;
-    uint32_t lo, hi;
-    block_range(cg.n, lo, hi);
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
-        int qx, qy, qz;
-        decode(cg, i, qx, qy, qz);
-        float rhs = 0.0f;
-        for (int dz = 0; dz < 2; dz++)
-            for (int dy = 0; dy < 2; dy++)
-                for (int dx = 0; dx < 2; dx++) {
-                    const int x = 2 * qx + dx, y = 2 * qy + dy, zz = 2 * qz + dz;
-                    if (x >= fg.nx || y >= fg.ny || zz >= fg.nz) continue;
-                    const uint32_t ci = (uint32_t)x + (uint32_t)fg.nx * ((uint32_t)y + (uint32_t)fg.ny * (uint32_t)zz);
-                    rhs += ldf(rf[ci]) - coarse_action(fg, x, y, zz, ci, cb0, cb1, cb2, react, zf, ldf(zf[ci]));
-                }
-        rc[i] = cvt<VT>(rhs);
+struct K_k_restrict_coarse {
+    static constexpr int MINB = 0;
+    struct Args {
+        Grid3 cg;
+        Grid3 fg;
+        const VT* rf;
+        const CT* cb0;
+        const CT* cb1;
+        const CT* cb2;
+        const CT* react;
+        const VT* zf;
+        VT* rc;
+        Scal* sc;
+    };
+    static __device__ void run(const Args& A_, const VGrid& vg) {
+        Grid3 cg = A_.cg;
+        Grid3 fg = A_.fg;
+        const VT* __restrict__ rf = A_.rf;
+        const CT* __restrict__ cb0 = A_.cb0;
+        const CT* __restrict__ cb1 = A_.cb1;
+        const CT* __restrict__ cb2 = A_.cb2;
+        const CT* __restrict__ react = A_.react;
+        const VT* __restrict__ zf = A_.zf;
+        VT* __restrict__ rc = A_.rc;
+        Scal* sc = A_.sc;
+        (void)vg;
+        if (CONVERGED(sc)) return;
+        uint32_t lo, hi;
+        block_range(vg, cg.n, lo, hi);
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+            int qx, qy, qz;
+            decode(cg, i, qx, qy, qz);
+            float rhs = 0.0f;
+            for (int dz = 0; dz < 2; dz++)
+                for (int dy = 0; dy < 2; dy++)
+                    for (int dx = 0; dx < 2; dx++) {
+                        const int x = 2 * qx + dx, y = 2 * qy + dy, zz = 2 * qz + dz;
+                        if (x >= fg.nx || y >= fg.ny || zz >= fg.nz) continue;
+                        const uint32_t ci = (uint32_t)x + (uint32_t)fg.nx * ((uint32_t)y + (uint32_t)fg.ny * (uint32_t)zz);
+                        rhs += ldf(rf[ci]) - coarse_action(fg, x, y, zz, ci, cb0, cb1, cb2, react, zf, ldf(zf[ci]));
+                    }
+            rc[i] = cvt<VT>(rhs);
+        }
     }
-}
+};
 
 // prolong (multigrid.wgsl:84-88): z_child += z_parent (fine children: active only).
 template <class CT, class VT>
-__global__ void __launch_bounds__(MHDP_BLOCK) k_prolong(Grid3 fg, Grid3 cg, int fine, const VT* __restrict__ zc,
-                                                        const CT* __restrict__ dinv, VT* __restrict__ zf, Scal* sc) {
-    if (CONVERGED(sc)) return;
-    uint32_t lo, hi;
-    block_range(fg.n, lo, hi);
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
-        int x, y, z;
-        decode(fg, i, x, y, z);
-        if (fine && ldf(dinv[i]) == 0.0f) continue;
-        const uint32_t pi =
-            (uint32_t)(x / 2) + (uint32_t)cg.nx * ((uint32_t)(y / 2) + (uint32_t)cg.ny * (uint32_t)(z / 2));
-        zf[i] = cvt<VT>(ldf(zf[i]) + (float)sc->pscale * ldf(zc[pi]));
+struct K_k_prolong {
+    static constexpr int MINB = 0;
+    struct Args {
+        Grid3 fg;
+        Grid3 cg;
+        int fine;
+        const VT* zc;
+        const CT* dinv;
+        VT* zf;
+        Scal* sc;
+    };
+    static __device__ void run(const Args& A_, const VGrid& vg) {
+        Grid3 fg = A_.fg;
+        Grid3 cg = A_.cg;
+        int fine = A_.fine;
+        const VT* __restrict__ zc = A_.zc;
+        const CT* __restrict__ dinv = A_.dinv;
+        VT* __restrict__ zf = A_.zf;
+        Scal* sc = A_.sc;
+        (void)vg;
+        if (CONVERGED(sc)) return;
+        uint32_t lo, hi;
+        block_range(vg, fg.n, lo, hi);
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+            int x, y, z;
+            decode(fg, i, x, y, z);
+            if (fine && ldf(dinv[i]) == 0.0f) continue;
+            const uint32_t pi =
+                (uint32_t)(x / 2) + (uint32_t)cg.nx * ((uint32_t)(y / 2) + (uint32_t)cg.ny * (uint32_t)(z / 2));
+            zf[i] = cvt<VT>(ldf(zf[i]) + (float)sc->pscale * ldf(zc[pi]));
+        }
     }
-}
+};
 
 // smooth_to_temp (multigrid.wgsl:66-73): t = z + omega (r - A z) / d, 0 where inactive.
 template <class CT, class RT, class VT, bool FINE>
-__global__ void __launch_bounds__(MHDP_BLOCK) k_smooth(Grid3 g, const RT* __restrict__ r, const CT* __restrict__ dinv,
-                                                       const CT* __restrict__ b0, const CT* __restrict__ b1,
-                                                       const CT* __restrict__ b2, const CT* __restrict__ react,
-                                                       const VT* __restrict__ z, VT* __restrict__ t, Scal* sc) {
-    if (CONVERGED(sc)) return;
-    const float omega = (float)sc->omega;  // smoother weight (reference: 2/3)
-    uint32_t lo, hi;
-    block_range(g.n, lo, hi);
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
-        const float di = ldf(dinv[i]);
-        float out = 0.0f;
-        if (di != 0.0f) {
-            int x, y, zz;
-            decode(g, i, x, y, zz);
-            const float c = ldf(z[i]);
-            const float a = FINE ? fine_action(g, x, y, zz, i, b0, b1, b2, z, c)
-                                 : coarse_action(g, x, y, zz, i, b0, b1, b2, react, z, c);
-            out = c + omega * (ldf(r[i]) - a) * di;
+struct K_k_smooth {
+    static constexpr int MINB = 0;
+    struct Args {
+        Grid3 g;
+        const RT* r;
+        const CT* dinv;
+        const CT* b0;
+        const CT* b1;
+        const CT* b2;
+        const CT* react;
+        const VT* z;
+        VT* t;
+        Scal* sc;
+    };
+    static __device__ void run(const Args& A_, const VGrid& vg) {
+        Grid3 g = A_.g;
+        const RT* __restrict__ r = A_.r;
+        const CT* __restrict__ dinv = A_.dinv;
+        const CT* __restrict__ b0 = A_.b0;
+        const CT* __restrict__ b1 = A_.b1;
+        const CT* __restrict__ b2 = A_.b2;
+        const CT* __restrict__ react = A_.react;
+        const VT* __restrict__ z = A_.z;
+        VT* __restrict__ t = A_.t;
+        Scal* sc = A_.sc;
+        (void)vg;
+        if (CONVERGED(sc)) return;
+        const float omega = (float)sc->omega;  // smoother weight (reference: 2/3)
+        uint32_t lo, hi;
+        block_range(vg, g.n, lo, hi);
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+            const float di = ldf(dinv[i]);
+            float out = 0.0f;
+            if (di != 0.0f) {
+                int x, y, zz;
+                decode(g, i, x, y, zz);
+                const float c = ldf(z[i]);
+                const float a = FINE ? fine_action(g, x, y, zz, i, b0, b1, b2, z, c)
+                                     : coarse_action(g, x, y, zz, i, b0, b1, b2, react, z, c);
+                out = c + omega * (ldf(r[i]) - a) * di;
+            }
+            t[i] = cvt<VT>(out);
         }
-        t[i] = cvt<VT>(out);
     }
-}
+};
 
 // coarse_solve (multigrid.wgsl:104-131): COARSE_SWEEPS = 8 weighted-Jacobi sweeps from zero
 // on the coarsest level (<= 256 cells) inside one workgroup.  AT is the arithmetic type.
 template <class CT, class RT, class VT, class AT, bool FINE>
-__global__ void __launch_bounds__(MHDP_BLOCK) k_coarse(Grid3 g, const RT* __restrict__ r,
-                                                       const CT* __restrict__ dinv, const CT* __restrict__ b0,
-                                                       const CT* __restrict__ b1, const CT* __restrict__ b2,
-                                                       const CT* __restrict__ react, VT* __restrict__ zout,
-                                                       Scal* sc) {
-    if (CONVERGED(sc)) return;
-    __shared__ AT cs[MHDP_BLOCK];
-    const uint32_t i = threadIdx.x;
-    const bool in_grid = i < g.n;
-    AT rhs = 0, di = 0, rc = 0;
-    int q[3] = {0, 0, 0};
-    if (in_grid) {
-        decode(g, i, q[0], q[1], q[2]);
-        di = (AT)ldf(dinv[i]);
-        rhs = di != (AT)0 ? (AT)ldf(r[i]) : (AT)0;
-        if (!FINE) rc = (AT)ldf(react[i]);
-    }
-    cs[i] = 0;
-    __syncthreads();
-    const int d[3] = {g.nx, g.ny, g.nz};
-    const CT* B[3] = {b0, b1, b2};
-    for (int step = 0; step < 8; step++) {
-        const AT center = cs[i];
-        AT next = 0;
-        if (in_grid && di != (AT)0) {
-            AT a = rc * center;
-            for (int ax = 0; ax < 3; ax++)
-                for (int s = -1; s <= 1; s += 2) {
-                    int n[3] = {q[0], q[1], q[2]};
-                    n[ax] = wrapc(n[ax] + s, d[ax], (g.per >> ax) & 1u);
-                    if (n[ax] < 0 || n[ax] >= d[ax]) continue;
-                    const uint32_t ni =
-                        (uint32_t)n[0] + (uint32_t)g.nx * ((uint32_t)n[1] + (uint32_t)g.ny * (uint32_t)n[2]);
-                    AT nbv = cs[ni];
-                    if (FINE && ldf(dinv[ni]) == 0.0f) nbv = 0;
-                    const AT beta = (AT)ldf(s > 0 ? B[ax][ni] : B[ax][i]);
-                    a += beta * (center - nbv);
-                }
-            next = center + (AT)sc->omega * (rhs - a) * di;
+struct K_k_coarse {
+    static constexpr int MINB = 0;
+    struct Args {
+        Grid3 g;
+        const RT* r;
+        const CT* dinv;
+        const CT* b0;
+        const CT* b1;
+        const CT* b2;
+        const CT* react;
+        VT* zout;
+        Scal* sc;
+    };
+    static __device__ void run(const Args& A_, const VGrid& vg) {
+        Grid3 g = A_.g;
+        const RT* __restrict__ r = A_.r;
+        const CT* __restrict__ dinv = A_.dinv;
+        const CT* __restrict__ b0 = A_.b0;
+        const CT* __restrict__ b1 = A_.b1;
+        const CT* __restrict__ b2 = A_.b2;
+        const CT* __restrict__ react = A_.react;
+        VT* __restrict__ zout = A_.zout;
+        Scal* sc = A_.sc;
+        (void)vg;
+        if (CONVERGED(sc)) return;
+        __shared__ AT cs[MHDP_BLOCK];
+        const uint32_t i = threadIdx.x;
+        const bool in_grid = i < g.n;
+        AT rhs = 0, di = 0, rc = 0;
+        int q[3] = {0, 0, 0};
+        if (in_grid) {
+            decode(g, i, q[0], q[1], q[2]);
+            di = (AT)ldf(dinv[i]);
+            rhs = di != (AT)0 ? (AT)ldf(r[i]) : (AT)0;
+            if (!FINE) rc = (AT)ldf(react[i]);
         }
+        cs[i] = 0;
         __syncthreads();
-        cs[i] = next;
-        __syncthreads();
+        const int d[3] = {g.nx, g.ny, g.nz};
+        const CT* B[3] = {b0, b1, b2};
+        for (int step = 0; step < 8; step++) {
+            const AT center = cs[i];
+            AT next = 0;
+            if (in_grid && di != (AT)0) {
+                AT a = rc * center;
+                for (int ax = 0; ax < 3; ax++)
+                    for (int s = -1; s <= 1; s += 2) {
+                        int n[3] = {q[0], q[1], q[2]};
+                        n[ax] = wrapc(n[ax] + s, d[ax], (g.per >> ax) & 1u);
+                        if (n[ax] < 0 || n[ax] >= d[ax]) continue;
+                        const uint32_t ni =
+                            (uint32_t)n[0] + (uint32_t)g.nx * ((uint32_t)n[1] + (uint32_t)g.ny * (uint32_t)n[2]);
+                        AT nbv = cs[ni];
+                        if (FINE && ldf(dinv[ni]) == 0.0f) nbv = 0;
+                        const AT beta = (AT)ldf(s > 0 ? B[ax][ni] : B[ax][i]);
+                        a += beta * (center - nbv);
+                    }
+                next = center + (AT)sc->omega * (rhs - a) * di;
+            }
+            __syncthreads();
+            cs[i] = next;
+            __syncthreads();
+        }
+        if (in_grid) zout[i] = cvt<VT>((float)cs[i]);
     }
-    if (in_grid) zout[i] = cvt<VT>((float)cs[i]);
-}
+};
 
 template <class CT, class VT, class AT>
 static void vcycle_t(hipStream_t s, int L, LevelDev* lv, const float* r0, Scal* sc, void** zfinal, bool f16,
@@ -244,8 +364,7 @@ static void vcycle_t(hipStream_t s, int L, LevelDev* lv, const float* r0, Scal* 
     const dim3 B(MHDP_BLOCK);
     if (L == 1) {
         LevelDev& f = lv[0];
-        hipLaunchKernelGGL((k_coarse<CT, float, VT, AT, true>), dim3(1), B, 0, s, f.g, r0, C(f, 4), C(f, 0), C(f, 1),
-                           C(f, 2), (const CT*)nullptr, (VT*)f.z, sc);
+        go<K_k_coarse<CT, float, VT, AT, true>>(s, 1, {f.g, r0, C(f, 4), C(f, 0), C(f, 1), C(f, 2), (const CT*)nullptr, (VT*)f.z, sc});
         *zfinal = f.z;
         return;
     }
@@ -254,39 +373,31 @@ static void vcycle_t(hipStream_t s, int L, LevelDev* lv, const float* r0, Scal* 
         LevelDev& a = lv[l];
         LevelDev& c = lv[l + 1];
         if (l == 0) {
-            hipLaunchKernelGGL((k_dstart<CT, float, VT>), dim3(grid_blocks(a.g.n)), B, 0, s, a.g.n, r0, C(a, 4),
-                               (VT*)a.z, sc);
+            go<K_k_dstart<CT, float, VT>>(s, grid_blocks(a.g.n), {a.g.n, r0, C(a, 4), (VT*)a.z, sc});
             T.before(3, s);
-            hipLaunchKernelGGL((k_restrict_fine<CT, VT>), dim3(grid_blocks(c.g.n)), B, 0, s, c.g, a.g, r0, C(a, 4),
-                               C(a, 0), C(a, 1), C(a, 2), (const VT*)a.z, (VT*)c.r, sc);
+            go<K_k_restrict_fine<CT, VT>>(s, grid_blocks(c.g.n), {c.g, a.g, r0, C(a, 4), C(a, 0), C(a, 1), C(a, 2), (const VT*)a.z, (VT*)c.r, sc});
             T.after(3, s);
         } else {
-            hipLaunchKernelGGL((k_dstart<CT, VT, VT>), dim3(grid_blocks(a.g.n)), B, 0, s, a.g.n, (const VT*)a.r,
-                               C(a, 4), (VT*)a.z, sc);
-            hipLaunchKernelGGL((k_restrict_coarse<CT, VT>), dim3(grid_blocks(c.g.n)), B, 0, s, c.g, a.g,
-                               (const VT*)a.r, C(a, 0), C(a, 1), C(a, 2), C(a, 3), (const VT*)a.z, (VT*)c.r, sc);
+            go<K_k_dstart<CT, VT, VT>>(s, grid_blocks(a.g.n), {a.g.n, (const VT*)a.r, C(a, 4), (VT*)a.z, sc});
+            go<K_k_restrict_coarse<CT, VT>>(s, grid_blocks(c.g.n), {c.g, a.g, (const VT*)a.r, C(a, 0), C(a, 1), C(a, 2), C(a, 3), (const VT*)a.z, (VT*)c.r, sc});
         }
     }
     {
         LevelDev& c = lv[L - 1];
-        hipLaunchKernelGGL((k_coarse<CT, VT, VT, AT, false>), dim3(1), B, 0, s, c.g, (const VT*)c.r, C(c, 4),
-                           C(c, 0), C(c, 1), C(c, 2), C(c, 3), (VT*)c.z, sc);
+        go<K_k_coarse<CT, VT, VT, AT, false>>(s, 1, {c.g, (const VT*)c.r, C(c, 4), C(c, 0), C(c, 1), C(c, 2), C(c, 3), (VT*)c.z, sc});
     }
     // ascend
     for (int l = L - 2; l >= 0; l--) {
         LevelDev& a = lv[l];
         LevelDev& c = lv[l + 1];
         const VT* zc = (const VT*)(l + 1 == L - 1 ? c.z : c.t);
-        hipLaunchKernelGGL((k_prolong<CT, VT>), dim3(grid_blocks(a.g.n)), B, 0, s, a.g, c.g, l == 0 ? 1 : 0, zc,
-                           C(a, 4), (VT*)a.z, sc);
+        go<K_k_prolong<CT, VT>>(s, grid_blocks(a.g.n), {a.g, c.g, l == 0 ? 1 : 0, zc, C(a, 4), (VT*)a.z, sc});
         if (l == 0) {
             T.before(2, s);
-            hipLaunchKernelGGL((k_smooth<CT, float, VT, true>), dim3(grid_blocks(a.g.n)), B, 0, s, a.g, r0, C(a, 4),
-                               C(a, 0), C(a, 1), C(a, 2), (const CT*)nullptr, (const VT*)a.z, (VT*)a.t, sc);
+            go<K_k_smooth<CT, float, VT, true>>(s, grid_blocks(a.g.n), {a.g, r0, C(a, 4), C(a, 0), C(a, 1), C(a, 2), (const CT*)nullptr, (const VT*)a.z, (VT*)a.t, sc});
             T.after(2, s);
         } else
-            hipLaunchKernelGGL((k_smooth<CT, VT, VT, false>), dim3(grid_blocks(a.g.n)), B, 0, s, a.g, (const VT*)a.r,
-                               C(a, 4), C(a, 0), C(a, 1), C(a, 2), C(a, 3), (const VT*)a.z, (VT*)a.t, sc);
+            go<K_k_smooth<CT, VT, VT, false>>(s, grid_blocks(a.g.n), {a.g, (const VT*)a.r, C(a, 4), C(a, 0), C(a, 1), C(a, 2), C(a, 3), (const VT*)a.z, (VT*)a.t, sc});
     }
     *zfinal = lv[0].t;
 }
@@ -310,15 +421,12 @@ static void vcycle_coarse_t(hipStream_t s, int L, LevelDev* lv, Scal* sc, void**
             launch_brick_restrict_coarse(s, store, c.g, a.g, a.r, C(a, 4), C(a, 0), C(a, 1), C(a, 2), C(a, 3), c.r, sc);
             continue;
         }
-        hipLaunchKernelGGL((k_dstart<CT, VT, VT>), dim3(grid_blocks(a.g.n)), B, 0, s, a.g.n, (const VT*)a.r, C(a, 4),
-                           (VT*)a.z, sc);
-        hipLaunchKernelGGL((k_restrict_coarse<CT, VT>), dim3(grid_blocks(c.g.n)), B, 0, s, c.g, a.g, (const VT*)a.r,
-                           C(a, 0), C(a, 1), C(a, 2), C(a, 3), (const VT*)a.z, (VT*)c.r, sc);
+        go<K_k_dstart<CT, VT, VT>>(s, grid_blocks(a.g.n), {a.g.n, (const VT*)a.r, C(a, 4), (VT*)a.z, sc});
+        go<K_k_restrict_coarse<CT, VT>>(s, grid_blocks(c.g.n), {c.g, a.g, (const VT*)a.r, C(a, 0), C(a, 1), C(a, 2), C(a, 3), (const VT*)a.z, (VT*)c.r, sc});
     }
     {
         LevelDev& c = lv[L - 1];
-        hipLaunchKernelGGL((k_coarse<CT, VT, VT, AT, false>), dim3(1), B, 0, s, c.g, (const VT*)c.r, C(c, 4), C(c, 0),
-                           C(c, 1), C(c, 2), C(c, 3), (VT*)c.z, sc);
+        go<K_k_coarse<CT, VT, VT, AT, false>>(s, 1, {c.g, (const VT*)c.r, C(c, 4), C(c, 0), C(c, 1), C(c, 2), C(c, 3), (VT*)c.z, sc});
     }
     for (int l = L - 2; l >= 1; l--) {
         LevelDev& a = lv[l];
@@ -328,10 +436,8 @@ static void vcycle_coarse_t(hipStream_t s, int L, LevelDev* lv, Scal* sc, void**
             launch_brick_smooth_coarse(s, store, a.g, c.g, a.r, C(a, 4), C(a, 0), C(a, 1), C(a, 2), C(a, 3), zc, a.t, sc);
             continue;
         }
-        hipLaunchKernelGGL((k_prolong<CT, VT>), dim3(grid_blocks(a.g.n)), B, 0, s, a.g, c.g, 0, zc, C(a, 4),
-                           (VT*)a.z, sc);
-        hipLaunchKernelGGL((k_smooth<CT, VT, VT, false>), dim3(grid_blocks(a.g.n)), B, 0, s, a.g, (const VT*)a.r,
-                           C(a, 4), C(a, 0), C(a, 1), C(a, 2), C(a, 3), (const VT*)a.z, (VT*)a.t, sc);
+        go<K_k_prolong<CT, VT>>(s, grid_blocks(a.g.n), {a.g, c.g, 0, zc, C(a, 4), (VT*)a.z, sc});
+        go<K_k_smooth<CT, VT, VT, false>>(s, grid_blocks(a.g.n), {a.g, (const VT*)a.r, C(a, 4), C(a, 0), C(a, 1), C(a, 2), C(a, 3), (const VT*)a.z, (VT*)a.t, sc});
     }
     *z1out = (L - 1 == 1) ? lv[1].z : lv[1].t;
 }

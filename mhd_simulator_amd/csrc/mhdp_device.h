@@ -90,6 +90,22 @@ __device__ __forceinline__ void block_range(uint32_t n, uint32_t& lo, uint32_t& 
     hi = h < n ? (uint32_t)h : n;
 }
 
+// The blocks of one problem inside a launch: its logical block and the number of its blocks.
+// A single-problem launch is (logical_block(), gridDim.x); a batched launch gives each problem
+// a contiguous run of blocks and the same per-problem numbering, so results are identical.
+struct VGrid {
+    uint32_t b, n;
+};
+__device__ __forceinline__ VGrid vgrid() { return VGrid{logical_block(), gridDim.x}; }
+__device__ __forceinline__ void block_range(const VGrid& vg, uint32_t n, uint32_t& lo, uint32_t& hi) {
+    uint32_t chunk = (n + vg.n - 1) / vg.n;
+    chunk = (chunk + MHDP_BLOCK - 1) / MHDP_BLOCK * MHDP_BLOCK;
+    uint64_t l = (uint64_t)vg.b * chunk;
+    uint64_t h = l + chunk;
+    lo = l < n ? (uint32_t)l : n;
+    hi = h < n ? (uint32_t)h : n;
+}
+
 __device__ __forceinline__ void decode(const Grid3& g, uint32_t i, int& x, int& y, int& z) {
     uint32_t nxy = (uint32_t)g.nx * (uint32_t)g.ny;
     z = (int)(i / nxy);
@@ -268,24 +284,24 @@ __device__ inline void finalize(Scal* s, int mode, const double t[4]) {
 // fixed order (deterministic) and runs the scalar update `mode` — one launch per
 // reference reduction pair (e.g. pressure_apply + alpha_reduce).
 template <int NV>
-__device__ __forceinline__ void reduce_finish(double (&v)[NV], double* __restrict__ part, Scal* s, int ticket,
-                                              int mode) {
+__device__ __forceinline__ void reduce_finish(const VGrid& vg, double (&v)[NV], double* __restrict__ part, Scal* s,
+                                              int ticket, int mode) {
     block_sum<NV>(v);
     __shared__ int last;
-    const uint32_t lb = logical_block();
+    const uint32_t lb = vg.b;
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < 4; k++) part[4 * lb + k] = k < NV ? v[k] : 0.0;
         __threadfence();
         unsigned t = atomicAdd(&s->ticket[ticket], 1u);
-        last = (t == gridDim.x - 1);
+        last = (t == vg.n - 1);
     }
     __syncthreads();
     if (!last) return;
     __threadfence();
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     const volatile double* vp = part;
-    for (uint32_t j = threadIdx.x; j < gridDim.x; j += MHDP_BLOCK) {
+    for (uint32_t j = threadIdx.x; j < vg.n; j += MHDP_BLOCK) {
 #pragma unroll
         for (int k = 0; k < 4; k++) acc[k] += vp[4 * j + k];
     }
@@ -295,6 +311,12 @@ __device__ __forceinline__ void reduce_finish(double (&v)[NV], double* __restric
         s->ticket[ticket] = 0u;
         __threadfence();
     }
+}
+
+template <int NV>
+__device__ __forceinline__ void reduce_finish(double (&v)[NV], double* __restrict__ part, Scal* s, int ticket,
+                                              int mode) {
+    reduce_finish<NV>(vgrid(), v, part, s, ticket, mode);
 }
 
 // ---------------------------------------------------------------- launch helpers (host)

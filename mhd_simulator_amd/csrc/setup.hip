@@ -2,6 +2,7 @@
 // correction and the Galerkin coarse graphs.  Built with -ffp-contract=off: their FP32
 // results are bit-identical to the reference formulas evaluated in IEEE order.
 #include "mhdp_kernels.h"
+#include "launch.h"
 #include "rows.h"
 
 namespace {
@@ -49,51 +50,72 @@ __device__ __forceinline__ float mobility_at(const Grid3& g, const PhaseP& ph, c
 
 // pressure_coefficients (solver.wgsl:93-103) in cell-indexed lower-face form, plus
 // diagonal() (solver.wgsl:60-64) and the active-cell reciprocal.
-__global__ void __launch_bounds__(MHDP_BLOCK) k_coefficients(Grid3 g, PhaseP ph, const uint32_t* __restrict__ mask,
-                                                             const float* __restrict__ alpha, float* __restrict__ bx,
-                                                             float* __restrict__ by, float* __restrict__ bz,
-                                                             float* __restrict__ diag, float* __restrict__ dinv) {
-    uint32_t lo, hi;
-    block_range(g.n, lo, hi);
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
-        int x, y, z;
-        decode(g, i, x, y, z);
-        Pos pos{{x, y, z}};
-        const uint32_t m = mask[i];
-        const bool fc = m != 0u;
-        float b[3];
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            Pos left = pos;
-            left.p[a] -= 1;
-            b[a] = (fc && fluid_at(g, mask, left)) ? mobility_at(g, ph, alpha, pos, a) : 0.0f;
-        }
-        bx[i] = b[0];
-        by[i] = b[1];
-        bz[i] = b[2];
-        float d = 0.0f;
-        if (fc) {
-#pragma unroll
+struct K_k_coefficients {
+    static constexpr int MINB = 0;
+    struct Args {
+        Grid3 g;
+        PhaseP ph;
+        const uint32_t* mask;
+        const float* alpha;
+        float* bx;
+        float* by;
+        float* bz;
+        float* diag;
+        float* dinv;
+    };
+    static __device__ void run(const Args& A_, const VGrid& vg) {
+        Grid3 g = A_.g;
+        PhaseP ph = A_.ph;
+        const uint32_t* __restrict__ mask = A_.mask;
+        const float* __restrict__ alpha = A_.alpha;
+        float* __restrict__ bx = A_.bx;
+        float* __restrict__ by = A_.by;
+        float* __restrict__ bz = A_.bz;
+        float* __restrict__ diag = A_.diag;
+        float* __restrict__ dinv = A_.dinv;
+        (void)vg;
+        uint32_t lo, hi;
+        block_range(vg, g.n, lo, hi);
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+            int x, y, z;
+            decode(g, i, x, y, z);
+            Pos pos{{x, y, z}};
+            const uint32_t m = mask[i];
+            const bool fc = m != 0u;
+            float b[3];
+    #pragma unroll
             for (int a = 0; a < 3; a++) {
-#pragma unroll
-                for (int s = -1; s <= 1; s += 2) {
-                    Pos q = pos;
-                    q.p[a] += s;
-                    if (fluid_at(g, mask, q)) {
-                        Pos fp = pos;
-                        if (s > 0) fp.p[a] += 1;
-                        d += mobility_at(g, ph, alpha, fp, a);
+                Pos left = pos;
+                left.p[a] -= 1;
+                b[a] = (fc && fluid_at(g, mask, left)) ? mobility_at(g, ph, alpha, pos, a) : 0.0f;
+            }
+            bx[i] = b[0];
+            by[i] = b[1];
+            bz[i] = b[2];
+            float d = 0.0f;
+            if (fc) {
+    #pragma unroll
+                for (int a = 0; a < 3; a++) {
+    #pragma unroll
+                    for (int s = -1; s <= 1; s += 2) {
+                        Pos q = pos;
+                        q.p[a] += s;
+                        if (fluid_at(g, mask, q)) {
+                            Pos fp = pos;
+                            if (s > 0) fp.p[a] += 1;
+                            d += mobility_at(g, ph, alpha, fp, a);
+                        }
                     }
                 }
+                if (!ph.meanfree && (m & 3u) == 2u) d = 1.0f;
+                else d = fmaxf(d, 1e-30f);
             }
-            if (!ph.meanfree && (m & 3u) == 2u) d = 1.0f;
-            else d = fmaxf(d, 1e-30f);
+            diag[i] = d;
+            const bool active = fc && (ph.meanfree || (m & 3u) == 1u);
+            dinv[i] = active ? 1.0f / d : 0.0f;
         }
-        diag[i] = d;
-        const bool active = fc && (ph.meanfree || (m & 3u) == 1u);
-        dinv[i] = active ? 1.0f / d : 0.0f;
     }
-}
+};
 
 // Canonical face index in plane `a` of the upper face of cell (x,y,z) along a.
 __device__ __forceinline__ uint32_t plane_index(const Grid3& g, int a, int fx, int fy, int fz) {
@@ -104,156 +126,231 @@ __device__ __forceinline__ uint32_t plane_index(const Grid3& g, int a, int fx, i
 // rhs() without the pinned-anchor zeroing (solver.wgsl:104-107, 192): b = -h^2 div(u*)
 // over open faces.  Openness is read from the mobilities (a face is open iff beta > 0).
 template <class T>
-__global__ void __launch_bounds__(MHDP_BLOCK) k_rhs(Grid3 g, float h, const uint32_t* __restrict__ mask,
-                                                    const float* __restrict__ bx, const float* __restrict__ by,
-                                                    const float* __restrict__ bz, const float* __restrict__ ustar,
-                                                    uint64_t off1, uint64_t off2, T* __restrict__ b) {
-    uint32_t lo, hi;
-    block_range(g.n, lo, hi);
-    const float* U[3] = {ustar, ustar + off1, ustar + off2};
-    const float* B[3] = {bx, by, bz};
-    const int d[3] = {g.nx, g.ny, g.nz};
-    const uint32_t st[3] = {1u, (uint32_t)g.nx, (uint32_t)g.nx * (uint32_t)g.ny};
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
-        T out = (T)0;
-        if (mask[i] != 0u) {
-            int x, y, z;
-            decode(g, i, x, y, z);
-            int c[3] = {x, y, z};
-            T sum = (T)0;
-#pragma unroll
-            for (int a = 0; a < 3; a++) {
-                // upper face: cell c + e_a (wrapped) lower face
-                float up = 0.0f;
-                int u[3] = {c[0], c[1], c[2]};
-                u[a] += 1;
-                bool exists = true;
-                uint32_t ui = i + st[a];
-                if (u[a] >= d[a]) {
-                    if (g.per & (1u << a)) { u[a] = 0; ui = i - (uint32_t)(d[a] - 1) * st[a]; }
-                    else exists = false;
+struct K_k_rhs {
+    static constexpr int MINB = 0;
+    struct Args {
+        Grid3 g;
+        float h;
+        const uint32_t* mask;
+        const float* bx;
+        const float* by;
+        const float* bz;
+        const float* ustar;
+        uint64_t off1;
+        uint64_t off2;
+        T* b;
+    };
+    static __device__ void run(const Args& A_, const VGrid& vg) {
+        Grid3 g = A_.g;
+        float h = A_.h;
+        const uint32_t* __restrict__ mask = A_.mask;
+        const float* __restrict__ bx = A_.bx;
+        const float* __restrict__ by = A_.by;
+        const float* __restrict__ bz = A_.bz;
+        const float* __restrict__ ustar = A_.ustar;
+        uint64_t off1 = A_.off1;
+        uint64_t off2 = A_.off2;
+        T* __restrict__ b = A_.b;
+        (void)vg;
+        uint32_t lo, hi;
+        block_range(vg, g.n, lo, hi);
+        const float* U[3] = {ustar, ustar + off1, ustar + off2};
+        const float* B[3] = {bx, by, bz};
+        const int d[3] = {g.nx, g.ny, g.nz};
+        const uint32_t st[3] = {1u, (uint32_t)g.nx, (uint32_t)g.nx * (uint32_t)g.ny};
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+            T out = (T)0;
+            if (mask[i] != 0u) {
+                int x, y, z;
+                decode(g, i, x, y, z);
+                int c[3] = {x, y, z};
+                T sum = (T)0;
+    #pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    // upper face: cell c + e_a (wrapped) lower face
+                    float up = 0.0f;
+                    int u[3] = {c[0], c[1], c[2]};
+                    u[a] += 1;
+                    bool exists = true;
+                    uint32_t ui = i + st[a];
+                    if (u[a] >= d[a]) {
+                        if (g.per & (1u << a)) { u[a] = 0; ui = i - (uint32_t)(d[a] - 1) * st[a]; }
+                        else exists = false;
+                    }
+                    if (exists && B[a][ui] > 0.0f) up = U[a][plane_index(g, a, u[0], u[1], u[2])];
+                    float dn = 0.0f;
+                    if (B[a][i] > 0.0f) dn = U[a][plane_index(g, a, c[0], c[1], c[2])];
+                    sum += (T)up - (T)dn;
                 }
-                if (exists && B[a][ui] > 0.0f) up = U[a][plane_index(g, a, u[0], u[1], u[2])];
-                float dn = 0.0f;
-                if (B[a][i] > 0.0f) dn = U[a][plane_index(g, a, c[0], c[1], c[2])];
-                sum += (T)up - (T)dn;
+                T div = sum / (T)h;
+                out = -(T)h * (T)h * div;
             }
-            T div = sum / (T)h;
-            out = -(T)h * (T)h * div;
+            b[i] = out;
         }
-        b[i] = out;
     }
-}
+};
 
 // correct() (solver.wgsl:245-250) over every entry of the three face planes.
 template <class PT>
-__global__ void __launch_bounds__(MHDP_BLOCK) k_correct(Grid3 g, float h, const float* __restrict__ bx,
-                                                        const float* __restrict__ by, const float* __restrict__ bz,
-                                                        const PT* __restrict__ phi, const float* __restrict__ ustar,
-                                                        float* __restrict__ uout, uint32_t f0, uint32_t f1,
-                                                        uint32_t f2) {
-    const uint32_t total = f0 + f1 + f2;
-    uint32_t lo, hi;
-    block_range(total, lo, hi);
-    const float* B[3] = {bx, by, bz};
-    const int d[3] = {g.nx, g.ny, g.nz};
-    const uint32_t st[3] = {1u, (uint32_t)g.nx, (uint32_t)g.nx * (uint32_t)g.ny};
-    for (uint32_t f = lo + threadIdx.x; f < hi; f += MHDP_BLOCK) {
-        int a;
-        uint32_t k;
-        if (f < f0) { a = 0; k = f; }
-        else if (f < f0 + f1) { a = 1; k = f - f0; }
-        else { a = 2; k = f - f0 - f1; }
-        const uint32_t Px = (uint32_t)g.nx + (a == 0), Py = (uint32_t)g.ny + (a == 1);
-        int q[3];
-        q[0] = (int)(k % Px);
-        q[1] = (int)((k / Px) % Py);
-        q[2] = (int)(k / (Px * Py));
-        float out = 0.0f;
-        bool ok = true;
-        if (q[a] >= d[a]) {
-            if (g.per & (1u << a)) q[a] = 0;
-            else ok = false;
-        }
-        if (ok) {
-            const uint32_t ci = (uint32_t)q[0] + (uint32_t)g.nx * ((uint32_t)q[1] + (uint32_t)g.ny * (uint32_t)q[2]);
-            const float beta = B[a][ci];
-            if (beta > 0.0f) {
-                uint32_t li = q[a] > 0 ? ci - st[a] : ci + (uint32_t)(d[a] - 1) * st[a];
-                const float* U = ustar + (a == 0 ? 0 : (a == 1 ? f0 : f0 + f1));
-                const float uf = U[plane_index(g, a, q[0], q[1], q[2])];
-                if (sizeof(PT) == 4) {
-                    out = uf - (float)beta * ((float)phi[ci] - (float)phi[li]) / h;
-                } else {
-                    double o = (double)uf - (double)beta * ((double)phi[ci] - (double)phi[li]) / (double)h;
-                    out = (float)o;
+struct K_k_correct {
+    static constexpr int MINB = 0;
+    struct Args {
+        Grid3 g;
+        float h;
+        const float* bx;
+        const float* by;
+        const float* bz;
+        const PT* phi;
+        const float* ustar;
+        float* uout;
+        uint32_t f0;
+        uint32_t f1;
+        uint32_t f2;
+    };
+    static __device__ void run(const Args& A_, const VGrid& vg) {
+        Grid3 g = A_.g;
+        float h = A_.h;
+        const float* __restrict__ bx = A_.bx;
+        const float* __restrict__ by = A_.by;
+        const float* __restrict__ bz = A_.bz;
+        const PT* __restrict__ phi = A_.phi;
+        const float* __restrict__ ustar = A_.ustar;
+        float* __restrict__ uout = A_.uout;
+        uint32_t f0 = A_.f0;
+        uint32_t f1 = A_.f1;
+        uint32_t f2 = A_.f2;
+        (void)vg;
+        const uint32_t total = f0 + f1 + f2;
+        uint32_t lo, hi;
+        block_range(vg, total, lo, hi);
+        const float* B[3] = {bx, by, bz};
+        const int d[3] = {g.nx, g.ny, g.nz};
+        const uint32_t st[3] = {1u, (uint32_t)g.nx, (uint32_t)g.nx * (uint32_t)g.ny};
+        for (uint32_t f = lo + threadIdx.x; f < hi; f += MHDP_BLOCK) {
+            int a;
+            uint32_t k;
+            if (f < f0) { a = 0; k = f; }
+            else if (f < f0 + f1) { a = 1; k = f - f0; }
+            else { a = 2; k = f - f0 - f1; }
+            const uint32_t Px = (uint32_t)g.nx + (a == 0), Py = (uint32_t)g.ny + (a == 1);
+            int q[3];
+            q[0] = (int)(k % Px);
+            q[1] = (int)((k / Px) % Py);
+            q[2] = (int)(k / (Px * Py));
+            float out = 0.0f;
+            bool ok = true;
+            if (q[a] >= d[a]) {
+                if (g.per & (1u << a)) q[a] = 0;
+                else ok = false;
+            }
+            if (ok) {
+                const uint32_t ci = (uint32_t)q[0] + (uint32_t)g.nx * ((uint32_t)q[1] + (uint32_t)g.ny * (uint32_t)q[2]);
+                const float beta = B[a][ci];
+                if (beta > 0.0f) {
+                    uint32_t li = q[a] > 0 ? ci - st[a] : ci + (uint32_t)(d[a] - 1) * st[a];
+                    const float* U = ustar + (a == 0 ? 0 : (a == 1 ? f0 : f0 + f1));
+                    const float uf = U[plane_index(g, a, q[0], q[1], q[2])];
+                    if (sizeof(PT) == 4) {
+                        out = uf - (float)beta * ((float)phi[ci] - (float)phi[li]) / h;
+                    } else {
+                        double o = (double)uf - (double)beta * ((double)phi[ci] - (double)phi[li]) / (double)h;
+                        out = (float)o;
+                    }
                 }
             }
+            uout[f] = out;
         }
-        uout[f] = out;
     }
-}
+};
 
 // build() (multigrid.wgsl:50-65): exact piecewise-constant Galerkin graph of one coarse
 // level.  FINE: children are the fine cells (cached mobilities, active mask, anchors
 // become reactions in the pinned gauge); otherwise children are the previous coarse level.
 template <bool FINE>
-__global__ void __launch_bounds__(MHDP_BLOCK) k_mg_build(Grid3 cg, Grid3 fg, int meanfree,
-                                                         const uint32_t* __restrict__ mask,
-                                                         const float* __restrict__ fdinv,
-                                                         const float* __restrict__ fbx, const float* __restrict__ fby,
-                                                         const float* __restrict__ fbz,
-                                                         const float* __restrict__ freact, float* __restrict__ cbx,
-                                                         float* __restrict__ cby, float* __restrict__ cbz,
-                                                         float* __restrict__ creact, float* __restrict__ cdiag,
-                                                         float* __restrict__ cdinv) {
-    uint32_t lo, hi;
-    block_range(cg.n, lo, hi);
-    const float* FB[3] = {fbx, fby, fbz};
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
-        int qx, qy, qz;
-        decode(cg, i, qx, qy, qz);
-        float c4[4] = {0.f, 0.f, 0.f, 0.f};
-        float diag = 0.f;
-        for (int z = 0; z < 2; z++)
-            for (int y = 0; y < 2; y++)
-                for (int x = 0; x < 2; x++) {
-                    Pos child{{2 * qx + x, 2 * qy + y, 2 * qz + z}};
-                    if (child.p[0] >= fg.nx || child.p[1] >= fg.ny || child.p[2] >= fg.nz) continue;
-                    const uint32_t ci = cidx(fg, child);
-                    if (FINE) {
-                        if (fdinv[ci] == 0.0f) continue;
-                    } else {
-                        const float gamma = freact[ci];
-                        c4[3] += gamma;
-                        diag += gamma;
-                    }
-                    for (int a = 0; a < 3; a++)
-                        for (int s = -1; s <= 1; s += 2) {
-                            Pos n = child;
-                            n.p[a] += s;
-                            if (!canon_valid(fg, n)) continue;
-                            const uint32_t ni = cidx(fg, n);
-                            const float beta = s > 0 ? FB[a][ni] : FB[a][ci];
-                            if (beta == 0.0f) continue;
-                            if (FINE && !meanfree && (mask[ni] & 3u) == 2u) {
-                                c4[3] += beta;
-                                diag += beta;
-                                continue;
-                            }
-                            if (n.p[0] / 2 == qx && n.p[1] / 2 == qy && n.p[2] / 2 == qz) continue;
-                            diag += beta;
-                            if (s < 0) c4[a] += beta;
+struct K_k_mg_build {
+    static constexpr int MINB = 0;
+    struct Args {
+        Grid3 cg;
+        Grid3 fg;
+        int meanfree;
+        const uint32_t* mask;
+        const float* fdinv;
+        const float* fbx;
+        const float* fby;
+        const float* fbz;
+        const float* freact;
+        float* cbx;
+        float* cby;
+        float* cbz;
+        float* creact;
+        float* cdiag;
+        float* cdinv;
+    };
+    static __device__ void run(const Args& A_, const VGrid& vg) {
+        Grid3 cg = A_.cg;
+        Grid3 fg = A_.fg;
+        int meanfree = A_.meanfree;
+        const uint32_t* __restrict__ mask = A_.mask;
+        const float* __restrict__ fdinv = A_.fdinv;
+        const float* __restrict__ fbx = A_.fbx;
+        const float* __restrict__ fby = A_.fby;
+        const float* __restrict__ fbz = A_.fbz;
+        const float* __restrict__ freact = A_.freact;
+        float* __restrict__ cbx = A_.cbx;
+        float* __restrict__ cby = A_.cby;
+        float* __restrict__ cbz = A_.cbz;
+        float* __restrict__ creact = A_.creact;
+        float* __restrict__ cdiag = A_.cdiag;
+        float* __restrict__ cdinv = A_.cdinv;
+        (void)vg;
+        uint32_t lo, hi;
+        block_range(vg, cg.n, lo, hi);
+        const float* FB[3] = {fbx, fby, fbz};
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+            int qx, qy, qz;
+            decode(cg, i, qx, qy, qz);
+            float c4[4] = {0.f, 0.f, 0.f, 0.f};
+            float diag = 0.f;
+            for (int z = 0; z < 2; z++)
+                for (int y = 0; y < 2; y++)
+                    for (int x = 0; x < 2; x++) {
+                        Pos child{{2 * qx + x, 2 * qy + y, 2 * qz + z}};
+                        if (child.p[0] >= fg.nx || child.p[1] >= fg.ny || child.p[2] >= fg.nz) continue;
+                        const uint32_t ci = cidx(fg, child);
+                        if (FINE) {
+                            if (fdinv[ci] == 0.0f) continue;
+                        } else {
+                            const float gamma = freact[ci];
+                            c4[3] += gamma;
+                            diag += gamma;
                         }
-                }
-        cbx[i] = c4[0];
-        cby[i] = c4[1];
-        cbz[i] = c4[2];
-        creact[i] = c4[3];
-        cdiag[i] = diag;
-        cdinv[i] = diag > 0.0f ? 1.0f / diag : 0.0f;
+                        for (int a = 0; a < 3; a++)
+                            for (int s = -1; s <= 1; s += 2) {
+                                Pos n = child;
+                                n.p[a] += s;
+                                if (!canon_valid(fg, n)) continue;
+                                const uint32_t ni = cidx(fg, n);
+                                const float beta = s > 0 ? FB[a][ni] : FB[a][ci];
+                                if (beta == 0.0f) continue;
+                                if (FINE && !meanfree && (mask[ni] & 3u) == 2u) {
+                                    c4[3] += beta;
+                                    diag += beta;
+                                    continue;
+                                }
+                                if (n.p[0] / 2 == qx && n.p[1] / 2 == qy && n.p[2] / 2 == qz) continue;
+                                diag += beta;
+                                if (s < 0) c4[a] += beta;
+                            }
+                    }
+            cbx[i] = c4[0];
+            cby[i] = c4[1];
+            cbz[i] = c4[2];
+            creact[i] = c4[3];
+            cdiag[i] = diag;
+            cdinv[i] = diag > 0.0f ? 1.0f / diag : 0.0f;
+        }
     }
-}
+};
 
 // ------------------------------------------------ row-vector forms (nx % 4 == 0)
 // One thread owns four consecutive cells of a row, so cell arrays move as 16-byte loads and
@@ -275,111 +372,103 @@ __device__ __forceinline__ float mob(const PhaseP& ph, float dl, float dc) {
 
 // Pass 1 of the coefficient build: the three lower-face mobilities of each cell (optional
 // BF16 copies for the mixed solver's V-cycle).
-__global__ void __launch_bounds__(MHDP_BLOCK) k_beta4(Grid3 g, PhaseP ph, const uint32_t* __restrict__ mask,
-                                                      const float* __restrict__ alpha, float* __restrict__ bx,
-                                                      float* __restrict__ by, float* __restrict__ bz,
-                                                      __hip_bfloat16* __restrict__ bx16,
-                                                      __hip_bfloat16* __restrict__ by16,
-                                                      __hip_bfloat16* __restrict__ bz16) {
-    uint32_t lo, hi;
-    block_range(g.n >> 2, lo, hi);
-    for (uint32_t t = lo + threadIdx.x; t < hi; t += MHDP_BLOCK) {
-        const Row4 r = row4(g, t);
-        uint32_t m[4], my[4], mz[4];
-        ld4u(mask, r.i0, m);
-        ld4u(mask, r.ym, my);
-        ld4u(mask, r.zm, mz);
-        const uint32_t mx = mask[r.xm];
-        float a[4] = {0.f, 0.f, 0.f, 0.f}, ay[4] = {0.f, 0.f, 0.f, 0.f}, az[4] = {0.f, 0.f, 0.f, 0.f}, ax = 0.f;
-        if (alpha) {
-            ld4f(alpha, r.i0, a);
-            ld4f(alpha, r.ym, ay);
-            ld4f(alpha, r.zm, az);
-            ax = alpha[r.xm];
-        }
-        float ox[4], oy[4], oz[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const bool fc = m[k] != 0u;
-            const bool lx = k > 0 ? m[k - 1] != 0u : (r.xm_ok && mx != 0u);
-            const float dlx = dens(ph, k > 0 ? a[k - 1] : ax);
-            const float dc = dens(ph, a[k]);
-            ox[k] = (fc && lx) ? mob(ph, dlx, dc) : 0.0f;
-            oy[k] = (fc && r.ym_ok && my[k] != 0u) ? mob(ph, dens(ph, ay[k]), dc) : 0.0f;
-            oz[k] = (fc && r.zm_ok && mz[k] != 0u) ? mob(ph, dens(ph, az[k]), dc) : 0.0f;
-        }
-        st4f(bx, r.i0, ox);
-        st4f(by, r.i0, oy);
-        st4f(bz, r.i0, oz);
-        if (bx16) {
-            st4h(bx16, r.i0, ox);
-            st4h(by16, r.i0, oy);
-            st4h(bz16, r.i0, oz);
+struct K_k_beta4 {
+    static constexpr int MINB = 0;
+    struct Args {
+        Grid3 g;
+        PhaseP ph;
+        const uint32_t* mask;
+        const float* alpha;
+        float* bx;
+        float* by;
+        float* bz;
+        __hip_bfloat16* bx16;
+        __hip_bfloat16* by16;
+        __hip_bfloat16* bz16;
+    };
+    static __device__ void run(const Args& A_, const VGrid& vg) {
+        Grid3 g = A_.g;
+        PhaseP ph = A_.ph;
+        const uint32_t* __restrict__ mask = A_.mask;
+        const float* __restrict__ alpha = A_.alpha;
+        float* __restrict__ bx = A_.bx;
+        float* __restrict__ by = A_.by;
+        float* __restrict__ bz = A_.bz;
+        __hip_bfloat16* __restrict__ bx16 = A_.bx16;
+        __hip_bfloat16* __restrict__ by16 = A_.by16;
+        __hip_bfloat16* __restrict__ bz16 = A_.bz16;
+        (void)vg;
+        uint32_t lo, hi;
+        block_range(vg, g.n >> 2, lo, hi);
+        for (uint32_t t = lo + threadIdx.x; t < hi; t += MHDP_BLOCK) {
+            const Row4 r = row4(g, t);
+            uint32_t m[4], my[4], mz[4];
+            ld4u(mask, r.i0, m);
+            ld4u(mask, r.ym, my);
+            ld4u(mask, r.zm, mz);
+            const uint32_t mx = mask[r.xm];
+            float a[4] = {0.f, 0.f, 0.f, 0.f}, ay[4] = {0.f, 0.f, 0.f, 0.f}, az[4] = {0.f, 0.f, 0.f, 0.f}, ax = 0.f;
+            if (alpha) {
+                ld4f(alpha, r.i0, a);
+                ld4f(alpha, r.ym, ay);
+                ld4f(alpha, r.zm, az);
+                ax = alpha[r.xm];
+            }
+            float ox[4], oy[4], oz[4];
+    #pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const bool fc = m[k] != 0u;
+                const bool lx = k > 0 ? m[k - 1] != 0u : (r.xm_ok && mx != 0u);
+                const float dlx = dens(ph, k > 0 ? a[k - 1] : ax);
+                const float dc = dens(ph, a[k]);
+                ox[k] = (fc && lx) ? mob(ph, dlx, dc) : 0.0f;
+                oy[k] = (fc && r.ym_ok && my[k] != 0u) ? mob(ph, dens(ph, ay[k]), dc) : 0.0f;
+                oz[k] = (fc && r.zm_ok && mz[k] != 0u) ? mob(ph, dens(ph, az[k]), dc) : 0.0f;
+            }
+            st4f(bx, r.i0, ox);
+            st4f(by, r.i0, oy);
+            st4f(bz, r.i0, oz);
+            if (bx16) {
+                st4h(bx16, r.i0, ox);
+                st4h(by16, r.i0, oy);
+                st4h(bz16, r.i0, oz);
+            }
         }
     }
-}
+};
 
 // Pass 2: diagonal() = sum of the six face mobilities in the order x-, x+, y-, y+, z-, z+
 // (a closed face contributes +0), the anchor row, and the active-cell reciprocal.
-__global__ void __launch_bounds__(MHDP_BLOCK) k_diag4(Grid3 g, PhaseP ph, const uint32_t* __restrict__ mask,
-                                                      const float* __restrict__ bx, const float* __restrict__ by,
-                                                      const float* __restrict__ bz, float* __restrict__ diag,
-                                                      float* __restrict__ dinv, __hip_bfloat16* __restrict__ dinv16) {
-    uint32_t lo, hi;
-    block_range(g.n >> 2, lo, hi);
-    for (uint32_t t = lo + threadIdx.x; t < hi; t += MHDP_BLOCK) {
-        const Row4 r = row4(g, t);
-        uint32_t m[4];
-        ld4u(mask, r.i0, m);
-        float xl[4], yl[4], zl[4], yu[4] = {0.f, 0.f, 0.f, 0.f}, zu[4] = {0.f, 0.f, 0.f, 0.f};
-        ld4f(bx, r.i0, xl);
-        ld4f(by, r.i0, yl);
-        ld4f(bz, r.i0, zl);
-        if (r.yp_ok) ld4f(by, r.yp, yu);
-        if (r.zp_ok) ld4f(bz, r.zp, zu);
-        const float xe = r.xp_ok ? bx[r.xp] : 0.0f;
-        float d[4], di[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            float v = 0.0f;
-            if (m[k] != 0u) {
-                v += xl[k];
-                v += k < 3 ? xl[k + 1] : xe;
-                v += yl[k];
-                v += yu[k];
-                v += zl[k];
-                v += zu[k];
-                if (!ph.meanfree && (m[k] & 3u) == 2u) v = 1.0f;
-                else v = fmaxf(v, 1e-30f);
-            }
-            d[k] = v;
-            const bool active = m[k] != 0u && (ph.meanfree || (m[k] & 3u) == 1u);
-            di[k] = active ? 1.0f / v : 0.0f;
-        }
-        st4f(diag, r.i0, d);
-        st4f(dinv, r.i0, di);
-        if (dinv16) st4h(dinv16, r.i0, di);
-    }
-}
-
-// rhs() (solver.wgsl:104-107) on four cells of a row: faces of the x plane are read as five
-// scalars (its rows hold nx+1 faces), the y and z planes as 16-byte rows.
-template <class T>
-__global__ void __launch_bounds__(MHDP_BLOCK) k_rhs4(Grid3 g, float h, const uint32_t* __restrict__ mask,
-                                                     const float* __restrict__ bx, const float* __restrict__ by,
-                                                     const float* __restrict__ bz, const float* __restrict__ ustar,
-                                                     uint64_t off1, uint64_t off2, T* __restrict__ b) {
-    uint32_t lo, hi;
-    block_range(g.n >> 2, lo, hi);
-    const float* __restrict__ U = ustar;
-    const float* __restrict__ V = ustar + off1;
-    const float* __restrict__ W = ustar + off2;
-    for (uint32_t t = lo + threadIdx.x; t < hi; t += MHDP_BLOCK) {
-        const Row4 r = row4(g, t);
-        uint32_t m[4];
-        ld4u(mask, r.i0, m);
-        T out[4] = {(T)0, (T)0, (T)0, (T)0};
-        if ((m[0] | m[1] | m[2] | m[3]) != 0u) {
+struct K_k_diag4 {
+    static constexpr int MINB = 0;
+    struct Args {
+        Grid3 g;
+        PhaseP ph;
+        const uint32_t* mask;
+        const float* bx;
+        const float* by;
+        const float* bz;
+        float* diag;
+        float* dinv;
+        __hip_bfloat16* dinv16;
+    };
+    static __device__ void run(const Args& A_, const VGrid& vg) {
+        Grid3 g = A_.g;
+        PhaseP ph = A_.ph;
+        const uint32_t* __restrict__ mask = A_.mask;
+        const float* __restrict__ bx = A_.bx;
+        const float* __restrict__ by = A_.by;
+        const float* __restrict__ bz = A_.bz;
+        float* __restrict__ diag = A_.diag;
+        float* __restrict__ dinv = A_.dinv;
+        __hip_bfloat16* __restrict__ dinv16 = A_.dinv16;
+        (void)vg;
+        uint32_t lo, hi;
+        block_range(vg, g.n >> 2, lo, hi);
+        for (uint32_t t = lo + threadIdx.x; t < hi; t += MHDP_BLOCK) {
+            const Row4 r = row4(g, t);
+            uint32_t m[4];
+            ld4u(mask, r.i0, m);
             float xl[4], yl[4], zl[4], yu[4] = {0.f, 0.f, 0.f, 0.f}, zu[4] = {0.f, 0.f, 0.f, 0.f};
             ld4f(bx, r.i0, xl);
             ld4f(by, r.i0, yl);
@@ -387,56 +476,129 @@ __global__ void __launch_bounds__(MHDP_BLOCK) k_rhs4(Grid3 g, float h, const uin
             if (r.yp_ok) ld4f(by, r.yp, yu);
             if (r.zp_ok) ld4f(bz, r.zp, zu);
             const float xe = r.xp_ok ? bx[r.xp] : 0.0f;
-            // x plane: faces x0..x0+4 of this row (the last wraps to face 0 when periodic)
-            const uint32_t ux = (uint32_t)(g.nx + 1) * ((uint32_t)r.y + (uint32_t)g.ny * (uint32_t)r.z);
-            float uf[5];
-#pragma unroll
-            for (int k = 0; k < 4; k++) uf[k] = U[ux + (uint32_t)r.x0 + (uint32_t)k];
-            const bool wrapx = r.x0 + 4 >= g.nx && (g.per & 1u);
-            uf[4] = U[ux + (wrapx ? 0u : (uint32_t)r.x0 + 4u)];
-            // y plane rows y and y+1 (y+1 wraps to 0 when periodic)
-            const uint32_t Py = (uint32_t)g.ny + 1u;
-            const uint32_t vy = (uint32_t)r.x0 + (uint32_t)g.nx * ((uint32_t)r.y + Py * (uint32_t)r.z);
-            const int yu_i = r.y + 1 < g.ny ? r.y + 1 : ((g.per & 2u) ? 0 : r.y + 1);
-            const uint32_t vyu = (uint32_t)r.x0 + (uint32_t)g.nx * ((uint32_t)yu_i + Py * (uint32_t)r.z);
-            float vl[4], vu[4];
-            ld4f(V, vy, vl);
-            ld4f(V, vyu, vu);
-            // z plane rows z and z+1
-            const uint32_t sz = (uint32_t)g.nx * (uint32_t)g.ny;
-            const int zu_i = r.z + 1 < g.nz ? r.z + 1 : ((g.per & 4u) ? 0 : r.z + 1);
-            const uint32_t wz = (uint32_t)r.x0 + (uint32_t)g.nx * (uint32_t)r.y;
-            float wl[4], wu[4];
-            ld4f(W, wz + sz * (uint32_t)r.z, wl);
-            ld4f(W, wz + sz * (uint32_t)zu_i, wu);
-#pragma unroll
+            float d[4], di[4];
+    #pragma unroll
             for (int k = 0; k < 4; k++) {
-                if (m[k] == 0u) continue;
-                const float bxu = k < 3 ? xl[k + 1] : xe;
-                T sum = (T)0;
-                {
-                    const float up = bxu > 0.0f ? uf[k + 1] : 0.0f;
-                    const float dn = xl[k] > 0.0f ? uf[k] : 0.0f;
-                    sum += (T)up - (T)dn;
+                float v = 0.0f;
+                if (m[k] != 0u) {
+                    v += xl[k];
+                    v += k < 3 ? xl[k + 1] : xe;
+                    v += yl[k];
+                    v += yu[k];
+                    v += zl[k];
+                    v += zu[k];
+                    if (!ph.meanfree && (m[k] & 3u) == 2u) v = 1.0f;
+                    else v = fmaxf(v, 1e-30f);
                 }
-                {
-                    const float up = yu[k] > 0.0f ? vu[k] : 0.0f;
-                    const float dn = yl[k] > 0.0f ? vl[k] : 0.0f;
-                    sum += (T)up - (T)dn;
-                }
-                {
-                    const float up = zu[k] > 0.0f ? wu[k] : 0.0f;
-                    const float dn = zl[k] > 0.0f ? wl[k] : 0.0f;
-                    sum += (T)up - (T)dn;
-                }
-                T div = sum / (T)h;
-                out[k] = -(T)h * (T)h * div;
+                d[k] = v;
+                const bool active = m[k] != 0u && (ph.meanfree || (m[k] & 3u) == 1u);
+                di[k] = active ? 1.0f / v : 0.0f;
             }
+            st4f(diag, r.i0, d);
+            st4f(dinv, r.i0, di);
+            if (dinv16) st4h(dinv16, r.i0, di);
         }
-#pragma unroll
-        for (int k = 0; k < 4; k++) b[r.i0 + k] = out[k];
     }
-}
+};
+
+// rhs() (solver.wgsl:104-107) on four cells of a row: faces of the x plane are read as five
+// scalars (its rows hold nx+1 faces), the y and z planes as 16-byte rows.
+template <class T>
+struct K_k_rhs4 {
+    static constexpr int MINB = 0;
+    struct Args {
+        Grid3 g;
+        float h;
+        const uint32_t* mask;
+        const float* bx;
+        const float* by;
+        const float* bz;
+        const float* ustar;
+        uint64_t off1;
+        uint64_t off2;
+        T* b;
+    };
+    static __device__ void run(const Args& A_, const VGrid& vg) {
+        Grid3 g = A_.g;
+        float h = A_.h;
+        const uint32_t* __restrict__ mask = A_.mask;
+        const float* __restrict__ bx = A_.bx;
+        const float* __restrict__ by = A_.by;
+        const float* __restrict__ bz = A_.bz;
+        const float* __restrict__ ustar = A_.ustar;
+        uint64_t off1 = A_.off1;
+        uint64_t off2 = A_.off2;
+        T* __restrict__ b = A_.b;
+        (void)vg;
+        uint32_t lo, hi;
+        block_range(vg, g.n >> 2, lo, hi);
+        const float* __restrict__ U = ustar;
+        const float* __restrict__ V = ustar + off1;
+        const float* __restrict__ W = ustar + off2;
+        for (uint32_t t = lo + threadIdx.x; t < hi; t += MHDP_BLOCK) {
+            const Row4 r = row4(g, t);
+            uint32_t m[4];
+            ld4u(mask, r.i0, m);
+            T out[4] = {(T)0, (T)0, (T)0, (T)0};
+            if ((m[0] | m[1] | m[2] | m[3]) != 0u) {
+                float xl[4], yl[4], zl[4], yu[4] = {0.f, 0.f, 0.f, 0.f}, zu[4] = {0.f, 0.f, 0.f, 0.f};
+                ld4f(bx, r.i0, xl);
+                ld4f(by, r.i0, yl);
+                ld4f(bz, r.i0, zl);
+                if (r.yp_ok) ld4f(by, r.yp, yu);
+                if (r.zp_ok) ld4f(bz, r.zp, zu);
+                const float xe = r.xp_ok ? bx[r.xp] : 0.0f;
+                // x plane: faces x0..x0+4 of this row (the last wraps to face 0 when periodic)
+                const uint32_t ux = (uint32_t)(g.nx + 1) * ((uint32_t)r.y + (uint32_t)g.ny * (uint32_t)r.z);
+                float uf[5];
+    #pragma unroll
+                for (int k = 0; k < 4; k++) uf[k] = U[ux + (uint32_t)r.x0 + (uint32_t)k];
+                const bool wrapx = r.x0 + 4 >= g.nx && (g.per & 1u);
+                uf[4] = U[ux + (wrapx ? 0u : (uint32_t)r.x0 + 4u)];
+                // y plane rows y and y+1 (y+1 wraps to 0 when periodic)
+                const uint32_t Py = (uint32_t)g.ny + 1u;
+                const uint32_t vy = (uint32_t)r.x0 + (uint32_t)g.nx * ((uint32_t)r.y + Py * (uint32_t)r.z);
+                const int yu_i = r.y + 1 < g.ny ? r.y + 1 : ((g.per & 2u) ? 0 : r.y + 1);
+                const uint32_t vyu = (uint32_t)r.x0 + (uint32_t)g.nx * ((uint32_t)yu_i + Py * (uint32_t)r.z);
+                float vl[4], vu[4];
+                ld4f(V, vy, vl);
+                ld4f(V, vyu, vu);
+                // z plane rows z and z+1
+                const uint32_t sz = (uint32_t)g.nx * (uint32_t)g.ny;
+                const int zu_i = r.z + 1 < g.nz ? r.z + 1 : ((g.per & 4u) ? 0 : r.z + 1);
+                const uint32_t wz = (uint32_t)r.x0 + (uint32_t)g.nx * (uint32_t)r.y;
+                float wl[4], wu[4];
+                ld4f(W, wz + sz * (uint32_t)r.z, wl);
+                ld4f(W, wz + sz * (uint32_t)zu_i, wu);
+    #pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    if (m[k] == 0u) continue;
+                    const float bxu = k < 3 ? xl[k + 1] : xe;
+                    T sum = (T)0;
+                    {
+                        const float up = bxu > 0.0f ? uf[k + 1] : 0.0f;
+                        const float dn = xl[k] > 0.0f ? uf[k] : 0.0f;
+                        sum += (T)up - (T)dn;
+                    }
+                    {
+                        const float up = yu[k] > 0.0f ? vu[k] : 0.0f;
+                        const float dn = yl[k] > 0.0f ? vl[k] : 0.0f;
+                        sum += (T)up - (T)dn;
+                    }
+                    {
+                        const float up = zu[k] > 0.0f ? wu[k] : 0.0f;
+                        const float dn = zl[k] > 0.0f ? wl[k] : 0.0f;
+                        sum += (T)up - (T)dn;
+                    }
+                    T div = sum / (T)h;
+                    out[k] = -(T)h * (T)h * div;
+                }
+            }
+    #pragma unroll
+            for (int k = 0; k < 4; k++) b[r.i0 + k] = out[k];
+        }
+    }
+};
 
 // correct() (solver.wgsl:245-250) on four cells of a row: each thread writes the lower faces
 // of its cells in all three planes, and the planes' last faces (x = nx, y = ny, z = nz) where
@@ -453,110 +615,155 @@ __device__ __forceinline__ void ld4p(const double* __restrict__ p, uint32_t i, d
     v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
 }
 template <class PT>
-__global__ void __launch_bounds__(MHDP_BLOCK) k_correct4(Grid3 g, float h, const float* __restrict__ bx,
-                                                         const float* __restrict__ by, const float* __restrict__ bz,
-                                                         const PT* __restrict__ phi, const float* __restrict__ ustar,
-                                                         float* __restrict__ uout, uint32_t f0, uint32_t f1) {
-    uint32_t lo, hi;
-    block_range(g.n >> 2, lo, hi);
-    const uint32_t sy = (uint32_t)g.nx, sz = (uint32_t)g.nx * (uint32_t)g.ny;
-    const float* __restrict__ U = ustar;
-    const float* __restrict__ V = ustar + f0;
-    const float* __restrict__ W = ustar + f0 + f1;
-    float* __restrict__ UO = uout;
-    float* __restrict__ VO = uout + f0;
-    float* __restrict__ WO = uout + f0 + f1;
-    for (uint32_t t = lo + threadIdx.x; t < hi; t += MHDP_BLOCK) {
-        const Row4 r = row4(g, t);
-        PT pc[4], py[4], pz[4];
-        ld4p(phi, r.i0, pc);
-        ld4p(phi, r.ym, py);
-        ld4p(phi, r.zm, pz);
-        const PT px = phi[r.xm];
-        float xl[4], yl[4], zl[4];
-        ld4f(bx, r.i0, xl);
-        ld4f(by, r.i0, yl);
-        ld4f(bz, r.i0, zl);
-        // x plane
-        const uint32_t ux = (uint32_t)(g.nx + 1) * ((uint32_t)r.y + (uint32_t)g.ny * (uint32_t)r.z) + (uint32_t)r.x0;
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            float o = 0.0f;
-            if (xl[k] > 0.0f) o = corrected<PT>(U[ux + k], xl[k], pc[k], k > 0 ? pc[k - 1] : px, h);
-            UO[ux + k] = o;
-        }
-        if (r.x0 + 4 == g.nx) {  // face x = nx of this row
-            float o = 0.0f;
-            if (g.per & 1u) {
-                const uint32_t c0 = r.i0 + 4u - (uint32_t)g.nx;  // cell x = 0 of the row
-                const float b0 = bx[c0];
-                if (b0 > 0.0f) o = corrected<PT>(U[ux + 4u - (uint32_t)g.nx], b0, phi[c0], pc[3], h);
+struct K_k_correct4 {
+    static constexpr int MINB = 0;
+    struct Args {
+        Grid3 g;
+        float h;
+        const float* bx;
+        const float* by;
+        const float* bz;
+        const PT* phi;
+        const float* ustar;
+        float* uout;
+        uint32_t f0;
+        uint32_t f1;
+    };
+    static __device__ void run(const Args& A_, const VGrid& vg) {
+        Grid3 g = A_.g;
+        float h = A_.h;
+        const float* __restrict__ bx = A_.bx;
+        const float* __restrict__ by = A_.by;
+        const float* __restrict__ bz = A_.bz;
+        const PT* __restrict__ phi = A_.phi;
+        const float* __restrict__ ustar = A_.ustar;
+        float* __restrict__ uout = A_.uout;
+        uint32_t f0 = A_.f0;
+        uint32_t f1 = A_.f1;
+        (void)vg;
+        uint32_t lo, hi;
+        block_range(vg, g.n >> 2, lo, hi);
+        const uint32_t sy = (uint32_t)g.nx, sz = (uint32_t)g.nx * (uint32_t)g.ny;
+        const float* __restrict__ U = ustar;
+        const float* __restrict__ V = ustar + f0;
+        const float* __restrict__ W = ustar + f0 + f1;
+        float* __restrict__ UO = uout;
+        float* __restrict__ VO = uout + f0;
+        float* __restrict__ WO = uout + f0 + f1;
+        for (uint32_t t = lo + threadIdx.x; t < hi; t += MHDP_BLOCK) {
+            const Row4 r = row4(g, t);
+            PT pc[4], py[4], pz[4];
+            ld4p(phi, r.i0, pc);
+            ld4p(phi, r.ym, py);
+            ld4p(phi, r.zm, pz);
+            const PT px = phi[r.xm];
+            float xl[4], yl[4], zl[4];
+            ld4f(bx, r.i0, xl);
+            ld4f(by, r.i0, yl);
+            ld4f(bz, r.i0, zl);
+            // x plane
+            const uint32_t ux = (uint32_t)(g.nx + 1) * ((uint32_t)r.y + (uint32_t)g.ny * (uint32_t)r.z) + (uint32_t)r.x0;
+    #pragma unroll
+            for (int k = 0; k < 4; k++) {
+                float o = 0.0f;
+                if (xl[k] > 0.0f) o = corrected<PT>(U[ux + k], xl[k], pc[k], k > 0 ? pc[k - 1] : px, h);
+                UO[ux + k] = o;
             }
-            UO[ux + 4] = o;
-        }
-        // y plane
-        const uint32_t Py = (uint32_t)g.ny + 1u;
-        const uint32_t vy = (uint32_t)r.x0 + sy * ((uint32_t)r.y + Py * (uint32_t)r.z);
-        {
-            float v[4], o[4];
-            ld4f(V, vy, v);
-#pragma unroll
-            for (int k = 0; k < 4; k++) o[k] = yl[k] > 0.0f ? corrected<PT>(v[k], yl[k], pc[k], py[k], h) : 0.0f;
-            st4f(VO, vy, o);
-        }
-        if (r.y + 1 == g.ny) {  // faces y = ny
-            float o[4] = {0.f, 0.f, 0.f, 0.f};
-            if (g.per & 2u) {
-                const uint32_t c0 = r.i0 - (uint32_t)(g.ny - 1) * sy;  // row y = 0
-                const uint32_t v0 = (uint32_t)r.x0 + sy * (Py * (uint32_t)r.z);
-                float b0[4], v[4];
-                ld4f(by, c0, b0);
-                ld4f(V, v0, v);
-                PT p0[4];
-                ld4p(phi, c0, p0);
-#pragma unroll
-                for (int k = 0; k < 4; k++) o[k] = b0[k] > 0.0f ? corrected<PT>(v[k], b0[k], p0[k], pc[k], h) : 0.0f;
+            if (r.x0 + 4 == g.nx) {  // face x = nx of this row
+                float o = 0.0f;
+                if (g.per & 1u) {
+                    const uint32_t c0 = r.i0 + 4u - (uint32_t)g.nx;  // cell x = 0 of the row
+                    const float b0 = bx[c0];
+                    if (b0 > 0.0f) o = corrected<PT>(U[ux + 4u - (uint32_t)g.nx], b0, phi[c0], pc[3], h);
+                }
+                UO[ux + 4] = o;
             }
-            st4f(VO, vy + sy, o);
-        }
-        // z plane
-        {
-            float w[4], o[4];
-            ld4f(W, r.i0, w);
-#pragma unroll
-            for (int k = 0; k < 4; k++) o[k] = zl[k] > 0.0f ? corrected<PT>(w[k], zl[k], pc[k], pz[k], h) : 0.0f;
-            st4f(WO, r.i0, o);
-        }
-        if (r.z + 1 == g.nz) {  // faces z = nz
-            float o[4] = {0.f, 0.f, 0.f, 0.f};
-            if (g.per & 4u) {
-                const uint32_t c0 = r.i0 - (uint32_t)(g.nz - 1) * sz;  // plane z = 0
-                float b0[4], w[4];
-                ld4f(bz, c0, b0);
-                ld4f(W, c0, w);
-                PT p0[4];
-                ld4p(phi, c0, p0);
-#pragma unroll
-                for (int k = 0; k < 4; k++) o[k] = b0[k] > 0.0f ? corrected<PT>(w[k], b0[k], p0[k], pc[k], h) : 0.0f;
+            // y plane
+            const uint32_t Py = (uint32_t)g.ny + 1u;
+            const uint32_t vy = (uint32_t)r.x0 + sy * ((uint32_t)r.y + Py * (uint32_t)r.z);
+            {
+                float v[4], o[4];
+                ld4f(V, vy, v);
+    #pragma unroll
+                for (int k = 0; k < 4; k++) o[k] = yl[k] > 0.0f ? corrected<PT>(v[k], yl[k], pc[k], py[k], h) : 0.0f;
+                st4f(VO, vy, o);
             }
-            st4f(WO, r.i0 + sz, o);
+            if (r.y + 1 == g.ny) {  // faces y = ny
+                float o[4] = {0.f, 0.f, 0.f, 0.f};
+                if (g.per & 2u) {
+                    const uint32_t c0 = r.i0 - (uint32_t)(g.ny - 1) * sy;  // row y = 0
+                    const uint32_t v0 = (uint32_t)r.x0 + sy * (Py * (uint32_t)r.z);
+                    float b0[4], v[4];
+                    ld4f(by, c0, b0);
+                    ld4f(V, v0, v);
+                    PT p0[4];
+                    ld4p(phi, c0, p0);
+    #pragma unroll
+                    for (int k = 0; k < 4; k++) o[k] = b0[k] > 0.0f ? corrected<PT>(v[k], b0[k], p0[k], pc[k], h) : 0.0f;
+                }
+                st4f(VO, vy + sy, o);
+            }
+            // z plane
+            {
+                float w[4], o[4];
+                ld4f(W, r.i0, w);
+    #pragma unroll
+                for (int k = 0; k < 4; k++) o[k] = zl[k] > 0.0f ? corrected<PT>(w[k], zl[k], pc[k], pz[k], h) : 0.0f;
+                st4f(WO, r.i0, o);
+            }
+            if (r.z + 1 == g.nz) {  // faces z = nz
+                float o[4] = {0.f, 0.f, 0.f, 0.f};
+                if (g.per & 4u) {
+                    const uint32_t c0 = r.i0 - (uint32_t)(g.nz - 1) * sz;  // plane z = 0
+                    float b0[4], w[4];
+                    ld4f(bz, c0, b0);
+                    ld4f(W, c0, w);
+                    PT p0[4];
+                    ld4p(phi, c0, p0);
+    #pragma unroll
+                    for (int k = 0; k < 4; k++) o[k] = b0[k] > 0.0f ? corrected<PT>(w[k], b0[k], p0[k], pc[k], h) : 0.0f;
+                }
+                st4f(WO, r.i0 + sz, o);
+            }
         }
     }
-}
+};
 
-__global__ void __launch_bounds__(MHDP_BLOCK) k_to_bf16(uint32_t n, const float* __restrict__ src,
-                                                        __hip_bfloat16* __restrict__ dst) {
-    uint32_t lo, hi;
-    block_range(n, lo, hi);
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) dst[i] = __float2bfloat16(src[i]);
-}
+struct K_k_to_bf16 {
+    static constexpr int MINB = 0;
+    struct Args {
+        uint32_t n;
+        const float* src;
+        __hip_bfloat16* dst;
+    };
+    static __device__ void run(const Args& A_, const VGrid& vg) {
+        uint32_t n = A_.n;
+        const float* __restrict__ src = A_.src;
+        __hip_bfloat16* __restrict__ dst = A_.dst;
+        (void)vg;
+        uint32_t lo, hi;
+        block_range(vg, n, lo, hi);
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) dst[i] = __float2bfloat16(src[i]);
+    }
+};
 
-__global__ void __launch_bounds__(MHDP_BLOCK) k_from_bf16(uint32_t n, const __hip_bfloat16* __restrict__ src,
-                                                          float* __restrict__ dst) {
-    uint32_t lo, hi;
-    block_range(n, lo, hi);
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) dst[i] = __bfloat162float(src[i]);
-}
+struct K_k_from_bf16 {
+    static constexpr int MINB = 0;
+    struct Args {
+        uint32_t n;
+        const __hip_bfloat16* src;
+        float* dst;
+    };
+    static __device__ void run(const Args& A_, const VGrid& vg) {
+        uint32_t n = A_.n;
+        const __hip_bfloat16* __restrict__ src = A_.src;
+        float* __restrict__ dst = A_.dst;
+        (void)vg;
+        uint32_t lo, hi;
+        block_range(vg, n, lo, hi);
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) dst[i] = __bfloat162float(src[i]);
+    }
+};
 
 // ---------------------------------------------------------------- launchers
 
@@ -569,34 +776,28 @@ bool launch_coefficients(hipStream_t s, const Grid3& g, const PhaseP& ph, const 
         const dim3 G(grid_blocks(g.n >> 2)), B(MHDP_BLOCK);
         const bool h = bx16 && by16 && bz16 && dinv16;
         using T = __hip_bfloat16;
-        hipLaunchKernelGGL(k_beta4, G, B, 0, s, g, ph, mask, alpha, bx, by, bz, h ? (T*)bx16 : (T*)nullptr,
-                           h ? (T*)by16 : (T*)nullptr, h ? (T*)bz16 : (T*)nullptr);
-        hipLaunchKernelGGL(k_diag4, G, B, 0, s, g, ph, mask, bx, by, bz, diag, dinv, h ? (T*)dinv16 : (T*)nullptr);
+        go<K_k_beta4>(s, G.x, {g, ph, mask, alpha, bx, by, bz, h ? (T*)bx16 : (T*)nullptr, h ? (T*)by16 : (T*)nullptr, h ? (T*)bz16 : (T*)nullptr});
+        go<K_k_diag4>(s, G.x, {g, ph, mask, bx, by, bz, diag, dinv, h ? (T*)dinv16 : (T*)nullptr});
         return h;
     }
-    hipLaunchKernelGGL(k_coefficients, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g, ph, mask, alpha, bx, by, bz,
-                       diag, dinv);
+    go<K_k_coefficients>(s, grid_blocks(g.n), {g, ph, mask, alpha, bx, by, bz, diag, dinv});
     return false;
 }
 void launch_rhs_f32(hipStream_t s, const Grid3& g, float h, const uint32_t* mask, const float* bx, const float* by,
                     const float* bz, const float* ustar, float* b) {
     uint64_t f0 = (uint64_t)(g.nx + 1) * g.ny * g.nz, f1 = (uint64_t)g.nx * (g.ny + 1) * g.nz;
     if (row_vector_ok(g))
-        hipLaunchKernelGGL(k_rhs4<float>, dim3(grid_blocks(g.n >> 2)), dim3(MHDP_BLOCK), 0, s, g, h, mask, bx, by, bz,
-                           ustar, f0, f0 + f1, b);
+        go<K_k_rhs4<float>>(s, grid_blocks(g.n >> 2), {g, h, mask, bx, by, bz, ustar, f0, f0 + f1, b});
     else
-        hipLaunchKernelGGL(k_rhs<float>, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g, h, mask, bx, by, bz, ustar,
-                           f0, f0 + f1, b);
+        go<K_k_rhs<float>>(s, grid_blocks(g.n), {g, h, mask, bx, by, bz, ustar, f0, f0 + f1, b});
 }
 void launch_rhs_f64(hipStream_t s, const Grid3& g, float h, const uint32_t* mask, const float* bx, const float* by,
                     const float* bz, const float* ustar, double* b) {
     uint64_t f0 = (uint64_t)(g.nx + 1) * g.ny * g.nz, f1 = (uint64_t)g.nx * (g.ny + 1) * g.nz;
     if (row_vector_ok(g))
-        hipLaunchKernelGGL(k_rhs4<double>, dim3(grid_blocks(g.n >> 2)), dim3(MHDP_BLOCK), 0, s, g, h, mask, bx, by, bz,
-                           ustar, f0, f0 + f1, b);
+        go<K_k_rhs4<double>>(s, grid_blocks(g.n >> 2), {g, h, mask, bx, by, bz, ustar, f0, f0 + f1, b});
     else
-        hipLaunchKernelGGL(k_rhs<double>, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, g, h, mask, bx, by, bz, ustar,
-                           f0, f0 + f1, b);
+        go<K_k_rhs<double>>(s, grid_blocks(g.n), {g, h, mask, bx, by, bz, ustar, f0, f0 + f1, b});
 }
 static void face_counts(const Grid3& g, uint32_t& f0, uint32_t& f1, uint32_t& f2) {
     f0 = (uint32_t)((g.nx + 1) * (uint64_t)g.ny * g.nz);
@@ -608,36 +809,30 @@ void launch_correct_f32(hipStream_t s, const Grid3& g, float h, const float* bx,
     uint32_t f0, f1, f2;
     face_counts(g, f0, f1, f2);
     if (row_vector_ok(g))
-        hipLaunchKernelGGL(k_correct4<float>, dim3(grid_blocks(g.n >> 2)), dim3(MHDP_BLOCK), 0, s, g, h, bx, by, bz, phi,
-                           ustar, uout, f0, f1);
+        go<K_k_correct4<float>>(s, grid_blocks(g.n >> 2), {g, h, bx, by, bz, phi, ustar, uout, f0, f1});
     else
-        hipLaunchKernelGGL(k_correct<float>, dim3(grid_blocks(f0 + f1 + f2)), dim3(MHDP_BLOCK), 0, s, g, h, bx, by, bz,
-                           phi, ustar, uout, f0, f1, f2);
+        go<K_k_correct<float>>(s, grid_blocks(f0 + f1 + f2), {g, h, bx, by, bz, phi, ustar, uout, f0, f1, f2});
 }
 void launch_correct_f64(hipStream_t s, const Grid3& g, float h, const float* bx, const float* by, const float* bz,
                         const double* phi, const float* ustar, float* uout) {
     uint32_t f0, f1, f2;
     face_counts(g, f0, f1, f2);
     if (row_vector_ok(g))
-        hipLaunchKernelGGL(k_correct4<double>, dim3(grid_blocks(g.n >> 2)), dim3(MHDP_BLOCK), 0, s, g, h, bx, by, bz, phi,
-                           ustar, uout, f0, f1);
+        go<K_k_correct4<double>>(s, grid_blocks(g.n >> 2), {g, h, bx, by, bz, phi, ustar, uout, f0, f1});
     else
-        hipLaunchKernelGGL(k_correct<double>, dim3(grid_blocks(f0 + f1 + f2)), dim3(MHDP_BLOCK), 0, s, g, h, bx, by, bz,
-                           phi, ustar, uout, f0, f1, f2);
+        go<K_k_correct<double>>(s, grid_blocks(f0 + f1 + f2), {g, h, bx, by, bz, phi, ustar, uout, f0, f1, f2});
 }
 void launch_mg_build(hipStream_t s, const Grid3& cg, const Grid3& fg, bool fine, int meanfree, const uint32_t* mask,
                      const float* fdinv, const float* fbx, const float* fby, const float* fbz, const float* freact,
                      float* cbx, float* cby, float* cbz, float* creact, float* cdiag, float* cdinv) {
     if (fine)
-        hipLaunchKernelGGL(k_mg_build<true>, dim3(grid_blocks(cg.n)), dim3(MHDP_BLOCK), 0, s, cg, fg, meanfree, mask,
-                           fdinv, fbx, fby, fbz, freact, cbx, cby, cbz, creact, cdiag, cdinv);
+        go<K_k_mg_build<true>>(s, grid_blocks(cg.n), {cg, fg, meanfree, mask, fdinv, fbx, fby, fbz, freact, cbx, cby, cbz, creact, cdiag, cdinv});
     else
-        hipLaunchKernelGGL(k_mg_build<false>, dim3(grid_blocks(cg.n)), dim3(MHDP_BLOCK), 0, s, cg, fg, meanfree, mask,
-                           fdinv, fbx, fby, fbz, freact, cbx, cby, cbz, creact, cdiag, cdinv);
+        go<K_k_mg_build<false>>(s, grid_blocks(cg.n), {cg, fg, meanfree, mask, fdinv, fbx, fby, fbz, freact, cbx, cby, cbz, creact, cdiag, cdinv});
 }
 void launch_to_bf16(hipStream_t s, uint32_t n, const float* src, void* dst) {
-    hipLaunchKernelGGL(k_to_bf16, dim3(grid_blocks(n)), dim3(MHDP_BLOCK), 0, s, n, src, (__hip_bfloat16*)dst);
+    go<K_k_to_bf16>(s, grid_blocks(n), {n, src, (__hip_bfloat16*)dst});
 }
 void launch_from_bf16(hipStream_t s, uint32_t n, const void* src, float* dst) {
-    hipLaunchKernelGGL(k_from_bf16, dim3(grid_blocks(n)), dim3(MHDP_BLOCK), 0, s, n, (const __hip_bfloat16*)src, dst);
+    go<K_k_from_bf16>(s, grid_blocks(n), {n, (const __hip_bfloat16*)src, dst});
 }

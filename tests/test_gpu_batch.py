@@ -1,6 +1,7 @@
-"""The batched entry point (mhdp_project_batch) over uneven droplet problems: every problem's
-potential and faces equal, bit for bit, those of its own single projection, and each meets
-the FP64 1e-8 bar against the oracle's operator."""
+"""The batched entry point (mhdp_project_batch) over uneven droplet problems (different grid
+sizes and hierarchy depths, brick and non-brick coarse levels): every problem's potential and
+faces equal, bit for bit, those of its own single projection, and each meets the FP64 1e-8 bar
+against the oracle's operator."""
 import numpy as np
 import pytest
 
@@ -19,15 +20,18 @@ def problems():
     return out
 
 
-def test_batch_matches_single_projections(orc):
+@pytest.mark.parametrize("mode", ["lockstep", "threads"])
+def test_batch_matches_single_projections(orc, mode, monkeypatch):
+    """lockstep: one launch per kernel for all problems; threads: per-context streams."""
     import torch
+    monkeypatch.setenv("MHDP_BATCH_MODE", mode)
     ps = problems()
     prs = [PressureProjection(p.grid, p.fluid, p.alpha) for p in ps]
     try:
         us = [torch.from_numpy(p.ustar).cuda() for p in ps]
         uo = [torch.zeros_like(u) for u in us]
         po = [torch.zeros(p.grid.cells, dtype=torch.float64, device="cuda") for p in ps]
-        print(f"[gpu] batch of {len(ps)}: sides {[p.grid.nx for p in ps]}", flush=True)
+        print(f"[gpu] batch ({mode}) of {len(ps)}: sides {[p.grid.nx for p in ps]}", flush=True)
         res = project_batch(prs, us, precision="mixed", precond="mg", u_outs=uo, phi_outs=po)
         torch.cuda.synchronize()
         for p, pr, u, phi, (it, r64) in zip(ps, prs, uo, po, res):
