@@ -92,6 +92,8 @@ def bench_batch(args):
     wall = (time.perf_counter() - t0) / args.steps
     worst = max(r for _, r in res)
     iters = sum(i for i, _ in res)
+    per_it = sorted(pr.stats["iterations"] for pr in prs)
+    per_outer = sorted(pr.stats["outer_steps"] for pr in prs)
     t0 = time.perf_counter()
     for pr, u, a, b in zip(prs, us, uo, po):
         pr.project(u, precision="mixed", precond="mg", u_out=a, phi_out=b)
@@ -105,6 +107,8 @@ def bench_batch(args):
         "config": {"workload": "64 droplet problems, sides 32/48/64/96, mixed to 1e-8 in one batch call",
                    "problems": len(ps), "cells": cells, "fluid_cells": fluid, "tolerance": 1e-8,
                    "iterations_total": iters, "worst_rel_residual_f64": worst,
+                   "iterations_min_median_max": [per_it[0], per_it[len(per_it) // 2], per_it[-1]],
+                   "outer_steps_min_max": [per_outer[0], per_outer[-1]],
                    "sequential_ms": seq * 1e3, "sequential_cells_per_s": fluid / seq},
         "roofline": None, "cpu_baseline": None,
     }

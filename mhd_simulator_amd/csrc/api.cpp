@@ -336,6 +336,7 @@ void vcycle_fused(mhdp_ctx* c, Solve& S, int mode, bool update = false) {
     c->timer.after(MHDP_K_RESTRICT, s);
     void* z1 = nullptr;
     launch_vcycle_coarse(s, S.store, L, lv, c->sc, S.coarse64, &z1);
+    record_mark();
     c->timer.before(MHDP_K_SMOOTH, s);
     if (brick && c->march_enabled && march_ok(c->g))
         launch_march_smooth(s, S.store, lv[0].g, lv[1].g, c->r, c->r2, dinv, bx, by, bz, z1, lv[0].t, c->part, c->sc,
@@ -367,6 +368,7 @@ void mg_initialize(mhdp_ctx* c, Solve& S) {  // multigrid.rs:287-291
 
 void iterate_once(mhdp_ctx* c, Solve& S) {  // mod.rs:1057-1075, multigrid.rs:302-326
     hipStream_t s = c->stream;
+    record_mark();
     if (S.fused) {
         c->timer.before(MHDP_K_APPLY, s);
         const bool brick = c->brick_enabled && brick_ok(c->g);
@@ -754,14 +756,22 @@ bool record_plan(const std::vector<int>& members, BatchMemory& mem, Plan& plan, 
     for (;;) {
         // the kernel waiting at the most sequence heads goes next (ties: lowest problem)
         std::vector<std::pair<const void*, size_t>> kinds;
+        bool any = false;
         for (size_t p = 0; p < recs.size(); p++) {
             if (head[p] >= recs[p].ops.size()) continue;
+            any = true;
             const void* k = recs[p].ops[head[p]].kind;
+            if (!k) continue;  // waiting at a mark
             auto it = std::find_if(kinds.begin(), kinds.end(), [&](const auto& e) { return e.first == k; });
             if (it == kinds.end()) kinds.push_back({k, 1});
             else it->second++;
         }
-        if (kinds.empty()) break;
+        if (!any) break;
+        if (kinds.empty()) {  // every unfinished sequence is at a mark: pass it together
+            for (size_t p = 0; p < recs.size(); p++)
+                if (head[p] < recs[p].ops.size()) head[p]++;
+            continue;
+        }
         const void* best = kinds[0].first;
         size_t bestn = kinds[0].second;
         for (const auto& e : kinds)
@@ -770,21 +780,24 @@ bool record_plan(const std::vector<int>& members, BatchMemory& mem, Plan& plan, 
                 bestn = e.second;
             }
         MergedOp m{};
-        std::vector<uint32_t> starts{0u};
+        std::vector<uint32_t> starts{0u}, counts;
         m.args_off = align(blob.size());
         for (size_t p = 0; p < recs.size(); p++) {
             if (head[p] >= recs[p].ops.size() || recs[p].ops[head[p]].kind != best) continue;
             const Recorded& op = recs[p].ops[head[p]];
             if (blob.size() < m.args_off) blob.resize(m.args_off);
             blob.insert(blob.end(), op.args.begin(), op.args.end());
-            starts.push_back(starts.back() + op.blocks);
+            // 8-aligned starts: each problem keeps its XCD pattern (see run_many)
+            starts.push_back(starts.back() + (op.blocks + MHDP_XCDS - 1) / MHDP_XCDS * MHDP_XCDS);
+            counts.push_back(op.blocks);
             m.issue = op.issue;
             head[p]++;
         }
-        m.count = (uint32_t)(starts.size() - 1);
+        m.count = (uint32_t)counts.size();
         m.blocks = starts.back();
         m.starts_off = align(blob.size());
         blob.resize(m.starts_off);
+        starts.insert(starts.end(), counts.begin(), counts.end());
         const unsigned char* sb = reinterpret_cast<const unsigned char*>(starts.data());
         blob.insert(blob.end(), sb, sb + starts.size() * sizeof(uint32_t));
         if (m.blocks) plan.ops.push_back(m);
@@ -910,96 +923,90 @@ int project_lockstep(mhdp_ctx* const* ctxs, int count, const float* const* ustar
         rcs[(size_t)p] = fail(ctxs[p], code, msg);
         ps[(size_t)p].running = false;
     };
-    for (bool first = true;; first = false) {
-        auto run = members([](const PS& q) { return q.running; });
-        if (run.empty()) break;
-        Plan start;
-        if (!record_plan(run, mem, start, [&](int p) {
-                mhdp_ctx* c = ctxs[p];
-                launch_outer_start(s, c->g, c->x64, c->b64, c->bx, c->by, c->bz, c->dinv, c->r, c->x, c->p, c->p2,
-                                   c->part, c->sc, first);
-                mg_initialize(c, ps[(size_t)p].S);
+    // Rounds of 8 iterations.  Each problem moves on to its next refinement step as soon as its
+    // inner solve has converged (no barrier between problems): a round first issues the step
+    // transitions of the problems that need one (fold the inner solution into phi64, new FP64
+    // residual and inner start, first V-cycle), then 8 iterations of every problem; a problem
+    // with nothing to do returns at once in those launches (its stop flag is set).
+    static const double kStop = 1.0;
+    auto stop = [&](int p) {  // a problem that leaves the rounds: its launches return at once
+        return hipMemcpyAsync(&ctxs[p]->sc->conv, &kStop, sizeof kStop, hipMemcpyHostToDevice, s) == hipSuccess;
+    };
+    Plan iter8;
+    {
+        auto all = members([](const PS& q) { return q.running; });
+        if (!all.empty() && !record_plan(all, mem, iter8, [&](int p) {
+                for (uint32_t k = 0; k < 8; k++) iterate_once(ctxs[p], ps[(size_t)p].S);
             }))
             return MHDP_NO_MEMORY;
-        issue_plan(s, start);
-        for (int p : run) {
-            ps[(size_t)p].it = 0;
-            ps[(size_t)p].checked = false;
-            ps[(size_t)p].inner = true;
-        }
-        // inner solves: batches of 8 iterations over the problems still iterating
-        std::vector<int> iterating;
-        Plan iter8;
-        for (;;) {
-            auto now = members([](const PS& q) { return q.running && q.inner; });
-            if (now.empty()) break;
-            if (now != iterating) {
-                iterating = now;
-                uint32_t most = 0;
-                for (int p : now) most = std::max(most, ps[(size_t)p].it);
-                const uint32_t batch = std::min(8u, o->max_iter - most);
-                if (!record_plan(now, mem, iter8, [&](int p) {
-                        for (uint32_t k = 0; k < batch; k++) iterate_once(ctxs[p], ps[(size_t)p].S);
-                    }))
-                    return MHDP_NO_MEMORY;
+    }
+    std::vector<int> trans = members([](const PS& q) { return q.running; });
+    for (int p : trans) ps[(size_t)p].inner = true;
+    for (bool first_round = true;; first_round = false) {
+        if (!trans.empty()) {
+            Plan tp;
+            if (!record_plan(trans, mem, tp, [&](int p) {
+                    mhdp_ctx* c = ctxs[p];
+                    PS& q = ps[(size_t)p];
+                    if (!first_round)
+                        launch_outer_update(s, c->g, c->x64, c->x, c->p, c->p2, c->dinv, c->sc, q.S.meanfree,
+                                            q.outer == 1);
+                    launch_outer_start(s, c->g, c->x64, c->b64, c->bx, c->by, c->bz, c->dinv, c->r, c->x, c->p,
+                                       c->p2, c->part, c->sc, q.outer == 0);
+                    mg_initialize(c, q.S);
+                }))
+                return MHDP_NO_MEMORY;
+            issue_plan(s, tp);
+            for (int p : trans) {
+                ps[(size_t)p].it = 0;
+                ps[(size_t)p].checked = false;
+                ps[(size_t)p].inner = true;
             }
-            issue_plan(s, iter8);
-            if (!read_all()) return MHDP_DEVICE;
-            for (int p : now) {
-                PS& q = ps[(size_t)p];
-                const Scal& v = sh[(size_t)p];
-                const double vals[10] = {v.rr64, v.bb64, v.rz, v.rr, v.bb, v.sr, v.alpha, v.beta, v.conv, v.iters};
-                bool bad = false;
-                for (double x : vals) bad |= !std::isfinite(x);
-                if (bad) {
-                    fail_p(p, MHDP_NONFINITE, "Nonfinite fluid pressure residual");
+        }
+        auto now = members([](const PS& q) { return q.running; });
+        if (now.empty()) break;
+        issue_plan(s, iter8);
+        if (!read_all()) return MHDP_DEVICE;
+        trans.clear();
+        for (int p : now) {
+            PS& q = ps[(size_t)p];
+            const Scal& v = sh[(size_t)p];
+            const double vals[10] = {v.rr64, v.bb64, v.rz, v.rr, v.bb, v.sr, v.alpha, v.beta, v.conv, v.iters};
+            bool bad = false;
+            for (double x : vals) bad |= !std::isfinite(x);
+            if (bad) {
+                fail_p(p, MHDP_NONFINITE, "Nonfinite fluid pressure residual");
+                if (!stop(p)) return MHDP_DEVICE;
+                continue;
+            }
+            if (!q.checked) {
+                q.res64 = v.bb64 > 0.0 ? std::sqrt(v.rr64 / v.bb64) : 0.0;
+                q.checked = true;
+                if (v.outer_done) {  // its stop flag is already set
+                    q.done = true;
+                    q.running = false;
                     continue;
                 }
-                if (!q.checked) {
-                    q.res64 = v.bb64 > 0.0 ? std::sqrt(v.rr64 / v.bb64) : 0.0;
-                    q.checked = true;
-                    if (v.outer_done) {
-                        q.done = true;
-                        q.running = false;
-                        q.inner = false;
-                        continue;
-                    }
-                    if (q.outer >= max_outer) {
-                        q.running = false;
-                        q.inner = false;
-                        continue;
-                    }
-                }
-                const bool conv = v.conv > 0.5;
-                const uint32_t seen = (uint32_t)v.iters;
-                if (seen <= q.it && !conv) {
-                    fail_p(p, MHDP_NO_PROGRESS, "Fluid pressure iteration made no progress");
+                if (q.outer >= max_outer) {
+                    q.running = false;
+                    if (!stop(p)) return MHDP_DEVICE;
                     continue;
                 }
-                q.it = seen;
-                if (conv || q.it >= o->max_iter) q.inner = false;
             }
-            // a problem stopped by max_iter must not iterate further: re-plan without it
-            auto left = members([](const PS& q) { return q.running && q.inner; });
-            bool changed = false;
-            for (int p : iterating)
-                if (std::find(left.begin(), left.end(), p) == left.end() && !(sh[(size_t)p].conv > 0.5)) changed = true;
-            if (changed) iterating.clear();
-        }
-        // fold the inner solutions into the FP64 potentials
-        auto fold = members([](const PS& q) { return q.running; });
-        if (fold.empty()) continue;
-        Plan upd;
-        if (!record_plan(fold, mem, upd, [&](int p) {
-                mhdp_ctx* c = ctxs[p];
-                launch_outer_update(s, c->g, c->x64, c->x, c->p, c->p2, c->dinv, c->sc, ps[(size_t)p].S.meanfree,
-                                    first);
-            }))
-            return MHDP_NO_MEMORY;
-        issue_plan(s, upd);
-        for (int p : fold) {
-            ps[(size_t)p].iterations += ps[(size_t)p].it;
-            ps[(size_t)p].outer++;
+            const bool conv = v.conv > 0.5;
+            const uint32_t seen = (uint32_t)v.iters;
+            if (seen <= q.it && !conv) {
+                fail_p(p, MHDP_NO_PROGRESS, "Fluid pressure iteration made no progress");
+                if (!stop(p)) return MHDP_DEVICE;
+                continue;
+            }
+            q.it = seen;
+            if (conv || q.it >= o->max_iter) {  // inner solve finished: next round moves it on
+                if (!conv && !stop(p)) return MHDP_DEVICE;
+                q.iterations += q.it;
+                q.outer++;
+                trans.push_back(p);
+            }
         }
     }
     // outputs of the problems that reached the target

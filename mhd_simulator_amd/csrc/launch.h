@@ -21,7 +21,10 @@ template <class K>
 __global__ void __launch_bounds__(MHDP_BLOCK, (K::MINB > 0 ? K::MINB : 1)) k_one(typename K::Args a) {
     K::run(a, vgrid());
 }
-// starts[0..count]: block offsets of the problems; a: their arguments.
+// starts[0..count]: first block of each problem (a multiple of 8) and the end; counts[p]: its
+// block count (the blocks between are idle padding); a: the problems' arguments.  Hardware hands
+// workgroups to the 8 XCDs round-robin, so with 8-aligned starts each problem sees the same
+// XCD pattern as alone and gets the same logical-block remap (contiguous ranges per XCD).
 template <class K>
 __device__ __forceinline__ void run_many(const typename K::Args* __restrict__ a, const uint32_t* __restrict__ starts,
                                          uint32_t count) {
@@ -32,8 +35,13 @@ __device__ __forceinline__ void run_many(const typename K::Args* __restrict__ a,
         if (starts[mid] <= b) lo = mid;
         else hi = mid;
     }
+    const uint32_t* counts = starts + count + 1;
+    const uint32_t j = b - starts[lo], n = counts[lo];
+    if (j >= n) return;
+    const uint32_t per = n / MHDP_XCDS;
+    const uint32_t lb = (per * MHDP_XCDS == n) ? (j % MHDP_XCDS) * per + j / MHDP_XCDS : j;
     const typename K::Args A = a[lo];
-    K::run(A, VGrid{b - starts[lo], starts[lo + 1] - starts[lo]});
+    K::run(A, VGrid{lb, n});
 }
 template <class K>
 __global__ void __launch_bounds__(MHDP_BLOCK) k_many0(const typename K::Args* a, const uint32_t* starts,
@@ -62,6 +70,12 @@ struct Recorder {
         return r;
     }
 };
+
+// Alignment point of recorded sequences: merging does not move any problem past a mark until
+// every problem has reached it (keeps problems of different hierarchy depth in step).
+inline void record_mark() {
+    if (Recorder* r = Recorder::active()) r->ops.push_back(Recorded{nullptr, nullptr, {}, 0u});
+}
 
 template <class K>
 void issue_many(hipStream_t s, const void* args_dev, const uint32_t* starts_dev, uint32_t count, uint32_t blocks) {

@@ -283,6 +283,10 @@ __device__ inline void finalize(Scal* s, int mode, const double t[4]) {
 // Every block stores its partial; the last block to finish folds all partials in a
 // fixed order (deterministic) and runs the scalar update `mode` — one launch per
 // reference reduction pair (e.g. pressure_apply + alpha_reduce).
+// The partials and the ticket are device-scope atomic accesses (they bypass the per-XCD L2),
+// and a block waits for its partial stores to complete before taking its ticket.  No
+// __threadfence(): on this device a device-scope fence writes back AND invalidates the XCD's
+// L2, which, once per block, throws away the halo data the other blocks are reading.
 template <int NV>
 __device__ __forceinline__ void reduce_finish(const VGrid& vg, double (&v)[NV], double* __restrict__ part, Scal* s,
                                               int ticket, int mode) {
@@ -291,25 +295,24 @@ __device__ __forceinline__ void reduce_finish(const VGrid& vg, double (&v)[NV], 
     const uint32_t lb = vg.b;
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) part[4 * lb + k] = k < NV ? v[k] : 0.0;
-        __threadfence();
-        unsigned t = atomicAdd(&s->ticket[ticket], 1u);
+        for (int k = 0; k < 4; k++)
+            __hip_atomic_store(part + 4 * lb + k, k < NV ? v[k] : 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_s_waitcnt(0);  // the partial stores have completed
+        const unsigned t = __hip_atomic_fetch_add(&s->ticket[ticket], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         last = (t == vg.n - 1);
     }
     __syncthreads();
     if (!last) return;
-    __threadfence();
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    const volatile double* vp = part;
     for (uint32_t j = threadIdx.x; j < vg.n; j += MHDP_BLOCK) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) acc[k] += vp[4 * j + k];
+        for (int k = 0; k < 4; k++)
+            acc[k] += __hip_atomic_load(part + 4 * j + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     block_sum<4>(acc);
     if (threadIdx.x == 0) {
         finalize(s, mode, acc);
         s->ticket[ticket] = 0u;
-        __threadfence();
     }
 }
 
