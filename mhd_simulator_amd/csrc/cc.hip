@@ -4,14 +4,17 @@
 // the other members, 0 for solid cells.
 //
 // Lock-free union-find (labels only ever point to smaller indices):
-//   init     lab[i] = i on fluid cells
-//   hook     every fluid cell unites with its fluid lower neighbour along each axis; a union
-//            links the larger of the two roots to the smaller with a compare-and-swap, so the
-//            root that survives in a component is its minimum index - the reference's BFS root
+//   init     lab[i] = first cell of i's run of fluid cells along x (the run is one set)
+//   hook     fluid cells unite with their fluid lower neighbours along y and z (only where a
+//            run starts) and across the periodic x seam; a union links the larger of the two
+//            roots to the smaller with a compare-and-swap, so the root that survives in a
+//            component is its minimum index - the reference's BFS root
 //   compress lab[i] = root(i) (read-only walks: every store is a cell's final root)
 //   encode   mask[i] from lab[i]; roots, fluid cells and the first anchor are counted.
 // Label reads go through device-coherent atomic loads (not the per-CU L1), and every loop has a
 // bound; a bound reached sets a flag and the host reports an error instead of a wrong mask.
+#include <algorithm>
+
 #include "mhdp_kernels.h"
 
 namespace {
@@ -55,13 +58,38 @@ __device__ __forceinline__ void unite(uint32_t* lab, uint32_t a, uint32_t b, uin
 
 }  // namespace
 
-__global__ void __launch_bounds__(MHDP_BLOCK) k_cc_init(uint32_t n, const uint8_t* __restrict__ fluid,
+// init: every fluid cell starts labelled with the first cell of its run of fluid cells along x,
+// so cells of a run are already united.  One wave per row, 64 cells at a time: the position of
+// the last solid cell so far is a running maximum (wave scan plus the carry of earlier chunks).
+__global__ void __launch_bounds__(MHDP_BLOCK) k_cc_rows(Grid3 g, const uint8_t* __restrict__ fluid,
                                                         uint32_t* __restrict__ lab) {
-    uint32_t lo, hi;
-    block_range(n, lo, hi);
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) lab[i] = fluid[i] ? i : kNone;
+    const uint32_t rows = (uint32_t)g.ny * (uint32_t)g.nz;
+    const uint32_t nx = (uint32_t)g.nx;
+    const int lane = threadIdx.x & 63;
+    const uint32_t waves = gridDim.x * (MHDP_BLOCK / 64);
+    for (uint32_t row = blockIdx.x * (MHDP_BLOCK / 64) + (threadIdx.x >> 6); row < rows; row += waves) {
+        const uint32_t base = row * nx;
+        int carry = -1;  // x of the last solid cell before this chunk (-1: none)
+        for (uint32_t x0 = 0; x0 < nx; x0 += 64) {
+            const uint32_t x = x0 + (uint32_t)lane;
+            const bool in = x < nx;
+            const bool f = in && fluid[base + x] != 0;
+            int last = (in && !f) ? (int)x : -1;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int v = __shfl_up(last, o, 64);
+                if (lane >= o) last = max(last, v);
+            }
+            last = max(last, carry);
+            if (in) lab[base + x] = f ? base + (uint32_t)(last + 1) : kNone;
+            carry = __shfl(last, 63, 64);
+        }
+    }
 }
 
+// hook: unions across y and z (and the periodic x wrap).  A y/z pair (i, j) whose x-predecessors
+// (i-1, j-1) are both fluid joins the same two runs as that pair, so only pairs at a run start
+// (of either run) unite; each pair of touching runs is still united at least once.
 __global__ void __launch_bounds__(MHDP_BLOCK) k_cc_hook(Grid3 g, const uint8_t* __restrict__ fluid, uint32_t* lab,
                                                         uint32_t* err) {
     uint32_t lo, hi;
@@ -71,17 +99,23 @@ __global__ void __launch_bounds__(MHDP_BLOCK) k_cc_hook(Grid3 g, const uint8_t* 
         if (!fluid[i]) continue;
         int x, y, z;
         decode(g, i, x, y, z);
-        uint32_t nb[3];
-        bool ok[3];
-        ok[0] = x > 0 || (g.per & 1u);
-        nb[0] = x > 0 ? i - 1 : i + (uint32_t)(g.nx - 1);
-        ok[1] = y > 0 || (g.per & 2u);
-        nb[1] = y > 0 ? i - sy : i + (uint32_t)(g.ny - 1) * sy;
-        ok[2] = z > 0 || (g.per & 4u);
-        nb[2] = z > 0 ? i - sz : i + (uint32_t)(g.nz - 1) * sz;
+        const bool first = x == 0 || !fluid[i - 1];
+        if (x == 0 && (g.per & 1u) && g.nx > 1) {
+            const uint32_t w = i + (uint32_t)(g.nx - 1);
+            if (fluid[w]) unite(lab, i, w, err);
+        }
+        uint32_t nb[2];
+        bool ok[2];
+        ok[0] = y > 0 || ((g.per & 2u) && g.ny > 1);
+        nb[0] = y > 0 ? i - sy : i + (uint32_t)(g.ny - 1) * sy;
+        ok[1] = z > 0 || ((g.per & 4u) && g.nz > 1);
+        nb[1] = z > 0 ? i - sz : i + (uint32_t)(g.nz - 1) * sz;
 #pragma unroll
-        for (int a = 0; a < 3; a++)
-            if (ok[a] && nb[a] != i && fluid[nb[a]]) unite(lab, i, nb[a], err);
+        for (int a = 0; a < 2; a++) {
+            if (!ok[a] || !fluid[nb[a]]) continue;
+            if (!first && fluid[nb[a] - 1]) continue;  // x > 0 here, so nb - 1 is in nb's row
+            unite(lab, i, nb[a], err);
+        }
     }
 }
 
@@ -138,7 +172,9 @@ __global__ void k_cc_counts_init(uint32_t* counts) {
 void launch_components(hipStream_t s, const Grid3& g, const uint8_t* fluid, uint32_t* mask, uint32_t* counts) {
     const dim3 B(MHDP_BLOCK), G(grid_blocks(g.n));
     hipLaunchKernelGGL(k_cc_counts_init, dim3(1), dim3(64), 0, s, counts);
-    hipLaunchKernelGGL(k_cc_init, G, B, 0, s, g.n, fluid, mask);
+    const uint32_t rows = (uint32_t)g.ny * (uint32_t)g.nz;
+    const uint32_t row_blocks = std::min<uint32_t>((rows + MHDP_BLOCK / 64 - 1) / (MHDP_BLOCK / 64), 4096u);
+    hipLaunchKernelGGL(k_cc_rows, dim3(row_blocks), B, 0, s, g, fluid, mask);
     hipLaunchKernelGGL(k_cc_hook, G, B, 0, s, g, fluid, mask, counts + 3);
     hipLaunchKernelGGL(k_cc_compress, G, B, 0, s, g.n, mask, counts + 3);
     hipLaunchKernelGGL(k_cc_encode, G, B, 0, s, g.n, mask, counts);
