@@ -79,6 +79,14 @@ typedef struct {
                             multigrid.wgsl:69) */
     float coarse_scale;  /* V-cycle coarse-correction scale; 0 = tuned default 1.6 (reference: 1,
                             multigrid.wgsl:87) */
+    int warm_start;      /* project(warm_start) (mod.rs:1015): start from the context's previous
+                            potential (the last successful projection's, or mhdp_set_potential's),
+                            relative to each component's anchor.  MHDP_FP32 as the reference: u*
+                            is first corrected by it (pressure_warm, solver.wgsl:170-176), the solve
+                            finds the remaining correction and the potential returned is their sum
+                            (pressure_commit, solver.wgsl:181-183).  MHDP_MIXED: the FP64 refinement
+                            starts from it (the same sum, without rounding the corrected u* to
+                            FP32 faces).  Ignored until a previous potential exists. */
 } mhdp_opts;
 
 typedef struct {
@@ -111,6 +119,10 @@ int mhdp_project(mhdp_ctx *ctx, const float *u_star, const mhdp_opts *opts, floa
 /* Same with device pointers already resident in HBM (no host transfer). */
 int mhdp_project_dev(mhdp_ctx *ctx, const float *u_star_dev, const mhdp_opts *opts, float *u_out_dev,
                      double *phi_out_dev, mhdp_stats *stats);
+/* The previous potential a warm start begins from (FluidState keeps it in its output fields,
+ * mod.rs:97-105): one double per cell, host (on_device = 0) or device memory.  Every successful
+ * projection replaces it with its own result. */
+int mhdp_set_potential(mhdp_ctx *ctx, const double *phi, int on_device);
 /* Many independent projections (uneven grids), device pointers.  No counterpart in the
  * reference (one FluidState per grid, mod.rs:1015): each context keeps its own stream and
  * a pool of host threads (MHDP_BATCH_THREADS, default 8) drives them concurrently, largest

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libmhdp.so")
 # exported entry points, in include/mhdp.h order
 SYMBOLS = (
     "mhdp_create", "mhdp_set_stream", "mhdp_set_cells", "mhdp_get_mask", "mhdp_project", "mhdp_project_dev",
+    "mhdp_set_potential",
     "mhdp_project_batch", "mhdp_last_error", "mhdp_destroy", "mhdp_device_bytes",
     "mhdp_anchors", "mhdp_hierarchy", "mhdp_set_kernel_timing", "mhdp_kernel_time",
     "mhdp_probe_coefficients", "mhdp_probe_apply", "mhdp_probe_vcycle", "mhdp_probe_mg_graph", "mhdp_probe_rhs",
@@ -39,7 +40,7 @@ class Opts(C.Structure):
     _fields_ = [("rel_tol", C.c_float), ("abs_div_tol", C.c_float), ("max_iter", C.c_uint32),
                 ("precond", C.c_int), ("precision", C.c_int), ("gauge", C.c_int), ("refine_tol", C.c_double),
                 ("max_outer", C.c_uint32), ("inner_tol", C.c_float), ("smooth_weight", C.c_float),
-                ("coarse_scale", C.c_float)]
+                ("coarse_scale", C.c_float), ("warm_start", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -75,6 +76,7 @@ def load(path: str = LIB_PATH):
         "mhdp_create": (I, [C.POINTER(P), C.POINTER(Grid), C.POINTER(Phase)]),
         "mhdp_set_stream": (I, [P, P]),
         "mhdp_set_cells": (I, [P, P, P]),
+        "mhdp_set_potential": (I, [P, P, I]),
         "mhdp_get_mask": (I, [P, P]),
         "mhdp_project": (I, [P, P, C.POINTER(Opts), P, P, C.POINTER(Stats)]),
         "mhdp_project_dev": (I, [P, P, C.POINTER(Opts), P, P, C.POINTER(Stats)]),

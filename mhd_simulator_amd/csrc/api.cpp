@@ -56,6 +56,9 @@ struct mhdp_ctx {
     float* r2 = nullptr;  // twin of r for the brick path's fused residual update
     void *z = nullptr, *t = nullptr;
     double *x64 = nullptr, *b64 = nullptr;
+    double* phi_prev = nullptr;  // potential of the last successful projection (warm start)
+    bool prev_valid = false;
+    float* uwarm = nullptr;      // u* corrected by the previous potential (warm start)
     void *bx16 = nullptr, *by16 = nullptr, *bz16 = nullptr, *dinv16 = nullptr;
     // multigrid
     std::vector<CoarseArrays> coarse;  // index l-1 for level l
@@ -326,7 +329,11 @@ void vcycle_fused(mhdp_ctx* c, Solve& S, int mode, bool update = false) {
     const int L = (int)S.lv.size();
     const bool brick = c->brick_enabled && brick_ok(c->g);
     c->timer.before(MHDP_K_RESTRICT, s);
-    if (brick && update)
+    const bool march = brick && c->march_enabled && march_ok(c->g);
+    if (march && update)
+        launch_march_restrict_update(s, S.store, lv[1].g, lv[0].g, c->r, c->r2, c->q, dinv, bx, by, bz, lv[1].r,
+                                     c->part, c->sc);
+    else if (brick && update)
         launch_brick_restrict_update(s, S.store, lv[1].g, lv[0].g, c->r, c->r2, c->q, dinv, bx, by, bz, lv[1].r,
                                      c->part, c->sc);
     else if (brick)
@@ -338,7 +345,7 @@ void vcycle_fused(mhdp_ctx* c, Solve& S, int mode, bool update = false) {
     launch_vcycle_coarse(s, S.store, L, lv, c->sc, S.coarse64, &z1);
     record_mark();
     c->timer.before(MHDP_K_SMOOTH, s);
-    if (brick && c->march_enabled && march_ok(c->g))
+    if (march)
         launch_march_smooth(s, S.store, lv[0].g, lv[1].g, c->r, c->r2, dinv, bx, by, bz, z1, lv[0].t, c->part, c->sc,
                             mode, update);
     else if (brick)
@@ -484,14 +491,14 @@ int upload_scal(mhdp_ctx* c, const Scal& s) {
 // projection, zeroed x and search directions), the inner V-cycle PCG in batches of 8, and one
 // pass folding the inner solution (with its owed last update) into x64.  The host reads the
 // scalars once per batch; the first read of a step carries the FP64 residual.
-int refine_fused(mhdp_ctx* c, Solve& S, const mhdp_opts* o, uint32_t max_outer, uint32_t* iterations,
+int refine_fused(mhdp_ctx* c, Solve& S, const mhdp_opts* o, uint32_t max_outer, bool warm, uint32_t* iterations,
                  uint32_t* outer, double* res64) {
     hipStream_t s = c->stream;
     *iterations = 0;
     *outer = 0;
     for (bool first = true;; first = false) {
         launch_outer_start(s, c->g, c->x64, c->b64, c->bx, c->by, c->bz, c->dinv, c->r, c->x, c->p, c->p2, c->part,
-                           c->sc, first);
+                           c->sc, first && !warm);
         mg_initialize(c, S);
         uint32_t it = 0;
         for (bool checked = false;;) {
@@ -506,7 +513,7 @@ int refine_fused(mhdp_ctx* c, Solve& S, const mhdp_opts* o, uint32_t max_outer, 
             if (!checked) {
                 *res64 = v.bb64 > 0.0 ? std::sqrt(v.rr64 / v.bb64) : 0.0;
                 if (v.outer_done) {
-                    if (first) launch_fill_f64(s, c->g.n, c->x64, 0.0);
+                    if (first && !warm) launch_fill_f64(s, c->g.n, c->x64, 0.0);
                     return MHDP_OK;
                 }
                 if (*outer >= max_outer) return MHDP_NOT_CONVERGED;
@@ -519,7 +526,7 @@ int refine_fused(mhdp_ctx* c, Solve& S, const mhdp_opts* o, uint32_t max_outer, 
             if (conv || it >= o->max_iter) break;
         }
         *iterations += it;
-        launch_outer_update(s, c->g, c->x64, c->x, c->p, c->p2, c->dinv, c->sc, S.meanfree, first);
+        launch_outer_update(s, c->g, c->x64, c->x, c->p, c->p2, c->dinv, c->sc, S.meanfree, first && !warm);
         (*outer)++;
     }
 }
@@ -570,10 +577,11 @@ int project_impl(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, float* uou
     const bool meanfree = o->gauge != MHDP_GAUGE_PINNED && c->components == 1;
     const bool mixed = o->precision == MHDP_MIXED;
     const bool preferred = o->precond == MHDP_PRECOND_MG || (o->precond == MHDP_PRECOND_AUTO && c->g.n >= kMgMinCells);
-    if (mixed) {
-        if ((rc = dalloc(c, (void**)&c->x64, (size_t)c->g.n * 8)) || (rc = dalloc(c, (void**)&c->b64, (size_t)c->g.n * 8)))
-            return rc;
-    }
+    if ((rc = dalloc(c, (void**)&c->x64, (size_t)c->g.n * 8)) || (rc = dalloc(c, (void**)&c->phi_prev, (size_t)c->g.n * 8)))
+        return rc;
+    if (mixed && (rc = dalloc(c, (void**)&c->b64, (size_t)c->g.n * 8))) return rc;
+    const bool warm = o->warm_start && c->prev_valid;
+    if (warm && !mixed && (rc = dalloc(c, (void**)&c->uwarm, face_count(c->grid) * sizeof(float)))) return rc;
     if (preferred && (rc = alloc_mg(c, mixed))) return rc;
 
     const double omega = o->smooth_weight > 0.0f ? (double)o->smooth_weight : kTunedOmega;
@@ -587,6 +595,10 @@ int project_impl(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, float* uou
     c->timer.used = 0;
     HIP_OK(hipEventRecord(c->ev0, c->stream));
     if ((rc = prepare(c, meanfree, mixed && preferred))) return rc;
+    if (warm && !mixed) {  // pressure_warm: the solve then finds the correction to the previous potential
+        launch_warm(c->stream, c->g, c->grid.h, c->mask, c->phi_prev, c->bx, c->by, c->bz, ustar, c->uwarm);
+        ustar = c->uwarm;
+    }
     uint32_t iterations = 0, outer = 0;
     float residual = 0.0f;
     double res64 = 0.0;
@@ -611,17 +623,20 @@ int project_impl(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, float* uou
         }
         if (rc) return rc;
         if (uout) launch_correct_f32(c->stream, c->g, c->grid.h, c->bx, c->by, c->bz, c->x, ustar, uout);
-        if (phi) launch_cast_f32_f64(c->stream, c->g.n, c->x, phi);
+        launch_cast_f32_f64(c->stream, c->g.n, c->x, c->x64);
     } else {
         // FP64 iterative refinement: residual in FP64, inner FP32 PCG with the BF16 V-cycle.
         const double tau = o->inner_tol > 0.0f ? (double)o->inner_tol : 1e-4;
         const uint32_t max_outer = o->max_outer ? o->max_outer : 16u;
-        launch_rhs_f64(c->stream, c->g, c->grid.h, c->mask, c->bx, c->by, c->bz, ustar, c->b64);
         Scal s;
         set_solve_params(c, s, meanfree, tau * tau, 0.0, 1.0, omega, pscale);
         s.refine_tol = o->refine_tol;
         s.tau = tau;
         if ((rc = upload_scal(c, s))) return rc;
+        launch_rhs_f64(c->stream, c->g, c->grid.h, c->mask, c->bx, c->by, c->bz, ustar, c->b64);
+        // a warm start is the refinement begun from the previous potential (the FP64 residual
+        // then sees exactly b - L phi_old; faces are corrected from u* with the full potential)
+        if (warm) launch_old_potential(c->stream, c->g.n, c->mask, c->phi_prev, c->x64);
         bool done = false;
         const bool fast = preferred && c->fused_enabled && c->levels.size() >= 2;
         if (fast) {
@@ -629,10 +644,10 @@ int project_impl(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, float* uou
             S.mg = true;
             S.lv = level_view(c);
             S.fused = true;
-            rc = refine_fused(c, S, o, max_outer, &iterations, &outer, &res64);
+            rc = refine_fused(c, S, o, max_outer, warm, &iterations, &outer, &res64);
             if (rc && rc != MHDP_NOT_CONVERGED) return rc;
             done = rc == MHDP_OK;
-        } else {
+        } else if (!warm) {
             launch_fill_f64(c->stream, c->g.n, c->x64, 0.0);
         }
         for (; !fast;) {
@@ -669,8 +684,12 @@ int project_impl(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, float* uou
         }
         residual = (float)res64;
         if (uout) launch_correct_f64(c->stream, c->g, c->grid.h, c->bx, c->by, c->bz, c->x64, ustar, uout);
-        if (phi) HIP_OK(hipMemcpyAsync(phi, c->x64, (size_t)c->g.n * 8, hipMemcpyDeviceToDevice, c->stream));
     }
+    // the potential (plus the previous one after an FP32 warm start) is returned and kept
+    if (warm && !mixed) launch_commit(c->stream, c->g.n, c->mask, c->phi_prev, c->x64);
+    if (phi) HIP_OK(hipMemcpyAsync(phi, c->x64, (size_t)c->g.n * 8, hipMemcpyDeviceToDevice, c->stream));
+    std::swap(c->x64, c->phi_prev);
+    c->prev_valid = true;
     HIP_OK(hipEventRecord(c->ev1, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
     HIP_OK(hipGetLastError());
@@ -833,7 +852,8 @@ struct KGatherScal {
 // Eligible for the lockstep driver: the fused multigrid path of the mixed solver.
 bool lockstep_ok(const mhdp_ctx* c, const mhdp_opts* o) {
     const bool preferred = o->precond == MHDP_PRECOND_MG || (o->precond == MHDP_PRECOND_AUTO && c->g.n >= kMgMinCells);
-    return c->cells && o->precision == MHDP_MIXED && preferred && c->fused_enabled && hierarchy(c->g).size() >= 2;
+    return c->cells && o->precision == MHDP_MIXED && preferred && !o->warm_start && c->fused_enabled &&
+           hierarchy(c->g).size() >= 2;
 }
 
 int project_lockstep(mhdp_ctx* const* ctxs, int count, const float* const* ustar, const mhdp_opts* o,
@@ -862,7 +882,8 @@ int project_lockstep(mhdp_ctx* const* ctxs, int count, const float* const* ustar
         int rc = validate_opts(c, o);
         if (!rc && !ustar[p]) rc = fail(c, MHDP_INVALID, "Missing candidate face velocities");
         if (!rc && ((rc = dalloc(c, (void**)&c->x64, (size_t)c->g.n * 8)) ||
-                    (rc = dalloc(c, (void**)&c->b64, (size_t)c->g.n * 8)) || (rc = alloc_mg(c, true)))) {
+                    (rc = dalloc(c, (void**)&c->b64, (size_t)c->g.n * 8)) ||
+                    (rc = dalloc(c, (void**)&c->phi_prev, (size_t)c->g.n * 8)) || (rc = alloc_mg(c, true)))) {
         }
         PS& q = ps[(size_t)p];
         q.S.meanfree = o->gauge != MHDP_GAUGE_PINNED && c->components == 1;
@@ -1043,6 +1064,8 @@ int project_lockstep(mhdp_ctx* const* ctxs, int count, const float* const* ustar
             rcs[(size_t)p] = fail(c, MHDP_NOT_CONVERGED, buf);
             continue;
         }
+        std::swap(c->x64, c->phi_prev);  // kept for a later warm start
+        c->prev_valid = true;
         if (st) {
             mhdp_stats& t = st[p];
             t.iterations = q.iterations;
@@ -1198,6 +1221,18 @@ int mhdp_get_mask(const mhdp_ctx* c, uint32_t* encoded) {
     return MHDP_OK;
 }
 
+int mhdp_set_potential(mhdp_ctx* c, const double* phi, int on_device) {
+    if (!c || !phi) return MHDP_INVALID;
+    const size_t bytes = (size_t)c->g.n * 8;
+    int rc = dalloc(c, (void**)&c->phi_prev, bytes);
+    if (rc) return rc;
+    HIP_OK(hipMemcpyAsync(c->phi_prev, phi, bytes, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                          c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    c->prev_valid = true;
+    return MHDP_OK;
+}
+
 int mhdp_project_dev(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, float* uout, double* phi, mhdp_stats* st) {
     if (!c) return MHDP_INVALID;
     c->err.clear();
@@ -1333,7 +1368,7 @@ void mhdp_destroy(mhdp_ctx* c) {
     else (void)hipDeviceSynchronize();
     void* ptrs[] = {c->mask, c->alpha, c->bx, c->by, c->bz, c->diag, c->dinv, c->b, c->x, c->r, c->r2, c->p, c->p2, c->q, c->z,
                     c->t, c->x64, c->b64, c->bx16, c->by16, c->bz16, c->dinv16, c->sc, c->part, c->ustar_stage,
-                    c->uout_stage, c->phi_stage, c->fluid8, c->cc_counts};
+                    c->uout_stage, c->phi_stage, c->fluid8, c->cc_counts, c->phi_prev, c->uwarm};
     for (void* p : ptrs) dfree(c, p);
     for (CoarseArrays& a : c->coarse) {
         void* q[] = {a.bx, a.by, a.bz, a.react, a.diag, a.dinv, a.bx16, a.by16, a.bz16, a.react16, a.dinv16,

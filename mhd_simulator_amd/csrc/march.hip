@@ -334,6 +334,159 @@ struct K_k_march_smooth {
     }
 };
 
+// Passes 2+3 (restriction with the residual update) on z-marching columns: r' = (r - m) - alpha q
+// on the column's planes and their halos, r' stored for the own cells in the twin buffer, the
+// pre-smoothed z1 = omega (r' - m') / d, the residual (r' - m') - A z1 summed over each 2x2x2
+// aggregate (two per column and plane pair), and sum r', sum r'^2 -> FIN_RESTRICT_UPD.
+// Arithmetic per cell as K_k_brick_restrict_update.
+template <class CT, class VT, int MB_>
+struct K_march_restrict_update {
+    static constexpr int MINB = MB_;
+    struct Args {
+        Grid3 cg, g;
+        float* r0;
+        float* r1;
+        const float* q;
+        const CT* dinv;
+        const CT* bx;
+        const CT* by;
+        const CT* bz;
+        VT* rc;
+        double* part;
+        Scal* sc;
+        uint32_t zlen;
+    };
+    struct Raw {
+        PR<float> r, q;
+        PR<CT> d;
+        PW<CT> w;
+    };
+    // r' and z1 of one plane (own cells and in-plane halo); rown: r' of the own cells
+    struct Fields {
+        PF z1;
+        float rown[2][4];
+    };
+    static __device__ void run(const Args& A, const VGrid& vg) {
+        const Grid3 g = A.g;
+        Scal* sc = A.sc;
+        if (CONVERGED(sc)) return;
+        const float omega = (float)sc->omega;
+        const float alpha = (float)sc->alpha;
+        const float m = (float)sc->mean;
+        const float mn = (float)sc->mean_next;
+        const float* __restrict__ rin = sc->rsel ? A.r1 : A.r0;
+        float* __restrict__ rout = sc->rsel ? A.r0 : A.r1;
+        const float* __restrict__ q = A.q;
+        const CT* __restrict__ dinv = A.dinv;
+        const uint32_t sy = (uint32_t)g.nx, sz = (uint32_t)g.nx * (uint32_t)g.ny;
+        const uint32_t csy = (uint32_t)A.cg.nx, csz = (uint32_t)A.cg.nx * (uint32_t)A.cg.ny;
+        const uint32_t zlen = A.zlen;
+        const uint32_t items = ((uint32_t)g.nx >> 2) * ((uint32_t)g.ny >> 1) * (((uint32_t)g.nz + zlen - 1) / zlen);
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (uint32_t it = vg.b * MHDP_BLOCK + threadIdx.x; it < items; it += vg.n * MHDP_BLOCK) {
+            const Item w0 = item_of(g, it, zlen);
+            const Col o = column(g, w0.cx, w0.cy);
+            auto load = [&](int p, Raw& x) {
+                const uint32_t base = sz * (uint32_t)plane_of(g, p);
+                load_pr(o, base, sy, rin, x.r);
+                load_pr(o, base, sy, q, x.q);
+                load_pr(o, base, sy, dinv, x.d);
+                load_pw(o, base, sy, A.bx, A.by, A.bz, x.w);
+            };
+            auto settle = [&](Raw& x) {
+                fence_pr(x.r);
+                fence_pr(x.q);
+                fence_pr(x.d);
+                fence_pw(x.w);
+            };
+            auto upd = [&](float r, float qq, float d) { return d != 0.0f ? (r - m) - alpha * qq : 0.0f; };
+            auto field = [&](const Raw& x, Fields& f) {
+#pragma unroll
+                for (int dy = 0; dy < 2; dy++) {
+                    const F4 a = un(x.r.c[dy]), b = un(x.q.c[dy]), d = un(x.d.c[dy]);
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const float rn = upd(a.v[k], b.v[k], d.v[k]);
+                        f.rown[dy][k] = rn;
+                        f.z1.c[dy][k] = omega * (rn - mn) * d.v[k];
+                    }
+                    const float dm = un(x.d.xm[dy]), dp = un(x.d.xp[dy]);
+                    f.z1.xm[dy] = omega * (upd(un(x.r.xm[dy]), un(x.q.xm[dy]), dm) - mn) * dm;
+                    f.z1.xp[dy] = omega * (upd(un(x.r.xp[dy]), un(x.q.xp[dy]), dp) - mn) * dp;
+                }
+                const F4 am = un(x.r.ym), bm = un(x.q.ym), dm = un(x.d.ym);
+                const F4 ap = un(x.r.yp), bp = un(x.q.yp), dp = un(x.d.yp);
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    f.z1.ym[k] = omega * (upd(am.v[k], bm.v[k], dm.v[k]) - mn) * dm.v[k];
+                    f.z1.yp[k] = omega * (upd(ap.v[k], bp.v[k], dp.v[k]) - mn) * dp.v[k];
+                }
+            };
+            Raw cur, nxt;
+            Fields fc, fn;
+            float below[2][4];
+            {
+                Raw prev;
+                load(w0.za - 1, prev);
+                load(w0.za, cur);
+                load(w0.za + 1, nxt);
+                settle(prev);
+                settle(cur);
+                Fields fp;
+                field(prev, fp);
+#pragma unroll
+                for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+                    for (int k = 0; k < 4; k++) below[dy][k] = fp.z1.c[dy][k];
+                field(cur, fc);
+            }
+            float s0 = 0.0f, s1 = 0.0f;
+            for (int k = w0.za; k < w0.zb; k++) {
+                Raw pre;
+                load(k + 2, pre);
+                settle(nxt);
+                field(nxt, fn);
+                const bool zp_ok = k + 1 < g.nz || (g.per & 4u);
+                const uint32_t base = sz * (uint32_t)k;
+#pragma unroll
+                for (int dy = 0; dy < 2; dy++) {
+                    const F4 di = un(cur.d.c[dy]);
+                    F4 ro;
+#pragma unroll
+                    for (int kk = 0; kk < 4; kk++) {
+                        const float rk = fc.rown[dy][kk];
+                        ro.v[kk] = rk;
+                        acc[1] += (double)rk * (double)rk;
+                        acc[3] += (double)rk;
+                        float fw[6];
+                        face6(o, cur.w, nxt.w, zp_ok, dy, kk, fw);
+                        const float a = act6(fc.z1, below, fn.z1.c, dy, kk, fw);
+                        float v = (rk - mn) - a;
+                        v = di.v[kk] != 0.0f ? v : 0.0f;
+                        if (kk < 2) s0 += v;
+                        else s1 += v;
+                    }
+                    st4(rout, base + o.row0 + (uint32_t)dy * sy, ro);
+                }
+                if (k & 1) {  // the second plane of an aggregate pair
+                    const uint32_t ci = (uint32_t)(o.x0 >> 1) + csy * (uint32_t)(o.y0 >> 1) + csz * (uint32_t)(k >> 1);
+                    st2(A.rc, ci, s0, s1);
+                    s0 = 0.0f;
+                    s1 = 0.0f;
+                }
+#pragma unroll
+                for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+                    for (int kk = 0; kk < 4; kk++) below[dy][kk] = fc.z1.c[dy][kk];
+                fc = fn;
+                cur = nxt;
+                nxt = pre;
+            }
+        }
+        reduce_finish<4>(vg, acc, A.part, sc, 3, FIN_RESTRICT_UPD);
+    }
+};
+
 // ---------------------------------------------------------------- launchers
 
 // Planes per work item (MHDP_MARCH_Z: even, 2..512; default 32) and the block cap
@@ -383,5 +536,20 @@ void launch_march_smooth(hipStream_t s, int store, const Grid3& g, const Grid3& 
     } else {
         using T = float;
         go<K_k_march_smooth<T, T, 2>>(s, G.x, {g, cg, r, r2, (const T*)dinv, (const T*)bx, (const T*)by, (const T*)bz, (const T*)zc, (T*)t, part, sc, mode, un, march_z()});
+    }
+}
+
+void launch_march_restrict_update(hipStream_t s, int store, const Grid3& cg, const Grid3& g, float* r, float* r2,
+                                  const float* q, const void* dinv, const void* bx, const void* by, const void* bz,
+                                  void* rc, double* part, Scal* sc) {
+    const uint32_t G = march_blocks(g);
+    if (store == STORE_BF16) {
+        using T = __hip_bfloat16;
+        go<K_march_restrict_update<T, T, 2>>(s, G, {cg, g, r, r2, q, (const T*)dinv, (const T*)bx, (const T*)by,
+                                                    (const T*)bz, (T*)rc, part, sc, march_z()});
+    } else {
+        using T = float;
+        go<K_march_restrict_update<T, T, 2>>(s, G, {cg, g, r, r2, q, (const T*)dinv, (const T*)bx, (const T*)by,
+                                                    (const T*)bz, (T*)rc, part, sc, march_z()});
     }
 }

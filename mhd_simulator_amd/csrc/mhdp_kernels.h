@@ -144,3 +144,13 @@ bool march_ok(const Grid3& g);
 void launch_march_smooth(hipStream_t s, int store, const Grid3& g, const Grid3& cg, const float* r, const float* r2,
                          const void* dinv, const void* bx, const void* by, const void* bz, const void* zc, void* t,
                          double* part, Scal* sc, int mode, bool use_next);
+void launch_march_restrict_update(hipStream_t s, int store, const Grid3& cg, const Grid3& g, float* r, float* r2,
+                                  const float* q, const void* dinv, const void* bx, const void* by, const void* bz,
+                                  void* rc, double* part, Scal* sc);
+
+// setup.hip — warm start (pressure_warm / pressure_commit)
+void launch_warm(hipStream_t s, const Grid3& g, float h, const uint32_t* mask, const double* prev, const float* bx,
+                 const float* by, const float* bz, const float* ustar, float* uout);
+void launch_commit(hipStream_t s, uint32_t n, const uint32_t* mask, const double* prev, double* phi);
+// phi = the previous potential relative to each component's anchor (0 off the fluid)
+void launch_old_potential(hipStream_t s, uint32_t n, const uint32_t* mask, const double* prev, double* phi);

@@ -765,6 +765,103 @@ struct K_k_from_bf16 {
     }
 };
 
+// ---------------------------------------------------------------- warm start
+// old_potential() (solver.wgsl:164-167): the previous potential relative to the cell's component
+// anchor on fluid cells, 0 elsewhere.
+__device__ __forceinline__ double old_potential(const uint32_t* __restrict__ mask, const double* __restrict__ prev,
+                                                uint32_t i) {
+    const uint32_t m = mask[i];
+    return m ? prev[i] - prev[m >> 2] : 0.0;
+}
+
+// pressure_warm (solver.wgsl:170-176): u* - beta (phi_old(c) - phi_old(c - e_a)) / h on every
+// open face; closed faces keep u*.
+struct K_warm {
+    static constexpr int MINB = 0;
+    struct Args {
+        Grid3 g;
+        float h;
+        const uint32_t* mask;
+        const double* prev;
+        const float* bx;
+        const float* by;
+        const float* bz;
+        const float* ustar;
+        float* uout;
+        uint32_t f0, f1, f2;
+    };
+    static __device__ void run(const Args& A, const VGrid& vg) {
+        const Grid3 g = A.g;
+        const uint32_t total = A.f0 + A.f1 + A.f2;
+        uint32_t lo, hi;
+        block_range(vg, total, lo, hi);
+        const float* B[3] = {A.bx, A.by, A.bz};
+        const int d[3] = {g.nx, g.ny, g.nz};
+        const uint32_t st[3] = {1u, (uint32_t)g.nx, (uint32_t)g.nx * (uint32_t)g.ny};
+        for (uint32_t f = lo + threadIdx.x; f < hi; f += MHDP_BLOCK) {
+            int a;
+            uint32_t k;
+            if (f < A.f0) { a = 0; k = f; }
+            else if (f < A.f0 + A.f1) { a = 1; k = f - A.f0; }
+            else { a = 2; k = f - A.f0 - A.f1; }
+            const uint32_t Px = (uint32_t)g.nx + (a == 0), Py = (uint32_t)g.ny + (a == 1);
+            int q[3];
+            q[0] = (int)(k % Px);
+            q[1] = (int)((k / Px) % Py);
+            q[2] = (int)(k / (Px * Py));
+            float out = A.ustar[f];
+            bool ok = true;
+            if (q[a] >= d[a]) {
+                if (g.per & (1u << a)) q[a] = 0;
+                else ok = false;
+            }
+            if (ok) {
+                const uint32_t ci = (uint32_t)q[0] + (uint32_t)g.nx * ((uint32_t)q[1] + (uint32_t)g.ny * (uint32_t)q[2]);
+                const float beta = B[a][ci];
+                if (beta > 0.0f) {
+                    const uint32_t li = q[a] > 0 ? ci - st[a] : ci + (uint32_t)(d[a] - 1) * st[a];
+                    const double dp = old_potential(A.mask, A.prev, ci) - old_potential(A.mask, A.prev, li);
+                    out = (float)((double)out - (double)beta * dp / (double)A.h);
+                }
+            }
+            A.uout[f] = out;
+        }
+    }
+};
+
+// The previous potential as a starting value: phi = phi_old (0 off the fluid).
+struct K_old_potential {
+    static constexpr int MINB = 0;
+    struct Args {
+        uint32_t n;
+        const uint32_t* mask;
+        const double* prev;
+        double* phi;
+    };
+    static __device__ void run(const Args& A, const VGrid& vg) {
+        uint32_t lo, hi;
+        block_range(vg, A.n, lo, hi);
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) A.phi[i] = old_potential(A.mask, A.prev, i);
+    }
+};
+
+// pressure_commit (solver.wgsl:181-183): phi += phi_old on fluid cells.
+struct K_commit {
+    static constexpr int MINB = 0;
+    struct Args {
+        uint32_t n;
+        const uint32_t* mask;
+        const double* prev;
+        double* phi;
+    };
+    static __device__ void run(const Args& A, const VGrid& vg) {
+        uint32_t lo, hi;
+        block_range(vg, A.n, lo, hi);
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK)
+            if (A.mask[i]) A.phi[i] = A.phi[i] + old_potential(A.mask, A.prev, i);
+    }
+};
+
 // ---------------------------------------------------------------- launchers
 
 static bool row_vector_ok(const Grid3& g) { return (g.nx & 3) == 0 && std::getenv("MHDP_SCALAR_SETUP") == nullptr; }
@@ -835,4 +932,17 @@ void launch_to_bf16(hipStream_t s, uint32_t n, const float* src, void* dst) {
 }
 void launch_from_bf16(hipStream_t s, uint32_t n, const void* src, float* dst) {
     go<K_k_from_bf16>(s, grid_blocks(n), {n, (const __hip_bfloat16*)src, dst});
+}
+
+void launch_warm(hipStream_t s, const Grid3& g, float h, const uint32_t* mask, const double* prev, const float* bx,
+                 const float* by, const float* bz, const float* ustar, float* uout) {
+    uint32_t f0, f1, f2;
+    face_counts(g, f0, f1, f2);
+    go<K_warm>(s, grid_blocks(f0 + f1 + f2), {g, h, mask, prev, bx, by, bz, ustar, uout, f0, f1, f2});
+}
+void launch_commit(hipStream_t s, uint32_t n, const uint32_t* mask, const double* prev, double* phi) {
+    go<K_commit>(s, grid_blocks(n), {n, mask, prev, phi});
+}
+void launch_old_potential(hipStream_t s, uint32_t n, const uint32_t* mask, const double* prev, double* phi) {
+    go<K_old_potential>(s, grid_blocks(n), {n, mask, prev, phi});
 }

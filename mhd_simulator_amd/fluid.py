@@ -147,21 +147,36 @@ class PressureProjection:
         return int(self.lib.mhdp_device_bytes(self.handle))
 
     def _opts(self, precision, precond, gauge, refine_tol, max_outer, inner_tol, smooth_weight=0.0,
-              coarse_scale=0.0):
+              coarse_scale=0.0, warm_start=False):
         c = self.config
         return _lib.Opts(c.relative_tolerance, c.absolute_divergence_tolerance,
                          c.max_iterations, _lib.PRECOND[precond], _lib.PRECISION[precision], _lib.GAUGE[gauge],
-                         refine_tol, max_outer, inner_tol, smooth_weight, coarse_scale)
+                         refine_tol, max_outer, inner_tol, smooth_weight, coarse_scale, int(bool(warm_start)))
+
+    def set_potential(self, phi):
+        """The previous potential a warm start begins from (numpy float64 or a device tensor)."""
+        if hasattr(phi, "data_ptr"):
+            if phi.numel() != self.grid.cells:
+                raise ValueError("one potential per cell")
+            self._check(self.lib.mhdp_set_potential(self.handle, _ptr(phi), 1))
+        else:
+            a = np.ascontiguousarray(np.asarray(phi, dtype=np.float64).ravel())
+            if a.size != self.grid.cells:
+                raise ValueError("one potential per cell")
+            self._check(self.lib.mhdp_set_potential(self.handle, _ptr(a), 0))
 
     # -- FluidState::project(false)
     def project(self, u_star, *, precision="fp32", precond="auto", gauge="auto", refine_tol=1e-8,
-                max_outer=0, inner_tol=0.0, smooth_weight=0.0, coarse_scale=0.0, u_out=None, phi_out=None):
+                max_outer=0, inner_tol=0.0, smooth_weight=0.0, coarse_scale=0.0, warm_start=False, u_out=None,
+                phi_out=None):
         """Project u_star; returns (iterations, relative residual) like the reference.
+        warm_start: FluidState::project(true) - begin from the previous potential.
 
         numpy input: host path, results in ``self.velocity`` / ``self.phi``.
         torch CUDA input: device path; ``u_out`` (float32, faces) and ``phi_out``
         (float64, cells) are device tensors the caller provides or None."""
-        o = self._opts(precision, precond, gauge, refine_tol, max_outer, inner_tol, smooth_weight, coarse_scale)
+        o = self._opts(precision, precond, gauge, refine_tol, max_outer, inner_tol, smooth_weight, coarse_scale,
+                       warm_start)
         st = _lib.Stats()
         if hasattr(u_star, "data_ptr"):
             if u_star.numel() != self.grid.faces:

@@ -279,3 +279,32 @@ def test_marching_smoother_matches_bricks(orc, name, monkeypatch):
         assert orc.true_residual(p.grid, pr.mask, p.alpha, p.ustar, pr.phi) <= 1e-8
         phi_ref, u_ref = oracle_solution(orc, p, pr.mask)
         assert relerr(pr.velocity, u_ref) < 1e-6
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+@pytest.mark.parametrize("name", ["config2-32", "walls-pinned", "plate-64"])
+def test_warm_start(orc, name, precision):
+    """project(warm_start=true) (mod.rs:1015, pressure_warm / pressure_commit): starting from the
+    previous potential, a repeated u* needs (almost) no iterations and returns the same field;
+    a new u* reaches the same projected faces as the FP64 oracle's cold solve."""
+    p = problem(name)
+    kw = dict(precision=precision, precond="mg", refine_tol=1e-8)
+    with projection(p) as pr:
+        pr.project(p.ustar, **kw)
+        u1, it1 = pr.velocity.copy(), pr.stats["iterations"]
+        pr.project(p.ustar, warm_start=True, **kw)
+        if precision == "mixed":  # the previous potential already meets the target
+            assert pr.stats["iterations"] == 0
+        else:
+            assert pr.stats["iterations"] < it1
+        tol_u = 1e-6 if precision == "mixed" else 3e-4
+        assert relerr(pr.velocity, u1) < tol_u
+        u2 = C.seeded_faces(p.grid, 777)
+        phi_prev = pr.phi.copy()
+        pr.set_potential(phi_prev)
+        pr.project(u2, warm_start=True, **kw)
+        q = C.Problem(p.name, p.grid, p.fluid, p.alpha, u2)
+        _, u_ref = oracle_solution(orc, q, pr.mask)
+        assert relerr(pr.velocity, u_ref) < tol_u
+        if precision == "mixed":
+            assert orc.true_residual(p.grid, pr.mask, p.alpha, u2, pr.phi) <= 1e-8
