@@ -124,13 +124,46 @@ int mhdp_project_dev(mhdp_ctx *ctx, const float *u_star_dev, const mhdp_opts *op
  * projection replaces it with its own result. */
 int mhdp_set_potential(mhdp_ctx *ctx, const double *phi, int on_device);
 /* Many independent projections (uneven grids), device pointers.  No counterpart in the
- * reference (one FluidState per grid, mod.rs:1015): each context keeps its own stream and
- * a pool of host threads (MHDP_BATCH_THREADS, default 8) drives them concurrently, largest
- * grids first.  Contexts must be distinct.  Returns the status of the first failing problem
+ * reference (one FluidState per grid, mod.rs:1015).  Default: lockstep - every kernel of the
+ * projection is issued ONCE for all problems (each problem owns its own blocks of the launch, so
+ * its results are bit-identical to its single projection).  MHDP_BATCH_MODE=threads: each
+ * context on its own stream, driven by a pool of host threads (MHDP_BATCH_THREADS, default 8),
+ * largest grids first.  Contexts must be distinct.  Returns the status of the first failing problem
  * in index order (its text in that context's mhdp_last_error); stats[k] is filled for every
  * problem that succeeded. */
 int mhdp_project_batch(mhdp_ctx *const *ctxs, int count, const float *const *u_star_dev, const mhdp_opts *opts,
                        float *const *u_out_dev, double *const *phi_out_dev, mhdp_stats *stats);
+/* fields() + diagnostics() (solver.wgsl:256-291) and read_diagnostics (mod.rs:1116-1146):
+ * the display packet of every cell after a projection and the step's diagnostics. */
+typedef struct {
+    float dt;              /* time step (Params.physical.y): pressure = rho_liquid * phi / dt */
+    float acceleration[3]; /* body acceleration (Params.acceleration.xyz), m/s^2 */
+    int mushy;             /* Params.flags.w != 0: reserved.y (melt fraction) is kept */
+    int gauge;             /* as mhdp_opts.gauge: mean-free pressure is relative to the anchor */
+} mhdp_fields_params;
+
+typedef struct {
+    double max_speed;      /* max_speed_m_s */
+    double max_divergence; /* max_divergence_s_inv */
+    double rms_divergence; /* sqrt(sum div^2 / fluid cells) */
+    double kinetic_energy; /* kinetic_energy_j */
+    double fluid_mass;     /* rho_liquid * sum(melt) * h^3 (fluid_mass_kg) */
+    double max_pressure;   /* max_pressure_pa */
+    double net_flux;       /* sum(div) * h^3 (net_flux_m3_s) */
+    double max_force;      /* FluidState.force_max */
+    double max_face_speed; /* FluidState.max_face_speed */
+    uint32_t nonfinite_cells;
+} mhdp_diagnostics;
+
+/* Device pointers: u_faces the projected faces (layout above), phi the potential (one double per
+ * cell), em NULL or 16 floats per cell with the force at [8..10] (EmCell.force), cells 16 floats
+ * per cell (FluidCell: velocity+speed, pressure, divergence, 0, 0, force, 0, reserved), read
+ * for its reserved words and rewritten.  Totals are FP64 per-block partials folded in block order.
+ * MHDP_NONFINITE ("Nonfinite fluid fields; state restored") when a total is not finite or a cell
+ * is flagged; unlike the other entry points, cells has then been written (the reference's step
+ * restores its checkpoint, mod.rs:1312-1328, and so must the caller). */
+int mhdp_fields_dev(mhdp_ctx *ctx, const float *u_faces_dev, const double *phi_dev, const float *em_dev,
+                    const mhdp_fields_params *params, float *cells_dev, mhdp_diagnostics *diag);
 const char *mhdp_last_error(const mhdp_ctx *ctx);
 void mhdp_destroy(mhdp_ctx *ctx);
 /* Device bytes held by the context (FluidState.bytes, mod.rs:591). */

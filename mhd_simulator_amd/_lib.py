@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libmhdp.so")
 SYMBOLS = (
     "mhdp_create", "mhdp_set_stream", "mhdp_set_cells", "mhdp_get_mask", "mhdp_project", "mhdp_project_dev",
     "mhdp_set_potential",
-    "mhdp_project_batch", "mhdp_last_error", "mhdp_destroy", "mhdp_device_bytes",
+    "mhdp_project_batch", "mhdp_fields_dev", "mhdp_last_error", "mhdp_destroy", "mhdp_device_bytes",
     "mhdp_anchors", "mhdp_hierarchy", "mhdp_set_kernel_timing", "mhdp_kernel_time",
     "mhdp_probe_coefficients", "mhdp_probe_apply", "mhdp_probe_vcycle", "mhdp_probe_mg_graph", "mhdp_probe_rhs",
 )
@@ -53,6 +53,20 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class FieldsParams(C.Structure):
+    _fields_ = [("dt", C.c_float), ("acceleration", C.c_float * 3), ("mushy", C.c_int), ("gauge", C.c_int)]
+
+
+class Diagnostics(C.Structure):
+    _fields_ = [("max_speed", C.c_double), ("max_divergence", C.c_double), ("rms_divergence", C.c_double),
+                ("kinetic_energy", C.c_double), ("fluid_mass", C.c_double), ("max_pressure", C.c_double),
+                ("net_flux", C.c_double), ("max_force", C.c_double), ("max_face_speed", C.c_double),
+                ("nonfinite_cells", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -81,6 +95,7 @@ def load(path: str = LIB_PATH):
         "mhdp_project": (I, [P, P, C.POINTER(Opts), P, P, C.POINTER(Stats)]),
         "mhdp_project_dev": (I, [P, P, C.POINTER(Opts), P, P, C.POINTER(Stats)]),
         "mhdp_project_batch": (I, [P, I, P, C.POINTER(Opts), P, P, P]),
+        "mhdp_fields_dev": (I, [P, P, P, P, C.POINTER(FieldsParams), P, C.POINTER(Diagnostics)]),
         "mhdp_last_error": (C.c_char_p, [P]),
         "mhdp_destroy": (None, [P]),
         "mhdp_device_bytes": (C.c_uint64, [P]),

@@ -59,6 +59,7 @@ struct mhdp_ctx {
     double* phi_prev = nullptr;  // potential of the last successful projection (warm start)
     bool prev_valid = false;
     float* uwarm = nullptr;      // u* corrected by the previous potential (warm start)
+    double* fields_part = nullptr;  // fields(): per-block totals, then the 10 folded totals
     void *bx16 = nullptr, *by16 = nullptr, *bz16 = nullptr, *dinv16 = nullptr;
     // multigrid
     std::vector<CoarseArrays> coarse;  // index l-1 for level l
@@ -1233,6 +1234,48 @@ int mhdp_set_potential(mhdp_ctx* c, const double* phi, int on_device) {
     return MHDP_OK;
 }
 
+int mhdp_fields_dev(mhdp_ctx* c, const float* u_faces, const double* phi, const float* em,
+                    const mhdp_fields_params* fp, float* cells, mhdp_diagnostics* diag) {
+    if (!c) return MHDP_INVALID;
+    c->err.clear();
+    if (!c->cells) return fail(c, MHDP_INVALID, "Fluid cells are not set");
+    if (!u_faces || !phi || !fp || !cells) return fail(c, MHDP_INVALID, "Missing fields argument");
+    if ((reinterpret_cast<uintptr_t>(cells) & 15u) != 0)
+        return fail(c, MHDP_INVALID, "Cell packets must be 16-byte aligned");
+    if (!(fp->dt > 0.0f) || !std::isfinite(fp->dt) || (fp->gauge != MHDP_GAUGE_AUTO && fp->gauge != MHDP_GAUGE_PINNED))
+        return fail(c, MHDP_INVALID, "Invalid fields parameters");
+    const uint32_t blocks = grid_blocks(c->g.n);
+    int rc = dalloc(c, (void**)&c->fields_part, ((size_t)blocks + 1) * kFieldTotals * sizeof(double));
+    if (rc) return rc;
+    const bool meanfree = fp->gauge != MHDP_GAUGE_PINNED && c->components == 1;
+    double* totals = c->fields_part + (size_t)blocks * kFieldTotals;
+    launch_fields(c->stream, c->g, c->grid.h, c->phase.rho_liquid, c->phase.rho_air, c->phase.two_phase ? 1 : 0,
+                  fp->dt, fp->acceleration, fp->mushy ? 1 : 0, meanfree ? 1 : 0, c->mask, u_faces, phi, em, cells,
+                  c->fields_part, totals);
+    HIP_OK(hipGetLastError());
+    double t[kFieldTotals];
+    HIP_OK(hipMemcpyAsync(t, totals, sizeof t, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    // read_diagnostics (src/fluid/mod.rs:1116-1146)
+    const double h3 = (double)c->grid.h * c->grid.h * c->grid.h;
+    mhdp_diagnostics d{};
+    d.max_speed = t[0];
+    d.max_divergence = t[1];
+    d.rms_divergence = std::sqrt(t[2] / (double)std::max<uint32_t>(c->fluid_cells, 1u));
+    d.kinetic_energy = t[3];
+    d.max_pressure = t[4];
+    d.net_flux = t[5] * h3;
+    d.max_force = t[6];
+    d.nonfinite_cells = (uint32_t)t[7];
+    d.max_face_speed = t[8];
+    d.fluid_mass = (double)c->phase.rho_liquid * t[9] * h3;
+    if (diag) *diag = d;
+    for (double v : t)
+        if (!std::isfinite(v)) return fail(c, MHDP_NONFINITE, "Nonfinite fluid fields; state restored");
+    if (t[7] != 0.0) return fail(c, MHDP_NONFINITE, "Nonfinite fluid fields; state restored");
+    return MHDP_OK;
+}
+
 int mhdp_project_dev(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, float* uout, double* phi, mhdp_stats* st) {
     if (!c) return MHDP_INVALID;
     c->err.clear();
@@ -1368,7 +1411,7 @@ void mhdp_destroy(mhdp_ctx* c) {
     else (void)hipDeviceSynchronize();
     void* ptrs[] = {c->mask, c->alpha, c->bx, c->by, c->bz, c->diag, c->dinv, c->b, c->x, c->r, c->r2, c->p, c->p2, c->q, c->z,
                     c->t, c->x64, c->b64, c->bx16, c->by16, c->bz16, c->dinv16, c->sc, c->part, c->ustar_stage,
-                    c->uout_stage, c->phi_stage, c->fluid8, c->cc_counts, c->phi_prev, c->uwarm};
+                    c->uout_stage, c->phi_stage, c->fluid8, c->cc_counts, c->phi_prev, c->uwarm, c->fields_part};
     for (void* p : ptrs) dfree(c, p);
     for (CoarseArrays& a : c->coarse) {
         void* q[] = {a.bx, a.by, a.bz, a.react, a.diag, a.dinv, a.bx16, a.by16, a.bz16, a.react16, a.dinv16,

@@ -154,3 +154,10 @@ void launch_warm(hipStream_t s, const Grid3& g, float h, const uint32_t* mask, c
 void launch_commit(hipStream_t s, uint32_t n, const uint32_t* mask, const double* prev, double* phi);
 // phi = the previous potential relative to each component's anchor (0 off the fluid)
 void launch_old_potential(hipStream_t s, uint32_t n, const uint32_t* mask, const double* prev, double* phi);
+
+// setup.hip — fields() + diagnostics(): FluidCell packets and the 10 totals (part: 10 doubles
+// per block of grid_blocks(n))
+constexpr int kFieldTotals = 10;
+void launch_fields(hipStream_t s, const Grid3& g, float h, float rho_l, float rho_air, int two_phase, float dt,
+                   const float acc[3], int mushy, int meanfree, const uint32_t* mask, const float* faces,
+                   const double* phi, const float* em, float* cells, double* part, double* totals);

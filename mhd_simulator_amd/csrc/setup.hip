@@ -862,6 +862,167 @@ struct K_commit {
     }
 };
 
+// ---------------------------------------------------------------- display packet
+// fields() (solver.wgsl:256-276): the 16-float FluidCell packet per cell from the projected faces
+// and the potential, in the reference's FP32 operation order, and the diagnostics() totals
+// (max speed, max |div|, sum div^2, sum kinetic energy, max |p|, sum div, max |force|,
+// non-finite cells, max |face velocity|, sum melt) as FP64 per-block partials.
+__device__ __forceinline__ bool fields_is_max(int k) { return k == 0 || k == 1 || k == 4 || k == 6 || k == 8; }
+
+struct K_fields {
+    static constexpr int MINB = 0;
+    struct Args {
+        Grid3 g;
+        float h, rho_l, rho_air, dt;
+        float acc[3];
+        int two_phase, mushy, meanfree;
+        const uint32_t* mask;
+        const float* faces;
+        uint32_t f0, f1;
+        const double* phi;
+        const float* em;
+        float* cells;
+        double* part;
+    };
+    static __device__ bool fl(const Grid3& g, const uint32_t* __restrict__ mask, int x, int y, int z) {
+        if (g.per & 1u) x = x < 0 ? x + g.nx : (x >= g.nx ? x - g.nx : x);
+        if (g.per & 2u) y = y < 0 ? y + g.ny : (y >= g.ny ? y - g.ny : y);
+        if (g.per & 4u) z = z < 0 ? z + g.nz : (z >= g.nz ? z - g.nz : z);
+        if (x < 0 || y < 0 || z < 0 || x >= g.nx || y >= g.ny || z >= g.nz) return false;
+        return mask[(uint32_t)x + (uint32_t)g.nx * ((uint32_t)y + (uint32_t)g.ny * (uint32_t)z)] != 0u;
+    }
+    // face(pos, a, false) (solver.wgsl:33-36): the face value if open, else 0
+    static __device__ float face(const Args& A, int a, int x, int y, int z) {
+        const Grid3& g = A.g;
+        int q[3] = {x, y, z};
+        const int d[3] = {g.nx, g.ny, g.nz};
+        for (int b = 0; b < 3; b++)
+            if ((g.per & (1u << b)) && q[b] >= d[b] + (b == a ? 1 : 0)) q[b] -= d[b];
+        if ((g.per & (1u << a)) && q[a] == d[a]) q[a] = 0;
+        if (q[a] < 0 || q[a] > d[a]) return 0.0f;
+        int l[3] = {q[0], q[1], q[2]};
+        l[a] -= 1;
+        if (!fl(g, A.mask, q[0], q[1], q[2]) || !fl(g, A.mask, l[0], l[1], l[2])) return 0.0f;
+        const uint32_t Px = (uint32_t)g.nx + (a == 0), Py = (uint32_t)g.ny + (a == 1);
+        const uint32_t off = a == 0 ? 0u : (a == 1 ? A.f0 : A.f0 + A.f1);
+        return A.faces[off + (uint32_t)q[0] + Px * ((uint32_t)q[1] + Py * (uint32_t)q[2])];
+    }
+    static __device__ void run(const Args& A, const VGrid& vg) {
+        const Grid3 g = A.g;
+        uint32_t lo, hi;
+        block_range(vg, g.n, lo, hi);
+        double tot[kFieldTotals];
+#pragma unroll
+        for (int k = 0; k < kFieldTotals; k++) tot[k] = 0.0;
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+            float* cell = A.cells + 16u * (size_t)i;
+            const float o1 = cell[13], o2 = cell[14], o3 = cell[15];
+            float out[16];
+#pragma unroll
+            for (int k = 0; k < 16; k++) out[k] = 0.0f;
+            if (A.two_phase) { out[13] = o1; out[14] = o2; out[15] = o3; }
+            const uint32_t m = A.mask[i];
+            if (m) {
+                int x, y, z;
+                decode(g, i, x, y, z);
+                const int pos[3] = {x, y, z};
+                float v[3], lo_[3], hi_[3], fe = 0.0f, mf = 0.0f, sum = 0.0f;
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    int q[3] = {x, y, z};
+                    q[a] += 1;
+                    lo_[a] = face(A, a, pos[0], pos[1], pos[2]);
+                    hi_[a] = face(A, a, q[0], q[1], q[2]);
+                    v[a] = 0.5f * (lo_[a] + hi_[a]);
+                    fe += 0.5f * (lo_[a] * lo_[a] + hi_[a] * hi_[a]);
+                    mf = fmaxf(mf, fmaxf(fabsf(lo_[a]), fabsf(hi_[a])));
+                }
+#pragma unroll
+                for (int a = 0; a < 3; a++) sum += hi_[a] - lo_[a];
+                const float div = sum / A.h;
+                float melt = A.mushy ? o1 : 1.0f;
+                if (A.two_phase) melt *= o2;
+                const float speed = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                const float pv = A.meanfree ? (float)A.phi[i] - (float)A.phi[m >> 2] : (float)A.phi[i];
+                const float pressure = pv * A.rho_l / A.dt;
+                float force[3] = {0.0f, 0.0f, 0.0f};
+                if (A.em)
+                    for (int a = 0; a < 3; a++) force[a] = A.em[16u * (size_t)i + 8u + (uint32_t)a];
+#pragma unroll
+                for (int a = 0; a < 3; a++) out[a] = v[a];
+                out[3] = speed;
+                out[4] = pressure;
+                out[5] = div;
+#pragma unroll
+                for (int a = 0; a < 3; a++) out[8 + a] = A.rho_l * A.acc[a] + force[a];
+                out[12] = 1.0f;
+                out[13] = A.mushy ? o1 : 1.0f;
+                out[14] = o2;
+                out[15] = o3;
+                bool bad = false;
+#pragma unroll
+                for (int a = 0; a < 3; a++) bad |= v[a] != v[a] || force[a] != force[a] || fabsf(force[a]) > 1e30f;
+                bad |= speed > 1e25f || !(fabsf(pressure) < 1e30f) || !(fabsf(div) < 1e30f);
+                const float rho = A.two_phase ? A.rho_air * (1.0f - o2) + A.rho_l * o2 : A.rho_l;
+                const float ke = 0.5f * rho * fe * A.h * A.h * A.h;
+                const float fl = sqrtf(force[0] * force[0] + force[1] * force[1] + force[2] * force[2]);
+                tot[0] = fmax(tot[0], (double)speed);
+                tot[1] = fmax(tot[1], (double)fabsf(div));
+                tot[2] += (double)(div * div);
+                tot[3] += (double)ke;
+                tot[4] = fmax(tot[4], (double)fabsf(pressure));
+                tot[5] += (double)div;
+                tot[6] = fmax(tot[6], (double)fl);
+                tot[7] += bad ? 1.0 : 0.0;
+                tot[8] = fmax(tot[8], (double)mf);
+                tot[9] += (double)melt;
+            }
+            float4* c4 = reinterpret_cast<float4*>(cell);
+#pragma unroll
+            for (int k = 0; k < 4; k++) c4[k] = make_float4(out[4 * k], out[4 * k + 1], out[4 * k + 2], out[4 * k + 3]);
+        }
+        // block totals (fixed order), one partial record per block
+        __shared__ double red[MHDP_BLOCK / 64][kFieldTotals];
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+        for (int k = 0; k < kFieldTotals; k++) {
+            double v = tot[k];
+            for (int o = 32; o > 0; o >>= 1) {
+                const double w = __shfl_down(v, o, 64);
+                v = fields_is_max(k) ? fmax(v, w) : v + w;
+            }
+            if (lane == 0) red[wid][k] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < kFieldTotals) {
+            const int k = threadIdx.x;
+            double v = red[0][k];
+            for (int w = 1; w < MHDP_BLOCK / 64; w++) v = fields_is_max(k) ? fmax(v, red[w][k]) : v + red[w][k];
+            A.part[(size_t)kFieldTotals * vg.b + (uint32_t)k] = v;
+        }
+    }
+};
+
+// diagnostics() (solver.wgsl:284-291): fold the per-block partials in block order.
+struct K_fields_total {
+    static constexpr int MINB = 0;
+    struct Args {
+        const double* part;
+        uint32_t blocks;
+        double* out;
+    };
+    static __device__ void run(const Args& A, const VGrid&) {
+        if (threadIdx.x >= kFieldTotals) return;
+        const int k = threadIdx.x;
+        double v = 0.0;
+        for (uint32_t b = 0; b < A.blocks; b++) {
+            const double w = A.part[(size_t)kFieldTotals * b + (uint32_t)k];
+            v = fields_is_max(k) ? fmax(v, w) : v + w;
+        }
+        A.out[k] = v;
+    }
+};
+
 // ---------------------------------------------------------------- launchers
 
 static bool row_vector_ok(const Grid3& g) { return (g.nx & 3) == 0 && std::getenv("MHDP_SCALAR_SETUP") == nullptr; }
@@ -945,4 +1106,16 @@ void launch_commit(hipStream_t s, uint32_t n, const uint32_t* mask, const double
 }
 void launch_old_potential(hipStream_t s, uint32_t n, const uint32_t* mask, const double* prev, double* phi) {
     go<K_old_potential>(s, grid_blocks(n), {n, mask, prev, phi});
+}
+
+void launch_fields(hipStream_t s, const Grid3& g, float h, float rho_l, float rho_air, int two_phase, float dt,
+                   const float acc[3], int mushy, int meanfree, const uint32_t* mask, const float* faces,
+                   const double* phi, const float* em, float* cells, double* part, double* totals) {
+    uint32_t f0, f1, f2;
+    face_counts(g, f0, f1, f2);
+    const uint32_t blocks = grid_blocks(g.n);
+    K_fields::Args a{g, h, rho_l, rho_air, dt, {acc[0], acc[1], acc[2]}, two_phase, mushy, meanfree, mask, faces,
+                     f0, f1, phi, em, cells, part};
+    go<K_fields>(s, blocks, a);
+    go<K_fields_total>(s, 1, {part, blocks, totals});
 }

@@ -316,6 +316,75 @@ void orc_correct_f64(const orc_grid *g, const uint32_t *mask, const float *alpha
     correct_f64(&c, ustar, phi, u_out);
 }
 
+/* fields() (solver.wgsl:256-276) per cell in its FP32 operation order; totals as diagnostics()
+ * (solver.wgsl:284-291) reduces them, here in FP64 and cell order. */
+void orc_fields(const orc_grid *g, const uint32_t *mask, const float *faces, const double *phi, const float *em,
+                const orc_fields_params *p, float *cells, double *totals) {
+    ctx_t c;
+    ctx_init(&c, g, mask, NULL, p->meanfree);
+    for (int k = 0; k < 10; k++) totals[k] = 0.0;
+    const float h = g->h, rho_l = g->rho_liquid;
+    for (uint64_t i = 0; i < c.n; i++) {
+        float *cell = cells + 16 * i;
+        const float old[4] = {cell[12], cell[13], cell[14], cell[15]};
+        float out[16];
+        memset(out, 0, sizeof out);
+        if (g->two_phase) { out[13] = old[1]; out[14] = old[2]; out[15] = old[3]; }
+        if (mask[i]) {
+            int32_t pos[3];
+            decode3(i, c.d, pos);
+            float v[3], fe = 0.0f, mf = 0.0f, sum = 0.0f;
+            for (int a = 0; a < 3; a++) {
+                int32_t q[3] = {pos[0], pos[1], pos[2]};
+                q[a] += 1;
+                const float lo = face_value(&c, faces, pos, a), hi = face_value(&c, faces, q, a);
+                v[a] = 0.5f * (lo + hi);
+                fe += 0.5f * (lo * lo + hi * hi);
+                mf = fmaxf(mf, fmaxf(fabsf(lo), fabsf(hi)));
+            }
+            for (int a = 0; a < 3; a++) {  /* divergence(pos, false), solver.wgsl:104-106 */
+                int32_t q[3] = {pos[0], pos[1], pos[2]};
+                q[a] += 1;
+                sum += face_value(&c, faces, q, a) - face_value(&c, faces, pos, a);
+            }
+            const float div = sum / h;
+            float melt = p->mushy ? old[1] : 1.0f;
+            if (g->two_phase) melt *= old[2];
+            const float speed = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            const float pv = p->meanfree ? (float)phi[i] - (float)phi[mask[i] >> 2] : (float)phi[i];
+            const float pressure = pv * rho_l / p->dt;
+            float force[3] = {0.0f, 0.0f, 0.0f};
+            if (em) for (int a = 0; a < 3; a++) force[a] = em[16 * i + 8 + a];
+            for (int a = 0; a < 3; a++) out[a] = v[a];
+            out[3] = speed;
+            out[4] = pressure;
+            out[5] = div;
+            for (int a = 0; a < 3; a++) out[8 + a] = rho_l * p->acceleration[a] + force[a];
+            out[12] = 1.0f;
+            out[13] = p->mushy ? old[1] : 1.0f;
+            out[14] = old[2];
+            out[15] = old[3];
+            int bad = 0;
+            for (int a = 0; a < 3; a++) bad |= v[a] != v[a] || force[a] != force[a] || fabsf(force[a]) > 1e30f;
+            bad |= speed > 1e25f || !(fabsf(pressure) < 1e30f) || !(fabsf(div) < 1e30f);
+            const float rho = g->two_phase ? g->rho_air * (1.0f - old[2]) + rho_l * old[2] : rho_l;
+            const float ke = 0.5f * rho * fe * h * h * h;
+            const float fl = sqrtf(force[0] * force[0] + force[1] * force[1] + force[2] * force[2]);
+            totals[0] = fmax(totals[0], speed);
+            totals[1] = fmax(totals[1], fabsf(div));
+            totals[2] += (double)(div * div);
+            totals[3] += (double)ke;
+            totals[4] = fmax(totals[4], fabsf(pressure));
+            totals[5] += (double)div;
+            totals[6] = fmax(totals[6], fl);
+            totals[7] += bad ? 1.0 : 0.0;
+            totals[8] = fmax(totals[8], mf);
+            totals[9] += (double)melt;
+        }
+        memcpy(cell, out, sizeof out);
+    }
+}
+
 int orc_project(const orc_grid *g, const uint32_t *mask, const float *alpha, const float *ustar,
                 const orc_opts *o, double *phi_out, float *u_out, orc_stats *st) {
 #ifdef _OPENMP

@@ -110,6 +110,20 @@ void orc_correct_f64(const orc_grid *g, const uint32_t *mask, const float *alpha
 int orc_project(const orc_grid *g, const uint32_t *mask, const float *alpha, const float *ustar,
                 const orc_opts *o, double *phi_out, float *u_out, orc_stats *st);
 
+/* ---- fields() + diagnostics() (solver.wgsl:256-291, mod.rs:1116-1146) ---- */
+typedef struct {
+    float dt;              /* Params.physical.y */
+    float acceleration[3]; /* Params.acceleration.xyz */
+    int mushy;             /* Params.flags.w != 0 */
+    int meanfree;          /* mean_free_pressure() */
+} orc_fields_params;
+/* cells: 16 floats per cell (velocity.xyzw, pressure.xyzw, force.xyzw, reserved.xyzw), read for
+ * the reserved lanes and overwritten; em: 16 floats per cell (force at 8..10) or NULL;
+ * totals[10]: max speed, max |div|, sum div^2, sum kinetic energy, max |p|, sum div, max |force|,
+ * non-finite cells, max |face velocity|, sum melt (FP64 sums, in cell order). */
+void orc_fields(const orc_grid *g, const uint32_t *mask, const float *faces, const double *phi, const float *em,
+                const orc_fields_params *p, float *cells, double *totals);
+
 /* ---- multigrid probes (mirror the reference's test-only entry points) ---- */
 typedef struct orc_mg orc_mg;
 orc_mg *orc_mg_new(const orc_grid *g, const uint32_t *mask, const float *alpha, int meanfree, int precision);

@@ -65,6 +65,7 @@ def lib():
         L.orc_correct_f64.argtypes = [P, P, P, P, P, P]
         L.orc_project.restype = C.c_int
         L.orc_project.argtypes = [P, P, P, P, P, P, P, P]
+        L.orc_fields.argtypes = [P, P, P, P, P, P, P, P]
         L.orc_mg_new.restype = P
         L.orc_mg_new.argtypes = [P, P, P, C.c_int, C.c_int]
         L.orc_mg_free.argtypes = [P]
@@ -164,6 +165,27 @@ def correct(g, mask, alpha, ustar, phi):
     lib().orc_correct_f64(C.byref(gg), _ptr(mask), _ptr(alpha), _ptr(np.ascontiguousarray(ustar, dtype=np.float32)),
                           _ptr(np.ascontiguousarray(phi, dtype=np.float64)), _ptr(out))
     return out
+
+
+class _FieldsParams(C.Structure):
+    _fields_ = [("dt", C.c_float), ("acceleration", C.c_float * 3), ("mushy", C.c_int), ("meanfree", C.c_int)]
+
+
+def fields(g, mask, faces, phi, cells, dt=1e-3, acceleration=(0.0, 0.0, -9.81), mushy=False, meanfree=None, em=None):
+    """fields() + diagnostics() (solver.wgsl:256-291): the 16-float FluidCell packet per cell
+    (cells: reserved lanes read, all overwritten; returned as a copy) and the 10 totals."""
+    gg = _grid(g)
+    if meanfree is None:
+        meanfree = int(np.count_nonzero((mask & 3) == 2)) == 1
+    prm = _FieldsParams(dt, (C.c_float * 3)(*acceleration), int(bool(mushy)), int(bool(meanfree)))
+    out = np.ascontiguousarray(np.array(cells, dtype=np.float32).reshape(-1))
+    assert out.size == 16 * g.cells
+    totals = np.zeros(10, dtype=np.float64)
+    lib().orc_fields(C.byref(gg), _ptr(mask), _ptr(np.ascontiguousarray(faces, dtype=np.float32)),
+                     _ptr(np.ascontiguousarray(phi, dtype=np.float64)),
+                     None if em is None else _ptr(np.ascontiguousarray(em, dtype=np.float32).reshape(-1)),
+                     C.byref(prm), _ptr(out), _ptr(totals))
+    return out.reshape(g.cells, 16), totals
 
 
 def project(g, mask, alpha, ustar, rel_tol=1e-5, abs_div_tol=0.0, max_iter=1000, precond=0, gauge=0,
