@@ -164,6 +164,26 @@ typedef struct {
  * restores its checkpoint, mod.rs:1312-1328, and so must the caller). */
 int mhdp_fields_dev(mhdp_ctx *ctx, const float *u_faces_dev, const double *phi_dev, const float *em_dev,
                     const mhdp_fields_params *params, float *cells_dev, mhdp_diagnostics *diag);
+/* The momentum step that produces u* for the projection (src/fluid/mod.rs:1285-1303): predict, then
+ * viscous (solver.wgsl:127-163), mushy mobility factor off.  Device pointers, faces as above. */
+typedef struct {
+    float dt;              /* Params.physical.y */
+    float viscosity;       /* kinematic viscosity, Params.physical.z (single phase; >= 0) */
+    float acceleration[3]; /* Params.acceleration.xyz */
+    float surface_tension; /* Params.surface.z (two-phase capillary force) */
+    int advect;            /* Params.flags.y != 0: semi-Lagrangian RK2 advection, else a copy */
+} mhdp_step_params;
+/* predict: u_pred = u advected along itself (departure point from the velocity at the face centre
+ * and at the half-step midpoint, trilinear samples of the open faces); closed faces 0.  u_pred may
+ * alias u. */
+int mhdp_predict_dev(mhdp_ctx *ctx, const float *u_dev, const mhdp_step_params *params, float *u_pred_dev);
+/* viscous: u_star = (u_pred + nu dt/h^2 lap(u_pred) + dt (g + (F_em,L + F_em,R)/(2 rho) + capillary))
+ * * mobility; the Laplacian reflects closed tangential neighbours (no slip) and is off for
+ * two-phase grids.  cells: the FluidCell packets (two-phase: reserved.z liquid fraction,
+ * reserved.w curvature; may be NULL for one phase); em NULL or 16 floats per cell (force at
+ * [8..10]).  u_star must not alias u_pred. */
+int mhdp_viscous_dev(mhdp_ctx *ctx, const float *u_pred_dev, const float *cells_dev, const float *em_dev,
+                     const mhdp_step_params *params, float *u_star_dev);
 const char *mhdp_last_error(const mhdp_ctx *ctx);
 void mhdp_destroy(mhdp_ctx *ctx);
 /* Device bytes held by the context (FluidState.bytes, mod.rs:591). */

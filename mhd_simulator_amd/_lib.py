@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libmhdp.so")
 SYMBOLS = (
     "mhdp_create", "mhdp_set_stream", "mhdp_set_cells", "mhdp_get_mask", "mhdp_project", "mhdp_project_dev",
     "mhdp_set_potential",
-    "mhdp_project_batch", "mhdp_fields_dev", "mhdp_last_error", "mhdp_destroy", "mhdp_device_bytes",
+    "mhdp_project_batch", "mhdp_fields_dev", "mhdp_predict_dev", "mhdp_viscous_dev", "mhdp_last_error", "mhdp_destroy", "mhdp_device_bytes",
     "mhdp_anchors", "mhdp_hierarchy", "mhdp_set_kernel_timing", "mhdp_kernel_time",
     "mhdp_probe_coefficients", "mhdp_probe_apply", "mhdp_probe_vcycle", "mhdp_probe_mg_graph", "mhdp_probe_rhs",
 )
@@ -67,6 +67,11 @@ class Diagnostics(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class StepParams(C.Structure):
+    _fields_ = [("dt", C.c_float), ("viscosity", C.c_float), ("acceleration", C.c_float * 3),
+                ("surface_tension", C.c_float), ("advect", C.c_int)]
+
+
 _lib = None
 
 
@@ -96,6 +101,8 @@ def load(path: str = LIB_PATH):
         "mhdp_project_dev": (I, [P, P, C.POINTER(Opts), P, P, C.POINTER(Stats)]),
         "mhdp_project_batch": (I, [P, I, P, C.POINTER(Opts), P, P, P]),
         "mhdp_fields_dev": (I, [P, P, P, P, C.POINTER(FieldsParams), P, C.POINTER(Diagnostics)]),
+        "mhdp_predict_dev": (I, [P, P, C.POINTER(StepParams), P]),
+        "mhdp_viscous_dev": (I, [P, P, P, P, C.POINTER(StepParams), P]),
         "mhdp_last_error": (C.c_char_p, [P]),
         "mhdp_destroy": (None, [P]),
         "mhdp_device_bytes": (C.c_uint64, [P]),

@@ -3,8 +3,9 @@
     python -m mhd_simulator_amd.build
 
 Every source is compiled with packed-FP32 instructions disabled; the setup kernels
-(face mobilities, RHS, Galerkin graphs, velocity correction) are compiled without
-floating-point contraction so they reproduce the reference's FP32 formulas exactly.
+(face mobilities, RHS, Galerkin graphs, velocity correction, display fields) and the
+momentum step (predict, viscous) are compiled without floating-point contraction so they
+reproduce the reference's FP32 formulas exactly.
 """
 from __future__ import annotations
 
@@ -30,6 +31,7 @@ SOURCES = {
     "brick.hip": [],
     "march.hip": [],
     "cc.hip": [],
+    "step.hip": ["-ffp-contract=off"],
     "api.cpp": ["-x", "hip", "-pthread"],
 }
 

@@ -60,6 +60,9 @@ struct mhdp_ctx {
     bool prev_valid = false;
     float* uwarm = nullptr;      // u* corrected by the previous potential (warm start)
     double* fields_part = nullptr;  // fields(): per-block totals, then the 10 folded totals
+    float* faces_open = nullptr;    // predict(): open-face copy of its input (one float per face)
+    uint8_t* open8 = nullptr;       // step kernels: per cell, bit a = lower face on axis a open
+    bool open8_fresh = false;       // open8 matches the current mask
     void *bx16 = nullptr, *by16 = nullptr, *bz16 = nullptr, *dinv16 = nullptr;
     // multigrid
     std::vector<CoarseArrays> coarse;  // index l-1 for level l
@@ -330,9 +333,9 @@ void vcycle_fused(mhdp_ctx* c, Solve& S, int mode, bool update = false) {
     const int L = (int)S.lv.size();
     const bool brick = c->brick_enabled && brick_ok(c->g);
     c->timer.before(MHDP_K_RESTRICT, s);
-    const bool march = brick && c->march_enabled && march_ok(c->g);
+    const bool march = brick && c->march_enabled && S.store == STORE_BF16 && march_ok(c->g);
     if (march && update)
-        launch_march_restrict_update(s, S.store, lv[1].g, lv[0].g, c->r, c->r2, c->q, dinv, bx, by, bz, lv[1].r,
+        launch_march_restrict_update(s, lv[1].g, lv[0].g, c->r, c->r2, c->q, dinv, bx, by, bz, lv[1].r,
                                      c->part, c->sc);
     else if (brick && update)
         launch_brick_restrict_update(s, S.store, lv[1].g, lv[0].g, c->r, c->r2, c->q, dinv, bx, by, bz, lv[1].r,
@@ -347,7 +350,7 @@ void vcycle_fused(mhdp_ctx* c, Solve& S, int mode, bool update = false) {
     record_mark();
     c->timer.before(MHDP_K_SMOOTH, s);
     if (march)
-        launch_march_smooth(s, S.store, lv[0].g, lv[1].g, c->r, c->r2, dinv, bx, by, bz, z1, lv[0].t, c->part, c->sc,
+        launch_march_smooth(s, lv[0].g, lv[1].g, c->r, c->r2, dinv, bx, by, bz, z1, lv[0].t, c->part, c->sc,
                             mode, update);
     else if (brick)
         launch_brick_smooth(s, S.store, lv[0].g, lv[1].g, c->r, c->r2, dinv, bx, by, bz, z1, lv[0].t, c->part, c->sc,
@@ -1170,6 +1173,7 @@ int mhdp_set_cells(mhdp_ctx* c, const uint8_t* fluid, const float* alpha) {
         for (uint32_t i = 0; i < n; i++)
             if (!std::isfinite(alpha[i])) return fail(c, MHDP_INVALID, "Nonfinite liquid fraction");
     c->cells = false;
+    c->open8_fresh = false;
     int rc;
     if ((rc = dalloc(c, (void**)&c->fluid8, n)) || (rc = dalloc(c, (void**)&c->cc_counts, 4 * sizeof(uint32_t))))
         return rc;
@@ -1273,6 +1277,51 @@ int mhdp_fields_dev(mhdp_ctx* c, const float* u_faces, const double* phi, const 
     for (double v : t)
         if (!std::isfinite(v)) return fail(c, MHDP_NONFINITE, "Nonfinite fluid fields; state restored");
     if (t[7] != 0.0) return fail(c, MHDP_NONFINITE, "Nonfinite fluid fields; state restored");
+    return MHDP_OK;
+}
+
+static int step_check(mhdp_ctx* c, const mhdp_step_params* sp) {
+    if (!c->cells) return fail(c, MHDP_INVALID, "Fluid cells are not set");
+    if (!sp || !(sp->dt > 0.0f) || !std::isfinite(sp->dt) || !std::isfinite(sp->viscosity) || sp->viscosity < 0.0f ||
+        !std::isfinite(sp->surface_tension) || !std::isfinite(sp->acceleration[0]) ||
+        !std::isfinite(sp->acceleration[1]) || !std::isfinite(sp->acceleration[2]))
+        return fail(c, MHDP_INVALID, "Invalid step parameters");
+    if (face_count(c->grid) >= (1ull << 32)) return fail(c, MHDP_INVALID, "Too many faces for the step kernels");
+    if (!c->open8_fresh) {
+        int rc = dalloc(c, (void**)&c->open8, c->g.n);
+        if (rc) return rc;
+        launch_open8(c->stream, c->g, c->mask, c->open8);
+        c->open8_fresh = true;
+    }
+    return MHDP_OK;
+}
+
+int mhdp_predict_dev(mhdp_ctx* c, const float* u, const mhdp_step_params* sp, float* out) {
+    if (!c) return MHDP_INVALID;
+    c->err.clear();
+    int rc = step_check(c, sp);
+    if (rc) return rc;
+    if (!u || !out) return fail(c, MHDP_INVALID, "Missing face velocities");
+    if ((rc = dalloc(c, (void**)&c->faces_open, face_count(c->grid) * sizeof(float)))) return rc;
+    launch_predict(c->stream, c->g, c->grid.h, sp->dt, sp->advect ? 1 : 0, c->open8, u, c->faces_open, out);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return MHDP_OK;
+}
+
+int mhdp_viscous_dev(mhdp_ctx* c, const float* pred, const float* cells, const float* em, const mhdp_step_params* sp,
+                     float* out) {
+    if (!c) return MHDP_INVALID;
+    c->err.clear();
+    int rc = step_check(c, sp);
+    if (rc) return rc;
+    if (!pred || !out) return fail(c, MHDP_INVALID, "Missing face velocities");
+    if (pred == out) return fail(c, MHDP_INVALID, "viscous reads neighbouring faces: output must not alias input");
+    if (c->phase.two_phase && !cells) return fail(c, MHDP_INVALID, "Two-phase viscous step needs the cell packets");
+    launch_viscous(c->stream, c->g, c->grid.h, c->phase.rho_liquid, c->phase.rho_air, c->phase.two_phase ? 1 : 0,
+                   sp->dt, sp->viscosity, sp->acceleration, sp->surface_tension, c->open8, pred, cells, em, out);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(c->stream));
     return MHDP_OK;
 }
 
@@ -1411,7 +1460,7 @@ void mhdp_destroy(mhdp_ctx* c) {
     else (void)hipDeviceSynchronize();
     void* ptrs[] = {c->mask, c->alpha, c->bx, c->by, c->bz, c->diag, c->dinv, c->b, c->x, c->r, c->r2, c->p, c->p2, c->q, c->z,
                     c->t, c->x64, c->b64, c->bx16, c->by16, c->bz16, c->dinv16, c->sc, c->part, c->ustar_stage,
-                    c->uout_stage, c->phi_stage, c->fluid8, c->cc_counts, c->phi_prev, c->uwarm, c->fields_part};
+                    c->uout_stage, c->phi_stage, c->fluid8, c->cc_counts, c->phi_prev, c->uwarm, c->fields_part, c->faces_open, c->open8};
     for (void* p : ptrs) dfree(c, p);
     for (CoarseArrays& a : c->coarse) {
         void* q[] = {a.bx, a.by, a.bz, a.react, a.diag, a.dinv, a.bx16, a.by16, a.bz16, a.react16, a.dinv16,

@@ -525,31 +525,20 @@ static uint32_t march_blocks(const Grid3& g) {
     return grid_blocks(items, cap);
 }
 
-void launch_march_smooth(hipStream_t s, int store, const Grid3& g, const Grid3& cg, const float* r, const float* r2,
+void launch_march_smooth(hipStream_t s, const Grid3& g, const Grid3& cg, const float* r, const float* r2,
                          const void* dinv, const void* bx, const void* by, const void* bz, const void* zc, void* t,
                          double* part, Scal* sc, int mode, bool use_next) {
-    const dim3 B(MHDP_BLOCK), G(march_blocks(g));
-    const int un = use_next ? 1 : 0;
-    if (store == STORE_BF16) {
-        using T = __hip_bfloat16;
-        go<K_k_march_smooth<T, T, 2>>(s, G.x, {g, cg, r, r2, (const T*)dinv, (const T*)bx, (const T*)by, (const T*)bz, (const T*)zc, (T*)t, part, sc, mode, un, march_z()});
-    } else {
-        using T = float;
-        go<K_k_march_smooth<T, T, 2>>(s, G.x, {g, cg, r, r2, (const T*)dinv, (const T*)bx, (const T*)by, (const T*)bz, (const T*)zc, (T*)t, part, sc, mode, un, march_z()});
-    }
+    using T = __hip_bfloat16;
+    go<K_k_march_smooth<T, T, 2>>(s, march_blocks(g), {g, cg, r, r2, (const T*)dinv, (const T*)bx, (const T*)by,
+                                                       (const T*)bz, (const T*)zc, (T*)t, part, sc, mode,
+                                                       use_next ? 1 : 0, march_z()});
 }
 
-void launch_march_restrict_update(hipStream_t s, int store, const Grid3& cg, const Grid3& g, float* r, float* r2,
+void launch_march_restrict_update(hipStream_t s, const Grid3& cg, const Grid3& g, float* r, float* r2,
                                   const float* q, const void* dinv, const void* bx, const void* by, const void* bz,
                                   void* rc, double* part, Scal* sc) {
-    const uint32_t G = march_blocks(g);
-    if (store == STORE_BF16) {
-        using T = __hip_bfloat16;
-        go<K_march_restrict_update<T, T, 2>>(s, G, {cg, g, r, r2, q, (const T*)dinv, (const T*)bx, (const T*)by,
-                                                    (const T*)bz, (T*)rc, part, sc, march_z()});
-    } else {
-        using T = float;
-        go<K_march_restrict_update<T, T, 2>>(s, G, {cg, g, r, r2, q, (const T*)dinv, (const T*)bx, (const T*)by,
-                                                    (const T*)bz, (T*)rc, part, sc, march_z()});
-    }
+    using T = __hip_bfloat16;
+    go<K_march_restrict_update<T, T, 2>>(s, march_blocks(g), {cg, g, r, r2, q, (const T*)dinv, (const T*)bx,
+                                                              (const T*)by, (const T*)bz, (T*)rc, part, sc,
+                                                              march_z()});
 }

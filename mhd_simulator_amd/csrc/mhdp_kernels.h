@@ -139,12 +139,13 @@ void launch_brick_smooth_coarse(hipStream_t s, int store, const Grid3& g, const 
                                 const void* dinv, const void* bx, const void* by, const void* bz, const void* react,
                                 const void* zc, void* t, Scal* sc);
 
-// march.hip — z-marching forms of the fine passes (nx % 4 == 0, even ny and nz).
+// march.hip — z-marching forms of the fine passes (nx % 4 == 0, even ny and nz), BF16 V-cycle
+// storage only (with FP32 storage the three-plane window does not fit the register file).
 bool march_ok(const Grid3& g);
-void launch_march_smooth(hipStream_t s, int store, const Grid3& g, const Grid3& cg, const float* r, const float* r2,
+void launch_march_smooth(hipStream_t s, const Grid3& g, const Grid3& cg, const float* r, const float* r2,
                          const void* dinv, const void* bx, const void* by, const void* bz, const void* zc, void* t,
                          double* part, Scal* sc, int mode, bool use_next);
-void launch_march_restrict_update(hipStream_t s, int store, const Grid3& cg, const Grid3& g, float* r, float* r2,
+void launch_march_restrict_update(hipStream_t s, const Grid3& cg, const Grid3& g, float* r, float* r2,
                                   const float* q, const void* dinv, const void* bx, const void* by, const void* bz,
                                   void* rc, double* part, Scal* sc);
 
@@ -161,3 +162,13 @@ constexpr int kFieldTotals = 10;
 void launch_fields(hipStream_t s, const Grid3& g, float h, float rho_l, float rho_air, int two_phase, float dt,
                    const float acc[3], int mushy, int meanfree, const uint32_t* mask, const float* faces,
                    const double* phi, const float* em, float* cells, double* part, double* totals);
+
+// step.hip — predict and viscous (solver.wgsl:127-163).  open8: one byte per cell, bit a set when
+// the cell's lower face on axis a is open (launch_open8 builds it from the encoded mask); uo:
+// scratch of one float per face for predict's open-face copy.
+void launch_open8(hipStream_t s, const Grid3& g, const uint32_t* mask, uint8_t* open8);
+void launch_predict(hipStream_t s, const Grid3& g, float h, float dt, int advect, const uint8_t* open8,
+                    const float* u, float* uo, float* out);
+void launch_viscous(hipStream_t s, const Grid3& g, float h, float rho_l, float rho_air, int two_phase, float dt,
+                    float viscosity, const float acc[3], float sigma, const uint8_t* open8, const float* pred,
+                    const float* cells, const float* em, float* out);

@@ -1003,7 +1003,8 @@ struct K_fields {
     }
 };
 
-// diagnostics() (solver.wgsl:284-291): fold the per-block partials in block order.
+// diagnostics() (solver.wgsl:284-291): fold the per-block partials - thread t takes records
+// t, t + 256, ... in order, then a fixed tree over the threads (deterministic).
 struct K_fields_total {
     static constexpr int MINB = 0;
     struct Args {
@@ -1012,14 +1013,29 @@ struct K_fields_total {
         double* out;
     };
     static __device__ void run(const Args& A, const VGrid&) {
-        if (threadIdx.x >= kFieldTotals) return;
-        const int k = threadIdx.x;
-        double v = 0.0;
-        for (uint32_t b = 0; b < A.blocks; b++) {
-            const double w = A.part[(size_t)kFieldTotals * b + (uint32_t)k];
-            v = fields_is_max(k) ? fmax(v, w) : v + w;
+        __shared__ double red[kFieldTotals][MHDP_BLOCK];
+        double v[kFieldTotals];
+#pragma unroll
+        for (int k = 0; k < kFieldTotals; k++) v[k] = 0.0;
+        for (uint32_t b = threadIdx.x; b < A.blocks; b += MHDP_BLOCK)
+#pragma unroll
+            for (int k = 0; k < kFieldTotals; k++) {
+                const double w = A.part[(size_t)kFieldTotals * b + (uint32_t)k];
+                v[k] = fields_is_max(k) ? fmax(v[k], w) : v[k] + w;
+            }
+#pragma unroll
+        for (int k = 0; k < kFieldTotals; k++) red[k][threadIdx.x] = v[k];
+        __syncthreads();
+        for (int s = MHDP_BLOCK / 2; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s)
+#pragma unroll
+                for (int k = 0; k < kFieldTotals; k++) {
+                    const double a = red[k][threadIdx.x], b = red[k][threadIdx.x + s];
+                    red[k][threadIdx.x] = fields_is_max(k) ? fmax(a, b) : a + b;
+                }
+            __syncthreads();
         }
-        A.out[k] = v;
+        if (threadIdx.x < kFieldTotals) A.out[threadIdx.x] = red[threadIdx.x][0];
     }
 };
 

@@ -1,0 +1,445 @@
+// The momentum step either side of the projection (solver.wgsl:127-163):
+//   predict  semi-Lagrangian RK2 advection of the face velocities (departure point from the
+//            velocity sampled at the face centre and at the half-step midpoint, trilinear samples)
+//   viscous  explicit face Laplacian with no-slip reflection at closed tangential neighbours,
+//            body + EM (+ capillary, two-phase) acceleration, times the face mobility.
+// Built with -ffp-contract=off: every FP32 expression keeps the reference's operation order, so
+// the results equal the oracle's bit for bit.
+//
+// Faces are the three SoA planes (plane a has dims n + e_a); each kernel is instantiated per
+// plane (the axis is a template argument).  Which faces are open depends only on the mask, so it
+// is kept per cell as one byte (bit a: the cell's lower face on axis a is open), rebuilt when the
+// cells change; an open test is then one byte load instead of two mask loads.  predict first
+// writes the open-face copy of its input (closed faces 0), so each of its 56 samples per face is
+// one load (periodic wrap and range check stay).
+#include <cstdlib>
+
+#include "launch.h"
+#include "mhdp_kernels.h"
+
+namespace {
+
+struct Planes {
+    int n[3];
+    uint32_t per;
+    uint32_t f0, f1;  // sizes of the x and y planes
+};
+
+Planes planes_of(const Grid3& g) {
+    Planes p;
+    p.n[0] = g.nx;
+    p.n[1] = g.ny;
+    p.n[2] = g.nz;
+    p.per = g.per;
+    p.f0 = ((uint32_t)g.nx + 1) * (uint32_t)g.ny * (uint32_t)g.nz;
+    p.f1 = (uint32_t)g.nx * ((uint32_t)g.ny + 1) * (uint32_t)g.nz;
+    return p;
+}
+uint32_t plane_size(const Planes& p, int a) {
+    uint32_t s = 1;
+    for (int b = 0; b < 3; b++) s *= (uint32_t)p.n[b] + (b == a ? 1u : 0u);
+    return s;
+}
+
+// wrap_periodic (src/periodic.wgsl:5-10): unsigned Euclidean wrap of any integer
+__device__ __forceinline__ int wrap_any(int v, int size) {
+    if (v >= 0) return (int)((uint32_t)v % (uint32_t)size);
+    const uint32_t magnitude = (uint32_t)(-(v + 1)) + 1u;
+    const uint32_t rem = magnitude % (uint32_t)size;
+    return rem == 0u ? 0 : (int)((uint32_t)size - rem);
+}
+// the same, without a division (and without branches) for values within one period of the range
+__device__ __forceinline__ int wrap1(int v, int size) {
+    int w = v < 0 ? v + size : v;
+    w = w >= size ? w - size : w;
+    if (w < 0 || w >= size) w = wrap_any(v, size);
+    return w;
+}
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+__device__ __forceinline__ void wrap3(const Planes& P, int q[3]) {
+#pragma unroll
+    for (int b = 0; b < 3; b++)
+        if (P.per & (1u << b)) q[b] = wrap1(q[b], P.n[b]);
+}
+__device__ __forceinline__ bool valid_cell(const Planes& P, const int q[3]) {
+    return q[0] >= 0 && q[1] >= 0 && q[2] >= 0 && q[0] < P.n[0] && q[1] < P.n[1] && q[2] < P.n[2];
+}
+__device__ __forceinline__ uint32_t cell_index(const Planes& P, const int q[3]) {
+    return (uint32_t)q[0] + (uint32_t)P.n[0] * ((uint32_t)q[1] + (uint32_t)P.n[1] * (uint32_t)q[2]);
+}
+template <int A>
+__device__ __forceinline__ uint32_t face_index(const Planes& P, const int q[3]) {
+    const uint32_t px = (uint32_t)P.n[0] + (A == 0), py = (uint32_t)P.n[1] + (A == 1);
+    const uint32_t off = A == 0 ? 0u : (A == 1 ? P.f0 : P.f0 + P.f1);
+    return off + (uint32_t)q[0] + px * ((uint32_t)q[1] + py * (uint32_t)q[2]);
+}
+// open_face(q, A) (solver.wgsl:31) from the open bytes; q is wrapped in place.  A face at the
+// upper end of a closed axis has no cell q and is closed.
+template <int A>
+__device__ __forceinline__ bool open_face(const Planes& P, const uint8_t* __restrict__ open8, int q[3]) {
+    wrap3(P, q);
+    const bool valid = valid_cell(P, q);
+    // the byte is read from a clamped (always valid) cell so that no branch guards the load
+    const int c[3] = {clampi(q[0], 0, P.n[0] - 1), clampi(q[1], 0, P.n[1] - 1), clampi(q[2], 0, P.n[2] - 1)};
+    return valid && ((open8[cell_index(P, c)] >> A) & 1u);
+}
+// Faces of plane A, one wave per row along x (consecutive lanes, consecutive faces): each block
+// takes a contiguous range of rows (y, z) so that stencil neighbours in y and z stay in its
+// caches; f(x, y, z, k) with k the face index within the plane (x + px * row).
+template <int A, class F>
+__device__ __forceinline__ void for_faces(const Planes& P, const VGrid& vg, F&& f) {
+    const uint32_t px = (uint32_t)P.n[0] + (A == 0), py = (uint32_t)P.n[1] + (A == 1);
+    const uint32_t rows = py * ((uint32_t)P.n[2] + (A == 2));
+    constexpr uint32_t kWaves = MHDP_BLOCK / 64;
+    const uint32_t per = (rows + vg.n - 1) / vg.n;
+    const uint32_t r0 = vg.b * per, r1 = min(rows, r0 + per);
+    const uint32_t lane = threadIdx.x & 63;
+    for (uint32_t row = r0 + (threadIdx.x >> 6); row < r1; row += kWaves) {
+        const int y = (int)(row % py), z = (int)(row / py);
+        for (uint32_t x = lane; x < px; x += 64) f((int)x, y, z, x + px * row);
+    }
+}
+// floor()ed sample coordinate -> integer, clamped to +-2^30 first (NaN -> -2^30: fmaxf returns
+// the number); such positions lie far outside any grid
+__device__ __forceinline__ int cell_coord(float v) { return (int)fminf(fmaxf(v, -1073741824.0f), 1073741824.0f); }
+// Trilinear samples of component C from the open-face copy, in two halves so that a face's
+// independent samples have all their loads in flight together: corners_load works out the corner
+// coordinates per axis (periodic: wrapped; closed: out-of-range corners read a clamped face and
+// count as 0 - both formed and selected, no branch on the axis kind) and issues the eight loads;
+// corners_sum adds the corners in the reference's order (z, y, x nested; value * wx * wy * wz).
+struct Corners {
+    float f[8];   // x fastest
+    bool ok[3][2];
+};
+template <int C>
+__device__ __forceinline__ void corners_load(const Planes& P, const float* __restrict__ uo, const int base[3],
+                                             Corners& k) {
+    const uint32_t px = (uint32_t)P.n[0] + (C == 0), py = (uint32_t)P.n[1] + (C == 1);
+    const uint32_t stride[3] = {1u, px, px * py};
+    uint32_t idx = C == 0 ? 0u : (C == 1 ? P.f0 : P.f0 + P.f1);  // index of the (0,0,0) corner
+    uint32_t step[3];                                              // corner 1 minus corner 0
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+        const bool per = (P.per >> b) & 1u;
+        const int n = P.n[b], lim = n + (b == C ? 1 : 0);
+        const int w0 = wrap1(base[b], n), w1 = w0 + 1 == n ? 0 : w0 + 1;
+        const int c0 = per ? w0 : clampi(base[b], 0, lim - 1), c1 = per ? w1 : clampi(base[b] + 1, 0, lim - 1);
+        k.ok[b][0] = per || (base[b] >= 0 && base[b] < lim);
+        k.ok[b][1] = per || (base[b] + 1 >= 0 && base[b] + 1 < lim);
+        idx += (uint32_t)c0 * stride[b];
+        step[b] = (uint32_t)(c1 - c0) * stride[b];
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+        k.f[j] = uo[idx + ((j & 1) ? step[0] : 0u) + ((j & 2) ? step[1] : 0u) + ((j & 4) ? step[2] : 0u)];
+}
+// pins the loaded values in registers: the loads stay issued ahead of any arithmetic
+__device__ __forceinline__ void corners_fence(Corners& k) {
+#pragma unroll
+    for (int j = 0; j < 8; j++) asm volatile("" : "+v"(k.f[j]));
+}
+__device__ __forceinline__ float corners_sum(const Corners& k, const float frac[3]) {
+    float value = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const int x = j & 1, y = (j >> 1) & 1, z = j >> 2;
+        const float wx = x ? frac[0] : 1.0f - frac[0];
+        const float wy = y ? frac[1] : 1.0f - frac[1];
+        const float wz = z ? frac[2] : 1.0f - frac[2];
+        value += (k.ok[0][x] && k.ok[1][y] && k.ok[2][z] ? k.f[j] : 0.0f) * wx * wy * wz;
+    }
+    return value;
+}
+// sample_component (solver.wgsl:109-116): base cell and fractions of an arbitrary position
+template <int C>
+__device__ __forceinline__ void sample_coords(const float position[3], int base[3], float frac[3]) {
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+        const float q = position[b] - (b == C ? 0.0f : 0.5f);
+        const float fl = floorf(q);
+        base[b] = cell_coord(fl);
+        frac[b] = q - fl;
+    }
+}
+// ... and of the centre of face pos of plane A, where every step is exact and known: on axis b
+// the sample coordinate is pos + 1/2 - 1/2 [b == A] - 1/2 [b != C], so the fraction is 1/2 when
+// exactly one of b == A, b == C holds (else 0) and the base is pos - [b == A and b != C]
+template <int C, int A>
+__device__ __forceinline__ void centre_coords(const int pos[3], int base[3], float frac[3]) {
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+        base[b] = pos[b] - ((b == A && b != C) ? 1 : 0);
+        frac[b] = (b == A) != (b == C) ? 0.5f : 0.0f;
+    }
+}
+
+// open bytes: bit a of cell i set when i and its lower neighbour on axis a (wrapped) are fluid
+__global__ void __launch_bounds__(MHDP_BLOCK) k_open8(Planes P, uint32_t n, const uint32_t* __restrict__ mask,
+                                                      uint8_t* __restrict__ open8) {
+    uint32_t lo, hi;
+    block_range(n, lo, hi);
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+        uint32_t bits = 0;
+        if (mask[i]) {
+            const uint32_t nxy = (uint32_t)P.n[0] * (uint32_t)P.n[1];
+            int pos[3];
+            pos[2] = (int)(i / nxy);
+            const uint32_t r = i - (uint32_t)pos[2] * nxy;
+            pos[1] = (int)(r / (uint32_t)P.n[0]);
+            pos[0] = (int)(r - (uint32_t)pos[1] * (uint32_t)P.n[0]);
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                int l[3] = {pos[0], pos[1], pos[2]};
+                l[a] -= 1;
+                wrap3(P, l);
+                if (valid_cell(P, l) && mask[cell_index(P, l)] != 0u) bits |= 1u << a;
+            }
+        }
+        open8[i] = (uint8_t)bits;
+    }
+}
+
+template <int A>
+struct K_open_faces {
+    static constexpr int MINB = 0;
+    struct Args {
+        Planes P;
+        const uint8_t* open8;
+        const float* u;
+        float* uo;
+    };
+    static __device__ void run(const Args& a, const VGrid& vg) {
+        const Planes P = a.P;
+        const uint32_t off = A == 0 ? 0u : (A == 1 ? P.f0 : P.f0 + P.f1);
+        const uint8_t* __restrict__ open8 = a.open8;
+        const float* __restrict__ u = a.u;
+        float* __restrict__ uo = a.uo;
+        for_faces<A>(P, vg, [&](int x, int y, int z, uint32_t k) {
+            int q[3] = {x, y, z};
+            float v = 0.0f;
+            if (open_face<A>(P, open8, q)) v = u[face_index<A>(P, q)];
+            uo[off + k] = v;
+        });
+    }
+};
+
+template <int A>
+struct K_predict {
+    static constexpr int MINB = 0;
+    struct Args {
+        Planes P;
+        float s_half, s_full;  // 0.5 dt / h, dt / h
+        const uint8_t* open8;
+        const float* uo;
+        float* out;
+    };
+    static __device__ void run(const Args& a, const VGrid& vg) {
+        const Planes P = a.P;
+        const uint32_t off = A == 0 ? 0u : (A == 1 ? P.f0 : P.f0 + P.f1);
+        const uint8_t* __restrict__ open8 = a.open8;
+        const float* __restrict__ uo = a.uo;
+        float* __restrict__ out = a.out;
+        for_faces<A>(P, vg, [&](int x, int y, int z, uint32_t k) {
+            const int pos[3] = {x, y, z};
+            int q[3] = {x, y, z};
+            float result = 0.0f;
+            if (open_face<A>(P, open8, q)) {
+                {
+                    int b0[3], b1[3], b2[3];
+                    float r0[3], r1[3], r2[3], v[3], at[3], mid[3], dep[3];
+                    Corners k0, k1, k2;
+                    // velocity at the face centre (three components, 24 loads in flight)
+                    centre_coords<0, A>(pos, b0, r0);
+                    centre_coords<1, A>(pos, b1, r1);
+                    centre_coords<2, A>(pos, b2, r2);
+                    corners_load<0>(P, uo, b0, k0);
+                    corners_load<1>(P, uo, b1, k1);
+                    corners_load<2>(P, uo, b2, k2);
+                    corners_fence(k0);
+                    corners_fence(k1);
+                    corners_fence(k2);
+                    v[0] = corners_sum(k0, r0);
+                    v[1] = corners_sum(k1, r1);
+                    v[2] = corners_sum(k2, r2);
+#pragma unroll
+                    for (int b = 0; b < 3; b++) at[b] = (float)pos[b] + 0.5f;
+                    at[A] -= 0.5f;
+#pragma unroll
+                    for (int b = 0; b < 3; b++) mid[b] = at[b] - v[b] * a.s_half;
+                    // velocity at the half-step midpoint
+                    sample_coords<0>(mid, b0, r0);
+                    sample_coords<1>(mid, b1, r1);
+                    sample_coords<2>(mid, b2, r2);
+                    corners_load<0>(P, uo, b0, k0);
+                    corners_load<1>(P, uo, b1, k1);
+                    corners_load<2>(P, uo, b2, k2);
+                    corners_fence(k0);
+                    corners_fence(k1);
+                    corners_fence(k2);
+                    v[0] = corners_sum(k0, r0);
+                    v[1] = corners_sum(k1, r1);
+                    v[2] = corners_sum(k2, r2);
+#pragma unroll
+                    for (int b = 0; b < 3; b++) dep[b] = at[b] - v[b] * a.s_full;
+                    // component A at the departure point
+                    sample_coords<A>(dep, b0, r0);
+                    corners_load<A>(P, uo, b0, k0);
+                    corners_fence(k0);
+                    result = corners_sum(k0, r0);
+                }
+            }
+            out[off + k] = result;
+        });
+    }
+};
+
+template <int A>
+struct K_viscous {
+    static constexpr int MINB = 0;
+    struct Args {
+        Planes P;
+        float dt, nu_dt_h2, h, rho_l, rho_air, sigma, acc;
+        int two_phase;
+        const uint8_t* open8;
+        const float* pred;
+        const float* cells;  // two-phase: reserved.z liquid fraction, reserved.w curvature
+        const float* em;     // NULL or 16 floats per cell, force at 8..10
+        float* out;
+    };
+    // the wrapped cell of q clamped into the grid (for loads that are used only when q is valid)
+    static __device__ __forceinline__ uint32_t clamped_cell(const Planes& P, int q[3], bool& valid) {
+        wrap3(P, q);
+        valid = valid_cell(P, q);
+        const int c[3] = {clampi(q[0], 0, P.n[0] - 1), clampi(q[1], 0, P.n[1] - 1), clampi(q[2], 0, P.n[2] - 1)};
+        q[0] = c[0];
+        q[1] = c[1];
+        q[2] = c[2];
+        return cell_index(P, c);
+    }
+    static __device__ void run(const Args& a, const VGrid& vg) {
+        const Planes P = a.P;
+        const uint32_t off = A == 0 ? 0u : (A == 1 ? P.f0 : P.f0 + P.f1);
+        const uint8_t* __restrict__ open8 = a.open8;
+        const float* __restrict__ pred = a.pred;
+        const float* __restrict__ cells = a.cells;
+        const float* __restrict__ em = a.em;
+        float* __restrict__ out = a.out;
+        const int nj = a.two_phase ? 1 : 7;  // two-phase grids have no viscous term
+        for_faces<A>(P, vg, [&](int x, int y, int z, uint32_t k) {
+            // the face (j = 0) and its neighbours j = 1 + 2 d + (sign > 0); every load below is
+            // issued from an always-valid index before any value is used
+            bool valid[7];
+            uint32_t o8[7];
+            float pv[7];
+#pragma unroll
+            for (int j = 0; j < 7; j++) {
+                if (j >= nj) break;
+                int q[3] = {x, y, z};
+                if (j > 0) q[(j - 1) >> 1] += ((j - 1) & 1) ? 1 : -1;
+                const uint32_t c = clamped_cell(P, q, valid[j]);
+                o8[j] = open8[c];
+                pv[j] = pred[face_index<A>(P, q)];
+            }
+            int l[3] = {x, y, z}, r[3] = {x, y, z};
+            l[A] -= 1;
+            bool vl, vr;
+            const uint32_t cl = clamped_cell(P, l, vl), cr = clamped_cell(P, r, vr);
+            float eml = 0.0f, emr = 0.0f, al = 0.0f, ar = 0.0f, kl = 0.0f, kr = 0.0f;
+            if (em) {
+                eml = em[16u * (size_t)cl + 8u + (uint32_t)A];
+                emr = em[16u * (size_t)cr + 8u + (uint32_t)A];
+            }
+            if (a.two_phase) {
+                al = cells[16u * (size_t)cl + 14u];
+                kl = cells[16u * (size_t)cl + 15u];
+                ar = cells[16u * (size_t)cr + 14u];
+                kr = cells[16u * (size_t)cr + 15u];
+            }
+#pragma unroll
+            for (int j = 0; j < 7; j++) {
+                if (j >= nj) break;
+                asm volatile("" : "+v"(o8[j]), "+v"(pv[j]));
+            }
+            asm volatile("" : "+v"(eml), "+v"(emr), "+v"(al), "+v"(ar), "+v"(kl), "+v"(kr));
+            float result = 0.0f;
+            if (valid[0] && ((o8[0] >> A) & 1u)) {  // open_face (solver.wgsl:31)
+                const float center = pv[0];
+                float laplace = 0.0f;
+                if (!a.two_phase) {
+#pragma unroll
+                    for (int j = 1; j < 7; j++) {
+                        const bool open = valid[j] && ((o8[j] >> A) & 1u);
+                        float nb = open ? pv[j] : 0.0f;
+                        if (!open && ((j - 1) >> 1) != A) nb = -center;  // no-slip reflection
+                        laplace += nb - center;
+                    }
+                }
+                float rho = a.rho_l, cap = 0.0f, beta = 1.0f;
+                if (a.two_phase) {
+                    // density, face_density (solver.wgsl:41-42), capillary_force (solver.wgsl:52-57)
+                    // and mobility (solver.wgsl:43-46) of an open face: both cells exist
+                    const float dl = a.rho_air * (1.0f - al) + a.rho_l * al;
+                    const float dr = a.rho_air * (1.0f - ar) + a.rho_l * ar;
+                    rho = 0.5f * (dl + dr);
+                    const float wl = al * (1.0f - al), wr = ar * (1.0f - ar);
+                    const float kk = wl + wr > 1e-6f ? (wl * kl + wr * kr) / fmaxf(wl + wr, 1e-20f) : 0.5f * (kl + kr);
+                    cap = a.sigma * kk * (ar - al) / a.h / rho;
+                    beta = a.rho_l / rho;
+                    beta *= rho / a.rho_l;
+                }
+                // em_force (solver.wgsl:108): zero without an EM field
+                const float acc = a.acc + ((vl ? eml : 0.0f) + (vr ? emr : 0.0f)) / (2.0f * rho) + cap;
+                result = (center + a.nu_dt_h2 * laplace + a.dt * acc) * beta;
+            }
+            out[off + k] = result;
+        });
+    }
+};
+
+// Block count of the step kernels (measured at 512^3): enough resident waves to overlap the
+// dependent load chains (open byte -> face value -> samples), fewer where a kernel reads the
+// 64-byte cell packets and a block's rows must share the cache; MHDP_STEP_CAP overrides it.
+uint32_t step_blocks(uint32_t faces, uint32_t cap) {
+    static const long env = [] {
+        const char* e = std::getenv("MHDP_STEP_CAP");
+        return e ? std::strtol(e, nullptr, 10) : 0L;
+    }();
+    if (env >= 8 && env <= 65536) cap = (uint32_t)env;
+    return grid_blocks(faces, cap);
+}
+
+}  // namespace
+
+void launch_open8(hipStream_t s, const Grid3& g, const uint32_t* mask, uint8_t* open8) {
+    hipLaunchKernelGGL(k_open8, dim3(grid_blocks(g.n)), dim3(MHDP_BLOCK), 0, s, planes_of(g), g.n, mask, open8);
+}
+
+void launch_predict(hipStream_t s, const Grid3& g, float h, float dt, int advect, const uint8_t* open8,
+                    const float* u, float* uo, float* out) {
+    const Planes P = planes_of(g);
+    // without advection the result is the open-face copy itself (it may overwrite u: a face and
+    // the periodic duplicate that reads it have the same open state and value)
+    float* copy = advect ? uo : out;
+    go<K_open_faces<0>>(s, step_blocks(plane_size(P, 0), 2048), {P, open8, u, copy});
+    go<K_open_faces<1>>(s, step_blocks(plane_size(P, 1), 2048), {P, open8, u, copy});
+    go<K_open_faces<2>>(s, step_blocks(plane_size(P, 2), 2048), {P, open8, u, copy});
+    if (!advect) return;
+    const float s_half = 0.5f * dt / h, s_full = dt / h;
+    go<K_predict<0>>(s, step_blocks(plane_size(P, 0), 8192), {P, s_half, s_full, open8, uo, out});
+    go<K_predict<1>>(s, step_blocks(plane_size(P, 1), 8192), {P, s_half, s_full, open8, uo, out});
+    go<K_predict<2>>(s, step_blocks(plane_size(P, 2), 8192), {P, s_half, s_full, open8, uo, out});
+}
+
+void launch_viscous(hipStream_t s, const Grid3& g, float h, float rho_l, float rho_air, int two_phase, float dt,
+                    float viscosity, const float acc[3], float sigma, const uint8_t* open8, const float* pred,
+                    const float* cells, const float* em, float* out) {
+    const Planes P = planes_of(g);
+    const float k = viscosity * dt / (h * h);
+    const uint32_t cap = two_phase || em ? 1024 : 8192;
+    go<K_viscous<0>>(s, step_blocks(plane_size(P, 0), cap),
+                     {P, dt, k, h, rho_l, rho_air, sigma, acc[0], two_phase, open8, pred, cells, em, out});
+    go<K_viscous<1>>(s, step_blocks(plane_size(P, 1), cap),
+                     {P, dt, k, h, rho_l, rho_air, sigma, acc[1], two_phase, open8, pred, cells, em, out});
+    go<K_viscous<2>>(s, step_blocks(plane_size(P, 2), cap),
+                     {P, dt, k, h, rho_l, rho_air, sigma, acc[2], two_phase, open8, pred, cells, em, out});
+}

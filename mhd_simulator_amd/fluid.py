@@ -196,6 +196,29 @@ class PressureProjection:
         self.stats = st.as_dict()
         return int(st.iterations), float(st.rel_residual if precision == "fp32" else st.rel_residual_f64)
 
+    # -- the momentum step that produces u* (predict + viscous, solver.wgsl:127-163)
+    def _faces_dev(self, *ts):
+        for t in ts:
+            if not hasattr(t, "data_ptr") or t.numel() != self.grid.faces:
+                raise ValueError("the step takes device tensors holding the three face planes")
+
+    def predict(self, u, u_out, *, dt, advect=True):
+        """u_out = u advected along itself (semi-Lagrangian RK2); closed faces 0."""
+        self._faces_dev(u, u_out)
+        sp = _lib.StepParams(dt, 0.0, (C.c_float * 3)(0.0, 0.0, 0.0), 0.0, int(bool(advect)))
+        self._check(self.lib.mhdp_predict_dev(self.handle, _ptr(u), C.byref(sp), _ptr(u_out)))
+
+    def viscous(self, u_pred, u_out, *, dt, viscosity=0.0, acceleration=(0.0, 0.0, 0.0), surface_tension=0.0,
+                cells=None, em=None):
+        """u_out = explicit viscous update + body/EM/capillary acceleration, times the mobility."""
+        self._faces_dev(u_pred, u_out)
+        for t in (cells, em):
+            if t is not None and t.numel() != 16 * self.grid.cells:
+                raise ValueError("cells / em hold 16 floats per cell")
+        sp = _lib.StepParams(dt, viscosity, (C.c_float * 3)(*acceleration), surface_tension, 0)
+        self._check(self.lib.mhdp_viscous_dev(self.handle, _ptr(u_pred), _ptr(cells), _ptr(em), C.byref(sp),
+                                              _ptr(u_out)))
+
     # -- fields() + diagnostics() (solver.wgsl:256-291, read_diagnostics mod.rs:1116-1146)
     def fields(self, u_faces, phi, cells, *, dt, acceleration=(0.0, 0.0, 0.0), mushy=False, gauge="auto",
                em=None):

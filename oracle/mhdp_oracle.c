@@ -385,6 +385,145 @@ void orc_fields(const orc_grid *g, const uint32_t *mask, const float *faces, con
     }
 }
 
+/* Float -> integer cell coordinate for the samplers: the (integral) value clamped to +-2^30 first,
+ * NaN giving -2^30 (fmaxf returns the number), so that base + 1 cannot overflow; such positions lie
+ * far outside any grid.  (The reference converts with WGSL's f32 -> i32, whose result for values
+ * out of range is not pinned by any of its tests.) */
+static inline int32_t cell_coord(float v) { return (int32_t)fminf(fmaxf(v, -1073741824.0f), 1073741824.0f); }
+/* sample_component (solver.wgsl:109-116): trilinear interpolation of face component `axis` */
+static float sample_component(const ctx_t *c, const float *faces, const float position[3], int axis) {
+    int32_t base[3];
+    float frac[3];
+    for (int b = 0; b < 3; b++) {
+        const float q = position[b] - (b == axis ? 0.0f : 0.5f);
+        const float fl = floorf(q);
+        base[b] = cell_coord(fl);
+        frac[b] = q - fl; /* fract() */
+    }
+    float value = 0.0f;
+    for (int z = 0; z < 2; z++)
+        for (int y = 0; y < 2; y++)
+            for (int x = 0; x < 2; x++) {
+                const int o[3] = {x, y, z};
+                float w[3];
+                for (int b = 0; b < 3; b++) w[b] = o[b] > 0 ? frac[b] : 1.0f - frac[b];
+                int32_t q[3] = {base[0] + x, base[1] + y, base[2] + z};
+                value += face_value(c, faces, q, axis) * w[0] * w[1] * w[2];
+            }
+    return value;
+}
+static void sample_velocity(const ctx_t *c, const float *faces, const float position[3], float v[3]) {
+    for (int a = 0; a < 3; a++) v[a] = sample_component(c, faces, position, a);
+}
+
+void orc_predict(const orc_grid *g, const uint32_t *mask, const float *faces, const orc_step_params *p, float *out) {
+    ctx_t c;
+    ctx_init(&c, g, mask, NULL, 0);
+    const float s_half = 0.5f * p->dt / g->h, s_full = p->dt / g->h;
+    for (int axis = 0; axis < 3; axis++) {
+        uint32_t P[3];
+        plane_dims(c.d, axis, P);
+        const uint64_t off = plane_offset(c.d, axis), np = plane_size(c.d, axis);
+#pragma omp parallel for schedule(static)
+        for (uint64_t k = 0; k < np; k++) {
+            int32_t pos[3];
+            decode3(k, P, pos);
+            float result = 0.0f;
+            if (open_face(&c, pos, axis)) {
+                result = face_value(&c, faces, pos, axis);
+                if (p->advect) {
+                    float at[3], v[3], mid[3], dep[3];
+                    for (int b = 0; b < 3; b++) at[b] = (float)pos[b] + 0.5f;
+                    at[axis] -= 0.5f;
+                    sample_velocity(&c, faces, at, v);
+                    for (int b = 0; b < 3; b++) mid[b] = at[b] - v[b] * s_half;
+                    sample_velocity(&c, faces, mid, v);
+                    for (int b = 0; b < 3; b++) dep[b] = at[b] - v[b] * s_full;
+                    result = sample_component(&c, faces, dep, axis);
+                }
+            }
+            out[off + k] = result;
+        }
+    }
+}
+
+/* em_force (solver.wgsl:108) */
+static inline float em_component(const ctx_t *c, const float *em, const int32_t pos[3], int axis) {
+    int32_t q[3] = {pos[0], pos[1], pos[2]};
+    wrap_pos(c->periodic, c->d, q);
+    if (!em || !valid_pos(c->d, q)) return 0.0f;
+    return em[16 * idx3(c->d, q) + 8 + axis];
+}
+/* reserved.w of a cell (the interface curvature) */
+static inline float curvature(const ctx_t *c, const float *cells, const int32_t pos[3]) {
+    int32_t q[3] = {pos[0], pos[1], pos[2]};
+    wrap_pos(c->periodic, c->d, q);
+    return cells[16 * idx3(c->d, q) + 15];
+}
+/* capillary_force (solver.wgsl:52-57) */
+static float capillary_force(const ctx_t *c, const float *cells, float sigma, const int32_t pos[3], int axis) {
+    if (!c->two_phase || !open_face(c, pos, axis)) return 0.0f;
+    int32_t left[3] = {pos[0], pos[1], pos[2]};
+    left[axis] -= 1;
+    const float al = occupancy(c, left), ar = occupancy(c, pos);
+    const float kl = curvature(c, cells, left), kr = curvature(c, cells, pos);
+    const float wl = al * (1.0f - al), wr = ar * (1.0f - ar);
+    const float k = wl + wr > 1e-6f ? (wl * kl + wr * kr) / fmaxf(wl + wr, 1e-20f) : 0.5f * (kl + kr);
+    return sigma * k * (ar - al) / c->h;
+}
+
+void orc_viscous(const orc_grid *g, const uint32_t *mask, const float *pred, const float *cells, const float *em,
+                 const orc_step_params *p, float *out) {
+    /* occupancy() reads reserved.z of the packet (solver.wgsl:40) */
+    float *alpha = NULL;
+    if (g->two_phase) {
+        alpha = (float *)malloc(((size_t)g->nx * g->ny * g->nz + 1) * sizeof(float));
+        for (uint64_t i = 0; i < (uint64_t)g->nx * g->ny * g->nz; i++) alpha[i] = cells[16 * i + 14];
+    }
+    ctx_t c;
+    ctx_init(&c, g, mask, alpha, 0);
+    const float h = g->h, nu_dt_h2 = p->viscosity * p->dt / (h * h);
+    for (int axis = 0; axis < 3; axis++) {
+        uint32_t P[3];
+        plane_dims(c.d, axis, P);
+        const uint64_t off = plane_offset(c.d, axis), np = plane_size(c.d, axis);
+#pragma omp parallel for schedule(static)
+        for (uint64_t k = 0; k < np; k++) {
+            int32_t pos[3];
+            decode3(k, P, pos);
+            float result = 0.0f;
+            if (open_face(&c, pos, axis)) {
+                const float center = face_value(&c, pred, pos, axis);
+                float laplace = 0.0f;
+                if (!c.two_phase)
+                    for (int d = 0; d < 3; d++)
+                        for (int sign = -1; sign <= 1; sign += 2) {
+                            int32_t q[3] = {pos[0], pos[1], pos[2]};
+                            q[d] += sign;
+                            float nb = face_value(&c, pred, q, axis);
+                            if (!open_face(&c, q, axis) && d != axis) nb = -center; /* no-slip wall */
+                            laplace += nb - center;
+                        }
+                int32_t left[3] = {pos[0], pos[1], pos[2]};
+                left[axis] -= 1;
+                float rho = c.rho_l, capillary = 0.0f;
+                if (c.two_phase) {
+                    rho = face_density(&c, pos, axis);
+                    capillary = capillary_force(&c, cells, p->surface_tension, pos, axis) / rho;
+                }
+                const float acc = p->acceleration[axis] +
+                                  (em_component(&c, em, left, axis) + em_component(&c, em, pos, axis)) / (2.0f * rho) +
+                                  capillary;
+                float beta = mobility(&c, pos, axis);
+                if (c.two_phase) beta *= rho / c.rho_l;
+                result = (center + nu_dt_h2 * laplace + p->dt * acc) * beta;
+            }
+            out[off + k] = result;
+        }
+    }
+    free(alpha);
+}
+
 int orc_project(const orc_grid *g, const uint32_t *mask, const float *alpha, const float *ustar,
                 const orc_opts *o, double *phi_out, float *u_out, orc_stats *st) {
 #ifdef _OPENMP

@@ -124,6 +124,22 @@ typedef struct {
 void orc_fields(const orc_grid *g, const uint32_t *mask, const float *faces, const double *phi, const float *em,
                 const orc_fields_params *p, float *cells, double *totals);
 
+/* ---- momentum step either side of the projection: predict + viscous (solver.wgsl:127-163) ---- */
+typedef struct {
+    float dt;              /* Params.physical.y */
+    float viscosity;       /* kinematic viscosity (Params.physical.z) */
+    float acceleration[3]; /* Params.acceleration.xyz */
+    float surface_tension; /* Params.surface.z (two-phase capillary force) */
+    int advect;            /* Params.flags.y != 0: semi-Lagrangian RK2 advection */
+} orc_step_params;
+/* predict (solver.wgsl:127-140): faces (three planes) -> out (three planes); closed faces 0. */
+void orc_predict(const orc_grid *g, const uint32_t *mask, const float *faces, const orc_step_params *p, float *out);
+/* viscous (solver.wgsl:143-163), mushy factor off: pred (three planes) -> out; cells: 16 floats per
+ * cell (reserved.z liquid fraction, reserved.w curvature; read only for two-phase grids); em: 16
+ * floats per cell (force at 8..10) or NULL. */
+void orc_viscous(const orc_grid *g, const uint32_t *mask, const float *pred, const float *cells, const float *em,
+                 const orc_step_params *p, float *out);
+
 /* ---- multigrid probes (mirror the reference's test-only entry points) ---- */
 typedef struct orc_mg orc_mg;
 orc_mg *orc_mg_new(const orc_grid *g, const uint32_t *mask, const float *alpha, int meanfree, int precision);

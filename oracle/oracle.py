@@ -188,6 +188,37 @@ def fields(g, mask, faces, phi, cells, dt=1e-3, acceleration=(0.0, 0.0, -9.81), 
     return out.reshape(g.cells, 16), totals
 
 
+class _StepParams(C.Structure):
+    _fields_ = [("dt", C.c_float), ("viscosity", C.c_float), ("acceleration", C.c_float * 3),
+                ("surface_tension", C.c_float), ("advect", C.c_int)]
+
+
+def _step_params(dt, viscosity, acceleration, surface_tension, advect):
+    return _StepParams(dt, viscosity, (C.c_float * 3)(*acceleration), surface_tension, int(bool(advect)))
+
+
+def predict(g, mask, faces, dt=1e-3, advect=True):
+    """predict (solver.wgsl:127-140): semi-Lagrangian RK2 advection of the face velocities."""
+    out = np.zeros(g.faces, dtype=np.float32)
+    prm = _step_params(dt, 0.0, (0.0, 0.0, 0.0), 0.0, advect)
+    lib().orc_predict(C.byref(_grid(g)), _ptr(mask), _ptr(np.ascontiguousarray(faces, dtype=np.float32)),
+                      C.byref(prm), _ptr(out))
+    return out
+
+
+def viscous(g, mask, pred, cells=None, dt=1e-3, viscosity=0.0, acceleration=(0.0, 0.0, -9.81), surface_tension=0.0,
+            em=None):
+    """viscous (solver.wgsl:143-163): explicit diffusion, body/EM/capillary acceleration, mobility."""
+    out = np.zeros(g.faces, dtype=np.float32)
+    prm = _step_params(dt, viscosity, acceleration, surface_tension, False)
+    cl = None if cells is None else np.ascontiguousarray(cells, dtype=np.float32).reshape(-1)
+    lib().orc_viscous(C.byref(_grid(g)), _ptr(mask), _ptr(np.ascontiguousarray(pred, dtype=np.float32)),
+                      None if cl is None else _ptr(cl),
+                      None if em is None else _ptr(np.ascontiguousarray(em, dtype=np.float32).reshape(-1)),
+                      C.byref(prm), _ptr(out))
+    return out
+
+
 def project(g, mask, alpha, ustar, rel_tol=1e-5, abs_div_tol=0.0, max_iter=1000, precond=0, gauge=0,
             precision=32, tol64=0.0, threads=0):
     """FluidState::project(false) (src/fluid/mod.rs:1015-1115) -> (status, phi f64, u f32, stats dict)."""

@@ -261,9 +261,10 @@ def test_device_labelling_matches_oracle(orc, dims, frac, seed, periodic):
 
 @pytest.mark.parametrize("name", ["walls-pinned", "periodic-even", "config2-32", "droplets-32", "plate-64"])
 def test_marching_smoother_matches_bricks(orc, name, monkeypatch):
-    """The z-marching post-smoother (used on large grids) against the 4x2x2 brick smoother on
-    the same problems: the same FP32 V-cycle to rounding, and the mixed projection through it
-    meets the 1e-8 bar against the oracle's operator and solution."""
+    """The z-marching fine passes (used on large grids, BF16 V-cycle storage) against the 4x2x2
+    brick passes on the same problems: the same BF16 V-cycle up to the odd flipped BF16 rounding,
+    and the mixed projection through them meets the 1e-8 bar against the oracle's operator and
+    solution."""
     p = problem(name)
     with projection(p) as pr:
         meanfree = pr.components == 1
@@ -271,10 +272,10 @@ def test_marching_smoother_matches_bricks(orc, name, monkeypatch):
         i = np.arange(p.grid.cells)
         r = np.where(active, np.cos(i * 0.17), 0).astype(np.float32)
         monkeypatch.setenv("MHDP_MARCH_MIN", "1000000000")
-        zb = pr.vcycle(r, meanfree)
+        zb = pr.vcycle(r, meanfree, precision="mixed")
         monkeypatch.setenv("MHDP_MARCH_MIN", "0")
-        zm = pr.vcycle(r, meanfree)
-        assert relerr(zm, zb) < 1e-6
+        zm = pr.vcycle(r, meanfree, precision="mixed")
+        assert relerr(zm, zb) < 2e-3
         pr.project(p.ustar, precision="mixed", precond="mg", refine_tol=1e-8)
         assert orc.true_residual(p.grid, pr.mask, p.alpha, p.ustar, pr.phi) <= 1e-8
         phi_ref, u_ref = oracle_solution(orc, p, pr.mask)
