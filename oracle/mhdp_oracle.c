@@ -275,6 +275,186 @@ static double now_seconds(void) {
     return ts.tv_sec + 1e-9 * ts.tv_nsec;
 }
 
+/* ------------------------------------------------------------------ EM edge lattice
+ * src/em/common.wgsl, src/em/grid.rs: edges of axis a have dims n + 1 except n_a on axis a,
+ * packed x | y | z; faces of axis a have dims n + e_a. */
+enum { EM_BOUNDARY = 1, EM_LOWER_AIR = 2, EM_UPPER_AIR = 4, EM_SCALAR_AIR = 8 };
+typedef struct {
+    uint32_t n[3], E[3], Etot, F[3];
+    float h, dt, mu0;
+    int magnetostatic;
+    const float *sigma, *diag, *nu;
+    const uint8_t *flags;
+} em_ctx;
+static void em_ctx_init(em_ctx *c, const orc_em_grid *g, const orc_em_params *prm, const float *sigma,
+                        const float *diag, const uint8_t *flags, const float *nu) {
+    c->n[0] = g->nx; c->n[1] = g->ny; c->n[2] = g->nz;
+    for (int a = 0; a < 3; a++) {
+        uint32_t e = 1, f = 1;
+        for (int b = 0; b < 3; b++) {
+            e *= c->n[b] + (b == a ? 0u : 1u);
+            f *= c->n[b] + (b == a ? 1u : 0u);
+        }
+        c->E[a] = e;
+        c->F[a] = f;
+    }
+    c->Etot = c->E[0] + c->E[1] + c->E[2];
+    c->h = g->h;
+    c->dt = prm ? prm->dt : 1.0f;
+    c->mu0 = prm ? prm->mu0 : 0.0f;
+    c->magnetostatic = prm ? prm->magnetostatic : 1;
+    c->sigma = sigma; c->diag = diag; c->flags = flags; c->nu = nu;
+}
+static inline void em_edge_dims(const em_ctx *c, int axis, uint32_t d[3]) {
+    for (int b = 0; b < 3; b++) d[b] = c->n[b] + (b == axis ? 0u : 1u);
+}
+static inline int em_edge_valid(const em_ctx *c, int axis, const int32_t p[3]) {
+    uint32_t d[3];
+    em_edge_dims(c, axis, d);
+    return p[0] >= 0 && p[1] >= 0 && p[2] >= 0 && (uint32_t)p[0] < d[0] && (uint32_t)p[1] < d[1] && (uint32_t)p[2] < d[2];
+}
+static inline uint32_t em_edge_index(const em_ctx *c, int axis, const int32_t p[3]) {
+    uint32_t d[3];
+    em_edge_dims(c, axis, d);
+    uint32_t off = 0;
+    for (int a = 0; a < axis; a++) off += c->E[a];
+    return off + (uint32_t)p[0] + d[0] * ((uint32_t)p[1] + d[1] * (uint32_t)p[2]);
+}
+static inline void em_edge_decode(const em_ctx *c, uint32_t i, int *axis, int32_t p[3]) {
+    int a = 0;
+    while (a < 2 && i >= c->E[a]) i -= c->E[a++];
+    uint32_t d[3];
+    em_edge_dims(c, a, d);
+    p[0] = (int32_t)(i % d[0]);
+    p[1] = (int32_t)((i / d[0]) % d[1]);
+    p[2] = (int32_t)(i / (d[0] * d[1]));
+    *axis = a;
+}
+static inline int em_face_valid(const em_ctx *c, int axis, const int32_t p[3]) {
+    return p[0] >= 0 && p[1] >= 0 && p[2] >= 0 && (uint32_t)p[0] < c->n[0] + (axis == 0) &&
+           (uint32_t)p[1] < c->n[1] + (axis == 1) && (uint32_t)p[2] < c->n[2] + (axis == 2);
+}
+static inline uint32_t em_face_index(const em_ctx *c, int axis, const int32_t p[3]) {
+    uint32_t off = 0;
+    for (int a = 0; a < axis; a++) off += c->F[a];
+    const uint32_t px = c->n[0] + (axis == 0), py = c->n[1] + (axis == 1);
+    return off + (uint32_t)p[0] + px * ((uint32_t)p[1] + py * (uint32_t)p[2]);
+}
+/* boundary (common.wgsl:9): an edge on the outer surface (tangential A is held) */
+static inline int em_boundary(const em_ctx *c, int axis, const int32_t p[3]) {
+    const int a = (axis + 1) % 3, b = (axis + 2) % 3;
+    return p[a] == 0 || (uint32_t)p[a] == c->n[a] || p[b] == 0 || (uint32_t)p[b] == c->n[b];
+}
+/* cell_properties (induction.wgsl:16-25): (conductivity, relative permeability) of a cell;
+ * props[2 k] / props[2 k + 1] for material id k; partially filled cells (bit 31) blend with id 0 */
+static inline void em_cell(const em_ctx *c, const uint32_t *materials, const float *props, int vacuum,
+                           const int32_t p[3], float out[2]) {
+    if (vacuum || p[0] < 0 || p[1] < 0 || p[2] < 0 || (uint32_t)p[0] >= c->n[0] || (uint32_t)p[1] >= c->n[1] ||
+        (uint32_t)p[2] >= c->n[2]) {
+        out[0] = 0.0f;
+        out[1] = 1.0f;
+        return;
+    }
+    const uint32_t id = materials[(uint32_t)p[0] + c->n[0] * ((uint32_t)p[1] + c->n[1] * (uint32_t)p[2])];
+    if (id == 0u) {
+        out[0] = props[0];
+        out[1] = props[1];
+        return;
+    }
+    if (id & 0x80000000u) {
+        const float alpha = (float)((id >> 8) & 0x7fffffu) / 8388607.0f;
+        const uint32_t k = id & 255u;
+        out[0] = alpha * props[2 * k];
+        out[1] = props[1] * (1.0f - alpha) + props[2 * k + 1] * alpha; /* mix(mu_0, mu_k, alpha) */
+        return;
+    }
+    out[0] = props[2 * id];
+    out[1] = props[2 * id + 1];
+}
+/* nu_face (induction.wgsl:26-30) */
+static float em_nu_face(const em_ctx *c, const uint32_t *materials, const float *props, int vacuum, int axis,
+                        const int32_t p[3]) {
+    if (!em_face_valid(c, axis, p)) return 0.0f;
+    int32_t q[3] = {p[0], p[1], p[2]};
+    q[axis] -= 1;
+    float a[2], b[2];
+    em_cell(c, materials, props, vacuum, p, a);
+    em_cell(c, materials, props, vacuum, q, b);
+    return 0.5f / a[1] + 0.5f / b[1];
+}
+/* air_node (induction.wgsl:58-63) */
+static int em_air_node(const em_ctx *c, const uint32_t *materials, const float *props, int vacuum, const int32_t p[3]) {
+    for (int a = 0; a < 3; a++)
+        if (p[a] <= 0 || (uint32_t)p[a] >= c->n[a]) return 0;
+    for (int x = 0; x < 2; x++)
+        for (int y = 0; y < 2; y++)
+            for (int z = 0; z < 2; z++) {
+                const int32_t q[3] = {p[0] - x, p[1] - y, p[2] - z};
+                float v[2];
+                em_cell(c, materials, props, vacuum, q, v);
+                if (v[0] > 0.0f) return 0;
+            }
+    return 1;
+}
+/* coefficients + coefficient_check (induction.wgsl:82-104) */
+void orc_em_coefficients(const orc_em_grid *g, const uint32_t *materials, const float *props, int vacuum,
+                         float *sigma, float *diag, uint8_t *flags, float *nu, float check[4]) {
+    em_ctx c;
+    em_ctx_init(&c, g, NULL, NULL, NULL, NULL, NULL);
+    for (int a = 0; a < 3; a++) {
+        uint32_t off = 0;
+        for (int b = 0; b < a; b++) off += c.F[b];
+        const uint32_t px = c.n[0] + (a == 0), py = c.n[1] + (a == 1);
+        for (uint32_t k = 0; k < c.F[a]; k++) {
+            const int32_t p[3] = {(int32_t)(k % px), (int32_t)((k / px) % py), (int32_t)(k / (px * py))};
+            nu[off + k] = em_nu_face(&c, materials, props, vacuum, a, p);
+        }
+    }
+    check[0] = 0.0f; check[1] = 1e30f; check[2] = 0.0f; check[3] = 0.0f;
+    for (uint32_t i = 0; i < c.Etot; i++) {
+        int axis;
+        int32_t p[3];
+        em_edge_decode(&c, i, &axis, p);
+        const int a = (axis + 1) % 3, b = (axis + 2) % 3;
+        float s = 0.0f;
+        for (int u = 0; u < 2; u++)
+            for (int v = 0; v < 2; v++) {
+                int32_t q[3] = {p[0], p[1], p[2]};
+                q[a] -= u;
+                q[b] -= v;
+                float cp[2];
+                em_cell(&c, materials, props, vacuum, q, cp);
+                s += 0.25f * cp[0];
+            }
+        int32_t qa[3] = {p[0], p[1], p[2]}, qb[3] = {p[0], p[1], p[2]};
+        qa[a] -= 1;
+        qb[b] -= 1;
+        const float n1 = em_nu_face(&c, materials, props, vacuum, b, p), n2 = em_nu_face(&c, materials, props, vacuum, b, qa);
+        const float n3 = em_nu_face(&c, materials, props, vacuum, a, p), n4 = em_nu_face(&c, materials, props, vacuum, a, qb);
+        float d = n1 + n2 + n3 + n4;
+        uint8_t fl = 0;
+        if (em_boundary(&c, axis, p)) {
+            d = 1.0f;
+            fl |= EM_BOUNDARY;
+            if (s > 0.0f) check[0] += 1.0f;
+        }
+        int32_t upper[3] = {p[0], p[1], p[2]};
+        upper[axis] += 1;
+        const int la = em_air_node(&c, materials, props, vacuum, p), ua = em_air_node(&c, materials, props, vacuum, upper);
+        if (la) fl |= EM_LOWER_AIR;
+        if (ua) fl |= EM_UPPER_AIR;
+        if (la && ua && n1 == 1.0f && n2 == 1.0f && n3 == 1.0f && n4 == 1.0f) fl |= EM_SCALAR_AIR;
+        sigma[i] = s;
+        diag[i] = d + (la ? 1.0f : 0.0f) + (ua ? 1.0f : 0.0f);
+        flags[i] = fl;
+        if (s > 0.0f) {
+            check[1] = fminf(check[1], s);
+            check[3] += 1.0f;
+        }
+        check[2] = fmaxf(check[2], s);
+    }
+}
+
 #define R float
 #define F(name) name##_f32
 #define RC(x) x##f
@@ -290,6 +470,17 @@ static double now_seconds(void) {
 #undef R
 #undef F
 #undef RC
+
+#define R float
+#define F(name) name##_f32
+#include "em.inc"
+#undef R
+#undef F
+#define R double
+#define F(name) name##_f64
+#include "em.inc"
+#undef R
+#undef F
 
 double orc_true_residual(const orc_grid *g, const uint32_t *mask, const float *alpha,
                          const float *ustar, const double *phi) {

@@ -140,6 +140,46 @@ void orc_predict(const orc_grid *g, const uint32_t *mask, const float *faces, co
 void orc_viscous(const orc_grid *g, const uint32_t *mask, const float *pred, const float *cells, const float *em,
                  const orc_step_params *p, float *out);
 
+/* ---- EM Jacobi-CG on the edge curl-curl operator (src/em/induction.wgsl, src/em/mod.rs:863-1010) ---- */
+typedef struct {
+    uint32_t nx, ny, nz; /* cells; edges of axis a: dims n + 1 except n_a on axis a, packed x|y|z */
+    float h;             /* cell size (Params.physical.z) */
+} orc_em_grid;
+typedef struct {
+    float dt;          /* Params.physical.x */
+    float mu0;         /* Params.physical.y */
+    float tol2;        /* relative tolerance squared, Params.physical.w */
+    uint32_t max_iter; /* EmConfig.max_iterations */
+    int magnetostatic; /* Params.control.y == 1 (initial solve): no mass term */
+} orc_em_params;
+typedef struct {
+    uint32_t iterations;
+    double rel_residual; /* sqrt(rr / bb) of the last reduction */
+    double scale;        /* power-of-two residual scale picked by initial_reduce */
+} orc_em_stats;
+/* coefficients (induction.wgsl:82-99): materials one u32 per cell, props[2k], props[2k+1] = (sigma,
+ * mu_r) of material k; vacuum = Params.control.z == 1.  Out per edge: sigma (mean of the four
+ * cells), diag (sum of the four face reluctivities, 1 on boundary edges, + 1 per air endpoint),
+ * flags (1 boundary, 2 lower air node, 4 upper air node, 8 unit-reluctivity air stencil); per face
+ * (three planes, dims n + e_a): nu; check = (boundary conductor edges, min sigma > 0, max sigma,
+ * conducting edges). */
+void orc_em_coefficients(const orc_em_grid *g, const uint32_t *materials, const float *props, int vacuum,
+                         float *sigma, float *diag, uint8_t *flags, float *nu, float check[4]);
+/* apply_operator (induction.wgsl:160-170) */
+void orc_em_apply_f32(const orc_em_grid *g, const orc_em_params *p, const float *sigma, const float *diag,
+                      const uint8_t *flags, const float *nu, const float *v, float *q);
+void orc_em_apply_f64(const orc_em_grid *g, const orc_em_params *p, const float *sigma, const float *diag,
+                      const uint8_t *flags, const float *nu, const double *v, double *q);
+/* One step's solve from a zero increment: as_now / as_prev the applied potential (source.x/.y), mot
+ * the motional source (source[i + E].x, NULL = 0), a_prev the accepted potential; a_out = a_prev +
+ * increment.  Returns 0 converged, 1 not converged, 3 non-finite. */
+int orc_em_solve_f32(const orc_em_grid *g, const orc_em_params *p, const float *sigma, const float *diag,
+                     const uint8_t *flags, const float *nu, const float *as_now, const float *as_prev,
+                     const float *mot, const float *a_prev, float *a_out, orc_em_stats *st);
+int orc_em_solve_f64(const orc_em_grid *g, const orc_em_params *p, const float *sigma, const float *diag,
+                     const uint8_t *flags, const float *nu, const float *as_now, const float *as_prev,
+                     const float *mot, const float *a_prev, double *a_out, orc_em_stats *st);
+
 /* ---- multigrid probes (mirror the reference's test-only entry points) ---- */
 typedef struct orc_mg orc_mg;
 orc_mg *orc_mg_new(const orc_grid *g, const uint32_t *mask, const float *alpha, int meanfree, int precision);

@@ -219,6 +219,79 @@ def viscous(g, mask, pred, cells=None, dt=1e-3, viscosity=0.0, acceleration=(0.0
     return out
 
 
+class _EmGrid(C.Structure):
+    _fields_ = [("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32), ("h", C.c_float)]
+
+
+class _EmParams(C.Structure):
+    _fields_ = [("dt", C.c_float), ("mu0", C.c_float), ("tol2", C.c_float), ("max_iter", C.c_uint32),
+                ("magnetostatic", C.c_int)]
+
+
+class _EmStats(C.Structure):
+    _fields_ = [("iterations", C.c_uint32), ("rel_residual", C.c_double), ("scale", C.c_double)]
+
+
+def em_counts(dims):
+    """edges per axis (dims + 1 except n_a on axis a) and faces per axis (dims + e_a)"""
+    n = list(dims)
+    edges = [int(np.prod([n[b] + (0 if b == a else 1) for b in range(3)])) for a in range(3)]
+    faces = [int(np.prod([n[b] + (1 if b == a else 0) for b in range(3)])) for a in range(3)]
+    return edges, faces
+
+
+def em_coefficients(dims, h, materials, props, vacuum=False):
+    """coefficients (induction.wgsl:82-99) -> dict(sigma, diag, flags, nu, check)"""
+    edges, faces = em_counts(dims)
+    E, Fn = sum(edges), sum(faces)
+    out = dict(sigma=np.zeros(E, np.float32), diag=np.zeros(E, np.float32), flags=np.zeros(E, np.uint8),
+               nu=np.zeros(Fn, np.float32), check=np.zeros(4, np.float32))
+    g = _EmGrid(dims[0], dims[1], dims[2], h)
+    lib().orc_em_coefficients(C.byref(g), _ptr(np.ascontiguousarray(materials, dtype=np.uint32)),
+                              _ptr(np.ascontiguousarray(props, dtype=np.float32).reshape(-1)), int(bool(vacuum)),
+                              _ptr(out["sigma"]), _ptr(out["diag"]), _ptr(out["flags"]), _ptr(out["nu"]),
+                              _ptr(out["check"]))
+    return out
+
+
+def _em_params(dt, mu0, tol, max_iter, magnetostatic):
+    return _EmParams(dt, mu0, tol * tol, max_iter, int(bool(magnetostatic)))
+
+
+def em_apply(dims, h, co, v, dt=1e-5, mu0=4e-7 * np.pi, magnetostatic=False, precision=32):
+    """apply_operator (induction.wgsl:160-170) on v (one value per edge)"""
+    dt_ = np.float64 if precision == 64 else np.float32
+    v = np.ascontiguousarray(v, dtype=dt_)
+    q = np.zeros_like(v)
+    g = _EmGrid(dims[0], dims[1], dims[2], h)
+    prm = _em_params(dt, mu0, 1e-5, 1, magnetostatic)
+    fn = lib().orc_em_apply_f64 if precision == 64 else lib().orc_em_apply_f32
+    fn(C.byref(g), C.byref(prm), _ptr(co["sigma"]), _ptr(co["diag"]), _ptr(co["flags"]), _ptr(co["nu"]),
+       _ptr(v), _ptr(q))
+    return q
+
+
+def em_solve(dims, h, co, as_now, as_prev, a_prev, mot=None, dt=1e-5, mu0=4e-7 * np.pi, tol=1e-5,
+             max_iter=2000, magnetostatic=False, precision=64):
+    """One step's Jacobi-CG solve (src/em/mod.rs:863-1010) -> (status, a_out, stats dict)"""
+    edges, _ = em_counts(dims)
+    E = sum(edges)
+    dt_ = np.float64 if precision == 64 else np.float32
+    a_out = np.zeros(E, dt_)
+    st = _EmStats()
+    g = _EmGrid(dims[0], dims[1], dims[2], h)
+    prm = _em_params(dt, mu0, tol, max_iter, magnetostatic)
+    f32 = lambda a: _ptr(np.ascontiguousarray(a, dtype=np.float32))  # noqa: E731
+    keep = [np.ascontiguousarray(a, dtype=np.float32) for a in (as_now, as_prev, a_prev)]
+    mot_ = None if mot is None else np.ascontiguousarray(mot, dtype=np.float32)
+    fn = lib().orc_em_solve_f64 if precision == 64 else lib().orc_em_solve_f32
+    rc = fn(C.byref(g), C.byref(prm), _ptr(co["sigma"]), _ptr(co["diag"]), _ptr(co["flags"]), _ptr(co["nu"]),
+            _ptr(keep[0]), _ptr(keep[1]), None if mot_ is None else _ptr(mot_), _ptr(keep[2]), _ptr(a_out),
+            C.byref(st))
+    del f32
+    return rc, a_out, {k: getattr(st, k) for k, _ in _EmStats._fields_}
+
+
 def project(g, mask, alpha, ustar, rel_tol=1e-5, abs_div_tol=0.0, max_iter=1000, precond=0, gauge=0,
             precision=32, tol64=0.0, threads=0):
     """FluidState::project(false) (src/fluid/mod.rs:1015-1115) -> (status, phi f64, u f32, stats dict)."""
