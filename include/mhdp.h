@@ -184,6 +184,49 @@ int mhdp_predict_dev(mhdp_ctx *ctx, const float *u_dev, const mhdp_step_params *
  * [8..10]).  u_star must not alias u_pred. */
 int mhdp_viscous_dev(mhdp_ctx *ctx, const float *u_pred_dev, const float *cells_dev, const float *em_dev,
                      const mhdp_step_params *params, float *u_star_dev);
+/* ---- EM step: Jacobi-CG on the edge curl-curl operator (src/em/induction.wgsl; the solve of
+ * EmState::advance_to, src/em/mod.rs:863-1010; coefficient set-up of EmState::new,
+ * mod.rs:557-571).  Edges of axis a have dims n + 1 except n_a on axis a, packed x|y|z, one f32
+ * each; the unknown is the magnetic vector potential A (tangential A held on the outer surface).
+ * Not included: increment prediction (the solve starts from a zero increment) and the nodal
+ * auxiliary-space preconditioner (plain Jacobi throughout). */
+typedef struct mhdp_em mhdp_em;
+typedef struct {
+    float dt;          /* EmConfig.dt_s (Params.physical.x) */
+    float mu0;         /* vacuum permeability (Params.physical.y, 4e-7 pi) */
+    float rel_tol;     /* EmConfig.relative_tolerance */
+    uint32_t max_iter; /* EmConfig.max_iterations */
+    int magnetostatic; /* Params.control.y = 1 (the initial solve): no sigma/dt mass term */
+} mhdp_em_opts;
+typedef struct {
+    uint32_t iterations;
+    double rel_residual; /* sqrt(rr / bb) of the verified residual */
+    double seconds;
+} mhdp_em_stats;
+/* A context for a grid of cells (periodic must be 0; h = GridSi.cell_m). */
+int mhdp_em_create(mhdp_em **out, const mhdp_grid *grid);
+/* coefficients + coefficient_check (induction.wgsl:82-104): materials one id per cell (host; bit 31
+ * marks a partially filled cell, fraction in bits 8..30, material in bits 0..7), props[2k] and
+ * props[2k+1] the conductivity (S/m) and relative permeability of material k (host, count <= 256),
+ * vacuum = Params.control.z.  sigma_range (may be NULL) receives the conducting-edge sigma range.
+ * A conductor on the outer surface is rejected with the reference's text. */
+int mhdp_em_set_materials(mhdp_em *em, const uint32_t *materials, const float *props, uint32_t count, int vacuum,
+                          float sigma_range[2]);
+/* One step: device pointers, one f32 per edge.  as_now / as_prev the applied (source) potential now
+ * and at the previous step, motional the motional source (NULL = none), a_prev the accepted
+ * potential; a_out = a_prev + the solved increment.  On MHDP_NOT_CONVERGED / MHDP_NONFINITE the
+ * text is the reference's "EM solve did not converge: ..." and a_out is untouched. */
+int mhdp_em_solve_dev(mhdp_em *em, const mhdp_em_opts *opts, const float *as_now_dev, const float *as_prev_dev,
+                      const float *motional_dev, const float *a_prev_dev, float *a_out_dev, mhdp_em_stats *stats);
+/* Test hooks: the per-edge coefficients (sigma, diag, flags: 1 boundary, 2/4 lower/upper air node,
+ * 8 air stencil) and per-face reluctivity, host pointers (any may be NULL); apply_operator on a
+ * host vector. */
+int mhdp_em_get_coefficients(const mhdp_em *em, float *sigma, float *diag, uint8_t *flags, float *nu);
+int mhdp_em_probe_apply(mhdp_em *em, const mhdp_em_opts *opts, const float *p, float *q);
+uint64_t mhdp_em_edges(const mhdp_em *em);
+const char *mhdp_em_last_error(const mhdp_em *em);
+void mhdp_em_destroy(mhdp_em *em);
+
 const char *mhdp_last_error(const mhdp_ctx *ctx);
 void mhdp_destroy(mhdp_ctx *ctx);
 /* Device bytes held by the context (FluidState.bytes, mod.rs:591). */

@@ -15,7 +15,9 @@ LIB_PATH = os.path.join(HERE, "libmhdp.so")
 SYMBOLS = (
     "mhdp_create", "mhdp_set_stream", "mhdp_set_cells", "mhdp_get_mask", "mhdp_project", "mhdp_project_dev",
     "mhdp_set_potential",
-    "mhdp_project_batch", "mhdp_fields_dev", "mhdp_predict_dev", "mhdp_viscous_dev", "mhdp_last_error", "mhdp_destroy", "mhdp_device_bytes",
+    "mhdp_project_batch", "mhdp_fields_dev", "mhdp_predict_dev", "mhdp_viscous_dev",
+    "mhdp_em_create", "mhdp_em_set_materials", "mhdp_em_solve_dev", "mhdp_em_get_coefficients", "mhdp_em_probe_apply",
+    "mhdp_em_edges", "mhdp_em_last_error", "mhdp_em_destroy", "mhdp_last_error", "mhdp_destroy", "mhdp_device_bytes",
     "mhdp_anchors", "mhdp_hierarchy", "mhdp_set_kernel_timing", "mhdp_kernel_time",
     "mhdp_probe_coefficients", "mhdp_probe_apply", "mhdp_probe_vcycle", "mhdp_probe_mg_graph", "mhdp_probe_rhs",
 )
@@ -72,6 +74,18 @@ class StepParams(C.Structure):
                 ("surface_tension", C.c_float), ("advect", C.c_int)]
 
 
+class EmOpts(C.Structure):
+    _fields_ = [("dt", C.c_float), ("mu0", C.c_float), ("rel_tol", C.c_float), ("max_iter", C.c_uint32),
+                ("magnetostatic", C.c_int)]
+
+
+class EmStats(C.Structure):
+    _fields_ = [("iterations", C.c_uint32), ("rel_residual", C.c_double), ("seconds", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -103,6 +117,14 @@ def load(path: str = LIB_PATH):
         "mhdp_fields_dev": (I, [P, P, P, P, C.POINTER(FieldsParams), P, C.POINTER(Diagnostics)]),
         "mhdp_predict_dev": (I, [P, P, C.POINTER(StepParams), P]),
         "mhdp_viscous_dev": (I, [P, P, P, P, C.POINTER(StepParams), P]),
+        "mhdp_em_create": (I, [C.POINTER(P), C.POINTER(Grid)]),
+        "mhdp_em_set_materials": (I, [P, P, P, U32, I, P]),
+        "mhdp_em_solve_dev": (I, [P, C.POINTER(EmOpts), P, P, P, P, P, C.POINTER(EmStats)]),
+        "mhdp_em_get_coefficients": (I, [P, P, P, P, P]),
+        "mhdp_em_probe_apply": (I, [P, C.POINTER(EmOpts), P, P]),
+        "mhdp_em_edges": (C.c_uint64, [P]),
+        "mhdp_em_last_error": (C.c_char_p, [P]),
+        "mhdp_em_destroy": (None, [P]),
         "mhdp_last_error": (C.c_char_p, [P]),
         "mhdp_destroy": (None, [P]),
         "mhdp_device_bytes": (C.c_uint64, [P]),

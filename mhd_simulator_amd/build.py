@@ -32,7 +32,9 @@ SOURCES = {
     "march.hip": [],
     "cc.hip": [],
     "step.hip": ["-ffp-contract=off"],
+    "em.hip": ["-ffp-contract=off"],
     "api.cpp": ["-x", "hip", "-pthread"],
+    "em_api.cpp": ["-x", "hip"],
 }
 
 

@@ -1,0 +1,674 @@
+// The EM step's Jacobi-CG on the edge curl-curl operator (src/em/induction.wgsl; the loop of
+// EmState::advance_to, src/em/mod.rs:863-1010), without increment prediction and without the
+// nodal auxiliary preconditioner (plain Jacobi, as the reference runs below its nodal switch).
+// Built with -ffp-contract=off: the operator, right-hand side and coefficients keep the
+// reference's FP32 operation order and equal the oracle's bit for bit.  Reductions are FP64
+// per-block partials folded in block order by one block (deterministic).
+//
+// Edges of axis a have dims n + 1 except n_a on axis a, packed x | y | z; faces of axis a have
+// dims n + e_a.  Per edge: sigma (mean conductivity of the four cells), diag (the four face
+// reluctivities, 1 on boundary edges, + 1 per air endpoint), flags (1 boundary, 2 lower air node,
+// 4 upper air node, 8 unit-reluctivity air stencil); per face: nu (kept instead of recomputing it
+// from the material table at every product).
+#include "launch.h"
+#include "mhdp_kernels.h"
+
+namespace {
+
+enum : uint32_t { kBoundary = 1, kLowerAir = 2, kUpperAir = 4, kScalarAir = 8 };
+
+struct EmGeo {
+    int n[3];
+    uint32_t E[3], F[3];
+    uint32_t Etot;
+    float h;
+};
+
+__device__ __forceinline__ void edge_dims(const EmGeo& g, int axis, uint32_t d[3]) {
+#pragma unroll
+    for (int b = 0; b < 3; b++) d[b] = (uint32_t)g.n[b] + (b == axis ? 0u : 1u);
+}
+__device__ __forceinline__ bool edge_valid(const EmGeo& g, int axis, const int p[3]) {
+    uint32_t d[3];
+    edge_dims(g, axis, d);
+    return p[0] >= 0 && p[1] >= 0 && p[2] >= 0 && (uint32_t)p[0] < d[0] && (uint32_t)p[1] < d[1] &&
+           (uint32_t)p[2] < d[2];
+}
+__device__ __forceinline__ uint32_t edge_index(const EmGeo& g, int axis, const int p[3]) {
+    uint32_t d[3];
+    edge_dims(g, axis, d);
+    const uint32_t off = axis == 0 ? 0u : (axis == 1 ? g.E[0] : g.E[0] + g.E[1]);
+    return off + (uint32_t)p[0] + d[0] * ((uint32_t)p[1] + d[1] * (uint32_t)p[2]);
+}
+__device__ __forceinline__ int edge_decode(const EmGeo& g, uint32_t i, int p[3]) {
+    int axis = 0;
+    if (i >= g.E[0]) {
+        i -= g.E[0];
+        axis = 1;
+        if (i >= g.E[1]) {
+            i -= g.E[1];
+            axis = 2;
+        }
+    }
+    uint32_t d[3];
+    edge_dims(g, axis, d);
+    p[0] = (int)(i % d[0]);
+    p[1] = (int)((i / d[0]) % d[1]);
+    p[2] = (int)(i / (d[0] * d[1]));
+    return axis;
+}
+__device__ __forceinline__ bool face_valid(const EmGeo& g, int axis, const int p[3]) {
+    return p[0] >= 0 && p[1] >= 0 && p[2] >= 0 && p[0] < g.n[0] + (axis == 0) && p[1] < g.n[1] + (axis == 1) &&
+           p[2] < g.n[2] + (axis == 2);
+}
+__device__ __forceinline__ uint32_t face_index(const EmGeo& g, int axis, const int p[3]) {
+    const uint32_t off = axis == 0 ? 0u : (axis == 1 ? g.F[0] : g.F[0] + g.F[1]);
+    const uint32_t px = (uint32_t)g.n[0] + (axis == 0), py = (uint32_t)g.n[1] + (axis == 1);
+    return off + (uint32_t)p[0] + px * ((uint32_t)p[1] + py * (uint32_t)p[2]);
+}
+__device__ __forceinline__ bool boundary(const EmGeo& g, int axis, const int p[3]) {
+    const int a = (axis + 1) % 3, b = (axis + 2) % 3;
+    return p[a] == 0 || p[a] == g.n[a] || p[b] == 0 || p[b] == g.n[b];
+}
+
+// a_edge (induction.wgsl:33-38)
+__device__ __forceinline__ float edge_val(const EmGeo& g, const float* __restrict__ v, int axis, const int p[3]) {
+    return edge_valid(g, axis, p) ? v[edge_index(g, axis, p)] : 0.0f;
+}
+// curl_face (induction.wgsl:40-44)
+__device__ __forceinline__ float curl(const EmGeo& g, const float* __restrict__ v, int axis, const int p[3]) {
+    if (!face_valid(g, axis, p)) return 0.0f;
+    const int a = (axis + 1) % 3, b = (axis + 2) % 3;
+    int qa[3] = {p[0], p[1], p[2]}, qb[3] = {p[0], p[1], p[2]};
+    qa[a] += 1;
+    qb[b] += 1;
+    return edge_val(g, v, b, qa) - edge_val(g, v, b, p) - edge_val(g, v, a, qb) + edge_val(g, v, a, p);
+}
+// face_term (induction.wgsl:45-49)
+__device__ __forceinline__ float face_term(const EmGeo& g, const float* __restrict__ nu, const float* __restrict__ v,
+                                           int axis, const int p[3], bool contrast) {
+    if (!face_valid(g, axis, p)) return 0.0f;
+    float n = nu[face_index(g, axis, p)];
+    if (contrast) n = 1.0f - n;
+    return n * curl(g, v, axis, p);
+}
+// adjoint (induction.wgsl:51-55)
+__device__ __forceinline__ float adjoint(const EmGeo& g, const float* __restrict__ nu, const float* __restrict__ v,
+                                         int axis, const int p[3], bool contrast) {
+    const int a = (axis + 1) % 3, b = (axis + 2) % 3;
+    int qa[3] = {p[0], p[1], p[2]}, qb[3] = {p[0], p[1], p[2]};
+    qa[a] -= 1;
+    qb[b] -= 1;
+    return face_term(g, nu, v, b, p, contrast) - face_term(g, nu, v, b, qa, contrast) -
+           face_term(g, nu, v, a, p, contrast) + face_term(g, nu, v, a, qb, contrast);
+}
+// node_div, gauge (induction.wgsl:64-71)
+__device__ __forceinline__ float node_div(const EmGeo& g, const float* __restrict__ v, const int p[3]) {
+    float value = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        int q[3] = {p[0], p[1], p[2]};
+        q[a] -= 1;
+        value += edge_val(g, v, a, p) - edge_val(g, v, a, q);
+    }
+    return value;
+}
+__device__ __forceinline__ float gauge(const EmGeo& g, uint32_t fl, const float* __restrict__ v, int axis,
+                                       const int p[3]) {
+    int upper[3] = {p[0], p[1], p[2]};
+    upper[axis] += 1;
+    float value = 0.0f;
+    if (fl & kLowerAir) value += node_div(g, v, p);
+    if (fl & kUpperAir) value -= node_div(g, v, upper);
+    return value;
+}
+using Phys = EmPhys;
+// mass, diagonal (induction.wgsl:72-73)
+__device__ __forceinline__ float mass(const EmGeo& g, const Phys& ph, float sigma) {
+    if (ph.magnetostatic) return 0.0f;
+    return ph.mu0 * sigma * g.h * g.h / ph.dt;
+}
+__device__ __forceinline__ float diagonal(const EmGeo& g, const Phys& ph, float diag, float sigma) {
+    const float d = diag + mass(g, ph, sigma);
+    return d > 1e-30f ? d : 1e-30f;
+}
+// air_laplacian (induction.wgsl:150-158)
+__device__ __forceinline__ float air_laplacian(const EmGeo& g, const float* __restrict__ v, uint32_t i, int axis) {
+    uint32_t d[3];
+    edge_dims(g, axis, d);
+    const uint32_t zs = d[0] * d[1];
+    const float center = v[i];
+    float result = 0.0f;
+    result += (center - v[i - 1u]) + (center - v[i + 1u]);
+    result += (center - v[i - d[0]]) + (center - v[i + d[0]]);
+    result += (center - v[i - zs]) + (center - v[i + zs]);
+    return result;
+}
+
+// cell_properties (induction.wgsl:16-25): (sigma, mu_r); props[2k], props[2k+1] for material k
+__device__ __forceinline__ float2 cell_props(const EmGeo& g, const uint32_t* __restrict__ materials,
+                                             const float* __restrict__ props, int vacuum, const int p[3]) {
+    if (vacuum || p[0] < 0 || p[1] < 0 || p[2] < 0 || p[0] >= g.n[0] || p[1] >= g.n[1] || p[2] >= g.n[2])
+        return make_float2(0.0f, 1.0f);
+    const uint32_t id = materials[(uint32_t)p[0] + (uint32_t)g.n[0] * ((uint32_t)p[1] + (uint32_t)g.n[1] * (uint32_t)p[2])];
+    if (id == 0u) return make_float2(props[0], props[1]);
+    if (id & 0x80000000u) {
+        const float alpha = (float)((id >> 8) & 0x7fffffu) / 8388607.0f;
+        const uint32_t k = id & 255u;
+        return make_float2(alpha * props[2 * k], props[1] * (1.0f - alpha) + props[2 * k + 1] * alpha);
+    }
+    return make_float2(props[2 * id], props[2 * id + 1]);
+}
+// nu_face (induction.wgsl:26-30)
+__device__ __forceinline__ float nu_face(const EmGeo& g, const uint32_t* __restrict__ materials,
+                                         const float* __restrict__ props, int vacuum, int axis, const int p[3]) {
+    if (!face_valid(g, axis, p)) return 0.0f;
+    int q[3] = {p[0], p[1], p[2]};
+    q[axis] -= 1;
+    return 0.5f / cell_props(g, materials, props, vacuum, p).y + 0.5f / cell_props(g, materials, props, vacuum, q).y;
+}
+// air_node (induction.wgsl:58-63)
+__device__ __forceinline__ bool air_node(const EmGeo& g, const uint32_t* __restrict__ materials,
+                                         const float* __restrict__ props, int vacuum, const int p[3]) {
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+        if (p[a] <= 0 || p[a] >= g.n[a]) return false;
+    for (int x = 0; x < 2; x++)
+        for (int y = 0; y < 2; y++)
+            for (int z = 0; z < 2; z++) {
+                const int q[3] = {p[0] - x, p[1] - y, p[2] - z};
+                if (cell_props(g, materials, props, vacuum, q).x > 0.0f) return false;
+            }
+    return true;
+}
+
+// one block's four FP64 partials, in a fixed order (wave shuffles, then the waves)
+template <int OP1, int OP2>  // 0 sum, 1 min, 2 max for components 1 and 2 (0 and 3 always sum)
+__device__ void block_partials(double v[4], double* __restrict__ part, uint32_t b) {
+    __shared__ double red[MHDP_BLOCK / 64][4];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int op = k == 1 ? OP1 : (k == 2 ? OP2 : 0);
+        double x = v[k];
+        for (int o = 32; o > 0; o >>= 1) {
+            const double w = __shfl_down(x, o, 64);
+            x = op == 0 ? x + w : (op == 1 ? fmin(x, w) : fmax(x, w));
+        }
+        if (lane == 0) red[wid][k] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const int k = threadIdx.x;
+        const int op = k == 1 ? OP1 : (k == 2 ? OP2 : 0);
+        double x = red[0][k];
+        for (int w = 1; w < MHDP_BLOCK / 64; w++) x = op == 0 ? x + red[w][k] : (op == 1 ? fmin(x, red[w][k]) : fmax(x, red[w][k]));
+        part[4u * b + (uint32_t)k] = x;
+    }
+}
+
+struct K_em_nu {
+    static constexpr int MINB = 0;
+    struct Args {
+        EmGeo g;
+        const uint32_t* materials;
+        const float* props;
+        int vacuum;
+        float* nu;
+    };
+    static __device__ void run(const Args& a, const VGrid& vg) {
+        const uint32_t total = a.g.F[0] + a.g.F[1] + a.g.F[2];
+        uint32_t lo, hi;
+        block_range(vg, total, lo, hi);
+        for (uint32_t k = lo + threadIdx.x; k < hi; k += MHDP_BLOCK) {
+            uint32_t j = k;
+            int axis = 0;
+            if (j >= a.g.F[0]) {
+                j -= a.g.F[0];
+                axis = 1;
+                if (j >= a.g.F[1]) {
+                    j -= a.g.F[1];
+                    axis = 2;
+                }
+            }
+            const uint32_t px = (uint32_t)a.g.n[0] + (axis == 0), py = (uint32_t)a.g.n[1] + (axis == 1);
+            const int p[3] = {(int)(j % px), (int)((j / px) % py), (int)(j / (px * py))};
+            a.nu[k] = nu_face(a.g, a.materials, a.props, a.vacuum, axis, p);
+        }
+    }
+};
+
+// coefficients (induction.wgsl:82-99); partials: boundary conductors, min sigma > 0, max sigma,
+// conducting edges (coefficient_check)
+struct K_em_coef {
+    static constexpr int MINB = 0;
+    struct Args {
+        EmGeo g;
+        const uint32_t* materials;
+        const float* props;
+        int vacuum;
+        float* sigma;
+        float* diag;
+        uint8_t* flags;
+        double* part;
+    };
+    static __device__ void run(const Args& a, const VGrid& vg) {
+        const EmGeo g = a.g;
+        uint32_t lo, hi;
+        block_range(vg, g.Etot, lo, hi);
+        double v[4] = {0.0, 1e30, 0.0, 0.0};
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+            int p[3];
+            const int axis = edge_decode(g, i, p);
+            const int ea = (axis + 1) % 3, eb = (axis + 2) % 3;
+            float s = 0.0f;
+            for (int u = 0; u < 2; u++)
+                for (int w = 0; w < 2; w++) {
+                    int q[3] = {p[0], p[1], p[2]};
+                    q[ea] -= u;
+                    q[eb] -= w;
+                    s += 0.25f * cell_props(g, a.materials, a.props, a.vacuum, q).x;
+                }
+            int qa[3] = {p[0], p[1], p[2]}, qb[3] = {p[0], p[1], p[2]};
+            qa[ea] -= 1;
+            qb[eb] -= 1;
+            const float n1 = nu_face(g, a.materials, a.props, a.vacuum, eb, p);
+            const float n2 = nu_face(g, a.materials, a.props, a.vacuum, eb, qa);
+            const float n3 = nu_face(g, a.materials, a.props, a.vacuum, ea, p);
+            const float n4 = nu_face(g, a.materials, a.props, a.vacuum, ea, qb);
+            float d = n1 + n2 + n3 + n4;
+            uint32_t fl = 0;
+            if (boundary(g, axis, p)) {
+                d = 1.0f;
+                fl |= kBoundary;
+                if (s > 0.0f) v[0] += 1.0;
+            }
+            int upper[3] = {p[0], p[1], p[2]};
+            upper[axis] += 1;
+            const bool la = air_node(g, a.materials, a.props, a.vacuum, p);
+            const bool ua = air_node(g, a.materials, a.props, a.vacuum, upper);
+            if (la) fl |= kLowerAir;
+            if (ua) fl |= kUpperAir;
+            if (la && ua && n1 == 1.0f && n2 == 1.0f && n3 == 1.0f && n4 == 1.0f) fl |= kScalarAir;
+            a.sigma[i] = s;
+            a.diag[i] = d + (la ? 1.0f : 0.0f) + (ua ? 1.0f : 0.0f);
+            a.flags[i] = (uint8_t)fl;
+            if (s > 0.0f) {
+                v[1] = fmin(v[1], (double)s);
+                v[3] += 1.0;
+            }
+            v[2] = fmax(v[2], (double)s);
+        }
+        block_partials<1, 2>(v, a.part, vg.b);
+    }
+};
+
+using Vecs = EmVecs;
+
+// the step's right-hand side at edge i (prepare, induction.wgsl:121-124; with the increment x
+// as well: true_residual, induction.wgsl:236-238)
+__device__ __forceinline__ float em_rhs(const EmGeo& g, const Phys& ph, const Vecs& v, uint32_t i, int axis,
+                                        const int p[3], bool with_x) {
+    const uint32_t fl = v.flags[i];
+    const float m = mass(g, ph, v.sigma[i]);
+    const float s2 = v.mot ? v.mot[i] : 0.0f;
+    const float held = with_x ? v.x[i] + (v.as_now[i] - v.as_prev[i]) : v.as_now[i] - v.as_prev[i];
+    float b = m * ph.dt * s2 - m * held + adjoint(g, v.nu, v.as_now, axis, p, true) -
+              adjoint(g, v.nu, v.a_prev, axis, p, false) - gauge(g, fl, v.a_prev, axis, p);
+    if (with_x) b = b - adjoint(g, v.nu, v.x, axis, p, false) - gauge(g, fl, v.x, axis, p);
+    if (fl & kBoundary) b = 0.0f;
+    return b;
+}
+
+// prepare (induction.wgsl:116-129) from a zero increment; partials (rz, rr, bb, |b| before scaling)
+struct K_em_prepare {
+    static constexpr int MINB = 0;
+    struct Args {
+        EmGeo g;
+        Phys ph;
+        Vecs v;
+        EmScal* sc;
+        double* part;
+    };
+    static __device__ void run(const Args& a, const VGrid& vg) {
+        const EmGeo g = a.g;
+        const float scale = (float)a.sc->scale;
+        uint32_t lo, hi;
+        block_range(vg, g.Etot, lo, hi);
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+            int p[3];
+            const int axis = edge_decode(g, i, p);
+            float b = em_rhs(g, a.ph, a.v, i, axis, p, false);
+            acc[3] += fabs((double)b);
+            b = b / scale;
+            const float z = b / diagonal(g, a.ph, a.v.diag[i], a.v.sigma[i]);
+            a.v.x[i] = 0.0f;
+            a.v.r[i] = b;
+            a.v.p[i] = z;
+            acc[0] += (double)b * (double)z;
+            acc[1] += (double)b * (double)b;
+            acc[2] += (double)b * (double)b;
+        }
+        block_partials<0, 0>(acc, a.part, vg.b);
+    }
+};
+
+// apply_operator (induction.wgsl:160-170) of one edge
+__device__ __forceinline__ float apply_edge(const EmGeo& g, const Phys& ph, const float* __restrict__ nu,
+                                            const uint8_t* __restrict__ flags, const float* __restrict__ sigma,
+                                            const float* __restrict__ v, uint32_t i) {
+    int p[3];
+    const int axis = edge_decode(g, i, p);
+    const uint32_t fl = flags[i];
+    float q;
+    if (fl & kScalarAir) q = air_laplacian(g, v, i, axis);
+    else q = adjoint(g, nu, v, axis, p, false) + gauge(g, fl, v, axis, p) + mass(g, ph, sigma[i]) * v[i];
+    if (fl & kBoundary) q = v[i];
+    return q;
+}
+
+struct K_em_apply {
+    static constexpr int MINB = 0;
+    struct Args {
+        EmGeo g;
+        Phys ph;
+        Vecs v;
+        const EmScal* sc;
+        double* part;
+    };
+    static __device__ void run(const Args& a, const VGrid& vg) {
+        if (a.sc->status != 0) return;
+        uint32_t lo, hi;
+        block_range(vg, a.g.Etot, lo, hi);
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+            const float q = apply_edge(a.g, a.ph, a.v.nu, a.v.flags, a.v.sigma, a.v.p, i);
+            a.v.q[i] = q;
+            acc[0] += (double)a.v.p[i] * (double)q;
+        }
+        block_partials<0, 0>(acc, a.part, vg.b);
+    }
+};
+
+// update_residual (induction.wgsl:177-187): partials (r^2 / d, r^2)
+struct K_em_update {
+    static constexpr int MINB = 0;
+    struct Args {
+        EmGeo g;
+        Phys ph;
+        Vecs v;
+        const EmScal* sc;
+        double* part;
+    };
+    static __device__ void run(const Args& a, const VGrid& vg) {
+        if (a.sc->status != 0) return;
+        const float alpha = (float)a.sc->alpha, scale = (float)a.sc->scale;
+        uint32_t lo, hi;
+        block_range(vg, a.g.Etot, lo, hi);
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+            const float r = a.v.r[i] - alpha * a.v.q[i];
+            a.v.r[i] = r;
+            a.v.x[i] = a.v.x[i] + scale * (alpha * a.v.p[i]);
+            const float d = diagonal(a.g, a.ph, a.v.diag[i], a.v.sigma[i]);
+            acc[0] += (double)r * (double)r / (double)d;
+            acc[1] += (double)r * (double)r;
+        }
+        block_partials<0, 0>(acc, a.part, vg.b);
+    }
+};
+
+// update_search (induction.wgsl:195): p = r / d + beta p
+struct K_em_search {
+    static constexpr int MINB = 0;
+    struct Args {
+        EmGeo g;
+        Phys ph;
+        Vecs v;
+        const EmScal* sc;
+    };
+    static __device__ void run(const Args& a, const VGrid& vg) {
+        if (a.sc->status != 0) return;
+        const float beta = (float)a.sc->beta;
+        uint32_t lo, hi;
+        block_range(vg, a.g.Etot, lo, hi);
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK)
+            a.v.p[i] = a.v.r[i] / diagonal(a.g, a.ph, a.v.diag[i], a.v.sigma[i]) + beta * a.v.p[i];
+    }
+};
+
+// true_residual (induction.wgsl:232-243): restart the search from the actual residual
+struct K_em_true_residual {
+    static constexpr int MINB = 0;
+    struct Args {
+        EmGeo g;
+        Phys ph;
+        Vecs v;
+        const EmScal* sc;
+        double* part;
+    };
+    static __device__ void run(const Args& a, const VGrid& vg) {
+        const float scale = (float)a.sc->scale;
+        uint32_t lo, hi;
+        block_range(vg, a.g.Etot, lo, hi);
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+            int p[3];
+            const int axis = edge_decode(a.g, i, p);
+            const float b = em_rhs(a.g, a.ph, a.v, i, axis, p, true) / scale;
+            const float z = b / diagonal(a.g, a.ph, a.v.diag[i], a.v.sigma[i]);
+            a.v.r[i] = b;
+            a.v.p[i] = z;
+            acc[0] += (double)b * (double)z;
+            acc[1] += (double)b * (double)b;
+        }
+        block_partials<0, 0>(acc, a.part, vg.b);
+    }
+};
+
+// commit_step: the new potential
+struct K_em_commit {
+    static constexpr int MINB = 0;
+    struct Args {
+        uint32_t n;
+        const float* a_prev;
+        const float* x;
+        float* out;
+    };
+    static __device__ void run(const Args& a, const VGrid& vg) {
+        uint32_t lo, hi;
+        block_range(vg, a.n, lo, hi);
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) a.out[i] = a.a_prev[i] + a.x[i];
+    }
+};
+
+enum EmFold { kFoldInitial, kFoldAlpha, kFoldBeta, kFoldRestart, kFoldCheck };
+
+// The single-block reductions (initial_reduce, alpha_reduce, beta_reduce, restart_reduce,
+// coefficient_check; induction.wgsl:100-104, 130-148, 171-176, 189-194, 244-247)
+struct K_em_fold {
+    static constexpr int MINB = 0;
+    struct Args {
+        const double* part;
+        uint32_t blocks;
+        int mode;
+        EmScal* sc;
+    };
+    static __device__ void run(const Args& a, const VGrid&) {
+        __shared__ double red[4][MHDP_BLOCK];
+        const bool check = a.mode == kFoldCheck;
+        double v[4] = {0.0, check ? 1e30 : 0.0, 0.0, 0.0};
+        for (uint32_t b = threadIdx.x; b < a.blocks; b += MHDP_BLOCK) {
+            const double* w = a.part + 4u * b;
+            v[0] += w[0];
+            v[1] = check ? fmin(v[1], w[1]) : v[1] + w[1];
+            v[2] = check ? fmax(v[2], w[2]) : v[2] + w[2];
+            v[3] += w[3];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) red[k][threadIdx.x] = v[k];
+        __syncthreads();
+        for (int s = MHDP_BLOCK / 2; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) {
+                const int t = threadIdx.x;
+                red[0][t] += red[0][t + s];
+                red[1][t] = check ? fmin(red[1][t], red[1][t + s]) : red[1][t] + red[1][t + s];
+                red[2][t] = check ? fmax(red[2][t], red[2][t + s]) : red[2][t] + red[2][t + s];
+                red[3][t] += red[3][t + s];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x != 0) return;
+        EmScal& s = *a.sc;
+        const double t0 = red[0][0], t1 = red[1][0], t2 = red[2][0], t3 = red[3][0];
+        switch (a.mode) {
+        case kFoldInitial: {
+            // initial_reduce: an exact power-of-two scale when |b|_1 is tiny or huge
+            const double scale = s.scale;
+            double next = scale;
+            const float l1 = (float)t3;
+            if (scale == 1.0 && l1 > 0.0f && (l1 < 1e-8f || l1 > 1e8f)) {
+                int ex = (int)((__float_as_uint(l1) >> 23) & 255u) - 127;
+                ex = ex < -100 ? -100 : (ex > 100 ? 100 : ex);
+                next = ldexp(1.0, ex);
+            }
+            double floor_sq = 1e-38;
+            if (scale != 1.0) {
+                const double fn = 1e-19 / scale;
+                floor_sq = fn * fn;
+            }
+            s.rz = t0;
+            s.rr = t1;
+            s.bb = t2;
+            s.l1 = t3;
+            s.threshold = t2 * s.tol2 + floor_sq;
+            s.status = t1 <= s.threshold ? 1 : 0;
+            s.rescale = next != scale ? 1 : 0;
+            s.scale = next;
+            break;
+        }
+        case kFoldAlpha:
+            if (s.status != 0) break;
+            if (!(t0 > 0.0) || fabs(t0) > 3e38) s.status = 2;
+            else s.alpha = s.rz / t0;
+            break;
+        case kFoldBeta:
+            if (s.status != 0) break;
+            s.beta = t0 / s.rz;
+            s.rz = t0;
+            s.rr = t1;
+            s.its += 1;
+            if (t1 <= s.threshold) s.status = 1;
+            if (fabs(t0) > 3e38 || fabs(t1) > 3e38) s.status = 2;
+            break;
+        case kFoldRestart:
+            s.rz = t0;
+            s.rr = t1;
+            s.status = t1 <= s.threshold ? 1 : 0;
+            break;
+        default:
+            s.check[0] = t0;
+            s.check[1] = t1;
+            s.check[2] = t2;
+            s.check[3] = t3;
+            break;
+        }
+    }
+};
+
+EmGeo geo_of(const EmDims& d) {
+    EmGeo g;
+    for (int a = 0; a < 3; a++) g.n[a] = (int)d.n[a];
+    for (int a = 0; a < 3; a++) {
+        uint32_t e = 1, f = 1;
+        for (int b = 0; b < 3; b++) {
+            e *= d.n[b] + (b == a ? 0u : 1u);
+            f *= d.n[b] + (b == a ? 1u : 0u);
+        }
+        g.E[a] = e;
+        g.F[a] = f;
+    }
+    g.Etot = g.E[0] + g.E[1] + g.E[2];
+    g.h = d.h;
+    return g;
+}
+
+}  // namespace
+
+uint32_t em_blocks(const EmDims& d) { return grid_blocks(geo_of(d).Etot); }
+
+void launch_em_coefficients(hipStream_t s, const EmDims& d, const uint32_t* materials, const float* props, int vacuum,
+                            float* sigma, float* diag, uint8_t* flags, float* nu, double* part, void* scal) {
+    const EmGeo g = geo_of(d);
+    const uint32_t faces = g.F[0] + g.F[1] + g.F[2];
+    go<K_em_nu>(s, grid_blocks(faces), {g, materials, props, vacuum, nu});
+    const uint32_t blocks = grid_blocks(g.Etot);
+    go<K_em_coef>(s, blocks, {g, materials, props, vacuum, sigma, diag, flags, part});
+    go<K_em_fold>(s, 1, {part, blocks, kFoldCheck, (EmScal*)scal});
+}
+
+void launch_em_prepare(hipStream_t s, const EmDims& d, const EmPhys& ph, const EmVecs& v, double* part, void* scal) {
+    const EmGeo g = geo_of(d);
+    const uint32_t blocks = grid_blocks(g.Etot);
+    const Phys& p = ph;
+    const Vecs& vv = v;
+    go<K_em_prepare>(s, blocks, {g, p, vv, (EmScal*)scal, part});
+    go<K_em_fold>(s, 1, {part, blocks, kFoldInitial, (EmScal*)scal});
+}
+
+void launch_em_iterations(hipStream_t s, const EmDims& d, const EmPhys& ph, const EmVecs& v, double* part, void* scal,
+                          uint32_t count) {
+    const EmGeo g = geo_of(d);
+    const uint32_t blocks = grid_blocks(g.Etot);
+    const Phys& p = ph;
+    const Vecs& vv = v;
+    EmScal* sc = (EmScal*)scal;
+    for (uint32_t k = 0; k < count; k++) {
+        go<K_em_apply>(s, blocks, {g, p, vv, sc, part});
+        go<K_em_fold>(s, 1, {part, blocks, kFoldAlpha, sc});
+        go<K_em_update>(s, blocks, {g, p, vv, sc, part});
+        go<K_em_fold>(s, 1, {part, blocks, kFoldBeta, sc});
+        go<K_em_search>(s, blocks, {g, p, vv, sc});
+    }
+}
+
+void launch_em_restart(hipStream_t s, const EmDims& d, const EmPhys& ph, const EmVecs& v, double* part, void* scal) {
+    const EmGeo g = geo_of(d);
+    const uint32_t blocks = grid_blocks(g.Etot);
+    const Phys& p = ph;
+    const Vecs& vv = v;
+    go<K_em_true_residual>(s, blocks, {g, p, vv, (EmScal*)scal, part});
+    go<K_em_fold>(s, 1, {part, blocks, kFoldRestart, (EmScal*)scal});
+}
+
+void launch_em_commit(hipStream_t s, const EmDims& d, const float* a_prev, const float* x, float* out) {
+    const EmGeo g = geo_of(d);
+    go<K_em_commit>(s, grid_blocks(g.Etot), {g.Etot, a_prev, x, out});
+}
+
+// apply_operator alone (test hook): q = A p
+struct K_em_probe {
+    static constexpr int MINB = 0;
+    struct Args {
+        EmGeo g;
+        Phys ph;
+        const float* nu;
+        const uint8_t* flags;
+        const float* sigma;
+        const float* p;
+        float* q;
+    };
+    static __device__ void run(const Args& a, const VGrid& vg) {
+        uint32_t lo, hi;
+        block_range(vg, a.g.Etot, lo, hi);
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK)
+            a.q[i] = apply_edge(a.g, a.ph, a.nu, a.flags, a.sigma, a.p, i);
+    }
+};
+
+void launch_em_probe_apply(hipStream_t s, const EmDims& d, const EmPhys& ph, const float* nu, const uint8_t* flags,
+                           const float* sigma, const float* p, float* q) {
+    const EmGeo g = geo_of(d);
+    go<K_em_probe>(s, grid_blocks(g.Etot), {g, ph, nu, flags, sigma, p, q});
+}

@@ -172,3 +172,39 @@ void launch_predict(hipStream_t s, const Grid3& g, float h, float dt, int advect
 void launch_viscous(hipStream_t s, const Grid3& g, float h, float rho_l, float rho_air, int two_phase, float dt,
                     float viscosity, const float acc[3], float sigma, const uint8_t* open8, const float* pred,
                     const float* cells, const float* em, float* out);
+
+// em.hip — the EM step's Jacobi-CG on the edge curl-curl operator (src/em/induction.wgsl).
+// Edges: per axis a dims n + 1 except n_a, packed x | y | z; faces: dims n + e_a, packed.
+struct EmDims {
+    uint32_t n[3];
+    float h;
+};
+struct EmPhys {
+    float dt, mu0;
+    int magnetostatic;  // no mass term (the initial magnetostatic solve)
+};
+struct EmVecs {
+    float *x, *r, *p, *q;                          // increment, residual, search direction, product
+    const float *sigma, *diag, *nu;                // per edge, per edge, per face
+    const uint8_t* flags;                          // per edge: 1 boundary, 2/4 lower/upper air, 8 air stencil
+    const float *as_now, *as_prev, *mot, *a_prev;  // per edge (mot may be NULL)
+};
+// device scalars of the solve (FP64; status 0 running, 1 converged, 2 breakdown)
+struct EmScal {
+    double rz, rr, bb, l1, threshold, alpha, beta, scale, tol2;
+    int status;
+    uint32_t its;
+    int rescale;
+    int pad;
+    double check[4];  // coefficient_check: boundary conductors, min sigma > 0, max sigma, conductors
+};
+uint32_t em_blocks(const EmDims& d);  // partial records (4 doubles each) of the edge kernels
+void launch_em_coefficients(hipStream_t s, const EmDims& d, const uint32_t* materials, const float* props, int vacuum,
+                            float* sigma, float* diag, uint8_t* flags, float* nu, double* part, void* scal);
+void launch_em_prepare(hipStream_t s, const EmDims& d, const EmPhys& ph, const EmVecs& v, double* part, void* scal);
+void launch_em_iterations(hipStream_t s, const EmDims& d, const EmPhys& ph, const EmVecs& v, double* part, void* scal,
+                          uint32_t count);
+void launch_em_restart(hipStream_t s, const EmDims& d, const EmPhys& ph, const EmVecs& v, double* part, void* scal);
+void launch_em_commit(hipStream_t s, const EmDims& d, const float* a_prev, const float* x, float* out);
+void launch_em_probe_apply(hipStream_t s, const EmDims& d, const EmPhys& ph, const float* nu, const uint8_t* flags,
+                           const float* sigma, const float* p, float* q);

@@ -255,7 +255,8 @@ def em_coefficients(dims, h, materials, props, vacuum=False):
 
 
 def _em_params(dt, mu0, tol, max_iter, magnetostatic):
-    return _EmParams(dt, mu0, tol * tol, max_iter, int(bool(magnetostatic)))
+    # Params.physical.w = relative_tolerance.powi(2) with an f32 tolerance (src/em/mod.rs:888)
+    return _EmParams(dt, mu0, float(np.float32(tol) * np.float32(tol)), max_iter, int(bool(magnetostatic)))
 
 
 def em_apply(dims, h, co, v, dt=1e-5, mu0=4e-7 * np.pi, magnetostatic=False, precision=32):
