@@ -184,6 +184,75 @@ def bench_step(args):
     pr.close()
 
 
+def bench_em(args):
+    """The EM step's Jacobi-CG (src/em/mod.rs:863-1010) at 128^3 cells (6.4M edges): a copper ring
+    and a permeable block in air; a uniform 10 mT field B0 z switched on at the first step (applied
+    potential A_s = B0 x r / 2 on the edges), then held while the eddy currents decay. Each timed
+    step solves to the reference's default relative tolerance 1e-5 from a zero increment."""
+    import numpy as np
+    import torch
+
+    from mhd_simulator_amd.em import EmSolver
+    torch.cuda.set_device(0)
+    n = args.size if args.size != 512 else 128
+    dims = (n, n, n)
+    h = 1e-3
+    i = np.arange(n ** 3)
+    x, y, z = (i % n + 0.5) - n / 2, ((i // n) % n + 0.5) - n / 2, (i // (n * n) + 0.5) - n / 2
+    rho = np.sqrt(x * x + y * y)
+    m = np.zeros(n ** 3, np.uint32)
+    m[(np.abs(rho - 0.25 * n) < 0.06 * n) & (np.abs(z) < 0.06 * n)] = 1  # ring
+    m[(np.abs(x) < 0.08 * n) & (np.abs(y) < 0.08 * n) & (np.abs(z - 0.25 * n) < 0.05 * n)] = 2  # permeable block
+    props = np.array([[0.0, 1.0], [5.8e7, 1.0], [0.0, 100.0]], np.float32)
+    em = EmSolver(dims, h, m, props)
+    E = em.edges
+    # A_s = B0 x r / 2 at the edge midpoints (x edges carry -B0 y / 2, y edges B0 x / 2)
+    b0 = 0.01
+    parts = []
+    for a in range(3):
+        d = [n + (0 if b == a else 1) for b in range(3)]
+        zz, yy, xx = np.meshgrid(*(np.arange(d[k], dtype=np.float32) for k in (2, 1, 0)), indexing="ij")
+        pos = [xx, yy, zz]
+        pos[a] = pos[a] + 0.5
+        cx, cy = (pos[0] - n / 2) * h, (pos[1] - n / 2) * h
+        parts.append((-0.5 * b0 * cy if a == 0 else (0.5 * b0 * cx if a == 1 else np.zeros_like(cx))).ravel())
+    a_s = torch.from_numpy(np.concatenate(parts).astype(np.float32)).cuda()
+    zero = torch.zeros(E, dtype=torch.float32, device="cuda")
+    a = torch.zeros(E, dtype=torch.float32, device="cuda")
+    out = torch.empty_like(a)
+    log(f"em: {n}^3 cells, {E} edges, conducting sigma range {em.sigma_range}")
+
+    def step(first):
+        its, res = em.solve(a_s, zero if first else a_s, a, out, dt=1e-5, rel_tol=1e-5, max_iter=20000)
+        a.copy_(out)
+        return its, res
+
+    for w in range(max(args.warmup, 1)):
+        step(w == 0)
+    torch.cuda.synchronize()
+    total_its = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        its, res = step(False)
+        total_its += its
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
+    per_it = wall / max(total_its / args.steps, 1)
+    log(f"em: {wall * 1e3:.1f} ms/step, {total_its / args.steps:.0f} iterations/step, {per_it * 1e3:.3f} ms/iteration")
+    out_line = {
+        "metric": "EM Jacobi-CG edge-iterations/s (rel. tol 1e-5)", "value": E * total_its / args.steps / wall,
+        "unit": "edge-iterations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (+f64 reductions)", "data": "synthetic (copper ring + permeable block in air)",
+        "config": {"workload": f"EM induction step {n}^3 cells, {E} edges, field held after switch-on",
+                   "iterations_per_step": total_its / args.steps, "ms_per_iteration": per_it * 1e3,
+                   "rel_residual": res},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out_line), flush=True)
+    em.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,9 +263,10 @@ def main():
     ap.add_argument("--cpu-size", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
-    ap.add_argument("--workload", default="plate512", choices=["plate512", "batch64", "step512"],
+    ap.add_argument("--workload", default="plate512", choices=["plate512", "batch64", "step512", "em128"],
                     help="plate512: the 512^3 headline; batch64: 64 uneven droplet problems in one batch call; "
-                         "step512: the whole fluid step (predict, viscous, projection, fields)")
+                         "step512: the whole fluid step (predict, viscous, projection, fields); "
+                         "em128: the EM step's Jacobi-CG")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device-to-device 5 times (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -218,6 +288,9 @@ def main():
         return
     if args.workload == "step512":
         bench_step(args)
+        return
+    if args.workload == "em128":
+        bench_em(args)
         return
 
     import numpy as np

@@ -354,16 +354,82 @@ struct K_em_prepare {
     }
 };
 
-// apply_operator (induction.wgsl:160-170) of one edge
-__device__ __forceinline__ float apply_edge(const EmGeo& g, const Phys& ph, const float* __restrict__ nu,
-                                            const uint8_t* __restrict__ flags, const float* __restrict__ sigma,
-                                            const float* __restrict__ v, uint32_t i) {
+// The operator product in two passes (the values are those of apply_edge, bit for bit): pass 1
+// stores every face's nu * curl(v) and every node's divergence of v, pass 2 gathers them per edge
+// (an edge reads four face terms and two node divergences instead of recomputing 16 + 12 edge
+// values).
+__device__ __forceinline__ uint32_t node_index(const EmGeo& g, const int p[3]) {
+    return (uint32_t)p[0] + (uint32_t)(g.n[0] + 1) * ((uint32_t)p[1] + (uint32_t)(g.n[1] + 1) * (uint32_t)p[2]);
+}
+struct K_em_curl {
+    static constexpr int MINB = 0;
+    struct Args {
+        EmGeo g;
+        const float* nu;
+        const float* v;
+        float* tface;  // per face: nu * curl(v)
+        float* ndiv;   // per node: node_div(v)
+        const EmScal* sc;  // NULL: always run
+    };
+    static __device__ void run(const Args& a, const VGrid& vg) {
+        if (a.sc && a.sc->status != 0) return;
+        const EmGeo g = a.g;
+        const uint32_t faces = g.F[0] + g.F[1] + g.F[2];
+        const uint32_t nodes = (uint32_t)(g.n[0] + 1) * (uint32_t)(g.n[1] + 1) * (uint32_t)(g.n[2] + 1);
+        uint32_t lo, hi;
+        block_range(vg, faces + nodes, lo, hi);
+        for (uint32_t k = lo + threadIdx.x; k < hi; k += MHDP_BLOCK) {
+            if (k < faces) {
+                uint32_t j = k;
+                int axis = 0;
+                if (j >= g.F[0]) {
+                    j -= g.F[0];
+                    axis = 1;
+                    if (j >= g.F[1]) {
+                        j -= g.F[1];
+                        axis = 2;
+                    }
+                }
+                const uint32_t px = (uint32_t)g.n[0] + (axis == 0), py = (uint32_t)g.n[1] + (axis == 1);
+                const int p[3] = {(int)(j % px), (int)((j / px) % py), (int)(j / (px * py))};
+                a.tface[k] = face_term(g, a.nu, a.v, axis, p, false);
+            } else {
+                const uint32_t j = k - faces;
+                const uint32_t nx = (uint32_t)g.n[0] + 1, ny = (uint32_t)g.n[1] + 1;
+                const int p[3] = {(int)(j % nx), (int)((j / nx) % ny), (int)(j / (nx * ny))};
+                a.ndiv[j] = node_div(g, a.v, p);
+            }
+        }
+    }
+};
+__device__ __forceinline__ float stored_term(const EmGeo& g, const float* __restrict__ t, int axis, const int p[3]) {
+    return face_valid(g, axis, p) ? t[face_index(g, axis, p)] : 0.0f;
+}
+// apply_operator from the stored face terms and node divergences
+__device__ __forceinline__ float apply_stored(const EmGeo& g, const Phys& ph, const float* __restrict__ tface,
+                                              const float* __restrict__ ndiv, const uint8_t* __restrict__ flags,
+                                              const float* __restrict__ sigma, const float* __restrict__ v,
+                                              uint32_t i) {
     int p[3];
     const int axis = edge_decode(g, i, p);
     const uint32_t fl = flags[i];
     float q;
-    if (fl & kScalarAir) q = air_laplacian(g, v, i, axis);
-    else q = adjoint(g, nu, v, axis, p, false) + gauge(g, fl, v, axis, p) + mass(g, ph, sigma[i]) * v[i];
+    if (fl & kScalarAir) {
+        q = air_laplacian(g, v, i, axis);
+    } else {
+        const int ea = (axis + 1) % 3, eb = (axis + 2) % 3;
+        int qa[3] = {p[0], p[1], p[2]}, qb[3] = {p[0], p[1], p[2]};
+        qa[ea] -= 1;
+        qb[eb] -= 1;
+        const float adj = stored_term(g, tface, eb, p) - stored_term(g, tface, eb, qa) - stored_term(g, tface, ea, p) +
+                          stored_term(g, tface, ea, qb);
+        int upper[3] = {p[0], p[1], p[2]};
+        upper[axis] += 1;
+        float gg = 0.0f;
+        if (fl & kLowerAir) gg += ndiv[node_index(g, p)];
+        if (fl & kUpperAir) gg -= ndiv[node_index(g, upper)];
+        q = adj + gg + mass(g, ph, sigma[i]) * v[i];
+    }
     if (fl & kBoundary) q = v[i];
     return q;
 }
@@ -374,6 +440,8 @@ struct K_em_apply {
         EmGeo g;
         Phys ph;
         Vecs v;
+        const float* tface;
+        const float* ndiv;
         const EmScal* sc;
         double* part;
     };
@@ -383,7 +451,7 @@ struct K_em_apply {
         block_range(vg, a.g.Etot, lo, hi);
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
         for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
-            const float q = apply_edge(a.g, a.ph, a.v.nu, a.v.flags, a.v.sigma, a.v.p, i);
+            const float q = apply_stored(a.g, a.ph, a.tface, a.ndiv, a.v.flags, a.v.sigma, a.v.p, i);
             a.v.q[i] = q;
             acc[0] += (double)a.v.p[i] * (double)q;
         }
@@ -577,6 +645,8 @@ struct K_em_fold {
     }
 };
 
+uint32_t em_nodes(const EmGeo& g) { return (uint32_t)(g.n[0] + 1) * (uint32_t)(g.n[1] + 1) * (uint32_t)(g.n[2] + 1); }
+
 EmGeo geo_of(const EmDims& d) {
     EmGeo g;
     for (int a = 0; a < 3; a++) g.n[a] = (int)d.n[a];
@@ -624,8 +694,10 @@ void launch_em_iterations(hipStream_t s, const EmDims& d, const EmPhys& ph, cons
     const Phys& p = ph;
     const Vecs& vv = v;
     EmScal* sc = (EmScal*)scal;
+    const uint32_t curl_blocks = grid_blocks(g.F[0] + g.F[1] + g.F[2] + em_nodes(g));
     for (uint32_t k = 0; k < count; k++) {
-        go<K_em_apply>(s, blocks, {g, p, vv, sc, part});
+        go<K_em_curl>(s, curl_blocks, {g, vv.nu, vv.p, v.tface, v.ndiv, sc});
+        go<K_em_apply>(s, blocks, {g, p, vv, v.tface, v.ndiv, sc, part});
         go<K_em_fold>(s, 1, {part, blocks, kFoldAlpha, sc});
         go<K_em_update>(s, blocks, {g, p, vv, sc, part});
         go<K_em_fold>(s, 1, {part, blocks, kFoldBeta, sc});
@@ -653,7 +725,8 @@ struct K_em_probe {
     struct Args {
         EmGeo g;
         Phys ph;
-        const float* nu;
+        const float* tface;
+        const float* ndiv;
         const uint8_t* flags;
         const float* sigma;
         const float* p;
@@ -663,12 +736,13 @@ struct K_em_probe {
         uint32_t lo, hi;
         block_range(vg, a.g.Etot, lo, hi);
         for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK)
-            a.q[i] = apply_edge(a.g, a.ph, a.nu, a.flags, a.sigma, a.p, i);
+            a.q[i] = apply_stored(a.g, a.ph, a.tface, a.ndiv, a.flags, a.sigma, a.p, i);
     }
 };
 
 void launch_em_probe_apply(hipStream_t s, const EmDims& d, const EmPhys& ph, const float* nu, const uint8_t* flags,
-                           const float* sigma, const float* p, float* q) {
+                           const float* sigma, const float* p, float* q, float* tface, float* ndiv) {
     const EmGeo g = geo_of(d);
-    go<K_em_probe>(s, grid_blocks(g.Etot), {g, ph, nu, flags, sigma, p, q});
+    go<K_em_curl>(s, grid_blocks(g.F[0] + g.F[1] + g.F[2] + em_nodes(g)), {g, nu, p, tface, ndiv, nullptr});
+    go<K_em_probe>(s, grid_blocks(g.Etot), {g, ph, tface, ndiv, flags, sigma, p, q});
 }

@@ -24,6 +24,7 @@ struct mhdp_em {
     float *sigma = nullptr, *diag = nullptr, *nu = nullptr;
     uint8_t* flags = nullptr;
     float *x = nullptr, *r = nullptr, *p = nullptr, *q = nullptr;
+    float *tface = nullptr, *ndiv = nullptr;  // operator scratch: per face, per node
     double* part = nullptr;
     EmScal* sc = nullptr;
     EmScal* sc_h = nullptr;  // pinned host copy
@@ -95,6 +96,7 @@ int mhdp_em_create(mhdp_em** out, const mhdp_grid* grid) {
         (rc = alloc(e, (void**)&e->nu, e->faces * 4)) || (rc = alloc(e, (void**)&e->flags, E)) ||
         (rc = alloc(e, (void**)&e->x, E * 4)) || (rc = alloc(e, (void**)&e->r, E * 4)) ||
         (rc = alloc(e, (void**)&e->p, E * 4)) || (rc = alloc(e, (void**)&e->q, E * 4)) ||
+        (rc = alloc(e, (void**)&e->tface, e->faces * 4)) || (rc = alloc(e, (void**)&e->ndiv, (x + 1) * (y + 1) * (z + 1) * 4)) ||
         (rc = alloc(e, (void**)&e->part, (size_t)em_blocks(e->d) * 4 * sizeof(double) + 64)) ||
         (rc = alloc(e, (void**)&e->sc, sizeof(EmScal)))) {
         mhdp_em_destroy(e);
@@ -148,7 +150,8 @@ int mhdp_em_solve_dev(mhdp_em* e, const mhdp_em_opts* o, const float* as_now, co
         return fail(e, MHDP_INVALID, "Invalid EM solve options");
     const auto t0 = std::chrono::steady_clock::now();
     const EmPhys ph{o->dt, o->mu0, o->magnetostatic ? 1 : 0};
-    const EmVecs v{e->x, e->r, e->p, e->q, e->sigma, e->diag, e->nu, e->flags, as_now, as_prev, motional, a_prev};
+    const EmVecs v{e->x,     e->r,    e->p,    e->q,     e->sigma, e->diag,  e->nu,
+                   e->flags, as_now,  as_prev, motional, a_prev,   e->tface, e->ndiv};
     // begin_step: scale 1, zero increment; prepare + initial_reduce (repeated once on a new scale)
     EmScal init{};
     init.scale = 1.0;
@@ -219,7 +222,7 @@ int mhdp_em_probe_apply(mhdp_em* e, const mhdp_em_opts* o, const float* p, float
     const size_t bytes = e->edges * 4;
     EM_HIP(hipMemcpyAsync(e->p, p, bytes, hipMemcpyHostToDevice, e->stream));
     launch_em_probe_apply(e->stream, e->d, EmPhys{o->dt, o->mu0, o->magnetostatic ? 1 : 0}, e->nu, e->flags, e->sigma,
-                          e->p, e->q);
+                          e->p, e->q, e->tface, e->ndiv);
     EM_HIP(hipMemcpyAsync(q, e->q, bytes, hipMemcpyDeviceToHost, e->stream));
     EM_HIP(hipStreamSynchronize(e->stream));
     EM_HIP(hipGetLastError());
@@ -233,7 +236,8 @@ const char* mhdp_em_last_error(const mhdp_em* e) { return e ? e->err.c_str() : "
 void mhdp_em_destroy(mhdp_em* e) {
     if (!e) return;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
-    void* ptrs[] = {e->materials, e->props, e->sigma, e->diag, e->nu, e->flags, e->x, e->r, e->p, e->q, e->part, e->sc};
+    void* ptrs[] = {e->materials, e->props, e->sigma, e->diag, e->nu,    e->flags, e->x,
+                    e->r,         e->p,     e->q,     e->tface, e->ndiv, e->part,  e->sc};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (e->sc_h) (void)hipHostFree(e->sc_h);

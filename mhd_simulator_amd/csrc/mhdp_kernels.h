@@ -188,6 +188,7 @@ struct EmVecs {
     const float *sigma, *diag, *nu;                // per edge, per edge, per face
     const uint8_t* flags;                          // per edge: 1 boundary, 2/4 lower/upper air, 8 air stencil
     const float *as_now, *as_prev, *mot, *a_prev;  // per edge (mot may be NULL)
+    float *tface, *ndiv;                           // scratch: per face nu curl p, per node div p
 };
 // device scalars of the solve (FP64; status 0 running, 1 converged, 2 breakdown)
 struct EmScal {
@@ -207,4 +208,4 @@ void launch_em_iterations(hipStream_t s, const EmDims& d, const EmPhys& ph, cons
 void launch_em_restart(hipStream_t s, const EmDims& d, const EmPhys& ph, const EmVecs& v, double* part, void* scal);
 void launch_em_commit(hipStream_t s, const EmDims& d, const float* a_prev, const float* x, float* out);
 void launch_em_probe_apply(hipStream_t s, const EmDims& d, const EmPhys& ph, const float* nu, const uint8_t* flags,
-                           const float* sigma, const float* p, float* q);
+                           const float* sigma, const float* p, float* q, float* tface, float* ndiv);
