@@ -361,44 +361,93 @@ struct K_em_prepare {
 __device__ __forceinline__ uint32_t node_index(const EmGeo& g, const int p[3]) {
     return (uint32_t)p[0] + (uint32_t)(g.n[0] + 1) * ((uint32_t)p[1] + (uint32_t)(g.n[1] + 1) * (uint32_t)p[2]);
 }
+// The elements of one lattice (dims dx, dy, ...) split into B contiguous block ranges; a thread
+// decodes its first element once and then steps by the block size, carrying x into y and z
+// (no division per element).  f(x, y, z, k) with k the flat index in the lattice.
+template <class F>
+__device__ __forceinline__ void for_lattice(uint32_t dx, uint32_t dy, uint32_t total, uint32_t local, uint32_t B,
+                                            F&& f) {
+    uint32_t lo, hi;
+    block_range(VGrid{local, B}, total, lo, hi);
+    uint32_t k = lo + threadIdx.x;
+    if (k >= hi) return;
+    uint32_t x = k % dx;
+    const uint32_t t = k / dx;
+    uint32_t y = t % dy, z = t / dy;
+    for (; k < hi; k += MHDP_BLOCK) {
+        f((int)x, (int)y, (int)z, k);
+        x += MHDP_BLOCK;
+        while (x >= dx) {
+            x -= dx;
+            if (++y == dy) {
+                y = 0;
+                ++z;
+            }
+        }
+    }
+}
+// pass 1: block group a < 3 the faces of axis a, group 3 the nodes (B blocks each)
 struct K_em_curl {
     static constexpr int MINB = 0;
     struct Args {
         EmGeo g;
         const float* nu;
         const float* v;
-        float* tface;  // per face: nu * curl(v)
-        float* ndiv;   // per node: node_div(v)
+        float* tface;      // per face: nu * curl(v)
+        float* ndiv;       // per node: node_div(v)
         const EmScal* sc;  // NULL: always run
+        uint32_t B;
     };
     static __device__ void run(const Args& a, const VGrid& vg) {
         if (a.sc && a.sc->status != 0) return;
         const EmGeo g = a.g;
-        const uint32_t faces = g.F[0] + g.F[1] + g.F[2];
-        const uint32_t nodes = (uint32_t)(g.n[0] + 1) * (uint32_t)(g.n[1] + 1) * (uint32_t)(g.n[2] + 1);
-        uint32_t lo, hi;
-        block_range(vg, faces + nodes, lo, hi);
-        for (uint32_t k = lo + threadIdx.x; k < hi; k += MHDP_BLOCK) {
-            if (k < faces) {
-                uint32_t j = k;
-                int axis = 0;
-                if (j >= g.F[0]) {
-                    j -= g.F[0];
-                    axis = 1;
-                    if (j >= g.F[1]) {
-                        j -= g.F[1];
-                        axis = 2;
-                    }
+        const uint32_t group = vg.b / a.B, local = vg.b % a.B;
+        const uint32_t nx = (uint32_t)g.n[0], ny = (uint32_t)g.n[1], nz = (uint32_t)g.n[2];
+        const float* __restrict__ v = a.v;
+        if (group < 3) {
+            const int axis = (int)group;
+            const uint32_t off = axis == 0 ? 0u : (axis == 1 ? g.F[0] : g.F[0] + g.F[1]);
+            float* __restrict__ t = a.tface + off;
+            const uint32_t dx = nx + (axis == 0), dy = ny + (axis == 1), dz = nz + (axis == 2);
+            const int ea = (axis + 1) % 3, eb = (axis + 2) % 3;
+            for_lattice(dx, dy, dx * dy * dz, local, a.B, [&](int x, int y, int z, uint32_t k) {
+                // face_term of a face in the lattice: nu * curl, the four edges loaded up front
+                const int p[3] = {x, y, z};
+                int qa[3] = {x, y, z}, qb[3] = {x, y, z};
+                qa[ea] += 1;
+                qb[eb] += 1;
+                const bool v0 = edge_valid(g, eb, qa), v1 = edge_valid(g, eb, p), v2 = edge_valid(g, ea, qb),
+                           v3 = edge_valid(g, ea, p);
+                float e0 = v[v0 ? edge_index(g, eb, qa) : 0u], e1 = v[v1 ? edge_index(g, eb, p) : 0u];
+                float e2 = v[v2 ? edge_index(g, ea, qb) : 0u], e3 = v[v3 ? edge_index(g, ea, p) : 0u];
+                float n = a.nu[off + k];
+                asm volatile("" : "+v"(e0), "+v"(e1), "+v"(e2), "+v"(e3), "+v"(n));
+                const float cu = (v0 ? e0 : 0.0f) - (v1 ? e1 : 0.0f) - (v2 ? e2 : 0.0f) + (v3 ? e3 : 0.0f);
+                t[k] = n * cu;
+            });
+        } else {
+            float* __restrict__ d = a.ndiv;
+            for_lattice(nx + 1, ny + 1, (nx + 1) * (ny + 1) * (nz + 1), local, a.B, [&](int x, int y, int z, uint32_t k) {
+                // node_div with its six edges loaded up front
+                const int p[3] = {x, y, z};
+                bool ok[3][2];
+                float e[3][2];
+#pragma unroll
+                for (int b = 0; b < 3; b++) {
+                    int q[3] = {x, y, z};
+                    q[b] -= 1;
+                    ok[b][0] = edge_valid(g, b, p);
+                    ok[b][1] = edge_valid(g, b, q);
+                    e[b][0] = v[ok[b][0] ? edge_index(g, b, p) : 0u];
+                    e[b][1] = v[ok[b][1] ? edge_index(g, b, q) : 0u];
                 }
-                const uint32_t px = (uint32_t)g.n[0] + (axis == 0), py = (uint32_t)g.n[1] + (axis == 1);
-                const int p[3] = {(int)(j % px), (int)((j / px) % py), (int)(j / (px * py))};
-                a.tface[k] = face_term(g, a.nu, a.v, axis, p, false);
-            } else {
-                const uint32_t j = k - faces;
-                const uint32_t nx = (uint32_t)g.n[0] + 1, ny = (uint32_t)g.n[1] + 1;
-                const int p[3] = {(int)(j % nx), (int)((j / nx) % ny), (int)(j / (nx * ny))};
-                a.ndiv[j] = node_div(g, a.v, p);
-            }
+#pragma unroll
+                for (int b = 0; b < 3; b++) asm volatile("" : "+v"(e[b][0]), "+v"(e[b][1]));
+                float value = 0.0f;
+#pragma unroll
+                for (int b = 0; b < 3; b++) value += (ok[b][0] ? e[b][0] : 0.0f) - (ok[b][1] ? e[b][1] : 0.0f);
+                d[k] = value;
+            });
         }
     }
 };
@@ -406,32 +455,57 @@ __device__ __forceinline__ float stored_term(const EmGeo& g, const float* __rest
     return face_valid(g, axis, p) ? t[face_index(g, axis, p)] : 0.0f;
 }
 // apply_operator from the stored face terms and node divergences
+__device__ __forceinline__ float apply_stored_at(const EmGeo& g, const Phys& ph, const float* __restrict__ tface,
+                                                 const float* __restrict__ ndiv, const uint8_t* __restrict__ flags,
+                                                 const float* __restrict__ sigma, const float* __restrict__ v,
+                                                 uint32_t i, int axis, const int p[3]) {
+    // every value either form may need is loaded first, from indices clamped into their arrays
+    // (a clamped value is never used: air edges are interior, invalid faces count as 0), and pinned
+    // so that the loads stay ahead of the arithmetic - one memory round trip per edge
+    uint32_t d[3];
+    edge_dims(g, axis, d);
+    const uint32_t zs = d[0] * d[1], last = g.Etot - 1u;
+    const uint32_t im = i >= 1u ? i - 1u : i, ip = i + 1u <= last ? i + 1u : i;
+    const uint32_t iym = i >= d[0] ? i - d[0] : i, iyp = i + d[0] <= last ? i + d[0] : i;
+    const uint32_t izm = i >= zs ? i - zs : i, izp = i + zs <= last ? i + zs : i;
+    const int ea = (axis + 1) % 3, eb = (axis + 2) % 3;
+    int qa[3] = {p[0], p[1], p[2]}, qb[3] = {p[0], p[1], p[2]}, upper[3] = {p[0], p[1], p[2]};
+    qa[ea] -= 1;
+    qb[eb] -= 1;
+    upper[axis] += 1;
+    const bool f0 = face_valid(g, eb, p), f1 = face_valid(g, eb, qa), f2 = face_valid(g, ea, p), f3 = face_valid(g, ea, qb);
+    uint32_t fl = flags[i];
+    float c = v[i], xm = v[im], xp = v[ip], ym = v[iym], yp = v[iyp], zm = v[izm], zp = v[izp];
+    float t0 = tface[f0 ? face_index(g, eb, p) : 0u], t1 = tface[f1 ? face_index(g, eb, qa) : 0u];
+    float t2 = tface[f2 ? face_index(g, ea, p) : 0u], t3 = tface[f3 ? face_index(g, ea, qb) : 0u];
+    float n0 = ndiv[node_index(g, p)], n1 = ndiv[node_index(g, upper)];
+    float sg = sigma[i];
+    asm volatile("" : "+v"(fl), "+v"(c), "+v"(xm), "+v"(xp), "+v"(ym), "+v"(yp), "+v"(zm), "+v"(zp));
+    asm volatile("" : "+v"(t0), "+v"(t1), "+v"(t2), "+v"(t3), "+v"(n0), "+v"(n1), "+v"(sg));
+    float q;
+    if (fl & kScalarAir) {  // air_laplacian (induction.wgsl:150-158)
+        float result = 0.0f;
+        result += (c - xm) + (c - xp);
+        result += (c - ym) + (c - yp);
+        result += (c - zm) + (c - zp);
+        q = result;
+    } else {  // adjoint + gauge + mass (induction.wgsl:164)
+        const float adj = (f0 ? t0 : 0.0f) - (f1 ? t1 : 0.0f) - (f2 ? t2 : 0.0f) + (f3 ? t3 : 0.0f);
+        float gg = 0.0f;
+        if (fl & kLowerAir) gg += n0;
+        if (fl & kUpperAir) gg -= n1;
+        q = adj + gg + mass(g, ph, sg) * c;
+    }
+    if (fl & kBoundary) q = c;
+    return q;
+}
 __device__ __forceinline__ float apply_stored(const EmGeo& g, const Phys& ph, const float* __restrict__ tface,
                                               const float* __restrict__ ndiv, const uint8_t* __restrict__ flags,
                                               const float* __restrict__ sigma, const float* __restrict__ v,
                                               uint32_t i) {
     int p[3];
     const int axis = edge_decode(g, i, p);
-    const uint32_t fl = flags[i];
-    float q;
-    if (fl & kScalarAir) {
-        q = air_laplacian(g, v, i, axis);
-    } else {
-        const int ea = (axis + 1) % 3, eb = (axis + 2) % 3;
-        int qa[3] = {p[0], p[1], p[2]}, qb[3] = {p[0], p[1], p[2]};
-        qa[ea] -= 1;
-        qb[eb] -= 1;
-        const float adj = stored_term(g, tface, eb, p) - stored_term(g, tface, eb, qa) - stored_term(g, tface, ea, p) +
-                          stored_term(g, tface, ea, qb);
-        int upper[3] = {p[0], p[1], p[2]};
-        upper[axis] += 1;
-        float gg = 0.0f;
-        if (fl & kLowerAir) gg += ndiv[node_index(g, p)];
-        if (fl & kUpperAir) gg -= ndiv[node_index(g, upper)];
-        q = adj + gg + mass(g, ph, sigma[i]) * v[i];
-    }
-    if (fl & kBoundary) q = v[i];
-    return q;
+    return apply_stored_at(g, ph, tface, ndiv, flags, sigma, v, i, axis, p);
 }
 
 struct K_em_apply {
@@ -444,17 +518,26 @@ struct K_em_apply {
         const float* ndiv;
         const EmScal* sc;
         double* part;
+        uint32_t B;
     };
+    // block group a: the edges of axis a, by rows
     static __device__ void run(const Args& a, const VGrid& vg) {
         if (a.sc->status != 0) return;
-        uint32_t lo, hi;
-        block_range(vg, a.g.Etot, lo, hi);
+        const EmGeo g = a.g;
+        const int axis = (int)(vg.b / a.B);
+        const uint32_t local = vg.b % a.B;
+        const uint32_t off = axis == 0 ? 0u : (axis == 1 ? g.E[0] : g.E[0] + g.E[1]);
+        const float* __restrict__ pv = a.v.p;
+        float* __restrict__ q = a.v.q;
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
-        for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
-            const float q = apply_stored(a.g, a.ph, a.tface, a.ndiv, a.v.flags, a.v.sigma, a.v.p, i);
-            a.v.q[i] = q;
-            acc[0] += (double)a.v.p[i] * (double)q;
-        }
+        const uint32_t dx = (uint32_t)g.n[0] + (axis != 0), dy = (uint32_t)g.n[1] + (axis != 1);
+        for_lattice(dx, dy, g.E[axis], local, a.B, [&](int x, int y, int z, uint32_t k) {
+                     const int p[3] = {x, y, z};
+                     const uint32_t i = off + k;
+                     const float qi = apply_stored_at(g, a.ph, a.tface, a.ndiv, a.v.flags, a.v.sigma, pv, i, axis, p);
+                     q[i] = qi;
+                     acc[0] += (double)pv[i] * (double)qi;
+                 });
         block_partials<0, 0>(acc, a.part, vg.b);
     }
 };
@@ -645,7 +728,12 @@ struct K_em_fold {
     }
 };
 
-uint32_t em_nodes(const EmGeo& g) { return (uint32_t)(g.n[0] + 1) * (uint32_t)(g.n[1] + 1) * (uint32_t)(g.n[2] + 1); }
+// blocks per lattice of the row kernels: about a third of the edge kernels' grid each, a multiple
+// of 8 (every XCD gets whole groups), so that 4 groups stay within the partial records
+uint32_t row_blocks(const EmGeo& g) {
+    const uint32_t b = grid_blocks(g.Etot) / 4 / 8 * 8;
+    return b < 8 ? 8 : b;
+}
 
 EmGeo geo_of(const EmDims& d) {
     EmGeo g;
@@ -666,7 +754,11 @@ EmGeo geo_of(const EmDims& d) {
 
 }  // namespace
 
-uint32_t em_blocks(const EmDims& d) { return grid_blocks(geo_of(d).Etot); }
+uint32_t em_blocks(const EmDims& d) {
+    const EmGeo g = geo_of(d);
+    const uint32_t a = grid_blocks(g.Etot), b = 4 * row_blocks(g);
+    return a > b ? a : b;
+}
 
 void launch_em_coefficients(hipStream_t s, const EmDims& d, const uint32_t* materials, const float* props, int vacuum,
                             float* sigma, float* diag, uint8_t* flags, float* nu, double* part, void* scal) {
@@ -694,11 +786,11 @@ void launch_em_iterations(hipStream_t s, const EmDims& d, const EmPhys& ph, cons
     const Phys& p = ph;
     const Vecs& vv = v;
     EmScal* sc = (EmScal*)scal;
-    const uint32_t curl_blocks = grid_blocks(g.F[0] + g.F[1] + g.F[2] + em_nodes(g));
+    const uint32_t B = row_blocks(g);
     for (uint32_t k = 0; k < count; k++) {
-        go<K_em_curl>(s, curl_blocks, {g, vv.nu, vv.p, v.tface, v.ndiv, sc});
-        go<K_em_apply>(s, blocks, {g, p, vv, v.tface, v.ndiv, sc, part});
-        go<K_em_fold>(s, 1, {part, blocks, kFoldAlpha, sc});
+        go<K_em_curl>(s, 4 * B, {g, vv.nu, vv.p, v.tface, v.ndiv, sc, B});
+        go<K_em_apply>(s, 3 * B, {g, p, vv, v.tface, v.ndiv, sc, part, B});
+        go<K_em_fold>(s, 1, {part, 3 * B, kFoldAlpha, sc});
         go<K_em_update>(s, blocks, {g, p, vv, sc, part});
         go<K_em_fold>(s, 1, {part, blocks, kFoldBeta, sc});
         go<K_em_search>(s, blocks, {g, p, vv, sc});
@@ -743,6 +835,7 @@ struct K_em_probe {
 void launch_em_probe_apply(hipStream_t s, const EmDims& d, const EmPhys& ph, const float* nu, const uint8_t* flags,
                            const float* sigma, const float* p, float* q, float* tface, float* ndiv) {
     const EmGeo g = geo_of(d);
-    go<K_em_curl>(s, grid_blocks(g.F[0] + g.F[1] + g.F[2] + em_nodes(g)), {g, nu, p, tface, ndiv, nullptr});
+    const uint32_t B = row_blocks(g);
+    go<K_em_curl>(s, 4 * B, {g, nu, p, tface, ndiv, nullptr, B});
     go<K_em_probe>(s, grid_blocks(g.Etot), {g, ph, tface, ndiv, flags, sigma, p, q});
 }
