@@ -275,6 +275,267 @@ static double now_seconds(void) {
     return ts.tv_sec + 1e-9 * ts.tv_nsec;
 }
 
+/* ------------------------------------------------------------------ viscous stress (two-phase)
+ * diffuse_surface_stress (src/fluid/mod.rs:1146-1234), src/fluid/viscosity.wgsl and the RKC2
+ * stages of src/fluid/stabilized.{rs,wgsl}.  The packet's reserved.z is the occupancy. */
+static inline float stress_mu(const ctx_t *c, float nu_rho, float mu_air, const int32_t q[3]) {
+    if (!fluid(c, q)) return 0.0f; /* dynamic_mu (viscosity.wgsl:3-6) */
+    const float t = fminf(fmaxf(occupancy(c, q), 0.0f), 1.0f);
+    return mu_air * (1.0f - t) + nu_rho * t;
+}
+static float stress_edge_mu(const ctx_t *c, float nu_rho, float mu_air, const int32_t q[3], int i, int j) {
+    float total = 0.0f, count = 0.0f; /* coefficient_edge_mu (viscosity.wgsl:16-22) */
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) {
+            int32_t at[3] = {q[0], q[1], q[2]};
+            at[i] -= a;
+            at[j] -= b;
+            if (fluid(c, at)) {
+                total += stress_mu(c, nu_rho, mu_air, at);
+                count += 1.0f;
+            }
+        }
+    return total / fmaxf(count, 1.0f);
+}
+static float normal_stress(const ctx_t *c, float nu_rho, float mu_air, const float *u, const int32_t q[3], int i) {
+    if (!fluid(c, q)) return 0.0f; /* viscosity.wgsl:27-30 */
+    int32_t upper[3] = {q[0], q[1], q[2]};
+    upper[i] += 1;
+    return 2.0f * stress_mu(c, nu_rho, mu_air, q) * (face_value(c, u, upper, i) - face_value(c, u, q, i)) / c->h;
+}
+static float shear_stress(const ctx_t *c, float nu_rho, float mu_air, const float *u, const int32_t q[3], int i, int j) {
+    int32_t li[3] = {q[0], q[1], q[2]}, lj[3] = {q[0], q[1], q[2]}; /* viscosity.wgsl:31-42 */
+    li[j] -= 1;
+    lj[i] -= 1;
+    float ui1 = face_value(c, u, q, i), ui0 = face_value(c, u, li, i);
+    float uj1 = face_value(c, u, q, j), uj0 = face_value(c, u, lj, j);
+    /* odd tangential ghosts: zero velocity halfway to a closed wall */
+    if (!open_face(c, q, i) && open_face(c, li, i)) ui1 = -ui0;
+    if (!open_face(c, li, i) && open_face(c, q, i)) ui0 = -ui1;
+    if (!open_face(c, q, j) && open_face(c, lj, j)) uj1 = -uj0;
+    if (!open_face(c, lj, j) && open_face(c, q, j)) uj0 = -uj1;
+    return stress_edge_mu(c, nu_rho, mu_air, q, i, j) * (ui1 - ui0 + uj1 - uj0) / c->h;
+}
+/* stress_acceleration (viscosity.wgsl:68-82): div(tau) / (h rho_face) */
+static float stress_acc(const ctx_t *c, float nu_rho, float mu_air, const float *u, const int32_t q[3], int i) {
+    int32_t left[3] = {q[0], q[1], q[2]};
+    left[i] -= 1;
+    float div = normal_stress(c, nu_rho, mu_air, u, q, i) - normal_stress(c, nu_rho, mu_air, u, left, i);
+    for (int j = 0; j < 3; j++) {
+        if (j == i) continue;
+        int32_t upper[3] = {q[0], q[1], q[2]};
+        upper[j] += 1;
+        div += shear_stress(c, nu_rho, mu_air, u, upper, i, j) - shear_stress(c, nu_rho, mu_air, u, q, i, j);
+    }
+    return div / (c->h * face_density(c, q, i));
+}
+
+static float *packet_alpha(const orc_grid *g, const float *cells) {
+    const uint64_t n = (uint64_t)g->nx * g->ny * g->nz;
+    float *alpha = (float *)malloc((n + 1) * sizeof(float));
+    for (uint64_t i = 0; i < n; i++) alpha[i] = cells[16 * i + 14];
+    return alpha;
+}
+
+/* viscosity_bound + reduce (viscosity.wgsl:83-104): the largest row sum of rho^-1 div(tau) */
+float orc_stress_bound(const orc_grid *g, const uint32_t *mask, const float *cells, float nu, float mu_air) {
+    float *alpha = packet_alpha(g, cells);
+    ctx_t c;
+    ctx_init(&c, g, mask, alpha, 0);
+    const float nu_rho = nu * g->rho_liquid;
+    float bound = 0.0f;
+    for (int axis = 0; axis < 3; axis++) {
+        uint32_t P[3];
+        plane_dims(c.d, axis, P);
+        const uint64_t np = plane_size(c.d, axis);
+        for (uint64_t k = 0; k < np; k++) {
+            int32_t q[3];
+            decode3(k, P, q);
+            if (!open_face(&c, q, axis)) continue;
+            int32_t left[3] = {q[0], q[1], q[2]};
+            left[axis] -= 1;
+            float sum = stress_mu(&c, nu_rho, mu_air, q) + stress_mu(&c, nu_rho, mu_air, left);
+            for (int j = 0; j < 3; j++) {
+                if (j == axis) continue;
+                int32_t upper[3] = {q[0], q[1], q[2]};
+                upper[j] += 1;
+                sum += stress_edge_mu(&c, nu_rho, mu_air, q, axis, j) + stress_edge_mu(&c, nu_rho, mu_air, upper, axis, j);
+            }
+            bound = fmaxf(bound, 4.0f * sum / (face_density(&c, q, axis) * c.h * c.h));
+        }
+    }
+    free(alpha);
+    return bound;
+}
+
+/* one explicit substep, stress_diffuse (viscosity.wgsl:105-112): out = u + dt_sub div(tau)/rho */
+void orc_stress_euler(const orc_grid *g, const uint32_t *mask, const float *cells, float nu, float mu_air,
+                      float dt_sub, const float *u, float *out) {
+    float *alpha = packet_alpha(g, cells);
+    ctx_t c;
+    ctx_init(&c, g, mask, alpha, 0);
+    const float nu_rho = nu * g->rho_liquid;
+    for (int axis = 0; axis < 3; axis++) {
+        uint32_t P[3];
+        plane_dims(c.d, axis, P);
+        const uint64_t off = plane_offset(c.d, axis), np = plane_size(c.d, axis);
+#pragma omp parallel for schedule(static)
+        for (uint64_t k = 0; k < np; k++) {
+            int32_t q[3];
+            decode3(k, P, q);
+            float v = 0.0f;
+            if (open_face(&c, q, axis))
+                v = face_value(&c, u, q, axis) + dt_sub * stress_acc(&c, nu_rho, mu_air, u, q, axis);
+            out[off + k] = v;
+        }
+    }
+    free(alpha);
+}
+
+/* RKC2 stage rows (stabilized.rs:13-53): damped second-order Runge-Kutta-Chebyshev (Verwer,
+ * Hundsdorfer & Sommeijer 1990) with damping 0.05 and the fewest stages whose stability interval
+ * covers 1.8 x euler_steps.  rows[8 k]: (mu, nu, 1 - mu - nu, dt-weighted operator, initial-operator
+ * weight, 0, 0, 0).  Returns the number of stages. */
+int orc_rkc_schedule(double dt, uint32_t euler_steps, float *rows, int max_rows) {
+    const double damping = 0.05, stiffness = 1.8 * (double)euler_steps;
+    const int capacity = (int)ceil(3.0 * sqrt((double)euler_steps)) + 2;
+    for (int s = 2; s <= capacity; s++) {
+        const double w0 = 1.0 + damping / ((double)s * s);
+        double t[512], d[512], dd[512], b[512], a[512];
+        if (s + 1 > 512 || s > max_rows) return -1;
+        t[0] = 1.0; t[1] = w0; d[0] = 0.0; d[1] = 1.0; dd[0] = 0.0; dd[1] = 0.0;
+        for (int j = 2; j <= s; j++) {
+            t[j] = 2.0 * w0 * t[j - 1] - t[j - 2];
+            d[j] = 2.0 * t[j - 1] + 2.0 * w0 * d[j - 1] - d[j - 2];
+            dd[j] = 4.0 * d[j - 1] + 2.0 * w0 * dd[j - 1] - dd[j - 2];
+        }
+        const double w1 = d[s] / dd[s];
+        if (stiffness > 0.9 * (1.0 + w0) / w1) continue;
+        for (int j = 2; j <= s; j++) b[j] = dd[j] / (d[j] * d[j]);
+        b[0] = b[2];
+        b[1] = b[2];
+        for (int j = 0; j <= s; j++) a[j] = 1.0 - b[j] * t[j];
+        float *r = rows;
+        memset(rows, 0, sizeof(float) * 8 * (size_t)s);
+        r[0] = 1.0f;
+        r[3] = (float)(dt * b[1] * w1);
+        for (int j = 2; j <= s; j++) {
+            float *row = rows + 8 * (j - 1);
+            const float mu = (float)(2.0 * w0 * b[j] / b[j - 1]);
+            const float nu = (float)(-b[j] / b[j - 2]);
+            const double op = dt * 2.0 * w1 * b[j] / b[j - 1];
+            row[0] = mu;
+            row[1] = nu;
+            row[2] = 1.0f - mu - nu;
+            row[3] = (float)op;
+            row[4] = (float)(-a[j - 1] * op);
+        }
+        return s;
+    }
+    return -1;
+}
+
+/* stress_stabilized stage (stabilized.wgsl:18-30) */
+static void rkc_stage(const ctx_t *c, float nu_rho, float mu_air, const float *row, const float *prev,
+                      const float *older, const float *y0, const float *f0, float *out /* may be older */) {
+    for (int axis = 0; axis < 3; axis++) {
+        uint32_t P[3];
+        plane_dims(c->d, axis, P);
+        const uint64_t off = plane_offset(c->d, axis), np = plane_size(c->d, axis);
+#pragma omp parallel for schedule(static)
+        for (uint64_t k = 0; k < np; k++) {
+            int32_t q[3];
+            decode3(k, P, q);
+            float v = 0.0f;
+            if (open_face(c, q, axis)) {
+                int32_t w[3] = {q[0], q[1], q[2]};
+                wrap_pos(c->periodic, c->d, w);
+                const uint64_t canon = face_idx(c->d, axis, w);
+                v = row[0] * face_value(c, prev, q, axis) + row[1] * older[off + k] + row[2] * y0[canon] +
+                    row[3] * stress_acc(c, nu_rho, mu_air, prev, q, axis) + row[4] * f0[canon];
+            }
+            out[off + k] = v;
+        }
+    }
+}
+
+/* diffuse_surface_stress (mod.rs:1160-1234) on u (three planes, in/out): returns 0, 4 (too many
+ * substeps for max_substeps) or 3 (non-finite bound); *substeps as the reference's diagnostics. */
+int orc_surface_stress(const orc_grid *g, const uint32_t *mask, const float *cells, double dt, float nu, float mu_air,
+                       uint32_t max_substeps, float *u, uint32_t *substeps, double *count_out) {
+    *substeps = 0;
+    const double liquid = g->rho_liquid, ambient = g->rho_air;
+    const double eps = 1.1920928955078125e-07;
+    const double slack = (double)1e-5f + 4.0 * eps;
+    const double floor_rho = fmin(liquid, ambient) - fabs(liquid - ambient) * slack;
+    double count;
+    int proved = 0;
+    if (floor_rho > 0.0) {
+        const double mu = (double)fmaxf(nu * g->rho_liquid, mu_air);
+        const double h = g->h;
+        const double gb = 24.0 * mu / (floor_rho * h * h) * (1.0 + 32.0 * eps);
+        if (dt * gb <= 1.8) {
+            if (gb == 0.0) return 0;
+            count = 1.0;
+            proved = 1;
+        }
+    }
+    if (!proved) {
+        const float bound = orc_stress_bound(g, mask, cells, nu, mu_air);
+        if (!isfinite(bound) || bound < 0.0f) return 3;
+        if (bound == 0.0f) return 0;
+        count = ceil(dt * (double)bound / 1.8);
+        if (count < 1.0) count = 1.0;
+    }
+    if (count_out) *count_out = count;
+    if (count > (double)max_substeps) return 4;
+    const uint64_t F = plane_offset((uint32_t[3]){g->nx, g->ny, g->nz}, 3);
+    float *tmp = (float *)malloc((F + 1) * sizeof(float));
+    const uint32_t n = (uint32_t)count;
+    if (n > 8) {
+        float rows[8 * 256];
+        const int stages = orc_rkc_schedule(dt, n, rows, 256);
+        float *alpha = packet_alpha(g, cells);
+        ctx_t c;
+        ctx_init(&c, g, mask, alpha, 0);
+        const float nu_rho = nu * g->rho_liquid;
+        float *y0 = (float *)malloc((F + 1) * sizeof(float)), *f0 = (float *)malloc((F + 1) * sizeof(float));
+        float *older = (float *)calloc(F + 1, sizeof(float));
+        memcpy(y0, u, F * sizeof(float));
+        for (int axis = 0; axis < 3; axis++) { /* stress_save_initial */
+            uint32_t P[3];
+            plane_dims(c.d, axis, P);
+            const uint64_t off = plane_offset(c.d, axis), np = plane_size(c.d, axis);
+            for (uint64_t k = 0; k < np; k++) {
+                int32_t q[3];
+                decode3(k, P, q);
+                f0[off + k] = open_face(&c, q, axis) ? stress_acc(&c, nu_rho, mu_air, u, q, axis) : 0.0f;
+            }
+        }
+        /* stage k reads Y_k (newest, at neighbours) and Y_{k-1} (its own element only) and writes
+         * Y_{k+1} over Y_{k-1}: two buffers alternate (stabilized.rs:234-247); Y_{-1} = 0 */
+        float *newest = u, *other = older;
+        for (int st = 0; st < stages; st++) {
+            rkc_stage(&c, nu_rho, mu_air, rows + 8 * st, newest, other, y0, f0, other);
+            float *t2 = newest;
+            newest = other;
+            other = t2;
+        }
+        if (newest != u) memcpy(u, newest, F * sizeof(float));
+        *substeps = (uint32_t)stages;
+        free(y0); free(f0); free(alpha); free(older);
+    } else {
+        const float dt_sub = (float)(dt / (double)n);
+        for (uint32_t k = 0; k < n; k++) {
+            orc_stress_euler(g, mask, cells, nu, mu_air, dt_sub, u, tmp);
+            memcpy(u, tmp, F * sizeof(float));
+        }
+        *substeps = n;
+    }
+    free(tmp);
+    return 0;
+}
+
 /* ------------------------------------------------------------------ EM edge lattice
  * src/em/common.wgsl, src/em/grid.rs: edges of axis a have dims n + 1 except n_a on axis a,
  * packed x | y | z; faces of axis a have dims n + e_a. */

@@ -140,6 +140,17 @@ void orc_predict(const orc_grid *g, const uint32_t *mask, const float *faces, co
 void orc_viscous(const orc_grid *g, const uint32_t *mask, const float *pred, const float *cells, const float *em,
                  const orc_step_params *p, float *out);
 
+/* ---- two-phase viscous stress diffusion: diffuse_surface_stress (src/fluid/mod.rs:1146-1234),
+ * src/fluid/viscosity.wgsl, src/fluid/stabilized.{rs,wgsl}.  cells: FluidCell packets (reserved.z =
+ * occupancy); nu: liquid kinematic viscosity (Params.physical.z); mu_air: ambient dynamic viscosity
+ * (Params.surface.w); u: face planes. ---- */
+float orc_stress_bound(const orc_grid *g, const uint32_t *mask, const float *cells, float nu, float mu_air);
+void orc_stress_euler(const orc_grid *g, const uint32_t *mask, const float *cells, float nu, float mu_air,
+                      float dt_sub, const float *u, float *out);
+int orc_rkc_schedule(double dt, uint32_t euler_steps, float *rows, int max_rows);
+int orc_surface_stress(const orc_grid *g, const uint32_t *mask, const float *cells, double dt, float nu, float mu_air,
+                       uint32_t max_substeps, float *u, uint32_t *substeps, double *count);
+
 /* ---- EM Jacobi-CG on the edge curl-curl operator (src/em/induction.wgsl, src/em/mod.rs:863-1010) ---- */
 typedef struct {
     uint32_t nx, ny, nz; /* cells; edges of axis a: dims n + 1 except n_a on axis a, packed x|y|z */

@@ -219,6 +219,42 @@ def viscous(g, mask, pred, cells=None, dt=1e-3, viscosity=0.0, acceleration=(0.0
     return out
 
 
+def stress_bound(g, mask, cells, nu, mu_air):
+    """viscosity_bound + reduce (src/fluid/viscosity.wgsl:83-104)"""
+    L = lib()
+    L.orc_stress_bound.restype = C.c_float
+    cl = np.ascontiguousarray(cells, dtype=np.float32).reshape(-1)
+    return float(L.orc_stress_bound(C.byref(_grid(g)), _ptr(mask), _ptr(cl), C.c_float(nu), C.c_float(mu_air)))
+
+
+def stress_euler(g, mask, cells, u, nu, mu_air, dt_sub):
+    """one explicit substep of stress_diffuse (src/fluid/viscosity.wgsl:105-112)"""
+    out = np.zeros(g.faces, np.float32)
+    cl = np.ascontiguousarray(cells, dtype=np.float32).reshape(-1)
+    lib().orc_stress_euler(C.byref(_grid(g)), _ptr(mask), _ptr(cl), C.c_float(nu), C.c_float(mu_air),
+                           C.c_float(dt_sub), _ptr(np.ascontiguousarray(u, dtype=np.float32)), _ptr(out))
+    return out
+
+
+def rkc_schedule(dt, euler_steps):
+    """stage rows of the damped RKC2 schedule (src/fluid/stabilized.rs:13-53), shape (stages, 8)"""
+    rows = np.zeros((256, 8), np.float32)
+    n = lib().orc_rkc_schedule(C.c_double(dt), C.c_uint32(euler_steps), _ptr(rows), 256)
+    assert n > 0
+    return rows[:n].copy()
+
+
+def surface_stress(g, mask, cells, u, dt, nu, mu_air, max_substeps=4096):
+    """diffuse_surface_stress (src/fluid/mod.rs:1160-1234) -> (status, u_new, substeps, euler count)"""
+    out = np.ascontiguousarray(np.array(u, dtype=np.float32))
+    sub = C.c_uint32(0)
+    cnt = C.c_double(0)
+    cl = np.ascontiguousarray(cells, dtype=np.float32).reshape(-1)
+    rc = lib().orc_surface_stress(C.byref(_grid(g)), _ptr(mask), _ptr(cl), C.c_double(dt), C.c_float(nu),
+                                  C.c_float(mu_air), C.c_uint32(max_substeps), _ptr(out), C.byref(sub), C.byref(cnt))
+    return rc, out, int(sub.value), float(cnt.value)
+
+
 class _EmGrid(C.Structure):
     _fields_ = [("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32), ("h", C.c_float)]
 
