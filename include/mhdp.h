@@ -184,6 +184,23 @@ int mhdp_predict_dev(mhdp_ctx *ctx, const float *u_dev, const mhdp_step_params *
  * [8..10]).  u_star must not alias u_pred. */
 int mhdp_viscous_dev(mhdp_ctx *ctx, const float *u_pred_dev, const float *cells_dev, const float *em_dev,
                      const mhdp_step_params *params, float *u_star_dev);
+/* diffuse_surface_stress (src/fluid/mod.rs:1146-1234; src/fluid/viscosity.wgsl, stabilized.wgsl):
+ * the two-phase viscous stress step the reference runs between predict and viscous.  Normal
+ * stresses at cells, shear stresses at edges (odd ghosts: zero velocity at closed walls), divided
+ * by h and the face density; explicit substeps of dt / count when count = ceil(dt bound / 1.8) <=
+ * 8, else damped RKC2 stages.  u_dev: the predicted faces (three planes), rewritten in place;
+ * cells: FluidCell packets (reserved.z = occupancy).  *substeps: the reference's
+ * viscous_substeps.  Errors with the reference's texts: a count above max_substeps (MHDP_INVALID,
+ * u untouched), a non-finite bound (MHDP_NONFINITE). */
+typedef struct {
+    double dt;             /* FluidConfig.dt_s */
+    float viscosity;       /* liquid kinematic viscosity (Params.physical.z) */
+    float mu_air;          /* ambient dynamic viscosity (Params.surface.w) */
+    uint32_t max_substeps; /* FluidConfig.max_viscous_substeps, 1..4096 */
+} mhdp_stress_params;
+int mhdp_surface_stress_dev(mhdp_ctx *ctx, float *u_dev, const float *cells_dev, const mhdp_stress_params *params,
+                            uint32_t *substeps);
+
 /* ---- EM step: Jacobi-CG on the edge curl-curl operator (src/em/induction.wgsl; the solve of
  * EmState::advance_to, src/em/mod.rs:863-1010; coefficient set-up of EmState::new,
  * mod.rs:557-571).  Edges of axis a have dims n + 1 except n_a on axis a, packed x|y|z, one f32

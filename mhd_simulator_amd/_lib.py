@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libmhdp.so")
 SYMBOLS = (
     "mhdp_create", "mhdp_set_stream", "mhdp_set_cells", "mhdp_get_mask", "mhdp_project", "mhdp_project_dev",
     "mhdp_set_potential",
-    "mhdp_project_batch", "mhdp_fields_dev", "mhdp_predict_dev", "mhdp_viscous_dev",
+    "mhdp_project_batch", "mhdp_fields_dev", "mhdp_predict_dev", "mhdp_viscous_dev", "mhdp_surface_stress_dev",
     "mhdp_em_create", "mhdp_em_set_materials", "mhdp_em_solve_dev", "mhdp_em_get_coefficients", "mhdp_em_probe_apply",
     "mhdp_em_edges", "mhdp_em_last_error", "mhdp_em_destroy", "mhdp_last_error", "mhdp_destroy", "mhdp_device_bytes",
     "mhdp_anchors", "mhdp_hierarchy", "mhdp_set_kernel_timing", "mhdp_kernel_time",
@@ -74,6 +74,10 @@ class StepParams(C.Structure):
                 ("surface_tension", C.c_float), ("advect", C.c_int)]
 
 
+class StressParams(C.Structure):
+    _fields_ = [("dt", C.c_double), ("viscosity", C.c_float), ("mu_air", C.c_float), ("max_substeps", C.c_uint32)]
+
+
 class EmOpts(C.Structure):
     _fields_ = [("dt", C.c_float), ("mu0", C.c_float), ("rel_tol", C.c_float), ("max_iter", C.c_uint32),
                 ("magnetostatic", C.c_int)]
@@ -117,6 +121,7 @@ def load(path: str = LIB_PATH):
         "mhdp_fields_dev": (I, [P, P, P, P, C.POINTER(FieldsParams), P, C.POINTER(Diagnostics)]),
         "mhdp_predict_dev": (I, [P, P, C.POINTER(StepParams), P]),
         "mhdp_viscous_dev": (I, [P, P, P, P, C.POINTER(StepParams), P]),
+        "mhdp_surface_stress_dev": (I, [P, P, P, C.POINTER(StressParams), P]),
         "mhdp_em_create": (I, [C.POINTER(P), C.POINTER(Grid)]),
         "mhdp_em_set_materials": (I, [P, P, P, U32, I, P]),
         "mhdp_em_solve_dev": (I, [P, C.POINTER(EmOpts), P, P, P, P, P, C.POINTER(EmStats)]),

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstdio>
@@ -62,6 +63,9 @@ struct mhdp_ctx {
     double* fields_part = nullptr;  // fields(): per-block totals, then the 10 folded totals
     float* faces_open = nullptr;    // predict(): open-face copy of its input (one float per face)
     uint8_t* open8 = nullptr;       // step kernels: per cell, bit a = lower face on axis a open
+    float *stress_mu = nullptr, *stress_rho = nullptr, *stress_part = nullptr;  // viscous stress diffusion
+    float *faces_b = nullptr, *faces_y0 = nullptr, *faces_f0 = nullptr;        // its face buffers
+    float* stress_tensor = nullptr;                                            // and its stress tensor
     bool open8_fresh = false;       // open8 matches the current mask
     void *bx16 = nullptr, *by16 = nullptr, *bz16 = nullptr, *dinv16 = nullptr;
     // multigrid
@@ -1296,6 +1300,137 @@ static int step_check(mhdp_ctx* c, const mhdp_step_params* sp) {
     return MHDP_OK;
 }
 
+// RKC2 stage rows of the damped Runge-Kutta-Chebyshev method (Verwer, Hundsdorfer & Sommeijer
+// 1990, damping 0.05) as the reference schedules them (src/fluid/stabilized.rs:13-53): the fewest
+// stages whose stability interval covers 1.8 x euler_steps; rows of (mu, nu, 1 - mu - nu,
+// operator weight, initial-operator weight).
+static std::vector<std::array<float, 5>> rkc_schedule(double dt, uint32_t euler_steps) {
+    const double damping = 0.05, stiffness = 1.8 * (double)euler_steps;
+    const int capacity = (int)std::ceil(3.0 * std::sqrt((double)euler_steps)) + 2;
+    for (int s = 2; s <= capacity; s++) {
+        const double w0 = 1.0 + damping / ((double)s * s);
+        std::vector<double> t(s + 1), d(s + 1), dd(s + 1), b(s + 1), a(s + 1);
+        t[0] = 1.0; t[1] = w0; d[0] = 0.0; d[1] = 1.0; dd[0] = 0.0; dd[1] = 0.0;
+        for (int j = 2; j <= s; j++) {
+            t[j] = 2.0 * w0 * t[j - 1] - t[j - 2];
+            d[j] = 2.0 * t[j - 1] + 2.0 * w0 * d[j - 1] - d[j - 2];
+            dd[j] = 4.0 * d[j - 1] + 2.0 * w0 * dd[j - 1] - dd[j - 2];
+        }
+        const double w1 = d[s] / dd[s];
+        if (stiffness > 0.9 * (1.0 + w0) / w1) continue;
+        for (int j = 2; j <= s; j++) b[j] = dd[j] / (d[j] * d[j]);
+        b[0] = b[2];
+        b[1] = b[2];
+        for (int j = 0; j <= s; j++) a[j] = 1.0 - b[j] * t[j];
+        std::vector<std::array<float, 5>> rows;
+        rows.push_back({1.0f, 0.0f, 0.0f, (float)(dt * b[1] * w1), 0.0f});
+        for (int j = 2; j <= s; j++) {
+            const float mu = (float)(2.0 * w0 * b[j] / b[j - 1]);
+            const float nu = (float)(-b[j] / b[j - 2]);
+            const double op = dt * 2.0 * w1 * b[j] / b[j - 1];
+            rows.push_back({mu, nu, 1.0f - mu - nu, (float)op, (float)(-a[j - 1] * op)});
+        }
+        return rows;
+    }
+    return {};
+}
+
+int mhdp_surface_stress_dev(mhdp_ctx* c, float* u, const float* cells, const mhdp_stress_params* sp, uint32_t* substeps) {
+    if (!c) return MHDP_INVALID;
+    c->err.clear();
+    if (substeps) *substeps = 0;
+    if (!c->cells) return fail(c, MHDP_INVALID, "Fluid cells are not set");
+    if (!c->phase.two_phase) return fail(c, MHDP_INVALID, "Surface stress diffusion needs a two-phase context");
+    if (!u || !cells || !sp) return fail(c, MHDP_INVALID, "Missing surface stress argument");
+    if (!(sp->dt > 0.0) || !std::isfinite(sp->dt) || !(sp->viscosity >= 0.0f) || !std::isfinite(sp->viscosity) ||
+        !(sp->mu_air >= 0.0f) || !std::isfinite(sp->mu_air) || sp->max_substeps < 1 || sp->max_substeps > 4096)
+        return fail(c, MHDP_INVALID, "Invalid surface stress parameters");
+    if (face_count(c->grid) >= (1ull << 32)) return fail(c, MHDP_INVALID, "Too many faces for the step kernels");
+    int rc;
+    if (!c->open8_fresh) {
+        if ((rc = dalloc(c, (void**)&c->open8, c->g.n))) return rc;
+        launch_open8(c->stream, c->g, c->mask, c->open8);
+        c->open8_fresh = true;
+    }
+    const size_t F = face_count(c->grid);
+    if ((rc = dalloc(c, (void**)&c->stress_mu, (size_t)c->g.n * 4)) ||
+        (rc = dalloc(c, (void**)&c->stress_rho, (size_t)c->g.n * 4)) ||
+        (rc = dalloc(c, (void**)&c->stress_part, ((size_t)stress_bound_blocks(c->g) + 1) * 4)) ||
+        (rc = dalloc(c, (void**)&c->faces_b, F * 4)) ||
+        (rc = dalloc(c, (void**)&c->stress_tensor, stress_tensor_len(c->g) * 4)))
+        return rc;
+    const float h = c->grid.h, rl = c->phase.rho_liquid, ra = c->phase.rho_air;
+    const float nu = sp->viscosity, mua = sp->mu_air;
+    const double dt = sp->dt;
+    // global_viscous_bound (mod.rs:1146-1158): one substep is proved safe without a reduction
+    const double eps = 1.1920928955078125e-07;
+    const double slack = (double)1e-5f + 4.0 * eps;  // surface::FRACTION_TOLERANCE + 4 f32 eps
+    const double floor_rho = std::min((double)rl, (double)ra) - std::fabs((double)rl - (double)ra) * slack;
+    double count = 0.0;
+    bool proved = false;
+    if (floor_rho > 0.0) {
+        const double mu = (double)std::max(nu * rl, mua);
+        const double gb = 24.0 * mu / (floor_rho * (double)h * (double)h) * (1.0 + 32.0 * eps);
+        if (dt * gb <= 1.8) {
+            if (gb == 0.0) return MHDP_OK;
+            count = 1.0;
+            proved = true;
+        }
+    }
+    launch_stress_cells(c->stream, c->g, c->mask, cells, nu, mua, rl, ra, c->stress_mu, c->stress_rho);
+    const StressBufs sb{h, nu, mua, rl, ra, c->mask, c->open8, c->stress_mu, c->stress_rho, c->stress_tensor};
+    if (!proved) {
+        const uint32_t nb = stress_bound_blocks(c->g);
+        launch_stress_bound(c->stream, c->g, sb, c->stress_part, c->stress_part + nb);
+        float bound = 0.0f;
+        HIP_OK(hipMemcpyAsync(&bound, c->stress_part + nb, 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        if (!std::isfinite(bound) || bound < 0.0f) return fail(c, MHDP_NONFINITE, "Nonfinite viscous stability bound");
+        if (bound == 0.0f) return MHDP_OK;
+        count = std::max(std::ceil(dt * (double)bound / 1.8), 1.0);
+    }
+    if (count > (double)sp->max_substeps) {
+        char buf[256];
+        std::snprintf(buf, sizeof buf,
+                      "Viscosity requires %.0f equivalent Euler substeps, above the %u work limit; reduce the common "
+                      "timestep or increase Viscosity work limit",
+                      count, sp->max_substeps);
+        return fail(c, MHDP_INVALID, buf);
+    }
+    const uint32_t n = (uint32_t)count;
+    if (n > 8) {  // RKC2 (stabilized.rs:199-250)
+        const auto rows = rkc_schedule(dt, n);
+        if (rows.empty()) return fail(c, MHDP_INVALID, "RKC stage-capacity bound");
+        if ((rc = dalloc(c, (void**)&c->faces_y0, F * 4)) || (rc = dalloc(c, (void**)&c->faces_f0, F * 4))) return rc;
+        HIP_OK(hipMemcpyAsync(c->faces_y0, u, F * 4, hipMemcpyDeviceToDevice, c->stream));
+        launch_stress_tensor(c->stream, c->g, sb, 1, u);
+        launch_stress_initial(c->stream, c->g, sb, u, c->faces_f0);
+        HIP_OK(hipMemsetAsync(c->faces_b, 0, F * 4, c->stream));
+        float* newest = u;
+        float* other = c->faces_b;
+        for (size_t i = 0; i < rows.size(); i++) {
+            if (i > 0) launch_stress_tensor(c->stream, c->g, sb, 1, newest);  // stage 0 reads u's tensor
+            launch_stress_stage(c->stream, c->g, sb, rows[i].data(), newest, other, c->faces_y0, c->faces_f0, other);
+            std::swap(newest, other);
+        }
+        if (newest != u) HIP_OK(hipMemcpyAsync(u, newest, F * 4, hipMemcpyDeviceToDevice, c->stream));
+        if (substeps) *substeps = (uint32_t)rows.size();
+    } else {  // explicit substeps of dt / count (mod.rs:1219-1233)
+        const float dt_sub = (float)(dt / (double)n);
+        for (uint32_t k = 0; k < n; k++) {
+            const float* src = (k % 2 == 0) ? u : c->faces_b;
+            float* dst = (k % 2 == 0) ? c->faces_b : u;
+            launch_stress_tensor(c->stream, c->g, sb, 0, src);
+            launch_stress_euler(c->stream, c->g, sb, dt_sub, src, dst);
+        }
+        if (n % 2 == 1) HIP_OK(hipMemcpyAsync(u, c->faces_b, F * 4, hipMemcpyDeviceToDevice, c->stream));
+        if (substeps) *substeps = n;
+    }
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return MHDP_OK;
+}
+
 int mhdp_predict_dev(mhdp_ctx* c, const float* u, const mhdp_step_params* sp, float* out) {
     if (!c) return MHDP_INVALID;
     c->err.clear();
@@ -1460,7 +1595,7 @@ void mhdp_destroy(mhdp_ctx* c) {
     else (void)hipDeviceSynchronize();
     void* ptrs[] = {c->mask, c->alpha, c->bx, c->by, c->bz, c->diag, c->dinv, c->b, c->x, c->r, c->r2, c->p, c->p2, c->q, c->z,
                     c->t, c->x64, c->b64, c->bx16, c->by16, c->bz16, c->dinv16, c->sc, c->part, c->ustar_stage,
-                    c->uout_stage, c->phi_stage, c->fluid8, c->cc_counts, c->phi_prev, c->uwarm, c->fields_part, c->faces_open, c->open8};
+                    c->uout_stage, c->phi_stage, c->fluid8, c->cc_counts, c->phi_prev, c->uwarm, c->fields_part, c->faces_open, c->open8, c->stress_mu, c->stress_rho, c->stress_part, c->faces_b, c->faces_y0, c->faces_f0, c->stress_tensor};
     for (void* p : ptrs) dfree(c, p);
     for (CoarseArrays& a : c->coarse) {
         void* q[] = {a.bx, a.by, a.bz, a.react, a.diag, a.dinv, a.bx16, a.by16, a.bz16, a.react16, a.dinv16,

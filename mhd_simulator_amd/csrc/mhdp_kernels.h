@@ -173,6 +173,28 @@ void launch_viscous(hipStream_t s, const Grid3& g, float h, float rho_l, float r
                     float viscosity, const float acc[3], float sigma, const uint8_t* open8, const float* pred,
                     const float* cells, const float* em, float* out);
 
+// step.hip — two-phase viscous stress diffusion (src/fluid/viscosity.wgsl, stabilized.wgsl): per cell
+// viscosity and density, the stability bound (part: stress_bound_blocks floats), the stress tensor
+// of a face field (cached: the RKC2 stages' form), one explicit substep, the RKC2 initial operator
+// and one stage (row: mu, nu, 1 - mu - nu, operator, initial); the last three read the tensor
+struct StressBufs {
+    float h, nu, mu_air, rho_l, rho_air;
+    const uint32_t* mask;
+    const uint8_t* open8;  // bit 3: fluid cell
+    const float *mu, *rho;
+    float* tensor;  // stress_tensor_len floats
+};
+size_t stress_tensor_len(const Grid3& g);
+void launch_stress_cells(hipStream_t s, const Grid3& g, const uint32_t* mask, const float* cells, float nu, float mu_air,
+                         float rho_l, float rho_air, float* mu, float* rho);
+uint32_t stress_bound_blocks(const Grid3& g);
+void launch_stress_bound(hipStream_t s, const Grid3& g, const StressBufs& b, float* part, float* out);
+void launch_stress_tensor(hipStream_t s, const Grid3& g, const StressBufs& b, int cached, const float* u);
+void launch_stress_euler(hipStream_t s, const Grid3& g, const StressBufs& b, float dt, const float* u, float* out);
+void launch_stress_initial(hipStream_t s, const Grid3& g, const StressBufs& b, const float* u, float* f0);
+void launch_stress_stage(hipStream_t s, const Grid3& g, const StressBufs& b, const float row[5], const float* prev,
+                         const float* older, const float* y0, const float* f0, float* out);
+
 // em.hip — the EM step's Jacobi-CG on the edge curl-curl operator (src/em/induction.wgsl).
 // Edges: per axis a dims n + 1 except n_a, packed x | y | z; faces: dims n + e_a, packed.
 struct EmDims {

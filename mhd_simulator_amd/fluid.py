@@ -219,6 +219,16 @@ class PressureProjection:
         self._check(self.lib.mhdp_viscous_dev(self.handle, _ptr(u_pred), _ptr(cells), _ptr(em), C.byref(sp),
                                               _ptr(u_out)))
 
+    def surface_stress(self, u, cells, *, dt, viscosity, mu_air, max_substeps=4096):
+        """diffuse_surface_stress (two-phase): u (device faces) in place; returns the substep count."""
+        self._faces_dev(u)
+        if not hasattr(cells, "data_ptr") or cells.numel() != 16 * self.grid.cells:
+            raise ValueError("cells holds 16 floats per cell")
+        sp = _lib.StressParams(dt, viscosity, mu_air, max_substeps)
+        n = C.c_uint32(0)
+        self._check(self.lib.mhdp_surface_stress_dev(self.handle, _ptr(u), _ptr(cells), C.byref(sp), C.byref(n)))
+        return int(n.value)
+
     # -- fields() + diagnostics() (solver.wgsl:256-291, read_diagnostics mod.rs:1116-1146)
     def fields(self, u_faces, phi, cells, *, dt, acceleration=(0.0, 0.0, 0.0), mushy=False, gauge="auto",
                em=None):

@@ -330,6 +330,28 @@ static float stress_acc(const ctx_t *c, float nu_rho, float mu_air, const float 
     return div / (c->h * face_density(c, q, i));
 }
 
+/* the cached form the RKC2 stages use (viscosity.wgsl:56-76, tensor_normal / tensor_shear): shear
+ * stresses come from the per-cell tensor, which holds each edge's stress once with the component
+ * pair in ascending order (xy, xz, yz); positions outside the grid compute them directly. */
+static float shear_cached(const ctx_t *c, float nu_rho, float mu_air, const float *u, const int32_t q[3], int i, int j) {
+    int32_t at[3] = {q[0], q[1], q[2]};
+    wrap_pos(c->periodic, c->d, at);
+    if (!valid_pos(c->d, at)) return shear_stress(c, nu_rho, mu_air, u, q, i, j);
+    return i < j ? shear_stress(c, nu_rho, mu_air, u, at, i, j) : shear_stress(c, nu_rho, mu_air, u, at, j, i);
+}
+static float stress_acc_cached(const ctx_t *c, float nu_rho, float mu_air, const float *u, const int32_t q[3], int i) {
+    int32_t left[3] = {q[0], q[1], q[2]};
+    left[i] -= 1;
+    float div = normal_stress(c, nu_rho, mu_air, u, q, i) - normal_stress(c, nu_rho, mu_air, u, left, i);
+    for (int j = 0; j < 3; j++) {
+        if (j == i) continue;
+        int32_t upper[3] = {q[0], q[1], q[2]};
+        upper[j] += 1;
+        div += shear_cached(c, nu_rho, mu_air, u, upper, i, j) - shear_cached(c, nu_rho, mu_air, u, q, i, j);
+    }
+    return div / (c->h * face_density(c, q, i));
+}
+
 static float *packet_alpha(const orc_grid *g, const float *cells) {
     const uint64_t n = (uint64_t)g->nx * g->ny * g->nz;
     float *alpha = (float *)malloc((n + 1) * sizeof(float));
@@ -452,7 +474,7 @@ static void rkc_stage(const ctx_t *c, float nu_rho, float mu_air, const float *r
                 wrap_pos(c->periodic, c->d, w);
                 const uint64_t canon = face_idx(c->d, axis, w);
                 v = row[0] * face_value(c, prev, q, axis) + row[1] * older[off + k] + row[2] * y0[canon] +
-                    row[3] * stress_acc(c, nu_rho, mu_air, prev, q, axis) + row[4] * f0[canon];
+                    row[3] * stress_acc_cached(c, nu_rho, mu_air, prev, q, axis) + row[4] * f0[canon];
             }
             out[off + k] = v;
         }
@@ -509,7 +531,7 @@ int orc_surface_stress(const orc_grid *g, const uint32_t *mask, const float *cel
             for (uint64_t k = 0; k < np; k++) {
                 int32_t q[3];
                 decode3(k, P, q);
-                f0[off + k] = open_face(&c, q, axis) ? stress_acc(&c, nu_rho, mu_air, u, q, axis) : 0.0f;
+                f0[off + k] = open_face(&c, q, axis) ? stress_acc_cached(&c, nu_rho, mu_air, u, q, axis) : 0.0f;
             }
         }
         /* stage k reads Y_k (newest, at neighbours) and Y_{k-1} (its own element only) and writes
