@@ -270,7 +270,7 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--workload", default="plate512", choices=["plate512", "batch64", "step512", "em128"],
                     help="plate512: the 512^3 headline; batch64: 64 uneven droplet problems in one batch call; "
-                         "step512: the whole fluid step (predict, viscous, projection, fields); "
+                         "step512: the whole fluid step (predict, viscous stresses, viscous, projection, fields); "
                          "em128: the EM step's Jacobi-CG")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device-to-device 5 times (calibrates HBM byte counters)")
