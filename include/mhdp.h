@@ -87,6 +87,12 @@ typedef struct {
                             (pressure_commit, solver.wgsl:181-183).  MHDP_MIXED: the FP64 refinement
                             starts from it (the same sum, without rounding the corrected u* to
                             FP32 faces).  Ignored until a previous potential exists. */
+    int coarse_visits;   /* multigrid: visits of level 1 per cycle, at most 4.  1 = the reference's
+                            V-cycle (multigrid.rs:67-82); 2 = a W-cycle at level 1 (the second
+                            visit solves for the residual the first leaves and is added damped by
+                            0.8); 0 = automatic: 2 on grids of at least 2^24 cells, else 1 (below
+                            that the second visit's small coarse launches cost more than the
+                            iterations it saves) */
 } mhdp_opts;
 
 typedef struct {
@@ -272,10 +278,10 @@ int mhdp_kernel_time(const mhdp_ctx *ctx, double *total_ms, uint32_t *launches);
 int mhdp_probe_coefficients(mhdp_ctx *ctx, int meanfree, float *beta3, float *diag, float *dinv);
 /* apply(): q = A p with the cached operator (identity rows at pinned anchors). */
 int mhdp_probe_apply(mhdp_ctx *ctx, int meanfree, const float *p, float *q);
-/* One V-cycle z = M r (precision MHDP_FP32 or MHDP_MIXED storage) with the given smoother
- * weight and coarse-correction scale (2/3 and 1 reproduce the reference's cycle). */
+/* One cycle z = M r (precision MHDP_FP32 or MHDP_MIXED storage) with the given smoother weight,
+ * coarse-correction scale and level-1 visits (2/3, 1 and 1 reproduce the reference's V-cycle). */
 int mhdp_probe_vcycle(mhdp_ctx *ctx, int meanfree, int precision, float smooth_weight, float coarse_scale,
-                      const float *r, float *z);
+                      int coarse_visits, const float *r, float *z);
 /* Coarse graph of level l >= 1 (lower-slot edges x,y,z, reaction, diagonal: 5 floats/cell). */
 int mhdp_probe_mg_graph(mhdp_ctx *ctx, int meanfree, int level, float *graph5, uint32_t *dims4);
 /* Divergence right-hand side b = -h^2 div u* (physical, no anchor zeroing), FP64. */

@@ -42,7 +42,7 @@ class Opts(C.Structure):
     _fields_ = [("rel_tol", C.c_float), ("abs_div_tol", C.c_float), ("max_iter", C.c_uint32),
                 ("precond", C.c_int), ("precision", C.c_int), ("gauge", C.c_int), ("refine_tol", C.c_double),
                 ("max_outer", C.c_uint32), ("inner_tol", C.c_float), ("smooth_weight", C.c_float),
-                ("coarse_scale", C.c_float), ("warm_start", C.c_int)]
+                ("coarse_scale", C.c_float), ("warm_start", C.c_int), ("coarse_visits", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -139,7 +139,7 @@ def load(path: str = LIB_PATH):
         "mhdp_hierarchy": (I, [P, P, I, P]),
         "mhdp_probe_coefficients": (I, [P, I, P, P, P]),
         "mhdp_probe_apply": (I, [P, I, P, P]),
-        "mhdp_probe_vcycle": (I, [P, I, I, C.c_float, C.c_float, P, P]),
+        "mhdp_probe_vcycle": (I, [P, I, I, C.c_float, C.c_float, I, P, P]),
         "mhdp_probe_mg_graph": (I, [P, I, I, P, P]),
         "mhdp_probe_rhs": (I, [P, I, P, P]),
     }

@@ -31,6 +31,7 @@ struct CoarseArrays {
     float *bx = nullptr, *by = nullptr, *bz = nullptr, *react = nullptr, *diag = nullptr, *dinv = nullptr;
     void *bx16 = nullptr, *by16 = nullptr, *bz16 = nullptr, *react16 = nullptr, *dinv16 = nullptr;
     void *r = nullptr, *z = nullptr, *t = nullptr;
+    void* w = nullptr;  // level 1 only: the second coarse visit's correction (W-cycle)
 };
 
 }  // namespace
@@ -85,6 +86,7 @@ struct mhdp_ctx {
     bool fused_enabled = true;  // MHDP_UNFUSED=1 selects the one-kernel-per-stage path
     bool brick_enabled = true;  // MHDP_NO_BRICK=1 keeps the 4-cell-group fused kernels
     bool march_enabled = true;  // z-marching fine passes where the grid allows (MHDP_NO_MARCH=1: bricks)
+    int coarse_visits = 1;      // level-1 visits per cycle (mhdp_opts.coarse_visits)
     KernelTimer timer;
     double timer_ms = 0.0;
     uint32_t timer_launches = 0;
@@ -195,6 +197,8 @@ int validate_opts(mhdp_ctx* c, const mhdp_opts* o) {
         return fail(c, MHDP_INVALID, "Invalid pressure solver selection");
     if (!(o->smooth_weight >= 0.0f && o->smooth_weight < 1.0f) || !(o->coarse_scale >= 0.0f && o->coarse_scale < 2.0f))
         return fail(c, MHDP_INVALID, "Invalid V-cycle smoother weight or coarse-correction scale");
+    if (o->coarse_visits < 0 || o->coarse_visits > 4)
+        return fail(c, MHDP_INVALID, "Invalid multigrid coarse visits");
     if (o->precision == MHDP_MIXED &&
         (!(o->refine_tol > 0.0) || !(o->refine_tol < 1.0) || o->inner_tol < 0.0f || o->inner_tol >= 1.0f))
         return fail(c, MHDP_INVALID, "Invalid refinement tolerance");
@@ -213,7 +217,8 @@ int alloc_mg(mhdp_ctx* c, bool f16) {
             if ((rc = dalloc(c, (void**)&a.bx, fb)) || (rc = dalloc(c, (void**)&a.by, fb)) ||
                 (rc = dalloc(c, (void**)&a.bz, fb)) || (rc = dalloc(c, (void**)&a.react, fb)) ||
                 (rc = dalloc(c, (void**)&a.diag, fb)) || (rc = dalloc(c, (void**)&a.dinv, fb)) ||
-                (rc = dalloc(c, &a.r, fb)) || (rc = dalloc(c, &a.z, fb)) || (rc = dalloc(c, &a.t, fb)))
+                (rc = dalloc(c, &a.r, fb)) || (rc = dalloc(c, &a.z, fb)) || (rc = dalloc(c, &a.t, fb)) ||
+                (l == 1 && (rc = dalloc(c, &a.w, fb))))
                 return rc;
         }
         c->mg_alloc = true;
@@ -282,7 +287,7 @@ std::vector<LevelDev> level_view(mhdp_ctx* c) {
         d.g = c->levels[l];
         d.bx = a.bx; d.by = a.by; d.bz = a.bz; d.react = a.react; d.diag = a.diag; d.dinv = a.dinv;
         d.bx16 = a.bx16; d.by16 = a.by16; d.bz16 = a.bz16; d.react16 = a.react16; d.dinv16 = a.dinv16;
-        d.r = a.r; d.z = a.z; d.t = a.t;
+        d.r = a.r; d.z = a.z; d.t = a.t; d.w = a.w;
     }
     return lv;
 }
@@ -350,7 +355,7 @@ void vcycle_fused(mhdp_ctx* c, Solve& S, int mode, bool update = false) {
         launch_fused_restrict(s, S.store, lv[1].g, lv[0].g, c->r, dinv, bx, by, bz, lv[1].r, c->sc);
     c->timer.after(MHDP_K_RESTRICT, s);
     void* z1 = nullptr;
-    launch_vcycle_coarse(s, S.store, L, lv, c->sc, S.coarse64, &z1);
+    launch_vcycle_coarse(s, S.store, L, lv, c->sc, S.coarse64, c->coarse_visits, &z1);
     record_mark();
     c->timer.before(MHDP_K_SMOOTH, s);
     if (march)
@@ -377,7 +382,8 @@ void mg_initialize(mhdp_ctx* c, Solve& S) {  // multigrid.rs:287-291
         return;
     }
     void* zf = nullptr;
-    launch_vcycle(c->stream, S.store, (int)S.lv.size(), S.lv.data(), c->r, c->mask, c->sc, S.coarse64, &zf);
+    launch_vcycle(c->stream, S.store, (int)S.lv.size(), S.lv.data(), c->r, c->mask, c->sc, S.coarse64, c->coarse_visits,
+                  &zf);
     launch_pdot(c->stream, c->g, S.store, c->r, zf, c->dinv, c->p, c->part, c->sc, 1, FIN_INITIAL_MG);
 }
 
@@ -412,7 +418,8 @@ void iterate_once(mhdp_ctx* c, Solve& S) {  // mod.rs:1057-1075, multigrid.rs:30
     if (S.meanfree) launch_subspace(s, c->g, c->r, c->p, c->dinv, c->part, c->sc, 0, S.mg ? FIN_STASH_RR : FIN_BETA);
     if (S.mg) {
         void* zf = nullptr;
-        launch_vcycle(s, S.store, (int)S.lv.size(), S.lv.data(), c->r, c->mask, c->sc, S.coarse64, &zf, &c->timer);
+        launch_vcycle(s, S.store, (int)S.lv.size(), S.lv.data(), c->r, c->mask, c->sc, S.coarse64, c->coarse_visits, &zf,
+                      &c->timer);
         launch_pdot(s, c->g, S.store, c->r, zf, c->dinv, c->p, c->part, c->sc, 0, FIN_BETA_MG);
         launch_psearch(s, c->g, S.store, zf, c->dinv, c->p, c->sc);
     } else {
@@ -422,6 +429,15 @@ void iterate_once(mhdp_ctx* c, Solve& S) {  // mod.rs:1057-1075, multigrid.rs:30
 
 constexpr double kTunedOmega = 0.9;   // smoother weight picked by the V-cycle tuner
 constexpr double kTunedScale = 1.6;   // coarse-correction scale picked by the V-cycle tuner
+// Level-1 visits per cycle when the caller leaves the choice (coarse_visits = 0): a second visit
+// (W-cycle at level 1) saves about a third of the iterations, but its coarse levels are launch-bound
+// small kernels; measured on the plate workload it pays from 256^3 (12.1 vs 12.7 ms; 512^3: 75 vs 83
+// ms) and costs 7-20% below 192^3.
+constexpr uint32_t kTwoVisitMinCells = 1u << 24;
+int coarse_visits_for(const mhdp_ctx* c, int requested) {
+    if (requested > 0) return requested;
+    return c->g.n >= kTwoVisitMinCells ? 2 : 1;
+}
 
 void set_solve_params(mhdp_ctx* c, Scal& s, bool meanfree, double tol2, double abs2, double bound,
                       double omega = kTunedOmega, double pscale = kTunedScale) {
@@ -594,6 +610,7 @@ int project_impl(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, float* uou
 
     const double omega = o->smooth_weight > 0.0f ? (double)o->smooth_weight : kTunedOmega;
     const double pscale = o->coarse_scale > 0.0f ? (double)o->coarse_scale : kTunedScale;
+    c->coarse_visits = coarse_visits_for(c, o->coarse_visits);
     Solve S;
     S.meanfree = meanfree;
     S.store = mixed ? STORE_BF16 : STORE_F32;
@@ -888,6 +905,7 @@ int project_lockstep(mhdp_ctx* const* ctxs, int count, const float* const* ustar
         mhdp_ctx* c = ctxs[p];
         c->err.clear();
         int rc = validate_opts(c, o);
+        if (!rc) c->coarse_visits = coarse_visits_for(c, o->coarse_visits);
         if (!rc && !ustar[p]) rc = fail(c, MHDP_INVALID, "Missing candidate face velocities");
         if (!rc && ((rc = dalloc(c, (void**)&c->x64, (size_t)c->g.n * 8)) ||
                     (rc = dalloc(c, (void**)&c->b64, (size_t)c->g.n * 8)) ||
@@ -1599,7 +1617,7 @@ void mhdp_destroy(mhdp_ctx* c) {
     for (void* p : ptrs) dfree(c, p);
     for (CoarseArrays& a : c->coarse) {
         void* q[] = {a.bx, a.by, a.bz, a.react, a.diag, a.dinv, a.bx16, a.by16, a.bz16, a.react16, a.dinv16,
-                     a.r, a.z, a.t};
+                     a.r, a.z, a.t, a.w};
         for (void* p : q) dfree(c, p);
     }
     for (hipEvent_t e : c->timer.ev) (void)hipEventDestroy(e);
@@ -1642,8 +1660,9 @@ int mhdp_probe_apply(mhdp_ctx* c, int meanfree, const float* p, float* q) {
 }
 
 int mhdp_probe_vcycle(mhdp_ctx* c, int meanfree, int precision, float smooth_weight, float coarse_scale,
-                      const float* r, float* z) {
-    if (!c || !r || !z) return MHDP_INVALID;
+                      int coarse_visits, const float* r, float* z) {
+    if (!c || !r || !z || coarse_visits < 0 || coarse_visits > 4) return MHDP_INVALID;
+    c->coarse_visits = coarse_visits_for(c, coarse_visits);
     const bool f16 = precision == MHDP_MIXED;
     int rc = prepare(c, meanfree != 0);
     if (rc) return rc;
@@ -1666,7 +1685,8 @@ int mhdp_probe_vcycle(mhdp_ctx* c, int meanfree, int precision, float smooth_wei
         vcycle_fused(c, S, FIN_NONE);
         zf = S.lv[0].t;
     } else {
-        launch_vcycle(c->stream, S.store, (int)S.lv.size(), S.lv.data(), c->r, c->mask, c->sc, S.coarse64, &zf);
+        launch_vcycle(c->stream, S.store, (int)S.lv.size(), S.lv.data(), c->r, c->mask, c->sc, S.coarse64, c->coarse_visits,
+                  &zf);
     }
     if (f16) {
         launch_from_bf16(c->stream, c->g.n, zf, c->q);

@@ -19,6 +19,7 @@ struct LevelDev {
     float *bx, *by, *bz, *react, *diag, *dinv;  // FP32 graph (level 0: fine mobilities)
     void *bx16, *by16, *bz16, *react16, *dinv16; // BF16 copies (mixed precision)
     void *r, *z, *t;                              // level vectors (storage type of the cycle)
+    void* w = nullptr;                            // level 1: second-visit correction (W-cycle)
 };
 
 // setup.hip
@@ -92,11 +93,14 @@ struct KernelTimer {
 
 // mg.hip — one V-cycle z = M r on the fine level (cycle_stages, multigrid.rs:67-82)
 void launch_vcycle(hipStream_t s, int store, int L, LevelDev* lv, const float* r0, const uint32_t* mask, Scal* sc,
-                   int coarse_f64, void** zfinal, KernelTimer* timer = nullptr);
+                   int coarse_f64, int visits, void** zfinal, KernelTimer* timer = nullptr);
 
 // mg.hip — the coarse part of the V-cycle: levels 1..L-1 given the level-1 residual in lv[1].r;
 // returns the level-1 correction to prolong (lv[1].z if level 1 is the coarsest, else lv[1].t).
-void launch_vcycle_coarse(hipStream_t s, int store, int L, LevelDev* lv, Scal* sc, int coarse_f64, void** z1out);
+// visits: how many times the fine level visits level 1 (1: V-cycle; 2: W-cycle at level 1, the
+// second visit solving for the residual the first leaves; needs lv[1].w and L >= 3)
+void launch_vcycle_coarse(hipStream_t s, int store, int L, LevelDev* lv, Scal* sc, int coarse_f64, int visits,
+                          void** z1out);
 
 // fused.hip — the fused fine-grid MG-PCG passes
 bool fused_vec4(const Grid3& g);

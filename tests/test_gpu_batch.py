@@ -20,9 +20,12 @@ def problems():
     return out
 
 
+@pytest.mark.parametrize("visits", [0, 2])
 @pytest.mark.parametrize("mode", ["lockstep", "threads"])
-def test_batch_matches_single_projections(orc, mode, monkeypatch):
-    """lockstep: one launch per kernel for all problems; threads: per-context streams."""
+def test_batch_matches_single_projections(orc, mode, visits, monkeypatch):
+    """lockstep: one launch per kernel for all problems; threads: per-context streams.  visits 0:
+    the automatic cycle (V on these small grids); 2: the level-1 W-cycle, whose extra kernels the
+    lockstep merge interleaves with the other problems' launches."""
     import torch
     monkeypatch.setenv("MHDP_BATCH_MODE", mode)
     ps = problems()
@@ -31,15 +34,15 @@ def test_batch_matches_single_projections(orc, mode, monkeypatch):
         us = [torch.from_numpy(p.ustar).cuda() for p in ps]
         uo = [torch.zeros_like(u) for u in us]
         po = [torch.zeros(p.grid.cells, dtype=torch.float64, device="cuda") for p in ps]
-        print(f"[gpu] batch ({mode}) of {len(ps)}: sides {[p.grid.nx for p in ps]}", flush=True)
-        res = project_batch(prs, us, precision="mixed", precond="mg", u_outs=uo, phi_outs=po)
+        print(f"[gpu] batch ({mode}, visits {visits}) of {len(ps)}: sides {[p.grid.nx for p in ps]}", flush=True)
+        res = project_batch(prs, us, precision="mixed", precond="mg", coarse_visits=visits, u_outs=uo, phi_outs=po)
         torch.cuda.synchronize()
         for p, pr, u, phi, (it, r64) in zip(ps, prs, uo, po, res):
             assert r64 <= 1e-8 and it > 0
             one_u = torch.zeros_like(u)
             one_phi = torch.zeros_like(phi)
-            pr.project(torch.from_numpy(p.ustar).cuda(), precision="mixed", precond="mg", u_out=one_u,
-                       phi_out=one_phi)
+            pr.project(torch.from_numpy(p.ustar).cuda(), precision="mixed", precond="mg", coarse_visits=visits,
+                       u_out=one_u, phi_out=one_phi)
             torch.cuda.synchronize()
             assert pr.stats["iterations"] == it
             assert torch.equal(one_u, u) and torch.equal(one_phi, phi)
