@@ -648,6 +648,8 @@ struct K_k_brick_smooth {
         Scal* sc;
         int mode;
         int use_next;
+        const VT* base;  // COARSE: when set, t = base + theta * (smoothed value); base may be t
+        float theta;
     };
     static __device__ void run(const Args& A_, const VGrid& vg) {
         Grid3 g = A_.g;
@@ -759,10 +761,81 @@ struct K_k_brick_smooth {
                         tv.v[k] = on ? out : 0.0f;
                         if (!COARSE) acc[0] += on ? (double)rk * (double)out : 0.0;
                     }
+                    if (COARSE && A_.base) {  // each thread reads and writes only its own cells
+                        const F4 bv = un(ldr4(A_.base, row));
+#pragma unroll
+                        for (int k = 0; k < 4; k++) tv.v[k] = bv.v[k] + A_.theta * tv.v[k];
+                    }
                     st4(t, row, tv);
                 }
         }
         if constexpr (!COARSE) reduce_finish<1>(vg, acc, part, sc, 4, mode);
+    }
+};
+
+// r = r - A z on a coarse level (Galerkin graph with reaction term), in place: the residual one
+// visit of the level leaves, the next visit's right-hand side.  Each thread reads its own r only.
+template <class CT, class VT, int MB_>
+struct K_k_brick_coarse_residual {
+    static constexpr int MINB = MB_;
+    struct Args {
+        Grid3 g;
+        const CT* bx;
+        const CT* by;
+        const CT* bz;
+        const CT* react;
+        const VT* z;
+        VT* r;
+        Scal* sc;
+    };
+    static __device__ void run(const Args& A_, const VGrid& vg) {
+        if (CONVERGED(A_.sc)) return;
+        const Grid3 g = A_.g;
+        uint32_t lo, hi;
+        block_range(vg, g.n >> 4, lo, hi);
+        for (uint32_t b = lo + threadIdx.x; b < hi; b += MHDP_BLOCK) {
+            const Geo o = geo(g, b);
+            Rows<VT> zr;
+            WRaw<CT> wr;
+            R4<CT> er[2][2];
+            R4<VT> rc[2][2];
+            load_rows(o, A_.z, zr);
+            load_weights(o, wr, A_.bx, A_.by, A_.bz);
+#pragma unroll
+            for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+                for (int dy = 0; dy < 2; dy++) {
+                    er[dz][dy] = ldr4(A_.react, o.row[dz][dy]);
+                    rc[dz][dy] = ldr4(A_.r, o.row[dz][dy]);
+                }
+            fence_rows(zr);
+            fence_weights(wr);
+#pragma unroll
+            for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+                for (int dy = 0; dy < 2; dy++) {
+                    fence(er[dz][dy]);
+                    fence(rc[dz][dy]);
+                }
+            typename WtsOf<CT>::type w;
+            weights(o, wr, w);
+            Field f;
+            field2(
+                zr, zr, f, [&](const F4& a, const F4&, int) { return a; }, [&](float a, float, int) { return a; });
+#pragma unroll
+            for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+                for (int dy = 0; dy < 2; dy++) {
+                    const F4 re = un(er[dz][dy]), rv = un(rc[dz][dy]);
+                    F4 out;
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const float a = re.v[k] * f.c[dz][dy][k] + act_at(w, f, dz, dy, k);
+                        out.v[k] = rv.v[k] - a;
+                    }
+                    st4(A_.r, o.row[dz][dy], out);
+                }
+        }
     }
 };
 
@@ -855,17 +928,30 @@ void launch_brick_restrict_coarse(hipStream_t s, int store, const Grid3& cg, con
 }
 void launch_brick_smooth_coarse(hipStream_t s, int store, const Grid3& g, const Grid3& cg, const void* r,
                                 const void* dinv, const void* bx, const void* by, const void* bz, const void* react,
-                                const void* zc, void* t, Scal* sc) {
+                                const void* zc, void* t, Scal* sc, const void* base, float theta) {
     const dim3 B(MHDP_BLOCK), G(grid_blocks(g.n >> 4, brick_grid_cap()));
     if (store == STORE_BF16) {
         using T = __hip_bfloat16;
         BRICK_DISPATCH(k_brick_smooth, T COMMA T COMMA T COMMA true, g, cg, (const T*)r, (const T*)r, (const T*)dinv,
                        (const T*)bx, (const T*)by, (const T*)bz, (const T*)react, (const T*)zc, (T*)t, (double*)nullptr,
-                       sc, 0, 0);
+                       sc, 0, 0, (const T*)base, theta);
     } else {
         using T = float;
         BRICK_DISPATCH(k_brick_smooth, T COMMA T COMMA T COMMA true, g, cg, (const T*)r, (const T*)r, (const T*)dinv,
                        (const T*)bx, (const T*)by, (const T*)bz, (const T*)react, (const T*)zc, (T*)t, (double*)nullptr,
-                       sc, 0, 0);
+                       sc, 0, 0, (const T*)base, theta);
+    }
+}
+void launch_brick_coarse_residual(hipStream_t s, int store, const Grid3& g, const void* bx, const void* by,
+                                  const void* bz, const void* react, const void* z, void* r, Scal* sc) {
+    const dim3 B(MHDP_BLOCK), G(grid_blocks(g.n >> 4, brick_grid_cap()));
+    if (store == STORE_BF16) {
+        using T = __hip_bfloat16;
+        BRICK_DISPATCH(k_brick_coarse_residual, T COMMA T, g, (const T*)bx, (const T*)by, (const T*)bz,
+                       (const T*)react, (const T*)z, (T*)r, sc);
+    } else {
+        using T = float;
+        BRICK_DISPATCH(k_brick_coarse_residual, T COMMA T, g, (const T*)bx, (const T*)by, (const T*)bz,
+                       (const T*)react, (const T*)z, (T*)r, sc);
     }
 }

@@ -467,9 +467,12 @@ struct K_k_add {
     }
 };
 
-// Levels 1..L-1 of the V-cycle (L >= 2) from the level-1 residual in lv[1].r.
+// Levels 1..L-1 of the V-cycle (L >= 2) from the level-1 residual in lv[1].r.  acc (brick level 1
+// and L >= 3 only): the level-1 smoother writes acc + theta * (its result) into acc instead of
+// lv[1].t, and acc is the output.
 template <class CT, class VT, class AT>
-static void vcycle_coarse_t(hipStream_t s, int L, LevelDev* lv, Scal* sc, void** z1out, bool f16) {
+static void vcycle_coarse_t(hipStream_t s, int L, LevelDev* lv, Scal* sc, void** z1out, bool f16,
+                            void* acc = nullptr, float theta = 0.0f) {
     auto C = [&](LevelDev& l, int which) -> const CT* {
         void* const a16[5] = {l.bx16, l.by16, l.bz16, l.react16, l.dinv16};
         float* const a32[5] = {l.bx, l.by, l.bz, l.react, l.dinv};
@@ -498,13 +501,15 @@ static void vcycle_coarse_t(hipStream_t s, int L, LevelDev* lv, Scal* sc, void**
         LevelDev& c = lv[l + 1];
         const VT* zc = (const VT*)(l + 1 == L - 1 ? c.z : c.t);
         if (brick(a)) {
-            launch_brick_smooth_coarse(s, store, a.g, c.g, a.r, C(a, 4), C(a, 0), C(a, 1), C(a, 2), C(a, 3), zc, a.t, sc);
+            void* const into = (l == 1 && acc) ? acc : a.t;
+            launch_brick_smooth_coarse(s, store, a.g, c.g, a.r, C(a, 4), C(a, 0), C(a, 1), C(a, 2), C(a, 3), zc, into,
+                                       sc, l == 1 ? acc : nullptr, theta);
             continue;
         }
         go<K_k_prolong<CT, VT>>(s, grid_blocks(a.g.n), {a.g, c.g, 0, zc, C(a, 4), (VT*)a.z, sc});
         go<K_k_smooth<CT, VT, VT, false>>(s, grid_blocks(a.g.n), {a.g, (const VT*)a.r, C(a, 4), C(a, 0), C(a, 1), C(a, 2), C(a, 3), (const VT*)a.z, (VT*)a.t, sc});
     }
-    *z1out = (L - 1 == 1) ? lv[1].z : lv[1].t;
+    *z1out = (L - 1 == 1) ? lv[1].z : (acc ? acc : lv[1].t);
 }
 
 // The coarse part of the cycle, visiting level 1 `visits` times: after each visit the level-1
@@ -521,12 +526,21 @@ static void coarse_visits_t(hipStream_t s, int L, LevelDev* lv, Scal* sc, int vi
     LevelDev& a = lv[1];
     const CT* B[4] = {(const CT*)(f16 ? a.bx16 : (void*)a.bx), (const CT*)(f16 ? a.by16 : (void*)a.by),
                       (const CT*)(f16 ? a.bz16 : (void*)a.bz), (const CT*)(f16 ? a.react16 : (void*)a.react)};
+    const int store = f16 ? STORE_BF16 : STORE_F32;
+    const bool brick = brick_enabled_env() && brick_ok(a.g);
     void* const sum = a.t;  // L >= 3: the first visit's correction is in lv[1].t
     const void* last = sum;
     for (int v = 1; v < visits; v++) {
-        go<K_k_coarse_residual<CT, VT>>(s, grid_blocks(a.g.n), {a.g, B[0], B[1], B[2], B[3], (const VT*)last, (VT*)a.r, sc});
-        a.t = a.w;
+        if (brick)
+            launch_brick_coarse_residual(s, store, a.g, B[0], B[1], B[2], B[3], last, a.r, sc);
+        else
+            go<K_k_coarse_residual<CT, VT>>(s, grid_blocks(a.g.n), {a.g, B[0], B[1], B[2], B[3], (const VT*)last, (VT*)a.r, sc});
         void* zv = nullptr;
+        if (brick && v == visits - 1) {  // the last visit's level-1 smoother adds into the sum itself
+            vcycle_coarse_t<CT, VT, AT>(s, L, lv, sc, &zv, f16, sum, kSecondVisit);
+            break;
+        }
+        a.t = a.w;
         vcycle_coarse_t<CT, VT, AT>(s, L, lv, sc, &zv, f16);
         a.t = sum;
         go<K_k_add<VT>>(s, grid_blocks(a.g.n), {a.g.n, kSecondVisit, (const VT*)zv, (VT*)sum, sc});

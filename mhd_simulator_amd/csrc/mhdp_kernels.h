@@ -141,7 +141,10 @@ void launch_brick_restrict_coarse(hipStream_t s, int store, const Grid3& cg, con
                                   void* rc, Scal* sc);
 void launch_brick_smooth_coarse(hipStream_t s, int store, const Grid3& g, const Grid3& cg, const void* r,
                                 const void* dinv, const void* bx, const void* by, const void* bz, const void* react,
-                                const void* zc, void* t, Scal* sc);
+                                const void* zc, void* t, Scal* sc, const void* base = nullptr, float theta = 0.0f);
+// r = r - A z on a coarse brick level, in place (the residual a visit leaves; W-cycle)
+void launch_brick_coarse_residual(hipStream_t s, int store, const Grid3& g, const void* bx, const void* by,
+                                  const void* bz, const void* react, const void* z, void* r, Scal* sc);
 
 // march.hip — z-marching forms of the fine passes (nx % 4 == 0, even ny and nz), BF16 V-cycle
 // storage only (with FP32 storage the three-plane window does not fit the register file).
