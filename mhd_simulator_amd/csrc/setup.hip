@@ -309,39 +309,65 @@ struct K_k_mg_build {
         for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
             int qx, qy, qz;
             decode(cg, i, qx, qy, qz);
+            const int q[3] = {qx, qy, qz};
+            // every value the children and their six neighbours may need, loaded first from
+            // always-valid indices (a child's own cell where a neighbour does not exist); the sums
+            // below then run in the reference's order (children z, y, x; axes; lower then upper)
+            bool cok[8], nok[8][6], inside[8][6];
+            float dv[8], bt[8][6];
+            uint32_t mk[8][6];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                Pos child{{2 * qx + (k & 1), 2 * qy + ((k >> 1) & 1), 2 * qz + (k >> 2)}};
+                cok[k] = child.p[0] < fg.nx && child.p[1] < fg.ny && child.p[2] < fg.nz;
+                const uint32_t ci = cok[k] ? cidx(fg, child) : 0u;
+                dv[k] = FINE ? fdinv[ci] : freact[ci];
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+#pragma unroll
+                    for (int h = 0; h < 2; h++) {
+                        Pos n = child;
+                        n.p[a] += h ? 1 : -1;
+                        const bool ok = cok[k] && canon_valid(fg, n);
+                        const uint32_t ni = ok ? cidx(fg, n) : ci;
+                        nok[k][2 * a + h] = ok;
+                        inside[k][2 * a + h] = n.p[0] / 2 == q[0] && n.p[1] / 2 == q[1] && n.p[2] / 2 == q[2];
+                        bt[k][2 * a + h] = FB[a][h ? ni : ci];
+                        mk[k][2 * a + h] = (FINE && !meanfree) ? mask[ni] : 0u;
+                    }
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                asm volatile("" : "+v"(dv[k]));
+#pragma unroll
+                for (int f = 0; f < 6; f++) asm volatile("" : "+v"(bt[k][f]), "+v"(mk[k][f]));
+            }
             float c4[4] = {0.f, 0.f, 0.f, 0.f};
             float diag = 0.f;
-            for (int z = 0; z < 2; z++)
-                for (int y = 0; y < 2; y++)
-                    for (int x = 0; x < 2; x++) {
-                        Pos child{{2 * qx + x, 2 * qy + y, 2 * qz + z}};
-                        if (child.p[0] >= fg.nx || child.p[1] >= fg.ny || child.p[2] >= fg.nz) continue;
-                        const uint32_t ci = cidx(fg, child);
-                        if (FINE) {
-                            if (fdinv[ci] == 0.0f) continue;
-                        } else {
-                            const float gamma = freact[ci];
-                            c4[3] += gamma;
-                            diag += gamma;
-                        }
-                        for (int a = 0; a < 3; a++)
-                            for (int s = -1; s <= 1; s += 2) {
-                                Pos n = child;
-                                n.p[a] += s;
-                                if (!canon_valid(fg, n)) continue;
-                                const uint32_t ni = cidx(fg, n);
-                                const float beta = s > 0 ? FB[a][ni] : FB[a][ci];
-                                if (beta == 0.0f) continue;
-                                if (FINE && !meanfree && (mask[ni] & 3u) == 2u) {
-                                    c4[3] += beta;
-                                    diag += beta;
-                                    continue;
-                                }
-                                if (n.p[0] / 2 == qx && n.p[1] / 2 == qy && n.p[2] / 2 == qz) continue;
-                                diag += beta;
-                                if (s < 0) c4[a] += beta;
-                            }
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                if (!cok[k]) continue;
+                if (FINE) {
+                    if (dv[k] == 0.0f) continue;
+                } else {
+                    c4[3] += dv[k];
+                    diag += dv[k];
+                }
+#pragma unroll
+                for (int f = 0; f < 6; f++) {
+                    if (!nok[k][f]) continue;
+                    const float beta = bt[k][f];
+                    if (beta == 0.0f) continue;
+                    if (FINE && !meanfree && (mk[k][f] & 3u) == 2u) {
+                        c4[3] += beta;
+                        diag += beta;
+                        continue;
                     }
+                    if (inside[k][f]) continue;
+                    diag += beta;
+                    if (!(f & 1)) c4[f >> 1] += beta;
+                }
+            }
             cbx[i] = c4[0];
             cby[i] = c4[1];
             cbz[i] = c4[2];
