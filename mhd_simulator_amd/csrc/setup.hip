@@ -396,9 +396,13 @@ __device__ __forceinline__ float mob(const PhaseP& ph, float dl, float dc) {
 
 }  // namespace
 
-// Pass 1 of the coefficient build: the three lower-face mobilities of each cell (optional
-// BF16 copies for the mixed solver's V-cycle).
-struct K_k_beta4 {
+// pressure_coefficients (solver.wgsl:93-103) in one pass on four cells of a row: the three
+// lower-face mobilities of each cell (and their BF16 copies), the diagonal = sum of the six face
+// mobilities in the order x-, x+, y-, y+, z-, z+ (a closed face contributes +0), the anchor row
+// and the active-cell reciprocal.  The upper faces are the neighbours' lower faces, evaluated
+// here from the same mask and liquid-fraction values with the same expression, so the diagonal
+// equals the sum of the stored mobilities bit for bit.
+struct K_k_coef4 {
     static constexpr int MINB = 0;
     struct Args {
         Grid3 g;
@@ -408,121 +412,87 @@ struct K_k_beta4 {
         float* bx;
         float* by;
         float* bz;
+        float* diag;
+        float* dinv;
         __hip_bfloat16* bx16;
         __hip_bfloat16* by16;
         __hip_bfloat16* bz16;
+        __hip_bfloat16* dinv16;
     };
     static __device__ void run(const Args& A_, const VGrid& vg) {
-        Grid3 g = A_.g;
-        PhaseP ph = A_.ph;
+        const Grid3 g = A_.g;
+        const PhaseP ph = A_.ph;
         const uint32_t* __restrict__ mask = A_.mask;
         const float* __restrict__ alpha = A_.alpha;
-        float* __restrict__ bx = A_.bx;
-        float* __restrict__ by = A_.by;
-        float* __restrict__ bz = A_.bz;
-        __hip_bfloat16* __restrict__ bx16 = A_.bx16;
-        __hip_bfloat16* __restrict__ by16 = A_.by16;
-        __hip_bfloat16* __restrict__ bz16 = A_.bz16;
-        (void)vg;
         uint32_t lo, hi;
         block_range(vg, g.n >> 2, lo, hi);
         for (uint32_t t = lo + threadIdx.x; t < hi; t += MHDP_BLOCK) {
             const Row4 r = row4(g, t);
-            uint32_t m[4], my[4], mz[4];
+            uint32_t m[4], my[4], mz[4], myu[4], mzu[4];
             ld4u(mask, r.i0, m);
             ld4u(mask, r.ym, my);
             ld4u(mask, r.zm, mz);
-            const uint32_t mx = mask[r.xm];
-            float a[4] = {0.f, 0.f, 0.f, 0.f}, ay[4] = {0.f, 0.f, 0.f, 0.f}, az[4] = {0.f, 0.f, 0.f, 0.f}, ax = 0.f;
+            ld4u(mask, r.yp, myu);
+            ld4u(mask, r.zp, mzu);
+            const uint32_t mx = mask[r.xm], mxu = mask[r.xp];
+            float a[4] = {0.f, 0.f, 0.f, 0.f}, ay[4] = {0.f, 0.f, 0.f, 0.f}, az[4] = {0.f, 0.f, 0.f, 0.f};
+            float ayu[4] = {0.f, 0.f, 0.f, 0.f}, azu[4] = {0.f, 0.f, 0.f, 0.f}, ax = 0.f, axu = 0.f;
             if (alpha) {
                 ld4f(alpha, r.i0, a);
                 ld4f(alpha, r.ym, ay);
                 ld4f(alpha, r.zm, az);
+                ld4f(alpha, r.yp, ayu);
+                ld4f(alpha, r.zp, azu);
                 ax = alpha[r.xm];
+                axu = alpha[r.xp];
             }
-            float ox[4], oy[4], oz[4];
-    #pragma unroll
+            float ox[4], oy[4], oz[4], d[4], di[4];
+#pragma unroll
             for (int k = 0; k < 4; k++) {
                 const bool fc = m[k] != 0u;
-                const bool lx = k > 0 ? m[k - 1] != 0u : (r.xm_ok && mx != 0u);
-                const float dlx = dens(ph, k > 0 ? a[k - 1] : ax);
                 const float dc = dens(ph, a[k]);
-                ox[k] = (fc && lx) ? mob(ph, dlx, dc) : 0.0f;
+                // lower faces (mobility(), solver.wgsl:43-51)
+                const bool lx = k > 0 ? m[k - 1] != 0u : (r.xm_ok && mx != 0u);
+                ox[k] = (fc && lx) ? mob(ph, dens(ph, k > 0 ? a[k - 1] : ax), dc) : 0.0f;
                 oy[k] = (fc && r.ym_ok && my[k] != 0u) ? mob(ph, dens(ph, ay[k]), dc) : 0.0f;
                 oz[k] = (fc && r.zm_ok && mz[k] != 0u) ? mob(ph, dens(ph, az[k]), dc) : 0.0f;
             }
-            st4f(bx, r.i0, ox);
-            st4f(by, r.i0, oy);
-            st4f(bz, r.i0, oz);
-            if (bx16) {
-                st4h(bx16, r.i0, ox);
-                st4h(by16, r.i0, oy);
-                st4h(bz16, r.i0, oz);
-            }
-        }
-    }
-};
-
-// Pass 2: diagonal() = sum of the six face mobilities in the order x-, x+, y-, y+, z-, z+
-// (a closed face contributes +0), the anchor row, and the active-cell reciprocal.
-struct K_k_diag4 {
-    static constexpr int MINB = 0;
-    struct Args {
-        Grid3 g;
-        PhaseP ph;
-        const uint32_t* mask;
-        const float* bx;
-        const float* by;
-        const float* bz;
-        float* diag;
-        float* dinv;
-        __hip_bfloat16* dinv16;
-    };
-    static __device__ void run(const Args& A_, const VGrid& vg) {
-        Grid3 g = A_.g;
-        PhaseP ph = A_.ph;
-        const uint32_t* __restrict__ mask = A_.mask;
-        const float* __restrict__ bx = A_.bx;
-        const float* __restrict__ by = A_.by;
-        const float* __restrict__ bz = A_.bz;
-        float* __restrict__ diag = A_.diag;
-        float* __restrict__ dinv = A_.dinv;
-        __hip_bfloat16* __restrict__ dinv16 = A_.dinv16;
-        (void)vg;
-        uint32_t lo, hi;
-        block_range(vg, g.n >> 2, lo, hi);
-        for (uint32_t t = lo + threadIdx.x; t < hi; t += MHDP_BLOCK) {
-            const Row4 r = row4(g, t);
-            uint32_t m[4];
-            ld4u(mask, r.i0, m);
-            float xl[4], yl[4], zl[4], yu[4] = {0.f, 0.f, 0.f, 0.f}, zu[4] = {0.f, 0.f, 0.f, 0.f};
-            ld4f(bx, r.i0, xl);
-            ld4f(by, r.i0, yl);
-            ld4f(bz, r.i0, zl);
-            if (r.yp_ok) ld4f(by, r.yp, yu);
-            if (r.zp_ok) ld4f(bz, r.zp, zu);
-            const float xe = r.xp_ok ? bx[r.xp] : 0.0f;
-            float d[4], di[4];
-    #pragma unroll
+#pragma unroll
             for (int k = 0; k < 4; k++) {
+                const bool fc = m[k] != 0u;
+                const float dc = dens(ph, a[k]);
+                // upper faces: the lower faces of the +x, +y, +z neighbours
+                const float xu = k < 3 ? ox[k + 1]
+                                       : ((r.xp_ok && mxu != 0u && fc) ? mob(ph, dc, dens(ph, axu)) : 0.0f);
+                const float yu = (r.yp_ok && myu[k] != 0u && fc) ? mob(ph, dc, dens(ph, ayu[k])) : 0.0f;
+                const float zu = (r.zp_ok && mzu[k] != 0u && fc) ? mob(ph, dc, dens(ph, azu[k])) : 0.0f;
+                // diagonal() (solver.wgsl:60-69)
                 float v = 0.0f;
-                if (m[k] != 0u) {
-                    v += xl[k];
-                    v += k < 3 ? xl[k + 1] : xe;
-                    v += yl[k];
-                    v += yu[k];
-                    v += zl[k];
-                    v += zu[k];
+                if (fc) {
+                    v += ox[k];
+                    v += xu;
+                    v += oy[k];
+                    v += yu;
+                    v += oz[k];
+                    v += zu;
                     if (!ph.meanfree && (m[k] & 3u) == 2u) v = 1.0f;
                     else v = fmaxf(v, 1e-30f);
                 }
                 d[k] = v;
-                const bool active = m[k] != 0u && (ph.meanfree || (m[k] & 3u) == 1u);
+                const bool active = fc && (ph.meanfree || (m[k] & 3u) == 1u);
                 di[k] = active ? 1.0f / v : 0.0f;
             }
-            st4f(diag, r.i0, d);
-            st4f(dinv, r.i0, di);
-            if (dinv16) st4h(dinv16, r.i0, di);
+            st4f(A_.bx, r.i0, ox);
+            st4f(A_.by, r.i0, oy);
+            st4f(A_.bz, r.i0, oz);
+            st4f(A_.diag, r.i0, d);
+            st4f(A_.dinv, r.i0, di);
+            if (A_.bx16) {
+                st4h(A_.bx16, r.i0, ox);
+                st4h(A_.by16, r.i0, oy);
+                st4h(A_.bz16, r.i0, oz);
+                st4h(A_.dinv16, r.i0, di);
+            }
         }
     }
 };
@@ -1076,8 +1046,8 @@ bool launch_coefficients(hipStream_t s, const Grid3& g, const PhaseP& ph, const 
         const dim3 G(grid_blocks(g.n >> 2)), B(MHDP_BLOCK);
         const bool h = bx16 && by16 && bz16 && dinv16;
         using T = __hip_bfloat16;
-        go<K_k_beta4>(s, G.x, {g, ph, mask, alpha, bx, by, bz, h ? (T*)bx16 : (T*)nullptr, h ? (T*)by16 : (T*)nullptr, h ? (T*)bz16 : (T*)nullptr});
-        go<K_k_diag4>(s, G.x, {g, ph, mask, bx, by, bz, diag, dinv, h ? (T*)dinv16 : (T*)nullptr});
+        go<K_k_coef4>(s, G.x, {g, ph, mask, alpha, bx, by, bz, diag, dinv, h ? (T*)bx16 : (T*)nullptr,
+                               h ? (T*)by16 : (T*)nullptr, h ? (T*)bz16 : (T*)nullptr, h ? (T*)dinv16 : (T*)nullptr});
         return h;
     }
     go<K_k_coefficients>(s, grid_blocks(g.n), {g, ph, mask, alpha, bx, by, bz, diag, dinv});
