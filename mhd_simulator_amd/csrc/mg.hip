@@ -1,7 +1,8 @@
 // Aggregation multigrid V(1,1) preconditioner (multigrid.wgsl): weighted-Jacobi (omega = 2/3)
 // smoothing, restriction by summing the 2x2x2 children's residuals, piecewise-constant
 // prolongation and a fixed 8-sweep Jacobi solve on the coarsest grid (<= 4^3 cells, one
-// workgroup).  Templated on the storage type of coefficients (CT) and vectors (VT):
+// workgroup).  Optionally level 1 is visited more than once per cycle (a W-cycle there:
+// coarse_visits_t).  Templated on the storage type of coefficients (CT) and vectors (VT):
 // float for the FP32 solver, bfloat16 for the mixed-precision solver; arithmetic is FP32
 // (FP64 on the coarsest grid in the mixed solver).
 #include "mhdp_kernels.h"

@@ -1,6 +1,8 @@
-// The momentum step either side of the projection (solver.wgsl:127-163):
+// The momentum step either side of the projection (solver.wgsl:127-163, mod.rs:1285-1303):
 //   predict  semi-Lagrangian RK2 advection of the face velocities (departure point from the
 //            velocity sampled at the face centre and at the half-step midpoint, trilinear samples)
+//   stress   the two-phase viscous stress diffusion (viscosity.wgsl, stabilized.wgsl): explicit
+//            substeps or damped RKC2 stages of the stress divergence
 //   viscous  explicit face Laplacian with no-slip reflection at closed tangential neighbours,
 //            body + EM (+ capillary, two-phase) acceleration, times the face mobility.
 // Built with -ffp-contract=off: every FP32 expression keeps the reference's operation order, so
@@ -9,9 +11,9 @@
 // Faces are the three SoA planes (plane a has dims n + e_a); each kernel is instantiated per
 // plane (the axis is a template argument).  Which faces are open depends only on the mask, so it
 // is kept per cell as one byte (bit a: the cell's lower face on axis a is open; bit 3: the cell is
-// fluid), rebuilt when the cells change; an open test is then one byte load instead of two mask loads.  predict first
-// writes the open-face copy of its input (closed faces 0), so each of its 56 samples per face is
-// one load (periodic wrap and range check stay).
+// fluid), rebuilt when the cells change; an open test is then one byte load instead of two mask
+// loads.  predict first writes the open-face copy of its input (closed faces 0), so each of its 56
+// samples per face is one load (periodic wrap and range check stay).
 #include <cstdlib>
 
 #include "launch.h"
