@@ -29,12 +29,12 @@ METRIC = "pressure-solve cells/s to 1e-8 rel. residual (mixed vs FP32 PCG-MG); H
 HBM_PEAK = 8.0e12
 # Algorithmic bytes per cell and launch of the three fine-grid passes of the mixed solver
 # (every array touched once; neighbour reads are not counted again):
-#   apply    z (bf16) 2 + p_old 4 + x 4 + 3 FP32 mobilities 12 + 1/diag 4 read,
-#            p_new 4 + q 4 + x 4 written                                        = 38
+#   apply    z (bf16) 2 + p_old 4 + x 4 + 3 FP32 mobilities 12 + 1/diag (bf16) 2 read,
+#            p_new 4 + q 4 + x 4 written                                        = 36
 #   restrict r 4 + q 4 + 1/diag (bf16) 2 + 3 bf16 mobilities 6 read, updated r 4 +
 #            coarse residual 1/8*2 written (the PCG residual update is fused here) = 20.25
 #   smooth   r 4 + 1/diag 2 + mobilities 6 + coarse z 1/8*2 read, z (bf16) 2 written = 14.25
-KERNEL_BYTES = {"apply": 38.0, "smooth": 14.25, "restrict": 20.25}
+KERNEL_BYTES = {"apply": 36.0, "smooth": 14.25, "restrict": 20.25}
 
 
 def log(msg):

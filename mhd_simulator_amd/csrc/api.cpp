@@ -394,8 +394,9 @@ void iterate_once(mhdp_ctx* c, Solve& S) {  // mod.rs:1057-1075, multigrid.rs:30
         c->timer.before(MHDP_K_APPLY, s);
         const bool brick = c->brick_enabled && brick_ok(c->g);
         if (brick)
-            launch_brick_apply(s, c->g, S.store, S.lv[0].t, c->p, c->p2, c->x, c->bx, c->by, c->bz, c->dinv, c->q,
-                               c->part, c->sc, S.meanfree ? 1 : 0);
+            launch_brick_apply(s, c->g, S.store, S.lv[0].t, c->p, c->p2, c->x, c->bx, c->by, c->bz,
+                               S.store == STORE_BF16 ? c->dinv16 : (const void*)c->dinv, c->q, c->part, c->sc,
+                               S.meanfree ? 1 : 0);
         else
             launch_fused_apply(s, c->g, S.store, S.lv[0].t, c->p, c->p2, c->x, c->bx, c->by, c->bz, c->dinv, c->q,
                                c->part, c->sc, S.meanfree ? 1 : 0);

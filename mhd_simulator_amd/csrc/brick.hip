@@ -297,7 +297,9 @@ bool brick_enabled_env() {
     return on;
 }
 
-// pass 1 (k_fused_apply on bricks)
+// pass 1 (k_fused_apply on bricks).  The reciprocal diagonal only marks the active cells here
+// (tested against zero), so the mixed solver passes its BF16 copy (VT): 1/diag lies in
+// [1e-5, 1e30], where BF16 rounding never gives zero.
 template <class VT, int MB_>
 struct K_k_brick_apply {
     static constexpr int MINB = MB_;
@@ -310,7 +312,7 @@ struct K_k_brick_apply {
         const float* bx;
         const float* by;
         const float* bz;
-        const float* dinv;
+        const VT* dinv;
         float* q;
         double* part;
         Scal* sc;
@@ -325,7 +327,7 @@ struct K_k_brick_apply {
         const float* __restrict__ bx = A_.bx;
         const float* __restrict__ by = A_.by;
         const float* __restrict__ bz = A_.bz;
-        const float* __restrict__ dinv = A_.dinv;
+        const VT* __restrict__ dinv = A_.dinv;
         float* __restrict__ q = A_.q;
         double* __restrict__ part = A_.part;
         Scal* sc = A_.sc;
@@ -347,7 +349,8 @@ struct K_k_brick_apply {
             Rows<VT> zr;
             Rows<float> pr;
             WRaw<float> wr;
-            R4<float> dr[2][2], xr[2][2];
+            R4<VT> dr[2][2];
+            R4<float> xr[2][2];
             load_rows(o, z, zr);
             load_rows(o, pold, pr);
             load_weights(o, wr, bx, by, bz);
@@ -859,14 +862,15 @@ static int brick_occ() {
     } while (0)
 
 void launch_brick_apply(hipStream_t s, const Grid3& g, int store, const void* z, float* p0, float* p1, float* x,
-                        const float* bx, const float* by, const float* bz, const float* dinv, float* q, double* part,
+                        const float* bx, const float* by, const float* bz, const void* dinv, float* q, double* part,
                         Scal* sc, int meanfree) {
     const dim3 B(MHDP_BLOCK), G(grid_blocks(g.n >> 4, brick_grid_cap()));
     if (store == STORE_BF16)
-        BRICK_DISPATCH(k_brick_apply, __hip_bfloat16, g, (const __hip_bfloat16*)z, p0, p1, x, bx, by, bz, dinv, q, part,
-                       sc, meanfree);
+        BRICK_DISPATCH(k_brick_apply, __hip_bfloat16, g, (const __hip_bfloat16*)z, p0, p1, x, bx, by, bz,
+                       (const __hip_bfloat16*)dinv, q, part, sc, meanfree);
     else
-        BRICK_DISPATCH(k_brick_apply, float, g, (const float*)z, p0, p1, x, bx, by, bz, dinv, q, part, sc, meanfree);
+        BRICK_DISPATCH(k_brick_apply, float, g, (const float*)z, p0, p1, x, bx, by, bz, (const float*)dinv, q, part, sc,
+                       meanfree);
 }
 #define COMMA ,
 void launch_brick_restrict(hipStream_t s, int store, const Grid3& cg, const Grid3& g, const float* r, const void* dinv,

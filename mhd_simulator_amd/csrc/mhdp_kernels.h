@@ -121,8 +121,9 @@ void launch_fused_smooth(hipStream_t s, int store, const Grid3& g, const Grid3& 
 // brick.hip — 4x2x2-cell-per-thread variants (nx % 4 == 0, even ny and nz)
 bool brick_ok(const Grid3& g);
 bool brick_enabled_env();
+// dinv: the storage type's reciprocal diagonal (BF16 copy with STORE_BF16), tested against zero only
 void launch_brick_apply(hipStream_t s, const Grid3& g, int store, const void* z, float* p0, float* p1, float* x,
-                        const float* bx, const float* by, const float* bz, const float* dinv, float* q, double* part,
+                        const float* bx, const float* by, const float* bz, const void* dinv, float* q, double* part,
                         Scal* sc, int meanfree);
 void launch_brick_restrict(hipStream_t s, int store, const Grid3& cg, const Grid3& g, const float* r, const void* dinv,
                            const void* bx, const void* by, const void* bz, void* rc, Scal* sc);
