@@ -1274,11 +1274,17 @@ int mhdp_fields_dev(mhdp_ctx* c, const float* u_faces, const double* phi, const 
     const uint32_t blocks = grid_blocks(c->g.n);
     int rc = dalloc(c, (void**)&c->fields_part, ((size_t)blocks + 1) * kFieldTotals * sizeof(double));
     if (rc) return rc;
+    if (face_count(c->grid) >= (1ull << 32)) return fail(c, MHDP_INVALID, "Too many faces for the step kernels");
+    if (!c->open8_fresh) {
+        if ((rc = dalloc(c, (void**)&c->open8, c->g.n))) return rc;
+        launch_open8(c->stream, c->g, c->mask, c->open8);
+        c->open8_fresh = true;
+    }
     const bool meanfree = fp->gauge != MHDP_GAUGE_PINNED && c->components == 1;
     double* totals = c->fields_part + (size_t)blocks * kFieldTotals;
     launch_fields(c->stream, c->g, c->grid.h, c->phase.rho_liquid, c->phase.rho_air, c->phase.two_phase ? 1 : 0,
-                  fp->dt, fp->acceleration, fp->mushy ? 1 : 0, meanfree ? 1 : 0, c->mask, u_faces, phi, em, cells,
-                  c->fields_part, totals);
+                  fp->dt, fp->acceleration, fp->mushy ? 1 : 0, meanfree ? 1 : 0, c->mask, c->open8, u_faces, phi, em,
+                  cells, c->fields_part, totals);
     HIP_OK(hipGetLastError());
     double t[kFieldTotals];
     HIP_OK(hipMemcpyAsync(t, totals, sizeof t, hipMemcpyDeviceToHost, c->stream));

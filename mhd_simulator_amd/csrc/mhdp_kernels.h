@@ -168,7 +168,8 @@ void launch_old_potential(hipStream_t s, uint32_t n, const uint32_t* mask, const
 // per block of grid_blocks(n))
 constexpr int kFieldTotals = 10;
 void launch_fields(hipStream_t s, const Grid3& g, float h, float rho_l, float rho_air, int two_phase, float dt,
-                   const float acc[3], int mushy, int meanfree, const uint32_t* mask, const float* faces,
+                   const float acc[3], int mushy, int meanfree, const uint32_t* mask, const uint8_t* open8,
+                   const float* faces,
                    const double* phi, const float* em, float* cells, double* part, double* totals);
 
 // step.hip — predict and viscous (solver.wgsl:127-163).  open8: one byte per cell, bit a set when

@@ -873,6 +873,7 @@ struct K_fields {
         float acc[3];
         int two_phase, mushy, meanfree;
         const uint32_t* mask;
+        const uint8_t* open8;  // per cell: bit a = its lower face on axis a is open
         const float* faces;
         uint32_t f0, f1;
         const double* phi;
@@ -880,31 +881,9 @@ struct K_fields {
         float* cells;
         double* part;
     };
-    static __device__ bool fl(const Grid3& g, const uint32_t* __restrict__ mask, int x, int y, int z) {
-        if (g.per & 1u) x = x < 0 ? x + g.nx : (x >= g.nx ? x - g.nx : x);
-        if (g.per & 2u) y = y < 0 ? y + g.ny : (y >= g.ny ? y - g.ny : y);
-        if (g.per & 4u) z = z < 0 ? z + g.nz : (z >= g.nz ? z - g.nz : z);
-        if (x < 0 || y < 0 || z < 0 || x >= g.nx || y >= g.ny || z >= g.nz) return false;
-        return mask[(uint32_t)x + (uint32_t)g.nx * ((uint32_t)y + (uint32_t)g.ny * (uint32_t)z)] != 0u;
-    }
-    // face(pos, a, false) (solver.wgsl:33-36): the face value if open, else 0
-    static __device__ float face(const Args& A, int a, int x, int y, int z) {
-        const Grid3& g = A.g;
-        int q[3] = {x, y, z};
-        const int d[3] = {g.nx, g.ny, g.nz};
-        for (int b = 0; b < 3; b++)
-            if ((g.per & (1u << b)) && q[b] >= d[b] + (b == a ? 1 : 0)) q[b] -= d[b];
-        if ((g.per & (1u << a)) && q[a] == d[a]) q[a] = 0;
-        if (q[a] < 0 || q[a] > d[a]) return 0.0f;
-        int l[3] = {q[0], q[1], q[2]};
-        l[a] -= 1;
-        if (!fl(g, A.mask, q[0], q[1], q[2]) || !fl(g, A.mask, l[0], l[1], l[2])) return 0.0f;
-        const uint32_t Px = (uint32_t)g.nx + (a == 0), Py = (uint32_t)g.ny + (a == 1);
-        const uint32_t off = a == 0 ? 0u : (a == 1 ? A.f0 : A.f0 + A.f1);
-        return A.faces[off + (uint32_t)q[0] + Px * ((uint32_t)q[1] + Py * (uint32_t)q[2])];
-    }
     static __device__ void run(const Args& A, const VGrid& vg) {
         const Grid3 g = A.g;
+        const int d[3] = {g.nx, g.ny, g.nz};
         uint32_t lo, hi;
         block_range(vg, g.n, lo, hi);
         double tot[kFieldTotals];
@@ -912,23 +891,50 @@ struct K_fields {
         for (int k = 0; k < kFieldTotals; k++) tot[k] = 0.0;
         for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
             float* cell = A.cells + 16u * (size_t)i;
+            const uint32_t m = A.mask[i];
+            int x, y, z;
+            decode(g, i, x, y, z);
+            const int pos[3] = {x, y, z};
+            // face(pos, a, false) and face(pos + e_a, a, false) (solver.wgsl:33-36): open from the
+            // open bytes of the cell and of its upper neighbour (periodic: wrapped; closed: none);
+            // every load below is issued from a valid index before any value is used
+            uint32_t ob[4];
+            float fv[6];
+            bool up_ok[3];
+            ob[0] = A.open8[i];
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                const uint32_t Px = (uint32_t)g.nx + (a == 0), Py = (uint32_t)g.ny + (a == 1);
+                const uint32_t off = a == 0 ? 0u : (a == 1 ? A.f0 : A.f0 + A.f1);
+                int u[3] = {x, y, z};
+                u[a] += 1;
+                const bool wrap = u[a] == d[a] && (g.per & (1u << a));
+                up_ok[a] = u[a] < d[a] || wrap;
+                if (wrap) u[a] = 0;
+                const uint32_t uc = up_ok[a] ? (uint32_t)u[0] + (uint32_t)g.nx * ((uint32_t)u[1] + (uint32_t)g.ny * (uint32_t)u[2]) : i;
+                ob[1 + a] = A.open8[uc];
+                fv[2 * a] = A.faces[off + (uint32_t)x + Px * ((uint32_t)y + Py * (uint32_t)z)];
+                fv[2 * a + 1] = A.faces[off + (uint32_t)u[0] + Px * ((uint32_t)u[1] + Py * (uint32_t)u[2])];
+            }
             const float o1 = cell[13], o2 = cell[14], o3 = cell[15];
+            const double phi_i = A.phi[i], phi_a = A.meanfree ? A.phi[m >> 2] : 0.0;
+            float force[3] = {0.0f, 0.0f, 0.0f};
+            if (A.em)
+                for (int a = 0; a < 3; a++) force[a] = A.em[16u * (size_t)i + 8u + (uint32_t)a];
+#pragma unroll
+            for (int k = 0; k < 4; k++) asm volatile("" : "+v"(ob[k]));
+#pragma unroll
+            for (int k = 0; k < 6; k++) asm volatile("" : "+v"(fv[k]));
             float out[16];
 #pragma unroll
             for (int k = 0; k < 16; k++) out[k] = 0.0f;
             if (A.two_phase) { out[13] = o1; out[14] = o2; out[15] = o3; }
-            const uint32_t m = A.mask[i];
             if (m) {
-                int x, y, z;
-                decode(g, i, x, y, z);
-                const int pos[3] = {x, y, z};
                 float v[3], lo_[3], hi_[3], fe = 0.0f, mf = 0.0f, sum = 0.0f;
 #pragma unroll
                 for (int a = 0; a < 3; a++) {
-                    int q[3] = {x, y, z};
-                    q[a] += 1;
-                    lo_[a] = face(A, a, pos[0], pos[1], pos[2]);
-                    hi_[a] = face(A, a, q[0], q[1], q[2]);
+                    lo_[a] = ((ob[0] >> a) & 1u) ? fv[2 * a] : 0.0f;
+                    hi_[a] = (up_ok[a] && ((ob[1 + a] >> a) & 1u)) ? fv[2 * a + 1] : 0.0f;
                     v[a] = 0.5f * (lo_[a] + hi_[a]);
                     fe += 0.5f * (lo_[a] * lo_[a] + hi_[a] * hi_[a]);
                     mf = fmaxf(mf, fmaxf(fabsf(lo_[a]), fabsf(hi_[a])));
@@ -939,11 +945,8 @@ struct K_fields {
                 float melt = A.mushy ? o1 : 1.0f;
                 if (A.two_phase) melt *= o2;
                 const float speed = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-                const float pv = A.meanfree ? (float)A.phi[i] - (float)A.phi[m >> 2] : (float)A.phi[i];
+                const float pv = A.meanfree ? (float)phi_i - (float)phi_a : (float)phi_i;
                 const float pressure = pv * A.rho_l / A.dt;
-                float force[3] = {0.0f, 0.0f, 0.0f};
-                if (A.em)
-                    for (int a = 0; a < 3; a++) force[a] = A.em[16u * (size_t)i + 8u + (uint32_t)a];
 #pragma unroll
                 for (int a = 0; a < 3; a++) out[a] = v[a];
                 out[3] = speed;
@@ -1121,12 +1124,13 @@ void launch_old_potential(hipStream_t s, uint32_t n, const uint32_t* mask, const
 }
 
 void launch_fields(hipStream_t s, const Grid3& g, float h, float rho_l, float rho_air, int two_phase, float dt,
-                   const float acc[3], int mushy, int meanfree, const uint32_t* mask, const float* faces,
-                   const double* phi, const float* em, float* cells, double* part, double* totals) {
+                   const float acc[3], int mushy, int meanfree, const uint32_t* mask, const uint8_t* open8,
+                   const float* faces, const double* phi, const float* em, float* cells, double* part,
+                   double* totals) {
     uint32_t f0, f1, f2;
     face_counts(g, f0, f1, f2);
     const uint32_t blocks = grid_blocks(g.n);
-    K_fields::Args a{g, h, rho_l, rho_air, dt, {acc[0], acc[1], acc[2]}, two_phase, mushy, meanfree, mask, faces,
+    K_fields::Args a{g, h, rho_l, rho_air, dt, {acc[0], acc[1], acc[2]}, two_phase, mushy, meanfree, mask, open8, faces,
                      f0, f1, phi, em, cells, part};
     go<K_fields>(s, blocks, a);
     go<K_fields_total>(s, 1, {part, blocks, totals});
