@@ -167,9 +167,13 @@ int mhdp_em_solve_dev(mhdp_em* e, const mhdp_em_opts* o, const float* as_now, co
     EmScal s = *e->sc_h;
     for (;;) {
         while (s.status == 0 && finite_status(s) && s.its < max_iter) {
+            const uint32_t before = s.its;
             launch_em_iterations(e->stream, e->d, ph, v, e->part, e->sc, std::min<uint32_t>(max_iter - s.its, 32u));
             if ((rc = read_scalars(e))) return rc;
             s = *e->sc_h;
+            // every batch either counts iterations or ends the solve (status); anything else is a
+            // device-side fault of the reductions, reported instead of waiting for ever
+            if (s.status == 0 && s.its == before) return fail(e, MHDP_DEVICE, "EM solve made no progress");
         }
         if (s.status != 1) break;
         // converged on the recursive residual: verify with the actual equation and restart from it
