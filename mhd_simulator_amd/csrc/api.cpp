@@ -67,6 +67,7 @@ struct mhdp_ctx {
     float *stress_mu = nullptr, *stress_rho = nullptr, *stress_part = nullptr;  // viscous stress diffusion
     float *faces_b = nullptr, *faces_y0 = nullptr, *faces_f0 = nullptr;        // its face buffers
     float* stress_tensor = nullptr;                                            // and its stress tensor
+    float* cell_pair = nullptr;  // viscous (two-phase): liquid fraction and curvature per cell
     bool open8_fresh = false;       // open8 matches the current mask
     void *bx16 = nullptr, *by16 = nullptr, *bz16 = nullptr, *dinv16 = nullptr;
     // multigrid
@@ -1478,8 +1479,10 @@ int mhdp_viscous_dev(mhdp_ctx* c, const float* pred, const float* cells, const f
     if (!pred || !out) return fail(c, MHDP_INVALID, "Missing face velocities");
     if (pred == out) return fail(c, MHDP_INVALID, "viscous reads neighbouring faces: output must not alias input");
     if (c->phase.two_phase && !cells) return fail(c, MHDP_INVALID, "Two-phase viscous step needs the cell packets");
+    if (c->phase.two_phase && (rc = dalloc(c, (void**)&c->cell_pair, (size_t)c->g.n * 8))) return rc;
     launch_viscous(c->stream, c->g, c->grid.h, c->phase.rho_liquid, c->phase.rho_air, c->phase.two_phase ? 1 : 0,
-                   sp->dt, sp->viscosity, sp->acceleration, sp->surface_tension, c->open8, pred, cells, em, out);
+                   sp->dt, sp->viscosity, sp->acceleration, sp->surface_tension, c->open8, pred, cells,
+                   (float2*)c->cell_pair, em, out);
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(c->stream));
     return MHDP_OK;
@@ -1620,7 +1623,7 @@ void mhdp_destroy(mhdp_ctx* c) {
     else (void)hipDeviceSynchronize();
     void* ptrs[] = {c->mask, c->alpha, c->bx, c->by, c->bz, c->diag, c->dinv, c->b, c->x, c->r, c->r2, c->p, c->p2, c->q, c->z,
                     c->t, c->x64, c->b64, c->bx16, c->by16, c->bz16, c->dinv16, c->sc, c->part, c->ustar_stage,
-                    c->uout_stage, c->phi_stage, c->fluid8, c->cc_counts, c->phi_prev, c->uwarm, c->fields_part, c->faces_open, c->open8, c->stress_mu, c->stress_rho, c->stress_part, c->faces_b, c->faces_y0, c->faces_f0, c->stress_tensor};
+                    c->uout_stage, c->phi_stage, c->fluid8, c->cc_counts, c->phi_prev, c->uwarm, c->fields_part, c->faces_open, c->open8, c->stress_mu, c->stress_rho, c->stress_part, c->faces_b, c->faces_y0, c->faces_f0, c->stress_tensor, c->cell_pair};
     for (void* p : ptrs) dfree(c, p);
     for (CoarseArrays& a : c->coarse) {
         void* q[] = {a.bx, a.by, a.bz, a.react, a.diag, a.dinv, a.bx16, a.by16, a.bz16, a.react16, a.dinv16,

@@ -178,9 +178,10 @@ void launch_fields(hipStream_t s, const Grid3& g, float h, float rho_l, float rh
 void launch_open8(hipStream_t s, const Grid3& g, const uint32_t* mask, uint8_t* open8);
 void launch_predict(hipStream_t s, const Grid3& g, float h, float dt, int advect, const uint8_t* open8,
                     const float* u, float* uo, float* out);
+// two-phase: ak is scratch of one float2 per cell (the packets' liquid fraction and curvature)
 void launch_viscous(hipStream_t s, const Grid3& g, float h, float rho_l, float rho_air, int two_phase, float dt,
                     float viscosity, const float acc[3], float sigma, const uint8_t* open8, const float* pred,
-                    const float* cells, const float* em, float* out);
+                    const float* cells, float2* ak, const float* em, float* out);
 
 // step.hip — two-phase viscous stress diffusion (src/fluid/viscosity.wgsl, stabilized.wgsl): per cell
 // viscosity and density, the stability bound (part: stress_bound_blocks floats), the stress tensor
