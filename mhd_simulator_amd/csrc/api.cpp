@@ -1382,7 +1382,6 @@ int mhdp_surface_stress_dev(mhdp_ctx* c, float* u, const float* cells, const mhd
     if ((rc = dalloc(c, (void**)&c->stress_mu, (size_t)c->g.n * 4)) ||
         (rc = dalloc(c, (void**)&c->stress_rho, (size_t)c->g.n * 4)) ||
         (rc = dalloc(c, (void**)&c->stress_part, ((size_t)stress_bound_blocks(c->g) + 1) * 4)) ||
-        (rc = dalloc(c, (void**)&c->faces_b, F * 4)) ||
         (rc = dalloc(c, (void**)&c->stress_tensor, stress_tensor_len(c->g) * 4)))
         return rc;
     const float h = c->grid.h, rl = c->phase.rho_liquid, ra = c->phase.rho_air;
@@ -1427,7 +1426,9 @@ int mhdp_surface_stress_dev(mhdp_ctx* c, float* u, const float* cells, const mhd
     if (n > 8) {  // RKC2 (stabilized.rs:199-250)
         const auto rows = rkc_schedule(dt, n);
         if (rows.empty()) return fail(c, MHDP_INVALID, "RKC stage-capacity bound");
-        if ((rc = dalloc(c, (void**)&c->faces_y0, F * 4)) || (rc = dalloc(c, (void**)&c->faces_f0, F * 4))) return rc;
+        if ((rc = dalloc(c, (void**)&c->faces_b, F * 4)) || (rc = dalloc(c, (void**)&c->faces_y0, F * 4)) ||
+            (rc = dalloc(c, (void**)&c->faces_f0, F * 4)))
+            return rc;
         HIP_OK(hipMemcpyAsync(c->faces_y0, u, F * 4, hipMemcpyDeviceToDevice, c->stream));
         launch_stress_tensor(c->stream, c->g, sb, 1, u);
         launch_stress_initial(c->stream, c->g, sb, u, c->faces_f0);
@@ -1441,15 +1442,13 @@ int mhdp_surface_stress_dev(mhdp_ctx* c, float* u, const float* cells, const mhd
         }
         if (newest != u) HIP_OK(hipMemcpyAsync(u, newest, F * 4, hipMemcpyDeviceToDevice, c->stream));
         if (substeps) *substeps = (uint32_t)rows.size();
-    } else {  // explicit substeps of dt / count (mod.rs:1219-1233)
+    } else {  // explicit substeps of dt / count (mod.rs:1219-1233), in place: each substep's tensor
+              // holds every neighbour value its face kernels need
         const float dt_sub = (float)(dt / (double)n);
         for (uint32_t k = 0; k < n; k++) {
-            const float* src = (k % 2 == 0) ? u : c->faces_b;
-            float* dst = (k % 2 == 0) ? c->faces_b : u;
-            launch_stress_tensor(c->stream, c->g, sb, 0, src);
-            launch_stress_euler(c->stream, c->g, sb, dt_sub, src, dst);
+            launch_stress_tensor(c->stream, c->g, sb, 0, u);
+            launch_stress_euler(c->stream, c->g, sb, dt_sub, u, u);
         }
-        if (n % 2 == 1) HIP_OK(hipMemcpyAsync(u, c->faces_b, F * 4, hipMemcpyDeviceToDevice, c->stream));
         if (substeps) *substeps = n;
     }
     HIP_OK(hipGetLastError());

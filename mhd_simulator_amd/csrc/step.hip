@@ -713,7 +713,9 @@ struct StressFace {
     }
 };
 
-// stress_diffuse (viscosity.wgsl:110-117): one explicit substep (tensor of u, direct form)
+// stress_diffuse (viscosity.wgsl:110-117): one explicit substep (tensor of u, direct form).  A face
+// reads u only at its own canonical index (the rest comes from the tensor), and on a periodic axis
+// the thread of face 0 also writes its duplicate at n_A (which nobody reads), so out may be u.
 template <int A>
 struct K_stress_euler {
     static constexpr int MINB = 0;
@@ -726,11 +728,18 @@ struct K_stress_euler {
     static __device__ void run(const Args& a, const VGrid& vg) {
         const StressCtx& c = a.c;
         const uint32_t off = A == 0 ? 0u : (A == 1 ? c.P.f0 : c.P.f0 + c.P.f1);
+        const bool per = (c.P.per >> A) & 1u;
+        const uint32_t px = (uint32_t)c.P.n[0] + (A == 0), py = (uint32_t)c.P.n[1] + (A == 1);
+        const uint32_t dup = (uint32_t)c.P.n[A] * (A == 0 ? 1u : (A == 1 ? px : px * py));
         for_faces<A>(c.P, vg, [&](int x, int y, int z, uint32_t k) {
+            const int qa = A == 0 ? x : (A == 1 ? y : z);
+            if (per && qa == c.P.n[A]) return;  // written with face 0
             StressFace<A> s;
             s.load(c, a.u, x, y, z);
             s.fence();
-            a.out[off + k] = s.open ? s.u + a.dt * s.acc(c) : 0.0f;
+            const float v = s.open ? s.u + a.dt * s.acc(c) : 0.0f;
+            a.out[off + k] = v;
+            if (per && qa == 0) a.out[off + k + dup] = v;
         });
     }
 };
