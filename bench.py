@@ -123,8 +123,11 @@ def bench_step(args):
     two-phase viscous stresses (diffuse_surface_stress), viscous (capillary + gravity), the mixed
     projection to 1e-8 (warm-started from the previous step's potential) and fields + diagnostics;
     the projected faces feed the next step."""
+    import ctypes as Cty
+
     import torch
 
+    from mhd_simulator_amd import _lib
     from mhd_simulator_amd import configs as C
     from mhd_simulator_amd.fluid import FluidConfig, PressureProjection
     torch.cuda.set_device(0)
@@ -168,10 +171,32 @@ def bench_step(args):
 
     for w in range(max(args.warmup, 1)):
         step(w == 0, False)
+    # the projection's apply (the step's dominant kernel), timed live with HIP events
+    lib = pr.lib
+    lib.mhdp_set_kernel_timing(pr.handle, _lib.KERNELS["apply"])
+    kern_ms, kern_launches = 0.0, 0
     t0 = time.perf_counter()
     for _ in range(args.steps):
         d = step(False, True)
+        ms, cnt = Cty.c_double(), Cty.c_uint32()
+        lib.mhdp_kernel_time(pr.handle, Cty.byref(ms), Cty.byref(cnt))
+        kern_ms += ms.value
+        kern_launches += cnt.value
     wall = (time.perf_counter() - t0) / args.steps
+    lib.mhdp_set_kernel_timing(pr.handle, -1)
+    avg_launch_s = kern_ms / max(kern_launches, 1) * 1e-3
+    alg_bytes = KERNEL_BYTES["apply"] * g.cells
+    achieved = alg_bytes / avg_launch_s if avg_launch_s > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK, "traffic": None, "kernel": "apply",
+                "avg_launch_us": avg_launch_s * 1e6, "algorithmic_bytes_per_launch": alg_bytes}
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_step_baseline(128)
+            log(f"cpu baseline: {cpu}")
+        except Exception as e:
+            log(f"cpu baseline failed: {e}")
     per = {k: v / args.steps * 1e3 for k, v in stages.items()}
     log("step: " + ", ".join(f"{k} {v:.2f} ms" for k, v in per.items()) +
         f"; iterations {pr.stats['iterations']}, outer {pr.stats['outer_steps']}")
@@ -183,10 +208,96 @@ def bench_step(args):
         "config": {"workload": f"two-phase fluid step {n}^3, 200 lognormal droplets 7000/1.2 over an 8-cell plate",
                    "stages_ms": per, "iterations": pr.stats["iterations"], "outer_steps": pr.stats["outer_steps"],
                    "diagnostics": d},
-        "roofline": None, "cpu_baseline": None,
+        "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     pr.close()
+
+
+def em_workload(n):
+    """The EM bench's cells: a copper ring and a permeable block in air on an n^3 grid (h = 1 mm),
+    and the applied potential A_s = B0 x r / 2 of a uniform 10 mT field B0 z on the edges (x edges
+    carry -B0 y / 2, y edges B0 x / 2)."""
+    import numpy as np
+    dims = (n, n, n)
+    h = 1e-3
+    i = np.arange(n ** 3)
+    x, y, z = (i % n + 0.5) - n / 2, ((i // n) % n + 0.5) - n / 2, (i // (n * n) + 0.5) - n / 2
+    rho = np.sqrt(x * x + y * y)
+    m = np.zeros(n ** 3, np.uint32)
+    m[(np.abs(rho - 0.25 * n) < 0.06 * n) & (np.abs(z) < 0.06 * n)] = 1  # ring
+    m[(np.abs(x) < 0.08 * n) & (np.abs(y) < 0.08 * n) & (np.abs(z - 0.25 * n) < 0.05 * n)] = 2  # permeable block
+    props = np.array([[0.0, 1.0], [5.8e7, 1.0], [0.0, 100.0]], np.float32)
+    b0 = 0.01
+    parts = []
+    for a in range(3):
+        d = [n + (0 if b == a else 1) for b in range(3)]
+        zz, yy, xx = np.meshgrid(*(np.arange(d[k], dtype=np.float32) for k in (2, 1, 0)), indexing="ij")
+        pos = [xx, yy, zz]
+        pos[a] = pos[a] + 0.5
+        cx, cy = (pos[0] - n / 2) * h, (pos[1] - n / 2) * h
+        parts.append((-0.5 * b0 * cy if a == 0 else (0.5 * b0 * cx if a == 1 else np.zeros_like(cx))).ravel())
+    return dims, h, m, props, np.concatenate(parts).astype(np.float32)
+
+
+def em_kernel_bytes(dims):
+    """Algorithmic bytes per launch of the EM CG kernels (every array touched once): E edges, F faces,
+    N nodes; curl reads p and the face reluctivities and writes the face terms and node divergences;
+    apply reads flags, p, sigma, face terms and node divergences and writes q; update reads r, q, x,
+    p, diag, sigma and writes r, x; search reads r, diag, sigma, p and writes p."""
+    n = dims
+    E = sum((n[0] + (a != 0)) * (n[1] + (a != 1)) * (n[2] + (a != 2)) for a in range(3))
+    F = sum((n[0] + (a == 0)) * (n[1] + (a == 1)) * (n[2] + (a == 2)) for a in range(3))
+    N = (n[0] + 1) * (n[1] + 1) * (n[2] + 1)
+    return {"curl": 4 * E + 8 * F + 4 * N, "apply": 13 * E + 4 * F + 4 * N, "update": 32 * E, "search": 20 * E}
+
+
+def cpu_em_baseline(size: int):
+    """The oracle's FP32 Jacobi-CG (the reference's arithmetic) on the EM bench's geometry at a
+    bounded size: the switch-on solve to 1e-5."""
+    import numpy as np
+    from oracle import oracle
+    oracle.build()
+    dims, h, m, props, a_s = em_workload(size)
+    co = oracle.em_coefficients(dims, h, m, props)
+    zero = np.zeros_like(a_s)
+    t0 = time.perf_counter()
+    rc, a, st = oracle.em_solve(dims, h, co, a_s, zero, zero, dt=1e-5, tol=1e-5, max_iter=20000, precision=32)
+    dt = time.perf_counter() - t0
+    E = a_s.size
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    return {"value": E * st["iterations"] / dt, "unit": "edge-iterations/s", "cores": threads, "kind": "port",
+            "sample": f"oracle FP32 Jacobi-CG, switch-on solve to 1e-5 on the same geometry at {size}^3 "
+                      f"({E} edges, {st['iterations']} iterations, {dt:.2f} s, status {rc}; the operator "
+                      f"runs on all threads, vector updates and reductions on one)"}
+
+
+def cpu_step_baseline(size: int):
+    """The oracle's step on the plate workload at a bounded size: predict, the two-phase viscous
+    stresses, viscous, the FP64 MG projection to 1e-8 (cold) and fields."""
+    import numpy as np
+    from mhd_simulator_amd import configs as C
+    from oracle import oracle
+    oracle.build()
+    p = C.config5(size, droplets=max(1, round(200 * (size / 512) ** 3)))
+    g = p.grid
+    mask, _ = oracle.anchors(p.fluid, g.dims, g.periodic)
+    cells = np.zeros((g.cells, 16), np.float32)
+    cells[:, 14] = p.alpha
+    dt = 0.25 * g.h
+    threads = len(os.sched_getaffinity(0))
+    t0 = time.perf_counter()
+    pred = oracle.predict(g, mask, p.ustar, dt=dt)
+    rc_s, pred, _, _ = oracle.surface_stress(g, mask, cells, pred, dt, nu=1e-6, mu_air=1.8e-5)
+    ustar = oracle.viscous(g, mask, pred, cells, dt=dt, acceleration=(0.0, 0.0, -9.81), surface_tension=0.07)
+    rc, phi, u, st = oracle.project(g, mask, p.alpha, ustar, precond=2, precision=64, tol64=1e-8, max_iter=20000,
+                                    threads=threads)
+    oracle.fields(g, mask, u, phi, cells, dt=dt, acceleration=(0.0, 0.0, -9.81))
+    dt_s = time.perf_counter() - t0
+    return {"value": st["fluid_cells"] / dt_s, "unit": "cells/s", "cores": threads, "kind": "port",
+            "sample": f"oracle step (predict, stresses, viscous, FP64 MG projection to 1e-8 from zero, fields) on "
+                      f"the same generator at {size}^3 ({st['fluid_cells']} fluid cells, {st['iterations']} "
+                      f"iterations, {dt_s:.2f} s, status {rc}/{rc_s})"}
 
 
 def bench_em(args):
@@ -200,28 +311,10 @@ def bench_em(args):
     from mhd_simulator_amd.em import EmSolver
     torch.cuda.set_device(0)
     n = args.size if args.size != 512 else 128
-    dims = (n, n, n)
-    h = 1e-3
-    i = np.arange(n ** 3)
-    x, y, z = (i % n + 0.5) - n / 2, ((i // n) % n + 0.5) - n / 2, (i // (n * n) + 0.5) - n / 2
-    rho = np.sqrt(x * x + y * y)
-    m = np.zeros(n ** 3, np.uint32)
-    m[(np.abs(rho - 0.25 * n) < 0.06 * n) & (np.abs(z) < 0.06 * n)] = 1  # ring
-    m[(np.abs(x) < 0.08 * n) & (np.abs(y) < 0.08 * n) & (np.abs(z - 0.25 * n) < 0.05 * n)] = 2  # permeable block
-    props = np.array([[0.0, 1.0], [5.8e7, 1.0], [0.0, 100.0]], np.float32)
+    dims, h, m, props, a_s_host = em_workload(n)
     em = EmSolver(dims, h, m, props)
     E = em.edges
-    # A_s = B0 x r / 2 at the edge midpoints (x edges carry -B0 y / 2, y edges B0 x / 2)
-    b0 = 0.01
-    parts = []
-    for a in range(3):
-        d = [n + (0 if b == a else 1) for b in range(3)]
-        zz, yy, xx = np.meshgrid(*(np.arange(d[k], dtype=np.float32) for k in (2, 1, 0)), indexing="ij")
-        pos = [xx, yy, zz]
-        pos[a] = pos[a] + 0.5
-        cx, cy = (pos[0] - n / 2) * h, (pos[1] - n / 2) * h
-        parts.append((-0.5 * b0 * cy if a == 0 else (0.5 * b0 * cx if a == 1 else np.zeros_like(cx))).ravel())
-    a_s = torch.from_numpy(np.concatenate(parts).astype(np.float32)).cuda()
+    a_s = torch.from_numpy(a_s_host).cuda()
     zero = torch.zeros(E, dtype=torch.float32, device="cuda")
     a = torch.zeros(E, dtype=torch.float32, device="cuda")
     out = torch.empty_like(a)
@@ -234,16 +327,43 @@ def bench_em(args):
 
     for w in range(max(args.warmup, 1)):
         step(w == 0)
+    # the dominant CG kernel, timed live with HIP events during the timed steps
+    kbytes = em_kernel_bytes(dims)
+    best = (-1.0, "apply")
+    for name in kbytes:
+        em.set_kernel_timing(name)
+        step(False)
+        ms, cnt = em.kernel_time()
+        log(f"em kernel {name}: {ms:.2f} ms over {cnt} launches in one solve")
+        best = max(best, (ms, name))
+    kernel = best[1]
+    em.set_kernel_timing(kernel)
     torch.cuda.synchronize()
-    total_its = 0
+    total_its, kern_ms, kern_launches = 0, 0.0, 0
     t0 = time.perf_counter()
     for _ in range(args.steps):
         its, res = step(False)
         total_its += its
+        ms, cnt = em.kernel_time()
+        kern_ms += ms
+        kern_launches += cnt
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) / args.steps
+    em.set_kernel_timing(None)
     per_it = wall / max(total_its / args.steps, 1)
     log(f"em: {wall * 1e3:.1f} ms/step, {total_its / args.steps:.0f} iterations/step, {per_it * 1e3:.3f} ms/iteration")
+    avg_launch_s = kern_ms / max(kern_launches, 1) * 1e-3
+    achieved = kbytes[kernel] / avg_launch_s if avg_launch_s > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK, "traffic": None, "kernel": f"em_{kernel}",
+                "avg_launch_us": avg_launch_s * 1e6, "algorithmic_bytes_per_launch": kbytes[kernel]}
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_em_baseline(64)
+            log(f"cpu baseline: {cpu}")
+        except Exception as e:
+            log(f"cpu baseline failed: {e}")
     out_line = {
         "metric": "EM Jacobi-CG edge-iterations/s (rel. tol 1e-5)", "value": E * total_its / args.steps / wall,
         "unit": "edge-iterations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -252,7 +372,7 @@ def bench_em(args):
         "config": {"workload": f"EM induction step {n}^3 cells, {E} edges, field held after switch-on",
                    "iterations_per_step": total_its / args.steps, "ms_per_iteration": per_it * 1e3,
                    "rel_residual": res},
-        "roofline": None, "cpu_baseline": None,
+        "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out_line), flush=True)
     em.close()

@@ -247,6 +247,15 @@ int mhdp_em_solve_dev(mhdp_em *em, const mhdp_em_opts *opts, const float *as_now
 int mhdp_em_get_coefficients(const mhdp_em *em, float *sigma, float *diag, uint8_t *flags, float *nu);
 int mhdp_em_probe_apply(mhdp_em *em, const mhdp_em_opts *opts, const float *p, float *q);
 uint64_t mhdp_em_edges(const mhdp_em *em);
+/* Live kernel timing of the CG iterations (measurement hook, no counterpart in the reference):
+ * while enabled, HIP events bracket every launch of `kernel` during mhdp_em_solve_dev;
+ * mhdp_em_kernel_time() returns the summed duration (ms) and count of the last solve's launches
+ * that did work (a launch queued after convergence inside a batch returns at once).  Kernels:
+ * MHDP_EM_K_CURL (face curl terms and node divergences), MHDP_EM_K_APPLY (operator gather + p.q),
+ * MHDP_EM_K_UPDATE (x/r update + norms), MHDP_EM_K_SEARCH (new search direction). */
+enum { MHDP_EM_K_NONE = -1, MHDP_EM_K_CURL = 0, MHDP_EM_K_APPLY = 1, MHDP_EM_K_UPDATE = 2, MHDP_EM_K_SEARCH = 3 };
+int mhdp_em_set_kernel_timing(mhdp_em *em, int kernel);
+int mhdp_em_kernel_time(const mhdp_em *em, double *total_ms, uint32_t *launches);
 const char *mhdp_em_last_error(const mhdp_em *em);
 void mhdp_em_destroy(mhdp_em *em);
 

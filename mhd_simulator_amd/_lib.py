@@ -17,7 +17,8 @@ SYMBOLS = (
     "mhdp_set_potential",
     "mhdp_project_batch", "mhdp_fields_dev", "mhdp_predict_dev", "mhdp_viscous_dev", "mhdp_surface_stress_dev",
     "mhdp_em_create", "mhdp_em_set_materials", "mhdp_em_solve_dev", "mhdp_em_get_coefficients", "mhdp_em_probe_apply",
-    "mhdp_em_edges", "mhdp_em_last_error", "mhdp_em_destroy", "mhdp_last_error", "mhdp_destroy", "mhdp_device_bytes",
+    "mhdp_em_edges", "mhdp_em_set_kernel_timing", "mhdp_em_kernel_time", "mhdp_em_last_error", "mhdp_em_destroy",
+    "mhdp_last_error", "mhdp_destroy", "mhdp_device_bytes",
     "mhdp_anchors", "mhdp_hierarchy", "mhdp_set_kernel_timing", "mhdp_kernel_time",
     "mhdp_probe_coefficients", "mhdp_probe_apply", "mhdp_probe_vcycle", "mhdp_probe_mg_graph", "mhdp_probe_rhs",
 )
@@ -27,6 +28,7 @@ PRECOND = {"auto": 0, "jacobi": 1, "mg": 2}
 PRECISION = {"fp32": 0, "mixed": 1}
 GAUGE = {"auto": 0, "pinned": 2}
 KERNELS = {"apply": 0, "update": 1, "smooth": 2, "restrict": 3}
+EM_KERNELS = {"curl": 0, "apply": 1, "update": 2, "search": 3}
 
 
 class Grid(C.Structure):
@@ -128,6 +130,8 @@ def load(path: str = LIB_PATH):
         "mhdp_em_get_coefficients": (I, [P, P, P, P, P]),
         "mhdp_em_probe_apply": (I, [P, C.POINTER(EmOpts), P, P]),
         "mhdp_em_edges": (C.c_uint64, [P]),
+        "mhdp_em_set_kernel_timing": (I, [P, I]),
+        "mhdp_em_kernel_time": (I, [P, P, P]),
         "mhdp_em_last_error": (C.c_char_p, [P]),
         "mhdp_em_destroy": (None, [P]),
         "mhdp_last_error": (C.c_char_p, [P]),

@@ -780,7 +780,9 @@ void launch_em_prepare(hipStream_t s, const EmDims& d, const EmPhys& ph, const E
 }
 
 void launch_em_iterations(hipStream_t s, const EmDims& d, const EmPhys& ph, const EmVecs& v, double* part, void* scal,
-                          uint32_t count) {
+                          uint32_t count, KernelTimer* timer) {
+    KernelTimer off;
+    KernelTimer& T = timer ? *timer : off;
     const EmGeo g = geo_of(d);
     const uint32_t blocks = grid_blocks(g.Etot);
     const Phys& p = ph;
@@ -788,12 +790,20 @@ void launch_em_iterations(hipStream_t s, const EmDims& d, const EmPhys& ph, cons
     EmScal* sc = (EmScal*)scal;
     const uint32_t B = row_blocks(g);
     for (uint32_t k = 0; k < count; k++) {
+        T.before(0, s);
         go<K_em_curl>(s, 4 * B, {g, vv.nu, vv.p, v.tface, v.ndiv, sc, B});
+        T.after(0, s);
+        T.before(1, s);
         go<K_em_apply>(s, 3 * B, {g, p, vv, v.tface, v.ndiv, sc, part, B});
+        T.after(1, s);
         go<K_em_fold>(s, 1, {part, 3 * B, kFoldAlpha, sc});
+        T.before(2, s);
         go<K_em_update>(s, blocks, {g, p, vv, sc, part});
+        T.after(2, s);
         go<K_em_fold>(s, 1, {part, blocks, kFoldBeta, sc});
+        T.before(3, s);
         go<K_em_search>(s, blocks, {g, p, vv, sc});
+        T.after(3, s);
     }
 }
 

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "mhdp.h"
 #include "mhdp_kernels.h"
@@ -28,6 +29,9 @@ struct mhdp_em {
     double* part = nullptr;
     EmScal* sc = nullptr;
     EmScal* sc_h = nullptr;  // pinned host copy
+    KernelTimer timer;       // mhdp_em_set_kernel_timing
+    double timer_ms = 0.0;
+    uint32_t timer_launches = 0;
 };
 
 namespace {
@@ -164,11 +168,13 @@ int mhdp_em_solve_dev(mhdp_em* e, const mhdp_em_opts* o, const float* as_now, co
         if (!e->sc_h->rescale) break;
     }
     const uint32_t max_iter = o->max_iter;
+    e->timer.used = 0;
     EmScal s = *e->sc_h;
     for (;;) {
         while (s.status == 0 && finite_status(s) && s.its < max_iter) {
             const uint32_t before = s.its;
-            launch_em_iterations(e->stream, e->d, ph, v, e->part, e->sc, std::min<uint32_t>(max_iter - s.its, 32u));
+            launch_em_iterations(e->stream, e->d, ph, v, e->part, e->sc, std::min<uint32_t>(max_iter - s.its, 32u),
+                                 &e->timer);
             if ((rc = read_scalars(e))) return rc;
             s = *e->sc_h;
             // every batch either counts iterations or ends the solve (status); anything else is a
@@ -186,6 +192,24 @@ int mhdp_em_solve_dev(mhdp_em* e, const mhdp_em_opts* o, const float* as_now, co
         launch_em_restart(e->stream, e->d, ph, v, e->part, e->sc);
         if ((rc = read_scalars(e))) return rc;
         s = *e->sc_h;
+    }
+    // timed launches that did work (longer than a tenth of the longest; the rest returned at once)
+    e->timer_ms = 0.0;
+    e->timer_launches = 0;
+    {
+        std::vector<float> dur;
+        for (size_t k = 0; k + 1 < e->timer.used; k += 2) {
+            float t = 0.0f;
+            EM_HIP(hipEventElapsedTime(&t, e->timer.ev[k], e->timer.ev[k + 1]));
+            dur.push_back(t);
+        }
+        float longest = 0.0f;
+        for (float t : dur) longest = std::max(longest, t);
+        for (float t : dur)
+            if (t >= 0.1f * longest) {
+                e->timer_ms += t;
+                e->timer_launches++;
+            }
     }
     const double rel = std::sqrt(s.rr / std::max(s.bb, 1e-38));
     if (st) {
@@ -235,6 +259,19 @@ int mhdp_em_probe_apply(mhdp_em* e, const mhdp_em_opts* o, const float* p, float
 
 uint64_t mhdp_em_edges(const mhdp_em* e) { return e ? e->edges : 0; }
 
+int mhdp_em_set_kernel_timing(mhdp_em* e, int kernel) {
+    if (!e || kernel < MHDP_EM_K_NONE || kernel > MHDP_EM_K_SEARCH) return MHDP_INVALID;
+    e->timer.kernel = kernel;
+    return MHDP_OK;
+}
+
+int mhdp_em_kernel_time(const mhdp_em* e, double* total_ms, uint32_t* launches) {
+    if (!e) return MHDP_INVALID;
+    if (total_ms) *total_ms = e->timer_ms;
+    if (launches) *launches = e->timer_launches;
+    return MHDP_OK;
+}
+
 const char* mhdp_em_last_error(const mhdp_em* e) { return e ? e->err.c_str() : "invalid EM context"; }
 
 void mhdp_em_destroy(mhdp_em* e) {
@@ -245,6 +282,7 @@ void mhdp_em_destroy(mhdp_em* e) {
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (e->sc_h) (void)hipHostFree(e->sc_h);
+    for (hipEvent_t ev : e->timer.ev) (void)hipEventDestroy(ev);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }

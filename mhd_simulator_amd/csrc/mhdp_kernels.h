@@ -236,7 +236,7 @@ void launch_em_coefficients(hipStream_t s, const EmDims& d, const uint32_t* mate
                             float* sigma, float* diag, uint8_t* flags, float* nu, double* part, void* scal);
 void launch_em_prepare(hipStream_t s, const EmDims& d, const EmPhys& ph, const EmVecs& v, double* part, void* scal);
 void launch_em_iterations(hipStream_t s, const EmDims& d, const EmPhys& ph, const EmVecs& v, double* part, void* scal,
-                          uint32_t count);
+                          uint32_t count, KernelTimer* timer = nullptr);
 void launch_em_restart(hipStream_t s, const EmDims& d, const EmPhys& ph, const EmVecs& v, double* part, void* scal);
 void launch_em_commit(hipStream_t s, const EmDims& d, const float* a_prev, const float* x, float* out);
 void launch_em_probe_apply(hipStream_t s, const EmDims& d, const EmPhys& ph, const float* nu, const uint8_t* flags,

@@ -90,6 +90,17 @@ class EmSolver:
         self._check(self.lib.mhdp_em_probe_apply(self.handle, C.byref(o), _ptr(v), _ptr(q)))
         return q
 
+    def set_kernel_timing(self, kernel=None):
+        """Time every launch of one CG kernel ("curl", "apply", "update", "search"; None: off)
+        during the following solves with HIP events (mhdp_em_set_kernel_timing)."""
+        self._check(self.lib.mhdp_em_set_kernel_timing(self.handle, -1 if kernel is None else _lib.EM_KERNELS[kernel]))
+
+    def kernel_time(self):
+        """(total ms, launches) of the timed kernel's working launches in the last solve."""
+        ms, n = C.c_double(), C.c_uint32()
+        self._check(self.lib.mhdp_em_kernel_time(self.handle, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
     def close(self):
         if getattr(self, "handle", None):
             self.lib.mhdp_em_destroy(self.handle)

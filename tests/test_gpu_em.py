@@ -120,3 +120,24 @@ def test_em_rejects_conductor_on_the_boundary():
     m[0] = 1
     with pytest.raises(PressureError, match="A conductor touches the EM outer boundary"):
         EmSolver(dims, 1e-3, m, np.array([[0, 1], [1e6, 1]], np.float32))
+
+
+def test_em_kernel_timing_counts_working_launches():
+    import torch
+    dims, h, m, props, vac = case("block")
+    with EmSolver(dims, h, m, props, vac) as em:
+        E = em.edges
+        rng = np.random.default_rng(5)
+        dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+        zero = torch.zeros(E, dtype=torch.float32, device="cuda")
+        a_prev = dev((rng.normal(size=E) * 1e-6).astype(np.float32))
+        out = torch.zeros(E, dtype=torch.float32, device="cuda")
+        em.set_kernel_timing("apply")
+        its, _ = em.solve(zero, zero, a_prev, out, rel_tol=1e-6, max_iter=5000)
+        ms, launches = em.kernel_time()
+        # every iteration launches the kernel once; on a grid this small the launches queued after
+        # convergence inside a batch take about as long as working ones and may be counted too
+        assert its > 0 and ms > 0.0 and launches >= its
+        em.set_kernel_timing(None)
+        em.solve(zero, zero, a_prev, out, rel_tol=1e-6, max_iter=5000)
+        assert em.kernel_time() == (0.0, 0)
