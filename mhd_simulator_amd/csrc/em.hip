@@ -728,10 +728,14 @@ struct K_em_fold {
     }
 };
 
-// blocks per lattice of the row kernels: about a third of the edge kernels' grid each, a multiple
+// Blocks of the edge kernels: up to 2048 (measured at 128^3: the passes move 85-205 MB in 23-51 us,
+// and twice the pressure kernels' cap gives the waves that hide their gathers' latency; 61.6 ->
+// 53.2 ms per solve step)
+uint32_t em_grid(uint32_t n) { return grid_blocks(n, 2048); }
+// blocks per lattice of the row kernels: about a quarter of the edge kernels' grid each, a multiple
 // of 8 (every XCD gets whole groups), so that 4 groups stay within the partial records
 uint32_t row_blocks(const EmGeo& g) {
-    const uint32_t b = grid_blocks(g.Etot) / 4 / 8 * 8;
+    const uint32_t b = em_grid(g.Etot) / 4 / 8 * 8;
     return b < 8 ? 8 : b;
 }
 
@@ -756,7 +760,7 @@ EmGeo geo_of(const EmDims& d) {
 
 uint32_t em_blocks(const EmDims& d) {
     const EmGeo g = geo_of(d);
-    const uint32_t a = grid_blocks(g.Etot), b = 4 * row_blocks(g);
+    const uint32_t a = em_grid(g.Etot), b = 4 * row_blocks(g);
     return a > b ? a : b;
 }
 
@@ -764,15 +768,15 @@ void launch_em_coefficients(hipStream_t s, const EmDims& d, const uint32_t* mate
                             float* sigma, float* diag, uint8_t* flags, float* nu, double* part, void* scal) {
     const EmGeo g = geo_of(d);
     const uint32_t faces = g.F[0] + g.F[1] + g.F[2];
-    go<K_em_nu>(s, grid_blocks(faces), {g, materials, props, vacuum, nu});
-    const uint32_t blocks = grid_blocks(g.Etot);
+    go<K_em_nu>(s, em_grid(faces), {g, materials, props, vacuum, nu});
+    const uint32_t blocks = em_grid(g.Etot);
     go<K_em_coef>(s, blocks, {g, materials, props, vacuum, sigma, diag, flags, part});
     go<K_em_fold>(s, 1, {part, blocks, kFoldCheck, (EmScal*)scal});
 }
 
 void launch_em_prepare(hipStream_t s, const EmDims& d, const EmPhys& ph, const EmVecs& v, double* part, void* scal) {
     const EmGeo g = geo_of(d);
-    const uint32_t blocks = grid_blocks(g.Etot);
+    const uint32_t blocks = em_grid(g.Etot);
     const Phys& p = ph;
     const Vecs& vv = v;
     go<K_em_prepare>(s, blocks, {g, p, vv, (EmScal*)scal, part});
@@ -784,7 +788,7 @@ void launch_em_iterations(hipStream_t s, const EmDims& d, const EmPhys& ph, cons
     KernelTimer off;
     KernelTimer& T = timer ? *timer : off;
     const EmGeo g = geo_of(d);
-    const uint32_t blocks = grid_blocks(g.Etot);
+    const uint32_t blocks = em_grid(g.Etot);
     const Phys& p = ph;
     const Vecs& vv = v;
     EmScal* sc = (EmScal*)scal;
@@ -809,7 +813,7 @@ void launch_em_iterations(hipStream_t s, const EmDims& d, const EmPhys& ph, cons
 
 void launch_em_restart(hipStream_t s, const EmDims& d, const EmPhys& ph, const EmVecs& v, double* part, void* scal) {
     const EmGeo g = geo_of(d);
-    const uint32_t blocks = grid_blocks(g.Etot);
+    const uint32_t blocks = em_grid(g.Etot);
     const Phys& p = ph;
     const Vecs& vv = v;
     go<K_em_true_residual>(s, blocks, {g, p, vv, (EmScal*)scal, part});
@@ -818,7 +822,7 @@ void launch_em_restart(hipStream_t s, const EmDims& d, const EmPhys& ph, const E
 
 void launch_em_commit(hipStream_t s, const EmDims& d, const float* a_prev, const float* x, float* out) {
     const EmGeo g = geo_of(d);
-    go<K_em_commit>(s, grid_blocks(g.Etot), {g.Etot, a_prev, x, out});
+    go<K_em_commit>(s, em_grid(g.Etot), {g.Etot, a_prev, x, out});
 }
 
 // apply_operator alone (test hook): q = A p
@@ -847,5 +851,5 @@ void launch_em_probe_apply(hipStream_t s, const EmDims& d, const EmPhys& ph, con
     const EmGeo g = geo_of(d);
     const uint32_t B = row_blocks(g);
     go<K_em_curl>(s, 4 * B, {g, nu, p, tface, ndiv, nullptr, B});
-    go<K_em_probe>(s, grid_blocks(g.Etot), {g, ph, tface, ndiv, flags, sigma, p, q});
+    go<K_em_probe>(s, em_grid(g.Etot), {g, ph, tface, ndiv, flags, sigma, p, q});
 }
