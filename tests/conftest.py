@@ -21,7 +21,7 @@ def orc():
 
 @pytest.fixture(scope="session")
 def gpu_lib():
-    """The HIP library on a GPU box; fails loudly (no fallback) when it is not built."""
+    """The HIP library on a machine with a GPU; fails loudly (no fallback) when it is not built."""
     import torch
     if not torch.cuda.is_available():
         pytest.fail("GPU test collected without a GPU")
