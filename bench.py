@@ -354,8 +354,17 @@ def bench_em(args):
     log(f"em: {wall * 1e3:.1f} ms/step, {total_its / args.steps:.0f} iterations/step, {per_it * 1e3:.3f} ms/iteration")
     avg_launch_s = kern_ms / max(kern_launches, 1) * 1e-3
     achieved = kbytes[kernel] / avg_launch_s if avg_launch_s > 0 else 0.0
+    # HBM bytes per launch of the same kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+    traffic = None
+    try:
+        prof = json.load(open(os.path.join(ROOT, "profiles", "em_128_hbm_counters.json")))
+        if prof["edges"] == E and kernel in prof["kernels"]:
+            kk = prof["kernels"][kernel]
+            traffic = 1024.0 * (kk["fetch_kib"] * prof["copy_scale_fetch"] + kk["write_kib"] * prof["copy_scale_write"])
+    except (OSError, KeyError, ValueError):
+        traffic = None
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK, "traffic": None, "kernel": f"em_{kernel}",
+                "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": f"em_{kernel}",
                 "avg_launch_us": avg_launch_s * 1e6, "algorithmic_bytes_per_launch": kbytes[kernel]}
     cpu = None
     if not args.no_cpu:
