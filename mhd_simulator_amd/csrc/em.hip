@@ -401,7 +401,9 @@ struct K_em_curl {
     static __device__ void run(const Args& a, const VGrid& vg) {
         if (a.sc && a.sc->status != 0) return;
         const EmGeo g = a.g;
-        const uint32_t group = vg.b / a.B, local = vg.b % a.B;
+        // the four groups interleaved: consecutive logical blocks (one XCD, one L2) take the same
+        // slab of the three face lattices and of the nodes
+        const uint32_t group = vg.b % 4u, local = vg.b / 4u;
         const uint32_t nx = (uint32_t)g.n[0], ny = (uint32_t)g.n[1], nz = (uint32_t)g.n[2];
         const float* __restrict__ v = a.v;
         if (group < 3) {
@@ -520,12 +522,14 @@ struct K_em_apply {
         double* part;
         uint32_t B;
     };
-    // block group a: the edges of axis a, by rows
+    // the edges of axis vg.b % 3, by rows
     static __device__ void run(const Args& a, const VGrid& vg) {
         if (a.sc->status != 0) return;
         const EmGeo g = a.g;
-        const int axis = (int)(vg.b / a.B);
-        const uint32_t local = vg.b % a.B;
+        // the three axes interleaved: consecutive logical blocks (one XCD, one L2) take the same slab of
+        // the three edge lattices, which share their face terms and node divergences
+        const int axis = (int)(vg.b % 3u);
+        const uint32_t local = vg.b / 3u;
         const uint32_t off = axis == 0 ? 0u : (axis == 1 ? g.E[0] : g.E[0] + g.E[1]);
         const float* __restrict__ pv = a.v.p;
         float* __restrict__ q = a.v.q;
