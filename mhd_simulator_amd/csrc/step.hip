@@ -122,12 +122,18 @@ __device__ __forceinline__ void corners_load(const Planes& P, const float* __res
     uint32_t step[3];                                              // corner 1 minus corner 0
 #pragma unroll
     for (int b = 0; b < 3; b++) {
-        const bool per = (P.per >> b) & 1u;
         const int n = P.n[b], lim = n + (b == C ? 1 : 0);
-        const int w0 = wrap1(base[b], n), w1 = w0 + 1 == n ? 0 : w0 + 1;
-        const int c0 = per ? w0 : clampi(base[b], 0, lim - 1), c1 = per ? w1 : clampi(base[b] + 1, 0, lim - 1);
-        k.ok[b][0] = per || (base[b] >= 0 && base[b] < lim);
-        k.ok[b][1] = per || (base[b] + 1 >= 0 && base[b] + 1 < lim);
+        int c0, c1;
+        if ((P.per >> b) & 1u) {  // uniform over the launch: closed axes skip the wrap entirely
+            c0 = wrap1(base[b], n);
+            c1 = c0 + 1 == n ? 0 : c0 + 1;
+            k.ok[b][0] = k.ok[b][1] = true;
+        } else {
+            c0 = clampi(base[b], 0, lim - 1);
+            c1 = clampi(base[b] + 1, 0, lim - 1);
+            k.ok[b][0] = base[b] >= 0 && base[b] < lim;
+            k.ok[b][1] = base[b] + 1 >= 0 && base[b] + 1 < lim;
+        }
         idx += (uint32_t)c0 * stride[b];
         step[b] = (uint32_t)(c1 - c0) * stride[b];
     }
