@@ -314,13 +314,17 @@ def test_warm_start(orc, name, precision):
             assert orc.true_residual(p.grid, pr.mask, p.alpha, u2, pr.phi) <= 1e-8
 
 
+@pytest.mark.parametrize("name", ["plate-64", "periodic-odd"])
 @pytest.mark.parametrize("precision", ["fp32", "mixed"])
-def test_level1_w_cycle_converges_in_fewer_iterations(orc, precision):
+def test_level1_w_cycle_converges_in_fewer_iterations(orc, precision, name):
     """Two visits of level 1 per cycle (a W-cycle there; the solver's choice on grids of 2^24 cells
-    and more) against the reference's V-cycle (coarse_visits=1): both meet the bar on the plate
-    problem, the two-visit cycle needs fewer iterations and differs from the V-cycle."""
-    p = problem("plate-64")
-    with projection(p, relative_tolerance=1e-6, absolute_divergence_tolerance=0.0, max_iterations=20000) as pr:
+    and more) against the reference's V-cycle (coarse_visits=1): both meet the bar, the two-visit
+    cycle differs from the V-cycle and needs fewer iterations on the plate (at most one more on the
+    small odd grid).  The odd periodic grid's level 1 (9 x 7 x 5) cannot use bricks: its visits go
+    through the one-cell residual and the separate damped sum."""
+    p = problem(name)
+    tol = FP32_TOL.get(name, 1e-6)
+    with projection(p, relative_tolerance=tol, absolute_divergence_tolerance=0.0, max_iterations=20000) as pr:
         meanfree = pr.components == 1
         r = np.where(pr.mask != 0, np.cos(np.arange(p.grid.cells) * 0.17), 0).astype(np.float32)
         zv = pr.vcycle(r, meanfree, smooth_weight=0.9, coarse_scale=1.6, coarse_visits=1)
@@ -334,6 +338,6 @@ def test_level1_w_cycle_converges_in_fewer_iterations(orc, precision):
                 assert orc.true_residual(p.grid, pr.mask, p.alpha, p.ustar, pr.phi) <= 1e-8
             else:
                 its[visits], res = pr.project(p.ustar, precond="mg", coarse_visits=visits)
-                assert res <= 1e-6
-        print(f"[gpu] {precision} iterations V {its[1]} W {its[2]}", flush=True)
-        assert its[2] < its[1]
+                assert res <= FP32_TOL.get(name, 1e-6)
+        print(f"[gpu] {name} {precision} iterations V {its[1]} W {its[2]}", flush=True)
+        assert its[2] < its[1] if name == "plate-64" else its[2] <= its[1] + 1
