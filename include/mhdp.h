@@ -211,8 +211,8 @@ int mhdp_surface_stress_dev(mhdp_ctx *ctx, float *u_dev, const float *cells_dev,
  * EmState::advance_to, src/em/mod.rs:863-1010; coefficient set-up of EmState::new,
  * mod.rs:557-571).  Edges of axis a have dims n + 1 except n_a on axis a, packed x|y|z, one f32
  * each; the unknown is the magnetic vector potential A (tangential A held on the outer surface).
- * Not included: increment prediction (the solve starts from a zero increment) and the nodal
- * auxiliary-space preconditioner (plain Jacobi throughout). */
+ * Not included: the nodal auxiliary-space preconditioner (plain Jacobi throughout; the reference
+ * switches it on only after 1024 iterations). */
 typedef struct mhdp_em mhdp_em;
 typedef struct {
     float dt;          /* EmConfig.dt_s (Params.physical.x) */
@@ -225,6 +225,9 @@ typedef struct {
     uint32_t iterations;
     double rel_residual; /* sqrt(rr / bb) of the verified residual */
     double seconds;
+    float prediction;            /* the prediction factor used (0: zero start; prediction_diagnostic[0]) */
+    double start_residual;       /* sqrt(rr / bb) of the first start (prediction_diagnostic[1]) */
+    uint32_t prediction_dropped; /* 1: the predicted start was rejected (prediction_diagnostic[2]) */
 } mhdp_em_stats;
 /* A context for a grid of cells (periodic must be 0; h = GridSi.cell_m). */
 int mhdp_em_create(mhdp_em **out, const mhdp_grid *grid);
@@ -241,6 +244,17 @@ int mhdp_em_set_materials(mhdp_em *em, const uint32_t *materials, const float *p
  * text is the reference's "EM solve did not converge: ..." and a_out is untouched. */
 int mhdp_em_solve_dev(mhdp_em *em, const mhdp_em_opts *opts, const float *as_now_dev, const float *as_prev_dev,
                       const float *motional_dev, const float *a_prev_dev, float *a_out_dev, mhdp_em_stats *stats);
+/* The same step with the reference's increment prediction (begin_step and zero_guess,
+ * src/em/induction.wgsl:251-258; EmState::advance_to, src/em/mod.rs:869-886 and 906-925, where
+ * EmState keeps a_before in its `previous` buffer): the increment starts from
+ * prediction * (a_prev - a_before) on every non-boundary edge, a_before_dev being the potential
+ * accepted one step before a_prev and prediction = (t - t_last) / (t_last - t_before) (origin.w).
+ * A start whose residual is non-finite or above the right-hand side's norm is dropped for the
+ * zero start; the acceptance test stays relative to the right-hand side.  prediction <= 0 or not
+ * finite, a_before_dev NULL or magnetostatic: the zero start (= mhdp_em_solve_dev). */
+int mhdp_em_solve_predicted_dev(mhdp_em *em, const mhdp_em_opts *opts, const float *as_now_dev,
+                                const float *as_prev_dev, const float *motional_dev, const float *a_prev_dev,
+                                const float *a_before_dev, float prediction, float *a_out_dev, mhdp_em_stats *stats);
 /* Test hooks: the per-edge coefficients (sigma, diag, flags: 1 boundary, 2/4 lower/upper air node,
  * 8 air stencil) and per-face reluctivity, host pointers (any may be NULL); apply_operator on a
  * host vector. */

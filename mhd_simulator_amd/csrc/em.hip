@@ -1,6 +1,6 @@
 // The EM step's Jacobi-CG on the edge curl-curl operator (src/em/induction.wgsl; the loop of
-// EmState::advance_to, src/em/mod.rs:863-1010), without increment prediction and without the
-// nodal auxiliary preconditioner (plain Jacobi, as the reference runs below its nodal switch).
+// EmState::advance_to, src/em/mod.rs:863-1010) with its increment prediction, without the nodal
+// auxiliary preconditioner (plain Jacobi, as the reference runs below its nodal switch).
 // Built with -ffp-contract=off: the operator, right-hand side and coefficients keep the
 // reference's FP32 operation order and equal the oracle's bit for bit.  Reductions are FP64
 // per-block partials folded in block order by one block (deterministic).
@@ -320,7 +320,9 @@ __device__ __forceinline__ float em_rhs(const EmGeo& g, const Phys& ph, const Ve
     return b;
 }
 
-// prepare (induction.wgsl:116-129) from a zero increment; partials (rz, rr, bb, |b| before scaling)
+// prepare (induction.wgsl:116-129): from a zero increment (guess 0: x is cleared) or from the
+// increment x already holds (guess 1: r = b - (C^T nu C x + gauge(x) + mass x), FP32 in that
+// order); partials (rz, rr, bb, |b| before scaling)
 struct K_em_prepare {
     static constexpr int MINB = 0;
     struct Args {
@@ -329,6 +331,7 @@ struct K_em_prepare {
         Vecs v;
         EmScal* sc;
         double* part;
+        int guess;
     };
     static __device__ void run(const Args& a, const VGrid& vg) {
         const EmGeo g = a.g;
@@ -340,17 +343,47 @@ struct K_em_prepare {
             int p[3];
             const int axis = edge_decode(g, i, p);
             float b = em_rhs(g, a.ph, a.v, i, axis, p, false);
+            float r = b;
+            if (a.guess) {
+                const uint32_t fl = a.v.flags[i];
+                if (!(fl & kBoundary)) {
+                    const float ax = adjoint(g, a.v.nu, a.v.x, axis, p, false) + gauge(g, fl, a.v.x, axis, p);
+                    r = b - (ax + mass(g, a.ph, a.v.sigma[i]) * a.v.x[i]);
+                }
+            } else {
+                a.v.x[i] = 0.0f;
+            }
             acc[3] += fabs((double)b);
             b = b / scale;
-            const float z = b / diagonal(g, a.ph, a.v.diag[i], a.v.sigma[i]);
-            a.v.x[i] = 0.0f;
-            a.v.r[i] = b;
+            r = r / scale;
+            const float z = r / diagonal(g, a.ph, a.v.diag[i], a.v.sigma[i]);
+            a.v.r[i] = r;
             a.v.p[i] = z;
-            acc[0] += (double)b * (double)z;
-            acc[1] += (double)b * (double)b;
+            acc[0] += (double)r * (double)z;
+            acc[1] += (double)r * (double)r;
             acc[2] += (double)b * (double)b;
         }
         block_partials<0, 0>(acc, a.part, vg.b);
+    }
+};
+
+// begin_step (induction.wgsl:251-256): the increment's start prediction * (a_prev - a_before) on
+// every non-boundary edge, 0 on the boundary
+struct K_em_begin {
+    static constexpr int MINB = 0;
+    struct Args {
+        uint32_t n;
+        const uint8_t* flags;
+        const float* a_prev;
+        const float* a_before;
+        float prediction;
+        float* x;
+    };
+    static __device__ void run(const Args& a, const VGrid& vg) {
+        uint32_t lo, hi;
+        block_range(vg, a.n, lo, hi);
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK)
+            a.x[i] = (a.flags[i] & kBoundary) ? 0.0f : a.prediction * (a.a_prev[i] - a.a_before[i]);
     }
 };
 
@@ -778,12 +811,19 @@ void launch_em_coefficients(hipStream_t s, const EmDims& d, const uint32_t* mate
     go<K_em_fold>(s, 1, {part, blocks, kFoldCheck, (EmScal*)scal});
 }
 
-void launch_em_prepare(hipStream_t s, const EmDims& d, const EmPhys& ph, const EmVecs& v, double* part, void* scal) {
+void launch_em_begin(hipStream_t s, const EmDims& d, const uint8_t* flags, const float* a_prev, const float* a_before,
+                     float prediction, float* x) {
+    const EmGeo g = geo_of(d);
+    go<K_em_begin>(s, em_grid(g.Etot), {g.Etot, flags, a_prev, a_before, prediction, x});
+}
+
+void launch_em_prepare(hipStream_t s, const EmDims& d, const EmPhys& ph, const EmVecs& v, double* part, void* scal,
+                       int guess) {
     const EmGeo g = geo_of(d);
     const uint32_t blocks = em_grid(g.Etot);
     const Phys& p = ph;
     const Vecs& vv = v;
-    go<K_em_prepare>(s, blocks, {g, p, vv, (EmScal*)scal, part});
+    go<K_em_prepare>(s, blocks, {g, p, vv, (EmScal*)scal, part, guess});
     go<K_em_fold>(s, 1, {part, blocks, kFoldInitial, (EmScal*)scal});
 }
 

@@ -145,6 +145,12 @@ int mhdp_em_set_materials(mhdp_em* e, const uint32_t* materials, const float* pr
 
 int mhdp_em_solve_dev(mhdp_em* e, const mhdp_em_opts* o, const float* as_now, const float* as_prev,
                       const float* motional, const float* a_prev, float* a_out, mhdp_em_stats* st) {
+    return mhdp_em_solve_predicted_dev(e, o, as_now, as_prev, motional, a_prev, nullptr, 0.0f, a_out, st);
+}
+
+int mhdp_em_solve_predicted_dev(mhdp_em* e, const mhdp_em_opts* o, const float* as_now, const float* as_prev,
+                                const float* motional, const float* a_prev, const float* a_before, float prediction,
+                                float* a_out, mhdp_em_stats* st) {
     if (!e) return MHDP_INVALID;
     e->err.clear();
     if (!e->ready) return fail(e, MHDP_INVALID, "EM materials are not set");
@@ -156,16 +162,29 @@ int mhdp_em_solve_dev(mhdp_em* e, const mhdp_em_opts* o, const float* as_now, co
     const EmPhys ph{o->dt, o->mu0, o->magnetostatic ? 1 : 0};
     const EmVecs v{e->x,     e->r,    e->p,    e->q,     e->sigma, e->diag,  e->nu,
                    e->flags, as_now,  as_prev, motional, a_prev,   e->tface, e->ndiv};
-    // begin_step: scale 1, zero increment; prepare + initial_reduce (repeated once on a new scale)
+    // begin_step: scale 1; the increment starts from prediction * (a_prev - a_before) (the
+    // reference predicts after a regular step with a positive finite ratio, mod.rs:869-886) or
+    // from zero; prepare + initial_reduce, repeated once on a new scale
+    const bool guess = a_before && !o->magnetostatic && std::isfinite(prediction) && prediction > 0.0f;
     EmScal init{};
     init.scale = 1.0;
     init.tol2 = (double)(o->rel_tol * o->rel_tol);  // Params.physical.w: f32 relative_tolerance.powi(2)
     EM_HIP(hipMemcpyAsync(e->sc, &init, sizeof init, hipMemcpyHostToDevice, e->stream));
+    if (guess) launch_em_begin(e->stream, e->d, e->flags, a_prev, a_before, prediction, e->x);
     int rc;
     for (int pass = 0; pass < 2; pass++) {
-        launch_em_prepare(e->stream, e->d, ph, v, e->part, e->sc);
+        launch_em_prepare(e->stream, e->d, ph, v, e->part, e->sc, guess ? 1 : 0);
         if ((rc = read_scalars(e))) return rc;
         if (!e->sc_h->rescale) break;
+    }
+    const double start_residual = std::sqrt(e->sc_h->rr / std::max(e->sc_h->bb, 1e-38));
+    bool dropped = false;
+    if (guess && (!finite_status(*e->sc_h) || !std::isfinite(e->sc_h->l1) || e->sc_h->rr > e->sc_h->bb)) {
+        // a bad extrapolation must not start farther from the same right-hand side: zero_guess
+        // and prepare again (mod.rs:906-925); the scale picked above stays
+        launch_em_prepare(e->stream, e->d, ph, v, e->part, e->sc, 0);
+        if ((rc = read_scalars(e))) return rc;
+        dropped = true;
     }
     const uint32_t max_iter = o->max_iter;
     e->timer.used = 0;
@@ -216,6 +235,9 @@ int mhdp_em_solve_dev(mhdp_em* e, const mhdp_em_opts* o, const float* as_now, co
         st->iterations = s.its;
         st->rel_residual = rel;
         st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        st->prediction = guess ? prediction : 0.0f;
+        st->start_residual = start_residual;
+        st->prediction_dropped = dropped ? 1u : 0u;
     }
     if (s.status != 1 || !finite_status(s)) {  // src/em/mod.rs:1011-1021
         char buf[320];

@@ -234,7 +234,12 @@ struct EmScal {
 uint32_t em_blocks(const EmDims& d);  // partial records (4 doubles each) of the edge kernels
 void launch_em_coefficients(hipStream_t s, const EmDims& d, const uint32_t* materials, const float* props, int vacuum,
                             float* sigma, float* diag, uint8_t* flags, float* nu, double* part, void* scal);
-void launch_em_prepare(hipStream_t s, const EmDims& d, const EmPhys& ph, const EmVecs& v, double* part, void* scal);
+// begin_step: x = prediction * (a_prev - a_before) off the boundary; prepare from x (guess 1) or
+// from a zero increment (guess 0, x cleared)
+void launch_em_begin(hipStream_t s, const EmDims& d, const uint8_t* flags, const float* a_prev, const float* a_before,
+                     float prediction, float* x);
+void launch_em_prepare(hipStream_t s, const EmDims& d, const EmPhys& ph, const EmVecs& v, double* part, void* scal,
+                       int guess);
 void launch_em_iterations(hipStream_t s, const EmDims& d, const EmPhys& ph, const EmVecs& v, double* part, void* scal,
                           uint32_t count, KernelTimer* timer = nullptr);
 void launch_em_restart(hipStream_t s, const EmDims& d, const EmPhys& ph, const EmVecs& v, double* part, void* scal);

@@ -63,15 +63,21 @@ class EmSolver:
         return _lib.EmOpts(dt, mu0, rel_tol, max_iter, int(bool(magnetostatic)))
 
     def solve(self, as_now, as_prev, a_prev, a_out, *, motional=None, dt=1e-5, rel_tol=1e-5, max_iter=1000,
-              magnetostatic=False, mu0=MU0):
-        """a_out = a_prev + the step's increment; returns (iterations, relative residual)."""
-        for t in (as_now, as_prev, a_prev, a_out) + ((motional,) if motional is not None else ()):
+              magnetostatic=False, mu0=MU0, a_before=None, prediction=0.0):
+        """a_out = a_prev + the step's increment; returns (iterations, relative residual).  With
+        a_before (the potential accepted one step before a_prev) and prediction > 0 (the ratio of
+        this step's dt to the last one's) the increment starts from prediction * (a_prev - a_before)
+        (mhdp_em_solve_predicted_dev); stats["prediction_dropped"] tells whether that start was
+        rejected for the zero start."""
+        extra = tuple(t for t in (motional, a_before) if t is not None)
+        for t in (as_now, as_prev, a_prev, a_out) + extra:
             if not hasattr(t, "data_ptr") or t.numel() != self.edges:
                 raise ValueError("the EM solve takes device tensors with one value per edge")
         o = self._opts(dt, rel_tol, max_iter, magnetostatic, mu0)
         st = _lib.EmStats()
-        self._check(self.lib.mhdp_em_solve_dev(self.handle, C.byref(o), _ptr(as_now), _ptr(as_prev), _ptr(motional),
-                                               _ptr(a_prev), _ptr(a_out), C.byref(st)))
+        self._check(self.lib.mhdp_em_solve_predicted_dev(self.handle, C.byref(o), _ptr(as_now), _ptr(as_prev),
+                                                         _ptr(motional), _ptr(a_prev), _ptr(a_before),
+                                                         float(prediction), _ptr(a_out), C.byref(st)))
         self.stats = st.as_dict()
         return int(st.iterations), float(st.rel_residual)
 

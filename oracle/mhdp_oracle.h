@@ -167,6 +167,8 @@ typedef struct {
     uint32_t iterations;
     double rel_residual; /* sqrt(rr / bb) of the last reduction */
     double scale;        /* power-of-two residual scale picked by initial_reduce */
+    double start_residual;       /* sqrt(rr / bb) of the first start (prediction_diagnostic[1]) */
+    uint32_t prediction_dropped; /* the predicted start was rejected (prediction_diagnostic[2]) */
 } orc_em_stats;
 /* coefficients (induction.wgsl:82-99): materials one u32 per cell, props[2k], props[2k+1] = (sigma,
  * mu_r) of material k; vacuum = Params.control.z == 1.  Out per edge: sigma (mean of the four
@@ -181,15 +183,19 @@ void orc_em_apply_f32(const orc_em_grid *g, const orc_em_params *p, const float 
                       const uint8_t *flags, const float *nu, const float *v, float *q);
 void orc_em_apply_f64(const orc_em_grid *g, const orc_em_params *p, const float *sigma, const float *diag,
                       const uint8_t *flags, const float *nu, const double *v, double *q);
-/* One step's solve from a zero increment: as_now / as_prev the applied potential (source.x/.y), mot
- * the motional source (source[i + E].x, NULL = 0), a_prev the accepted potential; a_out = a_prev +
- * increment.  Returns 0 converged, 1 not converged, 3 non-finite. */
+/* One step's solve: as_now / as_prev the applied potential (source.x/.y), mot the motional source
+ * (source[i + E].x, NULL = 0), a_prev the accepted potential; a_out = a_prev + increment.  With
+ * a_before (the potential accepted one step earlier, NULL = none) and prediction > 0 (origin.w) the
+ * increment starts from prediction * (a_prev - a_before) off the boundary, else from zero.  Returns 0
+ * converged, 1 not converged, 3 non-finite. */
 int orc_em_solve_f32(const orc_em_grid *g, const orc_em_params *p, const float *sigma, const float *diag,
                      const uint8_t *flags, const float *nu, const float *as_now, const float *as_prev,
-                     const float *mot, const float *a_prev, float *a_out, orc_em_stats *st);
+                     const float *mot, const float *a_prev, const float *a_before, float prediction, float *a_out,
+                     orc_em_stats *st);
 int orc_em_solve_f64(const orc_em_grid *g, const orc_em_params *p, const float *sigma, const float *diag,
                      const uint8_t *flags, const float *nu, const float *as_now, const float *as_prev,
-                     const float *mot, const float *a_prev, double *a_out, orc_em_stats *st);
+                     const float *mot, const float *a_prev, const float *a_before, float prediction, double *a_out,
+                     orc_em_stats *st);
 
 /* ---- multigrid probes (mirror the reference's test-only entry points) ---- */
 typedef struct orc_mg orc_mg;

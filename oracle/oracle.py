@@ -265,7 +265,8 @@ class _EmParams(C.Structure):
 
 
 class _EmStats(C.Structure):
-    _fields_ = [("iterations", C.c_uint32), ("rel_residual", C.c_double), ("scale", C.c_double)]
+    _fields_ = [("iterations", C.c_uint32), ("rel_residual", C.c_double), ("scale", C.c_double),
+                ("start_residual", C.c_double), ("prediction_dropped", C.c_uint32)]
 
 
 def em_counts(dims):
@@ -309,8 +310,9 @@ def em_apply(dims, h, co, v, dt=1e-5, mu0=4e-7 * np.pi, magnetostatic=False, pre
 
 
 def em_solve(dims, h, co, as_now, as_prev, a_prev, mot=None, dt=1e-5, mu0=4e-7 * np.pi, tol=1e-5,
-             max_iter=2000, magnetostatic=False, precision=64):
-    """One step's Jacobi-CG solve (src/em/mod.rs:863-1010) -> (status, a_out, stats dict)"""
+             max_iter=2000, magnetostatic=False, precision=64, a_before=None, prediction=0.0):
+    """One step's Jacobi-CG solve (src/em/mod.rs:863-1010) -> (status, a_out, stats dict); with
+    a_before and prediction > 0 the increment starts from prediction * (a_prev - a_before)."""
     edges, _ = em_counts(dims)
     E = sum(edges)
     dt_ = np.float64 if precision == 64 else np.float32
@@ -321,10 +323,11 @@ def em_solve(dims, h, co, as_now, as_prev, a_prev, mot=None, dt=1e-5, mu0=4e-7 *
     f32 = lambda a: _ptr(np.ascontiguousarray(a, dtype=np.float32))  # noqa: E731
     keep = [np.ascontiguousarray(a, dtype=np.float32) for a in (as_now, as_prev, a_prev)]
     mot_ = None if mot is None else np.ascontiguousarray(mot, dtype=np.float32)
+    before = None if a_before is None else np.ascontiguousarray(a_before, dtype=np.float32)
     fn = lib().orc_em_solve_f64 if precision == 64 else lib().orc_em_solve_f32
     rc = fn(C.byref(g), C.byref(prm), _ptr(co["sigma"]), _ptr(co["diag"]), _ptr(co["flags"]), _ptr(co["nu"]),
-            _ptr(keep[0]), _ptr(keep[1]), None if mot_ is None else _ptr(mot_), _ptr(keep[2]), _ptr(a_out),
-            C.byref(st))
+            _ptr(keep[0]), _ptr(keep[1]), None if mot_ is None else _ptr(mot_), _ptr(keep[2]),
+            None if before is None else _ptr(before), C.c_float(prediction), _ptr(a_out), C.byref(st))
     del f32
     return rc, a_out, {k: getattr(st, k) for k, _ in _EmStats._fields_}
 

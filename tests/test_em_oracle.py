@@ -185,3 +185,105 @@ def test_solve_satisfies_the_step_equation(orc, with_sources):
     err = np.linalg.norm((lhs - rhs)[inner]) / np.linalg.norm(rhs[inner])
     assert err < 1e-9, err
     assert np.array_equal(a[bnd], a_prev[bnd].astype(np.float64))  # held tangential potential
+
+
+def sourced_case(orc, seed=7, amp=1e-6):
+    dims = (7, 6, 5)
+    h, props, mat, co = interior_case(orc, dims)
+    E = co["sigma"].size
+    rng = np.random.default_rng(seed)
+    a_prev = (rng.normal(size=E) * amp).astype(np.float32)
+    as_now = (rng.normal(size=E) * amp).astype(np.float32)
+    as_prev = (rng.normal(size=E) * amp).astype(np.float32)
+    mot = (rng.normal(size=E) * amp * 1e3).astype(np.float32)
+    return dims, h, co, a_prev, as_now, as_prev, mot
+
+
+@pytest.mark.parametrize("amp", [1e-6, 1e-15])
+def test_prediction_from_an_exact_history_starts_at_the_solution(orc, amp):
+    """begin_step + prepare with a guess (induction.wgsl:116-124, 251-256): when prediction *
+    (a_prev - a_before) is the step's exact increment, the start's residual b - A x0 is the exact
+    solve's own residual plus FP32 rounding and the solve takes no iteration; the reversed history starts at residual 2 |b|
+    (b - A(-x)) and is dropped for the zero start.  amp 1e-15 takes the power-of-two rescale
+    (|b|_1 < 1e-8) with the guess in place."""
+    dims, h, co, a_prev, as_now, as_prev, mot = sourced_case(orc, amp=amp)
+    rc, a, st = orc.em_solve(dims, h, co, as_now, as_prev, a_prev, mot=mot, tol=1e-12, max_iter=5000)
+    assert rc == 0 and st["start_residual"] == 1.0 and st["prediction_dropped"] == 0
+    assert (st["scale"] == 1.0) == (amp == 1e-6)
+    x = a - a_prev
+    for f in (1.0, 2.0, 0.5):
+        before = (a_prev - x / f).astype(np.float32)
+        for prec in (64, 32):
+            rc, _, sp = orc.em_solve(dims, h, co, as_now, as_prev, a_prev, mot=mot, tol=1e-5, max_iter=5000,
+                                     a_before=before, prediction=f, precision=prec)
+            assert rc == 0 and sp["iterations"] == 0 and sp["prediction_dropped"] == 0, (f, prec, sp)
+            # the exact solve's own residual (its absolute floor at the tiny scale) + FP32 rounding
+            assert sp["start_residual"] < st["rel_residual"] + 3e-7, (f, prec, sp)
+            assert sp["scale"] == st["scale"]
+        rc, _, sn = orc.em_solve(dims, h, co, as_now, as_prev, a_prev, mot=mot, tol=1e-6, max_iter=5000,
+                                 a_before=(a_prev + x / f).astype(np.float32), prediction=f)
+        assert rc == 0 and sn["prediction_dropped"] == 1 and abs(sn["start_residual"] - 2.0) < 1e-6
+
+
+@pytest.mark.parametrize("poison", [1e-3, 1e20])
+def test_bad_history_falls_back_to_the_identical_zero_start(orc, poison):
+    """The reference's predicted_em_bad_history_falls_back_to_identical_zero_guess_fields
+    (src/em/prediction_tests.rs:170-188): a poisoned history is dropped and the step equals the
+    zero start bit for bit."""
+    dims, h, co, a_prev, as_now, as_prev, mot = sourced_case(orc)
+    kw = dict(mot=mot, tol=1e-6, max_iter=5000, precision=32)
+    rc0, a0, s0 = orc.em_solve(dims, h, co, as_now, as_prev, a_prev, **kw)
+    rc1, a1, s1 = orc.em_solve(dims, h, co, as_now, as_prev, a_prev, a_before=np.full(a_prev.size, poison, np.float32),
+                               prediction=1.0, **kw)
+    assert rc0 == rc1 == 0 and s1["prediction_dropped"] == 1 and s1["start_residual"] > 1.0
+    assert s1["iterations"] == s0["iterations"]
+    assert np.array_equal(a0.view(np.uint32), a1.view(np.uint32))
+
+
+def test_prediction_is_ignored_where_the_reference_does_not_predict(orc):
+    """advance_to predicts only on a regular step with a positive finite ratio (mod.rs:869-886):
+    the initial (magnetostatic) solve and factors 0, -1, nan start from zero."""
+    dims, h, co, a_prev, as_now, as_prev, mot = sourced_case(orc)
+    before = (a_prev * np.float32(0.5)).astype(np.float32)
+    for ms in (False, True):
+        kw = dict(mot=mot, tol=1e-6, max_iter=5000, precision=32, magnetostatic=ms)
+        _, a0, s0 = orc.em_solve(dims, h, co, as_now, as_prev, a_prev, **kw)
+        for f in ((0.0, -1.0, float("nan")) if not ms else (1.0,)):
+            _, a1, s1 = orc.em_solve(dims, h, co, as_now, as_prev, a_prev, a_before=before, prediction=f, **kw)
+            assert s1["start_residual"] == 1.0 and s1["prediction_dropped"] == 0
+            assert np.array_equal(a0.view(np.uint32), a1.view(np.uint32)), (ms, f)
+
+
+def test_predicted_steps_track_timestep_changes(orc):
+    """The reference's predicted_em_tracks_timestep_changes_with_the_original_rhs_tolerance
+    (src/em/prediction_tests.rs:190-210): a sinusoidally driven conductor stepped with dt 1e-5,
+    1e-5, 0.5e-5, 0.5e-5, 2e-5, 1e-5, the factor dt / previous dt; every predicted step meets the
+    tolerance relative to its right-hand side and stays within 2e-4 (relative L2) of the zero-start
+    steps, with fewer iterations in total."""
+    dims = (7, 6, 5)
+    h, props, mat, co = interior_case(orc, dims)
+    E = co["sigma"].size
+    rng = np.random.default_rng(5)
+    shape = (rng.normal(size=E) * 1e-5).astype(np.float32)
+    shape[boundary_edges(dims)] = 0.0
+    drive = lambda t: (shape * np.float32(math.sin(2 * math.pi * 2000.0 * t))).astype(np.float32)  # noqa: E731
+    tol = 1e-5
+    plain = pred = before = np.zeros(E, np.float32)
+    t, prev_dt, its_plain, its_pred = 0.0, 0.0, 0, 0
+    for dt in (1e-5, 1e-5, 0.5e-5, 0.5e-5, 2e-5, 1e-5):
+        as_prev, as_now = drive(t), drive(t + dt)
+        f = dt / prev_dt if prev_dt > 0 else 0.0
+        rc0, a0, s0 = orc.em_solve(dims, h, co, as_now, as_prev, plain, dt=dt, tol=tol, max_iter=5000, precision=32)
+        rc1, a1, s1 = orc.em_solve(dims, h, co, as_now, as_prev, pred, dt=dt, tol=tol, max_iter=5000, precision=32,
+                                   a_before=before, prediction=f)
+        assert rc0 == rc1 == 0 and s1["rel_residual"] <= tol and s1["prediction_dropped"] == 0
+        if f > 0:
+            assert s1["start_residual"] < 1.0
+        rel = np.linalg.norm(a1.astype(np.float64) - a0) / np.linalg.norm(a0.astype(np.float64))
+        assert rel < 2e-4, (dt, rel)
+        its_plain += s0["iterations"]
+        its_pred += s1["iterations"]
+        plain, before, pred = a0, pred, a1
+        t += dt
+        prev_dt = dt
+    assert its_pred < its_plain, (its_pred, its_plain)

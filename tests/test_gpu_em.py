@@ -141,3 +141,72 @@ def test_em_kernel_timing_counts_working_launches():
         em.set_kernel_timing(None)
         em.solve(zero, zero, a_prev, out, rel_tol=1e-6, max_iter=5000)
         assert em.kernel_time() == (0.0, 0)
+
+
+def sourced(em, seed=11):
+    rng = np.random.default_rng(seed)
+    E = em.edges
+    a_prev = (rng.normal(size=E) * 1e-6).astype(np.float32)
+    as_now = (rng.normal(size=E) * 1e-6).astype(np.float32)
+    as_prev = (rng.normal(size=E) * 1e-6).astype(np.float32)
+    mot = (rng.normal(size=E) * 1e-3).astype(np.float32)
+    return a_prev, as_now, as_prev, mot
+
+
+@pytest.mark.parametrize("name", ["block", "permeable"])
+@pytest.mark.parametrize("factor", [1.0, 2.0])
+def test_em_predicted_solve_matches_oracle(orc, name, factor):
+    """Increment prediction (mhdp_em_solve_predicted_dev): a history whose prediction is 0.9 of the
+    step's increment starts near the solution on the GPU as in the oracle (same start residual up
+    to FP64 summation order, not dropped), then meets the solve bars of the zero start; an exact
+    history starts within FP32 rounding of the solution and takes no iteration."""
+    import torch
+    dims, h, m, props, vac = case(name)
+    tol = 1e-6
+    with EmSolver(dims, h, m, props, vac) as em:
+        a_prev, as_now, as_prev, mot = sourced(em)
+        co = orc.em_coefficients(dims, h, m, props, vac)
+        kw = dict(mot=mot, dt=1e-5, max_iter=20000)
+        rc64, a64, _ = orc.em_solve(dims, h, co, as_now, as_prev, a_prev, tol=1e-12, precision=64, **kw)
+        assert rc64 == 0
+        x = a64 - a_prev
+        dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+        out = torch.zeros(em.edges, dtype=torch.float32, device="cuda")
+        for share, bound in ((0.9, None), (1.0, 3e-7)):
+            before = (a_prev - share * x / factor).astype(np.float32)
+            print(f"[gpu] em predicted solve {name} factor={factor} share={share} dims={dims}", flush=True)
+            its, res = em.solve(dev(as_now), dev(as_prev), dev(a_prev), out, motional=dev(mot), dt=1e-5, rel_tol=tol,
+                                max_iter=20000, a_before=dev(before), prediction=factor)
+            st = em.stats
+            rc32, a32, st32 = orc.em_solve(dims, h, co, as_now, as_prev, a_prev, tol=tol, precision=32,
+                                           a_before=before, prediction=factor, **kw)
+            assert rc32 == 0 and res <= tol
+            assert st["prediction"] == np.float32(factor) and st["prediction_dropped"] == st32["prediction_dropped"] == 0
+            assert abs(st["start_residual"] - st32["start_residual"]) <= 1e-5 * st32["start_residual"] + 1e-9
+            assert abs(its - st32["iterations"]) <= 2 + 0.05 * st32["iterations"], (its, st32["iterations"])
+            a_gpu = out.cpu().numpy().astype(np.float64)
+            ref = np.linalg.norm(a32.astype(np.float64) - a64)
+            assert np.linalg.norm(a_gpu - a64) <= 3 * ref + 1e-6 * np.linalg.norm(x)
+            if bound is not None:
+                assert its == 0 and st["start_residual"] < bound
+
+
+@pytest.mark.parametrize("poison", [1e-3, 1e20])
+def test_em_bad_history_gives_the_zero_start_bit_for_bit(poison):
+    """src/em/prediction_tests.rs:170-188 on the GPU: a poisoned history is dropped and the step
+    equals the zero-start step bit for bit, with the same iteration count."""
+    import torch
+    dims, h, m, props, vac = case("block")
+    with EmSolver(dims, h, m, props, vac) as em:
+        a_prev, as_now, as_prev, mot = (torch.from_numpy(a).cuda() for a in sourced(em, 3))
+        out0 = torch.zeros(em.edges, dtype=torch.float32, device="cuda")
+        out1 = torch.zeros_like(out0)
+        print(f"[gpu] em poisoned history {poison} dims={dims}", flush=True)
+        its0, _ = em.solve(as_now, as_prev, a_prev, out0, motional=mot, rel_tol=1e-6, max_iter=5000)
+        assert em.stats["start_residual"] == 1.0 and em.stats["prediction"] == 0.0
+        poisoned = torch.full_like(out0, poison)
+        its1, _ = em.solve(as_now, as_prev, a_prev, out1, motional=mot, rel_tol=1e-6, max_iter=5000,
+                           a_before=poisoned, prediction=1.0)
+        assert em.stats["prediction_dropped"] == 1 and em.stats["start_residual"] > 1.0
+        assert its0 == its1
+        assert torch.equal(out0.view(torch.int32), out1.view(torch.int32))
