@@ -858,6 +858,39 @@ void issue_plan(hipStream_t s, const Plan& plan) {
         m.issue(s, base + m.args_off, reinterpret_cast<const uint32_t*>(base + m.starts_off), m.count, m.blocks);
 }
 
+// A plan issued again and again with the same arguments (the rounds' 8 iterations), captured
+// once into a HIP graph on a private stream and then submitted as one graph launch per round.
+// The plan holds kernel launches only (no memset or copy nodes).  MHDP_BATCH_GRAPH=0 issues the
+// launches one by one instead (comparison runs).
+struct PlanGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~PlanGraph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+    bool build(const Plan& plan) {
+        hipStream_t cs = nullptr;
+        if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return false;
+        bool ok = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        if (ok) {
+            issue_plan(cs, plan);
+            ok = hipStreamEndCapture(cs, &graph) == hipSuccess && graph &&
+                 hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+        }
+        (void)hipStreamDestroy(cs);
+        if (!ok) (void)hipGetLastError();
+        return ok;
+    }
+};
+bool batch_graph_enabled() {
+    static const bool on = [] {
+        const char* e = std::getenv("MHDP_BATCH_GRAPH");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 // Copies the scalar blocks of `count` problems into one array (read back with one copy).
 struct KGatherScal {
     static constexpr int MINB = 0;
@@ -989,6 +1022,8 @@ int project_lockstep(mhdp_ctx* const* ctxs, int count, const float* const* ustar
             }))
             return MHDP_NO_MEMORY;
     }
+    PlanGraph rounds;
+    if (batch_graph_enabled() && !iter8.ops.empty() && !rounds.build(iter8)) return MHDP_DEVICE;
     std::vector<int> trans = members([](const PS& q) { return q.running; });
     for (int p : trans) ps[(size_t)p].inner = true;
     for (bool first_round = true;; first_round = false) {
@@ -1014,7 +1049,11 @@ int project_lockstep(mhdp_ctx* const* ctxs, int count, const float* const* ustar
         }
         auto now = members([](const PS& q) { return q.running; });
         if (now.empty()) break;
-        issue_plan(s, iter8);
+        if (rounds.exec) {
+            if (hipGraphLaunch(rounds.exec, s) != hipSuccess) return MHDP_DEVICE;
+        } else {
+            issue_plan(s, iter8);
+        }
         if (!read_all()) return MHDP_DEVICE;
         trans.clear();
         for (int p : now) {
