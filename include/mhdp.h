@@ -296,6 +296,22 @@ enum { MHDP_K_NONE = -1, MHDP_K_APPLY = 0, MHDP_K_UPDATE = 1, MHDP_K_SMOOTH = 2,
 int mhdp_set_kernel_timing(mhdp_ctx *ctx, int kernel);
 int mhdp_kernel_time(const mhdp_ctx *ctx, double *total_ms, uint32_t *launches);
 
+/* Launch shapes of the projection's kernels (no counterpart in the reference, whose WGSL
+ * workgroup shapes are fixed): process-wide, read from the environment at first use
+ * (MHDP_GRID_CAP, MHDP_BRICK_CAP, MHDP_MARCH_CAP, MHDP_MARCH_Z, MHDP_BRICK_OCC) and replaced by
+ * mhdp_set_launch_tuning - the hook of the autotuner (mhd_simulator_amd/tune.py).  Results stay
+ * within every solve bar for any valid shape (reduction partials follow the block count).  Set it
+ * while no projection runs.  Invalid values: MHDP_INVALID, nothing changed. */
+typedef struct {
+    uint32_t grid_cap;  /* block cap of a cell-parallel launch, 8..4096 (1024) */
+    uint32_t brick_cap; /* block cap of the 4x2x2 brick stencil kernels, 8..4096 (1024) */
+    uint32_t march_cap; /* block cap of the z-marching stencil kernels, 8..4096 (1024) */
+    uint32_t march_z;   /* planes per z-marching work item, even, 2..512 (32) */
+    int brick_occ;      /* resident blocks per CU the brick kernels target: 2, 3 or 4 (2) */
+} mhdp_launch_tuning;
+int mhdp_get_launch_tuning(mhdp_launch_tuning *out);
+int mhdp_set_launch_tuning(const mhdp_launch_tuning *in);
+
 /* ---- test hooks (the reference's cfg(test) probes: multigrid_probes.wgsl, tests.rs) ---- */
 /* pressure_coefficients: lower-face mobilities (3*cells, x|y|z), cached diagonal, 1/diag. */
 int mhdp_probe_coefficients(mhdp_ctx *ctx, int meanfree, float *beta3, float *diag, float *dinv);

@@ -20,7 +20,8 @@ SYMBOLS = (
     "mhdp_em_get_coefficients", "mhdp_em_probe_apply",
     "mhdp_em_edges", "mhdp_em_set_kernel_timing", "mhdp_em_kernel_time", "mhdp_em_last_error", "mhdp_em_destroy",
     "mhdp_last_error", "mhdp_destroy", "mhdp_device_bytes",
-    "mhdp_anchors", "mhdp_hierarchy", "mhdp_set_kernel_timing", "mhdp_kernel_time",
+    "mhdp_anchors", "mhdp_hierarchy", "mhdp_set_kernel_timing", "mhdp_kernel_time", "mhdp_get_launch_tuning",
+    "mhdp_set_launch_tuning",
     "mhdp_probe_coefficients", "mhdp_probe_apply", "mhdp_probe_vcycle", "mhdp_probe_mg_graph", "mhdp_probe_rhs",
 )
 
@@ -86,6 +87,14 @@ class EmOpts(C.Structure):
                 ("magnetostatic", C.c_int)]
 
 
+class LaunchTuning(C.Structure):
+    _fields_ = [("grid_cap", C.c_uint32), ("brick_cap", C.c_uint32), ("march_cap", C.c_uint32),
+                ("march_z", C.c_uint32), ("brick_occ", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class EmStats(C.Structure):
     _fields_ = [("iterations", C.c_uint32), ("rel_residual", C.c_double), ("seconds", C.c_double),
                 ("prediction", C.c_float), ("start_residual", C.c_double), ("prediction_dropped", C.c_uint32)]
@@ -142,6 +151,8 @@ def load(path: str = LIB_PATH):
         "mhdp_device_bytes": (C.c_uint64, [P]),
         "mhdp_anchors": (U32, [P, P, U32]),
         "mhdp_set_kernel_timing": (I, [P, I]),
+        "mhdp_get_launch_tuning": (I, [C.POINTER(LaunchTuning)]),
+        "mhdp_set_launch_tuning": (I, [C.POINTER(LaunchTuning)]),
         "mhdp_kernel_time": (I, [P, P, P]),
         "mhdp_hierarchy": (I, [P, P, I, P]),
         "mhdp_probe_coefficients": (I, [P, I, P, P, P]),

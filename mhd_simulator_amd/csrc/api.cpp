@@ -1171,9 +1171,56 @@ int batch_workers() {
 
 }  // namespace
 
+// ============================================================================ launch tuning
+namespace {
+uint32_t env_u32(const char* name, uint32_t lo, uint32_t hi, uint32_t def) {
+    const char* e = std::getenv(name);
+    const long v = e ? std::strtol(e, nullptr, 10) : 0;
+    return (v >= (long)lo && v <= (long)hi) ? (uint32_t)v : def;
+}
+bool tuning_valid(const mhdp_launch_tuning& t) {
+    // kMaxPartials bounds every per-block partial buffer
+    auto cap = [](uint32_t v) { return v >= 8 && v <= kMaxPartials; };
+    return cap(t.grid_cap) && cap(t.brick_cap) && cap(t.march_cap) && t.march_z >= 2 && t.march_z <= 512 &&
+           t.march_z % 2 == 0 && t.brick_occ >= 2 && t.brick_occ <= 4;
+}
+}  // namespace
+
+LaunchTuning& launch_tuning() {
+    static LaunchTuning t = [] {
+        LaunchTuning d;
+        d.grid_cap = env_u32("MHDP_GRID_CAP", 8, 4096, 1024);
+        d.brick_cap = env_u32("MHDP_BRICK_CAP", 8, 4096, 1024);
+        d.march_cap = env_u32("MHDP_MARCH_CAP", 8, 4096, 1024);
+        const uint32_t z = env_u32("MHDP_MARCH_Z", 2, 512, 32);
+        d.march_z = z % 2 == 0 ? z : 32;
+        d.brick_occ = (int)env_u32("MHDP_BRICK_OCC", 2, 4, 2);
+        return d;
+    }();
+    return t;
+}
+
 // ============================================================================ C ABI
 
 extern "C" {
+
+int mhdp_get_launch_tuning(mhdp_launch_tuning* out) {
+    if (!out) return MHDP_INVALID;
+    const LaunchTuning& t = launch_tuning();
+    *out = mhdp_launch_tuning{t.grid_cap, t.brick_cap, t.march_cap, t.march_z, t.brick_occ};
+    return MHDP_OK;
+}
+
+int mhdp_set_launch_tuning(const mhdp_launch_tuning* in) {
+    if (!in || !tuning_valid(*in)) return MHDP_INVALID;
+    LaunchTuning& t = launch_tuning();
+    t.grid_cap = in->grid_cap;
+    t.brick_cap = in->brick_cap;
+    t.march_cap = in->march_cap;
+    t.march_z = in->march_z;
+    t.brick_occ = in->brick_occ;
+    return MHDP_OK;
+}
 
 int mhdp_create(mhdp_ctx** out, const mhdp_grid* grid, const mhdp_phase* phase) {
     if (!out || !grid || !phase) return MHDP_INVALID;
@@ -1312,7 +1359,8 @@ int mhdp_fields_dev(mhdp_ctx* c, const float* u_faces, const double* phi, const 
     if (!(fp->dt > 0.0f) || !std::isfinite(fp->dt) || (fp->gauge != MHDP_GAUGE_AUTO && fp->gauge != MHDP_GAUGE_PINNED))
         return fail(c, MHDP_INVALID, "Invalid fields parameters");
     const uint32_t blocks = grid_blocks(c->g.n);
-    int rc = dalloc(c, (void**)&c->fields_part, ((size_t)blocks + 1) * kFieldTotals * sizeof(double));
+    // sized for the largest block cap a launch tuning allows (the buffer is kept across calls)
+    int rc = dalloc(c, (void**)&c->fields_part, ((size_t)kMaxPartials + 1) * kFieldTotals * sizeof(double));
     if (rc) return rc;
     if (face_count(c->grid) >= (1ull << 32)) return fail(c, MHDP_INVALID, "Too many faces for the step kernels");
     if (!c->open8_fresh) {

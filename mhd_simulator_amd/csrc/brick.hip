@@ -843,15 +843,8 @@ struct K_k_brick_coarse_residual {
 };
 
 // ---------------------------------------------------------------- launchers
-// Minimum resident blocks per CU the register allocation targets (MHDP_BRICK_OCC: 2, 3, 4).
-static int brick_occ() {
-    static int occ = [] {
-        const char* e = std::getenv("MHDP_BRICK_OCC");
-        int v = e ? std::atoi(e) : 0;
-        return (v == 2 || v == 3 || v == 4) ? v : 2;
-    }();
-    return occ;
-}
+// Minimum resident blocks per CU the register allocation targets (launch_tuning: 2, 3, 4).
+static int brick_occ() { return launch_tuning().brick_occ; }
 #define BRICK_DISPATCH(KERNEL, TARGS, ...)                                                          \
     do {                                                                                          \
         switch (brick_occ()) {                                                                    \

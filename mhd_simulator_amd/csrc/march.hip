@@ -489,16 +489,8 @@ struct K_march_restrict_update {
 
 // ---------------------------------------------------------------- launchers
 
-// Planes per work item (MHDP_MARCH_Z: even, 2..512; default 32) and the block cap
-// (MHDP_MARCH_CAP; default 1024).
-static uint32_t march_z() {
-    static const uint32_t z = [] {
-        const char* e = std::getenv("MHDP_MARCH_Z");
-        const long v = e ? std::strtol(e, nullptr, 10) : 0;
-        return (uint32_t)((v >= 2 && v <= 512 && (v % 2) == 0) ? v : 32);
-    }();
-    return z;
-}
+// Planes per work item and the block cap: launch_tuning() (defaults 32 and 1024).
+static uint32_t march_z() { return launch_tuning().march_z; }
 bool march_ok(const Grid3& g) {
     static const bool on = [] {
         const char* e = std::getenv("MHDP_NO_MARCH");
@@ -515,11 +507,7 @@ bool march_ok(const Grid3& g) {
     return items >= min_items;
 }
 static uint32_t march_blocks(const Grid3& g) {
-    static const uint32_t cap = [] {
-        const char* e = std::getenv("MHDP_MARCH_CAP");
-        const long v = e ? std::strtol(e, nullptr, 10) : 0;
-        return (uint32_t)((v >= 8 && v <= 4096) ? v : 1024);
-    }();
+    const uint32_t cap = launch_tuning().march_cap;
     const uint32_t zl = march_z();
     const uint32_t items = ((uint32_t)g.nx >> 2) * ((uint32_t)g.ny >> 1) * (((uint32_t)g.nz + zl - 1) / zl);
     return grid_blocks(items, cap);

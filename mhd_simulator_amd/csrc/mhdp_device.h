@@ -327,25 +327,19 @@ __device__ __forceinline__ void reduce_finish(double (&v)[NV], double* __restric
 struct Launch {
     uint32_t blocks;
 };
-// Upper bound on the blocks of a cell-parallel launch (each block then owns a contiguous
-// range).  MHDP_GRID_CAP overrides it for launch-shape sweeps.
-static inline uint32_t grid_cap() {
-    static uint32_t cap = [] {
-        const char* e = std::getenv("MHDP_GRID_CAP");
-        long v = e ? std::strtol(e, nullptr, 10) : 0;
-        return (uint32_t)(v >= 8 && v <= 4096 ? v : 1024);
-    }();
-    return cap;
-}
-// Block cap of the brick stencil kernels (MHDP_BRICK_CAP overrides it).
-static inline uint32_t brick_grid_cap() {
-    static uint32_t cap = [] {
-        const char* e = std::getenv("MHDP_BRICK_CAP");
-        long v = e ? std::strtol(e, nullptr, 10) : 0;
-        return (uint32_t)(v >= 8 && v <= 4096 ? v : 1024);
-    }();
-    return cap;
-}
+// Launch shapes of the cell-parallel and stencil kernels, process-wide: read from the
+// environment at first use (MHDP_GRID_CAP, MHDP_BRICK_CAP, MHDP_MARCH_CAP, MHDP_MARCH_Z,
+// MHDP_BRICK_OCC) and replaced by mhdp_set_launch_tuning (the autotuner).
+struct LaunchTuning {
+    uint32_t grid_cap;   // block cap of a cell-parallel launch (each block owns a contiguous range)
+    uint32_t brick_cap;  // block cap of the 4x2x2 brick stencil kernels
+    uint32_t march_cap;  // block cap of the z-marching kernels
+    uint32_t march_z;    // planes per z-marching work item (even)
+    int brick_occ;       // resident blocks per CU the brick kernels' registers target (2, 3, 4)
+};
+LaunchTuning& launch_tuning();
+static inline uint32_t grid_cap() { return launch_tuning().grid_cap; }
+static inline uint32_t brick_grid_cap() { return launch_tuning().brick_cap; }
 static inline uint32_t grid_blocks(uint32_t n, uint32_t cap = 0) {
     if (cap == 0) cap = grid_cap();
     uint32_t need = (n + MHDP_BLOCK - 1) / MHDP_BLOCK;

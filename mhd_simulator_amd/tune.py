@@ -1,0 +1,207 @@
+"""Autotuner of the projection's launch shapes and cycle schedule (no counterpart in the
+reference, whose WGSL workgroup shapes and V-cycle are fixed).
+
+    from mhd_simulator_amd import tune
+    best = tune.tune_projection(pr, u_star, refine_tol=1e-8)   # times the candidates on the GPU
+    tune.apply(best)                                            # process-wide launch shapes
+    pr.project(u_star, precision="mixed", **best["opts"])       # cycle schedule
+    tune.save(best, "tuning.json"); tune.apply(tune.load("tuning.json", key=best["key"]))
+
+A candidate is a set of launch shapes (mhdp_set_launch_tuning: the z-marching chunk length and
+block cap, the brick kernels' occupancy target) and a cycle schedule (level-1 visits per cycle).
+Every candidate runs one untimed projection, then `repeats` timed ones (wall clock around a
+synchronized call; inputs resident on the GPU); it must converge, and the fastest median wins.
+`tune_batch` does the same for one mhdp_project_batch call over many problems.  The launch
+shapes are process-wide: tune while nothing else projects, then apply the winner.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import itertools
+import json
+import os
+import statistics
+import time
+
+from . import _lib
+
+# the candidate axes (the defaults are the library's built-in shapes: z 32, cap 1024, occupancy 2)
+MARCH_Z = (16, 32, 64)
+MARCH_CAP = (1024, 2048)
+BRICK_OCC = (2, 3)
+COARSE_VISITS = (0, 1, 2)  # 0: the library's automatic choice by grid size
+
+
+def current() -> dict:
+    """The launch shapes in force (mhdp_get_launch_tuning)."""
+    t = _lib.LaunchTuning()
+    rc = _lib.load().mhdp_get_launch_tuning(C.byref(t))
+    if rc != _lib.OK:
+        raise RuntimeError(f"mhdp_get_launch_tuning failed ({rc})")
+    return t.as_dict()
+
+
+def set_launch(shapes: dict) -> None:
+    """Replace the process-wide launch shapes; unknown keys are an error, missing keys keep their
+    current value."""
+    now = current()
+    unknown = set(shapes) - set(now)
+    if unknown:
+        raise KeyError(f"unknown launch shape(s): {sorted(unknown)}")
+    now.update(shapes)
+    t = _lib.LaunchTuning(**now)
+    rc = _lib.load().mhdp_set_launch_tuning(C.byref(t))
+    if rc != _lib.OK:
+        raise ValueError(f"invalid launch shapes {now}")
+
+
+def candidates(march_z=MARCH_Z, march_cap=MARCH_CAP, brick_occ=BRICK_OCC, coarse_visits=COARSE_VISITS):
+    """Every combination of the axes: [{"launch": {...}, "opts": {...}}, ...]."""
+    out = []
+    for z, cap, occ, visits in itertools.product(march_z, march_cap, brick_occ, coarse_visits):
+        out.append({"launch": {"march_z": int(z), "march_cap": int(cap), "brick_occ": int(occ)},
+                    "opts": {"coarse_visits": int(visits)}})
+    return out
+
+
+def search(run, cands, repeats=3, clock=time.perf_counter):
+    """Time run(candidate) -> ok for every candidate (one untimed call first) and return
+    (best, table).  A candidate whose run raises or returns False is kept in the table as failed
+    and never wins; the launch shapes in force before the search are restored at the end."""
+    if repeats < 1:
+        raise ValueError("repeats must be >= 1")
+    before = current()
+    table = []
+    try:
+        for cand in cands:
+            set_launch(cand["launch"])
+            row = dict(cand, ms=None, ok=False)
+            try:
+                ok = bool(run(cand))
+                times = []
+                for _ in range(repeats):
+                    t0 = clock()
+                    ok = ok and bool(run(cand))
+                    times.append((clock() - t0) * 1e3)
+                row["ok"] = ok
+                row["ms"] = statistics.median(times) if ok else None
+            except Exception as e:  # a failing candidate is recorded, not fatal
+                row["error"] = str(e)
+            table.append(row)
+    finally:
+        set_launch(before)
+    good = [r for r in table if r["ok"]]
+    if not good:
+        raise RuntimeError("no candidate converged")
+    best = min(good, key=lambda r: r["ms"])
+    return best, table
+
+
+def _key(dims_list, precision, refine_tol):
+    dims = ";".join("x".join(str(int(v)) for v in d) for d in dims_list)
+    return f"{dims}|{precision}|{refine_tol:g}"
+
+
+def tune_projection(pr, u_star, *, precision="mixed", refine_tol=1e-8, repeats=3, cands=None, **project_kw):
+    """Tune one PressureProjection on a device-resident u_star (torch tensor on the GPU)."""
+    import torch
+
+    u_out = torch.empty_like(u_star)
+
+    def run(cand):
+        pr.project(u_star, precision=precision, refine_tol=refine_tol, u_out=u_out, **cand["opts"], **project_kw)
+        torch.cuda.synchronize()
+        return True
+
+    best, table = search(run, cands if cands is not None else candidates(), repeats)
+    return {"key": _key([pr.grid.dims], precision, refine_tol), "launch": best["launch"], "opts": best["opts"],
+            "ms": best["ms"], "table": table}
+
+
+def tune_batch(projections, u_stars, *, refine_tol=1e-8, repeats=3, cands=None, **batch_kw):
+    """Tune one mhdp_project_batch call over `projections` (device-resident u_stars)."""
+    import torch
+
+    from .fluid import project_batch
+
+    def run(cand):
+        project_batch(projections, u_stars, precision="mixed", refine_tol=refine_tol, **cand["opts"], **batch_kw)
+        torch.cuda.synchronize()
+        return True
+
+    best, table = search(run, cands if cands is not None else candidates(), repeats)
+    return {"key": _key([p.grid.dims for p in projections], "mixed", refine_tol), "launch": best["launch"],
+            "opts": best["opts"], "ms": best["ms"], "table": table}
+
+
+def apply(result: dict) -> dict:
+    """Put a tuning result's launch shapes in force; returns the projection options to pass."""
+    set_launch(result["launch"])
+    return dict(result["opts"])
+
+
+def save(result: dict, path: str) -> None:
+    """Add (or replace) `result` under its key in the JSON file at `path`."""
+    data = {}
+    if os.path.exists(path):
+        with open(path) as f:
+            data = json.load(f)
+    data[result["key"]] = {k: result[k] for k in ("key", "launch", "opts", "ms")}
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+
+
+def load(path: str, key: str) -> dict:
+    """The saved result for `key` (KeyError when the file holds none)."""
+    with open(path) as f:
+        data = json.load(f)
+    return data[key]
+
+
+def main(argv=None):
+    """python -m mhd_simulator_amd.tune [--workload plate|batch64] [--size N] [--out tuning.json]:
+    tune on a synthetic workload of the given shape (plate: the two-phase droplets over a plate,
+    mixed to 1e-8; batch64: the 64 uneven droplet problems in one batch call) and print every
+    candidate's time and the winner."""
+    import argparse
+
+    import torch
+
+    from . import configs as Cf
+    from .fluid import FluidConfig, PressureProjection
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--workload", choices=["plate", "batch64"], default="plate")
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--repeats", type=int, default=3)
+    ap.add_argument("--out", default="")
+    a = ap.parse_args(argv)
+    torch.cuda.set_device(0)
+    if a.workload == "plate":
+        p = Cf.config5(a.size, droplets=max(1, round(200 * (a.size / 512) ** 3)))
+        cfg = FluidConfig(relative_tolerance=1e-7, absolute_divergence_tolerance=0.0, max_iterations=4000)
+        prs = [PressureProjection(p.grid, p.fluid, p.alpha, cfg)]
+        u = torch.from_numpy(p.ustar).cuda()
+        res = tune_projection(prs[0], u, refine_tol=1e-8, repeats=a.repeats, precond="mg")
+    else:
+        ps = Cf.config4()
+        prs = [PressureProjection(q.grid, q.fluid, q.alpha) for q in ps]
+        us = [torch.from_numpy(q.ustar).cuda() for q in ps]
+        res = tune_batch(prs, us, refine_tol=1e-8, repeats=a.repeats, precond="mg")
+    default = next((r for r in res["table"] if r["launch"] == {"march_z": 32, "march_cap": 1024, "brick_occ": 2}
+                    and r["opts"] == {"coarse_visits": 0}), None)
+    for r in sorted(res["table"], key=lambda r: (r["ms"] is None, r["ms"] or 0.0)):
+        ms = f"{r['ms']:.2f} ms" if r["ms"] is not None else f"failed ({r.get('error', 'no convergence')})"
+        print(f"{json.dumps(r['launch'])} {json.dumps(r['opts'])}: {ms}", flush=True)
+    print(json.dumps({k: res[k] for k in ("key", "launch", "opts", "ms")}), flush=True)
+    if default is not None:
+        print(f"built-in shapes and schedule: {default['ms']:.2f} ms", flush=True)
+    if a.out:
+        save(res, a.out)
+    for pr in prs:
+        pr.close()
+
+
+if __name__ == "__main__":
+    main()
