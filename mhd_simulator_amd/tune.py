@@ -30,6 +30,7 @@ MARCH_Z = (16, 32, 64)
 MARCH_CAP = (1024, 2048)
 BRICK_OCC = (2, 3)
 COARSE_VISITS = (0, 1, 2)  # 0: the library's automatic choice by grid size
+INNER_TOL = (0.0,)  # the mixed solver's inner FP32 tolerance; 0: the library's (1e-4)
 
 
 def current() -> dict:
@@ -55,12 +56,15 @@ def set_launch(shapes: dict) -> None:
         raise ValueError(f"invalid launch shapes {now}")
 
 
-def candidates(march_z=MARCH_Z, march_cap=MARCH_CAP, brick_occ=BRICK_OCC, coarse_visits=COARSE_VISITS):
+def candidates(march_z=MARCH_Z, march_cap=MARCH_CAP, brick_occ=BRICK_OCC, coarse_visits=COARSE_VISITS,
+               inner_tol=INNER_TOL):
     """Every combination of the axes: [{"launch": {...}, "opts": {...}}, ...]."""
     out = []
-    for z, cap, occ, visits in itertools.product(march_z, march_cap, brick_occ, coarse_visits):
-        out.append({"launch": {"march_z": int(z), "march_cap": int(cap), "brick_occ": int(occ)},
-                    "opts": {"coarse_visits": int(visits)}})
+    for z, cap, occ, visits, tau in itertools.product(march_z, march_cap, brick_occ, coarse_visits, inner_tol):
+        opts = {"coarse_visits": int(visits)}
+        if tau:
+            opts["inner_tol"] = float(tau)
+        out.append({"launch": {"march_z": int(z), "march_cap": int(cap), "brick_occ": int(occ)}, "opts": opts})
     return out
 
 
@@ -176,19 +180,27 @@ def main(argv=None):
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--repeats", type=int, default=3)
     ap.add_argument("--out", default="")
+    ap.add_argument("--inner-tol", default="", help="comma list of inner tolerances to sweep (default: the library's)")
+    ap.add_argument("--builtin-launch", action="store_true", help="sweep the schedule only, at the built-in shapes")
     a = ap.parse_args(argv)
+    axes = {}
+    if a.inner_tol:
+        axes["inner_tol"] = tuple(float(v) for v in a.inner_tol.split(","))
+    if a.builtin_launch:
+        axes.update(march_z=(32,), march_cap=(1024,), brick_occ=(2,), coarse_visits=(0,))
+    cands = candidates(**axes)
     torch.cuda.set_device(0)
     if a.workload == "plate":
         p = Cf.config5(a.size, droplets=max(1, round(200 * (a.size / 512) ** 3)))
         cfg = FluidConfig(relative_tolerance=1e-7, absolute_divergence_tolerance=0.0, max_iterations=4000)
         prs = [PressureProjection(p.grid, p.fluid, p.alpha, cfg)]
         u = torch.from_numpy(p.ustar).cuda()
-        res = tune_projection(prs[0], u, refine_tol=1e-8, repeats=a.repeats, precond="mg")
+        res = tune_projection(prs[0], u, refine_tol=1e-8, repeats=a.repeats, cands=cands, precond="mg")
     else:
         ps = Cf.config4()
         prs = [PressureProjection(q.grid, q.fluid, q.alpha) for q in ps]
         us = [torch.from_numpy(q.ustar).cuda() for q in ps]
-        res = tune_batch(prs, us, refine_tol=1e-8, repeats=a.repeats, precond="mg")
+        res = tune_batch(prs, us, refine_tol=1e-8, repeats=a.repeats, cands=cands, precond="mg")
     default = next((r for r in res["table"] if r["launch"] == {"march_z": 32, "march_cap": 1024, "brick_occ": 2}
                     and r["opts"] == {"coarse_visits": 0}), None)
     for r in sorted(res["table"], key=lambda r: (r["ms"] is None, r["ms"] or 0.0)):

@@ -30,6 +30,12 @@ def test_defaults_and_every_candidate_is_a_valid_shape(restore, monkeypatch):
         assert c["opts"]["coarse_visits"] in (0, 1, 2)
 
 
+def test_inner_tolerance_axis_enters_the_schedule_only_when_set():
+    cands = tune.candidates(march_z=(32,), march_cap=(1024,), brick_occ=(2,), coarse_visits=(0,),
+                            inner_tol=(0.0, 3e-5))
+    assert [c["opts"] for c in cands] == [{"coarse_visits": 0}, {"coarse_visits": 0, "inner_tol": 3e-5}]
+
+
 @pytest.mark.parametrize("bad", [{"march_z": 3}, {"march_z": 0}, {"march_z": 514}, {"march_cap": 4},
                                  {"grid_cap": 5000}, {"brick_occ": 1}, {"brick_occ": 5}])
 def test_invalid_shapes_are_rejected_and_change_nothing(restore, bad):
