@@ -70,7 +70,8 @@ def candidates(march_z=MARCH_Z, march_cap=MARCH_CAP, brick_occ=BRICK_OCC, coarse
 
 def search(run, cands, repeats=3, clock=time.perf_counter):
     """Time run(candidate) -> ok for every candidate (one untimed call first) and return
-    (best, table).  A candidate whose run raises or returns False is kept in the table as failed
+    (best, table).  run may return a dict of solve statistics instead of True: the last one is
+    kept in the row.  A candidate whose run raises or returns False is kept in the table as failed
     and never wins; the launch shapes in force before the search are restored at the end."""
     if repeats < 1:
         raise ValueError("repeats must be >= 1")
@@ -81,12 +82,16 @@ def search(run, cands, repeats=3, clock=time.perf_counter):
             set_launch(cand["launch"])
             row = dict(cand, ms=None, ok=False)
             try:
-                ok = bool(run(cand))
+                out = run(cand)
+                ok = bool(out)
                 times = []
                 for _ in range(repeats):
                     t0 = clock()
-                    ok = ok and bool(run(cand))
+                    out = run(cand) if ok else out
                     times.append((clock() - t0) * 1e3)
+                    ok = ok and bool(out)
+                if ok and isinstance(out, dict):
+                    row["stats"] = out
                 row["ok"] = ok
                 row["ms"] = statistics.median(times) if ok else None
             except Exception as e:  # a failing candidate is recorded, not fatal
@@ -115,7 +120,7 @@ def tune_projection(pr, u_star, *, precision="mixed", refine_tol=1e-8, repeats=3
     def run(cand):
         pr.project(u_star, precision=precision, refine_tol=refine_tol, u_out=u_out, **cand["opts"], **project_kw)
         torch.cuda.synchronize()
-        return True
+        return {k: pr.stats[k] for k in ("iterations", "outer_steps") if k in pr.stats} or True
 
     best, table = search(run, cands if cands is not None else candidates(), repeats)
     return {"key": _key([pr.grid.dims], precision, refine_tol), "launch": best["launch"], "opts": best["opts"],
@@ -205,7 +210,8 @@ def main(argv=None):
                     and r["opts"] == {"coarse_visits": 0}), None)
     for r in sorted(res["table"], key=lambda r: (r["ms"] is None, r["ms"] or 0.0)):
         ms = f"{r['ms']:.2f} ms" if r["ms"] is not None else f"failed ({r.get('error', 'no convergence')})"
-        print(f"{json.dumps(r['launch'])} {json.dumps(r['opts'])}: {ms}", flush=True)
+        its = f" {json.dumps(r['stats'])}" if "stats" in r else ""
+        print(f"{json.dumps(r['launch'])} {json.dumps(r['opts'])}: {ms}{its}", flush=True)
     print(json.dumps({k: res[k] for k in ("key", "launch", "opts", "ms")}), flush=True)
     if default is not None:
         print(f"built-in shapes and schedule: {default['ms']:.2f} ms", flush=True)
