@@ -87,7 +87,7 @@ typedef struct {
                             (pressure_commit, solver.wgsl:181-183).  MHDP_MIXED: the FP64 refinement
                             starts from it (the same sum, without rounding the corrected u* to
                             FP32 faces).  Ignored until a previous potential exists. */
-    int coarse_visits;   /* multigrid: visits of level 1 per cycle, at most 4.  1 = the reference's
+    int coarse_visits;   /* multigrid: visits of level 1 per cycle, at most 2.  1 = the reference's
                             V-cycle (multigrid.rs:67-82); 2 = a W-cycle at level 1 (the second
                             visit solves for the residual the first leaves and is added damped by
                             0.8); 0 = automatic: 2 on grids of at least 2^24 cells, else 1 (below
@@ -106,6 +106,9 @@ typedef struct {
     float rel_residual;     /* sqrt(rr/bb) as FluidState::convergence() reports it */
     double rel_residual_f64;/* MHDP_MIXED: FP64 ||b - L phi|| / ||b|| over all fluid rows */
     double seconds;         /* device time of the whole projection (HIP events) */
+    int cycle_fallback;     /* 1: the requested (tuned) multigrid cycle broke down in PCG (r.z or
+                               p.Ap <= 0, i.e. not positive definite on this problem) and the
+                               projection was solved again with the reference's V-cycle */
 } mhdp_stats;
 
 /* FluidState::new (mod.rs:182): allocate a context for one grid on the current device. */
@@ -325,6 +328,17 @@ int mhdp_probe_vcycle(mhdp_ctx *ctx, int meanfree, int precision, float smooth_w
 int mhdp_probe_mg_graph(mhdp_ctx *ctx, int meanfree, int level, float *graph5, uint32_t *dims4);
 /* Divergence right-hand side b = -h^2 div u* (physical, no anchor zeroing), FP64. */
 int mhdp_probe_rhs(mhdp_ctx *ctx, int meanfree, const float *u_star, double *b);
+/* The optional multigrid storage (the state the reference's test
+ * optional_workspace_is_lazy_accounted_reused_and_can_be_unavailable drives,
+ * src/fluid/multigrid/tests.rs:620-660): action 0 only reports; 1 makes the next allocation
+ * attempt fail after `arg` of its arrays, as an exhausted device would; 2 frees the storage and
+ * records a refused attempt (the context's projections then use Jacobi).  state: 0 not allocated,
+ * 1 allocated, 2 refused; bytes: device bytes of the storage. */
+int mhdp_probe_multigrid(mhdp_ctx *ctx, int action, int arg, int *state, uint64_t *bytes);
+/* The next projection's first attempt uses this cycle unvalidated (a negative smooth_weight makes
+ * the preconditioner negative definite: the PCG breakdown path, which solves again with the
+ * reference's V-cycle and reports mhdp_stats.cycle_fallback). */
+int mhdp_probe_next_cycle(mhdp_ctx *ctx, float smooth_weight, float coarse_scale, int coarse_visits);
 
 #ifdef __cplusplus
 }

@@ -23,6 +23,7 @@ SYMBOLS = (
     "mhdp_anchors", "mhdp_hierarchy", "mhdp_set_kernel_timing", "mhdp_kernel_time", "mhdp_get_launch_tuning",
     "mhdp_set_launch_tuning",
     "mhdp_probe_coefficients", "mhdp_probe_apply", "mhdp_probe_vcycle", "mhdp_probe_mg_graph", "mhdp_probe_rhs",
+    "mhdp_probe_multigrid", "mhdp_probe_next_cycle",
 )
 
 OK, NOT_CONVERGED, NO_PROGRESS, NONFINITE, INVALID, NO_MEMORY, DEVICE = range(7)
@@ -53,7 +54,7 @@ class Stats(C.Structure):
     _fields_ = [("iterations", C.c_uint32), ("outer_steps", C.c_uint32), ("components", C.c_uint32),
                 ("fluid_cells", C.c_uint32), ("levels", C.c_uint32), ("meanfree", C.c_int),
                 ("multigrid", C.c_int), ("rel_residual", C.c_float), ("rel_residual_f64", C.c_double),
-                ("seconds", C.c_double)]
+                ("seconds", C.c_double), ("cycle_fallback", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -160,6 +161,8 @@ def load(path: str = LIB_PATH):
         "mhdp_probe_vcycle": (I, [P, I, I, C.c_float, C.c_float, I, P, P]),
         "mhdp_probe_mg_graph": (I, [P, I, I, P, P]),
         "mhdp_probe_rhs": (I, [P, I, P, P]),
+        "mhdp_probe_multigrid": (I, [P, I, I, P, P]),
+        "mhdp_probe_next_cycle": (I, [P, C.c_float, C.c_float, I]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -26,6 +26,18 @@ namespace {
 
 constexpr uint32_t kMaxPartials = 4096;
 constexpr uint32_t kMgMinCells = 96u * 96u * 96u;  // multigrid.rs:4
+// Visits of level 1 per cycle.  Two keep the damped sum t1 + 0.8 t2 positive definite with the
+// margin derived for coarse_visits_t; a third visit would start from the second's undamped
+// correction and shrink that margin, so the option stops at two.
+constexpr int kMaxVisits = 2;
+// Status of a solve whose multigrid PCG broke down (r.z or p.Ap <= 0): internal, never returned.
+constexpr int kBreakdown = 100;
+
+// Multigrid cycle of one solve: smoother weight, coarse-correction scale, level-1 visits.
+struct Cycle {
+    double omega, pscale;
+    int visits;
+};
 
 struct CoarseArrays {
     float *bx = nullptr, *by = nullptr, *bz = nullptr, *react = nullptr, *diag = nullptr, *dinv = nullptr;
@@ -75,6 +87,14 @@ struct mhdp_ctx {
     std::vector<Grid3> levels;         // level grids, [0] = fine
     bool mg_alloc = false, mg16_alloc = false;
     bool fine16_fresh = false;  // BF16 copies of the fine coefficients match the FP32 ones
+    // Optional multigrid (src/fluid/mod.rs:1031-1043): its storage is allocated the first time a
+    // nonzero residual asks for it; a refused allocation is final for the context, whose
+    // projections then use Jacobi (a fitting Jacobi solve never fails for the optional part).
+    bool mg_refused = false;
+    uint64_t mg_bytes = 0;      // device bytes of the multigrid storage
+    int mg_fail_after = -1;     // test hook: the next multigrid allocation fails after this many arrays
+    bool in_mg_alloc = false;   // alloc_mg is allocating (the hook counts only its arrays)
+    Cycle test_cycle{0.0, 0.0, 0};  // test hook: the next projection's first cycle (visits 0: none)
     // scalars
     Scal* sc = nullptr;
     Scal* sc_h = nullptr;
@@ -109,6 +129,13 @@ int fail(mhdp_ctx* c, int code, const std::string& msg) {
 
 int dalloc(mhdp_ctx* c, void** p, size_t bytes) {
     if (*p) return MHDP_OK;
+    if (c->in_mg_alloc && c->mg_fail_after >= 0) {  // test hook: refuse as an exhausted device would
+        if (c->mg_fail_after == 0) {
+            c->mg_fail_after = -1;
+            return fail(c, MHDP_NO_MEMORY, "Optional pressure preconditioner allocation refused");
+        }
+        c->mg_fail_after--;
+    }
     if (hipMalloc(p, bytes ? bytes : 16) != hipSuccess) {
         *p = nullptr;
         (void)hipGetLastError();
@@ -198,7 +225,7 @@ int validate_opts(mhdp_ctx* c, const mhdp_opts* o) {
         return fail(c, MHDP_INVALID, "Invalid pressure solver selection");
     if (!(o->smooth_weight >= 0.0f && o->smooth_weight < 1.0f) || !(o->coarse_scale >= 0.0f && o->coarse_scale < 2.0f))
         return fail(c, MHDP_INVALID, "Invalid V-cycle smoother weight or coarse-correction scale");
-    if (o->coarse_visits < 0 || o->coarse_visits > 4)
+    if (o->coarse_visits < 0 || o->coarse_visits > kMaxVisits)
         return fail(c, MHDP_INVALID, "Invalid multigrid coarse visits");
     if (o->precision == MHDP_MIXED &&
         (!(o->refine_tol > 0.0) || !(o->refine_tol < 1.0) || o->inner_tol < 0.0f || o->inner_tol >= 1.0f))
@@ -208,6 +235,15 @@ int validate_opts(mhdp_ctx* c, const mhdp_opts* o) {
 
 int alloc_mg(mhdp_ctx* c, bool f16) {
     int rc;
+    struct Account {  // every byte this call allocates is multigrid storage
+        mhdp_ctx* c;
+        uint64_t before;
+        ~Account() {
+            c->mg_bytes += c->bytes - before;
+            c->in_mg_alloc = false;
+        }
+    } account{c, c->bytes};
+    c->in_mg_alloc = true;
     if (!c->mg_alloc) {
         c->levels = hierarchy(c->g);
         c->coarse.assign(c->levels.size() - 1, CoarseArrays{});
@@ -239,6 +275,41 @@ int alloc_mg(mhdp_ctx* c, bool f16) {
         c->mg16_alloc = true;
     }
     return MHDP_OK;
+}
+
+// Frees every multigrid array (coarse levels and the BF16 copies of the fine coefficients).
+void release_mg(mhdp_ctx* c) {
+    (void)hipStreamSynchronize(c->stream);
+    for (CoarseArrays& a : c->coarse) {
+        void* q[] = {a.bx, a.by, a.bz, a.react, a.diag, a.dinv, a.bx16, a.by16, a.bz16, a.react16, a.dinv16,
+                     a.r, a.z, a.t, a.w};
+        for (void* p : q)
+            if (p) (void)hipFree(p);
+    }
+    c->coarse.clear();
+    void** fine16[] = {&c->bx16, &c->by16, &c->bz16, &c->dinv16};
+    for (void** p : fine16) {
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+    }
+    c->levels.clear();
+    c->bytes -= c->mg_bytes;
+    c->mg_bytes = 0;
+    c->mg_alloc = c->mg16_alloc = c->fine16_fresh = false;
+}
+
+// The optional multigrid storage (Multigrid::new, src/fluid/multigrid.rs:85-96; attempted once per
+// context, src/fluid/mod.rs:1035-1043): true when it is (or now is) allocated; a failed attempt
+// frees what it had allocated and marks the context, whose solves then use Jacobi.
+bool ensure_mg(mhdp_ctx* c, bool f16) {
+    if (c->mg_refused) return false;
+    if (c->mg_alloc && (!f16 || c->mg16_alloc)) return true;
+    if (alloc_mg(c, f16) == MHDP_OK) return true;
+    (void)hipGetLastError();
+    release_mg(c);
+    c->mg_refused = true;
+    c->err.clear();
+    return false;
 }
 
 // Galerkin coarse graphs of all levels (Multigrid::build, multigrid.rs:266-273) and, for
@@ -304,6 +375,7 @@ int convergence(mhdp_ctx* c, float* residual, bool* conv, uint32_t* iters) {
     int rc = read_scalars(c);
     if (rc) return rc;
     const Scal& s = *c->sc_h;
+    if (s.fault) return kBreakdown;
     const double res = std::sqrt(s.rr / std::max(s.bb, 1e-38));
     const double v[8] = {s.rz, s.rr, s.bb, s.sr, s.alpha, s.beta, s.conv, s.iters};
     bool bad = !std::isfinite(res) || s.rr < 0.0 || s.bb < 0.0;
@@ -468,7 +540,7 @@ int pcg(mhdp_ctx* c, Solve& S, bool want_mg, uint32_t max_iter, bool reference_c
     uint32_t seen = 0, it = 0;
     int rc = convergence(c, residual, &conv, &seen);
     if (rc) return rc;
-    if (!conv && want_mg && !S.mg) {
+    if (!conv && want_mg && !S.mg && ensure_mg(c, S.store == STORE_BF16)) {
         build_mg(c, S.meanfree ? 1 : 0, S.store == STORE_BF16);
         S.mg = true;
         S.lv = level_view(c);
@@ -517,14 +589,15 @@ int upload_scal(mhdp_ctx* c, const Scal& s) {
 // projection, zeroed x and search directions), the inner V-cycle PCG in batches of 8, and one
 // pass folding the inner solution (with its owed last update) into x64.  The host reads the
 // scalars once per batch; the first read of a step carries the FP64 residual.
-int refine_fused(mhdp_ctx* c, Solve& S, const mhdp_opts* o, uint32_t max_outer, bool warm, uint32_t* iterations,
-                 uint32_t* outer, double* res64) {
+int refine_fused(mhdp_ctx* c, Solve& S, const mhdp_opts* o, uint32_t max_outer, bool warm, bool started,
+                 uint32_t* iterations, uint32_t* outer, double* res64) {
     hipStream_t s = c->stream;
     *iterations = 0;
     *outer = 0;
     for (bool first = true;; first = false) {
-        launch_outer_start(s, c->g, c->x64, c->b64, c->bx, c->by, c->bz, c->dinv, c->r, c->x, c->p, c->p2, c->part,
-                           c->sc, first && !warm);
+        if (!(first && started))  // `started`: the caller has run the first step's residual pass
+            launch_outer_start(s, c->g, c->x64, c->b64, c->bx, c->by, c->bz, c->dinv, c->r, c->x, c->p, c->p2,
+                               c->part, c->sc, first && !warm);
         mg_initialize(c, S);
         uint32_t it = 0;
         for (bool checked = false;;) {
@@ -533,6 +606,7 @@ int refine_fused(mhdp_ctx* c, Solve& S, const mhdp_opts* o, uint32_t max_outer, 
             int rc = read_scalars(c);
             if (rc) return rc;
             const Scal& v = *c->sc_h;
+            if (v.fault) return kBreakdown;
             const double vals[10] = {v.rr64, v.bb64, v.rz, v.rr, v.bb, v.sr, v.alpha, v.beta, v.conv, v.iters};
             for (double x : vals)
                 if (!std::isfinite(x)) return fail(c, MHDP_NONFINITE, "Nonfinite fluid pressure residual");
@@ -568,6 +642,8 @@ int ensure_fine(mhdp_ctx* c) {
         (rc = dalloc(c, (void**)&c->p, fb)) || (rc = dalloc(c, (void**)&c->q, fb)) ||
         (rc = dalloc(c, (void**)&c->p2, fb)) || (rc = dalloc(c, (void**)&c->r2, fb)) ||
         (rc = dalloc(c, &c->z, fb)) || (rc = dalloc(c, &c->t, fb)) ||
+        (rc = dalloc(c, (void**)&c->x64, n * 8)) || (rc = dalloc(c, (void**)&c->b64, n * 8)) ||
+        (rc = dalloc(c, (void**)&c->phi_prev, n * 8)) ||
         (rc = dalloc(c, (void**)&c->sc, sizeof(Scal))) ||
         (rc = dalloc(c, (void**)&c->part, kMaxPartials * 4 * sizeof(double))))
         return rc;
@@ -595,24 +671,24 @@ int prepare(mhdp_ctx* c, bool meanfree, bool want16 = false) {
     return MHDP_OK;
 }
 
-int project_impl(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, float* uout, double* phi, mhdp_stats* st) {
-    int rc = validate_opts(c, o);
-    if (rc) return rc;
-    if (!c->cells) return fail(c, MHDP_INVALID, "Fluid cells have not been set");
-    if (!ustar) return fail(c, MHDP_INVALID, "Missing candidate face velocities");
+// multigrid.wgsl:69,87 (the weight is the f32 2/3 of the WGSL); multigrid.rs:67-82
+constexpr Cycle kReferenceCycle{(double)(2.0f / 3.0f), 1.0, 1};
+bool is_reference(const Cycle& cy) {
+    return cy.omega == kReferenceCycle.omega && cy.pscale == kReferenceCycle.pscale && cy.visits == 1;
+}
+
+// One projection with the cycle `cy`; kBreakdown when its multigrid PCG broke down.
+int project_once(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, const Cycle& cy, float* uout, double* phi,
+                 mhdp_stats* st) {
+    int rc;
     const bool meanfree = o->gauge != MHDP_GAUGE_PINNED && c->components == 1;
     const bool mixed = o->precision == MHDP_MIXED;
     const bool preferred = o->precond == MHDP_PRECOND_MG || (o->precond == MHDP_PRECOND_AUTO && c->g.n >= kMgMinCells);
-    if ((rc = dalloc(c, (void**)&c->x64, (size_t)c->g.n * 8)) || (rc = dalloc(c, (void**)&c->phi_prev, (size_t)c->g.n * 8)))
-        return rc;
-    if (mixed && (rc = dalloc(c, (void**)&c->b64, (size_t)c->g.n * 8))) return rc;
     const bool warm = o->warm_start && c->prev_valid;
     if (warm && !mixed && (rc = dalloc(c, (void**)&c->uwarm, face_count(c->grid) * sizeof(float)))) return rc;
-    if (preferred && (rc = alloc_mg(c, mixed))) return rc;
 
-    const double omega = o->smooth_weight > 0.0f ? (double)o->smooth_weight : kTunedOmega;
-    const double pscale = o->coarse_scale > 0.0f ? (double)o->coarse_scale : kTunedScale;
-    c->coarse_visits = coarse_visits_for(c, o->coarse_visits);
+    const double omega = cy.omega, pscale = cy.pscale;
+    c->coarse_visits = cy.visits;
     Solve S;
     S.meanfree = meanfree;
     S.store = mixed ? STORE_BF16 : STORE_F32;
@@ -637,6 +713,7 @@ int project_impl(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, float* uou
         const float abs2 = ((ab * ab) * (h2 * h2)) * (float)c->fluid_cells;
         Scal s;
         set_solve_params(c, s, meanfree, tol2, abs2, meanfree ? 1.0 : (double)nf, omega, pscale);
+        s.guard = preferred ? 1u : 0u;
         if ((rc = upload_scal(c, s))) return rc;
         launch_rhs_f32(c->stream, c->g, c->grid.h, c->mask, c->bx, c->by, c->bz, ustar, c->b);
         rc = pcg(c, S, preferred, o->max_iter, true, &iterations, &residual);
@@ -659,25 +736,36 @@ int project_impl(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, float* uou
         set_solve_params(c, s, meanfree, tau * tau, 0.0, 1.0, omega, pscale);
         s.refine_tol = o->refine_tol;
         s.tau = tau;
+        s.guard = preferred ? 1u : 0u;
         if ((rc = upload_scal(c, s))) return rc;
         launch_rhs_f64(c->stream, c->g, c->grid.h, c->mask, c->bx, c->by, c->bz, ustar, c->b64);
         // a warm start is the refinement begun from the previous potential (the FP64 residual
         // then sees exactly b - L phi_old; faces are corrected from u* with the full potential)
         if (warm) launch_old_potential(c->stream, c->g.n, c->mask, c->phi_prev, c->x64);
-        bool done = false;
-        const bool fast = preferred && c->fused_enabled && c->levels.size() >= 2;
-        if (fast) {
+        // The first refinement step's FP64 residual decides whether the optional multigrid is
+        // needed at all (a zero or already-met residual never allocates it, mod.rs:1031-1043).
+        launch_outer_start(c->stream, c->g, c->x64, c->b64, c->bx, c->by, c->bz, c->dinv, c->r, c->x, c->p, c->p2,
+                           c->part, c->sc, !warm);
+        if ((rc = read_scalars(c))) return rc;
+        const double r0 = c->sc_h->rr64, b0 = c->sc_h->bb64;
+        if (!std::isfinite(r0) || !std::isfinite(b0)) return fail(c, MHDP_NONFINITE, "Nonfinite fluid pressure residual");
+        bool done = c->sc_h->outer_done != 0u;
+        const bool fast = !done && preferred && c->fused_enabled && ensure_mg(c, true) && c->levels.size() >= 2;
+        if (done) {
+            res64 = b0 > 0.0 ? std::sqrt(r0 / b0) : 0.0;
+            if (!warm) launch_fill_f64(c->stream, c->g.n, c->x64, 0.0);
+        } else if (fast) {
             build_mg(c, meanfree ? 1 : 0, true);
             S.mg = true;
             S.lv = level_view(c);
             S.fused = true;
-            rc = refine_fused(c, S, o, max_outer, warm, &iterations, &outer, &res64);
+            rc = refine_fused(c, S, o, max_outer, warm, true, &iterations, &outer, &res64);
             if (rc && rc != MHDP_NOT_CONVERGED) return rc;
             done = rc == MHDP_OK;
         } else if (!warm) {
             launch_fill_f64(c->stream, c->g.n, c->x64, 0.0);
         }
-        for (; !fast;) {
+        for (; !fast && !done;) {
             launch_residual64(c->stream, c->g, c->x64, c->b64, c->bx, c->by, c->bz, c->dinv, c->b, c->part, c->sc);
             if ((rc = read_scalars(c))) return rc;
             const double rr = c->sc_h->rr, bb = c->sc_h->spare;
@@ -693,6 +781,7 @@ int project_impl(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, float* uou
             if (t > 0.5) t = 0.5;
             Scal si;
             set_solve_params(c, si, meanfree, t * t, 0.0, 1.0, omega, pscale);
+            si.guard = preferred ? 1u : 0u;
             if ((rc = upload_scal(c, si))) return rc;
             uint32_t it = 0;
             rc = pcg(c, S, preferred, o->max_iter, false, &it, &residual);
@@ -750,8 +839,33 @@ int project_impl(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, float* uou
         st->rel_residual = residual;
         st->rel_residual_f64 = mixed ? res64 : -1.0;
         st->seconds = ms * 1e-3;
+        st->cycle_fallback = 0;
     }
     return MHDP_OK;
+}
+
+// FluidState::project: validate, keep the FP64 vectors, solve with the requested (tuned) cycle;
+// if its multigrid PCG breaks down (r.z or p.Ap <= 0: the tuned cycle was not positive definite on
+// this problem), solve again with the reference's V-cycle, which is.
+int project_impl(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, float* uout, double* phi, mhdp_stats* st) {
+    int rc = validate_opts(c, o);
+    if (rc) return rc;
+    if (!c->cells) return fail(c, MHDP_INVALID, "Fluid cells have not been set");
+    if (!ustar) return fail(c, MHDP_INVALID, "Missing candidate face velocities");
+    Cycle cy{o->smooth_weight > 0.0f ? (double)o->smooth_weight : kTunedOmega,
+             o->coarse_scale > 0.0f ? (double)o->coarse_scale : kTunedScale, coarse_visits_for(c, o->coarse_visits)};
+    if (c->test_cycle.visits > 0) {  // test hook: the first attempt uses an unvalidated cycle
+        cy = c->test_cycle;
+        c->test_cycle = Cycle{0.0, 0.0, 0};
+    }
+    rc = project_once(c, ustar, o, cy, uout, phi, st);
+    if (rc == kBreakdown && !is_reference(cy)) {
+        rc = project_once(c, ustar, o, kReferenceCycle, uout, phi, st);
+        if (rc == MHDP_OK && st) st->cycle_fallback = 1;
+    }
+    if (rc == kBreakdown)
+        return fail(c, MHDP_NO_PROGRESS, "Fluid pressure iteration made no progress (the preconditioner is not positive definite)");
+    return rc;
 }
 
 // ============================================================================ batched projection
@@ -910,10 +1024,13 @@ struct KGatherScal {
 };
 
 // Eligible for the lockstep driver: the fused multigrid path of the mixed solver.
-bool lockstep_ok(const mhdp_ctx* c, const mhdp_opts* o) {
+// The batch allocates the multigrid storage before the first residual is known (it has no
+// counterpart in the reference's lazily allocating FluidState); a context whose allocation is refused
+// sends the whole batch to the per-context path, where that context solves with Jacobi.
+bool lockstep_ok(mhdp_ctx* c, const mhdp_opts* o) {
     const bool preferred = o->precond == MHDP_PRECOND_MG || (o->precond == MHDP_PRECOND_AUTO && c->g.n >= kMgMinCells);
     return c->cells && o->precision == MHDP_MIXED && preferred && !o->warm_start && c->fused_enabled &&
-           hierarchy(c->g).size() >= 2;
+           hierarchy(c->g).size() >= 2 && ensure_mg(c, true);
 }
 
 int project_lockstep(mhdp_ctx* const* ctxs, int count, const float* const* ustar, const mhdp_opts* o,
@@ -927,11 +1044,18 @@ int project_lockstep(mhdp_ctx* const* ctxs, int count, const float* const* ustar
         Solve S;
         bool running = false, done = false, inner = false;
         uint32_t outer = 0, iterations = 0, it = 0;
-        bool checked = false;
+        bool checked = false, breakdown = false;
         double res64 = 0.0;
-        int timer = -1;
     };
     std::vector<PS> ps((size_t)n);
+    // kernel timing is off during the batch; every exit restores each context's choice
+    struct TimerRestore {
+        mhdp_ctx* const* ctxs;
+        std::vector<int> kernel;
+        ~TimerRestore() {
+            for (size_t p = 0; p < kernel.size(); p++) ctxs[p]->timer.kernel = kernel[p];
+        }
+    } timers{ctxs, std::vector<int>((size_t)n, -1)};
     const double tau = o->inner_tol > 0.0f ? (double)o->inner_tol : 1e-4;
     const uint32_t max_outer = o->max_outer ? o->max_outer : 16u;
     const double omega = o->smooth_weight > 0.0f ? (double)o->smooth_weight : kTunedOmega;
@@ -940,26 +1064,28 @@ int project_lockstep(mhdp_ctx* const* ctxs, int count, const float* const* ustar
         mhdp_ctx* c = ctxs[p];
         c->err.clear();
         int rc = validate_opts(c, o);
-        if (!rc) c->coarse_visits = coarse_visits_for(c, o->coarse_visits);
-        if (!rc && !ustar[p]) rc = fail(c, MHDP_INVALID, "Missing candidate face velocities");
-        if (!rc && ((rc = dalloc(c, (void**)&c->x64, (size_t)c->g.n * 8)) ||
-                    (rc = dalloc(c, (void**)&c->b64, (size_t)c->g.n * 8)) ||
-                    (rc = dalloc(c, (void**)&c->phi_prev, (size_t)c->g.n * 8)) || (rc = alloc_mg(c, true)))) {
+        Cycle cy{omega, pscale, coarse_visits_for(c, o->coarse_visits)};
+        if (c->test_cycle.visits > 0) {  // test hook (mhdp_probe_next_cycle)
+            cy = c->test_cycle;
+            c->test_cycle = Cycle{0.0, 0.0, 0};
         }
+        if (!rc) c->coarse_visits = cy.visits;
+        if (!rc && !ustar[p]) rc = fail(c, MHDP_INVALID, "Missing candidate face velocities");
         PS& q = ps[(size_t)p];
         q.S.meanfree = o->gauge != MHDP_GAUGE_PINNED && c->components == 1;
         q.S.store = STORE_BF16;
         q.S.coarse64 = true;
         if (!rc) {
             Scal sc;
-            set_solve_params(c, sc, q.S.meanfree, tau * tau, 0.0, 1.0, omega, pscale);
+            set_solve_params(c, sc, q.S.meanfree, tau * tau, 0.0, 1.0, cy.omega, cy.pscale);
             sc.refine_tol = o->refine_tol;
             sc.tau = tau;
+            sc.guard = 1u;
             rc = upload_scal(c, sc);
         }
         rcs[(size_t)p] = rc;
         q.running = rc == MHDP_OK;
-        q.timer = c->timer.kernel;
+        timers.kernel[(size_t)p] = c->timer.kernel;
         c->timer.kernel = -1;
         c->timer.used = 0;
     }
@@ -1022,8 +1148,8 @@ int project_lockstep(mhdp_ctx* const* ctxs, int count, const float* const* ustar
             }))
             return MHDP_NO_MEMORY;
     }
-    PlanGraph rounds;
-    if (batch_graph_enabled() && !iter8.ops.empty() && !rounds.build(iter8)) return MHDP_DEVICE;
+    PlanGraph rounds;  // a capture or instantiation that fails leaves the rounds issued launch by launch
+    if (batch_graph_enabled() && !iter8.ops.empty() && !rounds.build(iter8)) rounds.exec = nullptr;
     std::vector<int> trans = members([](const PS& q) { return q.running; });
     for (int p : trans) ps[(size_t)p].inner = true;
     for (bool first_round = true;; first_round = false) {
@@ -1059,6 +1185,11 @@ int project_lockstep(mhdp_ctx* const* ctxs, int count, const float* const* ustar
         for (int p : now) {
             PS& q = ps[(size_t)p];
             const Scal& v = sh[(size_t)p];
+            if (v.fault) {  // multigrid PCG breakdown: solved again on its own after the batch
+                q.breakdown = true;
+                q.running = false;
+                continue;  // its stop flag (conv) is already set
+            }
             const double vals[10] = {v.rr64, v.bb64, v.rz, v.rr, v.bb, v.sr, v.alpha, v.beta, v.conv, v.iters};
             bool bad = false;
             for (double x : vals) bad |= !std::isfinite(x);
@@ -1120,8 +1251,19 @@ int project_lockstep(mhdp_ctx* const* ctxs, int count, const float* const* ustar
     for (int p = 0; p < n; p++) {
         mhdp_ctx* c = ctxs[p];
         PS& q = ps[(size_t)p];
-        c->timer.kernel = q.timer;
         if (rcs[(size_t)p] != MHDP_OK) continue;
+        if (q.breakdown) {  // solved again on its own with the reference's V-cycle (its single projection
+                            // would break down the same way: its results are those of the batch)
+            c->timer.kernel = timers.kernel[(size_t)p];
+            mhdp_stats* sp = st ? st + p : nullptr;
+            int rc = project_once(c, ustar[p], o, kReferenceCycle, uout ? uout[p] : nullptr, phi ? phi[p] : nullptr, sp);
+            if (rc == MHDP_OK && sp) sp->cycle_fallback = 1;
+            if (rc == kBreakdown)
+                rc = fail(c, MHDP_NO_PROGRESS,
+                          "Fluid pressure iteration made no progress (the preconditioner is not positive definite)");
+            rcs[(size_t)p] = rc;
+            continue;
+        }
         if (!q.done) {
             char buf[256];
             std::snprintf(buf, sizeof buf,
@@ -1145,6 +1287,7 @@ int project_lockstep(mhdp_ctx* const* ctxs, int count, const float* const* ustar
             t.rel_residual = (float)q.res64;
             t.rel_residual_f64 = q.res64;
             t.seconds = ms * 1e-3;
+            t.cycle_fallback = 0;
         }
     }
     return hipGetLastError() == hipSuccess ? MHDP_OK : MHDP_DEVICE;
@@ -1757,12 +1900,12 @@ int mhdp_probe_apply(mhdp_ctx* c, int meanfree, const float* p, float* q) {
 
 int mhdp_probe_vcycle(mhdp_ctx* c, int meanfree, int precision, float smooth_weight, float coarse_scale,
                       int coarse_visits, const float* r, float* z) {
-    if (!c || !r || !z || coarse_visits < 0 || coarse_visits > 4) return MHDP_INVALID;
+    if (!c || !r || !z || coarse_visits < 0 || coarse_visits > kMaxVisits) return MHDP_INVALID;
     c->coarse_visits = coarse_visits_for(c, coarse_visits);
     const bool f16 = precision == MHDP_MIXED;
     int rc = prepare(c, meanfree != 0);
     if (rc) return rc;
-    if ((rc = alloc_mg(c, f16))) return rc;
+    if (!ensure_mg(c, f16)) return fail(c, MHDP_NO_MEMORY, "Optional pressure preconditioner is unavailable");
     build_mg(c, meanfree != 0, f16);
     Scal s;
     set_solve_params(c, s, meanfree != 0, 0.0, 0.0, 1.0, smooth_weight, coarse_scale);
@@ -1799,7 +1942,7 @@ int mhdp_probe_mg_graph(mhdp_ctx* c, int meanfree, int level, float* graph5, uin
     if (!c) return MHDP_INVALID;
     int rc = prepare(c, meanfree != 0);
     if (rc) return rc;
-    if ((rc = alloc_mg(c, false))) return rc;
+    if (!ensure_mg(c, false)) return fail(c, MHDP_NO_MEMORY, "Optional pressure preconditioner is unavailable");
     if (level < 1 || level >= (int)c->levels.size()) return fail(c, MHDP_INVALID, "No such multigrid level");
     build_mg(c, meanfree != 0, false);
     const Grid3& g = c->levels[level];
@@ -1817,6 +1960,26 @@ int mhdp_probe_mg_graph(mhdp_ctx* c, int meanfree, int level, float* graph5, uin
             HIP_OK(hipMemcpyAsync(graph5 + (size_t)k * g.n, src[k], fb, hipMemcpyDeviceToHost, c->stream));
     }
     HIP_OK(hipStreamSynchronize(c->stream));
+    return MHDP_OK;
+}
+
+int mhdp_probe_multigrid(mhdp_ctx* c, int action, int arg, int* state, uint64_t* bytes) {
+    if (!c || action < 0 || action > 2) return MHDP_INVALID;
+    if (action == 1) c->mg_fail_after = arg < 0 ? 0 : arg;
+    if (action == 2) {
+        release_mg(c);
+        c->mg_refused = true;
+    }
+    if (state) *state = c->mg_refused ? 2 : (c->mg_alloc ? 1 : 0);
+    if (bytes) *bytes = c->mg_bytes;
+    return MHDP_OK;
+}
+
+int mhdp_probe_next_cycle(mhdp_ctx* c, float smooth_weight, float coarse_scale, int coarse_visits) {
+    if (!c || coarse_visits < 1 || coarse_visits > kMaxVisits || !std::isfinite(smooth_weight) ||
+        !std::isfinite(coarse_scale))
+        return MHDP_INVALID;
+    c->test_cycle = Cycle{(double)smooth_weight, (double)coarse_scale, coarse_visits};
     return MHDP_OK;
 }
 

@@ -44,6 +44,8 @@ struct Scal {
     uint32_t outer_done;              // FP64 refinement reached refine_tol (inner work is skipped)
     uint32_t rsel;                    // brick path: the residual lives in r (0) or its twin (1)
     double mean_next;                 // brick path: mean of the residual the next V-cycle sees
+    uint32_t guard;                   // multigrid PCG: watch for a breakdown (r.z or p.Ap <= 0)
+    uint32_t fault;                   // a watched breakdown happened (the solve then stops: conv = 1)
     uint32_t ticket[8];
 };
 
@@ -199,6 +201,10 @@ __device__ inline void finalize(Scal* s, int mode, const double t[4]) {
         break;
     case FIN_ALPHA:  // t[1] = sum of q over the active cells (brick apply)
         s->pq = t[0];
+        if (s->guard && s->conv < 0.5 && t[0] <= 0.0) {  // p.Ap <= 0: the preconditioner is not SPD
+            s->fault = 1u;
+            s->conv = 1.0;
+        }
         s->alpha = s->conv > 0.5 ? 0.0 : s->rz / fmax(t[0], 1e-35);
         s->pending = 1u;
         // mean of the next residual r' = (r - mean) - alpha q, known before r' exists
@@ -229,6 +235,10 @@ __device__ inline void finalize(Scal* s, int mode, const double t[4]) {
             s->rz = t[0];
             s->rr = rr;
             s->conv = rr <= (s->tol2 * s->bb + s->abs2) / s->bound ? 1.0 : 0.0;
+            if (mode == FIN_BETA_MG && s->guard && s->conv < 0.5 && t[0] <= 0.0) {  // r.z <= 0
+                s->fault = 1u;
+                s->conv = 1.0;
+            }
         }
         break;
     case FIN_STASH_RR:
@@ -239,6 +249,10 @@ __device__ inline void finalize(Scal* s, int mode, const double t[4]) {
         s->alpha = 0.0; s->beta = 0.0;
         s->pending = 0u;
         s->conv = (s->rr <= s->tol2 * s->bb + s->abs2 || s->bb == 0.0) ? 1.0 : 0.0;
+        if (s->guard && s->conv < 0.5 && t[0] <= 0.0) {  // r.z <= 0 for r != 0
+            s->fault = 1u;
+            s->conv = 1.0;
+        }
         break;
     case FIN_CHECK:
         s->rr = t[1];

@@ -282,6 +282,20 @@ class PressureProjection:
         self._check(self.lib.mhdp_probe_mg_graph(self.handle, int(meanfree), level, _ptr(g5), _ptr(dims)))
         return tuple(int(v) for v in dims), g5.reshape(5, int(dims[3]))
 
+    def multigrid_state(self, action=0, arg=0):
+        """The optional multigrid storage: (state, bytes) with state "none", "allocated" or
+        "refused".  action 1: the next allocation attempt fails after ``arg`` arrays; action 2: free
+        it and record a refused attempt (test hooks of the reference's lazy-workspace test)."""
+        st = C.c_int(0)
+        nb = C.c_uint64(0)
+        self._check(self.lib.mhdp_probe_multigrid(self.handle, int(action), int(arg), C.byref(st), C.byref(nb)))
+        return ("none", "allocated", "refused")[st.value], int(nb.value)
+
+    def next_cycle(self, smooth_weight, coarse_scale=1.0, coarse_visits=1):
+        """Test hook: the next projection's first attempt uses this cycle unvalidated."""
+        self._check(self.lib.mhdp_probe_next_cycle(self.handle, float(smooth_weight), float(coarse_scale),
+                                                   int(coarse_visits)))
+
     def rhs(self, u_star: np.ndarray, meanfree: bool):
         u = np.ascontiguousarray(np.asarray(u_star, dtype=np.float32).ravel())
         b = np.zeros(self.grid.cells, dtype=np.float64)
