@@ -173,6 +173,17 @@ typedef struct {
  * restores its checkpoint, mod.rs:1312-1328, and so must the caller). */
 int mhdp_fields_dev(mhdp_ctx *ctx, const float *u_faces_dev, const double *phi_dev, const float *em_dev,
                     const mhdp_fields_params *params, float *cells_dev, mhdp_diagnostics *diag);
+/* mhdp_fields_dev without any device allocation: the per-block partial records and the folded
+ * totals live in a static device array of the library (328 KB per GPU, held until the process
+ * ends; the partial records of at most 4096 blocks, the largest launch a tuning allows - a larger
+ * need is rejected with MHDP_INVALID).  No counterpart in the reference: for callers that cannot
+ * allocate device memory (HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The context must have its cells set (mhdp_set_cells builds the open-face bytes the
+ * kernels read).  Every call on one GPU shares the array: calls that overlap on one GPU (several
+ * streams or threads at once) race and must be serialised by the caller.  Results equal
+ * mhdp_fields_dev's bit for bit. */
+int mhdp_fields_noalloc_dev(mhdp_ctx *ctx, const float *u_faces_dev, const double *phi_dev, const float *em_dev,
+                            const mhdp_fields_params *params, float *cells_dev, mhdp_diagnostics *diag);
 /* The momentum step that produces u* for the projection (src/fluid/mod.rs:1285-1303): predict, then
  * viscous (solver.wgsl:127-163), mushy mobility factor off.  Device pointers, faces as above. */
 typedef struct {

@@ -23,7 +23,7 @@ SYMBOLS = (
     "mhdp_anchors", "mhdp_hierarchy", "mhdp_set_kernel_timing", "mhdp_kernel_time", "mhdp_get_launch_tuning",
     "mhdp_set_launch_tuning",
     "mhdp_probe_coefficients", "mhdp_probe_apply", "mhdp_probe_vcycle", "mhdp_probe_mg_graph", "mhdp_probe_rhs",
-    "mhdp_probe_multigrid", "mhdp_probe_next_cycle",
+    "mhdp_probe_multigrid", "mhdp_probe_next_cycle", "mhdp_fields_noalloc_dev",
 )
 
 OK, NOT_CONVERGED, NO_PROGRESS, NONFINITE, INVALID, NO_MEMORY, DEVICE = range(7)
@@ -133,6 +133,7 @@ def load(path: str = LIB_PATH):
         "mhdp_project_dev": (I, [P, P, C.POINTER(Opts), P, P, C.POINTER(Stats)]),
         "mhdp_project_batch": (I, [P, I, P, C.POINTER(Opts), P, P, P]),
         "mhdp_fields_dev": (I, [P, P, P, P, C.POINTER(FieldsParams), P, C.POINTER(Diagnostics)]),
+        "mhdp_fields_noalloc_dev": (I, [P, P, P, P, C.POINTER(FieldsParams), P, C.POINTER(Diagnostics)]),
         "mhdp_predict_dev": (I, [P, P, C.POINTER(StepParams), P]),
         "mhdp_viscous_dev": (I, [P, P, P, P, C.POINTER(StepParams), P]),
         "mhdp_surface_stress_dev": (I, [P, P, P, C.POINTER(StressParams), P]),

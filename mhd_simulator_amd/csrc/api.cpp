@@ -13,6 +13,7 @@
 #include <cstring>
 #include <numeric>
 #include <string>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -74,6 +75,7 @@ struct mhdp_ctx {
     bool prev_valid = false;
     float* uwarm = nullptr;      // u* corrected by the previous potential (warm start)
     double* fields_part = nullptr;  // fields(): per-block totals, then the 10 folded totals
+    double* fields_h = nullptr;     // pinned host copy of the 10 folded totals
     float* faces_open = nullptr;    // predict(): open-face copy of its input (one float per face)
     uint8_t* open8 = nullptr;       // step kernels: per cell, bit a = lower face on axis a open
     float *stress_mu = nullptr, *stress_rho = nullptr, *stress_part = nullptr;  // viscous stress diffusion
@@ -1314,6 +1316,41 @@ int batch_workers() {
 
 }  // namespace
 
+// ============================================================================ static fields scratch
+// The partial records of mhdp_fields_noalloc_dev live in a file-scope device array: one record of
+// kFieldTotals doubles per block of the largest launch a tuning allows, then the folded totals
+// (328 KB).  Each GPU holds its own copy for the life of the process; every call on a GPU shares
+// it (every block writes its whole record before the fold reads it, so nothing needs clearing), and
+// calls that overlap on one GPU race.  Host code never passes the array's name to a kernel: that is
+// a host-side stand-in, not a device address.  The address is taken once per device with
+// hipGetSymbolAddress, with that device current, and kept.
+namespace {
+constexpr size_t kFieldsScratch = ((size_t)kMaxPartials + 1) * kFieldTotals;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+static __device__ __align__(256) double scratch[kFieldsScratch];
+
+// This device's address of `scratch` (nullptr when the device cannot be named or the call fails).
+static double* fields_scratch() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = static_cast<double*>(ptr);
+    }
+    return g_scratch_ptr[dev];
+}
+
 // ============================================================================ launch tuning
 namespace {
 uint32_t env_u32(const char* name, uint32_t lo, uint32_t hi, uint32_t def) {
@@ -1386,7 +1423,8 @@ int mhdp_create(mhdp_ctx** out, const mhdp_grid* grid, const mhdp_phase* phase) 
     c->g.n = (uint32_t)n;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
-        hipHostMalloc((void**)&c->sc_h, sizeof(Scal), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void**)&c->sc_h, sizeof(Scal), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&c->fields_h, kFieldTotals * sizeof(double), hipHostMallocDefault) != hipSuccess) {
         (void)hipGetLastError();
         mhdp_destroy(c);
         return MHDP_DEVICE;
@@ -1467,6 +1505,12 @@ int mhdp_set_cells(mhdp_ctx* c, const uint8_t* fluid, const float* alpha) {
     c->fluid_cells = counts[1];
     c->anchor0 = counts[0] ? counts[2] : 0u;
     c->labels_on_host = host;
+    // the per-cell open-face bytes of the step and fields kernels depend only on the mask: built
+    // here, with the set-up, so no later entry point has to allocate them
+    if ((rc = dalloc(c, (void**)&c->open8, n))) return rc;
+    launch_open8(c->stream, c->g, c->mask, c->open8);
+    HIP_OK(hipStreamSynchronize(c->stream));
+    c->open8_fresh = true;
     c->cells = true;
     return MHDP_OK;
 }
@@ -1491,34 +1535,32 @@ int mhdp_set_potential(mhdp_ctx* c, const double* phi, int on_device) {
     return MHDP_OK;
 }
 
-int mhdp_fields_dev(mhdp_ctx* c, const float* u_faces, const double* phi, const float* em,
-                    const mhdp_fields_params* fp, float* cells, mhdp_diagnostics* diag) {
-    if (!c) return MHDP_INVALID;
-    c->err.clear();
+// Argument checks shared by the fields entry points.
+static int fields_check(mhdp_ctx* c, const float* u_faces, const double* phi, const mhdp_fields_params* fp,
+                        const float* cells) {
     if (!c->cells) return fail(c, MHDP_INVALID, "Fluid cells are not set");
     if (!u_faces || !phi || !fp || !cells) return fail(c, MHDP_INVALID, "Missing fields argument");
     if ((reinterpret_cast<uintptr_t>(cells) & 15u) != 0)
         return fail(c, MHDP_INVALID, "Cell packets must be 16-byte aligned");
     if (!(fp->dt > 0.0f) || !std::isfinite(fp->dt) || (fp->gauge != MHDP_GAUGE_AUTO && fp->gauge != MHDP_GAUGE_PINNED))
         return fail(c, MHDP_INVALID, "Invalid fields parameters");
-    const uint32_t blocks = grid_blocks(c->g.n);
-    // sized for the largest block cap a launch tuning allows (the buffer is kept across calls)
-    int rc = dalloc(c, (void**)&c->fields_part, ((size_t)kMaxPartials + 1) * kFieldTotals * sizeof(double));
-    if (rc) return rc;
     if (face_count(c->grid) >= (1ull << 32)) return fail(c, MHDP_INVALID, "Too many faces for the step kernels");
-    if (!c->open8_fresh) {
-        if ((rc = dalloc(c, (void**)&c->open8, c->g.n))) return rc;
-        launch_open8(c->stream, c->g, c->mask, c->open8);
-        c->open8_fresh = true;
-    }
+    return MHDP_OK;
+}
+
+// The fields + diagnostics launches and read_diagnostics, given the device partial records `part`
+// ((blocks + 1) * kFieldTotals doubles: one record per block, then the folded totals) and a host
+// buffer of kFieldTotals doubles for the totals.  Allocates nothing.
+static int fields_run(mhdp_ctx* c, const float* u_faces, const double* phi, const float* em,
+                      const mhdp_fields_params* fp, float* cells, mhdp_diagnostics* diag, double* part, double* t) {
+    const uint32_t blocks = grid_blocks(c->g.n);
     const bool meanfree = fp->gauge != MHDP_GAUGE_PINNED && c->components == 1;
-    double* totals = c->fields_part + (size_t)blocks * kFieldTotals;
+    double* totals = part + (size_t)blocks * kFieldTotals;
     launch_fields(c->stream, c->g, c->grid.h, c->phase.rho_liquid, c->phase.rho_air, c->phase.two_phase ? 1 : 0,
                   fp->dt, fp->acceleration, fp->mushy ? 1 : 0, meanfree ? 1 : 0, c->mask, c->open8, u_faces, phi, em,
-                  cells, c->fields_part, totals);
+                  cells, part, totals);
     HIP_OK(hipGetLastError());
-    double t[kFieldTotals];
-    HIP_OK(hipMemcpyAsync(t, totals, sizeof t, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipMemcpyAsync(t, totals, kFieldTotals * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
     // read_diagnostics (src/fluid/mod.rs:1116-1146)
     const double h3 = (double)c->grid.h * c->grid.h * c->grid.h;
@@ -1534,10 +1576,44 @@ int mhdp_fields_dev(mhdp_ctx* c, const float* u_faces, const double* phi, const 
     d.max_face_speed = t[8];
     d.fluid_mass = (double)c->phase.rho_liquid * t[9] * h3;
     if (diag) *diag = d;
-    for (double v : t)
-        if (!std::isfinite(v)) return fail(c, MHDP_NONFINITE, "Nonfinite fluid fields; state restored");
+    for (int k = 0; k < kFieldTotals; k++)
+        if (!std::isfinite(t[k])) return fail(c, MHDP_NONFINITE, "Nonfinite fluid fields; state restored");
     if (t[7] != 0.0) return fail(c, MHDP_NONFINITE, "Nonfinite fluid fields; state restored");
     return MHDP_OK;
+}
+
+int mhdp_fields_dev(mhdp_ctx* c, const float* u_faces, const double* phi, const float* em,
+                    const mhdp_fields_params* fp, float* cells, mhdp_diagnostics* diag) {
+    if (!c) return MHDP_INVALID;
+    c->err.clear();
+    int rc = fields_check(c, u_faces, phi, fp, cells);
+    if (rc) return rc;
+    // sized for the largest block cap a launch tuning allows (the buffer is kept across calls)
+    if ((rc = dalloc(c, (void**)&c->fields_part, ((size_t)kMaxPartials + 1) * kFieldTotals * sizeof(double))))
+        return rc;
+    if (!c->open8_fresh) {
+        if ((rc = dalloc(c, (void**)&c->open8, c->g.n))) return rc;
+        launch_open8(c->stream, c->g, c->mask, c->open8);
+        c->open8_fresh = true;
+    }
+    double t[kFieldTotals];
+    return fields_run(c, u_faces, phi, em, fp, cells, diag, c->fields_part, t);
+}
+
+int mhdp_fields_noalloc_dev(mhdp_ctx* c, const float* u_faces, const double* phi, const float* em,
+                            const mhdp_fields_params* fp, float* cells, mhdp_diagnostics* diag) {
+    if (!c) return MHDP_INVALID;
+    c->err.clear();
+    int rc = fields_check(c, u_faces, phi, fp, cells);
+    if (rc) return rc;
+    if (!c->open8_fresh || !c->open8) return fail(c, MHDP_INVALID, "Fluid cells are not set");
+    const uint32_t blocks = grid_blocks(c->g.n);
+    if (((size_t)blocks + 1) * kFieldTotals > kFieldsScratch)
+        return fail(c, MHDP_INVALID, "Fields partial records exceed the static scratch (" +
+                                         std::to_string(kFieldsScratch / kFieldTotals - 1) + " blocks)");
+    double* part = fields_scratch();
+    if (!part) return fail(c, MHDP_DEVICE, "HIP error: static fields scratch has no device address");
+    return fields_run(c, u_faces, phi, em, fp, cells, diag, part, c->fields_h);
 }
 
 static int step_check(mhdp_ctx* c, const mhdp_step_params* sp) {
@@ -1861,6 +1937,7 @@ void mhdp_destroy(mhdp_ctx* c) {
     }
     for (hipEvent_t e : c->timer.ev) (void)hipEventDestroy(e);
     if (c->sc_h) (void)hipHostFree(c->sc_h);
+    if (c->fields_h) (void)hipHostFree(c->fields_h);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);

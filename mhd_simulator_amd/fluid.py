@@ -234,11 +234,13 @@ class PressureProjection:
 
     # -- fields() + diagnostics() (solver.wgsl:256-291, read_diagnostics mod.rs:1116-1146)
     def fields(self, u_faces, phi, cells, *, dt, acceleration=(0.0, 0.0, 0.0), mushy=False, gauge="auto",
-               em=None):
+               em=None, no_alloc=False):
         """Rewrite ``cells`` (device float32, 16 per cell: the FluidCell packet, whose reserved words
         are read first) from the projected faces and the potential (device tensors); returns the
         diagnostics as a dict.  Raises PressureError ("Nonfinite fluid fields; state restored") when a
-        total is not finite or a cell is flagged - ``cells`` has then been written."""
+        total is not finite or a cell is flagged - ``cells`` has then been written.  ``no_alloc``: the
+        entry point that allocates no device memory (its partial records live in the library's static
+        per-GPU scratch; calls overlapping on one GPU race)."""
         for t, n in ((u_faces, self.grid.faces), (phi, self.grid.cells), (cells, 16 * self.grid.cells)):
             if not hasattr(t, "data_ptr") or t.numel() != n:
                 raise ValueError("fields takes device tensors of the grid's sizes")
@@ -246,8 +248,8 @@ class PressureProjection:
             raise ValueError("em holds 16 floats per cell")
         fp = _lib.FieldsParams(dt, (C.c_float * 3)(*acceleration), int(bool(mushy)), _lib.GAUGE[gauge])
         d = _lib.Diagnostics()
-        self._check(self.lib.mhdp_fields_dev(self.handle, _ptr(u_faces), _ptr(phi), _ptr(em), C.byref(fp),
-                                             _ptr(cells), C.byref(d)))
+        fn = self.lib.mhdp_fields_noalloc_dev if no_alloc else self.lib.mhdp_fields_dev
+        self._check(fn(self.handle, _ptr(u_faces), _ptr(phi), _ptr(em), C.byref(fp), _ptr(cells), C.byref(d)))
         return d.as_dict()
 
     # -- test hooks (the reference's cfg(test) probes)
