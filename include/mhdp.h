@@ -47,6 +47,7 @@ typedef enum {
 enum { MHDP_PRECOND_AUTO = 0, MHDP_PRECOND_JACOBI = 1, MHDP_PRECOND_MG = 2 };
 enum { MHDP_FP32 = 0, MHDP_MIXED = 1 };
 enum { MHDP_GAUGE_AUTO = 0, MHDP_GAUGE_PINNED = 2 };
+enum { MHDP_SMOOTH_AUTO = 0, MHDP_SMOOTH_JACOBI = 1, MHDP_SMOOTH_RED_BLACK = 2 };
 
 /* Grid of the pressure cells: Params.dims / flags.x / physical.x (mod.rs:118-130). */
 typedef struct {
@@ -93,6 +94,14 @@ typedef struct {
                             0.8); 0 = automatic: 2 on grids of at least 2^24 cells, else 1 (below
                             that the second visit's small coarse launches cost more than the
                             iterations it saves) */
+    int smoother;        /* multigrid smoother: MHDP_SMOOTH_JACOBI the reference's weighted Jacobi
+                            (multigrid.wgsl:66-73, weight smooth_weight); MHDP_SMOOTH_RED_BLACK
+                            symmetric red-black Gauss-Seidel (red then black before the coarse
+                            correction, black then red after it) on every level whose periodic
+                            axes have even length (Jacobi on the others); AUTO = Jacobi.
+                            The coarse scale's tuned default is 1.6 with Jacobi, 1.8 with
+                            red-black; the fused multigrid path only (the one-kernel-per-stage
+                            path, MHDP_UNFUSED=1, keeps Jacobi) */
 } mhdp_opts;
 
 typedef struct {
@@ -331,10 +340,11 @@ int mhdp_set_launch_tuning(const mhdp_launch_tuning *in);
 int mhdp_probe_coefficients(mhdp_ctx *ctx, int meanfree, float *beta3, float *diag, float *dinv);
 /* apply(): q = A p with the cached operator (identity rows at pinned anchors). */
 int mhdp_probe_apply(mhdp_ctx *ctx, int meanfree, const float *p, float *q);
-/* One cycle z = M r (precision MHDP_FP32 or MHDP_MIXED storage) with the given smoother weight,
- * coarse-correction scale and level-1 visits (2/3, 1 and 1 reproduce the reference's V-cycle). */
+/* One cycle z = M r (precision MHDP_FP32 or MHDP_MIXED storage) with the given smoother
+ * (MHDP_SMOOTH_*; AUTO = Jacobi here), its weight, coarse-correction scale and level-1 visits
+ * (Jacobi, 2/3, 1 and 1 reproduce the reference's V-cycle). */
 int mhdp_probe_vcycle(mhdp_ctx *ctx, int meanfree, int precision, float smooth_weight, float coarse_scale,
-                      int coarse_visits, const float *r, float *z);
+                      int coarse_visits, int smoother, const float *r, float *z);
 /* Coarse graph of level l >= 1 (lower-slot edges x,y,z, reaction, diagonal: 5 floats/cell). */
 int mhdp_probe_mg_graph(mhdp_ctx *ctx, int meanfree, int level, float *graph5, uint32_t *dims4);
 /* Divergence right-hand side b = -h^2 div u* (physical, no anchor zeroing), FP64. */

@@ -30,6 +30,7 @@ OK, NOT_CONVERGED, NO_PROGRESS, NONFINITE, INVALID, NO_MEMORY, DEVICE = range(7)
 PRECOND = {"auto": 0, "jacobi": 1, "mg": 2}
 PRECISION = {"fp32": 0, "mixed": 1}
 GAUGE = {"auto": 0, "pinned": 2}
+SMOOTHER = {"auto": 0, "jacobi": 1, "red-black": 2}
 KERNELS = {"apply": 0, "update": 1, "smooth": 2, "restrict": 3}
 EM_KERNELS = {"curl": 0, "apply": 1, "update": 2, "search": 3}
 
@@ -47,7 +48,8 @@ class Opts(C.Structure):
     _fields_ = [("rel_tol", C.c_float), ("abs_div_tol", C.c_float), ("max_iter", C.c_uint32),
                 ("precond", C.c_int), ("precision", C.c_int), ("gauge", C.c_int), ("refine_tol", C.c_double),
                 ("max_outer", C.c_uint32), ("inner_tol", C.c_float), ("smooth_weight", C.c_float),
-                ("coarse_scale", C.c_float), ("warm_start", C.c_int), ("coarse_visits", C.c_int)]
+                ("coarse_scale", C.c_float), ("warm_start", C.c_int), ("coarse_visits", C.c_int),
+                ("smoother", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -159,7 +161,7 @@ def load(path: str = LIB_PATH):
         "mhdp_hierarchy": (I, [P, P, I, P]),
         "mhdp_probe_coefficients": (I, [P, I, P, P, P]),
         "mhdp_probe_apply": (I, [P, I, P, P]),
-        "mhdp_probe_vcycle": (I, [P, I, I, C.c_float, C.c_float, I, P, P]),
+        "mhdp_probe_vcycle": (I, [P, I, I, C.c_float, C.c_float, I, I, P, P]),
         "mhdp_probe_mg_graph": (I, [P, I, I, P, P]),
         "mhdp_probe_rhs": (I, [P, I, P, P]),
         "mhdp_probe_multigrid": (I, [P, I, I, P, P]),

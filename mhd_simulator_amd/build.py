@@ -30,6 +30,7 @@ SOURCES = {
     "fused.hip": [],
     "brick.hip": [],
     "march.hip": [],
+    "rbgs.hip": [],
     "cc.hip": [],
     "step.hip": ["-ffp-contract=off"],
     "em.hip": ["-ffp-contract=off"],

@@ -38,6 +38,7 @@ constexpr int kBreakdown = 100;
 struct Cycle {
     double omega, pscale;
     int visits;
+    bool rb = false;  // red-black Gauss-Seidel (else weighted Jacobi with weight omega)
 };
 
 struct CoarseArrays {
@@ -110,6 +111,7 @@ struct mhdp_ctx {
     bool brick_enabled = true;  // MHDP_NO_BRICK=1 keeps the 4-cell-group fused kernels
     bool march_enabled = true;  // z-marching fine passes where the grid allows (MHDP_NO_MARCH=1: bricks)
     int coarse_visits = 1;      // level-1 visits per cycle (mhdp_opts.coarse_visits)
+    bool rb = false;            // red-black Gauss-Seidel smoothing (mhdp_opts.smoother)
     KernelTimer timer;
     double timer_ms = 0.0;
     uint32_t timer_launches = 0;
@@ -229,6 +231,8 @@ int validate_opts(mhdp_ctx* c, const mhdp_opts* o) {
         return fail(c, MHDP_INVALID, "Invalid V-cycle smoother weight or coarse-correction scale");
     if (o->coarse_visits < 0 || o->coarse_visits > kMaxVisits)
         return fail(c, MHDP_INVALID, "Invalid multigrid coarse visits");
+    if (o->smoother < MHDP_SMOOTH_AUTO || o->smoother > MHDP_SMOOTH_RED_BLACK)
+        return fail(c, MHDP_INVALID, "Invalid multigrid smoother");
     if (o->precision == MHDP_MIXED &&
         (!(o->refine_tol > 0.0) || !(o->refine_tol < 1.0) || o->inner_tol < 0.0f || o->inner_tol >= 1.0f))
         return fail(c, MHDP_INVALID, "Invalid refinement tolerance");
@@ -416,9 +420,13 @@ void vcycle_fused(mhdp_ctx* c, Solve& S, int mode, bool update = false) {
     LevelDev* lv = S.lv.data();
     const int L = (int)S.lv.size();
     const bool brick = c->brick_enabled && brick_ok(c->g);
+    const bool rb = c->rb && rb_level_ok(c->g);  // red-black passes on the fine level
     c->timer.before(MHDP_K_RESTRICT, s);
     const bool march = brick && c->march_enabled && S.store == STORE_BF16 && march_ok(c->g);
-    if (march && update)
+    if (rb)
+        launch_rb_restrict(s, S.store, false, lv[1].g, lv[0].g, c->r, c->r2, c->r, c->r2, update ? c->q : nullptr, dinv,
+                           bx, by, bz, lv[1].r, c->part, c->sc);
+    else if (march && update)
         launch_march_restrict_update(s, lv[1].g, lv[0].g, c->r, c->r2, c->q, dinv, bx, by, bz, lv[1].r,
                                      c->part, c->sc);
     else if (brick && update)
@@ -430,10 +438,13 @@ void vcycle_fused(mhdp_ctx* c, Solve& S, int mode, bool update = false) {
         launch_fused_restrict(s, S.store, lv[1].g, lv[0].g, c->r, dinv, bx, by, bz, lv[1].r, c->sc);
     c->timer.after(MHDP_K_RESTRICT, s);
     void* z1 = nullptr;
-    launch_vcycle_coarse(s, S.store, L, lv, c->sc, S.coarse64, c->coarse_visits, &z1);
+    launch_vcycle_coarse(s, S.store, L, lv, c->sc, S.coarse64, c->coarse_visits, &z1, c->rb);
     record_mark();
     c->timer.before(MHDP_K_SMOOTH, s);
-    if (march)
+    if (rb)
+        launch_rb_smooth(s, S.store, false, lv[0].g, lv[1].g, c->r, c->r2, dinv, bx, by, bz, z1, lv[0].t, nullptr, 0.0f,
+                         c->part, c->sc, mode, update);
+    else if (march)
         launch_march_smooth(s, lv[0].g, lv[1].g, c->r, c->r2, dinv, bx, by, bz, z1, lv[0].t, c->part, c->sc,
                             mode, update);
     else if (brick)
@@ -504,7 +515,8 @@ void iterate_once(mhdp_ctx* c, Solve& S) {  // mod.rs:1057-1075, multigrid.rs:30
 }
 
 constexpr double kTunedOmega = 0.9;   // smoother weight picked by the V-cycle tuner
-constexpr double kTunedScale = 1.6;   // coarse-correction scale picked by the V-cycle tuner
+constexpr double kTunedScale = 1.6;   // coarse-correction scale picked by the V-cycle tuner (Jacobi)
+constexpr double kTunedScaleRB = 1.8; // the same with red-black Gauss-Seidel smoothing
 // Level-1 visits per cycle when the caller leaves the choice (coarse_visits = 0): a second visit
 // (W-cycle at level 1) saves about a third of the iterations, but its coarse levels are launch-bound
 // small kernels; measured on the plate workload it pays from 256^3 (12.1 vs 12.7 ms; 512^3: 75 vs 83
@@ -674,9 +686,19 @@ int prepare(mhdp_ctx* c, bool meanfree, bool want16 = false) {
 }
 
 // multigrid.wgsl:69,87 (the weight is the f32 2/3 of the WGSL); multigrid.rs:67-82
-constexpr Cycle kReferenceCycle{(double)(2.0f / 3.0f), 1.0, 1};
+constexpr Cycle kReferenceCycle{(double)(2.0f / 3.0f), 1.0, 1, false};
 bool is_reference(const Cycle& cy) {
-    return cy.omega == kReferenceCycle.omega && cy.pscale == kReferenceCycle.pscale && cy.visits == 1;
+    return !cy.rb && cy.omega == kReferenceCycle.omega && cy.pscale == kReferenceCycle.pscale && cy.visits == 1;
+}
+
+// The cycle the options ask for: smoother, its weight and coarse scale (0 = tuned defaults), visits.
+Cycle requested_cycle(const mhdp_ctx* c, const mhdp_opts* o) {
+    Cycle cy;
+    cy.rb = o->smoother == MHDP_SMOOTH_RED_BLACK;  // AUTO: Jacobi (fewer, slower passes cost more at 512^3)
+    cy.omega = o->smooth_weight > 0.0f ? (double)o->smooth_weight : kTunedOmega;
+    cy.pscale = o->coarse_scale > 0.0f ? (double)o->coarse_scale : (cy.rb ? kTunedScaleRB : kTunedScale);
+    cy.visits = coarse_visits_for(c, o->coarse_visits);
+    return cy;
 }
 
 // One projection with the cycle `cy`; kBreakdown when its multigrid PCG broke down.
@@ -691,6 +713,7 @@ int project_once(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, const Cycl
 
     const double omega = cy.omega, pscale = cy.pscale;
     c->coarse_visits = cy.visits;
+    c->rb = cy.rb;
     Solve S;
     S.meanfree = meanfree;
     S.store = mixed ? STORE_BF16 : STORE_F32;
@@ -854,8 +877,7 @@ int project_impl(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, float* uou
     if (rc) return rc;
     if (!c->cells) return fail(c, MHDP_INVALID, "Fluid cells have not been set");
     if (!ustar) return fail(c, MHDP_INVALID, "Missing candidate face velocities");
-    Cycle cy{o->smooth_weight > 0.0f ? (double)o->smooth_weight : kTunedOmega,
-             o->coarse_scale > 0.0f ? (double)o->coarse_scale : kTunedScale, coarse_visits_for(c, o->coarse_visits)};
+    Cycle cy = requested_cycle(c, o);
     if (c->test_cycle.visits > 0) {  // test hook: the first attempt uses an unvalidated cycle
         cy = c->test_cycle;
         c->test_cycle = Cycle{0.0, 0.0, 0};
@@ -1066,12 +1088,15 @@ int project_lockstep(mhdp_ctx* const* ctxs, int count, const float* const* ustar
         mhdp_ctx* c = ctxs[p];
         c->err.clear();
         int rc = validate_opts(c, o);
-        Cycle cy{omega, pscale, coarse_visits_for(c, o->coarse_visits)};
+        Cycle cy = requested_cycle(c, o);
         if (c->test_cycle.visits > 0) {  // test hook (mhdp_probe_next_cycle)
             cy = c->test_cycle;
             c->test_cycle = Cycle{0.0, 0.0, 0};
         }
-        if (!rc) c->coarse_visits = cy.visits;
+        if (!rc) {
+            c->coarse_visits = cy.visits;
+            c->rb = cy.rb;
+        }
         if (!rc && !ustar[p]) rc = fail(c, MHDP_INVALID, "Missing candidate face velocities");
         PS& q = ps[(size_t)p];
         q.S.meanfree = o->gauge != MHDP_GAUGE_PINNED && c->components == 1;
@@ -1976,9 +2001,12 @@ int mhdp_probe_apply(mhdp_ctx* c, int meanfree, const float* p, float* q) {
 }
 
 int mhdp_probe_vcycle(mhdp_ctx* c, int meanfree, int precision, float smooth_weight, float coarse_scale,
-                      int coarse_visits, const float* r, float* z) {
-    if (!c || !r || !z || coarse_visits < 0 || coarse_visits > kMaxVisits) return MHDP_INVALID;
+                      int coarse_visits, int smoother, const float* r, float* z) {
+    if (!c || !r || !z || coarse_visits < 0 || coarse_visits > kMaxVisits || smoother < MHDP_SMOOTH_AUTO ||
+        smoother > MHDP_SMOOTH_RED_BLACK)
+        return MHDP_INVALID;
     c->coarse_visits = coarse_visits_for(c, coarse_visits);
+    c->rb = smoother == MHDP_SMOOTH_RED_BLACK;
     const bool f16 = precision == MHDP_MIXED;
     int rc = prepare(c, meanfree != 0);
     if (rc) return rc;

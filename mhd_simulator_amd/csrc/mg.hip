@@ -340,7 +340,8 @@ struct K_k_coarse {
 };
 
 template <class CT, class VT, class AT>
-static void coarse_visits_t(hipStream_t s, int L, LevelDev* lv, Scal* sc, int visits, void** z1out, bool f16);
+static void coarse_visits_t(hipStream_t s, int L, LevelDev* lv, Scal* sc, int visits, void** z1out, bool f16,
+                            bool rb);
 
 template <class CT, class VT, class AT>
 static void vcycle_t(hipStream_t s, int L, LevelDev* lv, const float* r0, Scal* sc, void** zfinal, bool f16,
@@ -380,7 +381,7 @@ static void vcycle_t(hipStream_t s, int L, LevelDev* lv, const float* r0, Scal* 
         go<K_k_restrict_fine<CT, VT>>(s, grid_blocks(c.g.n), {c.g, a.g, r0, C(a, 4), C(a, 0), C(a, 1), C(a, 2), (const VT*)a.z, (VT*)c.r, sc});
         T.after(3, s);
         void* z1 = nullptr;
-        coarse_visits_t<CT, VT, AT>(s, L, lv, sc, visits, &z1, f16);
+        coarse_visits_t<CT, VT, AT>(s, L, lv, sc, visits, &z1, f16, false);
         go<K_k_prolong<CT, VT>>(s, grid_blocks(a.g.n), {a.g, c.g, 1, (const VT*)z1, C(a, 4), (VT*)a.z, sc});
         T.before(2, s);
         go<K_k_smooth<CT, float, VT, true>>(s, grid_blocks(a.g.n), {a.g, r0, C(a, 4), C(a, 0), C(a, 1), C(a, 2), (const CT*)nullptr, (const VT*)a.z, (VT*)a.t, sc});
@@ -471,8 +472,17 @@ struct K_k_add {
 // Levels 1..L-1 of the V-cycle (L >= 2) from the level-1 residual in lv[1].r.  acc (brick level 1
 // and L >= 3 only): the level-1 smoother writes acc + theta * (its result) into acc instead of
 // lv[1].t, and acc is the output.
+// How a coarse level (not the coarsest) is smoothed: red-black passes, 4x2x2 Jacobi bricks, or the
+// one-cell Jacobi kernels.  The first two can add the level's result into an accumulator.
+enum LevelKind { LK_RB, LK_BRICK, LK_CELL };
+static LevelKind level_kind(const LevelDev& l, bool rb) {
+    if (rb && rb_level_ok(l.g)) return LK_RB;
+    if (brick_enabled_env() && brick_ok(l.g)) return LK_BRICK;
+    return LK_CELL;
+}
+
 template <class CT, class VT, class AT>
-static void vcycle_coarse_t(hipStream_t s, int L, LevelDev* lv, Scal* sc, void** z1out, bool f16,
+static void vcycle_coarse_t(hipStream_t s, int L, LevelDev* lv, Scal* sc, void** z1out, bool f16, bool rb,
                             void* acc = nullptr, float theta = 0.0f) {
     auto C = [&](LevelDev& l, int which) -> const CT* {
         void* const a16[5] = {l.bx16, l.by16, l.bz16, l.react16, l.dinv16};
@@ -481,12 +491,16 @@ static void vcycle_coarse_t(hipStream_t s, int L, LevelDev* lv, Scal* sc, void**
     };
     const dim3 B(MHDP_BLOCK);
     const int store = f16 ? STORE_BF16 : STORE_F32;
-    // levels whose dims allow bricks run the fused restrict / smooth passes
-    auto brick = [&](const LevelDev& l) { return brick_enabled_env() && brick_ok(l.g); };
     for (int l = 1; l < L - 1; l++) {
         LevelDev& a = lv[l];
         LevelDev& c = lv[l + 1];
-        if (brick(a)) {
+        const LevelKind kind = level_kind(a, rb);
+        if (kind == LK_RB) {
+            launch_rb_restrict(s, store, true, c.g, a.g, a.r, a.r, nullptr, nullptr, nullptr, C(a, 4), C(a, 0),
+                               C(a, 1), C(a, 2), c.r, nullptr, sc);
+            continue;
+        }
+        if (kind == LK_BRICK) {
             launch_brick_restrict_coarse(s, store, c.g, a.g, a.r, C(a, 4), C(a, 0), C(a, 1), C(a, 2), C(a, 3), c.r, sc);
             continue;
         }
@@ -501,7 +515,14 @@ static void vcycle_coarse_t(hipStream_t s, int L, LevelDev* lv, Scal* sc, void**
         LevelDev& a = lv[l];
         LevelDev& c = lv[l + 1];
         const VT* zc = (const VT*)(l + 1 == L - 1 ? c.z : c.t);
-        if (brick(a)) {
+        const LevelKind kind = level_kind(a, rb);
+        if (kind == LK_RB) {
+            void* const into = (l == 1 && acc) ? acc : a.t;
+            launch_rb_smooth(s, store, true, a.g, c.g, a.r, a.r, C(a, 4), C(a, 0), C(a, 1), C(a, 2), zc, into,
+                             l == 1 ? acc : nullptr, theta, nullptr, sc, 0, false);
+            continue;
+        }
+        if (kind == LK_BRICK) {
             void* const into = (l == 1 && acc) ? acc : a.t;
             launch_brick_smooth_coarse(s, store, a.g, c.g, a.r, C(a, 4), C(a, 0), C(a, 1), C(a, 2), C(a, 3), zc, into,
                                        sc, l == 1 ? acc : nullptr, theta);
@@ -521,14 +542,16 @@ static void vcycle_coarse_t(hipStream_t s, int L, LevelDev* lv, Scal* sc, void**
 // the over-corrected cycle (coarse scale 1.6) reaches about 1.75 on the droplet and plate grids.
 constexpr float kSecondVisit = 0.8f;
 template <class CT, class VT, class AT>
-static void coarse_visits_t(hipStream_t s, int L, LevelDev* lv, Scal* sc, int visits, void** z1out, bool f16) {
-    vcycle_coarse_t<CT, VT, AT>(s, L, lv, sc, z1out, f16);
+static void coarse_visits_t(hipStream_t s, int L, LevelDev* lv, Scal* sc, int visits, void** z1out, bool f16,
+                            bool rb) {
+    vcycle_coarse_t<CT, VT, AT>(s, L, lv, sc, z1out, f16, rb);
     if (visits < 2 || L < 3 || !lv[1].w) return;
     LevelDev& a = lv[1];
     const CT* B[4] = {(const CT*)(f16 ? a.bx16 : (void*)a.bx), (const CT*)(f16 ? a.by16 : (void*)a.by),
                       (const CT*)(f16 ? a.bz16 : (void*)a.bz), (const CT*)(f16 ? a.react16 : (void*)a.react)};
     const int store = f16 ? STORE_BF16 : STORE_F32;
     const bool brick = brick_enabled_env() && brick_ok(a.g);
+    const bool sums = level_kind(a, rb) != LK_CELL;  // the level-1 smoother adds into the sum itself
     void* const sum = a.t;  // L >= 3: the first visit's correction is in lv[1].t
     const void* last = sum;
     for (int v = 1; v < visits; v++) {
@@ -537,12 +560,12 @@ static void coarse_visits_t(hipStream_t s, int L, LevelDev* lv, Scal* sc, int vi
         else
             go<K_k_coarse_residual<CT, VT>>(s, grid_blocks(a.g.n), {a.g, B[0], B[1], B[2], B[3], (const VT*)last, (VT*)a.r, sc});
         void* zv = nullptr;
-        if (brick && v == visits - 1) {  // the last visit's level-1 smoother adds into the sum itself
-            vcycle_coarse_t<CT, VT, AT>(s, L, lv, sc, &zv, f16, sum, kSecondVisit);
+        if (sums && v == visits - 1) {  // the last visit's level-1 smoother adds into the sum itself
+            vcycle_coarse_t<CT, VT, AT>(s, L, lv, sc, &zv, f16, rb, sum, kSecondVisit);
             break;
         }
         a.t = a.w;
-        vcycle_coarse_t<CT, VT, AT>(s, L, lv, sc, &zv, f16);
+        vcycle_coarse_t<CT, VT, AT>(s, L, lv, sc, &zv, f16, rb);
         a.t = sum;
         go<K_k_add<VT>>(s, grid_blocks(a.g.n), {a.g.n, kSecondVisit, (const VT*)zv, (VT*)sum, sc});
         last = zv;
@@ -551,13 +574,13 @@ static void coarse_visits_t(hipStream_t s, int L, LevelDev* lv, Scal* sc, int vi
 }
 
 void launch_vcycle_coarse(hipStream_t s, int store, int L, LevelDev* lv, Scal* sc, int coarse_f64, int visits,
-                          void** z1out) {
+                          void** z1out, bool rb) {
     if (store == STORE_BF16) {
-        if (coarse_f64) coarse_visits_t<__hip_bfloat16, __hip_bfloat16, double>(s, L, lv, sc, visits, z1out, true);
-        else coarse_visits_t<__hip_bfloat16, __hip_bfloat16, float>(s, L, lv, sc, visits, z1out, true);
+        if (coarse_f64) coarse_visits_t<__hip_bfloat16, __hip_bfloat16, double>(s, L, lv, sc, visits, z1out, true, rb);
+        else coarse_visits_t<__hip_bfloat16, __hip_bfloat16, float>(s, L, lv, sc, visits, z1out, true, rb);
     } else {
-        if (coarse_f64) coarse_visits_t<float, float, double>(s, L, lv, sc, visits, z1out, false);
-        else coarse_visits_t<float, float, float>(s, L, lv, sc, visits, z1out, false);
+        if (coarse_f64) coarse_visits_t<float, float, double>(s, L, lv, sc, visits, z1out, false, rb);
+        else coarse_visits_t<float, float, float>(s, L, lv, sc, visits, z1out, false, rb);
     }
 }
 

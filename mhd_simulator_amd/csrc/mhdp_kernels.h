@@ -99,8 +99,9 @@ void launch_vcycle(hipStream_t s, int store, int L, LevelDev* lv, const float* r
 // returns the level-1 correction to prolong (lv[1].z if level 1 is the coarsest, else lv[1].t).
 // visits: how many times the fine level visits level 1 (1: V-cycle; 2: W-cycle at level 1, the
 // second visit solving for the residual the first leaves; needs lv[1].w and L >= 3)
+// rb: red-black Gauss-Seidel smoothing on every level rb_level_ok admits (weighted Jacobi elsewhere)
 void launch_vcycle_coarse(hipStream_t s, int store, int L, LevelDev* lv, Scal* sc, int coarse_f64, int visits,
-                          void** z1out);
+                          void** z1out, bool rb = false);
 
 // fused.hip — the fused fine-grid MG-PCG passes
 bool fused_vec4(const Grid3& g);
@@ -156,6 +157,17 @@ void launch_march_smooth(hipStream_t s, const Grid3& g, const Grid3& cg, const f
 void launch_march_restrict_update(hipStream_t s, const Grid3& cg, const Grid3& g, float* r, float* r2,
                                   const float* q, const void* dinv, const void* bx, const void* by, const void* bz,
                                   void* rc, double* part, Scal* sc);
+
+// rbgs.hip — red-black Gauss-Seidel passes of the multigrid cycle (any level; fine: mean-free
+// deferred mean and, for the restriction, the PCG residual update from r/r2 (Scal.rsel) into the
+// other buffer when q is given).  rb_level_ok: every periodic axis of the level has even length.
+bool rb_level_ok(const Grid3& g);
+void launch_rb_restrict(hipStream_t s, int store, bool coarse, const Grid3& cg, const Grid3& g, const void* r,
+                        const void* r2, float* w0, float* w1, const float* q, const void* dinv, const void* bx,
+                        const void* by, const void* bz, void* rc, double* part, Scal* sc);
+void launch_rb_smooth(hipStream_t s, int store, bool coarse, const Grid3& g, const Grid3& cg, const void* r,
+                      const void* r2, const void* dinv, const void* bx, const void* by, const void* bz, const void* zc,
+                      void* t, const void* base, float theta, double* part, Scal* sc, int mode, bool use_next);
 
 // setup.hip — warm start (pressure_warm / pressure_commit)
 void launch_warm(hipStream_t s, const Grid3& g, float h, const uint32_t* mask, const double* prev, const float* bx,

@@ -68,7 +68,7 @@ def hierarchy(dims4):
 
 def project_batch(projections, u_stars, *, precision="mixed", precond="auto", gauge="auto", refine_tol=1e-8,
                   max_outer=0, inner_tol=0.0, smooth_weight=0.0, coarse_scale=0.0, coarse_visits=0, u_outs=None,
-                  phi_outs=None):
+                  phi_outs=None, smoother="auto"):
     """Many independent projections in one call (mhdp_project_batch): device tensors only.
     The options of the first projection's FluidConfig apply to all.  Returns a list of
     (iterations, relative residual); raises PressureError for the first failing problem."""
@@ -82,7 +82,7 @@ def project_batch(projections, u_stars, *, precision="mixed", precond="auto", ga
             raise ValueError("u_star must be a device tensor holding the three face planes")
     lib = projections[0].lib
     o = projections[0]._opts(precision, precond, gauge, refine_tol, max_outer, inner_tol, smooth_weight, coarse_scale,
-                             coarse_visits=coarse_visits)
+                             coarse_visits=coarse_visits, smoother=smoother)
     P = C.c_void_p * n
     ctxs = P(*[pr.handle for pr in projections])
     us = P(*[u.data_ptr() for u in u_stars])
@@ -149,12 +149,12 @@ class PressureProjection:
         return int(self.lib.mhdp_device_bytes(self.handle))
 
     def _opts(self, precision, precond, gauge, refine_tol, max_outer, inner_tol, smooth_weight=0.0,
-              coarse_scale=0.0, warm_start=False, coarse_visits=0):
+              coarse_scale=0.0, warm_start=False, coarse_visits=0, smoother="auto"):
         c = self.config
         return _lib.Opts(c.relative_tolerance, c.absolute_divergence_tolerance,
                          c.max_iterations, _lib.PRECOND[precond], _lib.PRECISION[precision], _lib.GAUGE[gauge],
                          refine_tol, max_outer, inner_tol, smooth_weight, coarse_scale, int(bool(warm_start)),
-                         int(coarse_visits))
+                         int(coarse_visits), _lib.SMOOTHER[smoother])
 
     def set_potential(self, phi):
         """The previous potential a warm start begins from (numpy float64 or a device tensor)."""
@@ -171,7 +171,7 @@ class PressureProjection:
     # -- FluidState::project(false)
     def project(self, u_star, *, precision="fp32", precond="auto", gauge="auto", refine_tol=1e-8,
                 max_outer=0, inner_tol=0.0, smooth_weight=0.0, coarse_scale=0.0, warm_start=False, u_out=None,
-                phi_out=None, coarse_visits=0):
+                phi_out=None, coarse_visits=0, smoother="auto"):
         """Project u_star; returns (iterations, relative residual) like the reference.
         warm_start: FluidState::project(true) - begin from the previous potential.
 
@@ -179,7 +179,7 @@ class PressureProjection:
         torch CUDA input: device path; ``u_out`` (float32, faces) and ``phi_out``
         (float64, cells) are device tensors the caller provides or None."""
         o = self._opts(precision, precond, gauge, refine_tol, max_outer, inner_tol, smooth_weight, coarse_scale,
-                       warm_start, coarse_visits)
+                       warm_start, coarse_visits, smoother)
         st = _lib.Stats()
         if hasattr(u_star, "data_ptr"):
             if u_star.numel() != self.grid.faces:
@@ -268,13 +268,14 @@ class PressureProjection:
         return q
 
     def vcycle(self, r: np.ndarray, meanfree: bool, precision="fp32", smooth_weight=2.0 / 3.0,
-               coarse_scale=1.0, coarse_visits=1):
+               coarse_scale=1.0, coarse_visits=1, smoother="jacobi"):
         """One multigrid cycle; the defaults reproduce the reference's V-cycle (coarse_visits=2:
         the W-cycle at level 1 the solver uses on grids of 2^24 cells and more)."""
         r = np.ascontiguousarray(r, dtype=np.float32)
         z = np.zeros(self.grid.cells, dtype=np.float32)
         self._check(self.lib.mhdp_probe_vcycle(self.handle, int(meanfree), _lib.PRECISION[precision],
-                                               smooth_weight, coarse_scale, int(coarse_visits), _ptr(r), _ptr(z)))
+                                               smooth_weight, coarse_scale, int(coarse_visits),
+                                               _lib.SMOOTHER[smoother], _ptr(r), _ptr(z)))
         return z
 
     def mg_graph(self, level: int, meanfree: bool):

@@ -115,7 +115,7 @@ def test_breakdown_is_detected_and_solved_with_reference_cycle(orc, precision, n
         g, fluid, alpha = C.mg_fixture((16, 12, 8), (False,) * 3, True, True)
         p = C.Problem("walls-pinned", g, fluid, alpha, C.seeded_faces(g, 11))
     with make(p) as pr:
-        ref_cycle = dict(smooth_weight=2.0 / 3.0, coarse_scale=1.0, coarse_visits=1)
+        ref_cycle = dict(smooth_weight=2.0 / 3.0, coarse_scale=1.0, coarse_visits=1, smoother="jacobi")
         pr.project(p.ustar, precision=precision, precond="mg", **ref_cycle)
         assert pr.stats["cycle_fallback"] == 0
         u_ref, phi_ref, it_ref = pr.velocity.copy(), pr.phi.copy(), pr.stats["iterations"]
@@ -148,7 +148,7 @@ def test_batch_breakdown_is_solved_again_with_reference_cycle(orc):
             assert orc.true_residual(p.grid, pr.mask, p.alpha, p.ustar, po[k].cpu().numpy()) <= 1e-8
         one_u, one_phi = torch.zeros_like(uo[1]), torch.zeros_like(po[1])
         prs[1].project(us[1], precision="mixed", precond="mg", smooth_weight=2.0 / 3.0, coarse_scale=1.0,
-                       coarse_visits=1, u_out=one_u, phi_out=one_phi)
+                       coarse_visits=1, smoother="jacobi", u_out=one_u, phi_out=one_phi)
         torch.cuda.synchronize()
         assert torch.equal(one_u, uo[1]) and torch.equal(one_phi, po[1])
     finally:
