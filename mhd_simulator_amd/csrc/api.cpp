@@ -540,6 +540,19 @@ void set_solve_params(mhdp_ctx* c, Scal& s, bool meanfree, double tol2, double a
     s.anchor = c->anchor0;
 }
 
+// Iterations to issue before the next host read: as many as the residual reduction measured since
+// (rr_prev, it_prev) predicts are left to reach the stopping bound, at least 1 and at most 8.  A host
+// read costs about as much as a few launches that would only find the solve converged.
+uint32_t next_batch(double rr_prev, uint32_t it_prev, const Scal& v) {
+    const double target = (v.tol2 * v.bb + v.abs2) / std::max(v.bound, 1e-300);
+    const uint32_t it = (uint32_t)v.iters;
+    if (it <= it_prev || !(rr_prev > 0.0) || !(v.rr > 0.0) || !(target > 0.0) || v.rr >= rr_prev) return 8u;
+    const double per = std::log(v.rr / rr_prev) / (double)(it - it_prev);  // log reduction per iteration
+    const double left = std::log(target / v.rr) / per;
+    if (!(left > 1.0)) return 1u;
+    return (uint32_t)std::min(8.0, std::ceil(left));
+}
+
 // The PCG loop of FluidState::project (mod.rs:1015-1104) on the FP32 vectors x (from 0),
 // right-hand side c->b.  `reference_checks`: refresh on convergence and the independent
 // physical check every 64 iterations, as the reference; the inner solves of the mixed
@@ -565,14 +578,22 @@ int pcg(mhdp_ctx* c, Solve& S, bool want_mg, uint32_t max_iter, bool reference_c
         launch_fill_f32(c->stream, c->g.n, c->p2, 0.0f);
     }
     if (!conv && S.mg) mg_initialize(c, S);
+    // multigrid: batches of 8, then as many as the measured convergence rate predicts are left
+    uint32_t want = S.mg ? 8u : 32u, it_prev = 0;
+    double rr_prev = c->sc_h->rr;
     while (!conv && it < max_iter) {
-        uint32_t batch = S.mg ? 8u : 32u;
+        uint32_t batch = want;
         batch = std::min(batch, max_iter - it);
         batch = std::min(batch, 64u - it % 64u);
         for (uint32_t k = 0; k < batch; k++) iterate_once(c, S);
         if ((rc = convergence(c, residual, &conv, &seen))) return rc;
         if (seen <= it && !conv) return fail(c, MHDP_NO_PROGRESS, "Fluid pressure iteration made no progress");
         it = seen;
+        if (S.mg && !conv) {
+            want = next_batch(rr_prev, it_prev, *c->sc_h);
+            rr_prev = c->sc_h->rr;
+            it_prev = it;
+        }
         if (conv && reference_checks) {
             flush_x(c, S);
             refresh_stages(c, S);
@@ -600,44 +621,59 @@ int upload_scal(mhdp_ctx* c, const Scal& s) {
 
 // FP64 iterative refinement over the fused multigrid path.  Each outer step is one residual
 // pass that also starts the inner solve on the device (inner tolerance, deferred mean-free
-// projection, zeroed x and search directions), the inner V-cycle PCG in batches of 8, and one
-// pass folding the inner solution (with its owed last update) into x64.  The host reads the
-// scalars once per batch; the first read of a step carries the FP64 residual.
+// projection, zeroed x and search directions), the inner V-cycle PCG in batches (8, then as
+// many as the measured convergence rate predicts are left), and one pass folding the inner
+// solution (with its owed last update) into x64.  The host reads the scalars after each batch,
+// and after each later step's residual pass: a step whose FP64 residual already meets the
+// target launches nothing more.
 int refine_fused(mhdp_ctx* c, Solve& S, const mhdp_opts* o, uint32_t max_outer, bool warm, bool started,
                  uint32_t* iterations, uint32_t* outer, double* res64) {
     hipStream_t s = c->stream;
     *iterations = 0;
     *outer = 0;
+    auto finite = [](const Scal& v) {
+        const double vals[10] = {v.rr64, v.bb64, v.rz, v.rr, v.bb, v.sr, v.alpha, v.beta, v.conv, v.iters};
+        for (double x : vals)
+            if (!std::isfinite(x)) return false;
+        return true;
+    };
     for (bool first = true;; first = false) {
-        if (!(first && started))  // `started`: the caller has run the first step's residual pass
+        // `started`: the caller has run the first step's residual pass and read its scalars
+        if (!(first && started)) {
             launch_outer_start(s, c->g, c->x64, c->b64, c->bx, c->by, c->bz, c->dinv, c->r, c->x, c->p, c->p2,
                                c->part, c->sc, first && !warm);
+            int rc = read_scalars(c);
+            if (rc) return rc;
+        }
+        {
+            const Scal& v = *c->sc_h;
+            if (!finite(v)) return fail(c, MHDP_NONFINITE, "Nonfinite fluid pressure residual");
+            *res64 = v.bb64 > 0.0 ? std::sqrt(v.rr64 / v.bb64) : 0.0;
+            if (v.outer_done) {
+                if (first && !warm) launch_fill_f64(s, c->g.n, c->x64, 0.0);
+                return MHDP_OK;
+            }
+            if (*outer >= max_outer) return MHDP_NOT_CONVERGED;
+        }
         mg_initialize(c, S);
-        uint32_t it = 0;
-        for (bool checked = false;;) {
-            const uint32_t batch = std::min(8u, o->max_iter - it);
+        uint32_t it = 0, it_prev = 0, batch = 8u;
+        double rr_prev = c->sc_h->rr;  // the inner solve's initial |r|^2
+        for (;;) {
+            batch = std::min(batch, o->max_iter - it);
             for (uint32_t k = 0; k < batch; k++) iterate_once(c, S);
             int rc = read_scalars(c);
             if (rc) return rc;
             const Scal& v = *c->sc_h;
             if (v.fault) return kBreakdown;
-            const double vals[10] = {v.rr64, v.bb64, v.rz, v.rr, v.bb, v.sr, v.alpha, v.beta, v.conv, v.iters};
-            for (double x : vals)
-                if (!std::isfinite(x)) return fail(c, MHDP_NONFINITE, "Nonfinite fluid pressure residual");
-            if (!checked) {
-                *res64 = v.bb64 > 0.0 ? std::sqrt(v.rr64 / v.bb64) : 0.0;
-                if (v.outer_done) {
-                    if (first && !warm) launch_fill_f64(s, c->g.n, c->x64, 0.0);
-                    return MHDP_OK;
-                }
-                if (*outer >= max_outer) return MHDP_NOT_CONVERGED;
-                checked = true;
-            }
+            if (!finite(v)) return fail(c, MHDP_NONFINITE, "Nonfinite fluid pressure residual");
             const bool conv = v.conv > 0.5;
             const uint32_t seen = (uint32_t)v.iters;
             if (seen <= it && !conv) return fail(c, MHDP_NO_PROGRESS, "Fluid pressure iteration made no progress");
             it = seen;
             if (conv || it >= o->max_iter) break;
+            batch = next_batch(rr_prev, it_prev, v);
+            rr_prev = v.rr;
+            it_prev = it;
         }
         *iterations += it;
         launch_outer_update(s, c->g, c->x64, c->x, c->p, c->p2, c->dinv, c->sc, S.meanfree, first && !warm);
@@ -730,6 +766,7 @@ int project_once(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, const Cycl
     uint32_t iterations = 0, outer = 0;
     float residual = 0.0f;
     double res64 = 0.0;
+    bool phi_copied = false;  // the velocity correction also wrote the potential to `phi`
     if (!mixed) {
         // Params.solve (mod.rs:398-404, 978-1002) in FP32 as the reference computes it
         const float tol = o->rel_tol, ab = o->abs_div_tol, h = c->grid.h;
@@ -824,11 +861,13 @@ int project_once(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, const Cycl
             return fail(c, MHDP_NOT_CONVERGED, buf);
         }
         residual = (float)res64;
-        if (uout) launch_correct_f64(c->stream, c->g, c->grid.h, c->bx, c->by, c->bz, c->x64, ustar, uout);
+        if (uout) phi_copied = launch_correct_f64(c->stream, c->g, c->grid.h, c->bx, c->by, c->bz, c->x64, ustar,
+                                                  uout, phi);
     }
     // the potential (plus the previous one after an FP32 warm start) is returned and kept
     if (warm && !mixed) launch_commit(c->stream, c->g.n, c->mask, c->phi_prev, c->x64);
-    if (phi) HIP_OK(hipMemcpyAsync(phi, c->x64, (size_t)c->g.n * 8, hipMemcpyDeviceToDevice, c->stream));
+    if (phi && !phi_copied)
+        HIP_OK(hipMemcpyAsync(phi, c->x64, (size_t)c->g.n * 8, hipMemcpyDeviceToDevice, c->stream));
     std::swap(c->x64, c->phi_prev);
     c->prev_valid = true;
     HIP_OK(hipEventRecord(c->ev1, c->stream));
@@ -1258,17 +1297,19 @@ int project_lockstep(mhdp_ctx* const* ctxs, int count, const float* const* ustar
     // outputs of the problems that reached the target
     {
         auto fin = members([](const PS& q) { return q.done; });
+        std::vector<char> copied((size_t)n, 0);  // the correction pass also wrote the potential
         Plan plan;
         if (!record_plan(fin, mem, plan, [&](int p) {
                 mhdp_ctx* c = ctxs[p];
                 if (ps[(size_t)p].outer == 0) launch_fill_f64(s, c->g.n, c->x64, 0.0);
                 if (uout && uout[p])
-                    launch_correct_f64(s, c->g, c->grid.h, c->bx, c->by, c->bz, c->x64, ustar[p], uout[p]);
+                    copied[(size_t)p] = launch_correct_f64(s, c->g, c->grid.h, c->bx, c->by, c->bz, c->x64, ustar[p],
+                                                           uout[p], phi ? phi[p] : nullptr);
             }))
             return MHDP_NO_MEMORY;
         issue_plan(s, plan);
         for (int p : fin)
-            if (phi && phi[p] &&
+            if (phi && phi[p] && !copied[(size_t)p] &&
                 hipMemcpyAsync(phi[p], ctxs[p]->x64, (size_t)ctxs[p]->g.n * 8, hipMemcpyDeviceToDevice, s) != hipSuccess)
                 return MHDP_DEVICE;
     }

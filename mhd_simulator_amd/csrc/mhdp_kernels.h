@@ -33,8 +33,10 @@ void launch_rhs_f64(hipStream_t s, const Grid3& g, float h, const uint32_t* mask
                     const float* bz, const float* ustar, double* b);
 void launch_correct_f32(hipStream_t s, const Grid3& g, float h, const float* bx, const float* by, const float* bz,
                         const float* phi, const float* ustar, float* uout);
-void launch_correct_f64(hipStream_t s, const Grid3& g, float h, const float* bx, const float* by, const float* bz,
-                        const double* phi, const float* ustar, float* uout);
+// also copies phi to phi_copy (when set) in the same pass; returns whether it did
+bool launch_correct_f64(hipStream_t s, const Grid3& g, float h, const float* bx, const float* by, const float* bz,
+                        const double* phi, const float* ustar, float* uout,
+                        double* phi_copy = nullptr);
 void launch_mg_build(hipStream_t s, const Grid3& cg, const Grid3& fg, bool fine, int meanfree, const uint32_t* mask,
                      const float* fdinv, const float* fbx, const float* fby, const float* fbz, const float* freact,
                      float* cbx, float* cby, float* cbz, float* creact, float* cdiag, float* cdinv);

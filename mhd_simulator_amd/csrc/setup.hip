@@ -624,6 +624,7 @@ struct K_k_correct4 {
         float* uout;
         uint32_t f0;
         uint32_t f1;
+        PT* phi_copy;  // when set: the potential is copied here in the same pass
     };
     static __device__ void run(const Args& A_, const VGrid& vg) {
         Grid3 g = A_.g;
@@ -650,6 +651,10 @@ struct K_k_correct4 {
             const Row4 r = row4(g, t);
             PT pc[4], py[4], pz[4];
             ld4p(phi, r.i0, pc);
+            if (A_.phi_copy) {
+    #pragma unroll
+                for (int k = 0; k < 4; k++) A_.phi_copy[r.i0 + k] = pc[k];
+            }
             ld4p(phi, r.ym, py);
             ld4p(phi, r.zm, pz);
             const PT px = phi[r.xm];
@@ -1082,18 +1087,20 @@ void launch_correct_f32(hipStream_t s, const Grid3& g, float h, const float* bx,
     uint32_t f0, f1, f2;
     face_counts(g, f0, f1, f2);
     if (row_vector_ok(g))
-        go<K_k_correct4<float>>(s, grid_blocks(g.n >> 2), {g, h, bx, by, bz, phi, ustar, uout, f0, f1});
+        go<K_k_correct4<float>>(s, grid_blocks(g.n >> 2), {g, h, bx, by, bz, phi, ustar, uout, f0, f1, nullptr});
     else
         go<K_k_correct<float>>(s, grid_blocks(f0 + f1 + f2), {g, h, bx, by, bz, phi, ustar, uout, f0, f1, f2});
 }
-void launch_correct_f64(hipStream_t s, const Grid3& g, float h, const float* bx, const float* by, const float* bz,
-                        const double* phi, const float* ustar, float* uout) {
+bool launch_correct_f64(hipStream_t s, const Grid3& g, float h, const float* bx, const float* by, const float* bz,
+                        const double* phi, const float* ustar, float* uout, double* phi_copy) {
     uint32_t f0, f1, f2;
     face_counts(g, f0, f1, f2);
-    if (row_vector_ok(g))
-        go<K_k_correct4<double>>(s, grid_blocks(g.n >> 2), {g, h, bx, by, bz, phi, ustar, uout, f0, f1});
-    else
-        go<K_k_correct<double>>(s, grid_blocks(f0 + f1 + f2), {g, h, bx, by, bz, phi, ustar, uout, f0, f1, f2});
+    if (row_vector_ok(g)) {
+        go<K_k_correct4<double>>(s, grid_blocks(g.n >> 2), {g, h, bx, by, bz, phi, ustar, uout, f0, f1, phi_copy});
+        return phi_copy != nullptr;
+    }
+    go<K_k_correct<double>>(s, grid_blocks(f0 + f1 + f2), {g, h, bx, by, bz, phi, ustar, uout, f0, f1, f2});
+    return false;
 }
 void launch_mg_build(hipStream_t s, const Grid3& cg, const Grid3& fg, bool fine, int meanfree, const uint32_t* mask,
                      const float* fdinv, const float* fbx, const float* fby, const float* fbz, const float* freact,
