@@ -234,8 +234,8 @@ int mhdp_surface_stress_dev(mhdp_ctx *ctx, float *u_dev, const float *cells_dev,
  * EmState::advance_to, src/em/mod.rs:863-1010; coefficient set-up of EmState::new,
  * mod.rs:557-571).  Edges of axis a have dims n + 1 except n_a on axis a, packed x|y|z, one f32
  * each; the unknown is the magnetic vector potential A (tangential A held on the outer surface).
- * Not included: the nodal auxiliary-space preconditioner (plain Jacobi throughout; the reference
- * switches it on only after 1024 iterations). */
+ * Long induction solves switch to the nodal auxiliary-space correction (src/em/nodal.wgsl; the
+ * switch of mod.rs:887-960): see mhdp_em_opts.nodal. */
 typedef struct mhdp_em mhdp_em;
 typedef struct {
     float dt;          /* EmConfig.dt_s (Params.physical.x) */
@@ -243,6 +243,15 @@ typedef struct {
     float rel_tol;     /* EmConfig.relative_tolerance */
     uint32_t max_iter; /* EmConfig.max_iterations */
     int magnetostatic; /* Params.control.y = 1 (the initial solve): no sigma/dt mass term */
+    int nodal;         /* the nodal auxiliary-space correction z = r/d + G S^-1 G^T r (G the node-to-edge
+                          gradient, S = diag(G^T A G)): 0 the reference's policy (an induction solve,
+                          not magnetostatic, with max_iter > 1024 switches after 1024 iterations at a
+                          true-residual restart; the target and the iteration count carry over),
+                          -1 never (plain Jacobi), 1 switch after nodal_after iterations instead (0:
+                          from the start; the reference's nodal_switch_after test hook).  The switch
+                          is checked between batches of 32 iterations.  It needs no device memory
+                          beyond the operator's own (the node values reuse its per-node scratch) */
+    uint32_t nodal_after;
 } mhdp_em_opts;
 typedef struct {
     uint32_t iterations;
@@ -251,6 +260,7 @@ typedef struct {
     float prediction;            /* the prediction factor used (0: zero start; prediction_diagnostic[0]) */
     double start_residual;       /* sqrt(rr / bb) of the first start (prediction_diagnostic[1]) */
     uint32_t prediction_dropped; /* 1: the predicted start was rejected (prediction_diagnostic[2]) */
+    uint32_t nodal;              /* 1: the solve ran or switched to the nodal correction */
 } mhdp_em_stats;
 /* A context for a grid of cells (periodic must be 0; h = GridSi.cell_m). */
 int mhdp_em_create(mhdp_em **out, const mhdp_grid *grid);
@@ -283,6 +293,10 @@ int mhdp_em_solve_predicted_dev(mhdp_em *em, const mhdp_em_opts *opts, const flo
  * host vector. */
 int mhdp_em_get_coefficients(const mhdp_em *em, float *sigma, float *diag, uint8_t *flags, float *nu);
 int mhdp_em_probe_apply(mhdp_em *em, const mhdp_em_opts *opts, const float *p, float *q);
+/* Test hook: the nodal correction applied to a host vector r (nodal_build_initial + nodal_initial,
+ * src/em/nodal.wgsl:24-35): z (one per edge) and, if nodal is not NULL, the node values
+ * ((nx+1)(ny+1)(nz+1), x fastest). */
+int mhdp_em_probe_nodal(mhdp_em *em, const mhdp_em_opts *opts, const float *r, float *z, float *nodal);
 uint64_t mhdp_em_edges(const mhdp_em *em);
 /* Live kernel timing of the CG iterations (measurement hook, no counterpart in the reference):
  * while enabled, HIP events bracket every launch of `kernel` during mhdp_em_solve_dev;

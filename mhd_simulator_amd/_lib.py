@@ -17,7 +17,7 @@ SYMBOLS = (
     "mhdp_set_potential",
     "mhdp_project_batch", "mhdp_fields_dev", "mhdp_predict_dev", "mhdp_viscous_dev", "mhdp_surface_stress_dev",
     "mhdp_em_create", "mhdp_em_set_materials", "mhdp_em_solve_dev", "mhdp_em_solve_predicted_dev",
-    "mhdp_em_get_coefficients", "mhdp_em_probe_apply",
+    "mhdp_em_get_coefficients", "mhdp_em_probe_apply", "mhdp_em_probe_nodal",
     "mhdp_em_edges", "mhdp_em_set_kernel_timing", "mhdp_em_kernel_time", "mhdp_em_last_error", "mhdp_em_destroy",
     "mhdp_last_error", "mhdp_destroy", "mhdp_device_bytes",
     "mhdp_anchors", "mhdp_hierarchy", "mhdp_set_kernel_timing", "mhdp_kernel_time", "mhdp_get_launch_tuning",
@@ -87,7 +87,7 @@ class StressParams(C.Structure):
 
 class EmOpts(C.Structure):
     _fields_ = [("dt", C.c_float), ("mu0", C.c_float), ("rel_tol", C.c_float), ("max_iter", C.c_uint32),
-                ("magnetostatic", C.c_int)]
+                ("magnetostatic", C.c_int), ("nodal", C.c_int), ("nodal_after", C.c_uint32)]
 
 
 class LaunchTuning(C.Structure):
@@ -100,7 +100,8 @@ class LaunchTuning(C.Structure):
 
 class EmStats(C.Structure):
     _fields_ = [("iterations", C.c_uint32), ("rel_residual", C.c_double), ("seconds", C.c_double),
-                ("prediction", C.c_float), ("start_residual", C.c_double), ("prediction_dropped", C.c_uint32)]
+                ("prediction", C.c_float), ("start_residual", C.c_double), ("prediction_dropped", C.c_uint32),
+                ("nodal", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -145,6 +146,7 @@ def load(path: str = LIB_PATH):
         "mhdp_em_solve_predicted_dev": (I, [P, C.POINTER(EmOpts), P, P, P, P, P, C.c_float, P, C.POINTER(EmStats)]),
         "mhdp_em_get_coefficients": (I, [P, P, P, P, P]),
         "mhdp_em_probe_apply": (I, [P, C.POINTER(EmOpts), P, P]),
+        "mhdp_em_probe_nodal": (I, [P, C.POINTER(EmOpts), P, P, P]),
         "mhdp_em_edges": (C.c_uint64, [P]),
         "mhdp_em_set_kernel_timing": (I, [P, I]),
         "mhdp_em_kernel_time": (I, [P, P, P]),

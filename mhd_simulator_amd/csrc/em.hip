@@ -1,6 +1,6 @@
 // The EM step's Jacobi-CG on the edge curl-curl operator (src/em/induction.wgsl; the loop of
-// EmState::advance_to, src/em/mod.rs:863-1010) with its increment prediction, without the nodal
-// auxiliary preconditioner (plain Jacobi, as the reference runs below its nodal switch).
+// EmState::advance_to, src/em/mod.rs:863-1010) with its increment prediction and the nodal
+// auxiliary-space correction it switches to on long induction solves (src/em/nodal.wgsl).
 // Built with -ffp-contract=off: the operator, right-hand side and coefficients keep the
 // reference's FP32 operation order and equal the oracle's bit for bit.  Reductions are FP64
 // per-block partials folded in block order by one block (deterministic).
@@ -183,7 +183,9 @@ __device__ __forceinline__ bool air_node(const EmGeo& g, const uint32_t* __restr
 }
 
 // one block's four FP64 partials, in a fixed order (wave shuffles, then the waves)
-template <int OP1, int OP2>  // 0 sum, 1 min, 2 max for components 1 and 2 (0 and 3 always sum)
+// KEEP: how many components the block writes (the nodal passes replace only r.z and r.r of the
+// records the pass before them wrote)
+template <int OP1, int OP2, int KEEP = 4>  // 0 sum, 1 min, 2 max for components 1 and 2 (0 and 3 always sum)
 __device__ void block_partials(double v[4], double* __restrict__ part, uint32_t b) {
     __shared__ double red[MHDP_BLOCK / 64][4];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -198,7 +200,7 @@ __device__ void block_partials(double v[4], double* __restrict__ part, uint32_t 
         if (lane == 0) red[wid][k] = x;
     }
     __syncthreads();
-    if (threadIdx.x < 4) {
+    if (threadIdx.x < KEEP) {
         const int k = threadIdx.x;
         const int op = k == 1 ? OP1 : (k == 2 ? OP2 : 0);
         double x = red[0][k];
@@ -607,7 +609,8 @@ struct K_em_update {
     }
 };
 
-// update_search (induction.wgsl:195): p = r / d + beta p
+// update_search (induction.wgsl:195): p = z + beta p, z = r / d (Jacobi) or the nodal
+// correction's z, which the nodal pass left in q
 struct K_em_search {
     static constexpr int MINB = 0;
     struct Args {
@@ -615,14 +618,112 @@ struct K_em_search {
         Phys ph;
         Vecs v;
         const EmScal* sc;
+        int nodal;
     };
     static __device__ void run(const Args& a, const VGrid& vg) {
         if (a.sc->status != 0) return;
         const float beta = (float)a.sc->beta;
         uint32_t lo, hi;
         block_range(vg, a.g.Etot, lo, hi);
-        for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK)
-            a.v.p[i] = a.v.r[i] / diagonal(a.g, a.ph, a.v.diag[i], a.v.sigma[i]) + beta * a.v.p[i];
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+            const float z = a.nodal ? a.v.q[i] : a.v.r[i] / diagonal(a.g, a.ph, a.v.diag[i], a.v.sigma[i]);
+            a.v.p[i] = z + beta * a.v.p[i];
+        }
+    }
+};
+
+// ---- nodal auxiliary-space correction (nodal.wgsl): z = r / d + G S^-1 G^T r, G the node-to-edge
+// incidence gradient, S = diag(G^T A G) on interior nodes in the reference's closed form (36 air(p)
+// plus, over the three axes, the mass of the two incident edges and the air flags of the two
+// neighbours).  The node values live in the operator's per-node scratch (ndiv: written by the
+// curl pass, dead after the apply), so the correction allocates nothing.
+
+// cached_air_node (nodal.wgsl:6): the lower-air flag of the node's x edge, 0 off the interior
+__device__ __forceinline__ bool interior_node(const EmGeo& g, const int p[3]) {
+    return p[0] > 0 && p[1] > 0 && p[2] > 0 && p[0] < g.n[0] && p[1] < g.n[1] && p[2] < g.n[2];
+}
+__device__ __forceinline__ float air_at(const EmGeo& g, const uint8_t* __restrict__ flags, const int p[3]) {
+    if (!interior_node(g, p)) return 0.0f;
+    return (flags[edge_index(g, 0, p)] & kLowerAir) ? 1.0f : 0.0f;
+}
+
+// nodal_build (nodal.wgsl:7-19): the node values N = S^-1 (r_out - r_in summed over the axes)
+struct K_em_nodal_build {
+    static constexpr int MINB = 0;
+    struct Args {
+        EmGeo g;
+        Phys ph;
+        const float* r;
+        const float* sigma;
+        const uint8_t* flags;
+        float* nodal;
+        const EmScal* sc;  // NULL: always (nodal_build_initial)
+    };
+    static __device__ void run(const Args& a, const VGrid& vg) {
+        if (a.sc && a.sc->status != 0) return;
+        const EmGeo g = a.g;
+        const uint32_t NX = (uint32_t)g.n[0] + 1u, NY = (uint32_t)g.n[1] + 1u;
+        const uint32_t nodes = NX * NY * ((uint32_t)g.n[2] + 1u);
+        uint32_t lo, hi;
+        block_range(vg, nodes, lo, hi);
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+            const int p[3] = {(int)(i % NX), (int)((i / NX) % NY), (int)(i / (NX * NY))};
+            float value = 0.0f;
+            if (interior_node(g, p)) {
+                float rhs = 0.0f, diag = 36.0f * air_at(g, a.flags, p);
+#pragma unroll
+                for (int ax = 0; ax < 3; ax++) {
+                    int lower[3] = {p[0], p[1], p[2]}, upper[3] = {p[0], p[1], p[2]};
+                    lower[ax] -= 1;
+                    upper[ax] += 1;
+                    const uint32_t out = edge_index(g, ax, p), inc = edge_index(g, ax, lower);
+                    rhs += a.r[out] - a.r[inc];
+                    diag += mass(g, a.ph, a.sigma[out]) + mass(g, a.ph, a.sigma[inc]) + air_at(g, a.flags, lower) +
+                            air_at(g, a.flags, upper);
+                }
+                value = rhs / fmaxf(diag, 1e-30f);
+            }
+            a.nodal[i] = value;
+        }
+    }
+};
+
+// nodal_initial / nodal_update (nodal.wgsl:20-41): z = r / d + N(lower node) - N(upper node) into p
+// (initial: the search starts at z) or q (the update's z for the next search); partials r.z, r.r
+// replace those of the pass before (same blocks, same edges)
+struct K_em_nodal_z {
+    static constexpr int MINB = 0;
+    struct Args {
+        EmGeo g;
+        Phys ph;
+        Vecs v;
+        const float* nodal;
+        const EmScal* sc;
+        double* part;
+        int initial;
+    };
+    static __device__ void run(const Args& a, const VGrid& vg) {
+        if (!a.initial && a.sc->status != 0) return;
+        const EmGeo g = a.g;
+        const uint32_t NX = (uint32_t)g.n[0] + 1u, NY = (uint32_t)g.n[1] + 1u;
+        float* __restrict__ out = a.initial ? a.v.p : a.v.q;
+        uint32_t lo, hi;
+        block_range(vg, g.Etot, lo, hi);
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+            int p[3];
+            const int axis = edge_decode(g, i, p);
+            int u[3] = {p[0], p[1], p[2]};
+            u[axis] += 1;
+            const uint32_t lower = (uint32_t)p[0] + NX * ((uint32_t)p[1] + NY * (uint32_t)p[2]);
+            const uint32_t upper = (uint32_t)u[0] + NX * ((uint32_t)u[1] + NY * (uint32_t)u[2]);
+            const float r = a.v.r[i];
+            const float z = r / diagonal(g, a.ph, a.v.diag[i], a.v.sigma[i]) + a.nodal[lower] - a.nodal[upper];
+            out[i] = z;
+            acc[0] += (double)r * (double)z;
+            acc[1] += (double)r * (double)r;
+        }
+        block_partials<0, 0, 2>(acc, a.part, vg.b);
     }
 };
 
@@ -817,18 +918,29 @@ void launch_em_begin(hipStream_t s, const EmDims& d, const uint8_t* flags, const
     go<K_em_begin>(s, em_grid(g.Etot), {g.Etot, flags, a_prev, a_before, prediction, x});
 }
 
+namespace {
+// nodal_build_initial + nodal_initial after a pass that wrote r (prepare, true_residual) and its
+// partials over em_grid(Etot) blocks: the search starts at the corrected z
+void nodal_start(hipStream_t s, const EmGeo& g, const Phys& ph, const Vecs& v, double* part, EmScal* sc) {
+    const uint32_t nodes = ((uint32_t)g.n[0] + 1u) * ((uint32_t)g.n[1] + 1u) * ((uint32_t)g.n[2] + 1u);
+    go<K_em_nodal_build>(s, em_grid(nodes), {g, ph, v.r, v.sigma, v.flags, v.ndiv, nullptr});
+    go<K_em_nodal_z>(s, em_grid(g.Etot), {g, ph, v, v.ndiv, sc, part, 1});
+}
+}  // namespace
+
 void launch_em_prepare(hipStream_t s, const EmDims& d, const EmPhys& ph, const EmVecs& v, double* part, void* scal,
-                       int guess) {
+                       int guess, bool nodal) {
     const EmGeo g = geo_of(d);
     const uint32_t blocks = em_grid(g.Etot);
     const Phys& p = ph;
     const Vecs& vv = v;
     go<K_em_prepare>(s, blocks, {g, p, vv, (EmScal*)scal, part, guess});
+    if (nodal) nodal_start(s, g, p, vv, part, (EmScal*)scal);
     go<K_em_fold>(s, 1, {part, blocks, kFoldInitial, (EmScal*)scal});
 }
 
 void launch_em_iterations(hipStream_t s, const EmDims& d, const EmPhys& ph, const EmVecs& v, double* part, void* scal,
-                          uint32_t count, KernelTimer* timer) {
+                          uint32_t count, bool nodal, KernelTimer* timer) {
     KernelTimer off;
     KernelTimer& T = timer ? *timer : off;
     const EmGeo g = geo_of(d);
@@ -848,20 +960,34 @@ void launch_em_iterations(hipStream_t s, const EmDims& d, const EmPhys& ph, cons
         T.before(2, s);
         go<K_em_update>(s, blocks, {g, p, vv, sc, part});
         T.after(2, s);
+        if (nodal) {  // nodal_build + nodal_update: z of the new residual in q, its r.z in the partials
+            const uint32_t nodes = ((uint32_t)g.n[0] + 1u) * ((uint32_t)g.n[1] + 1u) * ((uint32_t)g.n[2] + 1u);
+            go<K_em_nodal_build>(s, em_grid(nodes), {g, p, vv.r, vv.sigma, vv.flags, vv.ndiv, sc});
+            go<K_em_nodal_z>(s, blocks, {g, p, vv, vv.ndiv, sc, part, 0});
+        }
         go<K_em_fold>(s, 1, {part, blocks, kFoldBeta, sc});
         T.before(3, s);
-        go<K_em_search>(s, blocks, {g, p, vv, sc});
+        go<K_em_search>(s, blocks, {g, p, vv, sc, nodal ? 1 : 0});
         T.after(3, s);
     }
 }
 
-void launch_em_restart(hipStream_t s, const EmDims& d, const EmPhys& ph, const EmVecs& v, double* part, void* scal) {
+void launch_em_restart(hipStream_t s, const EmDims& d, const EmPhys& ph, const EmVecs& v, double* part, void* scal,
+                       bool nodal) {
     const EmGeo g = geo_of(d);
     const uint32_t blocks = em_grid(g.Etot);
     const Phys& p = ph;
     const Vecs& vv = v;
     go<K_em_true_residual>(s, blocks, {g, p, vv, (EmScal*)scal, part});
+    if (nodal) nodal_start(s, g, p, vv, part, (EmScal*)scal);
     go<K_em_fold>(s, 1, {part, blocks, kFoldRestart, (EmScal*)scal});
+}
+
+// The nodal correction applied to v.r (test hook): z into v.p, node values into v.ndiv
+void launch_em_probe_nodal(hipStream_t s, const EmDims& d, const EmPhys& ph, const EmVecs& v, double* part,
+                           void* scal) {
+    const EmGeo g = geo_of(d);
+    nodal_start(s, g, ph, v, part, (EmScal*)scal);
 }
 
 void launch_em_commit(hipStream_t s, const EmDims& d, const float* a_prev, const float* x, float* out) {

@@ -156,8 +156,13 @@ int mhdp_em_solve_predicted_dev(mhdp_em* e, const mhdp_em_opts* o, const float* 
     if (!e->ready) return fail(e, MHDP_INVALID, "EM materials are not set");
     if (!o || !as_now || !as_prev || !a_prev || !a_out) return fail(e, MHDP_INVALID, "Missing EM solve argument");
     if (!(o->dt > 0.0f) || !std::isfinite(o->dt) || !(o->mu0 > 0.0f) || !(o->rel_tol > 0.0f) ||
-        !(o->rel_tol < 1.0f) || o->max_iter == 0)
+        !(o->rel_tol < 1.0f) || o->max_iter == 0 || o->nodal < -1 || o->nodal > 1)
         return fail(e, MHDP_INVALID, "Invalid EM solve options");
+    // the nodal correction (mod.rs:887-897): induction solves only, when the cap exceeds the switch
+    // point; from the start when that point is 0
+    const uint32_t switch_after = o->nodal == 1 ? o->nodal_after : 1024u;
+    const bool allow = !o->magnetostatic && o->nodal >= 0 && o->max_iter > switch_after;
+    bool nodal = allow && switch_after == 0;
     const auto t0 = std::chrono::steady_clock::now();
     const EmPhys ph{o->dt, o->mu0, o->magnetostatic ? 1 : 0};
     const EmVecs v{e->x,     e->r,    e->p,    e->q,     e->sigma, e->diag,  e->nu,
@@ -173,7 +178,7 @@ int mhdp_em_solve_predicted_dev(mhdp_em* e, const mhdp_em_opts* o, const float* 
     if (guess) launch_em_begin(e->stream, e->d, e->flags, a_prev, a_before, prediction, e->x);
     int rc;
     for (int pass = 0; pass < 2; pass++) {
-        launch_em_prepare(e->stream, e->d, ph, v, e->part, e->sc, guess ? 1 : 0);
+        launch_em_prepare(e->stream, e->d, ph, v, e->part, e->sc, guess ? 1 : 0, nodal);
         if ((rc = read_scalars(e))) return rc;
         if (!e->sc_h->rescale) break;
     }
@@ -182,7 +187,7 @@ int mhdp_em_solve_predicted_dev(mhdp_em* e, const mhdp_em_opts* o, const float* 
     if (guess && (!finite_status(*e->sc_h) || !std::isfinite(e->sc_h->l1) || e->sc_h->rr > e->sc_h->bb)) {
         // a bad extrapolation must not start farther from the same right-hand side: zero_guess
         // and prepare again (mod.rs:906-925); the scale picked above stays
-        launch_em_prepare(e->stream, e->d, ph, v, e->part, e->sc, 0);
+        launch_em_prepare(e->stream, e->d, ph, v, e->part, e->sc, 0, nodal);
         if ((rc = read_scalars(e))) return rc;
         dropped = true;
     }
@@ -193,22 +198,30 @@ int mhdp_em_solve_predicted_dev(mhdp_em* e, const mhdp_em_opts* o, const float* 
         while (s.status == 0 && finite_status(s) && s.its < max_iter) {
             const uint32_t before = s.its;
             launch_em_iterations(e->stream, e->d, ph, v, e->part, e->sc, std::min<uint32_t>(max_iter - s.its, 32u),
-                                 &e->timer);
+                                 nodal, &e->timer);
             if ((rc = read_scalars(e))) return rc;
             s = *e->sc_h;
             // every batch either counts iterations or ends the solve (status); anything else is a
             // device-side fault of the reductions, reported instead of waiting for ever
             if (s.status == 0 && s.its == before) return fail(e, MHDP_DEVICE, "EM solve made no progress");
+            if (allow && !nodal && s.status == 0 && finite_status(s) && s.its >= switch_after && s.its < max_iter) {
+                // switch at a true-residual restart: the right-hand side norm, the acceptance
+                // threshold and the iteration count carry over (mod.rs:941-959)
+                nodal = true;
+                launch_em_restart(e->stream, e->d, ph, v, e->part, e->sc, true);
+                if ((rc = read_scalars(e))) return rc;
+                s = *e->sc_h;
+            }
         }
         if (s.status != 1) break;
         // converged on the recursive residual: verify with the actual equation and restart from it
-        launch_em_restart(e->stream, e->d, ph, v, e->part, e->sc);
+        launch_em_restart(e->stream, e->d, ph, v, e->part, e->sc, nodal);
         if ((rc = read_scalars(e))) return rc;
         s = *e->sc_h;
         if (s.status == 1 || s.its >= max_iter) break;
     }
     if (s.status != 1 && finite_status(s)) {
-        launch_em_restart(e->stream, e->d, ph, v, e->part, e->sc);
+        launch_em_restart(e->stream, e->d, ph, v, e->part, e->sc, nodal);
         if ((rc = read_scalars(e))) return rc;
         s = *e->sc_h;
     }
@@ -238,6 +251,7 @@ int mhdp_em_solve_predicted_dev(mhdp_em* e, const mhdp_em_opts* o, const float* 
         st->prediction = guess ? prediction : 0.0f;
         st->start_residual = start_residual;
         st->prediction_dropped = dropped ? 1u : 0u;
+        st->nodal = nodal ? 1u : 0u;
     }
     if (s.status != 1 || !finite_status(s)) {  // src/em/mod.rs:1011-1021
         char buf[320];
@@ -274,6 +288,25 @@ int mhdp_em_probe_apply(mhdp_em* e, const mhdp_em_opts* o, const float* p, float
     launch_em_probe_apply(e->stream, e->d, EmPhys{o->dt, o->mu0, o->magnetostatic ? 1 : 0}, e->nu, e->flags, e->sigma,
                           e->p, e->q, e->tface, e->ndiv);
     EM_HIP(hipMemcpyAsync(q, e->q, bytes, hipMemcpyDeviceToHost, e->stream));
+    EM_HIP(hipStreamSynchronize(e->stream));
+    EM_HIP(hipGetLastError());
+    return MHDP_OK;
+}
+
+int mhdp_em_probe_nodal(mhdp_em* e, const mhdp_em_opts* o, const float* r, float* z, float* nodal) {
+    if (!e || !o || !r || !z) return MHDP_INVALID;
+    e->err.clear();
+    if (!e->ready) return fail(e, MHDP_INVALID, "EM materials are not set");
+    const size_t bytes = e->edges * 4;
+    const EmPhys ph{o->dt, o->mu0, o->magnetostatic ? 1 : 0};
+    const EmVecs v{e->x,     e->r,    e->p,    e->q,    e->sigma, e->diag,  e->nu,
+                   e->flags, nullptr, nullptr, nullptr, nullptr,  e->tface, e->ndiv};
+    EM_HIP(hipMemcpyAsync(e->r, r, bytes, hipMemcpyHostToDevice, e->stream));
+    launch_em_probe_nodal(e->stream, e->d, ph, v, e->part, e->sc);
+    EM_HIP(hipMemcpyAsync(z, e->p, bytes, hipMemcpyDeviceToHost, e->stream));
+    if (nodal)
+        EM_HIP(hipMemcpyAsync(nodal, e->ndiv, (size_t)(e->d.n[0] + 1) * (e->d.n[1] + 1) * (e->d.n[2] + 1) * 4,
+                              hipMemcpyDeviceToHost, e->stream));
     EM_HIP(hipStreamSynchronize(e->stream));
     EM_HIP(hipGetLastError());
     return MHDP_OK;

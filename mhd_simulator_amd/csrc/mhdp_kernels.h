@@ -252,11 +252,16 @@ void launch_em_coefficients(hipStream_t s, const EmDims& d, const uint32_t* mate
 // from a zero increment (guess 0, x cleared)
 void launch_em_begin(hipStream_t s, const EmDims& d, const uint8_t* flags, const float* a_prev, const float* a_before,
                      float prediction, float* x);
+// nodal: the nodal auxiliary-space correction (src/em/nodal.wgsl) is on: prepare and restart start
+// the search at its z, the iterations use it in place of r / d
 void launch_em_prepare(hipStream_t s, const EmDims& d, const EmPhys& ph, const EmVecs& v, double* part, void* scal,
-                       int guess);
+                       int guess, bool nodal = false);
 void launch_em_iterations(hipStream_t s, const EmDims& d, const EmPhys& ph, const EmVecs& v, double* part, void* scal,
-                          uint32_t count, KernelTimer* timer = nullptr);
-void launch_em_restart(hipStream_t s, const EmDims& d, const EmPhys& ph, const EmVecs& v, double* part, void* scal);
+                          uint32_t count, bool nodal, KernelTimer* timer);
+void launch_em_restart(hipStream_t s, const EmDims& d, const EmPhys& ph, const EmVecs& v, double* part, void* scal,
+                       bool nodal = false);
+void launch_em_probe_nodal(hipStream_t s, const EmDims& d, const EmPhys& ph, const EmVecs& v, double* part,
+                           void* scal);
 void launch_em_commit(hipStream_t s, const EmDims& d, const float* a_prev, const float* x, float* out);
 void launch_em_probe_apply(hipStream_t s, const EmDims& d, const EmPhys& ph, const float* nu, const uint8_t* flags,
                            const float* sigma, const float* p, float* q, float* tface, float* ndiv);

@@ -59,21 +59,23 @@ class EmSolver:
         self._check(self.lib.mhdp_em_set_materials(self.handle, _ptr(m), _ptr(p), p.shape[0], int(bool(vacuum)), rng))
         self.sigma_range = (rng[0], rng[1])
 
-    def _opts(self, dt, rel_tol, max_iter, magnetostatic, mu0):
-        return _lib.EmOpts(dt, mu0, rel_tol, max_iter, int(bool(magnetostatic)))
+    def _opts(self, dt, rel_tol, max_iter, magnetostatic, mu0, nodal=0, nodal_after=0):
+        return _lib.EmOpts(dt, mu0, rel_tol, max_iter, int(bool(magnetostatic)), int(nodal), int(nodal_after))
 
     def solve(self, as_now, as_prev, a_prev, a_out, *, motional=None, dt=1e-5, rel_tol=1e-5, max_iter=1000,
-              magnetostatic=False, mu0=MU0, a_before=None, prediction=0.0):
+              magnetostatic=False, mu0=MU0, a_before=None, prediction=0.0, nodal=0, nodal_after=0):
         """a_out = a_prev + the step's increment; returns (iterations, relative residual).  With
         a_before (the potential accepted one step before a_prev) and prediction > 0 (the ratio of
         this step's dt to the last one's) the increment starts from prediction * (a_prev - a_before)
         (mhdp_em_solve_predicted_dev); stats["prediction_dropped"] tells whether that start was
-        rejected for the zero start."""
+        rejected for the zero start.  nodal: the nodal auxiliary-space correction's policy (0 the
+        reference's: switch after 1024 iterations of an induction solve allowed more; -1 never; 1
+        after nodal_after iterations, 0 = from the start); stats["nodal"] tells whether it ran."""
         extra = tuple(t for t in (motional, a_before) if t is not None)
         for t in (as_now, as_prev, a_prev, a_out) + extra:
             if not hasattr(t, "data_ptr") or t.numel() != self.edges:
                 raise ValueError("the EM solve takes device tensors with one value per edge")
-        o = self._opts(dt, rel_tol, max_iter, magnetostatic, mu0)
+        o = self._opts(dt, rel_tol, max_iter, magnetostatic, mu0, nodal, nodal_after)
         st = _lib.EmStats()
         self._check(self.lib.mhdp_em_solve_predicted_dev(self.handle, C.byref(o), _ptr(as_now), _ptr(as_prev),
                                                          _ptr(motional), _ptr(a_prev), _ptr(a_before),
@@ -95,6 +97,15 @@ class EmSolver:
         o = self._opts(dt, 1e-5, 1, magnetostatic, mu0)
         self._check(self.lib.mhdp_em_probe_apply(self.handle, C.byref(o), _ptr(v), _ptr(q)))
         return q
+
+    def nodal(self, r, *, dt=1e-5, magnetostatic=False, mu0=MU0):
+        """The nodal correction applied to r (host vector) -> (z, node values)."""
+        r = np.ascontiguousarray(r, dtype=np.float32)
+        z = np.zeros_like(r)
+        nodes = np.zeros((self.dims[0] + 1) * (self.dims[1] + 1) * (self.dims[2] + 1), np.float32)
+        o = self._opts(dt, 1e-5, 1, magnetostatic, mu0)
+        self._check(self.lib.mhdp_em_probe_nodal(self.handle, C.byref(o), _ptr(r), _ptr(z), _ptr(nodes)))
+        return z, nodes
 
     def set_kernel_timing(self, kernel=None):
         """Time every launch of one CG kernel ("curl", "apply", "update", "search"; None: off)

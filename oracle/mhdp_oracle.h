@@ -162,6 +162,12 @@ typedef struct {
     float tol2;        /* relative tolerance squared, Params.physical.w */
     uint32_t max_iter; /* EmConfig.max_iterations */
     int magnetostatic; /* Params.control.y == 1 (initial solve): no mass term */
+    /* nodal auxiliary-space correction (src/em/nodal.wgsl; the switch of EmState::advance_to,
+     * src/em/mod.rs:887-960): 0 the reference's policy (an induction solve allowed more than 1024
+     * iterations switches at a true-residual restart after 1024), -1 never, 1 switch after
+     * nodal_after iterations (0: from the start; the reference's nodal_switch_after test hook) */
+    int nodal;
+    uint32_t nodal_after;
 } orc_em_params;
 typedef struct {
     uint32_t iterations;
@@ -169,6 +175,7 @@ typedef struct {
     double scale;        /* power-of-two residual scale picked by initial_reduce */
     double start_residual;       /* sqrt(rr / bb) of the first start (prediction_diagnostic[1]) */
     uint32_t prediction_dropped; /* the predicted start was rejected (prediction_diagnostic[2]) */
+    uint32_t nodal;              /* the solve ran (or switched to) the nodal correction */
 } orc_em_stats;
 /* coefficients (induction.wgsl:82-99): materials one u32 per cell, props[2k], props[2k+1] = (sigma,
  * mu_r) of material k; vacuum = Params.control.z == 1.  Out per edge: sigma (mean of the four
@@ -196,6 +203,14 @@ int orc_em_solve_f64(const orc_em_grid *g, const orc_em_params *p, const float *
                      const uint8_t *flags, const float *nu, const float *as_now, const float *as_prev,
                      const float *mot, const float *a_prev, const float *a_before, float prediction, double *a_out,
                      orc_em_stats *st);
+
+/* The nodal correction applied to r (nodal_build_initial + nodal_z, nodal.wgsl:9-30):
+ * z = r / d + N(lower node) - N(upper node), N = node values (dims n + 1, 0 off the interior);
+ * node_diag (may be NULL) receives diag(G^T A G) per node, nodal (may be NULL) the node values. */
+void orc_em_nodal_f32(const orc_em_grid *g, const orc_em_params *p, const float *sigma, const float *diag,
+                      const uint8_t *flags, const float *r, float *z, float *node_diag, float *nodal);
+void orc_em_nodal_f64(const orc_em_grid *g, const orc_em_params *p, const float *sigma, const float *diag,
+                      const uint8_t *flags, const double *r, double *z, double *node_diag, double *nodal);
 
 /* ---- multigrid probes (mirror the reference's test-only entry points) ---- */
 typedef struct orc_mg orc_mg;

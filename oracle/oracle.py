@@ -261,12 +261,12 @@ class _EmGrid(C.Structure):
 
 class _EmParams(C.Structure):
     _fields_ = [("dt", C.c_float), ("mu0", C.c_float), ("tol2", C.c_float), ("max_iter", C.c_uint32),
-                ("magnetostatic", C.c_int)]
+                ("magnetostatic", C.c_int), ("nodal", C.c_int), ("nodal_after", C.c_uint32)]
 
 
 class _EmStats(C.Structure):
     _fields_ = [("iterations", C.c_uint32), ("rel_residual", C.c_double), ("scale", C.c_double),
-                ("start_residual", C.c_double), ("prediction_dropped", C.c_uint32)]
+                ("start_residual", C.c_double), ("prediction_dropped", C.c_uint32), ("nodal", C.c_uint32)]
 
 
 def em_counts(dims):
@@ -291,9 +291,10 @@ def em_coefficients(dims, h, materials, props, vacuum=False):
     return out
 
 
-def _em_params(dt, mu0, tol, max_iter, magnetostatic):
+def _em_params(dt, mu0, tol, max_iter, magnetostatic, nodal=0, nodal_after=0):
     # Params.physical.w = relative_tolerance.powi(2) with an f32 tolerance (src/em/mod.rs:888)
-    return _EmParams(dt, mu0, float(np.float32(tol) * np.float32(tol)), max_iter, int(bool(magnetostatic)))
+    return _EmParams(dt, mu0, float(np.float32(tol) * np.float32(tol)), max_iter, int(bool(magnetostatic)),
+                     int(nodal), int(nodal_after))
 
 
 def em_apply(dims, h, co, v, dt=1e-5, mu0=4e-7 * np.pi, magnetostatic=False, precision=32):
@@ -310,16 +311,19 @@ def em_apply(dims, h, co, v, dt=1e-5, mu0=4e-7 * np.pi, magnetostatic=False, pre
 
 
 def em_solve(dims, h, co, as_now, as_prev, a_prev, mot=None, dt=1e-5, mu0=4e-7 * np.pi, tol=1e-5,
-             max_iter=2000, magnetostatic=False, precision=64, a_before=None, prediction=0.0):
+             max_iter=2000, magnetostatic=False, precision=64, a_before=None, prediction=0.0, nodal=0,
+             nodal_after=0):
     """One step's Jacobi-CG solve (src/em/mod.rs:863-1010) -> (status, a_out, stats dict); with
-    a_before and prediction > 0 the increment starts from prediction * (a_prev - a_before)."""
+    a_before and prediction > 0 the increment starts from prediction * (a_prev - a_before).
+    nodal: 0 the reference's switch to the nodal correction (after 1024 iterations of an induction
+    solve allowed more), -1 never, 1 after nodal_after iterations (0: from the start)."""
     edges, _ = em_counts(dims)
     E = sum(edges)
     dt_ = np.float64 if precision == 64 else np.float32
     a_out = np.zeros(E, dt_)
     st = _EmStats()
     g = _EmGrid(dims[0], dims[1], dims[2], h)
-    prm = _em_params(dt, mu0, tol, max_iter, magnetostatic)
+    prm = _em_params(dt, mu0, tol, max_iter, magnetostatic, nodal, nodal_after)
     f32 = lambda a: _ptr(np.ascontiguousarray(a, dtype=np.float32))  # noqa: E731
     keep = [np.ascontiguousarray(a, dtype=np.float32) for a in (as_now, as_prev, a_prev)]
     mot_ = None if mot is None else np.ascontiguousarray(mot, dtype=np.float32)
@@ -330,6 +334,21 @@ def em_solve(dims, h, co, as_now, as_prev, a_prev, mot=None, dt=1e-5, mu0=4e-7 *
             None if before is None else _ptr(before), C.c_float(prediction), _ptr(a_out), C.byref(st))
     del f32
     return rc, a_out, {k: getattr(st, k) for k, _ in _EmStats._fields_}
+
+
+def em_nodal(dims, h, co, r, dt=1e-5, mu0=4e-7 * np.pi, magnetostatic=False, precision=64):
+    """The nodal correction applied to r (nodal.wgsl) -> (z, node diag(G^T A G), node values);
+    nodes (n + 1)^3, x fastest."""
+    dt_ = np.float64 if precision == 64 else np.float32
+    r = np.ascontiguousarray(r, dtype=dt_)
+    nodes = (dims[0] + 1) * (dims[1] + 1) * (dims[2] + 1)
+    z, S, N = np.zeros_like(r), np.zeros(nodes, dt_), np.zeros(nodes, dt_)
+    g = _EmGrid(dims[0], dims[1], dims[2], h)
+    prm = _em_params(dt, mu0, 1e-5, 1, magnetostatic)
+    fn = lib().orc_em_nodal_f64 if precision == 64 else lib().orc_em_nodal_f32
+    fn(C.byref(g), C.byref(prm), _ptr(co["sigma"]), _ptr(co["diag"]), _ptr(co["flags"]), _ptr(r), _ptr(z),
+       _ptr(S), _ptr(N))
+    return z, S, N
 
 
 def project(g, mask, alpha, ustar, rel_tol=1e-5, abs_div_tol=0.0, max_iter=1000, precond=0, gauge=0,
