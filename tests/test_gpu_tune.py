@@ -69,3 +69,22 @@ def test_tuner_winner_reproduces(problem, restore):
     opts = tune.apply(best)
     its, r, _, _ = solve(pr, u, **opts)
     assert r <= 1e-8
+
+
+def test_two_stage_tuner_over_smoothers(problem, restore):
+    """Schedules first (Jacobi and red-black at their tuned weights and scales), then launch shapes
+    at the winner; every candidate converges with the cycle it asked for and the winner reproduces."""
+    pr, u = problem
+    sched = tune.schedules(smooth_weight=(0.0,), coarse_scale=(0.0,), coarse_visits=(0,))
+    print(f"[gpu] tune: two stages, {len(sched)} schedules", flush=True)
+    best = tune.tune_projection(pr, u, refine_tol=1e-8, repeats=1, schedule=sched,
+                                launch_axes={"march_z": (32,), "march_cap": (1024, 2048), "brick_occ": (2,)})
+    rows = best["table"]
+    assert len(rows) == len(sched) + 2 and all(r["ok"] for r in rows)
+    assert {r["opts"]["smoother"] for r in rows[:len(sched)]} == {"jacobi", "red-black"}
+    assert all(r["opts"] == rows[len(sched)]["opts"] for r in rows[len(sched):])
+    assert best["ms"] == min(r["ms"] for r in rows)
+    assert tune.current() == restore
+    opts = tune.apply(best)
+    its, r, _, _ = solve(pr, u, **opts)
+    assert r <= 1e-8

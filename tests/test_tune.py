@@ -94,3 +94,44 @@ def test_results_persist_by_key(tmp_path, restore):
     assert {k: tune.current()[k] for k in a["launch"]} == a["launch"]
     with pytest.raises(KeyError):
         tune.load(path, "missing")
+
+
+def test_schedule_stage_covers_smoothers_weights_and_scales():
+    cands = tune.schedules()
+    assert len(cands) == 2 * 3 * 3 * 3
+    assert all(c["launch"] == tune.BUILTIN_LAUNCH for c in cands)
+    assert {c["opts"]["smoother"] for c in cands} == {"jacobi", "red-black"}
+    # the library's defaults are left out of the options
+    assert {c["opts"].get("smooth_weight", 0.0) for c in cands} == {0.0, 0.8, 1.0}
+    assert {c["opts"].get("coarse_scale", 0.0) for c in cands} == {0.0, 1.4, 2.0}
+    assert not any("smooth_weight" in c["opts"] and c["opts"]["smooth_weight"] == 0.0 for c in cands)
+
+
+def test_two_stage_search_tunes_the_launch_at_the_winning_schedule(restore):
+    """Stage 1 picks the fastest schedule at the built-in shapes, stage 2 times the launch shapes
+    with that schedule; a schedule whose cycle breaks down (the run raises) never wins."""
+    clock_t = [0.0]
+    seen = []
+
+    def cost(c):
+        o = c["opts"]
+        ms = 10.0 - (3.0 if o["smoother"] == "red-black" else 0.0) - (1.0 if o.get("coarse_scale") == 1.4 else 0.0)
+        return ms - (0.5 if c["launch"]["march_z"] == 64 else 0.0)
+
+    def run(c):
+        o = c["opts"]
+        if o["smoother"] == "red-black" and o.get("coarse_scale") == 2.0:
+            raise RuntimeError("the cycle broke down")
+        seen.append(c)
+        clock_t[0] += cost(c) * 1e-3
+        return True
+
+    best, table = tune.two_stage(run, 1, tune.schedules(smooth_weight=(0.0,), coarse_visits=(1,)),
+                                 {"march_z": (32, 64), "march_cap": (1024,), "brick_occ": (2,)},
+                                 clock=lambda: clock_t[0])
+    assert best["opts"] == {"coarse_visits": 1, "smoother": "red-black", "coarse_scale": 1.4}
+    assert best["launch"]["march_z"] == 64 and best["ms"] == pytest.approx(5.5)
+    stage2 = table[-2:]
+    assert all(r["opts"] == best["opts"] for r in stage2)
+    assert any(not r["ok"] and "broke down" in r.get("error", "") for r in table)
+    assert tune.current() == restore
