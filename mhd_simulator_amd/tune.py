@@ -40,8 +40,8 @@ BUILTIN_LAUNCH = {"march_z": 32, "march_cap": 1024, "brick_occ": 2}
 # the schedule stage's axes (0: the library's tuned default for the smoother: weight 0.9, scale
 # 1.6 with Jacobi and 1.8 with red-black)
 SMOOTHER = ("jacobi", "red-black")
-SMOOTH_WEIGHT = (0.0, 0.8, 1.0)
-COARSE_SCALE = (0.0, 1.4, 2.0)
+SMOOTH_WEIGHT = (0.0, 0.8, 0.95)  # the library accepts weights in [0, 1)
+COARSE_SCALE = (0.0, 1.4, 1.9)    # and scales in [0, 2)
 
 
 def current() -> dict:
