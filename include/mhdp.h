@@ -98,7 +98,8 @@ typedef struct {
                             (multigrid.wgsl:66-73, weight smooth_weight); MHDP_SMOOTH_RED_BLACK
                             symmetric red-black Gauss-Seidel (red then black before the coarse
                             correction, black then red after it) on every level whose periodic
-                            axes have even length (Jacobi on the others); AUTO = Jacobi.
+                            axes have even length (Jacobi on the others); AUTO = red-black on
+                            grids of at least 2^24 cells, Jacobi below.
                             The coarse scale's tuned default is 1.6 with Jacobi, 1.8 with
                             red-black; the fused multigrid path only (the one-kernel-per-stage
                             path, MHDP_UNFUSED=1, keeps Jacobi) */

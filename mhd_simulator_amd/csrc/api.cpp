@@ -329,26 +329,16 @@ void build_mg(mhdp_ctx* c, int meanfree, bool f16) {
         const float* fbz = fine ? c->bz : c->coarse[l - 2].bz;
         const float* fdinv = fine ? c->dinv : c->coarse[l - 2].dinv;
         const float* freact = fine ? nullptr : c->coarse[l - 2].react;
+        void* const c16[5] = {a.bx16, a.by16, a.bz16, a.react16, a.dinv16};  // the graph pass writes them too
         launch_mg_build(c->stream, c->levels[l], c->levels[l - 1], fine, meanfree, c->mask, fdinv, fbx, fby, fbz,
-                        freact, a.bx, a.by, a.bz, a.react, a.diag, a.dinv);
+                        freact, a.bx, a.by, a.bz, a.react, a.diag, a.dinv, f16 ? c16 : nullptr);
     }
-    if (f16) {
-        if (!c->fine16_fresh) {  // the coefficient pass did not write the BF16 copies
-            launch_to_bf16(c->stream, c->g.n, c->bx, c->bx16);
-            launch_to_bf16(c->stream, c->g.n, c->by, c->by16);
-            launch_to_bf16(c->stream, c->g.n, c->bz, c->bz16);
-            launch_to_bf16(c->stream, c->g.n, c->dinv, c->dinv16);
-            c->fine16_fresh = true;
-        }
-        for (size_t l = 1; l < c->levels.size(); l++) {
-            CoarseArrays& a = c->coarse[l - 1];
-            const uint32_t n = c->levels[l].n;
-            launch_to_bf16(c->stream, n, a.bx, a.bx16);
-            launch_to_bf16(c->stream, n, a.by, a.by16);
-            launch_to_bf16(c->stream, n, a.bz, a.bz16);
-            launch_to_bf16(c->stream, n, a.react, a.react16);
-            launch_to_bf16(c->stream, n, a.dinv, a.dinv16);
-        }
+    if (f16 && !c->fine16_fresh) {  // the coefficient pass did not write the BF16 copies
+        launch_to_bf16(c->stream, c->g.n, c->bx, c->bx16);
+        launch_to_bf16(c->stream, c->g.n, c->by, c->by16);
+        launch_to_bf16(c->stream, c->g.n, c->bz, c->bz16);
+        launch_to_bf16(c->stream, c->g.n, c->dinv, c->dinv16);
+        c->fine16_fresh = true;
     }
 }
 
@@ -517,6 +507,9 @@ void iterate_once(mhdp_ctx* c, Solve& S) {  // mod.rs:1057-1075, multigrid.rs:30
 constexpr double kTunedOmega = 0.9;   // smoother weight picked by the V-cycle tuner
 constexpr double kTunedScale = 1.6;   // coarse-correction scale picked by the V-cycle tuner (Jacobi)
 constexpr double kTunedScaleRB = 1.8; // the same with red-black Gauss-Seidel smoothing
+// The automatic smoother: red-black Gauss-Seidel on grids of at least 2^24 cells (at 512^3 it needs
+// 13 iterations where weighted Jacobi needs 21, at about 1.4x the cost per pass), Jacobi below.
+constexpr uint32_t kRedBlackAutoCells = 1u << 24;
 // Level-1 visits per cycle when the caller leaves the choice (coarse_visits = 0): a second visit
 // (W-cycle at level 1) saves about a third of the iterations, but its coarse levels are launch-bound
 // small kernels; measured on the plate workload it pays from 256^3 (12.1 vs 12.7 ms; 512^3: 75 vs 83
@@ -730,7 +723,8 @@ bool is_reference(const Cycle& cy) {
 // The cycle the options ask for: smoother, its weight and coarse scale (0 = tuned defaults), visits.
 Cycle requested_cycle(const mhdp_ctx* c, const mhdp_opts* o) {
     Cycle cy;
-    cy.rb = o->smoother == MHDP_SMOOTH_RED_BLACK;  // AUTO: Jacobi (fewer, slower passes cost more at 512^3)
+    cy.rb = o->smoother == MHDP_SMOOTH_RED_BLACK ||
+            (o->smoother == MHDP_SMOOTH_AUTO && c->g.n >= kRedBlackAutoCells);
     cy.omega = o->smooth_weight > 0.0f ? (double)o->smooth_weight : kTunedOmega;
     cy.pscale = o->coarse_scale > 0.0f ? (double)o->coarse_scale : (cy.rb ? kTunedScaleRB : kTunedScale);
     cy.visits = coarse_visits_for(c, o->coarse_visits);
@@ -1121,8 +1115,6 @@ int project_lockstep(mhdp_ctx* const* ctxs, int count, const float* const* ustar
     } timers{ctxs, std::vector<int>((size_t)n, -1)};
     const double tau = o->inner_tol > 0.0f ? (double)o->inner_tol : 1e-4;
     const uint32_t max_outer = o->max_outer ? o->max_outer : 16u;
-    const double omega = o->smooth_weight > 0.0f ? (double)o->smooth_weight : kTunedOmega;
-    const double pscale = o->coarse_scale > 0.0f ? (double)o->coarse_scale : kTunedScale;
     for (int p = 0; p < n; p++) {
         mhdp_ctx* c = ctxs[p];
         c->err.clear();

@@ -1,6 +1,7 @@
 // One kernel body, two launches.  A kernel is a struct K with
 //   struct Args { ... };                        (trivially copyable: pointers and scalars)
 //   static constexpr int MINB;                  (resident blocks per CU to target; 0 = none)
+//   static constexpr int THREADS;               (optional: threads per block, else MHDP_BLOCK)
 //   static __device__ void run(const Args&, const VGrid&);
 // go<K>(stream, blocks, args) launches it for one problem (`blocks` blocks).  While a host thread
 // records (Recorder::active), go<K> appends the launch instead; the batch driver then issues
@@ -11,14 +12,22 @@
 #include <cstring>
 #include <vector>
 
+#include <type_traits>
+
 #include "mhdp_device.h"
 
+// Threads per block of kernel K: K::THREADS when it declares one, else MHDP_BLOCK.
+template <class K, class = void>
+struct KThreads : std::integral_constant<int, MHDP_BLOCK> {};
 template <class K>
-__global__ void __launch_bounds__(MHDP_BLOCK) k_one0(typename K::Args a) {
+struct KThreads<K, std::void_t<decltype(K::THREADS)>> : std::integral_constant<int, K::THREADS> {};
+
+template <class K>
+__global__ void __launch_bounds__(KThreads<K>::value) k_one0(typename K::Args a) {
     K::run(a, vgrid());
 }
 template <class K>
-__global__ void __launch_bounds__(MHDP_BLOCK, (K::MINB > 0 ? K::MINB : 1)) k_one(typename K::Args a) {
+__global__ void __launch_bounds__(KThreads<K>::value, (K::MINB > 0 ? K::MINB : 1)) k_one(typename K::Args a) {
     K::run(a, vgrid());
 }
 // starts[0..count]: first block of each problem (a multiple of 8) and the end; counts[p]: its
@@ -44,14 +53,14 @@ __device__ __forceinline__ void run_many(const typename K::Args* __restrict__ a,
     K::run(A, VGrid{lb, n});
 }
 template <class K>
-__global__ void __launch_bounds__(MHDP_BLOCK) k_many0(const typename K::Args* a, const uint32_t* starts,
-                                                      uint32_t count) {
+__global__ void __launch_bounds__(KThreads<K>::value) k_many0(const typename K::Args* a, const uint32_t* starts,
+                                                              uint32_t count) {
     run_many<K>(a, starts, count);
 }
 template <class K>
-__global__ void __launch_bounds__(MHDP_BLOCK, (K::MINB > 0 ? K::MINB : 1)) k_many(const typename K::Args* a,
-                                                                                const uint32_t* starts,
-                                                                                uint32_t count) {
+__global__ void __launch_bounds__(KThreads<K>::value, (K::MINB > 0 ? K::MINB : 1)) k_many(const typename K::Args* a,
+                                                                                        const uint32_t* starts,
+                                                                                        uint32_t count) {
     run_many<K>(a, starts, count);
 }
 
@@ -79,12 +88,13 @@ inline void record_mark() {
 
 template <class K>
 void issue_many(hipStream_t s, const void* args_dev, const uint32_t* starts_dev, uint32_t count, uint32_t blocks) {
+    const dim3 threads(KThreads<K>::value);
     if constexpr (K::MINB > 0)
-        hipLaunchKernelGGL(k_many<K>, dim3(blocks), dim3(MHDP_BLOCK), 0, s, (const typename K::Args*)args_dev,
-                           starts_dev, count);
+        hipLaunchKernelGGL(k_many<K>, dim3(blocks), threads, 0, s, (const typename K::Args*)args_dev, starts_dev,
+                           count);
     else
-        hipLaunchKernelGGL(k_many0<K>, dim3(blocks), dim3(MHDP_BLOCK), 0, s, (const typename K::Args*)args_dev,
-                           starts_dev, count);
+        hipLaunchKernelGGL(k_many0<K>, dim3(blocks), threads, 0, s, (const typename K::Args*)args_dev, starts_dev,
+                           count);
 }
 
 template <class K>
@@ -100,6 +110,7 @@ void go(hipStream_t s, uint32_t blocks, const typename K::Args& a) {
         r->ops.push_back(std::move(op));
         return;
     }
-    if constexpr (K::MINB > 0) hipLaunchKernelGGL(k_one<K>, dim3(blocks), dim3(MHDP_BLOCK), 0, s, a);
-    else hipLaunchKernelGGL(k_one0<K>, dim3(blocks), dim3(MHDP_BLOCK), 0, s, a);
+    const dim3 threads(KThreads<K>::value);
+    if constexpr (K::MINB > 0) hipLaunchKernelGGL(k_one<K>, dim3(blocks), threads, 0, s, a);
+    else hipLaunchKernelGGL(k_one0<K>, dim3(blocks), threads, 0, s, a);
 }

@@ -152,10 +152,10 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Fixed-order block sum of NV lanes; thread 0 receives the totals.
-template <int NV>
+// Fixed-order block sum of NV lanes over a block of NT threads; thread 0 receives the totals.
+template <int NV, int NT = MHDP_BLOCK>
 __device__ __forceinline__ void block_sum(double (&v)[NV]) {
-    __shared__ double red[MHDP_BLOCK / 64][NV];
+    __shared__ double red[NT / 64][NV];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < NV; k++) {
@@ -167,7 +167,7 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
 #pragma unroll
         for (int k = 0; k < NV; k++) {
             double s = 0.0;
-            for (int w = 0; w < MHDP_BLOCK / 64; w++) s += red[w][k];
+            for (int w = 0; w < NT / 64; w++) s += red[w][k];
             v[k] = s;
         }
     }
@@ -301,10 +301,10 @@ __device__ inline void finalize(Scal* s, int mode, const double t[4]) {
 // and a block waits for its partial stores to complete before taking its ticket.  No
 // __threadfence(): on this device a device-scope fence writes back AND invalidates the XCD's
 // L2, which, once per block, throws away the halo data the other blocks are reading.
-template <int NV>
+template <int NV, int NT = MHDP_BLOCK>
 __device__ __forceinline__ void reduce_finish(const VGrid& vg, double (&v)[NV], double* __restrict__ part, Scal* s,
                                               int ticket, int mode) {
-    block_sum<NV>(v);
+    block_sum<NV, NT>(v);
     __shared__ int last;
     const uint32_t lb = vg.b;
     if (threadIdx.x == 0) {
@@ -318,12 +318,12 @@ __device__ __forceinline__ void reduce_finish(const VGrid& vg, double (&v)[NV], 
     __syncthreads();
     if (!last) return;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (uint32_t j = threadIdx.x; j < vg.n; j += MHDP_BLOCK) {
+    for (uint32_t j = threadIdx.x; j < vg.n; j += NT) {
 #pragma unroll
         for (int k = 0; k < 4; k++)
             acc[k] += __hip_atomic_load(part + 4 * j + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    block_sum<4>(acc);
+    block_sum<4, NT>(acc);
     if (threadIdx.x == 0) {
         finalize(s, mode, acc);
         s->ticket[ticket] = 0u;

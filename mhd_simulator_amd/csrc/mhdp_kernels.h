@@ -39,7 +39,8 @@ bool launch_correct_f64(hipStream_t s, const Grid3& g, float h, const float* bx,
                         double* phi_copy = nullptr);
 void launch_mg_build(hipStream_t s, const Grid3& cg, const Grid3& fg, bool fine, int meanfree, const uint32_t* mask,
                      const float* fdinv, const float* fbx, const float* fby, const float* fbz, const float* freact,
-                     float* cbx, float* cby, float* cbz, float* creact, float* cdiag, float* cdinv);
+                     float* cbx, float* cby, float* cbz, float* creact, float* cdiag, float* cdinv,
+                     void* const c16[5] = nullptr);  // c16: BF16 copies of bx, by, bz, react, dinv
 void launch_to_bf16(hipStream_t s, uint32_t n, const float* src, void* dst);
 
 // pcg.hip

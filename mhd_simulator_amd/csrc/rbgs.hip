@@ -356,6 +356,298 @@ struct K_rb {
     }
 };
 
+// ---------------------------------------------------------------- pair-column kernel
+//
+// The same passes with the work laid out for fewer, wider loads and no halo rows in registers.
+// A lane owns a PAIR of columns (x even, x + 1): on every row and plane one of the two is red and
+// the other black, so each stage computes exactly one cell per lane, and the restriction's x pair
+// is the lane itself.  A block of 1024 threads (16 waves) owns a tile of 32 pairs by 32 rows
+// (64 x 32 columns, of which the middle 60 x 28 are outputs) of every plane of a z-chunk and
+// marches up z; each wave holds two rows (half-waves of 32 lanes).  x neighbours come from DPP
+// lane shifts, y neighbours from the LDS copy of the plane's values, z neighbours from the
+// lane's own register ring.  Per plane step p: stage 1 stages plane p (u, coefficient pairs) and
+// publishes the red values and the y-face pairs; stage 2 computes plane p - 1's black values and
+// publishes them; stage 3 finishes plane p - 2's red cells.  Double-buffered LDS planes make one
+// barrier per step enough.  Needs nx even (a pair never straddles a row end).
+typedef unsigned int rb_v2u __attribute__((ext_vector_type(2)));
+template <class T> struct RbP;
+template <> struct RbP<float> {
+    using raw = rb_v2u;
+    static constexpr uint32_t esize = 4;
+    static __device__ __forceinline__ raw ld(RbRsrc R, uint32_t v, uint32_t s) {
+        return __builtin_amdgcn_raw_buffer_load_b64(R, (int)v, (int)s, 0);
+    }
+    static __device__ __forceinline__ float lo(raw r) { return __uint_as_float(r.x); }
+    static __device__ __forceinline__ float hi(raw r) { return __uint_as_float(r.y); }
+    static __device__ __forceinline__ raw zero() { return raw{0u, 0u}; }
+    static __device__ __forceinline__ void st(RbRsrc R, float a, float b, uint32_t v, uint32_t s) {
+        raw o;
+        o.x = __float_as_uint(a);
+        o.y = __float_as_uint(b);
+        __builtin_amdgcn_raw_buffer_store_b64(o, R, (int)v, (int)s, 0);
+    }
+};
+template <> struct RbP<__hip_bfloat16> {
+    using raw = uint32_t;
+    static constexpr uint32_t esize = 2;
+    static __device__ __forceinline__ raw ld(RbRsrc R, uint32_t v, uint32_t s) {
+        return __builtin_amdgcn_raw_buffer_load_b32(R, (int)v, (int)s, 0);
+    }
+    static __device__ __forceinline__ float lo(raw r) { return __uint_as_float(r << 16); }
+    static __device__ __forceinline__ float hi(raw r) { return __uint_as_float(r & 0xFFFF0000u); }
+    static __device__ __forceinline__ raw zero() { return 0u; }
+    static __device__ __forceinline__ void st(RbRsrc R, float a, float b, uint32_t v, uint32_t s) {
+        const __hip_bfloat16 ba = __float2bfloat16(a), bb = __float2bfloat16(b);
+        unsigned short ua, ub;
+        __builtin_memcpy(&ua, &ba, 2);
+        __builtin_memcpy(&ub, &bb, 2);
+        __builtin_amdgcn_raw_buffer_store_b32((uint32_t)ua | ((uint32_t)ub << 16), R, (int)v, (int)s, 0);
+    }
+};
+template <class T>
+__device__ __forceinline__ float rbp_sel(typename RbP<T>::raw r, bool hi) {
+    return hi ? RbP<T>::hi(r) : RbP<T>::lo(r);
+}
+
+constexpr int RBP_LX = 32;                 // pairs per tile row (a half wave)
+constexpr int RBP_NY = 32;                 // rows per tile (two per wave)
+constexpr int RBP_OUTX = 2 * (RBP_LX - 2);  // 60 output columns
+constexpr int RBP_OUTY = RBP_NY - 4;        // 28 output rows
+constexpr int RBP_THREADS = RBP_LX * RBP_NY;
+
+template <class CT, class VT, class RT, int MODE, bool COARSE>
+struct K_rbp {
+    static constexpr int THREADS = RBP_THREADS;
+    static constexpr int MINB = 1;
+    using Args = typename K_rb<CT, VT, RT, MODE, COARSE, 4>::Args;
+    using PC = RbP<CT>;
+    using PR = RbP<RT>;
+    using PV = RbP<VT>;
+    // one plane's loads of a lane (kept raw: a conversion placed early by the compiler would wait
+    // for the load)
+    struct Raw {
+        typename PR::raw r;
+        rb_v2u q;
+        typename PC::raw d, bx, by, bz;
+        typename RbBuf<VT>::raw zc;
+        typename PV::raw base;
+    };
+
+    static __device__ void run(const Args& A, const VGrid& vg) {
+        Scal* sc = A.sc;
+        if (CONVERGED(sc)) return;
+        constexpr bool upd = !COARSE && MODE == RB_RESTRICT_UPDATE;
+        constexpr bool restrict_pass = MODE != RB_SMOOTH;
+        __shared__ float sU[2][RBP_NY][RBP_LX];                // red values of a plane (stage 1)
+        __shared__ float sW[2][RBP_NY][RBP_LX];                // black values of a plane (stage 2)
+        __shared__ typename PC::raw sB[3][RBP_NY][RBP_LX];     // y-face pairs of a plane
+        const Grid3 g = A.g;
+        const bool perx = g.per & 1u, pery = g.per & 2u, perz = g.per & 4u;
+        const float m = COARSE ? 0.0f : (float)(A.use_next ? sc->mean_next : sc->mean);
+        const float mn = COARSE ? 0.0f : (float)sc->mean_next;
+        const float alpha = (float)sc->alpha;
+        const float ps = (float)sc->pscale;
+        const uint32_t rsel = COARSE ? 0u : sc->rsel;
+        const RT* __restrict__ rin = rsel ? A.r1 : A.r0;
+        float* __restrict__ rout = rsel ? A.w0 : A.w1;
+        const uint32_t sz = (uint32_t)g.nx * (uint32_t)g.ny;
+        const uint32_t csy = (uint32_t)A.cg.nx, csz = (uint32_t)A.cg.nx * (uint32_t)A.cg.ny;
+        const uint32_t ntx = ((uint32_t)g.nx + RBP_OUTX - 1) / RBP_OUTX, nty = ((uint32_t)g.ny + RBP_OUTY - 1) / RBP_OUTY;
+        const uint32_t tiles = ntx * nty;
+        const uint32_t items = tiles * (((uint32_t)g.nz + A.zlen - 1) / A.zlen);
+        const uint64_t cells = (uint64_t)sz * (uint64_t)g.nz;
+        const uint64_t ccells = (uint64_t)csz * (uint64_t)A.cg.nz;
+        const RbRsrc Rr = rb_rsrc(rin, cells * PR::esize);
+        const RbRsrc Rq = rb_rsrc(upd ? (const void*)A.q : (const void*)rin, cells * 4u);
+        const RbRsrc Rd = rb_rsrc(A.dinv, cells * PC::esize);
+        const RbRsrc Rx = rb_rsrc(A.bx, cells * PC::esize);
+        const RbRsrc Ry = rb_rsrc(A.by, cells * PC::esize);
+        const RbRsrc Rz = rb_rsrc(A.bz, cells * PC::esize);
+        const RbRsrc Rc = rb_rsrc(MODE == RB_SMOOTH ? (const void*)A.zc : (const void*)A.dinv,
+                                  (MODE == RB_SMOOTH ? ccells : cells) * RbBuf<VT>::size);
+        const RbRsrc Rw = rb_rsrc((const void*)rout, cells * 4u);
+        const RbRsrc Ro = rb_rsrc(A.out, (restrict_pass ? ccells : cells) * RbBuf<VT>::size);
+        const RbRsrc Rb = rb_rsrc(COARSE && A.base ? (const void*)A.base : (const void*)A.out, cells * PV::esize);
+        const int lane = (int)(threadIdx.x & 63u);
+        const int lx = lane & (RBP_LX - 1);
+        const int ry = (int)(threadIdx.x >> 5);  // tile row: two per wave
+        const int ryl = ry > 0 ? ry - 1 : 0, ryh = ry < RBP_NY - 1 ? ry + 1 : RBP_NY - 1;
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};  // update: sum r'^2 in [1], sum r' in [3]
+        double dot[1] = {0.0};                 // smoother: (r - mean) . t
+        for (uint32_t it = vg.b; it < items; it += vg.n) {
+            const uint32_t ch = it / tiles, rem = it - ch * tiles;
+            const uint32_t ty = rem / ntx, tx = rem - ty * ntx;
+            const int xa = (int)tx * RBP_OUTX - 2 + 2 * lx;  // the pair's first column (even)
+            const int y = (int)ty * RBP_OUTY - 2 + ry;
+            const int za = (int)(ch * A.zlen), zb = min(za + (int)A.zlen, g.nz);
+            const int gx = rb_coord(xa, g.nx, perx), gy = rb_coord(y, g.ny, pery);
+            const bool lane_ok = gx >= 0 && gy >= 0;
+            const uint32_t e0 = lane_ok ? (uint32_t)gx + (uint32_t)gy * (uint32_t)g.nx : 0u;
+            // lane byte offsets (past the arrays when the pair is absent: loads read 0)
+            const uint32_t vR = lane_ok ? e0 * PR::esize : RB_ABSENT;
+            const uint32_t vQ = lane_ok ? e0 * 4u : RB_ABSENT;
+            const uint32_t vC = lane_ok ? e0 * PC::esize : RB_ABSENT;
+            const uint32_t vV = lane_ok ? e0 * PV::esize : RB_ABSENT;
+            const uint32_t vZ = lane_ok ? ((uint32_t)(gx >> 1) + (uint32_t)(gy >> 1) * csy) * RbBuf<VT>::size
+                                        : RB_ABSENT;  // parent (SMOOTH)
+            const bool own_x = lx >= 1 && lx <= RBP_LX - 2 && xa < g.nx;
+            const bool own_y = ry >= 2 && ry <= RBP_NY - 3 && y < g.ny;
+            const bool own = own_x && own_y;
+            // ring slots count planes from the chunk's first staged plane za - 2
+            float U[3], UB[3], W[3], T0[3], T1[3];
+            typename PC::raw BX[3], BY[3], BZ[3], DD[3];
+            typename PV::raw BS[3];  // coarse smoother with a base: the base pair of the plane
+            float sa = 0.0f;
+            auto load = [&](int p) {
+                Raw o;
+                const int pz = rb_coord(p, g.nz, perz);
+                const uint32_t so = pz >= 0 ? sz * (uint32_t)pz : 0u;
+                o.r = PR::ld(Rr, vR, so * PR::esize);
+                if constexpr (upd) o.q = RbP<float>::ld(Rq, vQ, so * 4u);
+                o.d = PC::ld(Rd, vC, so * PC::esize);
+                o.bx = PC::ld(Rx, vC, so * PC::esize);
+                o.by = PC::ld(Ry, vC, so * PC::esize);
+                o.bz = PC::ld(Rz, vC, so * PC::esize);
+                if constexpr (MODE == RB_SMOOTH) {
+                    o.zc = RbBuf<VT>::ld(Rc, vZ, (pz >= 0 ? csz * (uint32_t)(pz >> 1) : 0u) * RbBuf<VT>::size);
+                    if (COARSE && A.base) o.base = PV::ld(Rb, vV, so * PV::esize);
+                }
+                return o;
+            };
+            // loads run two planes ahead of the plane being staged, into a ring slot of their own
+            // (a register copy of a load in flight would wait for it)
+            Raw RW[3];
+            RW[0] = load(za - 2);
+            RW[1] = load(za - 1);
+            auto step = [&](int p, auto S0c) {
+                constexpr int s0 = decltype(S0c)::value, s1 = (s0 + 2) % 3, s2 = (s0 + 1) % 3;
+                // slot of plane p - 1 (staged already); issued on every step, also past the chunk's
+                // last staged plane, so that the wait for this step's loads never depends on a branch
+                RW[s1] = load(p + 2);
+                const Raw& cur = RW[s0];
+                // red position of the pair on this row at plane p (and the black position at
+                // p - 1, the red position at p - 2)
+                const bool pr = ((y + p) & 1) != 0;
+                // ---- stage 1: plane p
+                {
+                    const bool pok = rb_coord(p, g.nz, perz) >= 0;  // uniform
+                    const typename PC::raw dr = pok ? cur.d : PC::zero();
+                    const float d0 = PC::lo(dr), d1 = PC::hi(dr);
+                    float rt0, rt1;
+                    if constexpr (upd) {
+                        const float q0 = __uint_as_float(cur.q.x), q1 = __uint_as_float(cur.q.y);
+                        const float rp0 = d0 != 0.0f ? (PR::lo(cur.r) - m) - alpha * q0 : 0.0f;
+                        const float rp1 = d1 != 0.0f ? (PR::hi(cur.r) - m) - alpha * q1 : 0.0f;
+                        if (own && p >= za && p < zb) {
+                            RbP<float>::st(Rw, rp0, rp1, vQ, sz * (uint32_t)p * 4u);
+                            acc[1] += (double)rp0 * (double)rp0 + (double)rp1 * (double)rp1;
+                            acc[3] += (double)rp0 + (double)rp1;
+                        }
+                        rt0 = rp0 - mn;
+                        rt1 = rp1 - mn;
+                    } else {
+                        rt0 = PR::lo(cur.r) - m;
+                        rt1 = PR::hi(cur.r) - m;
+                    }
+                    const bool ok = lane_ok && pok;
+                    rt0 = ok ? rt0 : 0.0f;
+                    rt1 = ok ? rt1 : 0.0f;
+                    float u0 = d0 * rt0, u1 = d1 * rt1;
+                    if constexpr (MODE == RB_SMOOTH) {
+                        if (COARSE && A.base) BS[s0] = cur.base;
+                        const float zcv = ok ? ps * RbBuf<VT>::f(cur.zc) : 0.0f;
+                        if (pr) u1 += d1 != 0.0f ? zcv : 0.0f;
+                        else u0 += d0 != 0.0f ? zcv : 0.0f;
+                        T0[s0] = rt0;
+                        T1[s0] = rt1;
+                    }
+                    U[s0] = pr ? u1 : u0;
+                    UB[s0] = pr ? u0 : u1;
+                    DD[s0] = dr;
+                    BX[s0] = pok ? cur.bx : PC::zero();
+                    BY[s0] = pok ? cur.by : PC::zero();
+                    BZ[s0] = pok ? cur.bz : PC::zero();
+                    sU[p & 1][ry][lx] = U[s0];
+                    sB[s0][ry][lx] = BY[s0];
+                }
+                // ---- stage 2: black cells of plane c = p - 1 (position pr)
+                if (p >= za) {
+                    const int c = p - 1;
+                    const float uyl = sU[c & 1][ryl][lx], uyh = sU[c & 1][ryh][lx];
+                    const typename PC::raw byh = sB[s1][ryh][lx];
+                    const float uc = U[s1];
+                    const float up = rb_prev_lane(uc), un = rb_next_lane(uc);
+                    const float bxl = PC::lo(BX[s1]), bxh = PC::hi(BX[s1]);
+                    const float bxn = rb_next_lane(bxl);
+                    float s = pr ? bxh * uc : bxl * up;
+                    s += pr ? bxn * un : bxh * uc;
+                    s += rbp_sel<CT>(BY[s1], pr) * uyl;
+                    s += rbp_sel<CT>(byh, pr) * uyh;
+                    s += rbp_sel<CT>(BZ[s1], pr) * U[s2];
+                    s += rbp_sel<CT>(BZ[s0], pr) * U[s0];
+                    W[s1] = UB[s1] + rbp_sel<CT>(DD[s1], pr) * s;
+                    sW[c & 1][ry][lx] = W[s1];
+                }
+                // ---- stage 3: red cells of plane e = p - 2 (position pr); black values of e - 1
+                // in slot s0, e in s2, e + 1 in s1
+                const int e = p - 2;
+                if (e >= za) {
+                    const float wyl = sW[e & 1][ryl][lx], wyh = sW[e & 1][ryh][lx];
+                    const typename PC::raw byh = sB[s2][ryh][lx];
+                    const float wc = W[s2];
+                    const float wp = rb_prev_lane(wc), wn = rb_next_lane(wc);
+                    const float bxl = PC::lo(BX[s2]), bxh = PC::hi(BX[s2]);
+                    const float bxn = rb_next_lane(bxl);
+                    float s = pr ? bxh * wc : bxl * wp;
+                    s += pr ? bxn * wn : bxh * wc;
+                    s += rbp_sel<CT>(BY[s2], pr) * wyl;
+                    s += rbp_sel<CT>(byh, pr) * wyh;
+                    s += rbp_sel<CT>(BZ[s2], pr) * W[s0];
+                    s += rbp_sel<CT>(BZ[s1], pr) * W[s1];
+                    const float d = rbp_sel<CT>(DD[s2], pr);
+                    if constexpr (restrict_pass) {
+                        sa += (own && d != 0.0f) ? s : 0.0f;
+                        if ((e & 1) == 1 || e == g.nz - 1) {
+                            const float tot = sa + __shfl_xor(sa, 32, 64);  // the row pair (y even, y + 1)
+                            if (lane < 32 && own_x && ry >= 2 && ry <= RBP_NY - 3 && y < g.ny)
+                                RbBuf<VT>::st(Ro, tot, ((uint32_t)(xa >> 1) + csy * (uint32_t)(y >> 1)) * RbBuf<VT>::size,
+                                              csz * (uint32_t)(e >> 1) * RbBuf<VT>::size);
+                            sa = 0.0f;
+                        }
+                    } else {
+                        const float rtr = pr ? T1[s2] : T0[s2];
+                        const float tr = d * rtr + d * s;
+                        float o0 = pr ? wc : tr, o1 = pr ? tr : wc;
+                        if (own) {
+                            const uint32_t so = sz * (uint32_t)e * PV::esize;
+                            if (COARSE && A.base) {
+                                o0 = PV::lo(BS[s2]) + A.theta * o0;
+                                o1 = PV::hi(BS[s2]) + A.theta * o1;
+                            }
+                            PV::st(Ro, o0, o1, vV, so);
+                            if constexpr (!COARSE) {
+                                const typename PC::raw de = DD[s2];
+                                if (PC::lo(de) != 0.0f) dot[0] += (double)T0[s2] * (double)o0;
+                                if (PC::hi(de) != 0.0f) dot[0] += (double)T1[s2] * (double)o1;
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+            };
+            for (int p = za - 2; p <= zb + 1; p += 3) {
+                step(p, IC<0>());
+                if (p + 1 <= zb + 1) step(p + 1, IC<1>());
+                if (p + 2 <= zb + 1) step(p + 2, IC<2>());
+            }
+        }
+        if constexpr (!COARSE) {
+            if constexpr (upd) reduce_finish<4, RBP_THREADS>(vg, acc, A.part, sc, 3, FIN_RESTRICT_UPD);
+            else if constexpr (MODE == RB_SMOOTH) reduce_finish<1, RBP_THREADS>(vg, dot, A.part, sc, 4, A.mode);
+        }
+    }
+};
+
 // ---------------------------------------------------------------- launchers
 
 bool rb_level_ok(const Grid3& g) {
@@ -379,10 +671,51 @@ static uint32_t rbw_zlen(const Grid3& g) {
     return (uint32_t)(z < 8 ? 8 : (z > 64 ? 64 : z));
 }
 
+// Planes per z-chunk of the pair-column kernel: about two thousand work items where the level is
+// large enough (each item re-reads two planes above and below its chunk); even, 8..64.
+static uint32_t rbp_zlen(const Grid3& g) {
+    static const uint32_t target = [] {
+        const char* e = getenv("MHDP_RBP_ITEMS");
+        const long v = e ? strtol(e, nullptr, 10) : 0;
+        return (uint32_t)(v >= 64 && v <= 65536 ? v : 2048);
+    }();
+    const uint64_t tiles = (uint64_t)((g.nx + RBP_OUTX - 1) / RBP_OUTX) * (uint64_t)((g.ny + RBP_OUTY - 1) / RBP_OUTY);
+    uint64_t z = (uint64_t)g.nz * tiles / target;
+    z &= ~(uint64_t)1;
+    return (uint32_t)(z < 8 ? 8 : (z > 64 ? 64 : z));
+}
+
+// The pair-column kernel takes levels whose rows hold whole pairs (nx even); MHDP_RB_KERNEL=strip
+// keeps the wave-strip kernel everywhere.
+static bool rbp_enabled() {
+    static const bool on = [] {
+        const char* e = getenv("MHDP_RB_KERNEL");
+        return !(e && e[0] == 's');
+    }();
+    return on;
+}
+
 template <class CT, class VT, class RT, int MODE, bool COARSE>
 static void rb_go(hipStream_t s, const typename K_rb<CT, VT, RT, MODE, COARSE, RBW_ROWS>::Args& a0) {
     using K = K_rb<CT, VT, RT, MODE, COARSE, RBW_ROWS>;
     typename K::Args a = a0;
+    if (rbp_enabled() && (a.g.nx & 1) == 0) {
+        using KP = K_rbp<CT, VT, RT, MODE, COARSE>;
+        a.zlen = rbp_zlen(a.g);
+        const uint32_t tiles = (uint32_t)((a.g.nx + RBP_OUTX - 1) / RBP_OUTX) * (uint32_t)((a.g.ny + RBP_OUTY - 1) / RBP_OUTY);
+        const uint32_t items = tiles * (((uint32_t)a.g.nz + a.zlen - 1) / a.zlen);
+        // one resident block per CU (the registers allow one 1024-thread block): the blocks march
+        // in step, so the halo rows and columns one tile shares with the next are still in L2
+        static const uint32_t cap = [] {
+            const char* e = getenv("MHDP_RBP_BLOCKS");
+            const long v = e ? strtol(e, nullptr, 10) : 0;
+            return (uint32_t)(v >= 8 && v <= 4096 ? v : 256);
+        }();
+        uint32_t blocks = items < cap ? items : cap;
+        if (blocks >= 64) blocks = (blocks + MHDP_XCDS - 1) / MHDP_XCDS * MHDP_XCDS;
+        go<KP>(s, blocks, a);
+        return;
+    }
     a.zlen = rbw_zlen(a.g);
     const uint32_t tiles = (uint32_t)((a.g.nx + RBW_OUT - 1) / RBW_OUT) * (uint32_t)((a.g.ny + RBW_ROWS - 1) / RBW_ROWS);
     const uint32_t items = tiles * (((uint32_t)a.g.nz + a.zlen - 1) / a.zlen);

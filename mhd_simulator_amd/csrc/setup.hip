@@ -285,6 +285,7 @@ struct K_k_mg_build {
         float* creact;
         float* cdiag;
         float* cdinv;
+        __hip_bfloat16* c16[5];  // optional BF16 copies of bx, by, bz, react, dinv (null: none)
     };
     static __device__ void run(const Args& A_, const VGrid& vg) {
         Grid3 cg = A_.cg;
@@ -368,12 +369,20 @@ struct K_k_mg_build {
                     if (!(f & 1)) c4[f >> 1] += beta;
                 }
             }
+            const float dinv = diag > 0.0f ? 1.0f / diag : 0.0f;
             cbx[i] = c4[0];
             cby[i] = c4[1];
             cbz[i] = c4[2];
             creact[i] = c4[3];
             cdiag[i] = diag;
-            cdinv[i] = diag > 0.0f ? 1.0f / diag : 0.0f;
+            cdinv[i] = dinv;
+            if (A_.c16[0]) {
+                A_.c16[0][i] = __float2bfloat16(c4[0]);
+                A_.c16[1][i] = __float2bfloat16(c4[1]);
+                A_.c16[2][i] = __float2bfloat16(c4[2]);
+                A_.c16[3][i] = __float2bfloat16(c4[3]);
+                A_.c16[4][i] = __float2bfloat16(dinv);
+            }
         }
     }
 };
@@ -1104,11 +1113,16 @@ bool launch_correct_f64(hipStream_t s, const Grid3& g, float h, const float* bx,
 }
 void launch_mg_build(hipStream_t s, const Grid3& cg, const Grid3& fg, bool fine, int meanfree, const uint32_t* mask,
                      const float* fdinv, const float* fbx, const float* fby, const float* fbz, const float* freact,
-                     float* cbx, float* cby, float* cbz, float* creact, float* cdiag, float* cdinv) {
+                     float* cbx, float* cby, float* cbz, float* creact, float* cdiag, float* cdinv,
+                     void* const c16[5]) {
+    __hip_bfloat16* o16[5];
+    for (int k = 0; k < 5; k++) o16[k] = c16 ? (__hip_bfloat16*)c16[k] : nullptr;
     if (fine)
-        go<K_k_mg_build<true>>(s, grid_blocks(cg.n), {cg, fg, meanfree, mask, fdinv, fbx, fby, fbz, freact, cbx, cby, cbz, creact, cdiag, cdinv});
+        go<K_k_mg_build<true>>(s, grid_blocks(cg.n), {cg, fg, meanfree, mask, fdinv, fbx, fby, fbz, freact, cbx, cby, cbz, creact, cdiag, cdinv,
+                                                      {o16[0], o16[1], o16[2], o16[3], o16[4]}});
     else
-        go<K_k_mg_build<false>>(s, grid_blocks(cg.n), {cg, fg, meanfree, mask, fdinv, fbx, fby, fbz, freact, cbx, cby, cbz, creact, cdiag, cdinv});
+        go<K_k_mg_build<false>>(s, grid_blocks(cg.n), {cg, fg, meanfree, mask, fdinv, fbx, fby, fbz, freact, cbx, cby, cbz, creact, cdiag, cdinv,
+                                                       {o16[0], o16[1], o16[2], o16[3], o16[4]}});
 }
 void launch_to_bf16(hipStream_t s, uint32_t n, const float* src, void* dst) {
     go<K_k_to_bf16>(s, grid_blocks(n), {n, src, (__hip_bfloat16*)dst});
