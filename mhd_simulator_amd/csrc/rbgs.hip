@@ -410,15 +410,18 @@ __device__ __forceinline__ float rbp_sel(typename RbP<T>::raw r, bool hi) {
 }
 
 constexpr int RBP_LX = 32;                 // pairs per tile row (a half wave)
-constexpr int RBP_NY = 32;                 // rows per tile (two per wave)
 constexpr int RBP_OUTX = 2 * (RBP_LX - 2);  // 60 output columns
-constexpr int RBP_OUTY = RBP_NY - 4;        // 28 output rows
-constexpr int RBP_THREADS = RBP_LX * RBP_NY;
 
-template <class CT, class VT, class RT, int MODE, bool COARSE>
+// NY: rows per tile (two per wave; NY - 4 output rows): 32 (1024 threads, one block per CU) for
+// large levels, 16 (512 threads, two blocks per CU) for small ones, where a block's serial
+// z-march is the latency to hide.
+template <class CT, class VT, class RT, int MODE, bool COARSE, int NY>
 struct K_rbp {
+    static constexpr int RBP_NY = NY;
+    static constexpr int RBP_OUTY = NY - 4;
+    static constexpr int RBP_THREADS = RBP_LX * NY;
     static constexpr int THREADS = RBP_THREADS;
-    static constexpr int MINB = 1;
+    static constexpr int MINB = NY == 32 ? 1 : 2;
     using Args = typename K_rb<CT, VT, RT, MODE, COARSE, 4>::Args;
     using PC = RbP<CT>;
     using PR = RbP<RT>;
@@ -671,18 +674,27 @@ static uint32_t rbw_zlen(const Grid3& g) {
     return (uint32_t)(z < 8 ? 8 : (z > 64 ? 64 : z));
 }
 
-// Planes per z-chunk of the pair-column kernel: about two thousand work items where the level is
-// large enough (each item re-reads two planes above and below its chunk); even, 8..64.
-static uint32_t rbp_zlen(const Grid3& g) {
-    static const uint32_t target = [] {
-        const char* e = getenv("MHDP_RBP_ITEMS");
-        const long v = e ? strtol(e, nullptr, 10) : 0;
-        return (uint32_t)(v >= 64 && v <= 65536 ? v : 2048);
-    }();
-    const uint64_t tiles = (uint64_t)((g.nx + RBP_OUTX - 1) / RBP_OUTX) * (uint64_t)((g.ny + RBP_OUTY - 1) / RBP_OUTY);
-    uint64_t z = (uint64_t)g.nz * tiles / target;
-    z &= ~(uint64_t)1;
-    return (uint32_t)(z < 8 ? 8 : (z > 64 ? 64 : z));
+// Planes per z-chunk of the pair-column kernel.  A coarse level's time is its blocks' serial
+// marches: the even chunk length in 4..64 with the fewest plane steps per block (rounds of work
+// items times the chunk plus the two planes staged below and above it).  The fine level streams
+// HBM: about two thousand items (even, 8..64), which keeps the blocks in step and the chunk ends'
+// re-read planes few.
+static uint32_t rbp_zlen(const Grid3& g, int outy, uint32_t blocks, bool coarse) {
+    const uint64_t tiles = (uint64_t)((g.nx + RBP_OUTX - 1) / RBP_OUTX) * (uint64_t)((g.ny + outy - 1) / outy);
+    if (!coarse) {
+        uint64_t z = (uint64_t)g.nz * tiles / 2048u;
+        z &= ~(uint64_t)1;
+        return (uint32_t)(z < 8 ? 8 : (z > 64 ? 64 : z));
+    }
+    uint32_t best = 4;
+    uint64_t best_steps = ~(uint64_t)0;
+    for (uint32_t z = 4; z <= 64; z += 2) {
+        const uint64_t items = tiles * (((uint64_t)g.nz + z - 1) / z);
+        const uint64_t steps = (items + blocks - 1) / blocks * (z + 4);
+        if (steps < best_steps) best_steps = steps, best = z;
+        if (z >= (uint32_t)g.nz) break;
+    }
+    return best;
 }
 
 // The pair-column kernel takes levels whose rows hold whole pairs (nx even); MHDP_RB_KERNEL=strip
@@ -695,25 +707,29 @@ static bool rbp_enabled() {
     return on;
 }
 
+// Coarse levels of fewer cells than this run the 16-row tiles.
+constexpr uint32_t kRbpSmallCells = 1u << 25;
+
+template <class KP, bool COARSE, class A>
+static void rbp_launch(hipStream_t s, A a) {
+    // the resident blocks (one or two per CU): they march in step, so the halo rows and columns
+    // one tile shares with the next are still in L2
+    const uint32_t cap = 256u * (uint32_t)KP::MINB;
+    a.zlen = rbp_zlen(a.g, KP::RBP_OUTY, cap, COARSE);
+    const uint32_t tiles = (uint32_t)((a.g.nx + RBP_OUTX - 1) / RBP_OUTX) * (uint32_t)((a.g.ny + KP::RBP_OUTY - 1) / KP::RBP_OUTY);
+    const uint32_t items = tiles * (((uint32_t)a.g.nz + a.zlen - 1) / a.zlen);
+    uint32_t blocks = items < cap ? items : cap;
+    if (blocks >= 64) blocks = (blocks + MHDP_XCDS - 1) / MHDP_XCDS * MHDP_XCDS;
+    go<KP>(s, blocks, a);
+}
+
 template <class CT, class VT, class RT, int MODE, bool COARSE>
 static void rb_go(hipStream_t s, const typename K_rb<CT, VT, RT, MODE, COARSE, RBW_ROWS>::Args& a0) {
     using K = K_rb<CT, VT, RT, MODE, COARSE, RBW_ROWS>;
     typename K::Args a = a0;
     if (rbp_enabled() && (a.g.nx & 1) == 0) {
-        using KP = K_rbp<CT, VT, RT, MODE, COARSE>;
-        a.zlen = rbp_zlen(a.g);
-        const uint32_t tiles = (uint32_t)((a.g.nx + RBP_OUTX - 1) / RBP_OUTX) * (uint32_t)((a.g.ny + RBP_OUTY - 1) / RBP_OUTY);
-        const uint32_t items = tiles * (((uint32_t)a.g.nz + a.zlen - 1) / a.zlen);
-        // one resident block per CU (the registers allow one 1024-thread block): the blocks march
-        // in step, so the halo rows and columns one tile shares with the next are still in L2
-        static const uint32_t cap = [] {
-            const char* e = getenv("MHDP_RBP_BLOCKS");
-            const long v = e ? strtol(e, nullptr, 10) : 0;
-            return (uint32_t)(v >= 8 && v <= 4096 ? v : 256);
-        }();
-        uint32_t blocks = items < cap ? items : cap;
-        if (blocks >= 64) blocks = (blocks + MHDP_XCDS - 1) / MHDP_XCDS * MHDP_XCDS;
-        go<KP>(s, blocks, a);
+        if (COARSE && a.g.n < kRbpSmallCells) rbp_launch<K_rbp<CT, VT, RT, MODE, COARSE, 16>, COARSE>(s, a);
+        else rbp_launch<K_rbp<CT, VT, RT, MODE, COARSE, 32>, COARSE>(s, a);
         return;
     }
     a.zlen = rbw_zlen(a.g);
