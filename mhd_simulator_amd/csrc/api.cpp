@@ -794,14 +794,18 @@ int project_once(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, const Cycl
         s.tau = tau;
         s.guard = preferred ? 1u : 0u;
         if ((rc = upload_scal(c, s))) return rc;
-        launch_rhs_f64(c->stream, c->g, c->grid.h, c->mask, c->bx, c->by, c->bz, ustar, c->b64);
-        // a warm start is the refinement begun from the previous potential (the FP64 residual
-        // then sees exactly b - L phi_old; faces are corrected from u* with the full potential)
-        if (warm) launch_old_potential(c->stream, c->g.n, c->mask, c->phi_prev, c->x64);
         // The first refinement step's FP64 residual decides whether the optional multigrid is
         // needed at all (a zero or already-met residual never allocates it, mod.rs:1031-1043).
-        launch_outer_start(c->stream, c->g, c->x64, c->b64, c->bx, c->by, c->bz, c->dinv, c->r, c->x, c->p, c->p2,
-                           c->part, c->sc, !warm);
+        // From a zero potential it is the right-hand side itself: one pass forms both.
+        if (warm || !launch_rhs_start(c->stream, c->g, c->grid.h, c->mask, c->bx, c->by, c->bz, ustar, c->b64,
+                                      c->dinv, c->r, c->x, c->p, c->p2, c->part, c->sc)) {
+            launch_rhs_f64(c->stream, c->g, c->grid.h, c->mask, c->bx, c->by, c->bz, ustar, c->b64);
+            // a warm start is the refinement begun from the previous potential (the FP64 residual
+            // then sees exactly b - L phi_old; faces are corrected from u* with the full potential)
+            if (warm) launch_old_potential(c->stream, c->g.n, c->mask, c->phi_prev, c->x64);
+            launch_outer_start(c->stream, c->g, c->x64, c->b64, c->bx, c->by, c->bz, c->dinv, c->r, c->x, c->p,
+                               c->p2, c->part, c->sc, !warm);
+        }
         if ((rc = read_scalars(c))) return rc;
         const double r0 = c->sc_h->rr64, b0 = c->sc_h->bb64;
         if (!std::isfinite(r0) || !std::isfinite(b0)) return fail(c, MHDP_NONFINITE, "Nonfinite fluid pressure residual");
@@ -1102,6 +1106,7 @@ int project_lockstep(mhdp_ctx* const* ctxs, int count, const float* const* ustar
         bool running = false, done = false, inner = false;
         uint32_t outer = 0, iterations = 0, it = 0;
         bool checked = false, breakdown = false;
+        bool started = false;  // the set-up pass also ran the first step's residual pass
         double res64 = 0.0;
     };
     std::vector<PS> ps((size_t)n);
@@ -1164,7 +1169,10 @@ int project_lockstep(mhdp_ctx* const* ctxs, int count, const float* const* ustar
                 mhdp_ctx* c = ctxs[p];
                 PS& q = ps[(size_t)p];
                 prepare(c, q.S.meanfree, true);
-                launch_rhs_f64(s, c->g, c->grid.h, c->mask, c->bx, c->by, c->bz, ustar[p], c->b64);
+                // the first refinement step starts from a zero potential: its residual pass is the RHS pass
+                q.started = launch_rhs_start(s, c->g, c->grid.h, c->mask, c->bx, c->by, c->bz, ustar[p], c->b64,
+                                             c->dinv, c->r, c->x, c->p, c->p2, c->part, c->sc);
+                if (!q.started) launch_rhs_f64(s, c->g, c->grid.h, c->mask, c->bx, c->by, c->bz, ustar[p], c->b64);
                 build_mg(c, q.S.meanfree ? 1 : 0, true);
                 q.S.mg = true;
                 q.S.lv = level_view(c);
@@ -1219,8 +1227,9 @@ int project_lockstep(mhdp_ctx* const* ctxs, int count, const float* const* ustar
                     if (!first_round)
                         launch_outer_update(s, c->g, c->x64, c->x, c->p, c->p2, c->dinv, c->sc, q.S.meanfree,
                                             q.outer == 1);
-                    launch_outer_start(s, c->g, c->x64, c->b64, c->bx, c->by, c->bz, c->dinv, c->r, c->x, c->p,
-                                       c->p2, c->part, c->sc, q.outer == 0);
+                    if (!(first_round && q.started))
+                        launch_outer_start(s, c->g, c->x64, c->b64, c->bx, c->by, c->bz, c->dinv, c->r, c->x,
+                                           c->p, c->p2, c->part, c->sc, q.outer == 0);
                     mg_initialize(c, q.S);
                 }))
                 return MHDP_NO_MEMORY;

@@ -29,6 +29,11 @@ bool launch_coefficients(hipStream_t s, const Grid3& g, const PhaseP& ph, const 
                          void* by16 = nullptr, void* bz16 = nullptr, void* dinv16 = nullptr);
 void launch_rhs_f32(hipStream_t s, const Grid3& g, float h, const uint32_t* mask, const float* bx, const float* by,
                     const float* bz, const float* ustar, float* b);
+// launch_rhs_f64 and launch_outer_start(zero_x) in one pass (rows of four cells: nx % 4 == 0;
+// returns false and launches nothing otherwise)
+bool launch_rhs_start(hipStream_t s, const Grid3& g, float h, const uint32_t* mask, const float* bx, const float* by,
+                      const float* bz, const float* ustar, double* b, const float* dinv, float* r32, float* x32,
+                      float* p0, float* p1, double* part, Scal* sc);
 void launch_rhs_f64(hipStream_t s, const Grid3& g, float h, const uint32_t* mask, const float* bx, const float* by,
                     const float* bz, const float* ustar, double* b);
 void launch_correct_f32(hipStream_t s, const Grid3& g, float h, const float* bx, const float* by, const float* bz,

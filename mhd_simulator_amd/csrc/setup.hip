@@ -506,8 +506,72 @@ struct K_k_coef4 {
     }
 };
 
-// rhs() (solver.wgsl:104-107) on four cells of a row: faces of the x plane are read as five
-// scalars (its rows hold nx+1 faces), the y and z planes as 16-byte rows.
+// rhs() (solver.wgsl:104-107) on the four cells of row r, out[k] = -h^2 div: faces of the x plane
+// are read as five scalars (its rows hold nx+1 faces), the y and z planes as 16-byte rows.
+template <class T>
+__device__ __forceinline__ void rhs_row4(const Grid3& g, float h, const Row4& r, const uint32_t* __restrict__ mask,
+                                         const float* __restrict__ bx, const float* __restrict__ by,
+                                         const float* __restrict__ bz, const float* __restrict__ U,
+                                         const float* __restrict__ V, const float* __restrict__ W, T (&out)[4]) {
+    uint32_t m[4];
+    ld4u(mask, r.i0, m);
+#pragma unroll
+    for (int k = 0; k < 4; k++) out[k] = (T)0;
+    if ((m[0] | m[1] | m[2] | m[3]) == 0u) return;
+    float xl[4], yl[4], zl[4], yu[4] = {0.f, 0.f, 0.f, 0.f}, zu[4] = {0.f, 0.f, 0.f, 0.f};
+    ld4f(bx, r.i0, xl);
+    ld4f(by, r.i0, yl);
+    ld4f(bz, r.i0, zl);
+    if (r.yp_ok) ld4f(by, r.yp, yu);
+    if (r.zp_ok) ld4f(bz, r.zp, zu);
+    const float xe = r.xp_ok ? bx[r.xp] : 0.0f;
+    // x plane: faces x0..x0+4 of this row (the last wraps to face 0 when periodic)
+    const uint32_t ux = (uint32_t)(g.nx + 1) * ((uint32_t)r.y + (uint32_t)g.ny * (uint32_t)r.z);
+    float uf[5];
+#pragma unroll
+    for (int k = 0; k < 4; k++) uf[k] = U[ux + (uint32_t)r.x0 + (uint32_t)k];
+    const bool wrapx = r.x0 + 4 >= g.nx && (g.per & 1u);
+    uf[4] = U[ux + (wrapx ? 0u : (uint32_t)r.x0 + 4u)];
+    // y plane rows y and y+1 (y+1 wraps to 0 when periodic)
+    const uint32_t Py = (uint32_t)g.ny + 1u;
+    const uint32_t vy = (uint32_t)r.x0 + (uint32_t)g.nx * ((uint32_t)r.y + Py * (uint32_t)r.z);
+    const int yu_i = r.y + 1 < g.ny ? r.y + 1 : ((g.per & 2u) ? 0 : r.y + 1);
+    const uint32_t vyu = (uint32_t)r.x0 + (uint32_t)g.nx * ((uint32_t)yu_i + Py * (uint32_t)r.z);
+    float vl[4], vu[4];
+    ld4f(V, vy, vl);
+    ld4f(V, vyu, vu);
+    // z plane rows z and z+1
+    const uint32_t sz = (uint32_t)g.nx * (uint32_t)g.ny;
+    const int zu_i = r.z + 1 < g.nz ? r.z + 1 : ((g.per & 4u) ? 0 : r.z + 1);
+    const uint32_t wz = (uint32_t)r.x0 + (uint32_t)g.nx * (uint32_t)r.y;
+    float wl[4], wu[4];
+    ld4f(W, wz + sz * (uint32_t)r.z, wl);
+    ld4f(W, wz + sz * (uint32_t)zu_i, wu);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (m[k] == 0u) continue;
+        const float bxu = k < 3 ? xl[k + 1] : xe;
+        T sum = (T)0;
+        {
+            const float up = bxu > 0.0f ? uf[k + 1] : 0.0f;
+            const float dn = xl[k] > 0.0f ? uf[k] : 0.0f;
+            sum += (T)up - (T)dn;
+        }
+        {
+            const float up = yu[k] > 0.0f ? vu[k] : 0.0f;
+            const float dn = yl[k] > 0.0f ? vl[k] : 0.0f;
+            sum += (T)up - (T)dn;
+        }
+        {
+            const float up = zu[k] > 0.0f ? wu[k] : 0.0f;
+            const float dn = zl[k] > 0.0f ? wl[k] : 0.0f;
+            sum += (T)up - (T)dn;
+        }
+        T div = sum / (T)h;
+        out[k] = -(T)h * (T)h * div;
+    }
+}
+
 template <class T>
 struct K_k_rhs4 {
     static constexpr int MINB = 0;
@@ -524,84 +588,74 @@ struct K_k_rhs4 {
         T* b;
     };
     static __device__ void run(const Args& A_, const VGrid& vg) {
-        Grid3 g = A_.g;
-        float h = A_.h;
-        const uint32_t* __restrict__ mask = A_.mask;
-        const float* __restrict__ bx = A_.bx;
-        const float* __restrict__ by = A_.by;
-        const float* __restrict__ bz = A_.bz;
-        const float* __restrict__ ustar = A_.ustar;
-        uint64_t off1 = A_.off1;
-        uint64_t off2 = A_.off2;
-        T* __restrict__ b = A_.b;
-        (void)vg;
+        const Grid3 g = A_.g;
         uint32_t lo, hi;
         block_range(vg, g.n >> 2, lo, hi);
-        const float* __restrict__ U = ustar;
-        const float* __restrict__ V = ustar + off1;
-        const float* __restrict__ W = ustar + off2;
+        const float* __restrict__ U = A_.ustar;
         for (uint32_t t = lo + threadIdx.x; t < hi; t += MHDP_BLOCK) {
             const Row4 r = row4(g, t);
-            uint32_t m[4];
-            ld4u(mask, r.i0, m);
-            T out[4] = {(T)0, (T)0, (T)0, (T)0};
-            if ((m[0] | m[1] | m[2] | m[3]) != 0u) {
-                float xl[4], yl[4], zl[4], yu[4] = {0.f, 0.f, 0.f, 0.f}, zu[4] = {0.f, 0.f, 0.f, 0.f};
-                ld4f(bx, r.i0, xl);
-                ld4f(by, r.i0, yl);
-                ld4f(bz, r.i0, zl);
-                if (r.yp_ok) ld4f(by, r.yp, yu);
-                if (r.zp_ok) ld4f(bz, r.zp, zu);
-                const float xe = r.xp_ok ? bx[r.xp] : 0.0f;
-                // x plane: faces x0..x0+4 of this row (the last wraps to face 0 when periodic)
-                const uint32_t ux = (uint32_t)(g.nx + 1) * ((uint32_t)r.y + (uint32_t)g.ny * (uint32_t)r.z);
-                float uf[5];
-    #pragma unroll
-                for (int k = 0; k < 4; k++) uf[k] = U[ux + (uint32_t)r.x0 + (uint32_t)k];
-                const bool wrapx = r.x0 + 4 >= g.nx && (g.per & 1u);
-                uf[4] = U[ux + (wrapx ? 0u : (uint32_t)r.x0 + 4u)];
-                // y plane rows y and y+1 (y+1 wraps to 0 when periodic)
-                const uint32_t Py = (uint32_t)g.ny + 1u;
-                const uint32_t vy = (uint32_t)r.x0 + (uint32_t)g.nx * ((uint32_t)r.y + Py * (uint32_t)r.z);
-                const int yu_i = r.y + 1 < g.ny ? r.y + 1 : ((g.per & 2u) ? 0 : r.y + 1);
-                const uint32_t vyu = (uint32_t)r.x0 + (uint32_t)g.nx * ((uint32_t)yu_i + Py * (uint32_t)r.z);
-                float vl[4], vu[4];
-                ld4f(V, vy, vl);
-                ld4f(V, vyu, vu);
-                // z plane rows z and z+1
-                const uint32_t sz = (uint32_t)g.nx * (uint32_t)g.ny;
-                const int zu_i = r.z + 1 < g.nz ? r.z + 1 : ((g.per & 4u) ? 0 : r.z + 1);
-                const uint32_t wz = (uint32_t)r.x0 + (uint32_t)g.nx * (uint32_t)r.y;
-                float wl[4], wu[4];
-                ld4f(W, wz + sz * (uint32_t)r.z, wl);
-                ld4f(W, wz + sz * (uint32_t)zu_i, wu);
-    #pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    if (m[k] == 0u) continue;
-                    const float bxu = k < 3 ? xl[k + 1] : xe;
-                    T sum = (T)0;
-                    {
-                        const float up = bxu > 0.0f ? uf[k + 1] : 0.0f;
-                        const float dn = xl[k] > 0.0f ? uf[k] : 0.0f;
-                        sum += (T)up - (T)dn;
-                    }
-                    {
-                        const float up = yu[k] > 0.0f ? vu[k] : 0.0f;
-                        const float dn = yl[k] > 0.0f ? vl[k] : 0.0f;
-                        sum += (T)up - (T)dn;
-                    }
-                    {
-                        const float up = zu[k] > 0.0f ? wu[k] : 0.0f;
-                        const float dn = zl[k] > 0.0f ? wl[k] : 0.0f;
-                        sum += (T)up - (T)dn;
-                    }
-                    T div = sum / (T)h;
-                    out[k] = -(T)h * (T)h * div;
-                }
-            }
-    #pragma unroll
-            for (int k = 0; k < 4; k++) b[r.i0 + k] = out[k];
+            T out[4];
+            rhs_row4<T>(g, A_.h, r, A_.mask, A_.bx, A_.by, A_.bz, U, U + A_.off1, U + A_.off2, out);
+#pragma unroll
+            for (int k = 0; k < 4; k++) A_.b[r.i0 + k] = out[k];
         }
+    }
+};
+
+// The FP64 right-hand side and the first refinement step's start from a zero potential in one
+// pass (K_k_outer_start4 with zero_x: the FP32 inner right-hand side r32 = b on active cells, the
+// inner x and both search directions zeroed, the partials of FIN_OUTER).  The sums are the
+// fused multiply-adds that pass performs.
+struct K_k_rhs_start4 {
+    static constexpr int MINB = 0;
+    struct Args {
+        Grid3 g;
+        float h;
+        const uint32_t* mask;
+        const float* bx;
+        const float* by;
+        const float* bz;
+        const float* ustar;
+        uint64_t off1;
+        uint64_t off2;
+        double* b;
+        const float* dinv;
+        float* r32;
+        float* x32;
+        float* p0;
+        float* p1;
+        double* part;
+        Scal* sc;
+    };
+    static __device__ void run(const Args& A_, const VGrid& vg) {
+        const Grid3 g = A_.g;
+        uint32_t lo, hi;
+        block_range(vg, g.n >> 2, lo, hi);
+        const float* __restrict__ U = A_.ustar;
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        const float zero4[4] = {0.f, 0.f, 0.f, 0.f};
+        for (uint32_t t = lo + threadIdx.x; t < hi; t += MHDP_BLOCK) {
+            const Row4 r = row4(g, t);
+            double bv[4];
+            rhs_row4<double>(g, A_.h, r, A_.mask, A_.bx, A_.by, A_.bz, U, U + A_.off1, U + A_.off2, bv);
+            st4d(A_.b, r.i0, bv);
+            float di[4];
+            ld4f(A_.dinv, r.i0, di);
+            float rv[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                rv[k] = di[k] != 0.0f ? (float)bv[k] : 0.0f;
+                acc[0] = __fma_rn((double)rv[k], (double)rv[k], acc[0]);
+                acc[1] = __fma_rn(bv[k], bv[k], acc[1]);
+                acc[2] = __fma_rn(bv[k], bv[k], acc[2]);
+                acc[3] += (double)rv[k];
+            }
+            st4f(A_.r32, r.i0, rv);
+            st4f(A_.x32, r.i0, zero4);
+            st4f(A_.p0, r.i0, zero4);
+            st4f(A_.p1, r.i0, zero4);
+        }
+        reduce_finish<4>(vg, acc, A_.part, A_.sc, 5, FIN_OUTER);
     }
 };
 
@@ -1077,6 +1131,15 @@ void launch_rhs_f32(hipStream_t s, const Grid3& g, float h, const uint32_t* mask
         go<K_k_rhs4<float>>(s, grid_blocks(g.n >> 2), {g, h, mask, bx, by, bz, ustar, f0, f0 + f1, b});
     else
         go<K_k_rhs<float>>(s, grid_blocks(g.n), {g, h, mask, bx, by, bz, ustar, f0, f0 + f1, b});
+}
+bool launch_rhs_start(hipStream_t s, const Grid3& g, float h, const uint32_t* mask, const float* bx, const float* by,
+                      const float* bz, const float* ustar, double* b, const float* dinv, float* r32, float* x32,
+                      float* p0, float* p1, double* part, Scal* sc) {
+    if (!row_vector_ok(g)) return false;
+    uint64_t f0 = (uint64_t)(g.nx + 1) * g.ny * g.nz, f1 = (uint64_t)g.nx * (g.ny + 1) * g.nz;
+    go<K_k_rhs_start4>(s, grid_blocks(g.n >> 2),
+                       {g, h, mask, bx, by, bz, ustar, f0, f0 + f1, b, dinv, r32, x32, p0, p1, part, sc});
+    return true;
 }
 void launch_rhs_f64(hipStream_t s, const Grid3& g, float h, const uint32_t* mask, const float* bx, const float* by,
                     const float* bz, const float* ustar, double* b) {
