@@ -463,13 +463,29 @@ void mg_initialize(mhdp_ctx* c, Solve& S) {  // multigrid.rs:287-291
     launch_pdot(c->stream, c->g, S.store, c->r, zf, c->dinv, c->p, c->part, c->sc, 1, FIN_INITIAL_MG);
 }
 
+// MHDP_STORED_MOBILITY=1: the operator apply loads the stored mobilities (comparison runs).
+bool formed_mobility_env() {
+    static const bool on = [] {
+        const char* e = std::getenv("MHDP_STORED_MOBILITY");
+        return !(e && e[0] == '1');
+    }();
+    return on;
+}
+
 void iterate_once(mhdp_ctx* c, Solve& S) {  // mod.rs:1057-1075, multigrid.rs:302-326
     hipStream_t s = c->stream;
     record_mark();
     if (S.fused) {
         c->timer.before(MHDP_K_APPLY, s);
         const bool brick = c->brick_enabled && brick_ok(c->g);
-        if (brick)
+        // the mobilities formed in the pass from the liquid fraction and the open bytes (the
+        // coefficient pass's values bit for bit) where both are at hand
+        const bool formed = brick && c->open8_fresh && (!c->phase.two_phase || c->has_alpha) && formed_mobility_env();
+        if (formed)
+            launch_brick_apply_m(s, c->g, S.store, S.lv[0].t, c->p, c->p2, c->x,
+                                 c->phase.two_phase ? c->alpha : nullptr, c->open8, c->phase.rho_liquid,
+                                 c->phase.rho_air, c->q, c->part, c->sc, S.meanfree ? 1 : 0);
+        else if (brick)
             launch_brick_apply(s, c->g, S.store, S.lv[0].t, c->p, c->p2, c->x, c->bx, c->by, c->bz,
                                S.store == STORE_BF16 ? c->dinv16 : (const void*)c->dinv, c->q, c->part, c->sc,
                                S.meanfree ? 1 : 0);

@@ -410,6 +410,175 @@ struct K_k_brick_apply {
     }
 };
 
+// pass 1 with the face mobilities formed in the pass instead of loaded: from the liquid fraction
+// of each cell (two-phase) and the per-cell open-face bytes (bit a: the cell's lower face on axis a
+// is open, bit 3: fluid, bit 4: a component's anchor).  The operator is the stored one bit for bit:
+// density() and mobility() (solver.wgsl:40-51) in the coefficient pass's FP32 operations, rounded
+// one by one (no fused multiply-add), and a cell is active where the coefficient pass gives it a
+// nonzero reciprocal diagonal (fluid; in the pinned gauge not an anchor).  The pass reads 5 bytes
+// per cell of coefficients instead of 14 (three FP32 mobilities and the BF16 1/diag).
+struct MobP {
+    float rho_l, rho_air;
+};
+// (this file contracts multiply-adds elsewhere: these two forbid it)
+__device__ __forceinline__ float mob_dens(const MobP& m, float a) {
+#pragma clang fp contract(off)
+    return m.rho_air * (1.0f - a) + m.rho_l * a;
+}
+template <bool TWO>
+__device__ __forceinline__ float mob_face(const MobP& m, bool open, float dl, float dc) {
+#pragma clang fp contract(off)
+    if (!open) return 0.0f;
+    if (!TWO) return 1.0f;
+    const float fd = 0.5f * (dl + dc);
+    return m.rho_l / fd;
+}
+__device__ __forceinline__ uint32_t obyte(uint32_t w, int k) { return (w >> (8 * k)) & 0xFFu; }
+
+template <class VT, bool TWO, int MB_>
+struct K_k_brick_apply_m {
+    static constexpr int MINB = MB_;
+    struct Args {
+        Grid3 g;
+        const VT* z;
+        float* p0;
+        float* p1;
+        float* x;
+        const float* alpha;  // liquid fraction (TWO)
+        const uint8_t* open8;
+        MobP mp;
+        float* q;
+        double* part;
+        Scal* sc;
+        int meanfree;
+    };
+    static __device__ void run(const Args& A_, const VGrid& vg) {
+        const Grid3 g = A_.g;
+        const VT* __restrict__ z = A_.z;
+        float* __restrict__ x = A_.x;
+        const uint8_t* __restrict__ open8 = A_.open8;
+        const MobP mp = A_.mp;
+        float* __restrict__ q = A_.q;
+        Scal* sc = A_.sc;
+        const int meanfree = A_.meanfree;
+        if (CONVERGED(sc)) return;
+        const uint32_t it = (uint32_t)sc->iters;
+        const float* __restrict__ pold = (it & 1u) ? A_.p0 : A_.p1;
+        float* __restrict__ pnew = (it & 1u) ? A_.p1 : A_.p0;
+        const float beta = (float)sc->beta;
+        const bool pending = sc->pending != 0;
+        const float alpha = (float)sc->alpha;
+        const float pa = (pending && meanfree) ? pold[sc->anchor] : 0.0f;
+        uint32_t lo, hi;
+        block_range(vg, g.n >> 4, lo, hi);
+        double acc[2] = {0.0, 0.0};
+        for (uint32_t b = lo + threadIdx.x; b < hi; b += MHDP_BLOCK) {
+            const Geo o = geo(g, b);
+            Rows<VT> zr;
+            Rows<float> pr, ar;
+            R4<float> xr[2][2];
+            uint32_t ob[2][2], obx[2][2], oby[2], obz[2];
+            load_rows(o, z, zr);
+            load_rows(o, pold, pr);
+            if (TWO) load_rows(o, A_.alpha, ar);
+#pragma unroll
+            for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+                for (int dy = 0; dy < 2; dy++) {
+                    const uint32_t r = o.row[dz][dy];
+                    xr[dz][dy] = ldr4(x, r);
+                    ob[dz][dy] = *reinterpret_cast<const uint32_t*>(open8 + r);
+                    obx[dz][dy] = open8[r + (uint32_t)o.xp_off];
+                }
+#pragma unroll
+            for (int d = 0; d < 2; d++) {
+                oby[d] = *reinterpret_cast<const uint32_t*>(open8 + o.ypr[d]);
+                obz[d] = *reinterpret_cast<const uint32_t*>(open8 + o.zpr[d]);
+            }
+            fence_rows(zr);
+            fence_rows(pr);
+            if (TWO) fence_rows(ar);
+#pragma unroll
+            for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+                for (int dy = 0; dy < 2; dy++) {
+                    fence(xr[dz][dy]);
+                    asm volatile("" : "+v"(ob[dz][dy]), "+v"(obx[dz][dy]));
+                }
+#pragma unroll
+            for (int d = 0; d < 2; d++) asm volatile("" : "+v"(oby[d]), "+v"(obz[d]));
+            // densities of the brick and its face halo, then the mobilities of every face
+            Field dn;
+            if (TWO) field2(ar, ar, dn, [&](const F4& a, const F4&, int) {
+                    return F4{{mob_dens(mp, a.v[0]), mob_dens(mp, a.v[1]), mob_dens(mp, a.v[2]), mob_dens(mp, a.v[3])}};
+                }, [&](float a, float, int) { return mob_dens(mp, a); });
+            Wts w;
+#pragma unroll
+            for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+                for (int dy = 0; dy < 2; dy++) {
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const uint32_t bits = obyte(ob[dz][dy], k);
+                        const float dc = TWO ? dn.c[dz][dy][k] : 0.0f;
+                        const float dxm = TWO ? (k > 0 ? dn.c[dz][dy][k - 1] : dn.xm[dz][dy]) : 0.0f;
+                        const float dym = TWO ? (dy == 1 ? dn.c[dz][0][k] : dn.ym[dz][k]) : 0.0f;
+                        const float dzm = TWO ? (dz == 1 ? dn.c[0][dy][k] : dn.zm[dy][k]) : 0.0f;
+                        w.bx[dz][dy][k] = mob_face<TWO>(mp, bits & 1u, dxm, dc);
+                        w.by[dz][dy][k] = mob_face<TWO>(mp, bits & 2u, dym, dc);
+                        w.bz[dz][dy][k] = mob_face<TWO>(mp, bits & 4u, dzm, dc);
+                    }
+                    w.bxp[dz][dy] = mob_face<TWO>(mp, o.xp_ok && (obx[dz][dy] & 1u), TWO ? dn.c[dz][dy][3] : 0.0f,
+                                                  TWO ? dn.xp[dz][dy] : 0.0f);
+                }
+#pragma unroll
+            for (int d = 0; d < 2; d++)
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    w.byp[d][k] = mob_face<TWO>(mp, o.yp_ok && (obyte(oby[d], k) & 2u), TWO ? dn.c[d][1][k] : 0.0f,
+                                                TWO ? dn.yp[d][k] : 0.0f);
+                    w.bzp[d][k] = mob_face<TWO>(mp, o.zp_ok && (obyte(obz[d], k) & 4u), TWO ? dn.c[1][d][k] : 0.0f,
+                                                TWO ? dn.zp[d][k] : 0.0f);
+                }
+            Field f;
+            field2(
+                zr, pr, f,
+                [&](const F4& a, const F4& c, int) {
+                    return F4{{a.v[0] + beta * c.v[0], a.v[1] + beta * c.v[1], a.v[2] + beta * c.v[2], a.v[3] + beta * c.v[3]}};
+                },
+                [&](float a, float c, int) { return a + beta * c; });
+#pragma unroll
+            for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+                for (int dy = 0; dy < 2; dy++) {
+                    const uint32_t r = o.row[dz][dy];
+                    F4 qv, pn;
+                    bool act[4];
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const uint32_t bits = obyte(ob[dz][dy], k);
+                        act[k] = (bits & 8u) && (meanfree || !(bits & 16u));
+                        pn.v[k] = act[k] ? f.c[dz][dy][k] : 0.0f;
+                        qv.v[k] = act_at(w, f, dz, dy, k);
+                        acc[0] += (double)pn.v[k] * (double)qv.v[k];
+                        acc[1] += act[k] ? (double)qv.v[k] : 0.0;
+                    }
+                    st4(pnew, r, pn);
+                    st4(q, r, qv);
+                    if (pending) {
+                        const F4 po = un(pr.c[dz][dy]);
+                        F4 xv = un(xr[dz][dy]);
+#pragma unroll
+                        for (int k = 0; k < 4; k++)
+                            if (act[k]) xv.v[k] = xv.v[k] + alpha * (po.v[k] - pa);
+                        st4(x, r, xv);
+                    }
+                }
+        }
+        reduce_finish<2>(vg, acc, A_.part, sc, 2, FIN_ALPHA);
+    }
+};
+
 // pass 3 (k_fused_restrict on bricks): the two aggregates of a brick.  COARSE: a coarse level
 // (Galerkin graph with reaction term; every child contributes, multigrid.wgsl:75-83; no mean).
 template <class CT, class VT, class RT, bool COARSE, int MB_>
@@ -865,7 +1034,28 @@ void launch_brick_apply(hipStream_t s, const Grid3& g, int store, const void* z,
         BRICK_DISPATCH(k_brick_apply, float, g, (const float*)z, p0, p1, x, bx, by, bz, (const float*)dinv, q, part, sc,
                        meanfree);
 }
+// launch_brick_apply with the mobilities formed from the liquid fraction (null: single phase,
+// every open face 1) and the open-face bytes
+void launch_brick_apply_m(hipStream_t s, const Grid3& g, int store, const void* z, float* p0, float* p1, float* x,
+                          const float* alpha, const uint8_t* open8, float rho_l, float rho_air, float* q,
+                          double* part, Scal* sc, int meanfree) {
+    const dim3 B(MHDP_BLOCK), G(grid_blocks(g.n >> 4, brick_grid_cap()));
+    const MobP mp{rho_l, rho_air};
 #define COMMA ,
+    if (store == STORE_BF16) {
+        using T = __hip_bfloat16;
+        if (alpha)
+            BRICK_DISPATCH(k_brick_apply_m, T COMMA true, g, (const T*)z, p0, p1, x, alpha, open8, mp, q, part, sc, meanfree);
+        else
+            BRICK_DISPATCH(k_brick_apply_m, T COMMA false, g, (const T*)z, p0, p1, x, alpha, open8, mp, q, part, sc, meanfree);
+    } else {
+        using T = float;
+        if (alpha)
+            BRICK_DISPATCH(k_brick_apply_m, T COMMA true, g, (const T*)z, p0, p1, x, alpha, open8, mp, q, part, sc, meanfree);
+        else
+            BRICK_DISPATCH(k_brick_apply_m, T COMMA false, g, (const T*)z, p0, p1, x, alpha, open8, mp, q, part, sc, meanfree);
+    }
+}
 void launch_brick_restrict(hipStream_t s, int store, const Grid3& cg, const Grid3& g, const float* r, const void* dinv,
                            const void* bx, const void* by, const void* bz, void* rc, Scal* sc) {
     const dim3 B(MHDP_BLOCK), G(grid_blocks(g.n >> 4, brick_grid_cap()));

@@ -131,6 +131,11 @@ void launch_fused_smooth(hipStream_t s, int store, const Grid3& g, const Grid3& 
 bool brick_ok(const Grid3& g);
 bool brick_enabled_env();
 // dinv: the storage type's reciprocal diagonal (BF16 copy with STORE_BF16), tested against zero only
+// launch_brick_apply with the mobilities formed in the pass from the liquid fraction (null: single
+// phase) and the open-face bytes (bit 4: a component's anchor), equal to the stored ones bit for bit
+void launch_brick_apply_m(hipStream_t s, const Grid3& g, int store, const void* z, float* p0, float* p1, float* x,
+                          const float* alpha, const uint8_t* open8, float rho_l, float rho_air, float* q,
+                          double* part, Scal* sc, int meanfree);
 void launch_brick_apply(hipStream_t s, const Grid3& g, int store, const void* z, float* p0, float* p1, float* x,
                         const float* bx, const float* by, const float* bz, const void* dinv, float* q, double* part,
                         Scal* sc, int meanfree);

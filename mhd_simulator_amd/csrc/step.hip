@@ -182,7 +182,7 @@ __device__ __forceinline__ void centre_coords(const int pos[3], int base[3], flo
 }
 
 // open bytes: bit a of cell i set when i and its lower neighbour on axis a (wrapped) are fluid;
-// bit 3 set when i is fluid
+// bit 3 set when i is fluid, bit 4 when i is its component's anchor (gauge mask 2 | root << 2)
 __global__ void __launch_bounds__(MHDP_BLOCK) k_open8(Planes P, uint32_t n, const uint32_t* __restrict__ mask,
                                                       uint8_t* __restrict__ open8) {
     uint32_t lo, hi;
@@ -190,7 +190,7 @@ __global__ void __launch_bounds__(MHDP_BLOCK) k_open8(Planes P, uint32_t n, cons
     for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
         uint32_t bits = 0;
         if (mask[i]) {
-            bits = 8u;
+            bits = (mask[i] & 3u) == 2u ? 24u : 8u;
             const uint32_t nxy = (uint32_t)P.n[0] * (uint32_t)P.n[1];
             int pos[3];
             pos[2] = (int)(i / nxy);
