@@ -383,6 +383,16 @@ int convergence(mhdp_ctx* c, float* residual, bool* conv, uint32_t* iters) {
     return MHDP_OK;
 }
 
+// MHDP_STORED_MOBILITY=1: the operator apply and the FP64 residual pass load the stored mobilities
+// (comparison runs).
+bool formed_mobility_env() {
+    static const bool on = [] {
+        const char* e = std::getenv("MHDP_STORED_MOBILITY");
+        return !(e && e[0] == '1');
+    }();
+    return on;
+}
+
 struct Solve {
     bool meanfree = false;
     bool mg = false;
@@ -461,15 +471,6 @@ void mg_initialize(mhdp_ctx* c, Solve& S) {  // multigrid.rs:287-291
     launch_vcycle(c->stream, S.store, (int)S.lv.size(), S.lv.data(), c->r, c->mask, c->sc, S.coarse64, c->coarse_visits,
                   &zf);
     launch_pdot(c->stream, c->g, S.store, c->r, zf, c->dinv, c->p, c->part, c->sc, 1, FIN_INITIAL_MG);
-}
-
-// MHDP_STORED_MOBILITY=1: the operator apply loads the stored mobilities (comparison runs).
-bool formed_mobility_env() {
-    static const bool on = [] {
-        const char* e = std::getenv("MHDP_STORED_MOBILITY");
-        return !(e && e[0] == '1');
-    }();
-    return on;
 }
 
 void iterate_once(mhdp_ctx* c, Solve& S) {  // mod.rs:1057-1075, multigrid.rs:302-326
@@ -628,6 +629,21 @@ int upload_scal(mhdp_ctx* c, const Scal& s) {
     return MHDP_OK;
 }
 
+// The per-cell open-face bytes when they match the mask (null otherwise).
+const uint8_t* open8_of(const mhdp_ctx* c) { return c->open8_fresh ? c->open8 : nullptr; }
+
+// The FP64 residual pass that starts a refinement step (its stored-mobility form where the
+// mobilities cannot be formed in the pass).
+void outer_start(mhdp_ctx* c, hipStream_t s, bool meanfree, bool zero_x) {
+    const bool formable = (!c->phase.two_phase || c->has_alpha) && formed_mobility_env();
+    if (formable && launch_outer_start_m(s, c->g, c->x64, c->b64, c->phase.two_phase ? c->alpha : nullptr,
+                                         open8_of(c), c->phase.rho_liquid, c->phase.rho_air, meanfree, c->r, c->x,
+                                         c->p, c->p2, c->part, c->sc, zero_x))
+        return;
+    launch_outer_start(s, c->g, c->x64, c->b64, c->bx, c->by, c->bz, c->dinv, c->r, c->x, c->p, c->p2, c->part, c->sc,
+                       zero_x);
+}
+
 // FP64 iterative refinement over the fused multigrid path.  Each outer step is one residual
 // pass that also starts the inner solve on the device (inner tolerance, deferred mean-free
 // projection, zeroed x and search directions), the inner V-cycle PCG in batches (8, then as
@@ -649,8 +665,7 @@ int refine_fused(mhdp_ctx* c, Solve& S, const mhdp_opts* o, uint32_t max_outer, 
     for (bool first = true;; first = false) {
         // `started`: the caller has run the first step's residual pass and read its scalars
         if (!(first && started)) {
-            launch_outer_start(s, c->g, c->x64, c->b64, c->bx, c->by, c->bz, c->dinv, c->r, c->x, c->p, c->p2,
-                               c->part, c->sc, first && !warm);
+            outer_start(c, s, S.meanfree, first && !warm);
             int rc = read_scalars(c);
             if (rc) return rc;
         }
@@ -723,7 +738,7 @@ PhaseP phase_params(const mhdp_ctx* c, bool meanfree) {
 int prepare(mhdp_ctx* c, bool meanfree, bool want16 = false) {
     if (!c->cells) return fail(c, MHDP_INVALID, "Fluid cells have not been set");
     const bool h = want16 && c->mg16_alloc;
-    c->fine16_fresh = launch_coefficients(c->stream, c->g, phase_params(c, meanfree), c->mask,
+    c->fine16_fresh = launch_coefficients(c->stream, c->g, phase_params(c, meanfree), c->mask, open8_of(c),
                                           c->has_alpha ? c->alpha : nullptr, c->bx, c->by, c->bz, c->diag, c->dinv,
                                           h ? c->bx16 : nullptr, h ? c->by16 : nullptr, h ? c->bz16 : nullptr,
                                           h ? c->dinv16 : nullptr);
@@ -787,7 +802,7 @@ int project_once(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, const Cycl
         set_solve_params(c, s, meanfree, tol2, abs2, meanfree ? 1.0 : (double)nf, omega, pscale);
         s.guard = preferred ? 1u : 0u;
         if ((rc = upload_scal(c, s))) return rc;
-        launch_rhs_f32(c->stream, c->g, c->grid.h, c->mask, c->bx, c->by, c->bz, ustar, c->b);
+        launch_rhs_f32(c->stream, c->g, c->grid.h, c->mask, open8_of(c), c->bx, c->by, c->bz, ustar, c->b);
         rc = pcg(c, S, preferred, o->max_iter, true, &iterations, &residual);
         if (rc == MHDP_NOT_CONVERGED) {
             char buf[256];
@@ -813,14 +828,13 @@ int project_once(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, const Cycl
         // The first refinement step's FP64 residual decides whether the optional multigrid is
         // needed at all (a zero or already-met residual never allocates it, mod.rs:1031-1043).
         // From a zero potential it is the right-hand side itself: one pass forms both.
-        if (warm || !launch_rhs_start(c->stream, c->g, c->grid.h, c->mask, c->bx, c->by, c->bz, ustar, c->b64,
-                                      c->dinv, c->r, c->x, c->p, c->p2, c->part, c->sc)) {
-            launch_rhs_f64(c->stream, c->g, c->grid.h, c->mask, c->bx, c->by, c->bz, ustar, c->b64);
+        if (warm || !launch_rhs_start(c->stream, c->g, c->grid.h, open8_of(c), ustar, c->b64, meanfree, c->r, c->x,
+                                      c->p, c->p2, c->part, c->sc)) {
+            launch_rhs_f64(c->stream, c->g, c->grid.h, c->mask, open8_of(c), c->bx, c->by, c->bz, ustar, c->b64);
             // a warm start is the refinement begun from the previous potential (the FP64 residual
             // then sees exactly b - L phi_old; faces are corrected from u* with the full potential)
             if (warm) launch_old_potential(c->stream, c->g.n, c->mask, c->phi_prev, c->x64);
-            launch_outer_start(c->stream, c->g, c->x64, c->b64, c->bx, c->by, c->bz, c->dinv, c->r, c->x, c->p,
-                               c->p2, c->part, c->sc, !warm);
+            outer_start(c, c->stream, meanfree, !warm);
         }
         if ((rc = read_scalars(c))) return rc;
         const double r0 = c->sc_h->rr64, b0 = c->sc_h->bb64;
@@ -1186,9 +1200,10 @@ int project_lockstep(mhdp_ctx* const* ctxs, int count, const float* const* ustar
                 PS& q = ps[(size_t)p];
                 prepare(c, q.S.meanfree, true);
                 // the first refinement step starts from a zero potential: its residual pass is the RHS pass
-                q.started = launch_rhs_start(s, c->g, c->grid.h, c->mask, c->bx, c->by, c->bz, ustar[p], c->b64,
-                                             c->dinv, c->r, c->x, c->p, c->p2, c->part, c->sc);
-                if (!q.started) launch_rhs_f64(s, c->g, c->grid.h, c->mask, c->bx, c->by, c->bz, ustar[p], c->b64);
+                q.started = launch_rhs_start(s, c->g, c->grid.h, open8_of(c), ustar[p], c->b64, q.S.meanfree, c->r,
+                                             c->x, c->p, c->p2, c->part, c->sc);
+                if (!q.started)
+                    launch_rhs_f64(s, c->g, c->grid.h, c->mask, open8_of(c), c->bx, c->by, c->bz, ustar[p], c->b64);
                 build_mg(c, q.S.meanfree ? 1 : 0, true);
                 q.S.mg = true;
                 q.S.lv = level_view(c);
@@ -1244,8 +1259,7 @@ int project_lockstep(mhdp_ctx* const* ctxs, int count, const float* const* ustar
                         launch_outer_update(s, c->g, c->x64, c->x, c->p, c->p2, c->dinv, c->sc, q.S.meanfree,
                                             q.outer == 1);
                     if (!(first_round && q.started))
-                        launch_outer_start(s, c->g, c->x64, c->b64, c->bx, c->by, c->bz, c->dinv, c->r, c->x,
-                                           c->p, c->p2, c->part, c->sc, q.outer == 0);
+                        outer_start(c, s, q.S.meanfree, q.outer == 0);
                     mg_initialize(c, q.S);
                 }))
                 return MHDP_NO_MEMORY;
@@ -2154,7 +2168,7 @@ int mhdp_probe_rhs(mhdp_ctx* c, int meanfree, const float* ustar, double* b) {
     if ((rc = dalloc(c, (void**)&c->ustar_stage, fb)) || (rc = dalloc(c, (void**)&c->b64, (size_t)c->g.n * 8)))
         return rc;
     HIP_OK(hipMemcpyAsync(c->ustar_stage, ustar, fb, hipMemcpyHostToDevice, c->stream));
-    launch_rhs_f64(c->stream, c->g, c->grid.h, c->mask, c->bx, c->by, c->bz, c->ustar_stage, c->b64);
+    launch_rhs_f64(c->stream, c->g, c->grid.h, c->mask, open8_of(c), c->bx, c->by, c->bz, c->ustar_stage, c->b64);
     HIP_OK(hipMemcpyAsync(b, c->b64, (size_t)c->g.n * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
     return MHDP_OK;

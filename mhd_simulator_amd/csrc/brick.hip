@@ -8,6 +8,7 @@
 #include "mhdp_kernels.h"
 #include "launch.h"
 #include "raw.h"
+#include "mob.h"
 
 #define CONVERGED(sc) ((sc)->conv > 0.5)
 
@@ -417,24 +418,6 @@ struct K_k_brick_apply {
 // one by one (no fused multiply-add), and a cell is active where the coefficient pass gives it a
 // nonzero reciprocal diagonal (fluid; in the pinned gauge not an anchor).  The pass reads 5 bytes
 // per cell of coefficients instead of 14 (three FP32 mobilities and the BF16 1/diag).
-struct MobP {
-    float rho_l, rho_air;
-};
-// (this file contracts multiply-adds elsewhere: these two forbid it)
-__device__ __forceinline__ float mob_dens(const MobP& m, float a) {
-#pragma clang fp contract(off)
-    return m.rho_air * (1.0f - a) + m.rho_l * a;
-}
-template <bool TWO>
-__device__ __forceinline__ float mob_face(const MobP& m, bool open, float dl, float dc) {
-#pragma clang fp contract(off)
-    if (!open) return 0.0f;
-    if (!TWO) return 1.0f;
-    const float fd = 0.5f * (dl + dc);
-    return m.rho_l / fd;
-}
-__device__ __forceinline__ uint32_t obyte(uint32_t w, int k) { return (w >> (8 * k)) & 0xFFu; }
-
 template <class VT, bool TWO, int MB_>
 struct K_k_brick_apply_m {
     static constexpr int MINB = MB_;
@@ -557,7 +540,7 @@ struct K_k_brick_apply_m {
 #pragma unroll
                     for (int k = 0; k < 4; k++) {
                         const uint32_t bits = obyte(ob[dz][dy], k);
-                        act[k] = (bits & 8u) && (meanfree || !(bits & 16u));
+                        act[k] = open_active(bits, meanfree);
                         pn.v[k] = act[k] ? f.c[dz][dy][k] : 0.0f;
                         qv.v[k] = act_at(w, f, dz, dy, k);
                         acc[0] += (double)pn.v[k] * (double)qv.v[k];

@@ -24,18 +24,21 @@ struct LevelDev {
 
 // setup.hip
 // Returns true when the BF16 copies (all four pointers non-null) were written too.
-bool launch_coefficients(hipStream_t s, const Grid3& g, const PhaseP& ph, const uint32_t* mask, const float* alpha,
+// open8: the per-cell open-face bytes (the row-vector pass reads them instead of the mask)
+bool launch_coefficients(hipStream_t s, const Grid3& g, const PhaseP& ph, const uint32_t* mask, const uint8_t* open8,
+                         const float* alpha,
                          float* bx, float* by, float* bz, float* diag, float* dinv, void* bx16 = nullptr,
                          void* by16 = nullptr, void* bz16 = nullptr, void* dinv16 = nullptr);
-void launch_rhs_f32(hipStream_t s, const Grid3& g, float h, const uint32_t* mask, const float* bx, const float* by,
-                    const float* bz, const float* ustar, float* b);
-// launch_rhs_f64 and launch_outer_start(zero_x) in one pass (rows of four cells: nx % 4 == 0;
-// returns false and launches nothing otherwise)
-bool launch_rhs_start(hipStream_t s, const Grid3& g, float h, const uint32_t* mask, const float* bx, const float* by,
-                      const float* bz, const float* ustar, double* b, const float* dinv, float* r32, float* x32,
-                      float* p0, float* p1, double* part, Scal* sc);
-void launch_rhs_f64(hipStream_t s, const Grid3& g, float h, const uint32_t* mask, const float* bx, const float* by,
-                    const float* bz, const float* ustar, double* b);
+// open8: the per-cell open-face bytes (the row-vector passes read them instead of the mask and
+// the mobilities; null: the one-cell pass)
+void launch_rhs_f32(hipStream_t s, const Grid3& g, float h, const uint32_t* mask, const uint8_t* open8, const float* bx,
+                    const float* by, const float* bz, const float* ustar, float* b);
+// launch_rhs_f64 and launch_outer_start(zero_x) in one pass (rows of four cells: nx % 4 == 0, and
+// the open bytes; returns false and launches nothing otherwise)
+bool launch_rhs_start(hipStream_t s, const Grid3& g, float h, const uint8_t* open8, const float* ustar, double* b,
+                      bool meanfree, float* r32, float* x32, float* p0, float* p1, double* part, Scal* sc);
+void launch_rhs_f64(hipStream_t s, const Grid3& g, float h, const uint32_t* mask, const uint8_t* open8, const float* bx,
+                    const float* by, const float* bz, const float* ustar, double* b);
 void launch_correct_f32(hipStream_t s, const Grid3& g, float h, const float* bx, const float* by, const float* bz,
                         const float* phi, const float* ustar, float* uout);
 // also copies phi to phi_copy (when set) in the same pass; returns whether it did
@@ -67,6 +70,11 @@ void launch_residual64(hipStream_t s, const Grid3& g, const double* x64, const d
 void launch_outer_start(hipStream_t s, const Grid3& g, const double* x64, const double* b64, const float* bx,
                         const float* by, const float* bz, const float* dinv, float* r32, float* x32, float* p0,
                         float* p1, double* part, Scal* sc, bool zero_x);
+// launch_outer_start with the mobilities formed from the liquid fraction (null: single phase) and
+// the open bytes (rows of four cells, nx % 4 == 0; returns false and launches nothing otherwise)
+bool launch_outer_start_m(hipStream_t s, const Grid3& g, const double* x64, const double* b64, const float* alpha,
+                          const uint8_t* open8, float rho_l, float rho_air, bool meanfree, float* r32, float* x32,
+                          float* p0, float* p1, double* part, Scal* sc, bool zero_x);
 void launch_outer_update(hipStream_t s, const Grid3& g, double* x64, const float* x, const float* p0, const float* p1,
                          const float* dinv, const Scal* sc, bool meanfree, bool assign);
 void launch_xupdate64(hipStream_t s, const Grid3& g, double* x64, const float* e, const float* dinv);

@@ -6,6 +6,7 @@
 #include "mhdp_kernels.h"
 #include "launch.h"
 #include "rows.h"
+#include "mob.h"
 
 #define CONVERGED(sc) ((sc)->conv > 0.5)
 
@@ -592,6 +593,118 @@ struct K_k_outer_start4 {
     }
 };
 
+// K_k_outer_start4 with the mobilities formed from the liquid fraction and the open bytes (mob.h)
+// instead of loaded, and the active cells from the open bytes: the same residual bit for bit.
+template <bool TWO>
+struct K_k_outer_start4m {
+    static constexpr int MINB = 0;
+    struct Args {
+        Grid3 g;
+        const double* x;
+        const double* b;
+        const float* alpha;  // TWO
+        const uint8_t* open8;
+        MobP mp;
+        int meanfree;
+        float* r32;
+        float* x32;
+        float* p0;
+        float* p1;
+        double* part;
+        Scal* sc;
+        int zero_x;
+    };
+    static __device__ void run(const Args& A_, const VGrid& vg) {
+        const Grid3 g = A_.g;
+        const double* __restrict__ x = A_.x;
+        const uint8_t* __restrict__ open8 = A_.open8;
+        const float* __restrict__ al = A_.alpha;
+        const MobP mp = A_.mp;
+        uint32_t lo, hi;
+        block_range(vg, g.n >> 2, lo, hi);
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        const float zero4[4] = {0.f, 0.f, 0.f, 0.f};
+        for (uint32_t t = lo + threadIdx.x; t < hi; t += MHDP_BLOCK) {
+            const Row4 r = row4(g, t);
+            double bv[4], tv[4];
+            ld4d(A_.b, r.i0, bv);
+            const uint32_t ow = *reinterpret_cast<const uint32_t*>(open8 + r.i0);
+#pragma unroll
+            for (int k = 0; k < 4; k++) tv[k] = bv[k];
+            if (!A_.zero_x) {
+                double xc[4], xym[4], xyp[4], xzm[4], xzp[4];
+                ld4d(x, r.i0, xc);
+                ld4d(x, r.ym, xym);
+                ld4d(x, r.yp, xyp);
+                ld4d(x, r.zm, xzm);
+                ld4d(x, r.zp, xzp);
+                const double xxm = x[r.xm], xxp = x[r.xp];
+                const uint32_t oyu = *reinterpret_cast<const uint32_t*>(open8 + r.yp);
+                const uint32_t ozu = *reinterpret_cast<const uint32_t*>(open8 + r.zp);
+                const uint32_t oxu = open8[r.xp];
+                float dc[4] = {0.f, 0.f, 0.f, 0.f}, dym[4] = {0.f, 0.f, 0.f, 0.f}, dyp[4] = {0.f, 0.f, 0.f, 0.f};
+                float dzm[4] = {0.f, 0.f, 0.f, 0.f}, dzp[4] = {0.f, 0.f, 0.f, 0.f}, dxm = 0.f, dxp = 0.f;
+                if (TWO) {
+                    float a[4];
+                    ld4f(al, r.i0, a);
+#pragma unroll
+                    for (int k = 0; k < 4; k++) dc[k] = mob_dens(mp, a[k]);
+                    ld4f(al, r.ym, a);
+#pragma unroll
+                    for (int k = 0; k < 4; k++) dym[k] = mob_dens(mp, a[k]);
+                    ld4f(al, r.yp, a);
+#pragma unroll
+                    for (int k = 0; k < 4; k++) dyp[k] = mob_dens(mp, a[k]);
+                    ld4f(al, r.zm, a);
+#pragma unroll
+                    for (int k = 0; k < 4; k++) dzm[k] = mob_dens(mp, a[k]);
+                    ld4f(al, r.zp, a);
+#pragma unroll
+                    for (int k = 0; k < 4; k++) dzp[k] = mob_dens(mp, a[k]);
+                    dxm = mob_dens(mp, al[r.xm]);
+                    dxp = mob_dens(mp, al[r.xp]);
+                }
+                float wx[5];
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    wx[k] = mob_face<TWO>(mp, obyte(ow, k) & 1u, k > 0 ? dc[k - 1] : dxm, dc[k]);
+                wx[4] = mob_face<TWO>(mp, r.xp_ok && (oxu & 1u), dc[3], dxp);
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const uint32_t o = obyte(ow, k);
+                    const float wy = mob_face<TWO>(mp, o & 2u, dym[k], dc[k]);
+                    const float wyu = mob_face<TWO>(mp, r.yp_ok && (obyte(oyu, k) & 2u), dc[k], dyp[k]);
+                    const float wz = mob_face<TWO>(mp, o & 4u, dzm[k], dc[k]);
+                    const float wzu = mob_face<TWO>(mp, r.zp_ok && (obyte(ozu, k) & 4u), dc[k], dzp[k]);
+                    const double c = xc[k];
+                    double a = 0.0;
+                    a += (double)wx[k] * (c - (k > 0 ? xc[k - 1] : xxm));
+                    a += (double)wx[k + 1] * (c - (k < 3 ? xc[k + 1] : xxp));
+                    a += (double)wy * (c - xym[k]);
+                    a += (double)wyu * (c - xyp[k]);
+                    a += (double)wz * (c - xzm[k]);
+                    a += (double)wzu * (c - xzp[k]);
+                    tv[k] = bv[k] - a;
+                }
+            }
+            float rv[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                rv[k] = open_active(obyte(ow, k), A_.meanfree) ? (float)tv[k] : 0.0f;
+                acc[0] += (double)rv[k] * (double)rv[k];
+                acc[1] += tv[k] * tv[k];
+                acc[2] += bv[k] * bv[k];
+                acc[3] += (double)rv[k];
+            }
+            st4f(A_.r32, r.i0, rv);
+            st4f(A_.x32, r.i0, zero4);
+            st4f(A_.p0, r.i0, zero4);
+            st4f(A_.p1, r.i0, zero4);
+        }
+        reduce_finish<4>(vg, acc, A_.part, A_.sc, 5, FIN_OUTER);
+    }
+};
+
 struct K_k_outer_update4 {
     static constexpr int MINB = 0;
     struct Args {
@@ -766,6 +879,19 @@ void launch_outer_start(hipStream_t s, const Grid3& g, const double* x64, const 
         go<K_k_outer_start4>(s, grid_blocks(g.n >> 2), {g, x64, b64, bx, by, bz, dinv, r32, x32, p0, p1, part, sc, zero_x ? 1 : 0});
     else
         go<K_k_outer_start>(s, grid_blocks(g.n), {g, x64, b64, bx, by, bz, dinv, r32, x32, p0, p1, part, sc, zero_x ? 1 : 0});
+}
+bool launch_outer_start_m(hipStream_t s, const Grid3& g, const double* x64, const double* b64, const float* alpha,
+                          const uint8_t* open8, float rho_l, float rho_air, bool meanfree, float* r32, float* x32,
+                          float* p0, float* p1, double* part, Scal* sc, bool zero_x) {
+    if ((g.nx & 3) != 0 || !open8) return false;
+    const MobP mp{rho_l, rho_air};
+    if (alpha)
+        go<K_k_outer_start4m<true>>(s, grid_blocks(g.n >> 2), {g, x64, b64, alpha, open8, mp, meanfree ? 1 : 0, r32, x32,
+                                                               p0, p1, part, sc, zero_x ? 1 : 0});
+    else
+        go<K_k_outer_start4m<false>>(s, grid_blocks(g.n >> 2), {g, x64, b64, alpha, open8, mp, meanfree ? 1 : 0, r32, x32,
+                                                                p0, p1, part, sc, zero_x ? 1 : 0});
+    return true;
 }
 void launch_outer_update(hipStream_t s, const Grid3& g, double* x64, const float* x, const float* p0, const float* p1,
                          const float* dinv, const Scal* sc, bool meanfree, bool assign) {

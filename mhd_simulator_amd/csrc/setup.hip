@@ -409,14 +409,17 @@ __device__ __forceinline__ float mob(const PhaseP& ph, float dl, float dc) {
 // lower-face mobilities of each cell (and their BF16 copies), the diagonal = sum of the six face
 // mobilities in the order x-, x+, y-, y+, z-, z+ (a closed face contributes +0), the anchor row
 // and the active-cell reciprocal.  The upper faces are the neighbours' lower faces, evaluated
-// here from the same mask and liquid-fraction values with the same expression, so the diagonal
-// equals the sum of the stored mobilities bit for bit.
+// here from the same liquid-fraction values with the same expression, so the diagonal equals the
+// sum of the stored mobilities bit for bit.  Which faces are open, which cells are fluid and which
+// are anchors comes from the per-cell open bytes (bit a: lower face on axis a open, i.e. the cell
+// and its lower neighbour fluid; bit 3 fluid; bit 4 anchor), one byte per cell instead of the
+// four-byte gauge mask of the cell and its six neighbours.
 struct K_k_coef4 {
     static constexpr int MINB = 0;
     struct Args {
         Grid3 g;
         PhaseP ph;
-        const uint32_t* mask;
+        const uint8_t* open8;
         const float* alpha;
         float* bx;
         float* by;
@@ -431,19 +434,17 @@ struct K_k_coef4 {
     static __device__ void run(const Args& A_, const VGrid& vg) {
         const Grid3 g = A_.g;
         const PhaseP ph = A_.ph;
-        const uint32_t* __restrict__ mask = A_.mask;
+        const uint8_t* __restrict__ open8 = A_.open8;
         const float* __restrict__ alpha = A_.alpha;
         uint32_t lo, hi;
         block_range(vg, g.n >> 2, lo, hi);
         for (uint32_t t = lo + threadIdx.x; t < hi; t += MHDP_BLOCK) {
             const Row4 r = row4(g, t);
-            uint32_t m[4], my[4], mz[4], myu[4], mzu[4];
-            ld4u(mask, r.i0, m);
-            ld4u(mask, r.ym, my);
-            ld4u(mask, r.zm, mz);
-            ld4u(mask, r.yp, myu);
-            ld4u(mask, r.zp, mzu);
-            const uint32_t mx = mask[r.xm], mxu = mask[r.xp];
+            const uint32_t ow = *reinterpret_cast<const uint32_t*>(open8 + r.i0);
+            const uint32_t oyu = *reinterpret_cast<const uint32_t*>(open8 + r.yp);
+            const uint32_t ozu = *reinterpret_cast<const uint32_t*>(open8 + r.zp);
+            const uint32_t oxu = open8[r.xp];
+            auto byte = [](uint32_t w, int k) { return (w >> (8 * k)) & 0xFFu; };
             float a[4] = {0.f, 0.f, 0.f, 0.f}, ay[4] = {0.f, 0.f, 0.f, 0.f}, az[4] = {0.f, 0.f, 0.f, 0.f};
             float ayu[4] = {0.f, 0.f, 0.f, 0.f}, azu[4] = {0.f, 0.f, 0.f, 0.f}, ax = 0.f, axu = 0.f;
             if (alpha) {
@@ -458,23 +459,22 @@ struct K_k_coef4 {
             float ox[4], oy[4], oz[4], d[4], di[4];
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                const bool fc = m[k] != 0u;
+                const uint32_t o = byte(ow, k);
                 const float dc = dens(ph, a[k]);
                 // lower faces (mobility(), solver.wgsl:43-51)
-                const bool lx = k > 0 ? m[k - 1] != 0u : (r.xm_ok && mx != 0u);
-                ox[k] = (fc && lx) ? mob(ph, dens(ph, k > 0 ? a[k - 1] : ax), dc) : 0.0f;
-                oy[k] = (fc && r.ym_ok && my[k] != 0u) ? mob(ph, dens(ph, ay[k]), dc) : 0.0f;
-                oz[k] = (fc && r.zm_ok && mz[k] != 0u) ? mob(ph, dens(ph, az[k]), dc) : 0.0f;
+                ox[k] = (o & 1u) ? mob(ph, dens(ph, k > 0 ? a[k - 1] : ax), dc) : 0.0f;
+                oy[k] = (o & 2u) ? mob(ph, dens(ph, ay[k]), dc) : 0.0f;
+                oz[k] = (o & 4u) ? mob(ph, dens(ph, az[k]), dc) : 0.0f;
             }
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                const bool fc = m[k] != 0u;
+                const uint32_t o = byte(ow, k);
+                const bool fc = (o & 8u) != 0u;
                 const float dc = dens(ph, a[k]);
                 // upper faces: the lower faces of the +x, +y, +z neighbours
-                const float xu = k < 3 ? ox[k + 1]
-                                       : ((r.xp_ok && mxu != 0u && fc) ? mob(ph, dc, dens(ph, axu)) : 0.0f);
-                const float yu = (r.yp_ok && myu[k] != 0u && fc) ? mob(ph, dc, dens(ph, ayu[k])) : 0.0f;
-                const float zu = (r.zp_ok && mzu[k] != 0u && fc) ? mob(ph, dc, dens(ph, azu[k])) : 0.0f;
+                const float xu = k < 3 ? ox[k + 1] : ((r.xp_ok && (oxu & 1u)) ? mob(ph, dc, dens(ph, axu)) : 0.0f);
+                const float yu = (r.yp_ok && (byte(oyu, k) & 2u)) ? mob(ph, dc, dens(ph, ayu[k])) : 0.0f;
+                const float zu = (r.zp_ok && (byte(ozu, k) & 4u)) ? mob(ph, dc, dens(ph, azu[k])) : 0.0f;
                 // diagonal() (solver.wgsl:60-69)
                 float v = 0.0f;
                 if (fc) {
@@ -484,11 +484,11 @@ struct K_k_coef4 {
                     v += yu;
                     v += oz[k];
                     v += zu;
-                    if (!ph.meanfree && (m[k] & 3u) == 2u) v = 1.0f;
+                    if (!ph.meanfree && (o & 16u)) v = 1.0f;
                     else v = fmaxf(v, 1e-30f);
                 }
                 d[k] = v;
-                const bool active = fc && (ph.meanfree || (m[k] & 3u) == 1u);
+                const bool active = fc && (ph.meanfree || !(o & 16u));
                 di[k] = active ? 1.0f / v : 0.0f;
             }
             st4f(A_.bx, r.i0, ox);
@@ -507,24 +507,21 @@ struct K_k_coef4 {
 };
 
 // rhs() (solver.wgsl:104-107) on the four cells of row r, out[k] = -h^2 div: faces of the x plane
-// are read as five scalars (its rows hold nx+1 faces), the y and z planes as 16-byte rows.
+// are read as five scalars (its rows hold nx+1 faces), the y and z planes as 16-byte rows.  A face
+// counts where it is open (its mobility is nonzero): bit a of the open byte of the cell above it
+// on axis a; bit 3 marks a fluid cell.
 template <class T>
-__device__ __forceinline__ void rhs_row4(const Grid3& g, float h, const Row4& r, const uint32_t* __restrict__ mask,
-                                         const float* __restrict__ bx, const float* __restrict__ by,
-                                         const float* __restrict__ bz, const float* __restrict__ U,
-                                         const float* __restrict__ V, const float* __restrict__ W, T (&out)[4]) {
-    uint32_t m[4];
-    ld4u(mask, r.i0, m);
+__device__ __forceinline__ void rhs_row4(const Grid3& g, float h, const Row4& r, const uint8_t* __restrict__ open8,
+                                         const float* __restrict__ U, const float* __restrict__ V,
+                                         const float* __restrict__ W, T (&out)[4]) {
+    const uint32_t ow = *reinterpret_cast<const uint32_t*>(open8 + r.i0);
 #pragma unroll
     for (int k = 0; k < 4; k++) out[k] = (T)0;
-    if ((m[0] | m[1] | m[2] | m[3]) == 0u) return;
-    float xl[4], yl[4], zl[4], yu[4] = {0.f, 0.f, 0.f, 0.f}, zu[4] = {0.f, 0.f, 0.f, 0.f};
-    ld4f(bx, r.i0, xl);
-    ld4f(by, r.i0, yl);
-    ld4f(bz, r.i0, zl);
-    if (r.yp_ok) ld4f(by, r.yp, yu);
-    if (r.zp_ok) ld4f(bz, r.zp, zu);
-    const float xe = r.xp_ok ? bx[r.xp] : 0.0f;
+    if ((ow & 0x08080808u) == 0u) return;
+    const uint32_t oyu = r.yp_ok ? *reinterpret_cast<const uint32_t*>(open8 + r.yp) : 0u;
+    const uint32_t ozu = r.zp_ok ? *reinterpret_cast<const uint32_t*>(open8 + r.zp) : 0u;
+    const uint32_t oxu = r.xp_ok ? (uint32_t)open8[r.xp] : 0u;
+    auto byte = [](uint32_t w, int k) { return (w >> (8 * k)) & 0xFFu; };
     // x plane: faces x0..x0+4 of this row (the last wraps to face 0 when periodic)
     const uint32_t ux = (uint32_t)(g.nx + 1) * ((uint32_t)r.y + (uint32_t)g.ny * (uint32_t)r.z);
     float uf[5];
@@ -549,22 +546,23 @@ __device__ __forceinline__ void rhs_row4(const Grid3& g, float h, const Row4& r,
     ld4f(W, wz + sz * (uint32_t)zu_i, wu);
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        if (m[k] == 0u) continue;
-        const float bxu = k < 3 ? xl[k + 1] : xe;
+        const uint32_t o = byte(ow, k);
+        if (!(o & 8u)) continue;
+        const bool xu_open = k < 3 ? (byte(ow, k + 1) & 1u) != 0u : (oxu & 1u) != 0u;
         T sum = (T)0;
         {
-            const float up = bxu > 0.0f ? uf[k + 1] : 0.0f;
-            const float dn = xl[k] > 0.0f ? uf[k] : 0.0f;
+            const float up = xu_open ? uf[k + 1] : 0.0f;
+            const float dn = (o & 1u) ? uf[k] : 0.0f;
             sum += (T)up - (T)dn;
         }
         {
-            const float up = yu[k] > 0.0f ? vu[k] : 0.0f;
-            const float dn = yl[k] > 0.0f ? vl[k] : 0.0f;
+            const float up = (byte(oyu, k) & 2u) ? vu[k] : 0.0f;
+            const float dn = (o & 2u) ? vl[k] : 0.0f;
             sum += (T)up - (T)dn;
         }
         {
-            const float up = zu[k] > 0.0f ? wu[k] : 0.0f;
-            const float dn = zl[k] > 0.0f ? wl[k] : 0.0f;
+            const float up = (byte(ozu, k) & 4u) ? wu[k] : 0.0f;
+            const float dn = (o & 4u) ? wl[k] : 0.0f;
             sum += (T)up - (T)dn;
         }
         T div = sum / (T)h;
@@ -578,10 +576,7 @@ struct K_k_rhs4 {
     struct Args {
         Grid3 g;
         float h;
-        const uint32_t* mask;
-        const float* bx;
-        const float* by;
-        const float* bz;
+        const uint8_t* open8;
         const float* ustar;
         uint64_t off1;
         uint64_t off2;
@@ -595,7 +590,7 @@ struct K_k_rhs4 {
         for (uint32_t t = lo + threadIdx.x; t < hi; t += MHDP_BLOCK) {
             const Row4 r = row4(g, t);
             T out[4];
-            rhs_row4<T>(g, A_.h, r, A_.mask, A_.bx, A_.by, A_.bz, U, U + A_.off1, U + A_.off2, out);
+            rhs_row4<T>(g, A_.h, r, A_.open8, U, U + A_.off1, U + A_.off2, out);
 #pragma unroll
             for (int k = 0; k < 4; k++) A_.b[r.i0 + k] = out[k];
         }
@@ -611,15 +606,12 @@ struct K_k_rhs_start4 {
     struct Args {
         Grid3 g;
         float h;
-        const uint32_t* mask;
-        const float* bx;
-        const float* by;
-        const float* bz;
+        const uint8_t* open8;
         const float* ustar;
         uint64_t off1;
         uint64_t off2;
         double* b;
-        const float* dinv;
+        int meanfree;  // active cells: fluid and, in the pinned gauge, not an anchor
         float* r32;
         float* x32;
         float* p0;
@@ -637,14 +629,15 @@ struct K_k_rhs_start4 {
         for (uint32_t t = lo + threadIdx.x; t < hi; t += MHDP_BLOCK) {
             const Row4 r = row4(g, t);
             double bv[4];
-            rhs_row4<double>(g, A_.h, r, A_.mask, A_.bx, A_.by, A_.bz, U, U + A_.off1, U + A_.off2, bv);
+            rhs_row4<double>(g, A_.h, r, A_.open8, U, U + A_.off1, U + A_.off2, bv);
             st4d(A_.b, r.i0, bv);
-            float di[4];
-            ld4f(A_.dinv, r.i0, di);
+            const uint32_t ow = *reinterpret_cast<const uint32_t*>(A_.open8 + r.i0);
             float rv[4];
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                rv[k] = di[k] != 0.0f ? (float)bv[k] : 0.0f;
+                const uint32_t o = (ow >> (8 * k)) & 0xFFu;
+                const bool active = (o & 8u) && (A_.meanfree || !(o & 16u));
+                rv[k] = active ? (float)bv[k] : 0.0f;
                 acc[0] = __fma_rn((double)rv[k], (double)rv[k], acc[0]);
                 acc[1] = __fma_rn(bv[k], bv[k], acc[1]);
                 acc[2] = __fma_rn(bv[k], bv[k], acc[2]);
@@ -1110,42 +1103,42 @@ struct K_fields_total {
 
 static bool row_vector_ok(const Grid3& g) { return (g.nx & 3) == 0 && std::getenv("MHDP_SCALAR_SETUP") == nullptr; }
 
-bool launch_coefficients(hipStream_t s, const Grid3& g, const PhaseP& ph, const uint32_t* mask, const float* alpha,
+bool launch_coefficients(hipStream_t s, const Grid3& g, const PhaseP& ph, const uint32_t* mask, const uint8_t* open8,
+                         const float* alpha,
                          float* bx, float* by, float* bz, float* diag, float* dinv, void* bx16, void* by16,
                          void* bz16, void* dinv16) {
-    if (row_vector_ok(g)) {
+    if (row_vector_ok(g) && open8) {
         const dim3 G(grid_blocks(g.n >> 2)), B(MHDP_BLOCK);
         const bool h = bx16 && by16 && bz16 && dinv16;
         using T = __hip_bfloat16;
-        go<K_k_coef4>(s, G.x, {g, ph, mask, alpha, bx, by, bz, diag, dinv, h ? (T*)bx16 : (T*)nullptr,
+        go<K_k_coef4>(s, G.x, {g, ph, open8, alpha, bx, by, bz, diag, dinv, h ? (T*)bx16 : (T*)nullptr,
                                h ? (T*)by16 : (T*)nullptr, h ? (T*)bz16 : (T*)nullptr, h ? (T*)dinv16 : (T*)nullptr});
         return h;
     }
     go<K_k_coefficients>(s, grid_blocks(g.n), {g, ph, mask, alpha, bx, by, bz, diag, dinv});
     return false;
 }
-void launch_rhs_f32(hipStream_t s, const Grid3& g, float h, const uint32_t* mask, const float* bx, const float* by,
-                    const float* bz, const float* ustar, float* b) {
+void launch_rhs_f32(hipStream_t s, const Grid3& g, float h, const uint32_t* mask, const uint8_t* open8, const float* bx,
+                    const float* by, const float* bz, const float* ustar, float* b) {
     uint64_t f0 = (uint64_t)(g.nx + 1) * g.ny * g.nz, f1 = (uint64_t)g.nx * (g.ny + 1) * g.nz;
-    if (row_vector_ok(g))
-        go<K_k_rhs4<float>>(s, grid_blocks(g.n >> 2), {g, h, mask, bx, by, bz, ustar, f0, f0 + f1, b});
+    if (row_vector_ok(g) && open8)
+        go<K_k_rhs4<float>>(s, grid_blocks(g.n >> 2), {g, h, open8, ustar, f0, f0 + f1, b});
     else
         go<K_k_rhs<float>>(s, grid_blocks(g.n), {g, h, mask, bx, by, bz, ustar, f0, f0 + f1, b});
 }
-bool launch_rhs_start(hipStream_t s, const Grid3& g, float h, const uint32_t* mask, const float* bx, const float* by,
-                      const float* bz, const float* ustar, double* b, const float* dinv, float* r32, float* x32,
-                      float* p0, float* p1, double* part, Scal* sc) {
-    if (!row_vector_ok(g)) return false;
+bool launch_rhs_start(hipStream_t s, const Grid3& g, float h, const uint8_t* open8, const float* ustar, double* b,
+                      bool meanfree, float* r32, float* x32, float* p0, float* p1, double* part, Scal* sc) {
+    if (!row_vector_ok(g) || !open8) return false;
     uint64_t f0 = (uint64_t)(g.nx + 1) * g.ny * g.nz, f1 = (uint64_t)g.nx * (g.ny + 1) * g.nz;
     go<K_k_rhs_start4>(s, grid_blocks(g.n >> 2),
-                       {g, h, mask, bx, by, bz, ustar, f0, f0 + f1, b, dinv, r32, x32, p0, p1, part, sc});
+                       {g, h, open8, ustar, f0, f0 + f1, b, meanfree ? 1 : 0, r32, x32, p0, p1, part, sc});
     return true;
 }
-void launch_rhs_f64(hipStream_t s, const Grid3& g, float h, const uint32_t* mask, const float* bx, const float* by,
-                    const float* bz, const float* ustar, double* b) {
+void launch_rhs_f64(hipStream_t s, const Grid3& g, float h, const uint32_t* mask, const uint8_t* open8, const float* bx,
+                    const float* by, const float* bz, const float* ustar, double* b) {
     uint64_t f0 = (uint64_t)(g.nx + 1) * g.ny * g.nz, f1 = (uint64_t)g.nx * (g.ny + 1) * g.nz;
-    if (row_vector_ok(g))
-        go<K_k_rhs4<double>>(s, grid_blocks(g.n >> 2), {g, h, mask, bx, by, bz, ustar, f0, f0 + f1, b});
+    if (row_vector_ok(g) && open8)
+        go<K_k_rhs4<double>>(s, grid_blocks(g.n >> 2), {g, h, open8, ustar, f0, f0 + f1, b});
     else
         go<K_k_rhs<double>>(s, grid_blocks(g.n), {g, h, mask, bx, by, bz, ustar, f0, f0 + f1, b});
 }
