@@ -2,8 +2,9 @@
 liquid fraction and the per-cell open bytes) against the forms that load the coefficient pass's
 stored mobilities, through the C ABI: the same projections in a second process with
 MHDP_STORED_MOBILITY=1 give the same potential and faces bit for bit and the same iteration counts.
-Covers two-phase and single-phase grids, periodic and walled axes, a grid with several components (pinned anchors) and the pinned
-gauge forced on one component (the anchor is inactive there), in both precisions."""
+Covers two-phase and single-phase grids, periodic and walled axes, a grid with several components
+(pinned anchors) and the pinned gauge forced on one component (the anchor is inactive there), in
+both precisions."""
 import json
 import os
 import subprocess
