@@ -101,8 +101,9 @@ typedef struct {
                             axes have even length (Jacobi on the others); AUTO = red-black on
                             grids of at least 2^24 cells, Jacobi below.
                             The coarse scale's tuned default is 1.6 with Jacobi, 1.8 with
-                            red-black; the fused multigrid path only (the one-kernel-per-stage
-                            path, MHDP_UNFUSED=1, keeps Jacobi) */
+                            red-black (1.9 on grids of at least 2^24 cells); the fused multigrid
+                            path only (the one-kernel-per-stage path, MHDP_UNFUSED=1, keeps
+                            Jacobi) */
 } mhdp_opts;
 
 typedef struct {

@@ -524,6 +524,9 @@ void iterate_once(mhdp_ctx* c, Solve& S) {  // mod.rs:1057-1075, multigrid.rs:30
 constexpr double kTunedOmega = 0.9;   // smoother weight picked by the V-cycle tuner
 constexpr double kTunedScale = 1.6;   // coarse-correction scale picked by the V-cycle tuner (Jacobi)
 constexpr double kTunedScaleRB = 1.8; // the same with red-black Gauss-Seidel smoothing
+// ... and on grids of at least 2^24 cells, whose deep hierarchies keep the red-black cycle's
+// level-1 spectrum well inside the positive-definite range (512^3: 12 instead of 13 iterations)
+constexpr double kTunedScaleRBLarge = 1.9;
 // The automatic smoother: red-black Gauss-Seidel on grids of at least 2^24 cells (at 512^3 it needs
 // 13 iterations where weighted Jacobi needs 21, at about 1.4x the cost per pass), Jacobi below.
 constexpr uint32_t kRedBlackAutoCells = 1u << 24;
@@ -757,7 +760,9 @@ Cycle requested_cycle(const mhdp_ctx* c, const mhdp_opts* o) {
     cy.rb = o->smoother == MHDP_SMOOTH_RED_BLACK ||
             (o->smoother == MHDP_SMOOTH_AUTO && c->g.n >= kRedBlackAutoCells);
     cy.omega = o->smooth_weight > 0.0f ? (double)o->smooth_weight : kTunedOmega;
-    cy.pscale = o->coarse_scale > 0.0f ? (double)o->coarse_scale : (cy.rb ? kTunedScaleRB : kTunedScale);
+    cy.pscale = o->coarse_scale > 0.0f ? (double)o->coarse_scale
+                                       : (cy.rb ? (c->g.n >= kRedBlackAutoCells ? kTunedScaleRBLarge : kTunedScaleRB)
+                                                : kTunedScale);
     cy.visits = coarse_visits_for(c, o->coarse_visits);
     return cy;
 }

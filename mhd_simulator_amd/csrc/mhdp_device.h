@@ -235,7 +235,9 @@ __device__ inline void finalize(Scal* s, int mode, const double t[4]) {
             s->rz = t[0];
             s->rr = rr;
             s->conv = rr <= (s->tol2 * s->bb + s->abs2) / s->bound ? 1.0 : 0.0;
-            if (mode == FIN_BETA_MG && s->guard && s->conv < 0.5 && t[0] <= 0.0) {  // r.z <= 0
+            // r.z <= 0, or a residual grown 1e5-fold over the solve's right-hand side: the cycle is
+            // not positive definite on this problem
+            if (mode == FIN_BETA_MG && s->guard && s->conv < 0.5 && (t[0] <= 0.0 || rr > 1e10 * s->bb)) {
                 s->fault = 1u;
                 s->conv = 1.0;
             }

@@ -38,10 +38,10 @@ COARSE_VISITS = (0, 1, 2)  # 0: the library's automatic choice by grid size
 INNER_TOL = (0.0,)  # the mixed solver's inner FP32 tolerance; 0: the library's (1e-4)
 BUILTIN_LAUNCH = {"march_z": 32, "march_cap": 1024, "brick_occ": 2}
 # the schedule stage's axes (0: the library's tuned default for the smoother: weight 0.9, scale
-# 1.6 with Jacobi and 1.8 with red-black)
+# 1.6 with Jacobi, 1.8 with red-black, 1.9 from 2^24 cells)
 SMOOTHER = ("jacobi", "red-black")
-SMOOTH_WEIGHT = (0.0, 0.8, 0.95)  # the library accepts weights in [0, 1)
-COARSE_SCALE = (0.0, 1.4, 1.9)    # and scales in [0, 2)
+SMOOTH_WEIGHT = (0.0, 0.8, 0.95)   # the library accepts weights in [0, 1)
+COARSE_SCALE = (0.0, 1.4, 1.8, 1.95)  # and scales in [0, 2)
 
 
 def current() -> dict:

@@ -94,7 +94,7 @@ def assembled(cycle, lev0, active):
     return Mx
 
 
-@pytest.mark.parametrize("smoother,scale", [("red-black", 1.8), ("jacobi", 1.6)])
+@pytest.mark.parametrize("smoother,scale", [("red-black", 1.8), ("red-black", 1.9), ("jacobi", 1.6)])
 @pytest.mark.parametrize("visits", [1, 2])
 @pytest.mark.parametrize("name", ["plate", "drops", "walls", "periodic"])
 def test_cycle_is_symmetric_positive_definite(orc, name, smoother, scale, visits):

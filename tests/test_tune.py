@@ -98,12 +98,12 @@ def test_results_persist_by_key(tmp_path, restore):
 
 def test_schedule_stage_covers_smoothers_weights_and_scales():
     cands = tune.schedules()
-    assert len(cands) == 2 * 3 * 3 * 3
+    assert len(cands) == 2 * 3 * 4 * 3
     assert all(c["launch"] == tune.BUILTIN_LAUNCH for c in cands)
     assert {c["opts"]["smoother"] for c in cands} == {"jacobi", "red-black"}
     # the library's defaults are left out of the options
     assert {c["opts"].get("smooth_weight", 0.0) for c in cands} == {0.0, 0.8, 0.95}
-    assert {c["opts"].get("coarse_scale", 0.0) for c in cands} == {0.0, 1.4, 1.9}
+    assert {c["opts"].get("coarse_scale", 0.0) for c in cands} == {0.0, 1.4, 1.8, 1.95}
     assert not any("smooth_weight" in c["opts"] and c["opts"]["smooth_weight"] == 0.0 for c in cands)
 
 
@@ -120,7 +120,7 @@ def test_two_stage_search_tunes_the_launch_at_the_winning_schedule(restore):
 
     def run(c):
         o = c["opts"]
-        if o["smoother"] == "red-black" and o.get("coarse_scale") == 1.9:
+        if o["smoother"] == "red-black" and o.get("coarse_scale") == 1.95:
             raise RuntimeError("the cycle broke down")
         seen.append(c)
         clock_t[0] += cost(c) * 1e-3
