@@ -527,9 +527,12 @@ constexpr double kTunedScaleRB = 1.8; // the same with red-black Gauss-Seidel sm
 // ... and on grids of at least 2^24 cells, whose deep hierarchies keep the red-black cycle's
 // level-1 spectrum well inside the positive-definite range (512^3: 12 instead of 13 iterations)
 constexpr double kTunedScaleRBLarge = 1.9;
-// The automatic smoother: red-black Gauss-Seidel on grids of at least 2^24 cells (at 512^3 it needs
-// 13 iterations where weighted Jacobi needs 21, at about 1.4x the cost per pass), Jacobi below.
-constexpr uint32_t kRedBlackAutoCells = 1u << 24;
+// The automatic smoother: red-black Gauss-Seidel on grids of at least 2^21 cells (128^3: 18 instead
+// of 27 iterations, 2.9 instead of 3.3 ms; 512^3: 12 instead of 21), weighted Jacobi below (at 64^3
+// the two take the same time, and the 64-problem batch of 32^3..96^3 grids is faster with Jacobi).
+constexpr uint32_t kRedBlackAutoCells = 1u << 21;
+// Grids of at least 2^24 cells take the larger red-black coarse scale.
+constexpr uint32_t kRedBlackLargeCells = 1u << 24;
 // Level-1 visits per cycle when the caller leaves the choice (coarse_visits = 0): a second visit
 // (W-cycle at level 1) saves about a third of the iterations, but its coarse levels are launch-bound
 // small kernels; measured on the plate workload it pays from 256^3 (12.1 vs 12.7 ms; 512^3: 75 vs 83
@@ -761,7 +764,7 @@ Cycle requested_cycle(const mhdp_ctx* c, const mhdp_opts* o) {
             (o->smoother == MHDP_SMOOTH_AUTO && c->g.n >= kRedBlackAutoCells);
     cy.omega = o->smooth_weight > 0.0f ? (double)o->smooth_weight : kTunedOmega;
     cy.pscale = o->coarse_scale > 0.0f ? (double)o->coarse_scale
-                                       : (cy.rb ? (c->g.n >= kRedBlackAutoCells ? kTunedScaleRBLarge : kTunedScaleRB)
+                                       : (cy.rb ? (c->g.n >= kRedBlackLargeCells ? kTunedScaleRBLarge : kTunedScaleRB)
                                                 : kTunedScale);
     cy.visits = coarse_visits_for(c, o->coarse_visits);
     return cy;
