@@ -29,12 +29,46 @@ METRIC = "pressure-solve cells/s to 1e-8 rel. residual (mixed vs FP32 PCG-MG); H
 HBM_PEAK = 8.0e12
 # Algorithmic bytes per cell and launch of the three fine-grid passes of the mixed solver
 # (every array touched once; neighbour reads are not counted again):
-#   apply    z (bf16) 2 + p_old 4 + x 4 + 3 FP32 mobilities 12 + 1/diag (bf16) 2 read,
-#            p_new 4 + q 4 + x 4 written                                        = 36
+#   apply    z (bf16) 2 + p_old 4 + x 4 + liquid fraction 4 + open byte 1 read (the face
+#            mobilities and the diagonal are formed from the liquid fraction and open bytes),
+#            p_new 4 + q 4 + x 4 written                                        = 27
 #   restrict r 4 + q 4 + 1/diag (bf16) 2 + 3 bf16 mobilities 6 read, updated r 4 +
-#            coarse residual 1/8*2 written (the PCG residual update is fused here) = 20.25
+#            coarse residual 1/8*2 written (the PCG residual update is fused here;
+#            the red-black pre-smoother and the Jacobi one move the same bytes)  = 20.25
 #   smooth   r 4 + 1/diag 2 + mobilities 6 + coarse z 1/8*2 read, z (bf16) 2 written = 14.25
-KERNEL_BYTES = {"apply": 36.0, "smooth": 14.25, "restrict": 20.25}
+KERNEL_BYTES = {"apply": 27.0, "smooth": 14.25, "restrict": 20.25}
+
+
+def source_digest():
+    """SHA-256 over the native sources (csrc/*.hip, *.h, *.cpp, names and contents): the HBM
+    counter summaries under profiles/ record the digest of the sources they were taken on."""
+    import hashlib
+    d = os.path.join(ROOT, "mhd_simulator_amd", "csrc")
+    h = hashlib.sha256()
+    for f in sorted(os.listdir(d)):
+        if f.endswith((".hip", ".h", ".cpp")):
+            h.update(f.encode())
+            with open(os.path.join(d, f), "rb") as fh:
+                h.update(fh.read())
+    return h.hexdigest()
+
+
+def profiled_traffic(fname, size_key, size, kernel):
+    """HBM bytes per launch of `kernel` from a committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+    (KiB per dispatch, each scaled by what the counter reads on a plain 1 GiB copy).  Returns
+    (bytes, None) when the summary was taken on these kernel sources at this size, else
+    (None, why): counters of other sources would describe other kernels."""
+    try:
+        with open(os.path.join(ROOT, "profiles", fname)) as fh:
+            prof = json.load(fh)
+    except (OSError, ValueError) as e:
+        return None, f"no counter summary: {e}"
+    if prof.get(size_key) != size or kernel not in prof.get("kernels", {}):
+        return None, f"profiles/{fname} covers another size or kernel"
+    if prof.get("source_digest") != source_digest():
+        return None, f"profiles/{fname} was taken on other kernel sources (stale): traffic not reported"
+    kk = prof["kernels"][kernel]
+    return 1024.0 * (kk["fetch_kib"] * prof["copy_scale_fetch"] + kk["write_kib"] * prof["copy_scale_write"]), None
 
 
 def log(msg):
@@ -108,6 +142,7 @@ def bench_batch(args):
     alg = sum(pr.stats["iterations"] * per_cell * p.grid.cells for pr, p in zip(prs, ps))
     roofline = {"bound": "hbm", "achieved": alg / wall / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": alg / wall / HBM_PEAK, "traffic": None,
+                "traffic_note": "the batch's passes run inside captured graph rounds of many problems: no per-launch counters",
                 "kernel": "fine passes of every problem (apply + restriction + smoother per iteration)",
                 "algorithmic_bytes_per_step": alg}
     cpu = None
@@ -229,6 +264,7 @@ def bench_step(args):
     achieved = alg_bytes / avg_launch_s if avg_launch_s > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK, "traffic": None, "kernel": "apply",
+                "traffic_note": "counters are summarised for the 512^3 projection workload only",
                 "avg_launch_us": avg_launch_s * 1e6, "algorithmic_bytes_per_launch": alg_bytes}
     cpu = None
     if not args.no_cpu:
@@ -402,17 +438,9 @@ def bench_em(args):
         f"predicted start residual <= {max(starts):.2e}, {dropped} predictions dropped")
     avg_launch_s = kern_ms / max(kern_launches, 1) * 1e-3
     achieved = kbytes[kernel] / avg_launch_s if avg_launch_s > 0 else 0.0
-    # HBM bytes per launch of the same kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
-    traffic = None
-    try:
-        prof = json.load(open(os.path.join(ROOT, "profiles", "em_128_hbm_counters.json")))
-        if prof["edges"] == E and kernel in prof["kernels"]:
-            kk = prof["kernels"][kernel]
-            traffic = 1024.0 * (kk["fetch_kib"] * prof["copy_scale_fetch"] + kk["write_kib"] * prof["copy_scale_write"])
-    except (OSError, KeyError, ValueError):
-        traffic = None
+    traffic, traffic_note = profiled_traffic("em_128_hbm_counters.json", "edges", E, kernel)
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": f"em_{kernel}",
+                "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_note": traffic_note, "kernel": f"em_{kernel}",
                 "avg_launch_us": avg_launch_s * 1e6, "algorithmic_bytes_per_launch": kbytes[kernel]}
     cpu = None
     if not args.no_cpu:
@@ -550,18 +578,10 @@ def main():
     avg_launch_s = kern_ms / max(kern_launches, 1) * 1e-3
     alg_bytes = KERNEL_BYTES[kernel] * p.grid.cells
     achieved = alg_bytes / avg_launch_s if avg_launch_s > 0 else 0.0
-    # HBM bytes per launch of the same kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
-    # summary (KiB per dispatch, scaled by the counters' reading on a plain 1 GiB copy)
-    traffic = None
-    try:
-        prof = json.load(open(os.path.join(ROOT, "profiles", "projection_512_hbm_counters.json")))
-        if prof["cells"] == p.grid.cells and kernel in prof["kernels"]:
-            kk = prof["kernels"][kernel]
-            traffic = 1024.0 * (kk["fetch_kib"] * prof["copy_scale_fetch"] + kk["write_kib"] * prof["copy_scale_write"])
-    except (OSError, KeyError, ValueError):
-        traffic = None
+    # HBM bytes per launch of the same kernel from the committed rocprofv3 counter summary
+    traffic, traffic_note = profiled_traffic("projection_512_hbm_counters.json", "cells", p.grid.cells, kernel)
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": kernel,
+                "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_note": traffic_note, "kernel": kernel,
                 "avg_launch_us": avg_launch_s * 1e6, "algorithmic_bytes_per_launch": alg_bytes}
 
     # whole-solve algorithmic bandwidth (bytes every kernel must move) for context

@@ -666,6 +666,11 @@ __device__ __forceinline__ void ld4p(const double* __restrict__ p, uint32_t i, d
     const double2 a = *reinterpret_cast<const double2*>(p + i), b = *reinterpret_cast<const double2*>(p + i + 2);
     v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
 }
+__device__ __forceinline__ void st4p(float* __restrict__ p, uint32_t i, const float (&v)[4]) { st4f(p, i, v); }
+__device__ __forceinline__ void st4p(double* __restrict__ p, uint32_t i, const double (&v)[4]) {
+    *reinterpret_cast<double2*>(p + i) = make_double2(v[0], v[1]);
+    *reinterpret_cast<double2*>(p + i + 2) = make_double2(v[2], v[3]);
+}
 template <class PT>
 struct K_k_correct4 {
     static constexpr int MINB = 0;
@@ -707,10 +712,7 @@ struct K_k_correct4 {
             const Row4 r = row4(g, t);
             PT pc[4], py[4], pz[4];
             ld4p(phi, r.i0, pc);
-            if (A_.phi_copy) {
-    #pragma unroll
-                for (int k = 0; k < 4; k++) A_.phi_copy[r.i0 + k] = pc[k];
-            }
+            if (A_.phi_copy) st4p(A_.phi_copy, r.i0, pc);
             ld4p(phi, r.ym, py);
             ld4p(phi, r.zm, pz);
             const PT px = phi[r.xm];
