@@ -387,6 +387,154 @@ struct K_k_mg_build {
     }
 };
 
+// K_k_mg_build<true> on fine grids whose axes are all even and at least 4 long.  Every aggregate
+// then has its eight children, and no neighbour reached across a periodic wrap lies in the same
+// aggregate, so a child's face is inside exactly when its neighbour is the sibling.  The
+// children's values come as x pairs (8-byte loads) from their four rows, the outside neighbours
+// from the rows around; indices are row bases plus offsets (a wrap only at the grid's ends).
+// The sums run over the same values in the same order as K_k_mg_build<true>: bit-identical.
+struct K_k_mg_build_even {
+    static constexpr int MINB = 0;
+    using Args = K_k_mg_build<true>::Args;
+    static __device__ void run(const Args& A_, const VGrid& vg) {
+        const Grid3 cg = A_.cg, fg = A_.fg;
+        const uint32_t sy = (uint32_t)fg.nx, sz = (uint32_t)fg.nx * (uint32_t)fg.ny;
+        const bool px = fg.per & 1u, py = fg.per & 2u, pz = fg.per & 4u;
+        const bool anchors = !A_.meanfree;
+        const float2* __restrict__ fdinv = reinterpret_cast<const float2*>(A_.fdinv);
+        const float* __restrict__ fbx = A_.fbx;
+        const float* __restrict__ fby = A_.fby;
+        const float* __restrict__ fbz = A_.fbz;
+        const uint32_t* __restrict__ mask = A_.mask;
+        uint32_t lo, hi;
+        block_range(vg, cg.n, lo, hi);
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += MHDP_BLOCK) {
+            int qx, qy, qz;
+            decode(cg, i, qx, qy, qz);
+            const int X = 2 * qx, Y = 2 * qy, Z = 2 * qz;
+            // outside neighbours: valid, and their coordinate (the child's own where invalid: any
+            // in-range index, its value is never used)
+            const bool xl = X > 0 || px, xh = X + 2 < fg.nx || px;
+            const bool yl = Y > 0 || py, yh = Y + 2 < fg.ny || py;
+            const bool zl = Z > 0 || pz, zh = Z + 2 < fg.nz || pz;
+            const uint32_t xm = X > 0 ? (uint32_t)(X - 1) : (px ? (uint32_t)(fg.nx - 1) : (uint32_t)X);
+            const uint32_t xp = X + 2 < fg.nx ? (uint32_t)(X + 2) : (px ? 0u : (uint32_t)X);
+            const uint32_t ym = Y > 0 ? (uint32_t)(Y - 1) : (py ? (uint32_t)(fg.ny - 1) : (uint32_t)Y);
+            const uint32_t yp = Y + 2 < fg.ny ? (uint32_t)(Y + 2) : (py ? 0u : (uint32_t)Y);
+            const uint32_t zm = Z > 0 ? (uint32_t)(Z - 1) : (pz ? (uint32_t)(fg.nz - 1) : (uint32_t)Z);
+            const uint32_t zp = Z + 2 < fg.nz ? (uint32_t)(Z + 2) : (pz ? 0u : (uint32_t)Z);
+            uint32_t row[2][2];  // [dz][dy]: fine index of (X, Y + dy, Z + dz)
+#pragma unroll
+            for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+                for (int dy = 0; dy < 2; dy++)
+                    row[dz][dy] = (uint32_t)X + (uint32_t)(Y + dy) * sy + (uint32_t)(Z + dz) * sz;
+            // children: 1/diag and lower faces as x pairs; upper faces of the outer children
+            float2 dv[2][2], bxv[2][2], byv[2][2], bzv[2][2], byu[2], bzu[2];
+            float bxu[2][2];
+            uint2 mc[2][2], mym[2], myp[2], mzm[2], mzp[2];
+            uint32_t mxm[2][2], mxp[2][2];
+#pragma unroll
+            for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+                for (int dy = 0; dy < 2; dy++) {
+                    const uint32_t r = row[dz][dy];
+                    dv[dz][dy] = fdinv[r >> 1];
+                    bxv[dz][dy] = *reinterpret_cast<const float2*>(fbx + r);
+                    byv[dz][dy] = *reinterpret_cast<const float2*>(fby + r);
+                    bzv[dz][dy] = *reinterpret_cast<const float2*>(fbz + r);
+                    bxu[dz][dy] = fbx[r - (uint32_t)X + xp];
+                }
+#pragma unroll
+            for (int d = 0; d < 2; d++) {
+                // y: the row Y + 2 of plane Z + d; z: the plane Z + 2 of row Y + d
+                byu[d] = *reinterpret_cast<const float2*>(fby + (uint32_t)X + yp * sy + (uint32_t)(Z + d) * sz);
+                bzu[d] = *reinterpret_cast<const float2*>(fbz + (uint32_t)X + (uint32_t)(Y + d) * sy + zp * sz);
+            }
+            if (anchors) {
+#pragma unroll
+                for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+                    for (int dy = 0; dy < 2; dy++) {
+                        const uint32_t r = row[dz][dy];
+                        mc[dz][dy] = *reinterpret_cast<const uint2*>(mask + r);
+                        mxm[dz][dy] = mask[r - (uint32_t)X + xm];
+                        mxp[dz][dy] = mask[r - (uint32_t)X + xp];
+                    }
+#pragma unroll
+                for (int d = 0; d < 2; d++) {
+                    mym[d] = *reinterpret_cast<const uint2*>(mask + (uint32_t)X + ym * sy + (uint32_t)(Z + d) * sz);
+                    myp[d] = *reinterpret_cast<const uint2*>(mask + (uint32_t)X + yp * sy + (uint32_t)(Z + d) * sz);
+                    mzm[d] = *reinterpret_cast<const uint2*>(mask + (uint32_t)X + (uint32_t)(Y + d) * sy + zm * sz);
+                    mzp[d] = *reinterpret_cast<const uint2*>(mask + (uint32_t)X + (uint32_t)(Y + d) * sy + zp * sz);
+                }
+            } else {
+#pragma unroll
+                for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+                    for (int dy = 0; dy < 2; dy++) {
+                        mc[dz][dy] = make_uint2(0u, 0u);
+                        mxm[dz][dy] = mxp[dz][dy] = 0u;
+                    }
+#pragma unroll
+                for (int d = 0; d < 2; d++) mym[d] = myp[d] = mzm[d] = mzp[d] = make_uint2(0u, 0u);
+            }
+            auto pick = [](float2 v, int s) { return s ? v.y : v.x; };
+            auto pickm = [](uint2 v, int s) { return s ? v.y : v.x; };
+            float c4[4] = {0.f, 0.f, 0.f, 0.f};
+            float diag = 0.f;
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int dx = k & 1, dy = (k >> 1) & 1, dz = k >> 2;
+                if (pick(dv[dz][dy], dx) == 0.0f) continue;
+                // the six faces x-, x+, y-, y+, z-, z+: valid, mobility, neighbour mask, inside
+                const bool ok[6] = {dx || xl, !dx || xh, dy || yl, !dy || yh, dz || zl, !dz || zh};
+                const float bt[6] = {pick(bxv[dz][dy], dx),
+                                     dx ? bxu[dz][dy] : bxv[dz][dy].y,
+                                     pick(byv[dz][dy], dx),
+                                     dy ? pick(byu[dz], dx) : pick(byv[dz][1], dx),
+                                     pick(bzv[dz][dy], dx),
+                                     dz ? pick(bzu[dy], dx) : pick(bzv[1][dy], dx)};
+                const uint32_t mk[6] = {dx ? mc[dz][dy].x : mxm[dz][dy],
+                                        dx ? mxp[dz][dy] : mc[dz][dy].y,
+                                        dy ? pickm(mc[dz][0], dx) : pickm(mym[dz], dx),
+                                        dy ? pickm(myp[dz], dx) : pickm(mc[dz][1], dx),
+                                        dz ? pickm(mc[0][dy], dx) : pickm(mzm[dy], dx),
+                                        dz ? pickm(mzp[dy], dx) : pickm(mc[1][dy], dx)};
+                const bool inside[6] = {dx != 0, dx == 0, dy != 0, dy == 0, dz != 0, dz == 0};
+#pragma unroll
+                for (int f = 0; f < 6; f++) {
+                    if (!ok[f]) continue;
+                    const float beta = bt[f];
+                    if (beta == 0.0f) continue;
+                    if (anchors && (mk[f] & 3u) == 2u) {
+                        c4[3] += beta;
+                        diag += beta;
+                        continue;
+                    }
+                    if (inside[f]) continue;
+                    diag += beta;
+                    if (!(f & 1)) c4[f >> 1] += beta;
+                }
+            }
+            const float dinv = diag > 0.0f ? 1.0f / diag : 0.0f;
+            A_.cbx[i] = c4[0];
+            A_.cby[i] = c4[1];
+            A_.cbz[i] = c4[2];
+            A_.creact[i] = c4[3];
+            A_.cdiag[i] = diag;
+            A_.cdinv[i] = dinv;
+            if (A_.c16[0]) {
+                A_.c16[0][i] = __float2bfloat16(c4[0]);
+                A_.c16[1][i] = __float2bfloat16(c4[1]);
+                A_.c16[2][i] = __float2bfloat16(c4[2]);
+                A_.c16[3][i] = __float2bfloat16(c4[3]);
+                A_.c16[4][i] = __float2bfloat16(dinv);
+            }
+        }
+    }
+};
+
 // ------------------------------------------------ row-vector forms (nx % 4 == 0)
 // One thread owns four consecutive cells of a row, so cell arrays move as 16-byte loads and
 // stores and every face mobility is evaluated once.  Each value is produced by the same
@@ -1175,7 +1323,11 @@ void launch_mg_build(hipStream_t s, const Grid3& cg, const Grid3& fg, bool fine,
                      void* const c16[5]) {
     __hip_bfloat16* o16[5];
     for (int k = 0; k < 5; k++) o16[k] = c16 ? (__hip_bfloat16*)c16[k] : nullptr;
-    if (fine)
+    const bool even = fg.nx % 2 == 0 && fg.ny % 2 == 0 && fg.nz % 2 == 0 && fg.nx >= 4 && fg.ny >= 4 && fg.nz >= 4;
+    if (fine && even && std::getenv("MHDP_SCALAR_SETUP") == nullptr)
+        go<K_k_mg_build_even>(s, grid_blocks(cg.n), {cg, fg, meanfree, mask, fdinv, fbx, fby, fbz, freact, cbx, cby, cbz, creact, cdiag,
+                                                     cdinv, {o16[0], o16[1], o16[2], o16[3], o16[4]}});
+    else if (fine)
         go<K_k_mg_build<true>>(s, grid_blocks(cg.n), {cg, fg, meanfree, mask, fdinv, fbx, fby, fbz, freact, cbx, cby, cbz, creact, cdiag, cdinv,
                                                       {o16[0], o16[1], o16[2], o16[3], o16[4]}});
     else
