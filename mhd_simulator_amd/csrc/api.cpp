@@ -569,6 +569,11 @@ uint32_t next_batch(double rr_prev, uint32_t it_prev, const Scal& v) {
     return (uint32_t)std::min(8.0, std::ceil(left));
 }
 
+// The fused iteration applies the operator with the brick kernels, whose first iteration of a
+// solve reads no old search direction: the starts then leave p and p2 unset.
+bool brick_apply(const mhdp_ctx* c) { return c->brick_enabled && brick_ok(c->g); }
+float* start_p(mhdp_ctx* c, float* p) { return brick_apply(c) ? nullptr : p; }
+
 // The PCG loop of FluidState::project (mod.rs:1015-1104) on the FP32 vectors x (from 0),
 // right-hand side c->b.  `reference_checks`: refresh on convergence and the independent
 // physical check every 64 iterations, as the reference; the inner solves of the mixed
@@ -589,7 +594,7 @@ int pcg(mhdp_ctx* c, Solve& S, bool want_mg, uint32_t max_iter, bool reference_c
         S.lv = level_view(c);
         S.fused = S.lv.size() >= 2 && c->fused_enabled;
     }
-    if (!conv && S.fused) {  // finite search directions for the first p = z + 0 p
+    if (!conv && S.fused && !brick_apply(c)) {  // finite search directions for the first p = z + 0 p
         launch_fill_f32(c->stream, c->g.n, c->p, 0.0f);
         launch_fill_f32(c->stream, c->g.n, c->p2, 0.0f);
     }
@@ -644,10 +649,10 @@ void outer_start(mhdp_ctx* c, hipStream_t s, bool meanfree, bool zero_x) {
     const bool formable = (!c->phase.two_phase || c->has_alpha) && formed_mobility_env();
     if (formable && launch_outer_start_m(s, c->g, c->x64, c->b64, c->phase.two_phase ? c->alpha : nullptr,
                                          open8_of(c), c->phase.rho_liquid, c->phase.rho_air, meanfree, c->r, c->x,
-                                         c->p, c->p2, c->part, c->sc, zero_x))
+                                         start_p(c, c->p), start_p(c, c->p2), c->part, c->sc, zero_x))
         return;
-    launch_outer_start(s, c->g, c->x64, c->b64, c->bx, c->by, c->bz, c->dinv, c->r, c->x, c->p, c->p2, c->part, c->sc,
-                       zero_x);
+    launch_outer_start(s, c->g, c->x64, c->b64, c->bx, c->by, c->bz, c->dinv, c->r, c->x, start_p(c, c->p),
+                       start_p(c, c->p2), c->part, c->sc, zero_x);
 }
 
 // FP64 iterative refinement over the fused multigrid path.  Each outer step is one residual
@@ -837,7 +842,7 @@ int project_once(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, const Cycl
         // needed at all (a zero or already-met residual never allocates it, mod.rs:1031-1043).
         // From a zero potential it is the right-hand side itself: one pass forms both.
         if (warm || !launch_rhs_start(c->stream, c->g, c->grid.h, open8_of(c), ustar, c->b64, meanfree, c->r, c->x,
-                                      c->p, c->p2, c->part, c->sc)) {
+                                      start_p(c, c->p), start_p(c, c->p2), c->part, c->sc)) {
             launch_rhs_f64(c->stream, c->g, c->grid.h, c->mask, open8_of(c), c->bx, c->by, c->bz, ustar, c->b64);
             // a warm start is the refinement begun from the previous potential (the FP64 residual
             // then sees exactly b - L phi_old; faces are corrected from u* with the full potential)
@@ -1209,7 +1214,7 @@ int project_lockstep(mhdp_ctx* const* ctxs, int count, const float* const* ustar
                 prepare(c, q.S.meanfree, true);
                 // the first refinement step starts from a zero potential: its residual pass is the RHS pass
                 q.started = launch_rhs_start(s, c->g, c->grid.h, open8_of(c), ustar[p], c->b64, q.S.meanfree, c->r,
-                                             c->x, c->p, c->p2, c->part, c->sc);
+                                             c->x, start_p(c, c->p), start_p(c, c->p2), c->part, c->sc);
                 if (!q.started)
                     launch_rhs_f64(s, c->g, c->grid.h, c->mask, open8_of(c), c->bx, c->by, c->bz, ustar[p], c->b64);
                 build_mg(c, q.S.meanfree ? 1 : 0, true);

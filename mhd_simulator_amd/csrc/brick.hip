@@ -97,6 +97,20 @@ __device__ __forceinline__ void load_rows(const Geo& o, const T* __restrict__ p,
         r.zp[d] = ldr4(p, o.zpr[d]);
     }
 }
+// The rows of an array the pass must not read, as zeros.
+__device__ __forceinline__ void zero_rows(Rows<float>& r) {
+    const R4<float> z4{make_float4(0.0f, 0.0f, 0.0f, 0.0f)};
+#pragma unroll
+    for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+        for (int dy = 0; dy < 2; dy++) {
+            r.c[dz][dy] = z4;
+            r.xm[dz][dy] = R1<float>{0.0f};
+            r.xp[dz][dy] = R1<float>{0.0f};
+        }
+#pragma unroll
+    for (int d = 0; d < 2; d++) r.ym[d] = r.yp[d] = r.zm[d] = r.zp[d] = z4;
+}
 template <class T>
 __device__ __forceinline__ void fence_rows(Rows<T>& r) {
 #pragma unroll
@@ -342,6 +356,10 @@ struct K_k_brick_apply {
         const bool pending = sc->pending != 0;
         const float alpha = (float)sc->alpha;
         const float pa = (pending && meanfree) ? pold[sc->anchor] : 0.0f;
+        // a solve's first iteration (beta = 0, nothing owed) takes the old search direction as
+        // zeros: a refinement step's start leaves it unset (the loads stay unconditional, so that
+        // they issue with the others and no wait depends on a branch)
+        const bool first = it == 0u;
         uint32_t lo, hi;
         block_range(vg, g.n >> 4, lo, hi);
         double acc[2] = {0.0, 0.0};
@@ -372,6 +390,7 @@ struct K_k_brick_apply {
                     fence(dr[dz][dy]);
                     fence(xr[dz][dy]);
                 }
+            if (first) zero_rows(pr);
             Wts w;
             weights(o, wr, w);
             Field f;
@@ -452,6 +471,10 @@ struct K_k_brick_apply_m {
         const bool pending = sc->pending != 0;
         const float alpha = (float)sc->alpha;
         const float pa = (pending && meanfree) ? pold[sc->anchor] : 0.0f;
+        // a solve's first iteration (beta = 0, nothing owed) takes the old search direction as
+        // zeros: a refinement step's start leaves it unset (the loads stay unconditional, so that
+        // they issue with the others and no wait depends on a branch)
+        const bool first = it == 0u;
         uint32_t lo, hi;
         block_range(vg, g.n >> 4, lo, hi);
         double acc[2] = {0.0, 0.0};
@@ -490,6 +513,7 @@ struct K_k_brick_apply_m {
                 }
 #pragma unroll
             for (int d = 0; d < 2; d++) asm volatile("" : "+v"(oby[d]), "+v"(obz[d]));
+            if (first) zero_rows(pr);
             // densities of the brick and its face halo, then the mobilities of every face
             Field dn;
             if (TWO) field2(ar, ar, dn, [&](const F4& a, const F4&, int) {

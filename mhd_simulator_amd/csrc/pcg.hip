@@ -441,8 +441,10 @@ struct K_k_outer_start {
             const float rv = dinv[i] != 0.0f ? (float)t : 0.0f;
             r32[i] = rv;
             x32[i] = 0.0f;
-            p0[i] = 0.0f;
-            p1[i] = 0.0f;
+            if (p0) {  // null: the brick apply does not read the old search direction at a start
+                p0[i] = 0.0f;
+                p1[i] = 0.0f;
+            }
             acc[0] += (double)rv * (double)rv;
             acc[1] += t * t;
             acc[2] += bi * bi;
@@ -586,8 +588,10 @@ struct K_k_outer_start4 {
             }
             st4f(r32, r.i0, rv);
             st4f(x32, r.i0, zero4);
-            st4f(p0, r.i0, zero4);
-            st4f(p1, r.i0, zero4);
+            if (p0) {  // null: the brick apply does not read the old search direction at a start
+                st4f(p0, r.i0, zero4);
+                st4f(p1, r.i0, zero4);
+            }
         }
         reduce_finish<4>(vg, acc, part, sc, 5, FIN_OUTER);
     }
@@ -698,8 +702,10 @@ struct K_k_outer_start4m {
             }
             st4f(A_.r32, r.i0, rv);
             st4f(A_.x32, r.i0, zero4);
-            st4f(A_.p0, r.i0, zero4);
-            st4f(A_.p1, r.i0, zero4);
+            if (A_.p0) {  // null: the brick apply does not read the old search direction at a start
+                st4f(A_.p0, r.i0, zero4);
+                st4f(A_.p1, r.i0, zero4);
+            }
         }
         reduce_finish<4>(vg, acc, A_.part, A_.sc, 5, FIN_OUTER);
     }

@@ -793,8 +793,10 @@ struct K_k_rhs_start4 {
             }
             st4f(A_.r32, r.i0, rv);
             st4f(A_.x32, r.i0, zero4);
-            st4f(A_.p0, r.i0, zero4);
-            st4f(A_.p1, r.i0, zero4);
+            if (A_.p0) {  // null: the brick apply does not read the old search direction at a start
+                st4f(A_.p0, r.i0, zero4);
+                st4f(A_.p1, r.i0, zero4);
+            }
         }
         reduce_finish<4>(vg, acc, A_.part, A_.sc, 5, FIN_OUTER);
     }
