@@ -491,7 +491,10 @@ static void vcycle_coarse_t(hipStream_t s, int L, LevelDev* lv, Scal* sc, void**
     };
     const dim3 B(MHDP_BLOCK);
     const int store = f16 ? STORE_BF16 : STORE_F32;
-    for (int l = 1; l < L - 1; l++) {
+    // the levels from T down, with the coarsest solve, run as one launch
+    const int T = rb ? rb_tail_level(L, lv) : L;
+    const int down = T < L ? T : L - 1;
+    for (int l = 1; l < down; l++) {
         LevelDev& a = lv[l];
         LevelDev& c = lv[l + 1];
         const LevelKind kind = level_kind(a, rb);
@@ -507,11 +510,13 @@ static void vcycle_coarse_t(hipStream_t s, int L, LevelDev* lv, Scal* sc, void**
         go<K_k_dstart<CT, VT, VT>>(s, grid_blocks(a.g.n), {a.g.n, (const VT*)a.r, C(a, 4), (VT*)a.z, sc});
         go<K_k_restrict_coarse<CT, VT>>(s, grid_blocks(c.g.n), {c.g, a.g, (const VT*)a.r, C(a, 0), C(a, 1), C(a, 2), C(a, 3), (const VT*)a.z, (VT*)c.r, sc});
     }
-    {
+    if (T < L) {
+        launch_rb_tail(s, store, sizeof(AT) == 8, L, lv, T, sc);
+    } else {
         LevelDev& c = lv[L - 1];
         go<K_k_coarse<CT, VT, VT, AT, false>>(s, 1, {c.g, (const VT*)c.r, C(c, 4), C(c, 0), C(c, 1), C(c, 2), C(c, 3), (VT*)c.z, sc});
     }
-    for (int l = L - 2; l >= 1; l--) {
+    for (int l = down - 1; l >= 1; l--) {
         LevelDev& a = lv[l];
         LevelDev& c = lv[l + 1];
         const VT* zc = (const VT*)(l + 1 == L - 1 ? c.z : c.t);

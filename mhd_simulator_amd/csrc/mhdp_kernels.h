@@ -183,6 +183,12 @@ void launch_march_restrict_update(hipStream_t s, const Grid3& cg, const Grid3& g
 // deferred mean and, for the restriction, the PCG residual update from r/r2 (Scal.rsel) into the
 // other buffer when q is given).  rb_level_ok: every periodic axis of the level has even length.
 bool rb_level_ok(const Grid3& g);
+// The smallest red-black levels and the coarsest solve of a cycle in one launch of one block:
+// rb_tail_level gives the first level the tail takes (L: none), launch_rb_tail runs levels T..L-1
+// from the residual of level T and writes level T's correction, as the per-level passes do.
+// MHDP_RB_TAIL=0 keeps the per-level passes.
+int rb_tail_level(int L, const LevelDev* lv);
+void launch_rb_tail(hipStream_t s, int store, bool coarse_f64, int L, LevelDev* lv, int T, Scal* sc);
 void launch_rb_restrict(hipStream_t s, int store, bool coarse, const Grid3& cg, const Grid3& g, const void* r,
                         const void* r2, float* w0, float* w1, const float* q, const void* dinv, const void* bx,
                         const void* by, const void* bz, void* rc, double* part, Scal* sc);

@@ -651,6 +651,339 @@ struct K_rbp {
     }
 };
 
+// ---------------------------------------------------------------- tail of the cycle
+//
+// The smallest red-black levels and the coarsest solve in one launch of one block.  On levels of a
+// few thousand cells each pass above is a z-march of a handful of planes, one barrier and one
+// memory round trip per plane: 6-8 us per launch, two launches per level and visit.  Here every
+// value a level needs (1/diag, residual, the six face couplings of each cell) is loaded once, all
+// levels' loads issued together, and kept in registers; a level's values live in LDS, and each
+// sweep is one barrier.  Down: pre-smoother (red = u, black sweep), red residuals, restriction;
+// the coarsest level's weighted Jacobi sweeps (K_k_coarse); up: prolongation and post-smoother
+// (black, then red).  The arithmetic is the passes' (FP32, faces in the order x-, x+, y-, y+, z-,
+// z+, the same sums; stored values rounded to the storage type where the passes store them), so
+// only the order of a few FP32 roundings may differ.
+constexpr int kTailLevels = 3;           // red-black levels in the tail, at most
+constexpr uint32_t kTailCells = 4096;    // largest level the tail takes
+constexpr uint32_t kTailCoarsest = 64;   // largest coarsest level (every axis at most 4)
+constexpr int kTailThreads = 1024;
+constexpr int kTailJ = (int)(kTailCells / kTailThreads);  // cells per thread on the first tail level
+
+template <class CT, class VT, class AT>
+struct K_rb_tail {
+    static constexpr int THREADS = kTailThreads;
+    static constexpr int MINB = 1;
+    struct Args {
+        int levels;                        // red-black levels (1..kTailLevels); then the coarsest
+        Grid3 g[kTailLevels];
+        const CT* d[kTailLevels];          // 1/diag
+        const CT* bx[kTailLevels];         // lower-face couplings
+        const CT* by[kTailLevels];
+        const CT* bz[kTailLevels];
+        Grid3 gc;                          // the coarsest level
+        const CT* cd;
+        const CT* cbx;
+        const CT* cby;
+        const CT* cbz;
+        const CT* react;
+        const VT* r0;                      // residual of the first tail level
+        VT* t0;                            // its correction (written; deeper levels stay in LDS)
+        Scal* sc;
+    };
+
+    // neighbour coordinate along one axis, or -1 past a closed end
+    static __device__ __forceinline__ int nbr(int v, int n, bool per) {
+        if (v < 0) return per ? v + n : -1;
+        if (v >= n) return per ? v - n : -1;
+        return v;
+    }
+    // the six neighbours of cell i = (x, y, z) in face order (the cell itself past a closed end)
+    static __device__ __forceinline__ void nbrs(const Grid3& g, int x, int y, int z, uint32_t i, uint32_t (&nb)[6],
+                                                bool (&ok)[6]) {
+        const bool px = g.per & 1u, py = g.per & 2u, pz = g.per & 4u;
+        const uint32_t sy = (uint32_t)g.nx, sz = (uint32_t)g.nx * (uint32_t)g.ny;
+        const int c[6] = {nbr(x - 1, g.nx, px), nbr(x + 1, g.nx, px), nbr(y - 1, g.ny, py),
+                          nbr(y + 1, g.ny, py), nbr(z - 1, g.nz, pz), nbr(z + 1, g.nz, pz)};
+        const uint32_t base[3] = {i - (uint32_t)x, i - (uint32_t)y * sy, i - (uint32_t)z * sz};
+        const uint32_t str[3] = {1u, sy, sz};
+#pragma unroll
+        for (int f = 0; f < 6; f++) {
+            ok[f] = c[f] >= 0;
+            nb[f] = ok[f] ? base[f >> 1] + (uint32_t)c[f] * str[f >> 1] : i;
+        }
+    }
+    // the six face couplings: lower faces the cell's own, upper faces the neighbour's lower ones
+    static __device__ __forceinline__ void couplings(const CT* bx, const CT* by, const CT* bz, uint32_t i,
+                                                     const uint32_t (&nb)[6], const bool (&ok)[6], float (&b)[6]) {
+        b[0] = ok[0] ? ldf(bx[i]) : 0.0f;
+        b[1] = ok[1] ? ldf(bx[nb[1]]) : 0.0f;
+        b[2] = ok[2] ? ldf(by[i]) : 0.0f;
+        b[3] = ok[3] ? ldf(by[nb[3]]) : 0.0f;
+        b[4] = ok[4] ? ldf(bz[i]) : 0.0f;
+        b[5] = ok[5] ? ldf(bz[nb[5]]) : 0.0f;
+    }
+    static __device__ __forceinline__ uint32_t pack(int x, int y, int z) {
+        return (uint32_t)x | ((uint32_t)y << 10) | ((uint32_t)z << 20);
+    }
+    // sum over the faces of coupling * v(neighbour), in face order
+    static __device__ __forceinline__ float fsum(const Grid3& g, uint32_t xyz, uint32_t i, const float (&b)[6],
+                                                 const float* v) {
+        uint32_t nb[6];
+        bool ok[6];
+        nbrs(g, (int)(xyz & 1023u), (int)((xyz >> 10) & 1023u), (int)(xyz >> 20), i, nb, ok);
+        float s = b[0] * v[nb[0]];
+#pragma unroll
+        for (int f = 1; f < 6; f++) s += b[f] * v[nb[f]];
+        return s;
+    }
+    static __device__ __forceinline__ float rnd(float v) { return ldf(cvt<VT>(v)); }
+
+    static __device__ void run(const Args& A, const VGrid& vg) {
+        (void)vg;
+        Scal* sc = A.sc;
+        if (CONVERGED(sc)) return;
+        __shared__ float S[kTailCells];            // the current level's values
+        __shared__ float TC[kTailCells / 8];       // the level below's correction (prolonged up)
+        __shared__ AT cz[kTailCoarsest];
+        const float ps = (float)sc->pscale;
+        const uint32_t tid = threadIdx.x;
+        const int L = A.levels;
+        // ---- every level's values, loaded together: 1/diag and face couplings, and the first
+        // level's residual (deeper residuals come from the restriction)
+        float D[kTailLevels][kTailJ], R[kTailLevels][kTailJ], B[kTailLevels][kTailJ][6];
+        uint32_t XYZ[kTailLevels][kTailJ];
+#pragma unroll
+        for (int k = 0; k < kTailLevels; k++)
+#pragma unroll
+            for (int j = 0; j < kTailJ; j++) {
+                D[k][j] = R[k][j] = 0.0f;
+                XYZ[k][j] = 0u;
+#pragma unroll
+                for (int f = 0; f < 6; f++) B[k][j][f] = 0.0f;
+                if (k >= L || (k > 0 && j > 0)) continue;
+                const Grid3 g = A.g[k];
+                const uint32_t i = tid + (uint32_t)j * kTailThreads;
+                if (i >= g.n) continue;
+                int x, y, z;
+                decode(g, i, x, y, z);
+                XYZ[k][j] = pack(x, y, z);
+                D[k][j] = ldf(A.d[k][i]);
+                if (k == 0) R[k][j] = ldf(A.r0[i]);
+                uint32_t nb[6];
+                bool ok[6];
+                nbrs(g, x, y, z, i, nb, ok);
+                couplings(A.bx[k], A.by[k], A.bz[k], i, nb, ok, B[k][j]);
+            }
+        // the coarsest level: K_k_coarse's values
+        const Grid3 gc = A.gc;
+        const bool in_c = tid < gc.n;
+        AT crhs = 0, cdi = 0, crc = 0;
+        AT cb[6] = {0, 0, 0, 0, 0, 0};
+        uint32_t cnb[6] = {0, 0, 0, 0, 0, 0};
+        if (in_c) {
+            int x, y, z;
+            decode(gc, tid, x, y, z);
+            bool ok[6];
+            float b[6];
+            nbrs(gc, x, y, z, tid, cnb, ok);
+            couplings(A.cbx, A.cby, A.cbz, tid, cnb, ok, b);
+            cdi = (AT)ldf(A.cd[tid]);
+            crc = (AT)ldf(A.react[tid]);
+#pragma unroll
+            for (int f = 0; f < 6; f++) cb[f] = (AT)b[f];
+        }
+        // ---- down: pre-smoother, red residuals, restriction
+#pragma unroll
+        for (int k = 0; k < kTailLevels; k++) {
+            if (k >= L) break;
+            const Grid3 g = A.g[k];
+            const Grid3 cg = k + 1 < L ? A.g[k + 1 < kTailLevels ? k + 1 : 0] : gc;
+#pragma unroll
+            for (int j = 0; j < kTailJ; j++) {
+                const uint32_t i = tid + (uint32_t)j * kTailThreads;
+                if ((k == 0 || j == 0) && i < g.n) S[i] = D[k][j] * R[k][j];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < kTailJ; j++) {  // black sweep
+                const uint32_t i = tid + (uint32_t)j * kTailThreads;
+                const uint32_t xyz = XYZ[k][j];
+                const bool red = (((xyz & 1023u) + ((xyz >> 10) & 1023u) + (xyz >> 20)) & 1u) == 0u;
+                if ((k == 0 || j == 0) && i < g.n && !red)
+                    S[i] = D[k][j] * R[k][j] + D[k][j] * fsum(g, xyz, i, B[k][j], S);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < kTailJ; j++) {  // red residuals (black residuals are zero)
+                const uint32_t i = tid + (uint32_t)j * kTailThreads;
+                const uint32_t xyz = XYZ[k][j];
+                const bool red = (((xyz & 1023u) + ((xyz >> 10) & 1023u) + (xyz >> 20)) & 1u) == 0u;
+                if ((k == 0 || j == 0) && i < g.n && red) {
+                    const float s = fsum(g, xyz, i, B[k][j], S);
+                    S[i] = D[k][j] != 0.0f ? s : 0.0f;
+                }
+            }
+            __syncthreads();
+            // restriction: the coarse cell's red children, (row y0: z0 + z1) + (row y1: z0 + z1)
+            if (tid < cg.n) {
+                int cx, cy, cz_;
+                decode(cg, tid, cx, cy, cz_);
+                float rows[2] = {0.0f, 0.0f};
+#pragma unroll
+                for (int dy = 0; dy < 2; dy++) {
+                    const int y = 2 * cy + dy;
+                    if (y >= g.ny) continue;
+                    float sa = 0.0f;
+#pragma unroll
+                    for (int dz = 0; dz < 2; dz++) {
+                        const int z = 2 * cz_ + dz;
+                        if (z >= g.nz) continue;
+                        const int x = 2 * cx + ((y + z) & 1);
+                        if (x >= g.nx) continue;
+                        sa += S[(uint32_t)x + (uint32_t)g.nx * ((uint32_t)y + (uint32_t)g.ny * (uint32_t)z)];
+                    }
+                    rows[dy] = sa;
+                }
+                const float rc = rnd(rows[0] + rows[1]);
+                if (k + 1 < L) R[k + 1 < kTailLevels ? k + 1 : 0][0] = rc;
+                else crhs = cdi != (AT)0 ? (AT)rc : (AT)0;
+            }
+            __syncthreads();
+        }
+        // ---- the coarsest level: eight weighted Jacobi sweeps from zero
+        if (tid < kTailCoarsest) cz[tid] = 0;
+        __syncthreads();
+        {
+            const AT omega = (AT)sc->omega;
+            for (int step = 0; step < 8; step++) {
+                AT next = 0;
+                if (in_c) {
+                    const AT center = cz[tid];
+                    if (cdi != (AT)0) {
+                        AT a = crc * center;
+#pragma unroll
+                        for (int f = 0; f < 6; f++) a += cb[f] * (center - cz[cnb[f]]);
+                        next = center + omega * (crhs - a) * cdi;
+                    }
+                }
+                __syncthreads();
+                if (in_c) cz[tid] = next;
+                __syncthreads();
+            }
+            if (in_c) TC[tid] = rnd((float)cz[tid]);
+        }
+        __syncthreads();
+        // ---- up: prolongation, post-smoother (black, then red)
+#pragma unroll
+        for (int k = kTailLevels - 1; k >= 0; k--) {
+            if (k >= L) continue;
+            const Grid3 g = A.g[k];
+            const Grid3 cg = k + 1 < L ? A.g[k + 1 < kTailLevels ? k + 1 : 0] : gc;
+#pragma unroll
+            for (int j = 0; j < kTailJ; j++) {
+                const uint32_t i = tid + (uint32_t)j * kTailThreads;
+                if (!(k == 0 || j == 0) || i >= g.n) continue;
+                const uint32_t xyz = XYZ[k][j];
+                const int x = (int)(xyz & 1023u), y = (int)((xyz >> 10) & 1023u), z = (int)(xyz >> 20);
+                float u = D[k][j] * R[k][j];
+                if (((x + y + z) & 1) == 0) {
+                    const uint32_t pc = (uint32_t)(x >> 1) + (uint32_t)cg.nx * ((uint32_t)(y >> 1) + (uint32_t)cg.ny * (uint32_t)(z >> 1));
+                    const float zcv = ps * TC[pc];
+                    u += D[k][j] != 0.0f ? zcv : 0.0f;
+                }
+                S[i] = u;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < kTailJ; j++) {  // black sweep
+                const uint32_t i = tid + (uint32_t)j * kTailThreads;
+                const uint32_t xyz = XYZ[k][j];
+                const bool red = (((xyz & 1023u) + ((xyz >> 10) & 1023u) + (xyz >> 20)) & 1u) == 0u;
+                if ((k == 0 || j == 0) && i < g.n && !red)
+                    S[i] = D[k][j] * R[k][j] + D[k][j] * fsum(g, xyz, i, B[k][j], S);
+            }
+            __syncthreads();
+            float o[kTailJ];
+#pragma unroll
+            for (int j = 0; j < kTailJ; j++) {  // red sweep; the level's correction
+                const uint32_t i = tid + (uint32_t)j * kTailThreads;
+                const uint32_t xyz = XYZ[k][j];
+                const bool red = (((xyz & 1023u) + ((xyz >> 10) & 1023u) + (xyz >> 20)) & 1u) == 0u;
+                o[j] = 0.0f;
+                if (!(k == 0 || j == 0) || i >= g.n) continue;
+                o[j] = red ? D[k][j] * R[k][j] + D[k][j] * fsum(g, xyz, i, B[k][j], S) : S[i];
+            }
+            // TC still holds the level below's correction until every thread has read it (above)
+#pragma unroll
+            for (int j = 0; j < kTailJ; j++) {
+                const uint32_t i = tid + (uint32_t)j * kTailThreads;
+                if (!(k == 0 || j == 0) || i >= g.n) continue;
+                if (k == 0) A.t0[i] = cvt<VT>(o[j]);
+                else TC[i] = rnd(o[j]);
+            }
+            __syncthreads();
+        }
+    }
+};
+
+// The first level of the tail (L when there is none): the largest level from 2 on of at most
+// kTailCells cells such that it and every level below it but the coarsest are red-black levels,
+// at most kTailLevels of them, and the coarsest level fits the tail's solve.
+int rb_tail_level(int L, const LevelDev* lv) {
+    static const bool on = [] {
+        const char* e = getenv("MHDP_RB_TAIL");
+        return !(e && e[0] == '0');
+    }();
+    if (!on || L < 4 || lv[L - 1].g.n > kTailCoarsest) return L;
+    auto fits = [](const Grid3& g) { return g.n <= kTailCells && g.nx < 1024 && g.ny < 1024 && g.nz < 1024; };
+    int T = L - 1;
+    while (T - 1 >= 2 && fits(lv[T - 1].g) && rb_level_ok(lv[T - 1].g) && L - 1 - (T - 1) <= kTailLevels &&
+           (T - 1 == L - 2 || lv[T].g.n <= kTailCells / 8))
+        T--;
+    return T < L - 1 ? T : L;
+}
+
+template <class CT, class VT, class AT>
+static void rb_tail_go(hipStream_t s, bool f16, int L, LevelDev* lv, int T, Scal* sc) {
+    typename K_rb_tail<CT, VT, AT>::Args a{};
+    a.levels = L - 1 - T;
+    auto arr = [&](const LevelDev& l, int which) {
+        void* const a16[5] = {l.bx16, l.by16, l.bz16, l.react16, l.dinv16};
+        float* const a32[5] = {l.bx, l.by, l.bz, l.react, l.dinv};
+        return f16 ? (const CT*)a16[which] : (const CT*)a32[which];
+    };
+    for (int k = 0; k < a.levels; k++) {
+        const LevelDev& l = lv[T + k];
+        a.g[k] = l.g;
+        a.bx[k] = arr(l, 0);
+        a.by[k] = arr(l, 1);
+        a.bz[k] = arr(l, 2);
+        a.d[k] = arr(l, 4);
+    }
+    const LevelDev& c = lv[L - 1];
+    a.gc = c.g;
+    a.cbx = arr(c, 0);
+    a.cby = arr(c, 1);
+    a.cbz = arr(c, 2);
+    a.react = arr(c, 3);
+    a.cd = arr(c, 4);
+    a.r0 = (const VT*)lv[T].r;
+    a.t0 = (VT*)lv[T].t;
+    a.sc = sc;
+    go<K_rb_tail<CT, VT, AT>>(s, 1, a);
+}
+
+void launch_rb_tail(hipStream_t s, int store, bool coarse_f64, int L, LevelDev* lv, int T, Scal* sc) {
+    using T16 = __hip_bfloat16;
+    if (store == STORE_BF16) {
+        if (coarse_f64) rb_tail_go<T16, T16, double>(s, true, L, lv, T, sc);
+        else rb_tail_go<T16, T16, float>(s, true, L, lv, T, sc);
+    } else {
+        if (coarse_f64) rb_tail_go<float, float, double>(s, false, L, lv, T, sc);
+        else rb_tail_go<float, float, float>(s, false, L, lv, T, sc);
+    }
+}
+
 // ---------------------------------------------------------------- launchers
 
 bool rb_level_ok(const Grid3& g) {
