@@ -99,7 +99,7 @@ typedef struct {
                             symmetric red-black Gauss-Seidel (red then black before the coarse
                             correction, black then red after it) on every level whose periodic
                             axes have even length (Jacobi on the others); AUTO = red-black on
-                            grids of at least 2^21 cells, Jacobi below.
+                            grids of at least 2^15 cells, Jacobi below.
                             The coarse scale's tuned default is 1.6 with Jacobi, 1.8 with
                             red-black (1.9 on grids of at least 2^24 cells); the fused multigrid
                             path only (the one-kernel-per-stage path, MHDP_UNFUSED=1, keeps

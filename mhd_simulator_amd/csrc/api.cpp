@@ -527,10 +527,11 @@ constexpr double kTunedScaleRB = 1.8; // the same with red-black Gauss-Seidel sm
 // ... and on grids of at least 2^24 cells, whose deep hierarchies keep the red-black cycle's
 // level-1 spectrum well inside the positive-definite range (512^3: 12 instead of 13 iterations)
 constexpr double kTunedScaleRBLarge = 1.9;
-// The automatic smoother: red-black Gauss-Seidel on grids of at least 2^21 cells (128^3: 18 instead
-// of 27 iterations, 2.9 instead of 3.3 ms; 512^3: 12 instead of 21), weighted Jacobi below (at 64^3
-// the two take the same time, and the 64-problem batch of 32^3..96^3 grids is faster with Jacobi).
-constexpr uint32_t kRedBlackAutoCells = 1u << 21;
+// The automatic smoother: red-black Gauss-Seidel on grids of at least 2^15 cells (64^3: 14 instead
+// of 21 iterations, 1.60 instead of 1.81 ms; 128^3: 2.7 instead of 3.3 ms; 512^3: 12 instead of
+// 21 iterations; the 64-problem batch of 32^3..96^3 grids: 20.1 instead of 20.6 ms), weighted
+// Jacobi on smaller grids, whose few levels are all at the launch floor.
+constexpr uint32_t kRedBlackAutoCells = 1u << 15;
 // Grids of at least 2^24 cells take the larger red-black coarse scale.
 constexpr uint32_t kRedBlackLargeCells = 1u << 24;
 // Level-1 visits per cycle when the caller leaves the choice (coarse_visits = 0): a second visit

@@ -261,8 +261,8 @@ def main(argv=None):
         prs = [PressureProjection(q.grid, q.fluid, q.alpha) for q in ps]
         us = [torch.from_numpy(q.ustar).cuda() for q in ps]
         res = tune_batch(prs, us, refine_tol=1e-8, repeats=a.repeats, cands=cands, precond="mg")
-    # the library's automatic smoother: red-black on grids of at least 2^21 cells, else Jacobi
-    auto_rb = a.workload == "plate" and a.size ** 3 >= (1 << 21)
+    # the library's automatic smoother: red-black on grids of at least 2^15 cells, else Jacobi
+    auto_rb = a.workload == "batch64" or a.size ** 3 >= (1 << 15)
     builtin = ({"coarse_visits": 0}, {"coarse_visits": 0, "smoother": "red-black" if auto_rb else "jacobi"})
     default = next((r for r in res["table"] if r["launch"] == BUILTIN_LAUNCH and r["opts"] in builtin), None)
     for r in sorted(res["table"], key=lambda r: (r["ms"] is None, r["ms"] or 0.0)):

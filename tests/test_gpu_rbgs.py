@@ -121,3 +121,44 @@ def test_red_black_needs_fewer_iterations(name):
             assert res <= 1e-8
         print(f"[gpu] {name} iterations jacobi {its['jacobi']} red-black {its['red-black']}", flush=True)
         assert its["red-black"] < its["jacobi"]
+
+
+TAIL_CASES = ["plate-32", "droplets-48", "nozzle-40", "mixed-periodic"]
+
+
+def tail_cycles():
+    """One FP32 and one BF16 red-black cycle per case, in this process: {case/precision: z}."""
+    out = {}
+    for name in TAIL_CASES:
+        p = problem(name)
+        with PressureProjection(p.grid, p.fluid, p.alpha) as pr:
+            meanfree = pr.components == 1
+            i = np.arange(p.grid.cells)
+            r = np.where(pr.mask != 0, np.cos(i * 0.17), 0).astype(np.float32)
+            kw = dict(coarse_scale=1.8, coarse_visits=2, smoother="red-black")
+            out[f"{name}/fp32"] = pr.vcycle(r, meanfree, **kw).astype(np.float64)
+            out[f"{name}/mixed"] = pr.vcycle(r, meanfree, precision="mixed", **kw).astype(np.float64)
+    return out
+
+
+def test_cycle_tail_equals_per_level_passes(tmp_path):
+    """The single-launch tail (the levels of at most 4096 cells and the coarsest solve) against the
+    per-level passes (MHDP_RB_TAIL=0, in a second process): the same cycle up to the order of a few
+    FP32 roundings (1e-6 FP32; BF16 storage: 1e-3, a rounding of a stored value may flip)."""
+    import os
+    import subprocess
+    import sys
+    here = tail_cycles()
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    dst = tmp_path / "levels.npz"
+    code = ("import sys, numpy as np; sys.path.insert(0, %r); sys.path.insert(0, %r); "
+            "import test_gpu_rbgs as T; np.savez(%r, **T.tail_cycles())") % (root, os.path.join(root, "tests"), str(dst))
+    print("[gpu] per-level cycle in a second process", flush=True)
+    res = subprocess.run([sys.executable, "-c", code], env=dict(os.environ, MHDP_RB_TAIL="0"), capture_output=True,
+                         text=True, timeout=600)
+    assert res.returncode == 0, res.stderr[-2000:]
+    other = np.load(dst)
+    for key, z in here.items():
+        err = relerr(z, other[key])
+        print(f"[gpu]   {key}: tail vs per-level {err:.2e}", flush=True)
+        assert err < (1e-6 if key.endswith("fp32") else 1e-3), key
