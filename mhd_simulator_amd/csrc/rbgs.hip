@@ -1040,8 +1040,12 @@ static bool rbp_enabled() {
     return on;
 }
 
-// Coarse levels of fewer cells than this run the 16-row tiles.
+// Coarse levels of fewer cells than this run the 16-row tiles, and so do fine levels of fewer than
+// kRbpSmallFine cells (with the fine chunk rule): two blocks per CU hide the latency of their short
+// marches (64^3: 1.58 -> 1.43 ms per mixed projection; the 64-problem batch of 32^3..96^3 grids
+// 20.2 -> 18.2 ms; at 128^3 the two tile heights take the same time).
 constexpr uint32_t kRbpSmallCells = 1u << 25;
+constexpr uint32_t kRbpSmallFine = 1u << 22;
 
 template <class KP, bool COARSE, class A>
 static void rbp_launch(hipStream_t s, A a) {
@@ -1061,8 +1065,10 @@ static void rb_go(hipStream_t s, const typename K_rb<CT, VT, RT, MODE, COARSE, R
     using K = K_rb<CT, VT, RT, MODE, COARSE, RBW_ROWS>;
     typename K::Args a = a0;
     if (rbp_enabled() && (a.g.nx & 1) == 0) {
-        if (COARSE && a.g.n < kRbpSmallCells) rbp_launch<K_rbp<CT, VT, RT, MODE, COARSE, 16>, COARSE>(s, a);
-        else rbp_launch<K_rbp<CT, VT, RT, MODE, COARSE, 32>, COARSE>(s, a);
+        if (a.g.n < (COARSE ? kRbpSmallCells : kRbpSmallFine))
+            rbp_launch<K_rbp<CT, VT, RT, MODE, COARSE, 16>, COARSE>(s, a);
+        else
+            rbp_launch<K_rbp<CT, VT, RT, MODE, COARSE, 32>, COARSE>(s, a);
         return;
     }
     a.zlen = rbw_zlen(a.g);
