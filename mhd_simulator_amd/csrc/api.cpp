@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <numeric>
 #include <string>
 #include <mutex>
@@ -33,6 +34,11 @@ constexpr uint32_t kMgMinCells = 96u * 96u * 96u;  // multigrid.rs:4
 constexpr int kMaxVisits = 2;
 // Status of a solve whose multigrid PCG broke down (r.z or p.Ap <= 0): internal, never returned.
 constexpr int kBreakdown = 100;
+// Iterations without a new least inner residual after which an inner solve with a tuned cycle is
+// taken as broken down.  A cycle that is not positive definite can keep r.z and p.Ap positive
+// while the residual wanders without converging; a converging multigrid PCG (about a decade per
+// iteration) sets a new least residual every few iterations.  The reference's cycle is never cut.
+constexpr uint32_t kStallIterations = 100;
 
 // Multigrid cycle of one solve: smoother weight, coarse-correction scale, level-1 visits.
 struct Cycle {
@@ -399,6 +405,7 @@ struct Solve {
     bool fused = false;  // fused fine-grid passes (multigrid with at least one coarse level)
     int store = STORE_F32;
     bool coarse64 = false;
+    bool stall = false;  // a stalled inner solve counts as a breakdown (tuned cycles)
     std::vector<LevelDev> lv;
 };
 
@@ -692,8 +699,9 @@ int refine_fused(mhdp_ctx* c, Solve& S, const mhdp_opts* o, uint32_t max_outer, 
             if (*outer >= max_outer) return MHDP_NOT_CONVERGED;
         }
         mg_initialize(c, S);
-        uint32_t it = 0, it_prev = 0, batch = 8u;
+        uint32_t it = 0, it_prev = 0, batch = 8u, best_it = 0;
         double rr_prev = c->sc_h->rr;  // the inner solve's initial |r|^2
+        double best = rr_prev;
         for (;;) {
             batch = std::min(batch, o->max_iter - it);
             for (uint32_t k = 0; k < batch; k++) iterate_once(c, S);
@@ -707,6 +715,12 @@ int refine_fused(mhdp_ctx* c, Solve& S, const mhdp_opts* o, uint32_t max_outer, 
             if (seen <= it && !conv) return fail(c, MHDP_NO_PROGRESS, "Fluid pressure iteration made no progress");
             it = seen;
             if (conv || it >= o->max_iter) break;
+            if (v.rr < best) {
+                best = v.rr;
+                best_it = it;
+            } else if (S.stall && it - best_it >= kStallIterations) {
+                return kBreakdown;
+            }
             batch = next_batch(rr_prev, it_prev, v);
             rr_prev = v.rr;
             it_prev = it;
@@ -793,6 +807,7 @@ int project_once(mhdp_ctx* c, const float* ustar, const mhdp_opts* o, const Cycl
     S.meanfree = meanfree;
     S.store = mixed ? STORE_BF16 : STORE_F32;
     S.coarse64 = mixed;
+    S.stall = !is_reference(cy);
     const float nf = (float)std::max<uint32_t>(c->fluid_cells, 1);
 
     c->timer.used = 0;
@@ -1149,6 +1164,8 @@ int project_lockstep(mhdp_ctx* const* ctxs, int count, const float* const* ustar
         Solve S;
         bool running = false, done = false, inner = false;
         uint32_t outer = 0, iterations = 0, it = 0;
+        uint32_t best_it = 0;  // iteration of the inner solve's least residual `best`
+        double best = std::numeric_limits<double>::infinity();
         bool checked = false, breakdown = false;
         bool started = false;  // the set-up pass also ran the first step's residual pass
         double res64 = 0.0;
@@ -1182,6 +1199,7 @@ int project_lockstep(mhdp_ctx* const* ctxs, int count, const float* const* ustar
         q.S.meanfree = o->gauge != MHDP_GAUGE_PINNED && c->components == 1;
         q.S.store = STORE_BF16;
         q.S.coarse64 = true;
+        q.S.stall = !is_reference(cy);
         if (!rc) {
             Scal sc;
             set_solve_params(c, sc, q.S.meanfree, tau * tau, 0.0, 1.0, cy.omega, cy.pscale);
@@ -1280,6 +1298,8 @@ int project_lockstep(mhdp_ctx* const* ctxs, int count, const float* const* ustar
             issue_plan(s, tp);
             for (int p : trans) {
                 ps[(size_t)p].it = 0;
+                ps[(size_t)p].best_it = 0;
+                ps[(size_t)p].best = std::numeric_limits<double>::infinity();
                 ps[(size_t)p].checked = false;
                 ps[(size_t)p].inner = true;
             }
@@ -1336,6 +1356,13 @@ int project_lockstep(mhdp_ctx* const* ctxs, int count, const float* const* ustar
                 q.iterations += q.it;
                 q.outer++;
                 trans.push_back(p);
+            } else if (v.rr < q.best) {
+                q.best = v.rr;
+                q.best_it = q.it;
+            } else if (q.S.stall && q.it - q.best_it >= kStallIterations) {  // stalled: as a breakdown
+                q.breakdown = true;
+                q.running = false;
+                if (!stop(p)) return MHDP_DEVICE;
             }
         }
     }

@@ -663,12 +663,15 @@ struct K_rbp {
 // (black, then red).  The arithmetic is the passes' (FP32, faces in the order x-, x+, y-, y+, z-,
 // z+, the same sums; stored values rounded to the storage type where the passes store them), so
 // only the order of a few FP32 roundings may differ.
-constexpr int kTailLevels = 3;           // red-black levels in the tail, at most
+constexpr int kTailLevels = 2;           // red-black levels in the tail, at most
 constexpr uint32_t kTailCells = 4096;    // largest level the tail takes
 constexpr uint32_t kTailCoarsest = 64;   // largest coarsest level (every axis at most 4)
 constexpr int kTailThreads = 1024;
-constexpr int kTailJ = (int)(kTailCells / kTailThreads);  // cells per thread on the first tail level
+constexpr int kTailJ = (int)(kTailCells / 2 / kTailThreads);  // pairs per thread on the first tail level
 
+// A thread owns x pairs of cells (x even, x + 1; on a row of odd length the last pair holds one
+// cell): one cell of a pair is red and the other black, so each colour's sweep gives every thread
+// one cell per pair.
 template <class CT, class VT, class AT>
 struct K_rb_tail {
     static constexpr int THREADS = kTailThreads;
@@ -725,18 +728,54 @@ struct K_rb_tail {
     static __device__ __forceinline__ uint32_t pack(int x, int y, int z) {
         return (uint32_t)x | ((uint32_t)y << 10) | ((uint32_t)z << 20);
     }
-    // sum over the faces of coupling * v(neighbour), in face order
-    static __device__ __forceinline__ float fsum(const Grid3& g, uint32_t xyz, uint32_t i, const float (&b)[6],
+    // x pairs of a level: ceil(nx / 2) per row (the last one of an odd row holds one cell)
+    static __device__ __forceinline__ uint32_t pairs_of(const Grid3& g) {
+        return (uint32_t)((g.nx + 1) / 2) * (uint32_t)g.ny * (uint32_t)g.nz;
+    }
+    // pair p: its first cell (x even, y, z), the cell's index, whether the second cell exists
+    static __device__ __forceinline__ uint32_t pair_cell(const Grid3& g, uint32_t p, int& x, int& y, int& z, bool& has2) {
+        const uint32_t ppr = (uint32_t)((g.nx + 1) / 2), row = p / ppr;
+        x = (int)(2u * (p - row * ppr));
+        y = (int)(row % (uint32_t)g.ny);
+        z = (int)(row / (uint32_t)g.ny);
+        has2 = x + 1 < g.nx;
+        return row * (uint32_t)g.nx + (uint32_t)x;
+    }
+    // sum over the faces of cell (x0 + h, y, z) of coupling * v(neighbour), in face order; xyz: the
+    // pair's first cell, i its index
+    static __device__ __forceinline__ float fsum(const Grid3& g, uint32_t xyz, int h, uint32_t i, const float (&b)[6],
                                                  const float* v) {
         uint32_t nb[6];
         bool ok[6];
-        nbrs(g, (int)(xyz & 1023u), (int)((xyz >> 10) & 1023u), (int)(xyz >> 20), i, nb, ok);
+        nbrs(g, (int)(xyz & 1023u) + h, (int)((xyz >> 10) & 1023u), (int)((xyz >> 20) & 1023u), i + (uint32_t)h, nb, ok);
         float s = b[0] * v[nb[0]];
 #pragma unroll
         for (int f = 1; f < 6; f++) s += b[f] * v[nb[f]];
         return s;
     }
     static __device__ __forceinline__ float rnd(float v) { return ldf(cvt<VT>(v)); }
+    // the coarse cell c's restricted residual: its red children, (row y0: z0 + z1) + (row y1: ...)
+    static __device__ __forceinline__ float restrict_at(const Grid3& g, const Grid3& cg, uint32_t c, const float* S) {
+        int cx, cy, cz_;
+        decode(cg, c, cx, cy, cz_);
+        float rows[2] = {0.0f, 0.0f};
+#pragma unroll
+        for (int dy = 0; dy < 2; dy++) {
+            const int y = 2 * cy + dy;
+            if (y >= g.ny) continue;
+            float sa = 0.0f;
+#pragma unroll
+            for (int dz = 0; dz < 2; dz++) {
+                const int z = 2 * cz_ + dz;
+                if (z >= g.nz) continue;
+                const int x = 2 * cx + ((y + z) & 1);
+                if (x >= g.nx) continue;
+                sa += S[(uint32_t)x + (uint32_t)g.nx * ((uint32_t)y + (uint32_t)g.ny * (uint32_t)z)];
+            }
+            rows[dy] = sa;
+        }
+        return rnd(rows[0] + rows[1]);
+    }
 
     static __device__ void run(const Args& A, const VGrid& vg) {
         (void)vg;
@@ -748,33 +787,56 @@ struct K_rb_tail {
         const float ps = (float)sc->pscale;
         const uint32_t tid = threadIdx.x;
         const int L = A.levels;
-        // ---- every level's values, loaded together: 1/diag and face couplings, and the first
-        // level's residual (deeper residuals come from the restriction)
-        float D[kTailLevels][kTailJ], R[kTailLevels][kTailJ], B[kTailLevels][kTailJ][6];
-        uint32_t XYZ[kTailLevels][kTailJ];
+        // ---- every level's values, loaded together and kept by colour (red, black) of each owned
+        // pair: 1/diag, face couplings, and the first level's residual (deeper residuals come from
+        // the restriction)
+        float DR[kTailLevels][kTailJ], DB[kTailLevels][kTailJ], RR[kTailLevels][kTailJ], RB[kTailLevels][kTailJ];
+        float BR[kTailLevels][kTailJ][6], BB[kTailLevels][kTailJ][6];
+        uint32_t XYZ[kTailLevels][kTailJ];  // the pair's first cell; bit 31: the red cell is the second
+        uint32_t I0[kTailLevels][kTailJ];   // the first cell's index; bit 31: the second cell exists
 #pragma unroll
         for (int k = 0; k < kTailLevels; k++)
 #pragma unroll
             for (int j = 0; j < kTailJ; j++) {
-                D[k][j] = R[k][j] = 0.0f;
-                XYZ[k][j] = 0u;
+                XYZ[k][j] = I0[k][j] = 0u;
+                DR[k][j] = DB[k][j] = RR[k][j] = RB[k][j] = 0.0f;
 #pragma unroll
-                for (int f = 0; f < 6; f++) B[k][j][f] = 0.0f;
+                for (int f = 0; f < 6; f++) BR[k][j][f] = BB[k][j][f] = 0.0f;
                 if (k >= L || (k > 0 && j > 0)) continue;
                 const Grid3 g = A.g[k];
-                const uint32_t i = tid + (uint32_t)j * kTailThreads;
-                if (i >= g.n) continue;
+                const uint32_t p = tid + (uint32_t)j * kTailThreads;
+                if (p >= pairs_of(g)) continue;
                 int x, y, z;
-                decode(g, i, x, y, z);
-                XYZ[k][j] = pack(x, y, z);
-                D[k][j] = ldf(A.d[k][i]);
-                if (k == 0) R[k][j] = ldf(A.r0[i]);
-                uint32_t nb[6];
-                bool ok[6];
-                nbrs(g, x, y, z, i, nb, ok);
-                couplings(A.bx[k], A.by[k], A.bz[k], i, nb, ok, B[k][j]);
+                bool has2;
+                const uint32_t i0 = pair_cell(g, p, x, y, z, has2);
+                const bool hr = ((y + z) & 1) != 0;
+                XYZ[k][j] = pack(x, y, z) | (hr ? 0x80000000u : 0u);
+                I0[k][j] = i0 | (has2 ? 0x80000000u : 0u);
+                float d[2] = {0.0f, 0.0f}, r[2] = {0.0f, 0.0f}, b[2][6];
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+#pragma unroll
+                    for (int f = 0; f < 6; f++) b[h][f] = 0.0f;
+                    if (h == 1 && !has2) continue;
+                    const uint32_t i = i0 + (uint32_t)h;
+                    d[h] = ldf(A.d[k][i]);
+                    if (k == 0) r[h] = ldf(A.r0[i]);
+                    uint32_t nb[6];
+                    bool ok[6];
+                    nbrs(g, x + h, y, z, i, nb, ok);
+                    couplings(A.bx[k], A.by[k], A.bz[k], i, nb, ok, b[h]);
+                }
+                DR[k][j] = hr ? d[1] : d[0];
+                DB[k][j] = hr ? d[0] : d[1];
+                RR[k][j] = hr ? r[1] : r[0];
+                RB[k][j] = hr ? r[0] : r[1];
+#pragma unroll
+                for (int f = 0; f < 6; f++) {
+                    BR[k][j][f] = hr ? b[1][f] : b[0][f];
+                    BB[k][j][f] = hr ? b[0][f] : b[1][f];
+                }
             }
-        // the coarsest level: K_k_coarse's values
+        // the coarsest level: K_k_coarse's values, one cell per thread of the first wave
         const Grid3 gc = A.gc;
         const bool in_c = tid < gc.n;
         AT crhs = 0, cdi = 0, crc = 0;
@@ -792,68 +854,70 @@ struct K_rb_tail {
 #pragma unroll
             for (int f = 0; f < 6; f++) cb[f] = (AT)b[f];
         }
+        auto red_at = [](uint32_t xyz) { return (uint32_t)(xyz >> 31); };  // offset of the red cell
+        auto first = [](uint32_t i0) { return i0 & 0x7FFFFFFFu; };
+        // the cell at offset h of the pair exists (the second one may not, at the end of an odd row)
+        auto has = [](uint32_t i0, uint32_t h) { return h == 0u || (i0 >> 31) != 0u; };
         // ---- down: pre-smoother, red residuals, restriction
 #pragma unroll
         for (int k = 0; k < kTailLevels; k++) {
             if (k >= L) break;
             const Grid3 g = A.g[k];
-            const Grid3 cg = k + 1 < L ? A.g[k + 1 < kTailLevels ? k + 1 : 0] : gc;
 #pragma unroll
             for (int j = 0; j < kTailJ; j++) {
-                const uint32_t i = tid + (uint32_t)j * kTailThreads;
-                if ((k == 0 || j == 0) && i < g.n) S[i] = D[k][j] * R[k][j];
+                if (!(k == 0 || j == 0) || tid + (uint32_t)j * kTailThreads >= pairs_of(g)) continue;
+                const uint32_t i0 = first(I0[k][j]), hr = red_at(XYZ[k][j]);
+                if (has(I0[k][j], hr)) S[i0 + hr] = DR[k][j] * RR[k][j];
+                if (has(I0[k][j], 1u - hr)) S[i0 + 1u - hr] = DB[k][j] * RB[k][j];
             }
             __syncthreads();
 #pragma unroll
             for (int j = 0; j < kTailJ; j++) {  // black sweep
-                const uint32_t i = tid + (uint32_t)j * kTailThreads;
-                const uint32_t xyz = XYZ[k][j];
-                const bool red = (((xyz & 1023u) + ((xyz >> 10) & 1023u) + (xyz >> 20)) & 1u) == 0u;
-                if ((k == 0 || j == 0) && i < g.n && !red)
-                    S[i] = D[k][j] * R[k][j] + D[k][j] * fsum(g, xyz, i, B[k][j], S);
+                if (!(k == 0 || j == 0) || tid + (uint32_t)j * kTailThreads >= pairs_of(g)) continue;
+                const uint32_t i0 = first(I0[k][j]), hb = 1u - red_at(XYZ[k][j]);
+                if (!has(I0[k][j], hb)) continue;
+                S[i0 + hb] = DB[k][j] * RB[k][j] + DB[k][j] * fsum(g, XYZ[k][j] & 0x7FFFFFFFu, (int)hb, i0, BB[k][j], S);
             }
             __syncthreads();
 #pragma unroll
             for (int j = 0; j < kTailJ; j++) {  // red residuals (black residuals are zero)
-                const uint32_t i = tid + (uint32_t)j * kTailThreads;
-                const uint32_t xyz = XYZ[k][j];
-                const bool red = (((xyz & 1023u) + ((xyz >> 10) & 1023u) + (xyz >> 20)) & 1u) == 0u;
-                if ((k == 0 || j == 0) && i < g.n && red) {
-                    const float s = fsum(g, xyz, i, B[k][j], S);
-                    S[i] = D[k][j] != 0.0f ? s : 0.0f;
-                }
+                if (!(k == 0 || j == 0) || tid + (uint32_t)j * kTailThreads >= pairs_of(g)) continue;
+                const uint32_t i0 = first(I0[k][j]), hr = red_at(XYZ[k][j]);
+                if (!has(I0[k][j], hr)) continue;
+                const float s = fsum(g, XYZ[k][j] & 0x7FFFFFFFu, (int)hr, i0, BR[k][j], S);
+                S[i0 + hr] = DR[k][j] != 0.0f ? s : 0.0f;
             }
             __syncthreads();
-            // restriction: the coarse cell's red children, (row y0: z0 + z1) + (row y1: z0 + z1)
-            if (tid < cg.n) {
-                int cx, cy, cz_;
-                decode(cg, tid, cx, cy, cz_);
-                float rows[2] = {0.0f, 0.0f};
-#pragma unroll
-                for (int dy = 0; dy < 2; dy++) {
-                    const int y = 2 * cy + dy;
-                    if (y >= g.ny) continue;
-                    float sa = 0.0f;
-#pragma unroll
-                    for (int dz = 0; dz < 2; dz++) {
-                        const int z = 2 * cz_ + dz;
-                        if (z >= g.nz) continue;
-                        const int x = 2 * cx + ((y + z) & 1);
-                        if (x >= g.nx) continue;
-                        sa += S[(uint32_t)x + (uint32_t)g.nx * ((uint32_t)y + (uint32_t)g.ny * (uint32_t)z)];
-                    }
-                    rows[dy] = sa;
+            // restriction into the next level's owned pairs, or the coarsest level's cells
+            if (k + 1 < L) {
+                const int kn = k + 1 < kTailLevels ? k + 1 : 0;
+                const Grid3 cg = A.g[kn];
+                if (tid < pairs_of(cg)) {
+                    const uint32_t c0 = first(I0[kn][0]);
+                    const float r0 = restrict_at(g, cg, c0, S);
+                    const float r1 = has(I0[kn][0], 1u) ? restrict_at(g, cg, c0 + 1, S) : 0.0f;
+                    const bool hr = red_at(XYZ[kn][0]) != 0u;
+                    RR[kn][0] = hr ? r1 : r0;
+                    RB[kn][0] = hr ? r0 : r1;
                 }
-                const float rc = rnd(rows[0] + rows[1]);
-                if (k + 1 < L) R[k + 1 < kTailLevels ? k + 1 : 0][0] = rc;
-                else crhs = cdi != (AT)0 ? (AT)rc : (AT)0;
+            } else if (in_c) {
+                const float rc = restrict_at(g, gc, tid, S);
+                crhs = cdi != (AT)0 ? (AT)rc : (AT)0;
             }
             __syncthreads();
         }
-        // ---- the coarsest level: eight weighted Jacobi sweeps from zero
-        if (tid < kTailCoarsest) cz[tid] = 0;
-        __syncthreads();
-        {
+        // ---- the coarsest level: eight weighted Jacobi sweeps from zero, by the first wave alone
+        // (at most 64 cells): a wave's LDS accesses complete in order, so between its sweeps the
+        // lanes need only a wave-scope fence, not a block barrier
+        static_assert(kTailCoarsest <= 64, "the coarsest level fits one wave");
+        if (tid < 64) {
+            auto wave_sync = [] {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            };
+            cz[tid] = 0;
+            wave_sync();
             const AT omega = (AT)sc->omega;
             for (int step = 0; step < 8; step++) {
                 AT next = 0;
@@ -866,9 +930,9 @@ struct K_rb_tail {
                         next = center + omega * (crhs - a) * cdi;
                     }
                 }
-                __syncthreads();
+                wave_sync();
                 if (in_c) cz[tid] = next;
-                __syncthreads();
+                wave_sync();
             }
             if (in_c) TC[tid] = rnd((float)cz[tid]);
         }
@@ -881,45 +945,50 @@ struct K_rb_tail {
             const Grid3 cg = k + 1 < L ? A.g[k + 1 < kTailLevels ? k + 1 : 0] : gc;
 #pragma unroll
             for (int j = 0; j < kTailJ; j++) {
-                const uint32_t i = tid + (uint32_t)j * kTailThreads;
-                if (!(k == 0 || j == 0) || i >= g.n) continue;
+                if (!(k == 0 || j == 0) || tid + (uint32_t)j * kTailThreads >= pairs_of(g)) continue;
+                const uint32_t i0 = first(I0[k][j]);
                 const uint32_t xyz = XYZ[k][j];
-                const int x = (int)(xyz & 1023u), y = (int)((xyz >> 10) & 1023u), z = (int)(xyz >> 20);
-                float u = D[k][j] * R[k][j];
-                if (((x + y + z) & 1) == 0) {
-                    const uint32_t pc = (uint32_t)(x >> 1) + (uint32_t)cg.nx * ((uint32_t)(y >> 1) + (uint32_t)cg.ny * (uint32_t)(z >> 1));
-                    const float zcv = ps * TC[pc];
-                    u += D[k][j] != 0.0f ? zcv : 0.0f;
-                }
-                S[i] = u;
+                const uint32_t hr = red_at(xyz);
+                const int x = (int)(xyz & 1023u) + (int)hr, y = (int)((xyz >> 10) & 1023u), z = (int)((xyz >> 20) & 1023u);
+                const uint32_t pc = (uint32_t)(x >> 1) + (uint32_t)cg.nx * ((uint32_t)(y >> 1) + (uint32_t)cg.ny * (uint32_t)(z >> 1));
+                const float zcv = ps * TC[pc];
+                float u = DR[k][j] * RR[k][j];
+                u += DR[k][j] != 0.0f ? zcv : 0.0f;
+                if (has(I0[k][j], hr)) S[i0 + hr] = u;
+                if (has(I0[k][j], 1u - hr)) S[i0 + 1u - hr] = DB[k][j] * RB[k][j];
             }
             __syncthreads();
 #pragma unroll
             for (int j = 0; j < kTailJ; j++) {  // black sweep
-                const uint32_t i = tid + (uint32_t)j * kTailThreads;
-                const uint32_t xyz = XYZ[k][j];
-                const bool red = (((xyz & 1023u) + ((xyz >> 10) & 1023u) + (xyz >> 20)) & 1u) == 0u;
-                if ((k == 0 || j == 0) && i < g.n && !red)
-                    S[i] = D[k][j] * R[k][j] + D[k][j] * fsum(g, xyz, i, B[k][j], S);
+                if (!(k == 0 || j == 0) || tid + (uint32_t)j * kTailThreads >= pairs_of(g)) continue;
+                const uint32_t i0 = first(I0[k][j]), hb = 1u - red_at(XYZ[k][j]);
+                if (!has(I0[k][j], hb)) continue;
+                S[i0 + hb] = DB[k][j] * RB[k][j] + DB[k][j] * fsum(g, XYZ[k][j] & 0x7FFFFFFFu, (int)hb, i0, BB[k][j], S);
             }
             __syncthreads();
-            float o[kTailJ];
+            float ored[kTailJ], oblack[kTailJ];
 #pragma unroll
             for (int j = 0; j < kTailJ; j++) {  // red sweep; the level's correction
-                const uint32_t i = tid + (uint32_t)j * kTailThreads;
-                const uint32_t xyz = XYZ[k][j];
-                const bool red = (((xyz & 1023u) + ((xyz >> 10) & 1023u) + (xyz >> 20)) & 1u) == 0u;
-                o[j] = 0.0f;
-                if (!(k == 0 || j == 0) || i >= g.n) continue;
-                o[j] = red ? D[k][j] * R[k][j] + D[k][j] * fsum(g, xyz, i, B[k][j], S) : S[i];
+                ored[j] = oblack[j] = 0.0f;
+                if (!(k == 0 || j == 0) || tid + (uint32_t)j * kTailThreads >= pairs_of(g)) continue;
+                const uint32_t i0 = first(I0[k][j]), hr = red_at(XYZ[k][j]);
+                if (has(I0[k][j], hr))
+                    ored[j] = DR[k][j] * RR[k][j] + DR[k][j] * fsum(g, XYZ[k][j] & 0x7FFFFFFFu, (int)hr, i0, BR[k][j], S);
+                if (has(I0[k][j], 1u - hr)) oblack[j] = S[i0 + 1u - hr];
             }
             // TC still holds the level below's correction until every thread has read it (above)
 #pragma unroll
             for (int j = 0; j < kTailJ; j++) {
-                const uint32_t i = tid + (uint32_t)j * kTailThreads;
-                if (!(k == 0 || j == 0) || i >= g.n) continue;
-                if (k == 0) A.t0[i] = cvt<VT>(o[j]);
-                else TC[i] = rnd(o[j]);
+                if (!(k == 0 || j == 0) || tid + (uint32_t)j * kTailThreads >= pairs_of(g)) continue;
+                const uint32_t i0 = first(I0[k][j]), hr = red_at(XYZ[k][j]);
+                const bool hasr = has(I0[k][j], hr), hasb = has(I0[k][j], 1u - hr);
+                if (k == 0) {
+                    if (hasr) A.t0[i0 + hr] = cvt<VT>(ored[j]);
+                    if (hasb) A.t0[i0 + 1u - hr] = cvt<VT>(oblack[j]);
+                } else {
+                    if (hasr) TC[i0 + hr] = rnd(ored[j]);
+                    if (hasb) TC[i0 + 1u - hr] = rnd(oblack[j]);
+                }
             }
             __syncthreads();
         }
@@ -935,7 +1004,13 @@ int rb_tail_level(int L, const LevelDev* lv) {
         return !(e && e[0] == '0');
     }();
     if (!on || L < 4 || lv[L - 1].g.n > kTailCoarsest) return L;
-    auto fits = [](const Grid3& g) { return g.n <= kTailCells && g.nx < 1024 && g.ny < 1024 && g.nz < 1024; };
+    // the first tail level's pairs fill at most kTailJ per thread (the deeper levels' at most 512
+    // cells, one pair per thread)
+    auto fits = [](const Grid3& g) {
+        const uint64_t pairs = (uint64_t)((g.nx + 1) / 2) * (uint64_t)g.ny * (uint64_t)g.nz;
+        return g.n <= kTailCells && pairs <= (uint64_t)kTailJ * kTailThreads && g.nx < 1024 && g.ny < 1024 &&
+               g.nz < 1024;
+    };
     int T = L - 1;
     while (T - 1 >= 2 && fits(lv[T - 1].g) && rb_level_ok(lv[T - 1].g) && L - 1 - (T - 1) <= kTailLevels &&
            (T - 1 == L - 2 || lv[T].g.n <= kTailCells / 8))
