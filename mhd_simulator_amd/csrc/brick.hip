@@ -1046,6 +1046,7 @@ void launch_brick_apply(hipStream_t s, const Grid3& g, int store, const void* z,
 void launch_brick_apply_m(hipStream_t s, const Grid3& g, int store, const void* z, float* p0, float* p1, float* x,
                           const float* alpha, const uint8_t* open8, float rho_l, float rho_air, float* q,
                           double* part, Scal* sc, int meanfree) {
+    if (launch_apply_march(s, g, store, z, p0, p1, x, alpha, open8, rho_l, rho_air, q, part, sc, meanfree)) return;
     const dim3 B(MHDP_BLOCK), G(grid_blocks(g.n >> 4, brick_grid_cap()));
     const MobP mp{rho_l, rho_air};
 #define COMMA ,

@@ -144,6 +144,11 @@ bool brick_enabled_env();
 void launch_brick_apply_m(hipStream_t s, const Grid3& g, int store, const void* z, float* p0, float* p1, float* x,
                           const float* alpha, const uint8_t* open8, float rho_l, float rho_air, float* q,
                           double* part, Scal* sc, int meanfree);
+// the same pass on the z-marching pair-column layout (nx % 4 == 0, with MHDP_APPLY_MARCH=1); false
+// when it does not apply (nothing launched)
+bool launch_apply_march(hipStream_t s, const Grid3& g, int store, const void* z, float* p0, float* p1, float* x,
+                        const float* alpha, const uint8_t* open8, float rho_l, float rho_air, float* q, double* part,
+                        Scal* sc, int meanfree);
 void launch_brick_apply(hipStream_t s, const Grid3& g, int store, const void* z, float* p0, float* p1, float* x,
                         const float* bx, const float* by, const float* bz, const void* dinv, float* q, double* part,
                         Scal* sc, int meanfree);

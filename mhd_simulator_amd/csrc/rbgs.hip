@@ -20,6 +20,7 @@
 // mixed solver, FP32 otherwise).
 #include "mhdp_kernels.h"
 #include "launch.h"
+#include "mob.h"
 
 #include <type_traits>
 
@@ -404,6 +405,28 @@ template <> struct RbP<__hip_bfloat16> {
         __builtin_amdgcn_raw_buffer_store_b32((uint32_t)ua | ((uint32_t)ub << 16), R, (int)v, (int)s, 0);
     }
 };
+// The coarse correction at a lane's parent (RB_SMOOTH), by element index.  BF16: the aligned dword
+// holding it (a 16-bit load in the plane loop makes the compiler wait for it at once and copy the
+// load ring at the loop's back edge); the array's range is rounded up to whole dwords.
+template <class T> struct RbPar;
+template <> struct RbPar<float> {
+    using raw = float;
+    static __device__ __forceinline__ raw ld(RbRsrc R, uint32_t e) {
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(R, (int)(e * 4u), 0, 0));
+    }
+    static __device__ __forceinline__ float f(raw r, uint32_t) { return r; }
+};
+template <> struct RbPar<__hip_bfloat16> {
+    using raw = uint32_t;
+    static __device__ __forceinline__ raw ld(RbRsrc R, uint32_t e) {
+        return __builtin_amdgcn_raw_buffer_load_b32(R, (int)((e & ~1u) * 2u), 0, 0);
+    }
+    static __device__ __forceinline__ float f(raw r, uint32_t e) {
+        return __uint_as_float((e & 1u) ? (r & 0xFFFF0000u) : (r << 16));
+    }
+};
+constexpr uint32_t RBPAR_ABSENT = 0x1FFFFFF0u;  // parent index past every coarse array
+
 template <class T>
 __device__ __forceinline__ float rbp_sel(typename RbP<T>::raw r, bool hi) {
     return hi ? RbP<T>::hi(r) : RbP<T>::lo(r);
@@ -432,7 +455,7 @@ struct K_rbp {
         typename PR::raw r;
         rb_v2u q;
         typename PC::raw d, bx, by, bz;
-        typename RbBuf<VT>::raw zc;
+        typename RbPar<VT>::raw zc;
         typename PV::raw base;
     };
 
@@ -467,7 +490,7 @@ struct K_rbp {
         const RbRsrc Ry = rb_rsrc(A.by, cells * PC::esize);
         const RbRsrc Rz = rb_rsrc(A.bz, cells * PC::esize);
         const RbRsrc Rc = rb_rsrc(MODE == RB_SMOOTH ? (const void*)A.zc : (const void*)A.dinv,
-                                  (MODE == RB_SMOOTH ? ccells : cells) * RbBuf<VT>::size);
+                                  ((MODE == RB_SMOOTH ? ccells : cells) * RbBuf<VT>::size + 3u) & ~(uint64_t)3);
         const RbRsrc Rw = rb_rsrc((const void*)rout, cells * 4u);
         const RbRsrc Ro = rb_rsrc(A.out, (restrict_pass ? ccells : cells) * RbBuf<VT>::size);
         const RbRsrc Rb = rb_rsrc(COARSE && A.base ? (const void*)A.base : (const void*)A.out, cells * PV::esize);
@@ -491,8 +514,7 @@ struct K_rbp {
             const uint32_t vQ = lane_ok ? e0 * 4u : RB_ABSENT;
             const uint32_t vC = lane_ok ? e0 * PC::esize : RB_ABSENT;
             const uint32_t vV = lane_ok ? e0 * PV::esize : RB_ABSENT;
-            const uint32_t vZ = lane_ok ? ((uint32_t)(gx >> 1) + (uint32_t)(gy >> 1) * csy) * RbBuf<VT>::size
-                                        : RB_ABSENT;  // parent (SMOOTH)
+            const uint32_t eZ = lane_ok ? (uint32_t)(gx >> 1) + (uint32_t)(gy >> 1) * csy : RBPAR_ABSENT;  // parent
             const bool own_x = lx >= 1 && lx <= RBP_LX - 2 && xa < g.nx;
             const bool own_y = ry >= 2 && ry <= RBP_NY - 3 && y < g.ny;
             const bool own = own_x && own_y;
@@ -512,7 +534,7 @@ struct K_rbp {
                 o.by = PC::ld(Ry, vC, so * PC::esize);
                 o.bz = PC::ld(Rz, vC, so * PC::esize);
                 if constexpr (MODE == RB_SMOOTH) {
-                    o.zc = RbBuf<VT>::ld(Rc, vZ, (pz >= 0 ? csz * (uint32_t)(pz >> 1) : 0u) * RbBuf<VT>::size);
+                    o.zc = RbPar<VT>::ld(Rc, eZ + (pz >= 0 ? csz * (uint32_t)(pz >> 1) : 0u));
                     if (COARSE && A.base) o.base = PV::ld(Rb, vV, so * PV::esize);
                 }
                 return o;
@@ -533,7 +555,8 @@ struct K_rbp {
                 const bool pr = ((y + p) & 1) != 0;
                 // ---- stage 1: plane p
                 {
-                    const bool pok = rb_coord(p, g.nz, perz) >= 0;  // uniform
+                    const int pzs = rb_coord(p, g.nz, perz);  // uniform
+                    const bool pok = pzs >= 0;
                     const typename PC::raw dr = pok ? cur.d : PC::zero();
                     const float d0 = PC::lo(dr), d1 = PC::hi(dr);
                     float rt0, rt1;
@@ -558,7 +581,8 @@ struct K_rbp {
                     float u0 = d0 * rt0, u1 = d1 * rt1;
                     if constexpr (MODE == RB_SMOOTH) {
                         if (COARSE && A.base) BS[s0] = cur.base;
-                        const float zcv = ok ? ps * RbBuf<VT>::f(cur.zc) : 0.0f;
+                        const float zcv =
+                            ok ? ps * RbPar<VT>::f(cur.zc, eZ + (pok ? csz * (uint32_t)(pzs >> 1) : 0u)) : 0.0f;
                         if (pr) u1 += d1 != 0.0f ? zcv : 0.0f;
                         else u0 += d0 != 0.0f ? zcv : 0.0f;
                         T0[s0] = rt0;
@@ -594,7 +618,7 @@ struct K_rbp {
                 // ---- stage 3: red cells of plane e = p - 2 (position pr); black values of e - 1
                 // in slot s0, e in s2, e + 1 in s1
                 const int e = p - 2;
-                if (e >= za) {
+                if (e >= za && e < zb) {
                     const float wyl = sW[e & 1][ryl][lx], wyh = sW[e & 1][ryh][lx];
                     const typename PC::raw byh = sB[s2][ryh][lx];
                     const float wc = W[s2];
@@ -638,16 +662,235 @@ struct K_rbp {
                 }
                 __syncthreads();
             };
-            for (int p = za - 2; p <= zb + 1; p += 3) {
+            // whole rounds of three steps (the chunk's zb - za + 4 steps, padded with steps that
+            // stage planes past the chunk and store nothing): a loop exit between the steps of a
+            // round would make the compiler copy the ring's loads in flight at the back edge
+            const int rounds = (zb - za + 6) / 3;
+            for (int k = 0, p = za - 2; k < rounds; k++, p += 3) {
                 step(p, IC<0>());
-                if (p + 1 <= zb + 1) step(p + 1, IC<1>());
-                if (p + 2 <= zb + 1) step(p + 2, IC<2>());
+                step(p + 1, IC<1>());
+                step(p + 2, IC<2>());
             }
         }
         if constexpr (!COARSE) {
             if constexpr (upd) reduce_finish<4, RBP_THREADS>(vg, acc, A.part, sc, 3, FIN_RESTRICT_UPD);
             else if constexpr (MODE == RB_SMOOTH) reduce_finish<1, RBP_THREADS>(vg, dot, A.part, sc, 4, A.mode);
         }
+    }
+};
+
+// ---------------------------------------------------------------- z-marching operator apply
+//
+// Pass 1 of the fused iteration with the face mobilities formed in the pass (the brick pass
+// K_k_brick_apply_m) on the pair-column layout above: a lane owns a pair of columns, a block of
+// 32 x NY lanes a tile of 64 columns by NY rows (the middle 60 x (NY - 2) are outputs) of every
+// plane of a z-chunk, marching up z.  Each array is read once per plane of the tile, where a
+// brick reads the rows and planes around it again for its neighbours.  Per plane step t: stage
+// plane t (search direction f = z + beta p_old, density from the liquid fraction, open bytes)
+// into the lane's register rings and the LDS copy of the plane, then apply the operator on plane
+// t - 1: x neighbours by DPP lane moves, y neighbours from LDS, z neighbours from the rings.
+// Every face mobility is formed from the same two densities and open bit, and the six terms of
+// a cell are summed in the same order (x-, x+, y-, y+, z-, z+), as in the brick pass: p and q are
+// that pass's bit for bit; only the partial sums of p.Ap and sum q run in another order.  Needs
+// nx % 4 == 0 (a pair's open bytes are one half of an aligned dword).
+template <class VT>
+struct ApzArgs {
+    Grid3 g;
+    const VT* z;
+    float* p0;
+    float* p1;
+    float* x;
+    const float* alpha;  // liquid fraction (two-phase)
+    const uint8_t* open8;
+    MobP mp;
+    float* q;
+    double* part;
+    Scal* sc;
+    int meanfree;
+    uint32_t zlen;
+};
+template <class VT, bool TWO, int NY>
+struct K_apz {
+    static constexpr int THREADS = RBP_LX * NY;
+    static constexpr int MINB = NY == 32 ? 1 : 2;
+    static constexpr int OUTY = NY - 2;
+    using Args = ApzArgs<VT>;
+    using PV = RbP<VT>;
+    using PF = RbP<float>;
+    struct Raw {  // one plane's loads of a lane (kept raw until staged)
+        typename PV::raw z;
+        rb_v2u p, a;
+        uint32_t o;
+    };
+
+    static __device__ void run(const Args& A, const VGrid& vg) {
+        Scal* sc = A.sc;
+        if (CONVERGED(sc)) return;
+        __shared__ float2 sF[2][NY][RBP_LX];    // f of a plane
+        __shared__ float2 sD[2][NY][RBP_LX];    // densities of a plane (TWO)
+        __shared__ uint32_t sO[2][NY][RBP_LX];  // open bytes of a plane (two per lane)
+        const Grid3 g = A.g;
+        const MobP mp = A.mp;
+        const bool perx = g.per & 1u, pery = g.per & 2u, perz = g.per & 4u;
+        const int meanfree = A.meanfree;
+        const uint32_t it0 = (uint32_t)sc->iters;
+        const float* __restrict__ pold = (it0 & 1u) ? A.p0 : A.p1;
+        float* __restrict__ pnew = (it0 & 1u) ? A.p1 : A.p0;
+        const float beta = (float)sc->beta;
+        const bool pending = sc->pending != 0;
+        const float alpha = (float)sc->alpha;
+        const float pa = (pending && meanfree) ? pold[sc->anchor] : 0.0f;
+        // a solve's first iteration (beta = 0, nothing owed) takes the old search direction as
+        // zeros: a refinement step's start leaves it unset
+        const bool first = it0 == 0u;
+        const uint32_t sz = (uint32_t)g.nx * (uint32_t)g.ny;
+        const uint32_t ntx = ((uint32_t)g.nx + RBP_OUTX - 1) / RBP_OUTX, nty = ((uint32_t)g.ny + OUTY - 1) / OUTY;
+        const uint32_t tiles = ntx * nty;
+        const uint32_t items = tiles * (((uint32_t)g.nz + A.zlen - 1) / A.zlen);
+        const uint64_t cells = (uint64_t)sz * (uint64_t)g.nz;
+        const RbRsrc Rz = rb_rsrc(A.z, cells * PV::esize);
+        const RbRsrc Rp = rb_rsrc(pold, cells * 4u);
+        const RbRsrc Ra = rb_rsrc(TWO ? (const void*)A.alpha : (const void*)pold, cells * 4u);
+        const RbRsrc Rx = rb_rsrc(A.x, cells * 4u);
+        const RbRsrc Ro = rb_rsrc(A.open8, cells);
+        const RbRsrc Rn = rb_rsrc(pnew, cells * 4u);
+        const RbRsrc Rq = rb_rsrc(A.q, cells * 4u);
+        const int lane = (int)(threadIdx.x & 63u);
+        const int lx = lane & (RBP_LX - 1);
+        const int ry = (int)(threadIdx.x >> 5);  // tile row: two per wave
+        const int ryl = ry > 0 ? ry - 1 : 0, ryh = ry < NY - 1 ? ry + 1 : NY - 1;
+        double acc[2] = {0.0, 0.0};
+        for (uint32_t it = vg.b; it < items; it += vg.n) {
+            const uint32_t ch = it / tiles, rem = it - ch * tiles;
+            const uint32_t ty = rem / ntx, tx = rem - ty * ntx;
+            const int xa = (int)tx * RBP_OUTX - 2 + 2 * lx;  // the pair's first column (even)
+            const int y = (int)ty * OUTY - 1 + ry;
+            const int za = (int)(ch * A.zlen), zb = min(za + (int)A.zlen, g.nz);
+            const int gx = rb_coord(xa, g.nx, perx), gy = rb_coord(y, g.ny, pery);
+            const bool lane_ok = gx >= 0 && gy >= 0;
+            const bool own = lx >= 1 && lx <= RBP_LX - 2 && xa < g.nx && ry >= 1 && ry <= NY - 2 && y < g.ny;
+            const uint32_t e0 = lane_ok ? (uint32_t)gx + (uint32_t)gy * (uint32_t)g.nx : 0u;
+            // lane byte offsets (past the arrays when the pair is absent: loads read 0)
+            const uint32_t v4 = lane_ok ? e0 * 4u : RB_ABSENT;
+            const uint32_t vV = lane_ok ? e0 * PV::esize : RB_ABSENT;
+            // open bytes: the aligned dword holding the pair (nx % 4 == 0: the pair is its low or
+            // high half); a 16-bit load would make the compiler wait for it at once
+            const uint32_t vO = lane_ok ? (e0 & ~3u) : RB_ABSENT, osh = (e0 & 2u) * 8u;
+            const uint32_t vX = lane_ok && own && pending ? e0 * 4u : RB_ABSENT;  // x: outputs only
+            // ring slots count planes from the chunk's first staged plane za - 1
+            // lower-face mobilities (x: the pair's two lower x faces; z) formed once, at staging
+            float F0[3], F1[3], D0[3] = {0.0f, 0.0f, 0.0f}, D1[3] = {0.0f, 0.0f, 0.0f}, P0[3], P1[3];
+            float WX0[3], WX1[3], WZ0[3], WZ1[3];
+            uint32_t OB[3];
+            rb_v2u XL[3];  // x of the plane whose operator step is two steps ahead
+            auto load = [&](int p) {
+                Raw o;
+                const int pz = rb_coord(p, g.nz, perz);
+                const uint32_t so = pz >= 0 ? sz * (uint32_t)pz : 0u;
+                o.z = PV::ld(Rz, vV, so * PV::esize);
+                o.p = PF::ld(Rp, v4, so * 4u);
+                if constexpr (TWO) o.a = PF::ld(Ra, v4, so * 4u);
+                o.o = __builtin_amdgcn_raw_buffer_load_b32(Ro, (int)vO, (int)so, 0);
+                return o;
+            };
+            // loads run two planes ahead of the plane being staged, into a ring slot of their own
+            // (a register copy of a load in flight would wait for it)
+            Raw RW[3];
+            RW[0] = load(za - 1);
+            RW[1] = load(za);
+            auto step = [&](int t, auto S0c) {
+                constexpr int s0 = decltype(S0c)::value, sp = (s0 + 2) % 3, sm = (s0 + 1) % 3;
+                // slot of plane t - 1 (staged already); issued on every step, also past the chunk's
+                // last staged plane, so that the wait for this step's loads never depends on a branch
+                RW[sp] = load(t + 2);
+                {  // x of plane t + 1, used by step t + 2 (slot sm held plane t - 2's, used already)
+                    const int pz = rb_coord(t + 1, g.nz, perz);
+                    XL[sm] = PF::ld(Rx, vX, (pz >= 0 ? sz * (uint32_t)pz : 0u) * 4u);
+                }
+                // ---- stage plane t
+                {
+                    const Raw& cur = RW[s0];
+                    const bool ok = lane_ok && rb_coord(t, g.nz, perz) >= 0;
+                    const float z0 = ok ? PV::lo(cur.z) : 0.0f, z1 = ok ? PV::hi(cur.z) : 0.0f;
+                    const float q0 = ok && !first ? __uint_as_float(cur.p.x) : 0.0f;
+                    const float q1 = ok && !first ? __uint_as_float(cur.p.y) : 0.0f;
+                    F0[s0] = z0 + beta * q0;
+                    F1[s0] = z1 + beta * q1;
+                    P0[s0] = q0;
+                    P1[s0] = q1;
+                    const float d0 = TWO ? mob_dens(mp, ok ? __uint_as_float(cur.a.x) : 0.0f) : 0.0f;
+                    const float d1 = TWO ? mob_dens(mp, ok ? __uint_as_float(cur.a.y) : 0.0f) : 0.0f;
+                    const uint32_t ob = ok ? (cur.o >> osh) & 0xFFFFu : 0u;
+                    // ring slot sp still holds plane t - 1 (the plane below)
+                    WX0[s0] = mob_face<TWO>(mp, ob & 1u, TWO ? rb_prev_lane(d1) : 0.0f, d0);
+                    WX1[s0] = mob_face<TWO>(mp, (ob >> 8) & 1u, d0, d1);
+                    WZ0[s0] = mob_face<TWO>(mp, ob & 4u, D0[sp], d0);
+                    WZ1[s0] = mob_face<TWO>(mp, (ob >> 8) & 4u, D1[sp], d1);
+                    D0[s0] = d0;
+                    D1[s0] = d1;
+                    OB[s0] = ob;
+                    sF[t & 1][ry][lx] = make_float2(F0[s0], F1[s0]);
+                    if constexpr (TWO) sD[t & 1][ry][lx] = make_float2(D0[s0], D1[s0]);
+                    sO[t & 1][ry][lx] = OB[s0];
+                }
+                // ---- the operator on plane p = t - 1: rings hold p - 1 in sm, p in sp, p + 1 in s0
+                const int p = t - 1;
+                if (p >= za && p < zb) {
+                    const float2 fyl = sF[p & 1][ryl][lx], fyh = sF[p & 1][ryh][lx];
+                    const float2 dyl = TWO ? sD[p & 1][ryl][lx] : make_float2(0.0f, 0.0f);
+                    const float2 dyh = TWO ? sD[p & 1][ryh][lx] : make_float2(0.0f, 0.0f);
+                    const uint32_t oyh = sO[p & 1][ryh][lx];
+                    const float f0 = F0[sp], f1 = F1[sp], d0 = D0[sp], d1 = D1[sp];
+                    const uint32_t ob = OB[sp];
+                    const float fxm = rb_prev_lane(f1), fxp = rb_next_lane(f0);
+                    // column 0's upper x face is column 1's lower one; column 1's upper x face is
+                    // the next lane's column 0 lower one; plane p's upper z faces are p + 1's lower
+                    const float wx0 = WX0[sp], wx1 = WX1[sp], wx2 = rb_next_lane(WX0[sp]);
+                    float a0 = 0.0f;
+                    a0 += wx0 * (f0 - fxm);
+                    a0 += wx1 * (f0 - f1);
+                    a0 += mob_face<TWO>(mp, ob & 2u, dyl.x, d0) * (f0 - fyl.x);
+                    a0 += mob_face<TWO>(mp, oyh & 2u, d0, dyh.x) * (f0 - fyh.x);
+                    a0 += WZ0[sp] * (f0 - F0[sm]);
+                    a0 += WZ0[s0] * (f0 - F0[s0]);
+                    float a1 = 0.0f;
+                    a1 += wx1 * (f1 - f0);
+                    a1 += wx2 * (f1 - fxp);
+                    a1 += mob_face<TWO>(mp, (ob >> 8) & 2u, dyl.y, d1) * (f1 - fyl.y);
+                    a1 += mob_face<TWO>(mp, (oyh >> 8) & 2u, d1, dyh.y) * (f1 - fyh.y);
+                    a1 += WZ1[sp] * (f1 - F1[sm]);
+                    a1 += WZ1[s0] * (f1 - F1[s0]);
+                    if (own) {
+                        const bool act0 = open_active(ob & 0xFFu, meanfree), act1 = open_active((ob >> 8) & 0xFFu, meanfree);
+                        const float pn0 = act0 ? f0 : 0.0f, pn1 = act1 ? f1 : 0.0f;
+                        acc[0] += (double)pn0 * (double)a0;
+                        acc[0] += (double)pn1 * (double)a1;
+                        acc[1] += act0 ? (double)a0 : 0.0;
+                        acc[1] += act1 ? (double)a1 : 0.0;
+                        const uint32_t so = sz * (uint32_t)p * 4u;
+                        PF::st(Rn, pn0, pn1, v4, so);
+                        PF::st(Rq, a0, a1, v4, so);
+                        if (pending) {
+                            float x0 = __uint_as_float(XL[sp].x), x1 = __uint_as_float(XL[sp].y);
+                            if (act0) x0 = x0 + alpha * (P0[sp] - pa);
+                            if (act1) x1 = x1 + alpha * (P1[sp] - pa);
+                            PF::st(Rx, x0, x1, v4, so);
+                        }
+                    }
+                }
+                __syncthreads();
+            };
+            // whole rounds of three steps (the chunk's zb - za + 2 steps, padded with steps that
+            // stage planes past the chunk and store nothing): a loop exit between the steps of a
+            // round would make the compiler copy the ring's loads in flight at the back edge
+            const int rounds = (zb - za + 4) / 3;
+            for (int k = 0, t = za - 1; k < rounds; k++, t += 3) {
+                step(t, IC<0>());
+                step(t + 1, IC<1>());
+                step(t + 2, IC<2>());
+            }
+        }
+        reduce_finish<2, THREADS>(vg, acc, A.part, sc, 2, FIN_ALPHA);
     }
 };
 
@@ -1089,16 +1332,16 @@ static uint32_t rbw_zlen(const Grid3& g) {
 // re-read planes few.
 static uint32_t rbp_zlen(const Grid3& g, int outy, uint32_t blocks, bool coarse) {
     const uint64_t tiles = (uint64_t)((g.nx + RBP_OUTX - 1) / RBP_OUTX) * (uint64_t)((g.ny + outy - 1) / outy);
-    if (!coarse) {
+    if (!coarse) {  // even, and z + 4 a whole number of the kernel's three-step rounds
         uint64_t z = (uint64_t)g.nz * tiles / 2048u;
-        z &= ~(uint64_t)1;
-        return (uint32_t)(z < 8 ? 8 : (z > 64 ? 64 : z));
+        z = (z + 1) / 6 * 6 + 2;
+        return (uint32_t)(z < 8 ? 8 : (z > 62 ? 62 : z));
     }
     uint32_t best = 4;
     uint64_t best_steps = ~(uint64_t)0;
     for (uint32_t z = 4; z <= 64; z += 2) {
         const uint64_t items = tiles * (((uint64_t)g.nz + z - 1) / z);
-        const uint64_t steps = (items + blocks - 1) / blocks * (z + 4);
+        const uint64_t steps = (items + blocks - 1) / blocks * ((z + 6) / 3 * 3);
         if (steps < best_steps) best_steps = steps, best = z;
         if (z >= (uint32_t)g.nz) break;
     }
@@ -1133,6 +1376,49 @@ static void rbp_launch(hipStream_t s, A a) {
     uint32_t blocks = items < cap ? items : cap;
     if (blocks >= 64) blocks = (blocks + MHDP_XCDS - 1) / MHDP_XCDS * MHDP_XCDS;
     go<KP>(s, blocks, a);
+}
+
+// The z-marching apply replaces the brick apply with MHDP_APPLY_MARCH=1 (grids with nx % 4 == 0).
+// It is not the default: at 512^3 it takes 880 us per launch against the brick pass's 847 us
+// (it forms four face mobilities per cell with IEEE divisions, as the brick does, but with a
+// barrier per plane and less independent work per lane to hide them).
+static bool apz_enabled() {
+    static const bool on = [] {
+        const char* e = getenv("MHDP_APPLY_MARCH");
+        return e && e[0] && e[0] != '0';
+    }();
+    return on;
+}
+// z-chunk of the apply: the fewest plane steps per resident block (a chunk of z planes costs its
+// z + 2 steps with the two halo planes, rounded up to whole rounds of three), among chunks of at
+// most 12 planes (longer ones ran slower at 512^3: 16 planes 916 us, 64 planes 1163 us, against
+// 880 us at 10).
+static uint32_t apz_zlen(const Grid3& g, int outy, uint32_t blocks) {
+    const uint64_t tiles = (uint64_t)((g.nx + RBP_OUTX - 1) / RBP_OUTX) * (uint64_t)((g.ny + outy - 1) / outy);
+    uint32_t best = 4;
+    uint64_t best_steps = ~(uint64_t)0;
+    for (uint32_t z = 4; z <= 12; z++) {
+        const uint64_t items = tiles * (((uint64_t)g.nz + z - 1) / z);
+        const uint64_t steps = (items + blocks - 1) / blocks * ((z + 4) / 3 * 3);
+        if (steps < best_steps) best_steps = steps, best = z;
+        if (z >= (uint32_t)g.nz) break;
+    }
+    return best;
+}
+template <class KP>
+static void apz_launch(hipStream_t s, typename KP::Args a) {
+    const uint32_t cap = 256u * (uint32_t)KP::MINB;  // the resident blocks
+    a.zlen = apz_zlen(a.g, KP::OUTY, cap);
+    const uint32_t tiles = (uint32_t)((a.g.nx + RBP_OUTX - 1) / RBP_OUTX) * (uint32_t)((a.g.ny + KP::OUTY - 1) / KP::OUTY);
+    const uint32_t items = tiles * (((uint32_t)a.g.nz + a.zlen - 1) / a.zlen);
+    uint32_t blocks = items < cap ? items : cap;
+    if (blocks >= 64) blocks = (blocks + MHDP_XCDS - 1) / MHDP_XCDS * MHDP_XCDS;
+    go<KP>(s, blocks, a);
+}
+template <class VT, bool TWO>
+static void apz_go(hipStream_t s, const ApzArgs<VT>& a) {
+    if (a.g.n < kRbpSmallFine) apz_launch<K_apz<VT, TWO, 16>>(s, a);
+    else apz_launch<K_apz<VT, TWO, 32>>(s, a);
 }
 
 template <class CT, class VT, class RT, int MODE, bool COARSE>
@@ -1212,4 +1498,21 @@ void launch_rb_smooth(hipStream_t s, int store, bool coarse, const Grid3& g, con
                                                       (const T*)bz, (const T*)zc, (T*)t, nullptr, 0.0f, part, sc,
                                                       mode, use_next ? 1 : 0, 0});
     }
+}
+
+// launch_brick_apply_m on the z-marching kernel; false (nothing launched) where it does not apply.
+bool launch_apply_march(hipStream_t s, const Grid3& g, int store, const void* z, float* p0, float* p1, float* x,
+                        const float* alpha, const uint8_t* open8, float rho_l, float rho_air, float* q, double* part,
+                        Scal* sc, int meanfree) {
+    if ((g.nx & 3) || !apz_enabled()) return false;
+    const MobP mp{rho_l, rho_air};
+    if (store == STORE_BF16) {
+        using T = __hip_bfloat16;
+        if (alpha) apz_go<T, true>(s, {g, (const T*)z, p0, p1, x, alpha, open8, mp, q, part, sc, meanfree, 0});
+        else apz_go<T, false>(s, {g, (const T*)z, p0, p1, x, alpha, open8, mp, q, part, sc, meanfree, 0});
+    } else {
+        if (alpha) apz_go<float, true>(s, {g, (const float*)z, p0, p1, x, alpha, open8, mp, q, part, sc, meanfree, 0});
+        else apz_go<float, false>(s, {g, (const float*)z, p0, p1, x, alpha, open8, mp, q, part, sc, meanfree, 0});
+    }
+    return true;
 }

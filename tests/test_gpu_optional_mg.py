@@ -130,14 +130,20 @@ def test_breakdown_is_detected_and_solved_with_reference_cycle(orc, precision, n
         assert pr.stats["cycle_fallback"] == 0
 
 
-def test_diverging_cycle_is_detected_and_solved_with_reference_cycle(orc):
-    """A red-black cycle over-corrected to scale 1.9 on a tall layered grid with odd coarse levels
-    keeps r.z > 0 but is not positive definite there: the inner residual grows without bound.  The
-    device guard stops the solve once the residual exceeds 1e5 times the solve's right-hand side,
-    and the projection is solved again with the reference's V-cycle and meets the 1e-8 bar."""
+def tall_layers():
+    """20x18x70 two-phase grid of alternating 3- and 4-cell liquid and air layers."""
     g = C.Grid(20, 18, 70, two_phase=True, rho_liquid=C.RHO_LIQUID, rho_air=C.RHO_AIR)
     alpha = ((np.arange(g.cells) // (g.nx * g.ny)) % 7 < 3).astype(np.float32)
-    p = C.Problem("tall", g, np.ones(g.cells, np.uint8), alpha, C.seeded_faces(g, 5))
+    return C.Problem("tall", g, np.ones(g.cells, np.uint8), alpha, C.seeded_faces(g, 5))
+
+
+def test_diverging_cycle_is_detected_and_solved_with_reference_cycle(orc):
+    """A red-black cycle over-corrected to scale 1.9 on a tall layered grid with odd coarse levels
+    keeps r.z > 0 but is not positive definite there: the inner residual grows or wanders without
+    converging.  The device guard stops the solve once the residual exceeds 1e5 times the solve's
+    right-hand side, the host once 100 iterations set no new least residual; the projection is
+    solved again with the reference's V-cycle and meets the 1e-8 bar."""
+    p = tall_layers()
     with make(p) as pr:
         print("[gpu] diverging red-black cycle on the tall grid", flush=True)
         it, res = pr.project(p.ustar, precision="mixed", precond="mg", smoother="red-black", coarse_scale=1.9,
@@ -168,6 +174,31 @@ def test_batch_breakdown_is_solved_again_with_reference_cycle(orc):
                        coarse_visits=1, smoother="jacobi", u_out=one_u, phi_out=one_phi)
         torch.cuda.synchronize()
         assert torch.equal(one_u, uo[1]) and torch.equal(one_phi, po[1])
+    finally:
+        for pr in prs:
+            pr.close()
+
+
+def test_batch_diverging_cycle_is_solved_again_with_reference_cycle(orc):
+    """The tall layered grid between two droplet boxes in one batch with the over-corrected
+    red-black cycle: only its solve is stopped (device guard or no new least residual in 100
+    iterations) and solved again on its own with the reference's V-cycle; all three meet 1e-8."""
+    import torch
+    from mhd_simulator_amd.fluid import project_batch
+    ps = [C.droplet_box(16, 316), tall_layers(), C.droplet_box(24, 324)]
+    prs = [make(p) for p in ps]
+    try:
+        us = [torch.from_numpy(p.ustar).cuda() for p in ps]
+        po = [torch.zeros(p.grid.cells, dtype=torch.float64, device="cuda") for p in ps]
+        print("[gpu] batch with the diverging red-black cycle", flush=True)
+        res = project_batch(prs, us, precision="mixed", precond="mg", smoother="red-black", coarse_scale=1.9,
+                            phi_outs=po)
+        torch.cuda.synchronize()
+        print(f"[gpu]   fallback={[pr.stats['cycle_fallback'] for pr in prs]}", flush=True)
+        assert [pr.stats["cycle_fallback"] for pr in prs] == [0, 1, 0]
+        for k, (p, pr) in enumerate(zip(ps, prs)):
+            assert res[k][1] <= 1e-8
+            assert orc.true_residual(p.grid, pr.mask, p.alpha, p.ustar, po[k].cpu().numpy()) <= 1e-8
     finally:
         for pr in prs:
             pr.close()
