@@ -1361,8 +1361,10 @@ static bool rbp_enabled() {
 // Coarse levels of fewer cells than this run the 16-row tiles, and so do fine levels of fewer than
 // kRbpSmallFine cells (with the fine chunk rule): two blocks per CU hide the latency of their short
 // marches (64^3: 1.58 -> 1.43 ms per mixed projection; the 64-problem batch of 32^3..96^3 grids
-// 20.2 -> 18.2 ms; at 128^3 the two tile heights take the same time).
-constexpr uint32_t kRbpSmallCells = 1u << 25;
+// 20.2 -> 18.2 ms; at 128^3 the two tile heights take the same time).  A 256^3 coarse level (the
+// first of a 512^3 grid) runs the 32-row tiles: 50-63 us per pass against 53-88 us, 41.7 -> 41.4 ms
+// per 512^3 projection.
+constexpr uint32_t kRbpSmallCells = 1u << 24;
 constexpr uint32_t kRbpSmallFine = 1u << 22;
 
 template <class KP, bool COARSE, class A>
