@@ -1332,11 +1332,24 @@ static uint32_t rbw_zlen(const Grid3& g) {
 // re-read planes few.
 static uint32_t rbp_zlen(const Grid3& g, int outy, uint32_t blocks, bool coarse) {
     const uint64_t tiles = (uint64_t)((g.nx + RBP_OUTX - 1) / RBP_OUTX) * (uint64_t)((g.ny + outy - 1) / outy);
-    if (!coarse) {  // even, and z + 4 a whole number of the kernel's three-step rounds; at 512^3 32
-        // planes (41.3 ms per projection; 26: 41.4, 44: 41.6, 56: 41.7, 62: 42.0 ms)
+    if (!coarse && outy < 28) {  // fine levels in 16-row tiles: 8 planes (their grids are small)
         uint64_t z = (uint64_t)g.nz * tiles * 3u / 8192u;
         z = (z + 1) / 6 * 6 + 2;
         return (uint32_t)(z < 8 ? 8 : (z > 62 ? 62 : z));
+    }
+    if (!coarse) {
+        // fine levels in 32-row tiles: the fewest plane steps per resident block among chunks of
+        // 8..32 planes with z + 4 a whole number of three-step rounds (256^3: 26 planes, 11.45 ms per
+        // nozzle projection against 12.31 ms at 8 and 12.15 ms at 32; 512^3: 32 planes, 41.3 ms
+        // against 41.4 at 26, 41.6 at 44 and 42.0 ms at 62)
+        uint32_t best = 8;
+        uint64_t best_steps = ~(uint64_t)0;
+        for (uint32_t z = 8; z <= 32; z += 6) {
+            const uint64_t items = tiles * (((uint64_t)g.nz + z - 1) / z);
+            const uint64_t steps = (items + blocks - 1) / blocks * (z + 4);
+            if (steps < best_steps) best_steps = steps, best = z;
+        }
+        return best;
     }
     uint32_t best = 4;
     uint64_t best_steps = ~(uint64_t)0;
